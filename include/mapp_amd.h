@@ -1,0 +1,102 @@
+/*
+ * mapp_amd.h -- C ABI of the MI355X (gfx950) implementation of MAPP's EAM /
+ * DMD hot path.
+ *
+ * Conventions
+ *   - Every call returns a status (0 = MAPP_OK). On failure mapp_last_error()
+ *     returns a message for the calling thread. No exception crosses the ABI.
+ *   - Arrays passed by pointer are host arrays owned by the caller, in the
+ *     caller's atom order; x, v, f are [natms][3] doubles.
+ *   - H is the 3x3 lower-triangular box matrix, row-major, rows = box vectors
+ *     (the reference's atoms->H, src/atoms.cpp:187-194).
+ *   - Handles are not thread safe; each owns one HIP stream.
+ *   - Units are the caller's (the reference uses eV, Angstrom, amu).
+ */
+#ifndef MAPP_AMD_H
+#define MAPP_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int mapp_status;
+#define MAPP_OK 0
+#define MAPP_ERR_ARG 1
+#define MAPP_ERR_FILE 2
+#define MAPP_ERR_HIP 3
+#define MAPP_ERR_CAPACITY 4
+#define MAPP_ERR_STATE 5
+
+typedef struct mapp_eam mapp_eam; /* tabulated EAM tables resident on the GPU */
+typedef struct mapp_md mapp_md;   /* atoms + neighbor list + dynamic state on the GPU */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* mapp_last_error(void);
+/* Library version string. */
+const char* mapp_version(void);
+
+/* ---------------- force field construction ----------------
+ * Replace ForceFieldEAM construction by sim.ff_eam_setfl / sim.ff_eam_fs
+ * (src/ff_eam.cpp:594-801) over ImportEAM::setfl / fs (src/import_eam.h:348-901).
+ * elems selects and orders the file's elements (NULL / 0: all, file order).
+ * dmd != 0 builds the DMD variant (quartic F, r*rho tables) used by
+ * ForceFieldEAMDMD (src/ff_eam_dmd.cpp:17-106). */
+mapp_status mapp_eam_create_setfl(const char* path, const char* const* elems, int nelems, int dmd, mapp_eam** out);
+mapp_status mapp_eam_create_fs(const char* path, const char* const* elems, int nelems, int dmd, mapp_eam** out);
+/* From knot arrays: F[ne][nrho], rho[ne][ne][nr] (rho[a][b] = density
+ * function listed in element a's block at position b), r_phi[ne][ne][nr]
+ * symmetric; fs == 0 uses rho[a][0] for every b as setfl does. */
+mapp_status mapp_eam_create_knots(int ne, int nr, int nrho, double dr, double drho, const double* F,
+                                  const double* rho, const double* r_phi, int fs, int dmd, mapp_eam** out);
+void mapp_eam_destroy(mapp_eam* ff);
+/* r_c[ne][ne] derived from the tables (src/import_eam.h:52-64) */
+mapp_status mapp_eam_cutoff(const mapp_eam* ff, int* ne, double* rc);
+/* Spline coefficients for cross-checking: which 0 = r_phi[a][b] ([nr][4]),
+ * 1 = rho[a][b] ([nr][4]), 2 = F[a] ([nrho][4] or [5] for dmd). */
+mapp_status mapp_eam_coefs(const mapp_eam* ff, int which, int a, int b, double* out);
+
+/* ---------------- MD system ----------------
+ * Replaces DynamicMD::init (src/dynamic_md.cpp:34-52): wraps the atoms into
+ * the box, bins them into cells and builds the neighbor list with skin. */
+mapp_status mapp_md_create(mapp_eam* ff, int natms, const double* H, const double* x, const uint8_t* elem,
+                           double skin, mapp_md** out);
+void mapp_md_destroy(mapp_md* md);
+mapp_status mapp_md_set_x(mapp_md* md, const double* x);
+mapp_status mapp_md_get_x(mapp_md* md, double* x);
+mapp_status mapp_md_set_v(mapp_md* md, const double* v);
+mapp_status mapp_md_get_v(mapp_md* md, double* v);
+/* DynamicMD::update<true> (src/dynamic_md.h:38-68, decide at
+ * src/dynamic_md.cpp:90-105): rebuilds when an atom moved more than skin/2. */
+mapp_status mapp_md_update(mapp_md* md, int* rebuilt);
+/* ForceFieldMD::force_calc (src/ff_md.cpp:125-144) with
+ * ForceFieldEAM::__force_calc (src/ff_eam.cpp:65-240). out7[0] = energy,
+ * out7[1..6] = F_H in Voigt order xx,xy,xz,yy,yz,zz (S_pe = -F_H / V).
+ * f may be NULL. */
+mapp_status mapp_md_force_calc(mapp_md* md, double* f, double* out7);
+/* ForceFieldMD::value (src/ff_md.cpp:148-155) with
+ * ForceFieldEAM::__energy_calc (src/ff_eam.cpp:244-318). */
+mapp_status mapp_md_value(mapp_md* md, double* energy);
+/* per-atom electron density of the last force_calc (the "rho" vector) */
+mapp_status mapp_md_get_rho(mapp_md* md, double* rho);
+/* Half list as NeighborMD::create_list stores it (src/neighbor_md.cpp:82-152):
+ * triples (i, owner j, image shift) with i<j inside the box and every
+ * periodic-image pair. */
+mapp_status mapp_md_pairs_count(mapp_md* md, int64_t* npairs);
+mapp_status mapp_md_pairs(mapp_md* md, int32_t* i, int32_t* j, int8_t* img);
+mapp_status mapp_md_list_stats(mapp_md* md, int* max_neighbors, int64_t* full_entries, int* rebuilds);
+/* MDNVT::run (src/md_nvt.cpp:383-480): velocity Verlet with a Nose-Hoover
+ * chain (L=3, one iteration, t_relax = 100 dt), centre-of-mass drift removed
+ * each step, neighbor rebuild on the skin criterion. masses[ne]. */
+mapp_status mapp_md_nvt_run(mapp_md* md, const double* masses, double kB, double T, double dt, int nsteps,
+                            double* out7, double* T_part, int* rebuilds);
+/* AtomsMD::create_T (src/atoms_md.cpp:130-160): Maxwell velocities from the
+ * Park-Miller / Marsaglia stream of src/random.cpp:7-68 seeded with seed. */
+mapp_status mapp_create_T(int natms, const uint8_t* elem, const double* masses, double kB, double T, int seed,
+                          double* v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

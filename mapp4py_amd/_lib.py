@@ -1,0 +1,93 @@
+"""ctypes binding of libmapp_amd.so (the C ABI declared in include/mapp_amd.h).
+
+The library is built in-tree (``python -c "import __graft_entry__ as g; g.build()"``
+or ``make -C mapp4py_amd/csrc``). Importing the package fails loudly when the
+shared object is missing: there is no CPU fallback on the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmapp_amd.so")
+
+_lib = None
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+i8p = C.POINTER(C.c_int8)
+u8p = C.POINTER(C.c_uint8)
+vpp = C.POINTER(C.c_void_p)
+fp = C.POINTER(C.c_float)
+
+# name -> (restype, argtypes); every exported symbol of include/mapp_amd.h
+SIGNATURES = {
+    "mapp_last_error": (C.c_char_p, []),
+    "mapp_version": (C.c_char_p, []),
+    "mapp_eam_create_setfl": (C.c_int, [C.c_char_p, C.POINTER(C.c_char_p), C.c_int, C.c_int, vpp]),
+    "mapp_eam_create_fs": (C.c_int, [C.c_char_p, C.POINTER(C.c_char_p), C.c_int, C.c_int, vpp]),
+    "mapp_eam_create_knots": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dp, dp, dp,
+                                        C.c_int, C.c_int, vpp]),
+    "mapp_eam_destroy": (None, [C.c_void_p]),
+    "mapp_eam_cutoff": (C.c_int, [C.c_void_p, ip, dp]),
+    "mapp_eam_coefs": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, dp]),
+    "mapp_md_create": (C.c_int, [C.c_void_p, C.c_int, dp, dp, u8p, C.c_double, vpp]),
+    "mapp_md_destroy": (None, [C.c_void_p]),
+    "mapp_md_set_x": (C.c_int, [C.c_void_p, dp]),
+    "mapp_md_get_x": (C.c_int, [C.c_void_p, dp]),
+    "mapp_md_set_v": (C.c_int, [C.c_void_p, dp]),
+    "mapp_md_get_v": (C.c_int, [C.c_void_p, dp]),
+    "mapp_md_update": (C.c_int, [C.c_void_p, ip]),
+    "mapp_md_force_calc": (C.c_int, [C.c_void_p, dp, dp]),
+    "mapp_md_value": (C.c_int, [C.c_void_p, dp]),
+    "mapp_md_get_rho": (C.c_int, [C.c_void_p, dp]),
+    "mapp_md_pairs_count": (C.c_int, [C.c_void_p, i64p]),
+    "mapp_md_pairs": (C.c_int, [C.c_void_p, i32p, i32p, i8p]),
+    "mapp_md_list_stats": (C.c_int, [C.c_void_p, ip, i64p, ip]),
+    "mapp_md_nvt_run": (C.c_int, [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, ip]),
+    "mapp_create_T": (C.c_int, [C.c_int, u8p, dp, C.c_double, C.c_double, C.c_int, dp]),
+}
+
+
+class MappError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, status, msg):
+        super().__init__(f"mapp status {status}: {msg}")
+        self.status = status
+
+
+def lib():
+    """Load libmapp_amd.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build the HIP library first (make -C mapp4py_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != 0:
+        raise MappError(status, lib().mapp_last_error().decode())
+
+
+def ptr(a, t=dp):
+    return a.ctypes.data_as(t)
+
+
+def f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a

@@ -1,0 +1,238 @@
+// capi.cpp -- extern "C" boundary: status codes, thread-local error text,
+// opaque handles over the C++ classes.
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/mapp_amd.h"
+#include "md.hpp"
+#include "rng.hpp"
+
+namespace mapp {
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string& m) { g_last_error = m; }
+}  // namespace mapp
+
+struct mapp_eam {
+    std::shared_ptr<mapp::EamForceField> ff;
+};
+struct mapp_md {
+    std::unique_ptr<mapp::MDSystem> sys;
+};
+
+namespace {
+
+template <class F>
+mapp_status guard(F&& f)
+{
+    try {
+        mapp::set_last_error("");
+        f();
+        return MAPP_OK;
+    } catch (const mapp::Error& e) {
+        mapp::set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        mapp::set_last_error("host out of memory");
+        return MAPP_ERR_CAPACITY;
+    } catch (const std::exception& e) {
+        mapp::set_last_error(e.what());
+        return MAPP_ERR_STATE;
+    }
+}
+
+std::vector<std::string> names(const char* const* elems, int nelems)
+{
+    std::vector<std::string> v;
+    for (int i = 0; elems && i < nelems; ++i) v.emplace_back(elems[i]);
+    return v;
+}
+
+#define NEED(p) MAPP_REQUIRE((p) != nullptr, mapp::ERR_ARG, #p " must not be NULL")
+
+}  // namespace
+
+extern "C" {
+
+const char* mapp_last_error(void) { return mapp::g_last_error.c_str(); }
+
+const char* mapp_version(void) { return "mapp4py_amd 0.1 (gfx950)"; }
+
+mapp_status mapp_eam_create_setfl(const char* path, const char* const* elems, int nelems, int dmd, mapp_eam** out)
+{
+    return guard([&] {
+        NEED(path);
+        NEED(out);
+        auto t = mapp::read_dynamo(path, 0, names(elems, nelems), dmd != 0);
+        *out = new mapp_eam{std::make_shared<mapp::EamForceField>(std::move(t))};
+    });
+}
+
+mapp_status mapp_eam_create_fs(const char* path, const char* const* elems, int nelems, int dmd, mapp_eam** out)
+{
+    return guard([&] {
+        NEED(path);
+        NEED(out);
+        auto t = mapp::read_dynamo(path, 1, names(elems, nelems), dmd != 0);
+        *out = new mapp_eam{std::make_shared<mapp::EamForceField>(std::move(t))};
+    });
+}
+
+mapp_status mapp_eam_create_knots(int ne, int nr, int nrho, double dr, double drho, const double* F,
+                                  const double* rho, const double* r_phi, int fs, int dmd, mapp_eam** out)
+{
+    return guard([&] {
+        NEED(F);
+        NEED(rho);
+        NEED(r_phi);
+        NEED(out);
+        auto t = mapp::tables_from_knots(ne, nr, nrho, dr, drho, F, rho, r_phi, fs == 0, dmd != 0);
+        *out = new mapp_eam{std::make_shared<mapp::EamForceField>(std::move(t))};
+    });
+}
+
+void mapp_eam_destroy(mapp_eam* ff) { delete ff; }
+
+mapp_status mapp_eam_cutoff(const mapp_eam* ff, int* ne, double* rc)
+{
+    return guard([&] {
+        NEED(ff);
+        const auto& t = ff->ff->tables();
+        if (ne) *ne = t.ne;
+        if (rc) std::memcpy(rc, t.rc.data(), sizeof(double) * t.rc.size());
+    });
+}
+
+mapp_status mapp_eam_coefs(const mapp_eam* ff, int which, int a, int b, double* out)
+{
+    return guard([&] {
+        NEED(ff);
+        NEED(out);
+        const auto& t = ff->ff->tables();
+        MAPP_REQUIRE(a >= 0 && a < t.ne && b >= 0 && b < t.ne, mapp::ERR_ARG, "element index out of range");
+        if (which == 0) std::memcpy(out, t.rphi_at(a, b), sizeof(double) * 4 * t.nr);
+        else if (which == 1) std::memcpy(out, t.rho_at(a, b), sizeof(double) * 4 * t.nr);
+        else if (which == 2) std::memcpy(out, t.F_at(a), sizeof(double) * t.nf() * t.nrho);
+        else throw mapp::Error(mapp::ERR_ARG, "which must be 0, 1 or 2");
+    });
+}
+
+mapp_status mapp_md_create(mapp_eam* ff, int natms, const double* H, const double* x, const uint8_t* elem,
+                           double skin, mapp_md** out)
+{
+    return guard([&] {
+        NEED(ff);
+        NEED(H);
+        NEED(x);
+        NEED(elem);
+        NEED(out);
+        *out = new mapp_md{std::make_unique<mapp::MDSystem>(ff->ff, natms, H, x, elem, skin)};
+    });
+}
+
+void mapp_md_destroy(mapp_md* md) { delete md; }
+
+mapp_status mapp_md_set_x(mapp_md* md, const double* x)
+{
+    return guard([&] { NEED(md); NEED(x); md->sys->set_x(x); });
+}
+mapp_status mapp_md_get_x(mapp_md* md, double* x)
+{
+    return guard([&] { NEED(md); NEED(x); md->sys->get_x(x); });
+}
+mapp_status mapp_md_set_v(mapp_md* md, const double* v)
+{
+    return guard([&] { NEED(md); NEED(v); md->sys->set_v(v); });
+}
+mapp_status mapp_md_get_v(mapp_md* md, double* v)
+{
+    return guard([&] { NEED(md); NEED(v); md->sys->get_v(v); });
+}
+
+mapp_status mapp_md_update(mapp_md* md, int* rebuilt)
+{
+    return guard([&] {
+        NEED(md);
+        const bool r = md->sys->update();
+        if (rebuilt) *rebuilt = r ? 1 : 0;
+    });
+}
+
+mapp_status mapp_md_force_calc(mapp_md* md, double* f, double* out7)
+{
+    return guard([&] {
+        NEED(md);
+        const mapp::MDReport r = md->sys->force_calc();
+        if (out7) std::memcpy(out7, r.e, sizeof r.e);
+        if (f) md->sys->get_f(f);
+    });
+}
+
+mapp_status mapp_md_value(mapp_md* md, double* energy)
+{
+    return guard([&] {
+        NEED(md);
+        NEED(energy);
+        *energy = md->sys->value();
+    });
+}
+
+mapp_status mapp_md_get_rho(mapp_md* md, double* rho)
+{
+    return guard([&] { NEED(md); NEED(rho); md->sys->get_rho(rho); });
+}
+
+mapp_status mapp_md_pairs_count(mapp_md* md, int64_t* npairs)
+{
+    return guard([&] { NEED(md); NEED(npairs); *npairs = md->sys->half_pairs_count(); });
+}
+
+mapp_status mapp_md_pairs(mapp_md* md, int32_t* i, int32_t* j, int8_t* img)
+{
+    return guard([&] { NEED(md); NEED(i); NEED(j); NEED(img); md->sys->half_pairs(i, j, img); });
+}
+
+mapp_status mapp_md_list_stats(mapp_md* md, int* max_neighbors, int64_t* full_entries, int* rebuilds)
+{
+    return guard([&] {
+        NEED(md);
+        int m = 0;
+        int64_t s = 0;
+        md->sys->list_stats(&m, &s);
+        if (max_neighbors) *max_neighbors = m;
+        if (full_entries) *full_entries = s;
+        if (rebuilds) *rebuilds = md->sys->last_rebuilds();
+    });
+}
+
+mapp_status mapp_md_nvt_run(mapp_md* md, const double* masses, double kB, double T, double dt, int nsteps,
+                            double* out7, double* T_part, int* rebuilds)
+{
+    return guard([&] {
+        NEED(md);
+        NEED(masses);
+        mapp::MDReport r{};
+        const int nr = md->sys->nvt_run(masses, kB, T, dt, nsteps, &r, T_part);
+        if (out7) std::memcpy(out7, r.e, sizeof r.e);
+        if (rebuilds) *rebuilds = nr;
+    });
+}
+
+mapp_status mapp_create_T(int natms, const uint8_t* elem, const double* masses, double kB, double T, int seed,
+                          double* v)
+{
+    return guard([&] {
+        NEED(elem);
+        NEED(masses);
+        NEED(v);
+        mapp::ParkMiller rng(seed);
+        for (int i = 0; i < natms; ++i) {
+            const double fac = std::sqrt(kB * T / masses[elem[i]]);
+            for (int d = 0; d < 3; ++d) v[3 * (size_t)i + d] = rng.gaussian() * fac;
+        }
+    });
+}
+
+}  // extern "C"

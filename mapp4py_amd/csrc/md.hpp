@@ -1,0 +1,123 @@
+// md.hpp -- device-resident MD system: atoms, neighbor list, tabulated EAM.
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "box.hpp"
+#include "common.hpp"
+#include "tables.hpp"
+
+namespace mapp {
+
+constexpr int kMaxElems = 8;
+constexpr int kIndexBits = 25;                 // neighbor entry: j in the low bits,
+constexpr uint32_t kIndexMask = (1u << kIndexBits) - 1u;  // image code in the high 7
+constexpr int kZeroImage = 62;                  // code of the (0,0,0) shift
+
+// Kernel argument block describing the EAM tables on the device.
+struct EamArgs {
+    int ne, nr, nrho;
+    double dr_inv, drho_inv, rho_max;
+    float dr_invf;
+    double cut_sq[kMaxElems * kMaxElems];
+    const float4* tabA;   // [ne*ne][nr]: density at i due to j (rho[tj][ti])
+    const float4* tabB;   // [ne*ne][nr][3]: rho[tj][ti], rho[ti][tj], r*phi[ti][tj]
+    const double* Ftab;   // [ne][nrho][4]
+};
+
+// ForceFieldEAM with its tables uploaded once (pair splines in FP32, the
+// embedding function in FP64).
+class EamForceField {
+public:
+    explicit EamForceField(EamTables t);
+    const EamTables& tables() const { return t_; }
+    EamArgs args() const;
+private:
+    EamTables t_;
+    DevBuf<float4> tabA_, tabB_;
+    DevBuf<double> Ftab_;
+};
+
+// Box, cell grid and search radius of one neighbor build.
+struct CellGrid {
+    int nc[3];
+    int R[3];
+    int ncells;
+    double cut_s[3];
+};
+
+struct MDReport {
+    double e[7];   // energy, F_H Voigt (xx,xy,xz,yy,yz,zz)
+};
+
+// AtomsMD + NeighborMD + DynamicMD + ForceFieldEAM state resident in HBM.
+// Atoms are kept in cell order ("slots"); id[slot] is the caller's index.
+class MDSystem {
+public:
+    MDSystem(std::shared_ptr<EamForceField> ff, int natms, const double* H, const double* x,
+             const uint8_t* elem, double skin);
+    ~MDSystem();
+
+    int natms() const { return n_; }
+    void set_x(const double* x_host);          // caller order
+    void get_x(double* x_host);
+    void set_v(const double* v_host);
+    void get_v(double* v_host);
+    void get_f(double* f_host);
+    void get_rho(double* rho_host);
+    // DynamicMD::update: returns true if the list was rebuilt
+    bool update();
+    void rebuild();
+    // ForceFieldMD::force_calc -> energy + F_H
+    MDReport force_calc();
+    double value();
+    int64_t half_pairs_count();
+    void half_pairs(int32_t* i, int32_t* j, int8_t* img);
+    void list_stats(int* maxn, int64_t* nfull);
+    // MDNVT::run; masses per element; returns rebuild count
+    int nvt_run(const double* masses, double kB, double T, double dt, int nsteps, MDReport* rep, double* T_part);
+    hipStream_t stream() const { return stream_; }
+    void sync();
+    int last_rebuilds() const { return nrebuild_; }
+
+private:
+    void setup_grid();
+    void build_list();
+    void force_passes(bool energy_virial);
+    MDReport reduce_report(int nparts);
+    bool decide();
+
+    std::shared_ptr<EamForceField> ff_;
+    int n_;
+    double skin_, max_cut_;
+    Box box_;
+    CellGrid grid_{};
+    hipStream_t stream_ = nullptr;
+    int nrebuild_ = 0;
+
+    DevBuf<double4> x_, x_alt_;      // x,y,z,type
+    DevBuf<double4> s_;              // fractional coordinates at the last build
+    DevBuf<double4> x0_;             // positions at the last build
+    DevBuf<double4> v_, v_alt_;      // velocities (w unused)
+    DevBuf<int> id_, id_alt_;        // slot -> caller index
+    DevBuf<int> slot_of_;            // caller index -> slot
+    DevBuf<float4> f_;               // forces (w unused)
+    DevBuf<double> rho_;
+    DevBuf<float> dF_;
+    DevBuf<uint32_t> keys_, keys_alt_;
+    DevBuf<int> vals_, vals_alt_;
+    DevBuf<double4> s_tmp_;
+    DevBuf<unsigned char> sort_tmp_;
+    size_t sort_tmp_bytes_ = 0;
+    DevBuf<int> cell_rank_, cell_start_;
+    DevBuf<int> nn_;
+    DevBuf<uint32_t> list_;
+    int maxn_ = 0;
+    int ldl_ = 0;                    // leading dimension of the list (slots)
+    DevBuf<int> flags_;              // [0] overflow max count, [1] rebuild needed
+    DevBuf<double> part_;            // per-block partial sums
+    DevBuf<double> red_;             // reduced values
+    std::vector<double> masses_;
+};
+
+}  // namespace mapp

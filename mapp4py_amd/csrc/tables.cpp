@@ -1,0 +1,227 @@
+// tables.cpp -- DYNAMO setfl / fs readers and the spline coefficient builder.
+//
+// Behaviour follows the reference importer (src/import_eam.h:52-64, 348-901;
+// src/import_eam.cpp:62-115, 150-182): tokens are read across lines, '#'
+// starts a comment, the per-element header line is skipped, the cutoff in
+// the grid line is ignored and r_c is derived from the last non-zero knot.
+#include "tables.hpp"
+
+#include <cctype>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+#include "common.hpp"
+
+namespace mapp {
+
+void spline_coefficients(int n, double* s, int stride)
+{
+    MAPP_REQUIRE(n >= 5, ERR_ARG, "spline needs at least 5 knots");
+    auto y = [&](int k) { return s[(size_t)k * stride]; };
+    auto c = [&](int k, int col) -> double& { return s[(size_t)k * stride + col]; };
+    // slopes (per unit p): one-sided at both ends, 3-point next to them,
+    // 5-point (exact for quartics) inside
+    c(0, 1) = y(1) - y(0);
+    c(1, 1) = 0.5 * (y(2) - y(0));
+    c(n - 2, 1) = 0.5 * (y(n - 1) - y(n - 3));
+    c(n - 1, 1) = y(n - 1) - y(n - 2);
+    for (int k = 2; k < n - 2; ++k) c(k, 1) = ((y(k - 2) - y(k + 2)) + 8.0 * (y(k + 1) - y(k - 1))) / 12.0;
+    // Hermite cubic on each interval from the end values and slopes
+    for (int k = 0; k < n - 1; ++k) {
+        const double dy = y(k + 1) - y(k);
+        c(k, 2) = 3.0 * dy - 2.0 * c(k, 1) - c(k + 1, 1);
+        c(k, 3) = c(k, 1) + c(k + 1, 1) - 2.0 * dy;
+    }
+    c(n - 1, 2) = 0.0;
+    c(n - 1, 3) = 0.0;
+    if (stride == 5) {
+        for (int k = 0; k < n - 1; ++k) c(k, 4) = (c(k + 1, 2) - c(k, 2)) / 6.0 - 0.5 * c(k, 3);
+        c(n - 1, 4) = 0.0;
+    }
+}
+
+namespace {
+
+EamTables alloc_tables(int ne, int nr, int nrho, double dr, double drho, bool dmd)
+{
+    EamTables t;
+    t.ne = ne;
+    t.nr = nr;
+    t.nrho = nrho;
+    t.dr = dr;
+    t.drho = drho;
+    t.dmd = dmd;
+    t.rphi.assign((size_t)ne * ne * nr * 4, 0.0);
+    t.rho.assign((size_t)ne * ne * nr * 4, 0.0);
+    t.F.assign((size_t)ne * nrho * t.nf(), 0.0);
+    t.rc.assign((size_t)ne * ne, 0.0);
+    return t;
+}
+
+double* mrho(EamTables& t, int a, int b) { return t.rho.data() + ((size_t)a * t.ne + b) * t.nr * 4; }
+double* mphi(EamTables& t, int a, int b) { return t.rphi.data() + ((size_t)a * t.ne + b) * t.nr * 4; }
+
+// splines, r_c from the knots, and for the DMD force field the densities are
+// turned into r*rho(r) before their final spline (src/ff_eam_dmd.cpp:80-94)
+void finalize(EamTables& t)
+{
+    const int ne = t.ne;
+    for (int a = 0; a < ne; ++a) {
+        spline_coefficients(t.nrho, t.F.data() + (size_t)a * t.nrho * t.nf(), t.nf());
+        for (int b = 0; b < ne; ++b) {
+            spline_coefficients(t.nr, mrho(t, a, b), 4);
+            spline_coefficients(t.nr, mphi(t, a, b), 4);
+        }
+    }
+    for (int a = 0; a < ne; ++a)
+        for (int b = 0; b <= a; ++b) {
+            int k = t.nr - 1;
+            while (k >= 1 && mphi(t, a, b)[(size_t)k * 4] == 0.0 && mrho(t, a, b)[(size_t)k * 4] == 0.0 &&
+                   mrho(t, b, a)[(size_t)k * 4] == 0.0)
+                --k;
+            t.rc[(size_t)a * ne + b] = t.rc[(size_t)b * ne + a] = (double)k * t.dr;
+        }
+    if (t.dmd) {
+        for (int a = 0; a < ne; ++a)
+            for (int b = 0; b < ne; ++b) {
+                double* s = mrho(t, a, b);
+                for (int k = 0; k < t.nr; ++k) s[(size_t)k * 4] *= (double)k * t.dr;
+                spline_coefficients(t.nr, s, 4);
+            }
+    }
+}
+
+struct LineReader {
+    std::ifstream in;
+    std::string path;
+    int line_no = 0;
+    explicit LineReader(const std::string& p) : in(p), path(p) {
+        MAPP_REQUIRE(in.good(), ERR_FILE, "cannot open potential file " + p);
+    }
+    bool next(std::string& line) {
+        if (!std::getline(in, line)) return false;
+        ++line_no;
+        return true;
+    }
+    void need(std::string& line) {
+        MAPP_REQUIRE(next(line), ERR_FILE, path + " file ended immaturely");
+    }
+    static std::vector<std::string> split(const std::string& line) {
+        std::string s = line.substr(0, line.find('#'));
+        std::istringstream is(s);
+        std::vector<std::string> out;
+        std::string tok;
+        while (is >> tok) out.push_back(tok);
+        return out;
+    }
+    // n numbers spread over as many lines as needed
+    void numbers(std::vector<double>& out, size_t n) {
+        out.resize(n);
+        size_t i = 0;
+        std::string line;
+        while (i < n) {
+            need(line);
+            auto tok = split(line);
+            MAPP_REQUIRE(tok.size() + i <= n, ERR_FILE, path + " file has extra arguments at line " + std::to_string(line_no));
+            for (auto& s : tok) out[i++] = std::atof(s.c_str());
+        }
+    }
+};
+
+}  // namespace
+
+EamTables read_dynamo(const std::string& path, int fmt, const std::vector<std::string>& elems, bool dmd)
+{
+    MAPP_REQUIRE(fmt == 0 || fmt == 1, ERR_ARG, "unknown EAM file format");
+    LineReader rd(path);
+    std::string line;
+    for (int k = 0; k < 3; ++k) rd.need(line);
+    rd.need(line);
+    auto tok = LineReader::split(line);
+    MAPP_REQUIRE(tok.size() >= 2, ERR_FILE, "invalid element line in " + path + ": " + line);
+    const int nfile = (int)tok.size() - 1;
+    std::vector<std::string> file_names(tok.begin() + 1, tok.end());
+    std::vector<int> map(nfile, -1);  // file element -> requested slot
+    std::vector<std::string> names = elems.empty() ? file_names : elems;
+    const int ne = (int)names.size();
+    for (int f = 0; f < nfile; ++f)
+        for (int e = 0; e < ne; ++e)
+            if (names[e] == file_names[f] && map[f] < 0) {
+                bool taken = false;
+                for (int g = 0; g < f; ++g) taken |= map[g] == e;
+                if (!taken) map[f] = e;
+            }
+    for (int e = 0; e < ne; ++e) {
+        bool found = false;
+        for (int f = 0; f < nfile; ++f) found |= map[f] == e;
+        MAPP_REQUIRE(found, ERR_FILE, path + " file does not contain parameters for element " + names[e]);
+    }
+    rd.need(line);
+    size_t nrho = 0, nr = 0;
+    double drho = 0.0, dr = 0.0, rc_hdr = 0.0;
+    {
+        std::istringstream is(line);
+        MAPP_REQUIRE(bool(is >> nrho >> drho >> nr >> dr >> rc_hdr), ERR_FILE, "invalid grid line in " + path + ": " + line);
+    }
+    MAPP_REQUIRE(nrho >= 5 && nr >= 5, ERR_FILE, "nr and nrho in " + path + " should be larger than 5");
+    MAPP_REQUIRE(drho > 0.0 && dr > 0.0, ERR_FILE, "dr and drho in " + path + " should be larger than 0.0");
+    EamTables t = alloc_tables(ne, (int)nr, (int)nrho, dr, drho, dmd);
+    t.names = names;
+    std::vector<double> buf;
+    const size_t nblock = fmt == 1 ? nrho + nr * (size_t)nfile : nrho + nr;
+    for (int f = 0; f < nfile; ++f) {
+        rd.need(line);  // atomic number, mass, lattice constant, lattice
+        rd.numbers(buf, nblock);
+        const int a = map[f];
+        if (a < 0) continue;
+        for (size_t k = 0; k < nrho; ++k) t.F[((size_t)a * nrho + k) * t.nf()] = buf[k];
+        for (int g = 0; g < nfile; ++g) {
+            const int b = fmt == 1 ? map[g] : -1;
+            if (fmt == 1 && b < 0) continue;
+            const double* src = buf.data() + nrho + (fmt == 1 ? (size_t)g * nr : 0);
+            if (fmt == 1) {
+                for (size_t k = 0; k < nr; ++k) mrho(t, a, b)[k * 4] = src[k];
+            } else {
+                for (int bb = 0; bb < ne; ++bb)
+                    for (size_t k = 0; k < nr; ++k) mrho(t, a, bb)[k * 4] = src[k];
+                break;
+            }
+        }
+    }
+    rd.numbers(buf, (size_t)nfile * (nfile + 1) / 2 * nr);
+    size_t pos = 0;
+    for (int f = 0; f < nfile; ++f)
+        for (int g = 0; g <= f; ++g) {
+            const int a = map[f], b = map[g];
+            if (a >= 0 && b >= 0)
+                for (size_t k = 0; k < nr; ++k) mphi(t, a, b)[k * 4] = mphi(t, b, a)[k * 4] = buf[pos + k];
+            pos += nr;
+        }
+    finalize(t);
+    return t;
+}
+
+EamTables tables_from_knots(int ne, int nr, int nrho, double dr, double drho, const double* F,
+                            const double* rho, const double* r_phi, bool setfl_alias, bool dmd)
+{
+    MAPP_REQUIRE(ne >= 1 && ne <= 8, ERR_ARG, "1 to 8 elements are supported");
+    MAPP_REQUIRE(nr >= 5 && nrho >= 5 && dr > 0.0 && drho > 0.0, ERR_ARG, "invalid table grid");
+    EamTables t = alloc_tables(ne, nr, nrho, dr, drho, dmd);
+    for (int a = 0; a < ne; ++a) {
+        t.names.push_back("E" + std::to_string(a));
+        for (int k = 0; k < nrho; ++k) t.F[((size_t)a * nrho + k) * t.nf()] = F[(size_t)a * nrho + k];
+        for (int b = 0; b < ne; ++b) {
+            const int bb = setfl_alias ? 0 : b;
+            for (int k = 0; k < nr; ++k) {
+                mrho(t, a, b)[(size_t)k * 4] = rho[((size_t)a * ne + bb) * nr + k];
+                mphi(t, a, b)[(size_t)k * 4] = r_phi[((size_t)a * ne + b) * nr + k];
+            }
+        }
+    }
+    finalize(t);
+    return t;
+}
+
+}  // namespace mapp

@@ -1,0 +1,399 @@
+/*
+ * o_dmd.c -- oracle: DMD (Gaussian-smeared EAM) neighbor list, free energy,
+ * forces on x and alpha, and the per-component chemical-potential sums.
+ *
+ * TEST INFRASTRUCTURE ONLY (the oracle).
+ */
+#include "o_internal.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define O_PI_IN_SQ 0.564189583547756286948079451561 /* 1/sqrt(pi) */
+
+struct o_dmd {
+    const o_eam* ff;
+    o_box bx;
+    o_ghosts g;
+    int n, c_dim, N;
+    double skin, max_cut, scl, kBT;
+    double xi[32], wi[32];
+    double* cut;     /* [ne*ne] r_c */
+    double* cut_sk;  /* [ne*ne] r_c + skin */
+    double* mu0;     /* [ne] */
+    uint8_t* elem_lcl;
+    double *alpha_lcl, *c_lcl;   /* [n*c_dim] */
+    uint8_t* elem;   /* [nall*c_dim] */
+    double *alpha, *c;           /* [nall*c_dim] */
+    int64_t* row;    /* [n*c_dim + 1] */
+    int32_t* col;    /* component indices j*c_dim + jc */
+    int64_t npairs, cap;
+    int32_t** tl;    /* per-component scratch lists while visiting site i */
+    int* tl_n;
+    int tl_cap;
+};
+
+static inline double hval(const double* c, double p) { return ((c[3] * p + c[2]) * p + c[1]) * p + c[0]; }
+
+/* ghost copies of the per-component arrays follow the owner */
+static void dmd_sync_ghost_state(o_dmd* d)
+{
+    int cd = d->c_dim;
+    d->elem = (uint8_t*)realloc(d->elem, (size_t)d->g.nall * cd + 1);
+    d->alpha = (double*)realloc(d->alpha, sizeof(double) * ((size_t)d->g.nall * cd + 1));
+    d->c = (double*)realloc(d->c, sizeof(double) * ((size_t)d->g.nall * cd + 1));
+    for (int k = 0; k < d->g.nall; k++) {
+        int o = d->g.owner[k];
+        for (int ic = 0; ic < cd; ic++) {
+            d->elem[k * cd + ic] = d->elem_lcl[o * cd + ic];
+            d->alpha[k * cd + ic] = d->alpha_lcl[o * cd + ic];
+            d->c[k * cd + ic] = d->c_lcl[o * cd + ic];
+        }
+    }
+}
+
+typedef struct { o_dmd* d; } dctx;
+
+/* NeighborDMD::create_list mid() (src/neighbor_dmd.cpp:150-167) */
+static void dmd_visit(void* v, int i, int j)
+{
+    o_dmd* d = ((dctx*)v)->d;
+    if (i >= j) return;
+    int cd = d->c_dim, ne = d->ff->ne;
+    double r = sqrt(o_rsq(d->g.x + 3 * i, d->g.x + 3 * j));
+    const double* ci = d->c + (size_t)i * cd;
+    const double* cj = d->c + (size_t)j * cd;
+    const double* ai = d->alpha + (size_t)i * cd;
+    const double* aj = d->alpha + (size_t)j * cd;
+    const uint8_t* ei = d->elem + (size_t)i * cd;
+    const uint8_t* ej = d->elem + (size_t)j * cd;
+    for (int ic = 0; ic < cd && ci[ic] >= 0.0; ic++)
+        for (int jc = 0; jc < cd && cj[jc] >= 0.0; jc++)
+            if (r - d->scl * sqrt(ai[ic] * ai[ic] + aj[jc] * aj[jc]) < d->cut_sk[ei[ic] * ne + ej[jc]]) {
+                if (d->tl_n[ic] == d->tl_cap) {
+                    d->tl_cap = d->tl_cap * 2 + 256;
+                    for (int q = 0; q < cd; q++) d->tl[q] = (int32_t*)realloc(d->tl[q], sizeof(int32_t) * d->tl_cap);
+                }
+                d->tl[ic][d->tl_n[ic]++] = j * cd + jc;
+            }
+}
+
+static void dmd_post(void* v, int i)
+{
+    o_dmd* d = ((dctx*)v)->d;
+    int cd = d->c_dim;
+    for (int ic = 0; ic < cd; ic++) {
+        int64_t need = d->npairs + d->tl_n[ic];
+        if (need > d->cap) {
+            d->cap = need * 2 + 1024;
+            d->col = (int32_t*)realloc(d->col, sizeof(int32_t) * d->cap);
+        }
+        memcpy(d->col + d->npairs, d->tl[ic], sizeof(int32_t) * d->tl_n[ic]);
+        d->npairs += d->tl_n[ic];
+        d->row[(size_t)i * cd + ic + 1] = d->npairs;
+        d->tl_n[ic] = 0;
+    }
+}
+
+static void dmd_rebuild(o_dmd* d, const double* x_lcl)
+{
+    double* s = (double*)malloc(sizeof(double) * 3 * (d->n > 0 ? d->n : 1));
+    for (int i = 0; i < d->n; i++) o_x2s(&d->bx, x_lcl + 3 * i, s + 3 * i);
+    o_ghosts_build(&d->g, &d->bx, d->n, s, d->max_cut);
+    free(s);
+    dmd_sync_ghost_state(d);
+    d->row[0] = 0;
+    d->npairs = 0;
+    dctx c = {d};
+    o_cell_do(&d->g, &d->bx, d->max_cut, NULL, dmd_visit, dmd_post, &c);
+}
+
+o_dmd* o_dmd_create(const o_eam* ff, int natms, int c_dim, int nquad,
+                    const double* H, const double* x, const uint8_t* elem,
+                    const double* alpha, const double* c, double skin,
+                    double kB, double T, double hP, const double* masses)
+{
+    o_dmd* d = (o_dmd*)calloc(1, sizeof(o_dmd));
+    int ne = ff->ne;
+    d->ff = ff;
+    d->n = natms;
+    d->c_dim = c_dim;
+    d->N = nquad;
+    d->skin = skin;
+    o_quadrature_hg(nquad, d->xi, d->wi);
+    d->scl = d->xi[nquad - 1];
+    o_box_set(&d->bx, H);
+    d->kBT = kB * T;
+    d->cut = (double*)malloc(sizeof(double) * ne * ne);
+    d->cut_sk = (double*)malloc(sizeof(double) * ne * ne);
+    d->mu0 = (double*)malloc(sizeof(double) * ne);
+    double ff_max_cut = 0.0;
+    /* ForceFieldDMD::pre_init (src/ff_dmd.cpp:157-204) */
+    for (int i = 0; i < ne; i++) {
+        double lambda = hP / sqrt(2.0 * M_PI * d->kBT * masses[i]);
+        d->mu0[i] = d->kBT * 1.5 * (2.0 * log(lambda) - 1.0 - log(M_PI));
+        for (int j = 0; j <= i; j++) {
+            double t = ff->rc[i * ne + j];
+            if (t > ff_max_cut) ff_max_cut = t;
+            d->cut[i * ne + j] = d->cut[j * ne + i] = t;
+            d->cut_sk[i * ne + j] = d->cut_sk[j * ne + i] = t + skin;
+        }
+    }
+    double max_alpha = 0.0;
+    for (int k = 0; k < natms * c_dim; k++)
+        if (c[k] != -1.0 && alpha[k] > max_alpha) max_alpha = alpha[k];
+    /* atoms->max_cut (src/dynamic_dmd.cpp:46) */
+    d->max_cut = ff_max_cut + skin + d->scl * sqrt(2.0) * max_alpha;
+    d->elem_lcl = (uint8_t*)malloc((size_t)natms * c_dim + 1);
+    memcpy(d->elem_lcl, elem, (size_t)natms * c_dim);
+    d->alpha_lcl = (double*)malloc(sizeof(double) * ((size_t)natms * c_dim + 1));
+    d->c_lcl = (double*)malloc(sizeof(double) * ((size_t)natms * c_dim + 1));
+    memcpy(d->alpha_lcl, alpha, sizeof(double) * natms * c_dim);
+    memcpy(d->c_lcl, c, sizeof(double) * natms * c_dim);
+    d->row = (int64_t*)malloc(sizeof(int64_t) * ((size_t)natms * c_dim + 1));
+    d->tl = (int32_t**)calloc(c_dim, sizeof(int32_t*));
+    d->tl_n = (int*)calloc(c_dim, sizeof(int));
+    dmd_rebuild(d, x);
+    return d;
+}
+
+void o_dmd_free(o_dmd* d)
+{
+    if (!d) return;
+    o_ghosts_free(&d->g);
+    for (int q = 0; q < d->c_dim; q++) free(d->tl[q]);
+    free(d->tl); free(d->tl_n);
+    free(d->cut); free(d->cut_sk); free(d->mu0); free(d->elem_lcl); free(d->alpha_lcl); free(d->c_lcl);
+    free(d->elem); free(d->alpha); free(d->c); free(d->row); free(d->col);
+    free(d);
+}
+
+int64_t o_dmd_npairs(const o_dmd* d) { return d->npairs; }
+
+void o_dmd_pairs(const o_dmd* d, int32_t* pi, int32_t* pj, int8_t* img)
+{
+    int cd = d->c_dim;
+    for (int i = 0; i < d->n * cd; i++)
+        for (int64_t k = d->row[i]; k < d->row[i + 1]; k++) {
+            int j = d->col[k];
+            int js = j / cd;
+            pi[k] = i;
+            pj[k] = d->g.owner[js] * cd + j % cd;
+            memcpy(img + 3 * k, d->g.img + 3 * js, 3);
+        }
+}
+
+/* new state between list builds: ghosts follow their owners */
+void o_dmd_set_state(o_dmd* d, const double* x, const double* alpha, const double* c)
+{
+    memcpy(d->alpha_lcl, alpha, sizeof(double) * d->n * d->c_dim);
+    memcpy(d->c_lcl, c, sizeof(double) * d->n * d->c_dim);
+    o_ghosts_refresh(&d->g, &d->bx, x);
+    dmd_sync_ghost_state(d);
+}
+
+/* Smeared phi, rho_ij, rho_ji and their r/alpha derivatives by N-point
+ * Gauss-Hermite over the odd extension of r*f(r) (src/ff_eam_dmd.cpp:195-253) */
+static void smear(const o_dmd* d, int ei, int ej, double r, double a, double* rp, double* dr_, double* da_)
+{
+    const o_eam* e = d->ff;
+    double dr_inv = 1.0 / e->dr;
+    double H[3][3] = {{0}};
+    for (int l = 0; l < d->N; l++) {
+        double xl = d->xi[l], xl2 = xl * xl;
+        double u = r - xl * a;
+        size_t m;
+        double p;
+        o_spline_pm(fabs(u), dr_inv, e->nr, &m, &p);
+        const double* tabs[3] = {OPHI(e, ei, ej)[m], ORHO(e, ei, ej)[m], ORHO(e, ej, ei)[m]};
+        for (int k = 0; k < 3; k++) {
+            double t = hval(tabs[k], p);
+            if (u < 0.0) t *= -1.0;
+            t *= d->wi[l];
+            H[k][0] += t;
+            H[k][1] += t * xl;
+            H[k][2] += t * xl2;
+        }
+    }
+    double r_inv = 1.0 / r, r2_inv = r_inv * r_inv;
+    double a_inv = 1.0 / a, a2_inv = a_inv * a_inv;
+    for (int k = 0; k < 3; k++) {
+        rp[k] = O_PI_IN_SQ * r_inv * H[k][0];
+        dr_[k] = -r2_inv * (rp[k] + 2.0 * O_PI_IN_SQ * a_inv * H[k][1]);
+        da_[k] = -a2_inv * (rp[k] - 2.0 * O_PI_IN_SQ * r_inv * H[k][2]);
+    }
+}
+
+void o_dmd_pair_eval(const o_dmd* d, int ei, int ej, double r, double alpha_ij, double* out9)
+{
+    smear(d, ei, ej, r, alpha_ij, out9, out9 + 3, out9 + 6);
+}
+
+static double ent(double x) { return x == 0.0 ? 0.0 : x * log(x); }
+
+static void femb(const o_dmd* d, int el, double rho, double* F, double* dF)
+{
+    const o_eam* e = d->ff;
+    double drho_inv = 1.0 / e->drho, rho_max = (double)e->nrho * e->drho;
+    size_t m;
+    double p;
+    o_spline_pm(rho, drho_inv, e->nrho, &m, &p);
+    const double* c = OF(e, el) + m * 5;
+    double t0 = (((c[4] * p + c[3]) * p + c[2]) * p + c[1]) * p + c[0];
+    double t1 = (((4.0 * c[4] * p + 3.0 * c[3]) * p + 2.0 * c[2]) * p + c[1]) * drho_inv;
+    if (rho > rho_max) t0 += t1 * (rho - rho_max);
+    *F = t0;
+    *dF = t1;
+}
+
+/* ForceFieldEAMDMD::__force_calc (src/ff_eam_dmd.cpp:126-360) followed by
+ * the reductions of ForceFieldDMD::derivative (src/ff_dmd.cpp:220-237) */
+void o_dmd_force(o_dmd* d, double* fx, double* falpha, double* mu, double* out8)
+{
+    int cd = d->c_dim, ne = d->ff->ne;
+    int n = d->n * cd;
+    int nall = d->g.nall * cd;
+    const double* x = d->g.x;
+    const double* c = d->c;
+    const double* al = d->alpha;
+    const uint8_t* el = d->elem;
+    double* rp = (double*)calloc((size_t)d->npairs * 3 + 1, sizeof(double));
+    double* drp = (double*)calloc((size_t)d->npairs * 3 + 1, sizeof(double));
+    double* dap = (double*)calloc((size_t)d->npairs * 3 + 1, sizeof(double));
+    double* rho = (double*)calloc((size_t)n + 1, sizeof(double));
+    double* dE = (double*)calloc((size_t)nall + 1, sizeof(double));
+    double v[9] = {0};
+    for (int i = 0; i < 3 * d->n; i++) fx[i] = 0.0;
+    for (int i = 0; i < n; i++) { falpha[i] = 0.0; mu[i] = 0.0; }
+    for (int i = 0; i < n; i++) {
+        double ci = c[i];
+        if (ci < 0.0) continue;
+        int ei = el[i];
+        for (int64_t k = d->row[i]; k < d->row[i + 1]; k++) {
+            int j = d->col[k];
+            int ej = el[j];
+            double r = sqrt(o_rsq(x + 3 * (i / cd), x + 3 * (j / cd)));
+            double a = sqrt(al[i] * al[i] + al[j] * al[j]);
+            if (r - a * d->xi[d->N - 1] >= d->cut[ei * ne + ej]) continue;
+            smear(d, ei, ej, r, a, rp + 3 * k, drp + 3 * k, dap + 3 * k);
+            rho[i] += c[j] * rp[3 * k + 2];
+            if (j < n) {
+                rho[j] += ci * rp[3 * k + 1];
+                v[0] += ci * c[j] * rp[3 * k];
+            } else
+                v[0] += 0.5 * ci * c[j] * rp[3 * k];
+        }
+        double F, dF;
+        femb(d, ei, rho[i], &F, &dF);
+        rho[i] = F;
+        dE[i] = dF;
+        mu[i] = F;
+        v[0] += ci * (F + d->mu0[ei] - 3.0 * d->kBT * log(al[i]));
+        v[7] += ent(ci);
+    }
+    for (int i = 0; i < d->n; i++) {
+        double cv = 1.0;
+        for (int ic = 0; ic < cd; ic++)
+            if (c[i * cd + ic] > 0.0) cv -= c[i * cd + ic];
+        v[7] += ent(cv);
+    }
+    v[0] += d->kBT * v[7];
+    /* ghost update of dE (src/ff_eam_dmd.cpp:302) */
+    for (int k = d->n; k < d->g.nall; k++)
+        for (int ic = 0; ic < cd; ic++) dE[k * cd + ic] = dE[d->g.src[k] * cd + ic];
+    double f_i[3] = {0, 0, 0};
+    for (int i = 0; i < n; i++) {
+        if (i % cd == 0) f_i[0] = f_i[1] = f_i[2] = 0.0;
+        const double* xi = x + 3 * (i / cd);
+        double ci = c[i], ai = al[i], dEi = dE[i];
+        double fa = 0.0, mui = 0.0;
+        for (int64_t k = d->row[i]; k < d->row[i + 1]; k++) {
+            int j = d->col[k];
+            const double* xj = x + 3 * (j / cd);
+            double dx[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+            double fp = -(drp[3 * k + 2] * dEi + drp[3 * k + 1] * dE[j] + drp[3 * k]) * ci * c[j];
+            double ap = -(dap[3 * k + 2] * dEi + dap[3 * k + 1] * dE[j] + dap[3 * k]) * ci * c[j];
+            mui += c[j] * (rp[3 * k] + rp[3 * k + 1] * dE[j]);
+            if (j < n) mu[j] += ci * (rp[3 * k] + rp[3 * k + 2] * dEi);
+            for (int q = 0; q < 3; q++) f_i[q] += fp * dx[q];
+            fa += ai * ap;
+            if (j < n) {
+                for (int q = 0; q < 3; q++) fx[3 * (j / cd) + q] += -fp * dx[q];
+                falpha[j] += al[j] * ap;
+            } else
+                fp *= 0.5;
+            /* DyadicV(-fp, dx, W): W += dx_a dx_b (-fp) */
+            v[1] += dx[0] * dx[0] * -fp; v[2] += dx[0] * dx[1] * -fp; v[3] += dx[0] * dx[2] * -fp;
+            v[4] += dx[1] * dx[1] * -fp; v[5] += dx[1] * dx[2] * -fp; v[6] += dx[2] * dx[2] * -fp;
+        }
+        fa += 3.0 * d->kBT * ci / ai;
+        falpha[i] += fa;
+        mu[i] += mui;
+        if ((i + 1) % cd == 0)
+            for (int q = 0; q < 3; q++) fx[3 * (i / cd) + q] += f_i[q];
+    }
+    out8[0] = v[0];
+    for (int q = 1; q < 7; q++) out8[q] = -v[q];
+    out8[7] = v[7];
+    free(rp); free(drp); free(dap); free(rho); free(dE);
+}
+
+/* ForceFieldEAMDMD::__energy_calc (src/ff_eam_dmd.cpp:835-953) */
+double o_dmd_energy(o_dmd* d)
+{
+    int cd = d->c_dim, ne = d->ff->ne;
+    int n = d->n * cd;
+    const o_eam* e = d->ff;
+    const double* x = d->g.x;
+    const double* c = d->c;
+    const double* al = d->alpha;
+    const uint8_t* el = d->elem;
+    double dr_inv = 1.0 / e->dr;
+    double* rho = (double*)calloc((size_t)n + 1, sizeof(double));
+    double en = 0.0, s = 0.0;
+    for (int i = 0; i < n; i++) {
+        int ei = el[i];
+        for (int64_t k = d->row[i]; k < d->row[i + 1]; k++) {
+            int j = d->col[k];
+            int ej = el[j];
+            double r = sqrt(o_rsq(x + 3 * (i / cd), x + 3 * (j / cd)));
+            double a = sqrt(al[i] * al[i] + al[j] * al[j]);
+            if (r - a * d->xi[d->N - 1] >= d->cut[ei * ne + ej]) continue;
+            double arr[3] = {0, 0, 0};
+            for (int l = 0; l < d->N; l++) {
+                double u = r - d->xi[l] * a;
+                size_t m;
+                double p;
+                o_spline_pm(fabs(u), dr_inv, e->nr, &m, &p);
+                const double* tabs[3] = {OPHI(e, ei, ej)[m], ORHO(e, ei, ej)[m], ORHO(e, ej, ei)[m]};
+                for (int q = 0; q < 3; q++) {
+                    double t = hval(tabs[q], p);
+                    if (u < 0.0) t *= -1.0;
+                    arr[q] += d->wi[l] * t;
+                }
+            }
+            double t0 = O_PI_IN_SQ * (1.0 / r);
+            for (int q = 0; q < 3; q++) arr[q] *= t0;
+            rho[i] += c[j] * arr[2];
+            if (j < n) {
+                rho[j] += c[i] * arr[1];
+                en += c[i] * c[j] * arr[0];
+            } else
+                en += 0.5 * c[i] * c[j] * arr[0];
+        }
+        if (c[i] < 0.0) continue;
+        double F, dF;
+        femb(d, ei, rho[i], &F, &dF);
+        if (c[i] != 0.0) en += c[i] * (F + d->mu0[ei] - 3.0 * d->kBT * log(al[i]));
+        s += ent(c[i]);
+    }
+    for (int i = 0; i < d->n; i++) {
+        double cv = 1.0;
+        for (int ic = 0; ic < cd; ic++)
+            if (c[i * cd + ic] > 0.0) cv -= c[i * cd + ic];
+        s += ent(cv);
+    }
+    en += d->kBT * s;
+    free(rho);
+    return en;
+}

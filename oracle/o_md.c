@@ -1,0 +1,661 @@
+/*
+ * o_md.c -- oracle: periodic box, ghost images, Cell<1> binning, the MD
+ * half neighbor list, two-pass EAM force/energy/virial, the skin check and
+ * the NVT (velocity Verlet + Nose-Hoover chain) driver.
+ *
+ * TEST INFRASTRUCTURE ONLY (the oracle). Built with -ffp-contract=off so
+ * that every product is rounded before it is added, as in the reference.
+ */
+#include "o_internal.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---------------- box (src/atoms.cpp:187-194, src/xmath.h:1403-1485) ---------------- */
+void o_box_set(o_box* bx, const double* H)
+{
+    memset(bx, 0, sizeof(*bx));
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) bx->H[i][j] = j <= i ? H[i * 3 + j] : 0.0;
+    double(*h)[3] = bx->H;
+    double(*b)[3] = bx->B;
+    /* lower-triangular inverse, row by row */
+    b[0][0] = 1.0 / h[0][0];
+    b[1][1] = 1.0 / h[1][1];
+    b[1][0] = -b[1][1] * (b[0][0] * h[1][0]);
+    b[2][2] = 1.0 / h[2][2];
+    b[2][1] = -b[2][2] * (b[1][1] * h[2][1]);
+    b[2][0] = -b[2][2] * (b[0][0] * h[2][0] + b[1][0] * h[2][1]);
+    bx->vol = h[0][0] * (h[1][1] * h[2][2]);
+    bx->depth_inv[0] = sqrt(b[0][0] * b[0][0] + (b[1][0] * b[1][0] + b[2][0] * b[2][0]));
+    bx->depth_inv[1] = sqrt(b[1][1] * b[1][1] + b[2][1] * b[2][1]);
+    bx->depth_inv[2] = fabs(b[2][2]);
+    /* packed by columns of the lower triangle */
+    bx->h[0] = h[0][0]; bx->h[1] = h[1][0]; bx->h[2] = h[2][0];
+    bx->h[3] = h[1][1]; bx->h[4] = h[2][1]; bx->h[5] = h[2][2];
+    bx->b[0] = b[0][0]; bx->b[1] = b[1][0]; bx->b[2] = b[2][0];
+    bx->b[3] = b[1][1]; bx->b[4] = b[2][1]; bx->b[5] = b[2][2];
+}
+
+static double wrap01(double s)
+{
+    while (s < 0.0) ++s;
+    while (s >= 1.0) --s;
+    return s;
+}
+
+/* x -> s = x B with wrap into [0,1) per component (src/xmath.h:2660-2700) */
+void o_x2s(const o_box* bx, const double* x, double* s)
+{
+    const double* b = bx->b;
+    double s0 = b[0] * x[0] + (b[1] * x[1] + b[2] * x[2]);
+    s0 = wrap01(s0);
+    double s1 = b[3] * x[1] + b[4] * x[2];
+    s1 = wrap01(s1);
+    double s2 = b[5] * x[2];
+    s2 = wrap01(s2);
+    s[0] = s0; s[1] = s1; s[2] = s2;
+}
+
+/* s -> x = s H (src/xmath.h:2636-2657) */
+void o_s2x(const o_box* bx, const double* s, double* x)
+{
+    const double* h = bx->h;
+    double x0 = h[0] * s[0] + (h[1] * s[1] + h[2] * s[2]);
+    double x1 = h[3] * s[1] + h[4] * s[2];
+    double x2 = h[5] * s[2];
+    x[0] = x0; x[1] = x1; x[2] = x2;
+}
+
+/* dx0^2 + (dx1^2 + dx2^2), the association of src/xmath.h:2464-2477 */
+double o_rsq(const double* a, const double* b)
+{
+    double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+    return d0 * d0 + (d1 * d1 + d2 * d2);
+}
+
+/* ---------------- ghosts (src/exchange.h:501-1096, one rank) ---------------- */
+static void ghosts_grow(o_ghosts* g, int need)
+{
+    if (need <= g->cap) return;
+    int cap = need + need / 2 + 64;
+    g->s = (double*)realloc(g->s, sizeof(double) * 3 * cap);
+    g->x = (double*)realloc(g->x, sizeof(double) * 3 * cap);
+    g->src = (int*)realloc(g->src, sizeof(int) * cap);
+    g->owner = (int*)realloc(g->owner, sizeof(int) * cap);
+    g->img = (int8_t*)realloc(g->img, 3 * cap);
+    g->sdim = (int8_t*)realloc(g->sdim, cap);
+    g->sdir = (int8_t*)realloc(g->sdir, cap);
+    g->cap = cap;
+}
+
+static void ghost_copy(o_ghosts* g, int from, int d, int dir)
+{
+    int k = g->nall++;
+    memcpy(g->s + 3 * k, g->s + 3 * from, sizeof(double) * 3);
+    if (dir > 0) ++g->s[3 * k + d]; else --g->s[3 * k + d];
+    g->src[k] = from;
+    g->owner[k] = g->owner[from];
+    memcpy(g->img + 3 * k, g->img + 3 * from, 3);
+    g->img[3 * k + d] += dir;
+    g->sdim[k] = (int8_t)d;
+    g->sdir[k] = (int8_t)dir;
+}
+
+/* Per dimension: atoms within max_cut*depth_inv of the low face are copied
+ * with s+1, then atoms within it of the high face with s-1; thin boxes repeat
+ * the copy on the freshly made ghosts (ncomms layers). */
+void o_ghosts_build(o_ghosts* g, const o_box* bx, int nlcl, const double* s_lcl, double max_cut)
+{
+    g->nlcl = nlcl;
+    g->nall = nlcl;
+    ghosts_grow(g, nlcl);
+    memcpy(g->s, s_lcl, sizeof(double) * 3 * nlcl);
+    for (int i = 0; i < nlcl; i++) {
+        g->src[i] = -1; g->owner[i] = i;
+        g->img[3 * i] = g->img[3 * i + 1] = g->img[3 * i + 2] = 0;
+        g->sdim[i] = -1; g->sdir[i] = 0;
+    }
+    int* snd = NULL;
+    int snd_cap = 0;
+    for (int d = 0; d < 3; d++) {
+        double cut_s = max_cut * bx->depth_inv[d];
+        int ncomms = (int)(cut_s / 1.0) + 1;
+        int last = g->nall;
+        for (int dir = 0; dir < 2; dir++) {
+            double bnd = dir == 0 ? 0.0 + cut_s : 1.0 - cut_s;
+            int lo = 0, hi = last;
+            for (int ic = 0; ic < ncomms; ic++) {
+                int ns = 0;
+                for (int i = lo; i < hi; i++) {
+                    double v = g->s[3 * i + d];
+                    int send = dir == 0 ? (v < bnd) : (v >= bnd);
+                    if (send) {
+                        if (ns == snd_cap) { snd_cap = snd_cap * 2 + 1024; snd = (int*)realloc(snd, sizeof(int) * snd_cap); }
+                        snd[ns++] = i;
+                    }
+                }
+                ghosts_grow(g, g->nall + ns);
+                for (int k = 0; k < ns; k++) ghost_copy(g, snd[k], d, dir == 0 ? 1 : -1);
+                lo = g->nall - ns;
+                hi = g->nall;
+            }
+        }
+    }
+    free(snd);
+    for (int i = 0; i < g->nall; i++) o_s2x(bx, g->s + 3 * i, g->x + 3 * i);
+}
+
+/* Update::update_w_x: ghost x = source x +/- row d of H (src/exchange.h:797-827, 959-967) */
+void o_ghosts_refresh(o_ghosts* g, const o_box* bx, const double* x_lcl)
+{
+    memcpy(g->x, x_lcl, sizeof(double) * 3 * g->nlcl);
+    for (int k = g->nlcl; k < g->nall; k++) {
+        const double* xs = g->x + 3 * g->src[k];
+        double* xk = g->x + 3 * k;
+        int d = g->sdim[k];
+        xk[0] = xs[0]; xk[1] = xs[1]; xk[2] = xs[2];
+        for (int c = 0; c <= d; c++) {
+            if (g->sdir[k] > 0) xk[c] += bx->H[d][c]; else xk[c] -= bx->H[d][c];
+        }
+    }
+}
+
+void o_ghosts_free(o_ghosts* g)
+{
+    free(g->s); free(g->x); free(g->src); free(g->owner); free(g->img); free(g->sdim); free(g->sdir);
+    memset(g, 0, sizeof(*g));
+}
+
+/* ---------------- Cell<1>::DO (src/cell.h:38-233) ---------------- */
+void o_cell_do(const o_ghosts* g, const o_box* bx, double max_cut,
+               void (*pre)(void*, int), void (*visit)(void*, int, int), void (*post)(void*, int), void* ctx)
+{
+    const int M = 1;
+    double cut_s[3], csz[3];
+    int nc[3], den[3], ncells = 1;
+    for (int d = 0; d < 3; d++) {
+        cut_s[d] = max_cut * bx->depth_inv[d];
+        csz[d] = cut_s[d] / (double)M;
+        nc[d] = (int)((1.0 - 0.0) / csz[d]) + 1 + 2 * M;
+        den[d] = ncells;
+        ncells *= nc[d];
+    }
+    int rel[27];
+    {
+        int cnt[3] = {-1, -1, -1};
+        for (int k = 0; k < 27; k++) {
+            rel[k] = cnt[0] * den[0] + cnt[1] * den[1] + cnt[2] * den[2];
+            cnt[0]++;
+            for (int d = 0; d < 2; d++)
+                if (cnt[d] == M + 1) { cnt[d] = -M; cnt[d + 1]++; }
+        }
+    }
+    int* head = (int*)malloc(sizeof(int) * ncells);
+    for (int c = 0; c < ncells; c++) head[c] = -1;
+    int* next = (int*)malloc(sizeof(int) * (g->nall > 0 ? g->nall : 1));
+    int* cell = (int*)malloc(sizeof(int) * (g->nall > 0 ? g->nall : 1));
+    for (int i = g->nall - 1; i > g->nlcl - 1; i--) {
+        const double* s = g->s + 3 * i;
+        int c = 0;
+        for (int d = 0; d < 3; d++) {
+            int k;
+            if (s[d] < 0.0) {
+                k = (int)((s[d] - (0.0 - cut_s[d])) / csz[d]);
+                if (k > M - 1) k = M - 1;
+            } else if (1.0 <= s[d]) {
+                k = (int)((s[d] - 0.0) / csz[d]);
+                if (k < nc[d] - 2 * M - 1) k = nc[d] - 2 * M - 1;
+                if (k > nc[d] - 1 - M) k = nc[d] - 1 - M;
+                k += M;
+            } else {
+                k = (int)((s[d] - 0.0) / csz[d]);
+                if (k > nc[d] - 2 * M - 1) k = nc[d] - 2 * M - 1;
+                k += M;
+            }
+            c += den[d] * k;
+        }
+        cell[i] = c;
+        next[i] = head[c];
+        head[c] = i;
+    }
+    for (int i = g->nlcl - 1; i > -1; i--) {
+        const double* s = g->s + 3 * i;
+        int c = 0;
+        for (int d = 0; d < 3; d++) {
+            int k = (int)((s[d] - 0.0) / csz[d]);
+            if (k > nc[d] - 2 * M - 1) k = nc[d] - 2 * M - 1;
+            c += den[d] * (k + M);
+        }
+        cell[i] = c;
+        next[i] = head[c];
+        head[c] = i;
+    }
+    for (int i = 0; i < g->nlcl; i++) {
+        if (pre) pre(ctx, i);
+        for (int k = 0; k < 27; k++) {
+            for (int j = head[cell[i] + rel[k]]; j != -1; j = next[j])
+                if (i != j) visit(ctx, i, j);
+        }
+        if (post) post(ctx, i);
+    }
+    free(head); free(next); free(cell);
+}
+
+/* ---------------- MD system ---------------- */
+struct o_md {
+    const o_eam* ff;
+    o_box bx;
+    o_ghosts g;
+    int n;
+    double skin, max_cut;
+    double* cut_sq;    /* [ne*ne] */
+    double* cut_sk_sq; /* [ne*ne] */
+    uint8_t* elem;     /* [nall] after ghosts */
+    uint8_t* elem_lcl;
+    double* x0;
+    int64_t* row;      /* [n+1] CSR of the half list */
+    int32_t* col;
+    int64_t npairs, cap;
+    double *rho, *dF, *drhoi, *drhoj;
+    int64_t nscratch;
+};
+
+typedef struct { o_md* md; const double* xi; int ei; } list_ctx;
+
+static void list_visit(void* v, int i, int j)
+{
+    list_ctx* c = (list_ctx*)v;
+    o_md* md = c->md;
+    if (i < j) {
+        int ne = md->ff->ne;
+        if (o_rsq(c->xi, md->g.x + 3 * j) < md->cut_sk_sq[c->ei * ne + md->elem[j]]) {
+            if (md->npairs == md->cap) {
+                md->cap = md->cap * 2 + 1024;
+                md->col = (int32_t*)realloc(md->col, sizeof(int32_t) * md->cap);
+            }
+            md->col[md->npairs++] = j;
+        }
+    }
+}
+
+static void list_pre(void* v, int i)
+{
+    list_ctx* c = (list_ctx*)v;
+    c->xi = c->md->g.x + 3 * i;
+    c->ei = c->md->elem[i];
+}
+
+static void list_post(void* v, int i)
+{
+    list_ctx* c = (list_ctx*)v;
+    c->md->row[i + 1] = c->md->npairs;
+}
+
+/* rebuild: wrap locals (x2s_lcl), ghosts (Update::list), Cell<1>::DO with
+ * s2x_all, NeighborMD::create_list (src/neighbor_md.cpp:82-152), store x0 */
+static void md_rebuild(o_md* md, const double* x_lcl)
+{
+    double* s = (double*)malloc(sizeof(double) * 3 * (md->n > 0 ? md->n : 1));
+    for (int i = 0; i < md->n; i++) o_x2s(&md->bx, x_lcl + 3 * i, s + 3 * i);
+    o_ghosts_build(&md->g, &md->bx, md->n, s, md->max_cut);
+    free(s);
+    md->elem = (uint8_t*)realloc(md->elem, md->g.nall > 0 ? md->g.nall : 1);
+    for (int k = 0; k < md->g.nall; k++) md->elem[k] = md->elem_lcl[md->g.owner[k]];
+    md->row[0] = 0;
+    md->npairs = 0;
+    list_ctx c = {md, NULL, 0};
+    o_cell_do(&md->g, &md->bx, md->max_cut, list_pre, list_visit, list_post, &c);
+    memcpy(md->x0, md->g.x, sizeof(double) * 3 * md->n);
+}
+
+o_md* o_md_create(const o_eam* ff, int natms, const double* H, const double* x,
+                  const uint8_t* elem, double skin)
+{
+    o_md* md = (o_md*)calloc(1, sizeof(o_md));
+    md->ff = ff;
+    md->n = natms;
+    md->skin = skin;
+    o_box_set(&md->bx, H);
+    int ne = ff->ne;
+    md->cut_sq = (double*)malloc(sizeof(double) * ne * ne);
+    md->cut_sk_sq = (double*)malloc(sizeof(double) * ne * ne);
+    md->max_cut = 0.0;
+    /* ForceFieldMD::pre_init (src/ff_md.cpp:80-94) */
+    for (int i = 0; i < ne; i++)
+        for (int j = 0; j <= i; j++) {
+            double t = ff->rc[i * ne + j];
+            if (t > md->max_cut) md->max_cut = t;
+            md->cut_sq[i * ne + j] = md->cut_sq[j * ne + i] = t * t;
+            t += skin;
+            md->cut_sk_sq[i * ne + j] = md->cut_sk_sq[j * ne + i] = t * t;
+        }
+    md->max_cut += skin; /* atoms->max_cut = ff->max_cut + skin (src/dynamic_md.cpp:41) */
+    md->elem_lcl = (uint8_t*)malloc(natms > 0 ? natms : 1);
+    memcpy(md->elem_lcl, elem, natms);
+    md->x0 = (double*)malloc(sizeof(double) * 3 * (natms > 0 ? natms : 1));
+    md->row = (int64_t*)malloc(sizeof(int64_t) * (natms + 1));
+    md_rebuild(md, x);
+    return md;
+}
+
+void o_md_free(o_md* md)
+{
+    if (!md) return;
+    o_ghosts_free(&md->g);
+    free(md->cut_sq); free(md->cut_sk_sq); free(md->elem); free(md->elem_lcl); free(md->x0);
+    free(md->row); free(md->col); free(md->rho); free(md->dF); free(md->drhoi); free(md->drhoj);
+    free(md);
+}
+
+int o_md_natms(const o_md* md) { return md->n; }
+int o_md_nghosts(const o_md* md) { return md->g.nall - md->g.nlcl; }
+int64_t o_md_npairs(const o_md* md) { return md->npairs; }
+
+void o_md_pairs(const o_md* md, int32_t* pi, int32_t* pj, int8_t* img)
+{
+    for (int i = 0; i < md->n; i++)
+        for (int64_t k = md->row[i]; k < md->row[i + 1]; k++) {
+            int j = md->col[k];
+            pi[k] = i;
+            pj[k] = md->g.owner[j];
+            img[3 * k] = md->g.img[3 * j];
+            img[3 * k + 1] = md->g.img[3 * j + 1];
+            img[3 * k + 2] = md->g.img[3 * j + 2];
+        }
+}
+
+void o_md_get_x(const o_md* md, double* x) { memcpy(x, md->g.x, sizeof(double) * 3 * md->n); }
+
+void o_md_set_x(o_md* md, const double* x) { memcpy(md->g.x, x, sizeof(double) * 3 * md->n); }
+
+/* DynamicMD::decide (src/dynamic_md.cpp:90-105) */
+static int md_decide(const o_md* md)
+{
+    double skin_sq = 0.25 * md->skin * md->skin;
+    for (int i = 0; i < md->n; i++)
+        if (o_rsq(md->x0 + 3 * i, md->g.x + 3 * i) > skin_sq) return 0;
+    return 1;
+}
+
+int o_md_update(o_md* md)
+{
+    if (md_decide(md)) {
+        o_ghosts_refresh(&md->g, &md->bx, md->g.x);
+        return 0;
+    }
+    double* x = (double*)malloc(sizeof(double) * 3 * (md->n > 0 ? md->n : 1));
+    memcpy(x, md->g.x, sizeof(double) * 3 * md->n);
+    md_rebuild(md, x);
+    free(x);
+    return 1;
+}
+
+static inline double hval(const double* c, double p) { return ((c[3] * p + c[2]) * p + c[1]) * p + c[0]; }
+static inline double hder(const double* c, double p, double inv) { return ((3.0 * c[3] * p + 2.0 * c[2]) * p + c[1]) * inv; }
+
+/* ForceFieldEAM::__force_calc (src/ff_eam.cpp:65-240): pass 1 gives rho,
+ * the pair part and F(rho_i) in half-list order; the F' of every atom is then
+ * known and pass 2 adds the embedding forces. */
+static void eam_force(o_md* md, double* f, double* v7, double* rho_out)
+{
+    const o_eam* e = md->ff;
+    int ne = e->ne;
+    int n = md->n;
+    const double* x = md->g.x;
+    const uint8_t* el = md->elem;
+    double dr_inv = 1.0 / e->dr, drho_inv = 1.0 / e->drho;
+    double rho_max = (double)e->nrho * e->drho;
+    if (md->nscratch < md->npairs || !md->rho) {
+        free(md->drhoi); free(md->drhoj);
+        md->nscratch = md->npairs > 0 ? md->npairs : 1;
+        md->drhoi = (double*)malloc(sizeof(double) * md->nscratch);
+        md->drhoj = (double*)malloc(sizeof(double) * md->nscratch);
+    }
+    md->rho = (double*)realloc(md->rho, sizeof(double) * (n > 0 ? n : 1));
+    md->dF = (double*)realloc(md->dF, sizeof(double) * (md->g.nall > 0 ? md->g.nall : 1));
+    double* rho = md->rho;
+    double* dF = md->dF;
+    for (int i = 0; i < 3 * n; i++) f[i] = 0.0;
+    for (int k = 0; k < 7; k++) v7[k] = 0.0;
+    for (int i = 0; i < n; i++) rho[i] = 0.0;
+    for (int i = 0; i < n; i++) {
+        int ei = el[i];
+        for (int64_t k = md->row[i]; k < md->row[i + 1]; k++) {
+            int j = md->col[k];
+            int ej = el[j];
+            double dx0 = x[3 * i] - x[3 * j];
+            double dx1 = x[3 * i + 1] - x[3 * j + 1];
+            double dx2 = x[3 * i + 2] - x[3 * j + 2];
+            double rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+            md->drhoi[k] = md->drhoj[k] = 0.0;
+            if (rsq >= md->cut_sq[ei * ne + ej]) continue;
+            double r = sqrt(rsq);
+            double r_inv = 1.0 / r;
+            size_t m;
+            double p;
+            o_spline_pm(r, dr_inv, e->nr, &m, &p);
+            const double* c = ORHO(e, ej, ei)[m];
+            double rho_i = hval(c, p), drho_i = hder(c, p, dr_inv);
+            c = ORHO(e, ei, ej)[m];
+            double rho_j = hval(c, p), drho_j = hder(c, p, dr_inv);
+            c = OPHI(e, ei, ej)[m];
+            double z2p = hder(c, p, dr_inv), z2 = hval(c, p);
+            double phi = z2 * r_inv;
+            double dphi = z2p * r_inv - phi * r_inv;
+            rho[i] += rho_i;
+            if (j < n) rho[j] += rho_j;
+            double fp = -dphi * r_inv;
+            f[3 * i] += fp * dx0; f[3 * i + 1] += fp * dx1; f[3 * i + 2] += fp * dx2;
+            if (j < n) { f[3 * j] -= fp * dx0; f[3 * j + 1] -= fp * dx1; f[3 * j + 2] -= fp * dx2; }
+            else { fp *= 0.5; phi *= 0.5; }
+            v7[0] += phi;
+            v7[1] -= fp * dx0 * dx0; v7[2] -= fp * dx0 * dx1; v7[3] -= fp * dx0 * dx2;
+            v7[4] -= fp * dx1 * dx1; v7[5] -= fp * dx1 * dx2; v7[6] -= fp * dx2 * dx2;
+            md->drhoi[k] = -drho_i * r_inv;
+            md->drhoj[k] = -drho_j * r_inv;
+        }
+        size_t m;
+        double p;
+        o_spline_pm(rho[i], drho_inv, e->nrho, &m, &p);
+        const double* c = OF(e, ei) + m * e->nf;
+        double t1 = hder(c, p, drho_inv);
+        double t0 = hval(c, p);
+        if (rho[i] > rho_max) t0 += t1 * (rho[i] - rho_max);
+        v7[0] += t0;
+        dF[i] = t1;
+    }
+    /* Update::update_wo_x(dF): ghosts take their owner's F' (src/exchange.h:1074-1096) */
+    for (int k = n; k < md->g.nall; k++) dF[k] = dF[md->g.src[k]];
+    for (int i = 0; i < n; i++) {
+        for (int64_t k = md->row[i]; k < md->row[i + 1]; k++) {
+            if (md->drhoi[k] == 0.0 && md->drhoj[k] == 0.0) continue;
+            int j = md->col[k];
+            double fp = dF[i] * md->drhoi[k] + dF[j] * md->drhoj[k];
+            double dx0 = x[3 * i] - x[3 * j];
+            double dx1 = x[3 * i + 1] - x[3 * j + 1];
+            double dx2 = x[3 * i + 2] - x[3 * j + 2];
+            f[3 * i] += dx0 * fp; f[3 * i + 1] += dx1 * fp; f[3 * i + 2] += dx2 * fp;
+            if (j < n) { f[3 * j] -= dx0 * fp; f[3 * j + 1] -= dx1 * fp; f[3 * j + 2] -= dx2 * fp; }
+            else fp *= 0.5;
+            v7[1] -= fp * dx0 * dx0; v7[2] -= fp * dx0 * dx1; v7[3] -= fp * dx0 * dx2;
+            v7[4] -= fp * dx1 * dx1; v7[5] -= fp * dx1 * dx2; v7[6] -= fp * dx2 * dx2;
+        }
+    }
+    if (rho_out) memcpy(rho_out, rho, sizeof(double) * n);
+}
+
+/* ForceFieldMD::force_calc (src/ff_md.cpp:125-144): out7[1..6] = -W (F_H) */
+void o_md_force(o_md* md, double* f, double* out7, double* rho_out)
+{
+    double v7[7];
+    eam_force(md, f, v7, rho_out);
+    out7[0] = v7[0];
+    for (int k = 1; k < 7; k++) out7[k] = -v7[k];
+}
+
+/* ForceFieldEAM::__energy_calc (src/ff_eam.cpp:244-318) */
+double o_md_energy(o_md* md)
+{
+    const o_eam* e = md->ff;
+    int ne = e->ne, n = md->n;
+    const double* x = md->g.x;
+    const uint8_t* el = md->elem;
+    double dr_inv = 1.0 / e->dr, drho_inv = 1.0 / e->drho;
+    double rho_max = (double)e->nrho * e->drho;
+    double* rho = (double*)calloc(n > 0 ? n : 1, sizeof(double));
+    double en = 0.0;
+    for (int i = 0; i < n; i++) {
+        int ei = el[i];
+        for (int64_t k = md->row[i]; k < md->row[i + 1]; k++) {
+            int j = md->col[k];
+            int ej = el[j];
+            double dx0 = x[3 * i] - x[3 * j];
+            double dx1 = x[3 * i + 1] - x[3 * j + 1];
+            double dx2 = x[3 * i + 2] - x[3 * j + 2];
+            double rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+            if (rsq < md->cut_sq[ei * ne + ej]) {
+                double r = sqrt(rsq);
+                size_t m;
+                double p;
+                o_spline_pm(r, dr_inv, e->nr, &m, &p);
+                double phi = hval(OPHI(e, ei, ej)[m], p) / r;
+                rho[i] += hval(ORHO(e, ej, ei)[m], p);
+                if (j < n) {
+                    rho[j] += hval(ORHO(e, ei, ej)[m], p);
+                    en += phi;
+                } else
+                    en += 0.5 * phi;
+            }
+        }
+        size_t m;
+        double p;
+        o_spline_pm(rho[i], drho_inv, e->nrho, &m, &p);
+        const double* c = OF(e, ei) + m * e->nf;
+        double t0 = hval(c, p);
+        if (rho[i] > rho_max) t0 += hder(c, p, drho_inv) * (rho[i] - rho_max);
+        en += t0;
+    }
+    free(rho);
+    return en;
+}
+
+/* ---------------- NVT (src/md_nvt.cpp, src/thermostat.cpp:47-143) ---------------- */
+typedef struct { double ddt, ddt2, freq_sq; double eta[3]; int L, niters; } o_nhc;
+
+static double nhc_f(double x)
+{
+    if (fabs(x) < 0.015625) return 1.0 + x / 2.0 + x * x / 6.0 + x * x * x / 24.0;
+    return (exp(x) - 1.0) / x;
+}
+
+static double nhc_step(o_nhc* t, double T_r, int ndof)
+{
+    double rescale = 1.0, prev, tmp;
+    int L = t->L;
+    double* e = t->eta;
+    for (int it = 0; it < t->niters; it++) {
+        e[L - 1] += t->ddt2 * (e[L - 2] * e[L - 2] - t->freq_sq);
+        prev = -e[L - 1];
+        for (int ich = L - 2; ich > 0; ich--) {
+            e[ich] += t->ddt2 * nhc_f(prev * t->ddt2) * (prev * e[ich] + e[ich - 1] * e[ich - 1] - t->freq_sq);
+            prev = -e[ich];
+        }
+        e[0] += t->ddt2 * nhc_f(prev * t->ddt2) * (prev * e[0] + t->freq_sq * (T_r - 1.0));
+        prev = -e[0];
+        tmp = exp(t->ddt * prev);
+        rescale *= tmp;
+        T_r *= tmp * tmp;
+        prev = (T_r - 1.0) * t->freq_sq;
+        e[0] += t->ddt2 * nhc_f(-e[1] * t->ddt2) * (-e[1] * e[0] + prev);
+        prev = ndof * e[0] * e[0] - t->freq_sq;
+        for (int ich = 1; ich < L - 1; ich++) {
+            e[ich] += t->ddt2 * nhc_f(-e[ich + 1] * t->ddt2) * (-e[ich + 1] * e[ich] + prev);
+            prev = e[ich] * e[ich] - t->freq_sq;
+        }
+        e[L - 1] += t->ddt2 * prev;
+    }
+    return rescale;
+}
+
+static void kin_tensor(const o_md* md, const double* v, const double* masses, double* mvv)
+{
+    for (int k = 0; k < 6; k++) mvv[k] = 0.0;
+    for (int i = 0; i < md->n; i++) {
+        double m = masses[md->elem_lcl[i]];
+        const double* u = v + 3 * i;
+        mvv[0] += u[0] * u[0] * m; mvv[1] += u[1] * u[0] * m; mvv[2] += u[2] * u[0] * m;
+        mvv[3] += u[1] * u[1] * m; mvv[4] += u[2] * u[1] * m; mvv[5] += u[2] * u[2] * m;
+    }
+}
+
+int o_md_nvt_run(o_md* md, double* v, const double* masses, double kB, double T,
+                 double dt, int nsteps, double* out7, double* T_part_out)
+{
+    int n = md->n;
+    double dt2 = dt / 2.0;
+    int ndof = n * 3;
+    o_nhc th;
+    /* ThermostatNHC(dt/2, 100 dt, 3, 1) (src/md_nvt.cpp:19) */
+    th.L = 3; th.niters = 1;
+    th.ddt = (dt / 2.0) / 1.0;
+    th.ddt2 = 0.5 * (dt / 2.0) / 1.0;
+    th.freq_sq = 1.0 / ((100.0 * dt) * (100.0 * dt));
+    th.eta[0] = th.eta[1] = th.eta[2] = 0.0;
+    double* f = (double*)malloc(sizeof(double) * 3 * (n > 0 ? n : 1));
+    double mvv[6];
+    kin_tensor(md, v, masses, mvv);
+    double T_part = (mvv[0] + (mvv[3] + mvv[5])) / (ndof * kB);
+    o_md_force(md, f, out7, NULL);
+    int nrebuild = 0;
+    double fac, fac_x_d = 1.0;
+    for (int step = 0; step < nsteps; step++) {
+        fac_x_d *= fac = nhc_step(&th, T_part / T, ndof);
+        fac *= fac;
+        for (int k = 0; k < 6; k++) mvv[k] *= fac;
+        T_part *= fac;
+        /* update_x_d__x (src/md_nvt.cpp:70-102) */
+        double dx_l[3] = {0.0, 0.0, 0.0};
+        double* x = md->g.x;
+        for (int i = 0; i < n; i++) {
+            double m = masses[md->elem_lcl[i]];
+            for (int d = 0; d < 3; d++) {
+                v[3 * i + d] = v[3 * i + d] * fac_x_d + f[3 * i + d] * dt2 / m;
+                dx_l[d] += v[3 * i + d] * dt;
+                x[3 * i + d] += v[3 * i + d] * dt;
+            }
+        }
+        for (int d = 0; d < 3; d++) dx_l[d] /= (double)n;
+        for (int i = 0; i < n; i++)
+            for (int d = 0; d < 3; d++) x[3 * i + d] -= dx_l[d];
+        nrebuild += o_md_update(md);
+        o_md_force(md, f, out7, NULL);
+        /* update_x_d (src/md_nvt.cpp:106-128) */
+        for (int i = 0; i < n; i++) {
+            double m = masses[md->elem_lcl[i]];
+            for (int d = 0; d < 3; d++) v[3 * i + d] += f[3 * i + d] * dt2 / m;
+        }
+        kin_tensor(md, v, masses, mvv);
+        T_part = (mvv[0] + (mvv[3] + mvv[5])) / (ndof * kB);
+        fac_x_d = fac = nhc_step(&th, T_part / T, ndof);
+        fac *= fac;
+        for (int k = 0; k < 6; k++) mvv[k] *= fac;
+        T_part *= fac;
+    }
+    for (int i = 0; i < 3 * n; i++) v[i] *= fac_x_d;
+    if (T_part_out) *T_part_out = T_part;
+    free(f);
+    return nrebuild;
+}
+
+/* AtomsMD::create_T (src/atoms_md.cpp:130-160), one rank (seed + 0) */
+void o_create_T(int natms, const uint8_t* elem, const double* masses, double kB,
+                double T, int seed, double* v)
+{
+    o_rng r;
+    o_rng_init(&r, seed);
+    for (int i = 0; i < natms; i++) {
+        double fac = sqrt(kB * T / masses[elem[i]]);
+        for (int d = 0; d < 3; d++) v[3 * i + d] = o_rng_gaussian(&r) * fac;
+    }
+}

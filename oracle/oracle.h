@@ -1,0 +1,136 @@
+/*
+ * oracle.h -- FP64 CPU restatement of MAPP's EAM / DMD / GCMC hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the correctness oracle and the
+ * CPU baseline. Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it; the product library (mapp4py_amd) never does.
+ *
+ * Every function restates the algorithm of the reference file:line given in
+ * its comment; none of the reference's source is copied. All arithmetic is
+ * IEEE double with floating-point contraction disabled (-ffp-contract=off)
+ * so that the neighbor predicate reproduces the reference bit for bit.
+ *
+ * Parity pinning (what checks each part; see tests/test_oracle_*.py):
+ *   splines       exact reproduction of cubics/quartics on a uniform grid,
+ *                 knot interpolation, closed-form coefficient formulas
+ *   setfl / fs    round trip through a generated file, shipped example tables
+ *   neighbor list O(N^2) brute force over explicit periodic images
+ *   EAM force     finite differences of the energy, Newton III, sum f = 0,
+ *                 pair-only limit vs analytic pair sum, perfect lattice f = 0,
+ *                 finite-strain derivative vs virial
+ *   Gauss-Hermite closed form n=3, exactness on polynomials, sum w = sqrt(pi)
+ *   DMD           closed-form Gaussian smearing, finite differences of the
+ *                 free energy vs f_x, f_alpha, mu; alpha -> 0 gives MD
+ *   Park-Miller   published check value (seed 1 -> 10000th = 1043618065)
+ *   NVT / NHC     energy conservation of the extended system (NVE limit)
+ */
+#ifndef MAPP_ORACLE_H
+#define MAPP_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- splines (src/import_eam.cpp:62-115) ---------------- */
+void o_interpolate4(size_t n, double (*s)[4]);
+void o_interpolate5(size_t n, double (*s)[5]);
+
+/* ---------------- tabulated EAM (src/import_eam.h:52-901) ---------------- */
+typedef struct o_eam o_eam;
+/* fmt: 0 = setfl (eam/alloy), 1 = fs (eam/fs). dmd != 0 builds the DMD
+ * variant: quartic F ([5] coefficients) and r*rho tables
+ * (src/ff_eam_dmd.cpp:80-94). Returns NULL and fills err on failure. */
+o_eam* o_eam_read(const char* path, int fmt, int dmd, char* err, int errlen);
+/* Builds an EAM object from knot arrays: F[ne][nrho], rho[ne][ne][nr]
+ * (rho[i][j] = density function listed in element i's block at position j),
+ * r_phi[ne][ne][nr] (symmetric). setfl_alias != 0 makes rho[i][j] alias
+ * rho[i][0] as setfl does. */
+o_eam* o_eam_from_knots(int ne, int nr, int nrho, double dr, double drho,
+                        const double* F, const double* rho, const double* r_phi,
+                        int setfl_alias, int dmd);
+void o_eam_free(o_eam*);
+void o_eam_info(const o_eam*, int* ne, int* nr, int* nrho, double* dr, double* drho);
+void o_eam_cut(const o_eam*, double* rc /* [ne*ne] */);
+/* coefficient export for cross-checking: which 0=r_phi 1=rho 2=F */
+void o_eam_coefs(const o_eam*, int which, int i, int j, double* out);
+
+/* ---------------- Gauss-Hermite (src/xmath.cpp:440-589) ---------------- */
+void o_quadrature_hg(int n, double* x, double* w);
+
+/* ---------------- random numbers (src/random.cpp:7-68) ---------------- */
+typedef struct { int seed; int gauss_chk; double reserved; } o_rng;
+void o_rng_init(o_rng*, int seed);
+double o_rng_uniform(o_rng*);
+double o_rng_gaussian(o_rng*);
+
+/* ---------------- MD system: ghosts, cells, half list, EAM ---------------- */
+typedef struct o_md o_md;
+/* DynamicMD::init (src/dynamic_md.cpp:34-52): wrap, ghosts, list, store x0.
+ * H is the lower-triangular box, row-major H[3][3] (rows = box vectors). */
+o_md* o_md_create(const o_eam* ff, int natms, const double* H, const double* x,
+                  const uint8_t* elem, double skin);
+void o_md_free(o_md*);
+int o_md_natms(const o_md*);
+int o_md_nghosts(const o_md*);
+int64_t o_md_npairs(const o_md*);
+/* half list as (i, owner j, image shift) triples in list order */
+void o_md_pairs(const o_md*, int32_t* i, int32_t* j, int8_t* img /* [npairs][3] */);
+void o_md_get_x(const o_md*, double* x);
+void o_md_set_x(o_md*, const double* x);
+/* DynamicMD::update<true> (src/dynamic_md.h:38-68). Returns 1 on rebuild. */
+int o_md_update(o_md*);
+/* force_calc (src/ff_md.cpp:125-144 + src/ff_eam.cpp:65-240).
+ * out[0] = pe, out[1..6] = F_H Voigt (xx,xy,xz,yy,yz,zz). rho_out may be NULL. */
+void o_md_force(o_md*, double* f, double* out7, double* rho_out);
+/* value() (src/ff_md.cpp:148-155 + src/ff_eam.cpp:244-318) */
+double o_md_energy(o_md*);
+/* MDNVT::run (src/md_nvt.cpp:383-480) with NHC L=3, niters=1 on a system
+ * whose velocities are v [N][3]; masses per element. Returns the number of
+ * neighbor-list rebuilds. out7 as o_md_force after the last step. */
+int o_md_nvt_run(o_md*, double* v, const double* masses, double kB, double T,
+                 double dt, int nsteps, double* out7, double* T_part_out);
+/* AtomsMD::create_T (src/atoms_md.cpp:130-160) */
+void o_create_T(int natms, const uint8_t* elem, const double* masses, double kB,
+                double T, int seed, double* v);
+
+/* ---------------- DMD (src/ff_eam_dmd.cpp, src/neighbor_dmd.cpp) ---------------- */
+typedef struct o_dmd o_dmd;
+/* c_dim components per site; elem/alpha/c are [natms][c_dim]; c < 0 marks
+ * an absent component. ff must have been built with dmd != 0. */
+o_dmd* o_dmd_create(const o_eam* ff, int natms, int c_dim, int nquad,
+                    const double* H, const double* x, const uint8_t* elem,
+                    const double* alpha, const double* c, double skin,
+                    double kB, double T, double hP, const double* masses);
+void o_dmd_free(o_dmd*);
+int64_t o_dmd_npairs(const o_dmd*);
+void o_dmd_pairs(const o_dmd*, int32_t* i, int32_t* j, int8_t* img);
+void o_dmd_set_state(o_dmd*, const double* x, const double* alpha, const double* c);
+/* __force_calc (src/ff_eam_dmd.cpp:126-360) wrapped by derivative()
+ * (src/ff_dmd.cpp:220-237). out: [0]=fe, [1..6]=F_H Voigt, [7]=s.
+ * f [N][3], f_alpha [N*c_dim], mu [N*c_dim] (= f_c after __force_calc,
+ * i.e. F(rho_i) + sum_j c_j (phi + rho dE_j)), E [N*c_dim] = F(rho). */
+void o_dmd_force(o_dmd*, double* f, double* f_alpha, double* mu, double* out8);
+/* value() (src/ff_dmd.cpp:223-231 + src/ff_eam_dmd.cpp:835-953) */
+double o_dmd_energy(o_dmd*);
+/* smeared pair quantities for one pair (r, alpha_ij): out[9] = rho_phi[3],
+ * drho_phi_dr[3], drho_phi_dalpha[3] in the order phi(ei,ej), rho(ei,ej),
+ * rho(ej,ei) (src/ff_eam_dmd.cpp:195-253). */
+void o_dmd_pair_eval(const o_dmd*, int ei, int ej, double r, double alpha_ij, double* out9);
+
+/* ---------------- GCMC (src/sgcmc.cpp, src/ff_eam.cpp:323-589) ---------------- */
+/* Runs nattempts SGCMC insert/delete attempts of element gas_elem on one
+ * periodic cell (sites given by x/elem). Reproduces the RNG draw order of
+ * SGCMC::attmpt. decisions[k] = 1 accepted insertion, 2 accepted deletion,
+ * -1 rejected insertion, -2 rejected deletion; dU[k] = energy delta.
+ * Returns the final atom count; x/elem must have room for natms+nattempts. */
+int o_gcmc_run(const o_eam* ff, int natms, double* x, uint8_t* elem,
+               const double* H, int gas_elem, double mu, double T, double kB,
+               double hP, const double* masses, int seed, int nattempts,
+               int* decisions, double* dU);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

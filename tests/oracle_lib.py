@@ -1,0 +1,241 @@
+"""ctypes binding of the oracle (oracle/liboracle.so): the FP64 CPU restatement
+of the reference algorithm. Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+i32p = C.POINTER(C.c_int32)
+i8p = C.POINTER(C.c_int8)
+u8p = C.POINTER(C.c_uint8)
+
+_L = None
+
+
+class ORng(C.Structure):
+    _fields_ = [("seed", C.c_int), ("gauss_chk", C.c_int), ("reserved", C.c_double)]
+
+
+def L():
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB):
+            subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+        lib = C.CDLL(LIB)
+        sig = {
+            "o_interpolate4": (None, [C.c_size_t, dp]),
+            "o_interpolate5": (None, [C.c_size_t, dp]),
+            "o_eam_read": (C.c_void_p, [C.c_char_p, C.c_int, C.c_int, C.c_char_p, C.c_int]),
+            "o_eam_from_knots": (C.c_void_p, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dp, dp, dp,
+                                              C.c_int, C.c_int]),
+            "o_eam_free": (None, [C.c_void_p]),
+            "o_eam_info": (None, [C.c_void_p, ip, ip, ip, dp, dp]),
+            "o_eam_cut": (None, [C.c_void_p, dp]),
+            "o_eam_coefs": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, dp]),
+            "o_quadrature_hg": (None, [C.c_int, dp, dp]),
+            "o_rng_init": (None, [C.POINTER(ORng), C.c_int]),
+            "o_rng_uniform": (C.c_double, [C.POINTER(ORng)]),
+            "o_rng_gaussian": (C.c_double, [C.POINTER(ORng)]),
+            "o_md_create": (C.c_void_p, [C.c_void_p, C.c_int, dp, dp, u8p, C.c_double]),
+            "o_md_free": (None, [C.c_void_p]),
+            "o_md_natms": (C.c_int, [C.c_void_p]),
+            "o_md_nghosts": (C.c_int, [C.c_void_p]),
+            "o_md_npairs": (C.c_int64, [C.c_void_p]),
+            "o_md_pairs": (None, [C.c_void_p, i32p, i32p, i8p]),
+            "o_md_get_x": (None, [C.c_void_p, dp]),
+            "o_md_set_x": (None, [C.c_void_p, dp]),
+            "o_md_update": (C.c_int, [C.c_void_p]),
+            "o_md_force": (None, [C.c_void_p, dp, dp, dp]),
+            "o_md_energy": (C.c_double, [C.c_void_p]),
+            "o_md_nvt_run": (C.c_int, [C.c_void_p, dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp]),
+            "o_create_T": (None, [C.c_int, u8p, dp, C.c_double, C.c_double, C.c_int, dp]),
+            "o_dmd_create": (C.c_void_p, [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, dp, u8p, dp, dp, C.c_double,
+                                          C.c_double, C.c_double, C.c_double, dp]),
+            "o_dmd_free": (None, [C.c_void_p]),
+            "o_dmd_npairs": (C.c_int64, [C.c_void_p]),
+            "o_dmd_pairs": (None, [C.c_void_p, i32p, i32p, i8p]),
+            "o_dmd_set_state": (None, [C.c_void_p, dp, dp, dp]),
+            "o_dmd_force": (None, [C.c_void_p, dp, dp, dp, dp]),
+            "o_dmd_energy": (C.c_double, [C.c_void_p]),
+            "o_dmd_pair_eval": (None, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, dp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _L = lib
+    return _L
+
+
+def P(a, t=dp):
+    return a.ctypes.data_as(t)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class OEam:
+    def __init__(self, h):
+        self.h = h
+        ne, nr, nrho = C.c_int(), C.c_int(), C.c_int()
+        dr, drho = C.c_double(), C.c_double()
+        L().o_eam_info(h, C.byref(ne), C.byref(nr), C.byref(nrho), C.byref(dr), C.byref(drho))
+        self.ne, self.nr, self.nrho, self.dr, self.drho = ne.value, nr.value, nrho.value, dr.value, drho.value
+
+    @classmethod
+    def read(cls, path, fmt=0, dmd=False):
+        err = C.create_string_buffer(256)
+        h = L().o_eam_read(str(path).encode(), fmt, int(dmd), err, 256)
+        if not h:
+            raise RuntimeError(err.value.decode())
+        return cls(h)
+
+    @classmethod
+    def from_knots(cls, pot, dmd=False):
+        F, rho, rphi = f64(pot.F), f64(pot.rho), f64(pot.r_phi)
+        h = L().o_eam_from_knots(pot.ne, pot.nr, pot.nrho, pot.dr, pot.drho, P(F), P(rho), P(rphi),
+                                 0 if pot.fs else 1, int(dmd))
+        return cls(h)
+
+    def cut(self):
+        rc = np.zeros((self.ne, self.ne))
+        L().o_eam_cut(self.h, P(rc))
+        return rc
+
+    def coefs(self, which, a, b=0, ncols=4):
+        n = self.nrho if which == 2 else self.nr
+        out = np.zeros((n, ncols))
+        L().o_eam_coefs(self.h, which, a, b, P(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            L().o_eam_free(self.h)
+            self.h = None
+
+
+class OMD:
+    """Oracle MD system (reference semantics: half list over ghosts)."""
+
+    def __init__(self, ff: OEam, H, x, elem, skin=0.5):
+        self.ff = ff
+        x = f64(x).reshape(-1, 3)
+        self.n = x.shape[0]
+        H = f64(H).reshape(3, 3)
+        self.elem = np.ascontiguousarray(elem, dtype=np.uint8)
+        self.h = L().o_md_create(ff.h, self.n, P(H), P(x), P(self.elem, u8p), float(skin))
+
+    def pairs(self):
+        n = L().o_md_npairs(self.h)
+        i = np.zeros(n, np.int32)
+        j = np.zeros(n, np.int32)
+        img = np.zeros((n, 3), np.int8)
+        L().o_md_pairs(self.h, P(i, i32p), P(j, i32p), P(img, i8p))
+        return i, j, img
+
+    def force(self):
+        f = np.zeros((self.n, 3))
+        out = np.zeros(7)
+        rho = np.zeros(self.n)
+        L().o_md_force(self.h, P(f), P(out), P(rho))
+        return f, out[0], out[1:], rho
+
+    def energy(self):
+        return L().o_md_energy(self.h)
+
+    @property
+    def x(self):
+        out = np.zeros((self.n, 3))
+        L().o_md_get_x(self.h, P(out))
+        return out
+
+    @x.setter
+    def x(self, v):
+        v = f64(v).reshape(self.n, 3)
+        L().o_md_set_x(self.h, P(v))
+
+    def update(self):
+        return bool(L().o_md_update(self.h))
+
+    def nvt_run(self, v, masses, kB, T, dt, nsteps):
+        v = f64(v).copy()
+        masses = f64(masses)
+        out = np.zeros(7)
+        Tp = C.c_double()
+        nreb = L().o_md_nvt_run(self.h, P(v), P(masses), kB, T, dt, nsteps, P(out), C.byref(Tp))
+        return v, out, Tp.value, nreb
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            L().o_md_free(self.h)
+            self.h = None
+
+
+def o_create_T(elem, masses, kB, T, seed):
+    elem = np.ascontiguousarray(elem, dtype=np.uint8)
+    masses = f64(masses)
+    v = np.zeros((elem.shape[0], 3))
+    L().o_create_T(elem.shape[0], P(elem, u8p), P(masses), kB, T, seed, P(v))
+    return v
+
+
+def gauss_hermite(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    L().o_quadrature_hg(n, P(x), P(w))
+    return x, w
+
+
+def rng_stream(seed, n, gaussian=False):
+    r = ORng()
+    L().o_rng_init(C.byref(r), seed)
+    fn = L().o_rng_gaussian if gaussian else L().o_rng_uniform
+    return np.array([fn(C.byref(r)) for _ in range(n)])
+
+
+def rng_seed_after(seed, n):
+    r = ORng()
+    L().o_rng_init(C.byref(r), seed)
+    for _ in range(n):
+        L().o_rng_uniform(C.byref(r))
+    return r.seed
+
+
+def brute_half_pairs(H, x, elem, cut_sk, kmax=3):
+    """O(N^2) reference set with explicit images: (i, j, n) with
+    |x_i - x_j - nH|^2 < (rc+skin)^2 using the FP64 predicate on s-space
+    images, keeping n != 0 or i < j."""
+    H = np.asarray(H, dtype=np.float64)
+    B = np.linalg.inv(H)
+    x = np.asarray(x, dtype=np.float64)
+    out = set()
+    n = len(x)
+    s = x @ B
+    s = s - np.floor(s)
+    s[s >= 1.0] -= 1.0
+    xs = s @ H
+    rng = range(-kmax, kmax + 1)
+    for a in rng:
+        for b in rng:
+            for c in rng:
+                img = np.array([a, b, c], dtype=np.float64)
+                xj = (s + img) @ H
+                d = xs[:, None, :] - xj[None, :, :]
+                rsq = (d ** 2).sum(-1)
+                cs = cut_sk[elem[:, None], elem[None, :]]
+                ii, jj = np.nonzero(rsq < cs ** 2)
+                for i, j in zip(ii, jj):
+                    if (a, b, c) == (0, 0, 0) and i >= j:
+                        continue
+                    out.add((int(i), int(j), a, b, c))
+    return out

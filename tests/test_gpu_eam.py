@@ -1,0 +1,214 @@
+"""GPU parity of the EAM path against the oracle, through the C ABI.
+
+Bars: neighbor sets bit-exact; forces within 1e-5 eV/A of the FP64 oracle
+(FP32 pair/force arithmetic). Energy and virial carry the rounding of FP32
+pair spline values (about 1e-7 relative per pair) summed in FP64, so their
+bars are per atom plus a floor for cells too small for the pair errors to
+average out: |dE| < 5e-8 eV * N + 1e-6 eV and max |dF_H| < 2e-7 eV * N +
+5e-6 eV. The
+virial is not bounded relative to itself because near zero pressure its
+components are small differences of large pair terms.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from mapp4py_amd import potentials as pot
+
+pytestmark = pytest.mark.gpu
+
+F_TOL = 1e-5       # eV/A, max abs force error vs FP64
+E_ATOM = 5e-8      # eV per atom, energy
+W_ATOM = 2e-7      # eV per atom, virial components
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def M():
+    import mapp4py_amd
+
+    return mapp4py_amd
+
+
+@pytest.fixture(scope="module")
+def cu_pot():
+    return pot.johnson_cu(nr=10000, nrho=10000)
+
+
+@pytest.fixture(scope="module")
+def feni_pot():
+    return pot.johnson_feni_fs(nr=5000, nrho=5000)
+
+
+def jitter(x, s, seed):
+    return x + np.random.default_rng(seed).normal(0.0, s, x.shape)
+
+
+def triples(i, j, img):
+    return {(int(a), int(b), int(c[0]), int(c[1]), int(c[2])) for a, b, c in zip(i, j, img)}
+
+
+def both(M, p, H, x, elem, skin=0.5):
+    g = M.AtomsMD(M.ForceFieldEAM.from_knots(p), H, x, elem, skin=skin)
+    o = O.OMD(O.OEam.from_knots(p), H, x, elem, skin=skin)
+    return g, o
+
+
+def check_forces(g, o, tag):
+    f, e, FH = g.force_calc()
+    fo, eo, FHo, rho_o = o.force()
+    n = len(f)
+    df = np.abs(f - fo).max()
+    de = abs(e - eo) / n
+    dW = np.abs(FH - FHo).max() / n
+    say(f"{tag}: max|df|={df:.2e} eV/A  |dE|/N={de:.2e} eV (dE/E={abs(e - eo) / abs(eo):.1e})  "
+        f"max|dF_H|/N={dW:.2e} eV (|F_H|max/N={np.abs(FHo).max() / n:.2e})")
+    assert df < F_TOL, (tag, df)
+    assert abs(e - eo) < E_ATOM * n + 1e-6, (tag, de)
+    assert np.abs(FH - FHo).max() < W_ATOM * n + 5e-6, (tag, dW)
+    np.testing.assert_allclose(g.rho(), rho_o, rtol=1e-7)  # FP32 spline values, FP64 sum
+    return f, e, FH
+
+
+@pytest.mark.parametrize("cells,tilt,seed", [((4, 4, 4), (0, 0, 0), 1), ((3, 4, 5), (0.7, -0.4, 0.9), 2),
+                                             ((1, 2, 2), (0.3, 0.1, -0.2), 3), ((6, 6, 6), (0, 0, 0), 4)])
+def test_neighbor_sets_bit_exact(M, cu_pot, cells, tilt, seed):
+    say(f"neighbor parity: fcc {cells} tilt {tilt}")
+    x, H = pot.fcc(3.615, cells)
+    H = H.copy()
+    H[1, 0], H[2, 0], H[2, 1] = tilt
+    x = jitter(x, 0.1, seed)
+    elem = np.zeros(len(x), np.uint8)
+    g, o = both(M, cu_pot, H, x, elem)
+    gs = triples(*g.half_pairs())
+    os_ = triples(*o.pairs())
+    assert len(gs) == len(os_) and gs == os_
+    np.testing.assert_array_equal(g.x, o.x)  # wrapped positions re-expressed identically
+
+
+def test_neighbor_sets_on_the_cutoff_sphere(M, cu_pot):
+    # choose the lattice so the 3rd fcc shell sits exactly on r_c + skin: the
+    # FP64 predicate's rounding alone decides membership of those pairs
+    rc = float(O.OEam.from_knots(cu_pot).cut()[0, 0]) + 0.5
+    a = rc / np.sqrt(1.5)
+    x, H = pot.fcc(a, (5, 5, 5))
+    elem = np.zeros(len(x), np.uint8)
+    g, o = both(M, cu_pot, H, x, elem)
+    gs, os_ = triples(*g.half_pairs()), triples(*o.pairs())
+    say(f"cutoff-sphere lattice a={a:.6f}: {len(gs)} pairs")
+    assert gs == os_
+
+
+def test_forces_energy_virial_cu_32000(M, cu_pot):
+    # config 1 geometry: fcc Cu 20^3 cells, jitter N(0, 0.05 A)
+    x, H = pot.fcc(3.615, (20, 20, 20))
+    x = jitter(x, 0.05, 2024)
+    elem = np.zeros(len(x), np.uint8)
+    g, o = both(M, cu_pot, H, x, elem)
+    f, e, FH = check_forces(g, o, "Cu 32000")
+    assert abs(g.value() - e) < E_ATOM * len(f)
+    np.testing.assert_allclose(f.sum(0), 0.0, atol=2e-3)
+
+
+def test_forces_fs_two_elements_triclinic(M, feni_pot):
+    x, H = pot.bcc(2.86, (8, 8, 8))
+    H = H.copy()
+    H[1, 0], H[2, 1] = 0.8, -0.5
+    x = jitter(x, 0.06, 7)
+    elem = (np.random.default_rng(3).random(len(x)) < 0.2).astype(np.uint8)
+    g, o = both(M, feni_pot, H, x, elem, skin=0.4)
+    assert triples(*g.half_pairs()) == triples(*o.pairs())
+    check_forces(g, o, "FeNi fs 1024")
+
+
+def test_small_box_images(M, cu_pot):
+    x, H = pot.fcc(3.615, (1, 1, 1))
+    x = jitter(x, 0.05, 5)
+    elem = np.zeros(len(x), np.uint8)
+    g, o = both(M, cu_pot, H, x, elem)
+    assert triples(*g.half_pairs()) == triples(*o.pairs())
+    check_forces(g, o, "Cu 4 atoms, box < 2 r_c")
+
+
+def test_shipped_setfl_and_fs_tables(M):
+    import os
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    path = os.path.join(root, "mapp4py_amd", "data", "NiAlH_jea.eam.alloy")
+    ffg = M.ForceFieldEAM.setfl(path, ["Ni", "H"])
+    ffo = O.OEam.read(path, 0)
+    # element subset: file order Ni Al H -> Ni, H
+    assert ffg.ne == 2
+    x, H = pot.fcc(3.52, (4, 4, 4))
+    x = jitter(x, 0.05, 8)
+    elem = (np.random.default_rng(1).random(len(x)) < 0.1).astype(np.uint8)
+    g = M.AtomsMD(ffg, H, x, elem)
+    # oracle with the full three-element file: map H to index 2
+    o = O.OMD(ffo, H, x, np.where(elem == 1, 2, 0).astype(np.uint8))
+    check_forces(g, o, "NiAlH_jea (Ni, H subset)")
+
+
+def test_update_decides_like_reference(M, cu_pot):
+    x, H = pot.fcc(3.615, (4, 4, 4))
+    x = jitter(x, 0.05, 9)
+    elem = np.zeros(len(x), np.uint8)
+    g, o = both(M, cu_pot, H, x, elem)
+    xw = g.x
+    # below skin/2 everywhere: no rebuild; one atom past skin/2: rebuild
+    small = xw.copy()
+    small[:, 0] += 0.2
+    g.x = small
+    o.x = small
+    assert g.update() is False and o.update() is False
+    check_forces(g, o, "after small move")
+    big = small.copy()
+    big[7, 1] += 0.3
+    g.x = big
+    o.x = big
+    assert g.update() is True and o.update() is True
+    assert triples(*g.half_pairs()) == triples(*o.pairs())
+    check_forces(g, o, "after rebuild")
+
+
+def test_nvt_matches_oracle_short_run(M, cu_pot):
+    x, H = pot.fcc(3.615, (6, 6, 6))
+    x = jitter(x, 0.05, 10)
+    elem = np.zeros(len(x), np.uint8)
+    masses = [pot.MASS["Cu"]]
+    v = O.o_create_T(elem, masses, pot.KB_EV, 300.0, 3)
+    g, o = both(M, cu_pot, H, x, elem)
+    g.x_d = v
+    nvt = M.NVT(300.0, 1.0 * pot.FS)
+    out = nvt.run(g, 40, masses)
+    vo, outo, Tpo, nreb_o = o.nvt_run(v, masses, pot.KB_EV, 300.0, 1.0 * pot.FS, 40)
+    dx = np.abs(g.x - o.x).max()
+    dv = np.abs(g.x_d - vo).max()
+    say(f"NVT 40 steps: max|dx|={dx:.2e} A  max|dv|={dv:.2e}  T {nvt.T_part:.6f} vs {Tpo:.6f}  "
+        f"rebuilds {nvt.rebuilds} vs {nreb_o}")
+    assert nvt.rebuilds == nreb_o
+    assert dx < 1e-6 and dv < 1e-6
+    assert abs(nvt.T_part - Tpo) < 1e-6 * Tpo
+    assert abs(out[0] - outo[0]) < E_ATOM * len(x)
+
+
+def test_nvt_long_run_forces_stay_exact(M, cu_pot):
+    # after many steps and rebuilds the GPU list must still cover every pair
+    # within r_c: compare with a fresh oracle at the GPU's positions
+    x, H = pot.fcc(3.615, (8, 8, 8))
+    x = jitter(x, 0.05, 11)
+    elem = np.zeros(len(x), np.uint8)
+    masses = [pot.MASS["Cu"]]
+    g = M.AtomsMD(M.ForceFieldEAM.from_knots(cu_pot), H, x, elem)
+    g.x_d = O.o_create_T(elem, masses, pot.KB_EV, 1200.0, 4)
+    nvt = M.NVT(1200.0, 2.0 * pot.FS)
+    nvt.run(g, 300, masses)
+    say(f"NVT 300 steps at 1200 K: {nvt.rebuilds} rebuilds")
+    assert nvt.rebuilds >= 2
+    o = O.OMD(O.OEam.from_knots(cu_pot), H, g.x, elem)
+    check_forces(g, o, "after 300 NVT steps")
