@@ -91,6 +91,17 @@ mapp_status mapp_md_list_stats(mapp_md* md, int* max_neighbors, int64_t* full_en
  * each step, neighbor rebuild on the skin criterion. masses[ne]. */
 mapp_status mapp_md_nvt_run(mapp_md* md, const double* masses, double kB, double T, double dt, int nsteps,
                             double* out7, double* T_part, int* rebuilds);
+/* Kernel timing with HIP events on the system's stream (no counterpart in
+ * the reference, which has no profiler). enable != 0 starts recording and
+ * clears the totals; read returns the summed duration and launch count of a
+ * kernel group: "eam_rho", "eam_force", "build_list", "cell_sort",
+ * "integrate", "eam_energy". */
+mapp_status mapp_md_profile(mapp_md* md, int enable);
+mapp_status mapp_md_profile_read(mapp_md* md, const char* kernel, double* total_ms, int64_t* launches);
+/* Launch-shape knobs (no counterpart in the reference): neighbors resolved
+ * per batch in the density pass (1, 4, 8) and the force pass (1, 2, 4, 8).
+ * Any value gives the same results; the tuner picks the fastest. */
+mapp_status mapp_md_set_batch(mapp_md* md, int rho_batch, int force_batch);
 /* AtomsMD::create_T (src/atoms_md.cpp:130-160): Maxwell velocities from the
  * Park-Miller / Marsaglia stream of src/random.cpp:7-68 seeded with seed. */
 mapp_status mapp_create_T(int natms, const uint8_t* elem, const double* masses, double kB, double T, int seed,

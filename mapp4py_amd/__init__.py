@@ -152,6 +152,17 @@ class AtomsMD:
         check(lib().mapp_md_list_stats(self._h, C.byref(m), C.byref(s), C.byref(r)))
         return m.value, s.value, r.value
 
+    def set_batch(self, rho, force):
+        check(lib().mapp_md_set_batch(self._h, int(rho), int(force)))
+
+    def profile(self, enable=True):
+        check(lib().mapp_md_profile(self._h, int(enable)))
+
+    def profile_read(self, kernel):
+        ms, n = C.c_double(), C.c_int64()
+        check(lib().mapp_md_profile_read(self._h, kernel.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def __del__(self):
         if getattr(self, "_h", None):
             lib().mapp_md_destroy(self._h)

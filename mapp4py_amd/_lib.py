@@ -50,6 +50,9 @@ SIGNATURES = {
     "mapp_md_pairs": (C.c_int, [C.c_void_p, i32p, i32p, i8p]),
     "mapp_md_list_stats": (C.c_int, [C.c_void_p, ip, i64p, ip]),
     "mapp_md_nvt_run": (C.c_int, [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, ip]),
+    "mapp_md_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "mapp_md_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, dp, i64p]),
+    "mapp_md_set_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mapp_create_T": (C.c_int, [C.c_int, u8p, dp, C.c_double, C.c_double, C.c_int, dp]),
 }
 

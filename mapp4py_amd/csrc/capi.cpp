@@ -220,6 +220,38 @@ mapp_status mapp_md_nvt_run(mapp_md* md, const double* masses, double kB, double
     });
 }
 
+mapp_status mapp_md_profile(mapp_md* md, int enable)
+{
+    return guard([&] {
+        NEED(md);
+        auto& p = md->sys->profiler();
+        p.reset();
+        p.on = enable != 0;
+    });
+}
+
+mapp_status mapp_md_profile_read(mapp_md* md, const char* kernel, double* total_ms, int64_t* launches)
+{
+    return guard([&] {
+        NEED(md);
+        NEED(kernel);
+        auto& p = md->sys->profiler();
+        p.flush();
+        for (int k = 0; k < mapp::K_COUNT; ++k)
+            if (std::strcmp(kernel, mapp::kernel_name(k)) == 0) {
+                if (total_ms) *total_ms = p.total_ms[k];
+                if (launches) *launches = p.launches[k];
+                return;
+            }
+        throw mapp::Error(mapp::ERR_ARG, std::string("unknown kernel group ") + kernel);
+    });
+}
+
+mapp_status mapp_md_set_batch(mapp_md* md, int rho_batch, int force_batch)
+{
+    return guard([&] { NEED(md); md->sys->set_batch(rho_batch, force_batch); });
+}
+
 mapp_status mapp_create_T(int natms, const uint8_t* elem, const double* masses, double kB, double T, int seed,
                           double* v)
 {

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace mapp {
 
@@ -65,5 +66,99 @@ struct DevBuf {
 };
 
 constexpr int kWave = 64;
+
+// Per-kernel timing with HIP events recorded on the stream the kernels run
+// on; enabled on request, read back after a synchronisation.
+enum KernelId : int {
+    K_EAM_RHO = 0,
+    K_EAM_FORCE,
+    K_BUILD_LIST,
+    K_CELL_SORT,
+    K_INTEGRATE,
+    K_EAM_ENERGY,
+    K_DMD_PASS1,
+    K_DMD_PASS2,
+    K_DMD_ENERGY,
+    K_SURR_INFER,
+    K_SURR_TRAIN,
+    K_GCMC,
+    K_COUNT
+};
+
+inline const char* kernel_name(int k)
+{
+    static const char* names[K_COUNT] = {"eam_rho",    "eam_force",  "build_list", "cell_sort",
+                                         "integrate",  "eam_energy", "dmd_pass1",  "dmd_pass2",
+                                         "dmd_energy", "surr_infer", "surr_train", "gcmc"};
+    return (k >= 0 && k < K_COUNT) ? names[k] : "";
+}
+
+class Profiler {
+public:
+    bool on = false;
+    double total_ms[K_COUNT] = {};
+    long launches[K_COUNT] = {};
+    ~Profiler()
+    {
+        for (auto& r : recs_) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        for (auto e : pool_) (void)hipEventDestroy(e);
+    }
+    void reset()
+    {
+        flush();
+        for (int k = 0; k < K_COUNT; ++k) {
+            total_ms[k] = 0.0;
+            launches[k] = 0;
+        }
+    }
+    void begin(int kid, hipStream_t s)
+    {
+        if (!on) return;
+        Rec r{kid, get(), get()};
+        MAPP_HIP(hipEventRecord(r.a, s));
+        recs_.push_back(r);
+    }
+    void end(hipStream_t s)
+    {
+        if (!on || recs_.empty()) return;
+        MAPP_HIP(hipEventRecord(recs_.back().b, s));
+    }
+    // waits for the recorded events and folds them into the totals
+    void flush()
+    {
+        for (auto& r : recs_) {
+            MAPP_HIP(hipEventSynchronize(r.b));
+            float ms = 0.f;
+            MAPP_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            total_ms[r.kid] += ms;
+            launches[r.kid] += 1;
+            pool_.push_back(r.a);
+            pool_.push_back(r.b);
+        }
+        recs_.clear();
+    }
+
+private:
+    struct Rec {
+        int kid;
+        hipEvent_t a, b;
+    };
+    hipEvent_t get()
+    {
+        if (!pool_.empty()) {
+            hipEvent_t e = pool_.back();
+            pool_.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        MAPP_HIP(hipEventCreate(&e));
+        return e;
+    }
+    std::vector<Rec> recs_;
+    std::vector<hipEvent_t> pool_;
+};
 
 }  // namespace mapp

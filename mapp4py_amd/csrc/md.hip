@@ -239,8 +239,34 @@ __device__ inline void embed(const EamArgs& P, int t, double rho, double& F, dou
     if (rho > P.rho_max) F += dF * (rho - P.rho_max);
 }
 
+// A batch of B neighbor entries is resolved in three stages -- all list
+// entries, then all neighbor positions, then all spline rows -- so each thread
+// has B independent loads in flight per stage instead of one dependent chain
+// per neighbor (the passes are latency bound, not bandwidth bound).
+struct PairGeo {
+    double dx, dy, dz, r;
+    int row;   // spline row (pair type * nr + knot), 0 when outside the cutoff
+    float p;
+    bool in;
+};
+
+__device__ inline PairGeo pair_geo(const double4& xi, int ti, uint32_t e, const double4& xj, const EamArgs& P,
+                                   const BoxDev& bd)
+{
+    PairGeo g;
+    pair_dx(xi, xj, e >> kIndexBits, bd.h, g.dx, g.dy, g.dz);
+    const double rsq = g.dx * g.dx + g.dy * g.dy + g.dz * g.dz;
+    const int tt = ti * P.ne + (int)xj.w;
+    g.in = rsq < P.cut_sq[tt];
+    g.r = root64(rsq);
+    const int m = knot(g.r, P.dr_inv, P.nr, g.p);
+    g.row = g.in ? tt * P.nr + m : 0;
+    return g;
+}
+
 // pass 1: rho_i = sum_j rho_{j->i}(r_ij) (FP64 sum of FP32 spline values),
 // F(rho_i) and F'(rho_i); embedding energy reduced per workgroup
+template <int B>
 __global__ __launch_bounds__(kBlock) void k_eam_rho(int n, const double4* __restrict__ x, const int* __restrict__ nn,
                                                     const uint32_t* __restrict__ list, int ldl, EamArgs P, BoxDev bd,
                                                     double* __restrict__ rho_out, float* __restrict__ dF_out,
@@ -254,18 +280,27 @@ __global__ __launch_bounds__(kBlock) void k_eam_rho(int n, const double4* __rest
         const int ti = (int)xi.w;
         const int cnt = nn[i];
         double rho = 0.0;
-        for (int k = 0; k < cnt; ++k) {
-            const uint32_t e = list[(size_t)k * ldl + i];
-            const double4 xj = x[e & kIndexMask];
-            double dx, dy, dz;
-            pair_dx(xi, xj, e >> kIndexBits, bd.h, dx, dy, dz);
-            const double rsq = dx * dx + dy * dy + dz * dz;
-            const int tt = ti * P.ne + (int)xj.w;
-            if (rsq < P.cut_sq[tt]) {
-                float p;
-                const int m = knot(root64(rsq), P.dr_inv, P.nr, p);
-                const float4 c = P.tabA[(size_t)tt * P.nr + m];
-                rho += (double)(((c.w * p + c.z) * p + c.y) * p + c.x);
+        for (int k0 = 0; k0 < cnt; k0 += B) {
+            uint32_t e[B];
+            double4 xj[B];
+            float4 c[B];
+            PairGeo g[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) e[b] = k0 + b < cnt ? list[(size_t)(k0 + b) * ldl + i] : (uint32_t)i | (kZeroImage << kIndexBits);
+#pragma unroll
+            for (int b = 0; b < B; ++b) xj[b] = x[e[b] & kIndexMask];
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                g[b] = pair_geo(xi, ti, e[b], xj[b], P, bd);
+                g[b].in = g[b].in && (k0 + b < cnt);
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) c[b] = P.tabA[g[b].in ? g[b].row : 0];
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const float p = g[b].p;
+                const float v = ((c[b].w * p + c[b].z) * p + c[b].y) * p + c[b].x;
+                if (g[b].in) rho += (double)v;
             }
         }
         double F, dF;
@@ -279,6 +314,7 @@ __global__ __launch_bounds__(kBlock) void k_eam_rho(int n, const double4* __rest
 
 // pass 2: pair + embedding forces on the owner only (no scatter), half of
 // each full-list pair's energy and virial
+template <int B>
 __global__ __launch_bounds__(kBlock) void k_eam_force(int n, const double4* __restrict__ x,
                                                       const int* __restrict__ nn, const uint32_t* __restrict__ list,
                                                       int ldl, EamArgs P, BoxDev bd, const float* __restrict__ dF,
@@ -293,28 +329,44 @@ __global__ __launch_bounds__(kBlock) void k_eam_force(int n, const double4* __re
         const int cnt = nn[i];
         const float dFi = dF[i];
         float fx = 0.f, fy = 0.f, fz = 0.f;
-        for (int k = 0; k < cnt; ++k) {
-            const uint32_t e = list[(size_t)k * ldl + i];
-            const int j = (int)(e & kIndexMask);
-            const double4 xj = x[j];
-            double dx, dy, dz;
-            pair_dx(xi, xj, e >> kIndexBits, bd.h, dx, dy, dz);
-            const double rsq = dx * dx + dy * dy + dz * dz;
-            const int tt = ti * P.ne + (int)xj.w;
-            if (rsq < P.cut_sq[tt]) {
-                const double r = root64(rsq);
-                float p;
-                const int m = knot(r, P.dr_inv, P.nr, p);
-                const float4* c = P.tabB + ((size_t)tt * P.nr + m) * 3;
-                const float4 ca = c[0], cb = c[1], cp = c[2];
-                const float rinv = 1.0f / (float)r;
-                const float d_in = ((3.f * ca.w * p + 2.f * ca.z) * p + ca.y) * P.dr_invf;
-                const float d_out = ((3.f * cb.w * p + 2.f * cb.z) * p + cb.y) * P.dr_invf;
-                const float z = ((cp.w * p + cp.z) * p + cp.y) * p + cp.x;
-                const float zp = ((3.f * cp.w * p + 2.f * cp.z) * p + cp.y) * P.dr_invf;
+        for (int k0 = 0; k0 < cnt; k0 += B) {
+            uint32_t e[B];
+            double4 xj[B];
+            float dFj[B];
+            float4 ca[B], cb[B], cp[B];
+            PairGeo g[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) e[b] = k0 + b < cnt ? list[(size_t)(k0 + b) * ldl + i] : (uint32_t)i | (kZeroImage << kIndexBits);
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                xj[b] = x[e[b] & kIndexMask];
+                dFj[b] = dF[e[b] & kIndexMask];
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                g[b] = pair_geo(xi, ti, e[b], xj[b], P, bd);
+                g[b].in = g[b].in && (k0 + b < cnt);
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const float4* c = P.tabB + (size_t)(g[b].in ? g[b].row : 0) * 3;
+                ca[b] = c[0];
+                cb[b] = c[1];
+                cp[b] = c[2];
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                if (!g[b].in) continue;
+                const float p = g[b].p;
+                const float rinv = 1.0f / (float)g[b].r;
+                const float d_in = ((3.f * ca[b].w * p + 2.f * ca[b].z) * p + ca[b].y) * P.dr_invf;
+                const float d_out = ((3.f * cb[b].w * p + 2.f * cb[b].z) * p + cb[b].y) * P.dr_invf;
+                const float z = ((cp[b].w * p + cp[b].z) * p + cp[b].y) * p + cp[b].x;
+                const float zp = ((3.f * cp[b].w * p + 2.f * cp[b].z) * p + cp[b].y) * P.dr_invf;
                 const float phi = z * rinv;
                 const float dphi = (zp - phi) * rinv;
-                const float fpair = -(dphi + dFi * d_in + dF[j] * d_out) * rinv;
+                const float fpair = -(dphi + dFi * d_in + dFj[b] * d_out) * rinv;
+                const double dx = g[b].dx, dy = g[b].dy, dz = g[b].dz;
                 fx += fpair * (float)dx;
                 fy += fpair * (float)dy;
                 fz += fpair * (float)dz;
@@ -667,6 +719,14 @@ MDSystem::~MDSystem()
 
 void MDSystem::sync() { MAPP_HIP(hipStreamSynchronize(stream_)); }
 
+void MDSystem::set_batch(int rho, int force)
+{
+    MAPP_REQUIRE(rho == 1 || rho == 4 || rho == 8, ERR_ARG, "rho batch must be 1, 4 or 8");
+    MAPP_REQUIRE(force == 1 || force == 2 || force == 4 || force == 8, ERR_ARG, "force batch must be 1, 2, 4 or 8");
+    batch_rho_ = rho;
+    batch_force_ = force;
+}
+
 // cells of at least max_cut in every perpendicular width; boxes thinner than
 // max_cut are searched over +/-R images (R <= 2 fits the 7-bit image code)
 void MDSystem::setup_grid()
@@ -722,6 +782,7 @@ void MDSystem::rebuild()
     BoxDev bd;
     std::memcpy(bd.h, box_.h, sizeof bd.h);
     std::memcpy(bd.b, box_.b, sizeof bd.b);
+    prof_.begin(K_CELL_SORT, stream_);
     hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), bd, g, cell_rank_.get(),
                        s_tmp_.get(), keys_.get(), vals_.get());
     int end_bit = 1;
@@ -742,6 +803,7 @@ void MDSystem::rebuild()
     std::swap(id_.p, id_alt_.p);
     hipLaunchKernelGGL(k_cell_start, dim3((grid_.ncells + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n_,
                        grid_.ncells, keys_alt_.get(), cell_start_.get());
+    prof_.end(stream_);
     build_list();
     ++nrebuild_;
 }
@@ -768,8 +830,10 @@ void MDSystem::build_list()
     for (int attempt = 0; attempt < 2; ++attempt) {
         list_.reserve((size_t)maxn_ * ldl_);
         MAPP_HIP(hipMemsetAsync(flags_.get(), 0, sizeof(int), stream_));
+        prof_.begin(K_BUILD_LIST, stream_);
         hipLaunchKernelGGL(k_build_list, dim3(nb), dim3(kBlock), 0, stream_, n_, s_.get(), bd, g, cell_rank_.get(),
                            cell_start_.get(), maxn_, ldl_, list_.get(), nn_.get(), flags_.get());
+        prof_.end(stream_);
         int over = 0;
         MAPP_HIP(hipMemcpyAsync(&over, flags_.get(), sizeof(int), hipMemcpyDeviceToHost, stream_));
         MAPP_HIP(hipStreamSynchronize(stream_));
@@ -786,10 +850,22 @@ void MDSystem::force_passes(bool)
     BoxDev bd;
     std::memcpy(bd.h, box_.h, sizeof bd.h);
     std::memcpy(bd.b, box_.b, sizeof bd.b);
-    hipLaunchKernelGGL(k_eam_rho, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_, P,
-                       bd, rho_.get(), dF_.get(), part_.get());
-    hipLaunchKernelGGL(k_eam_force, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_,
-                       P, bd, dF_.get(), f_.get(), part_.get() + (size_t)nb * kPartW);
+    prof_.begin(K_EAM_RHO, stream_);
+    switch (batch_rho_) {
+        case 1: hipLaunchKernelGGL(k_eam_rho<1>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), dF_.get(), part_.get()); break;
+        case 4: hipLaunchKernelGGL(k_eam_rho<4>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), dF_.get(), part_.get()); break;
+        default: hipLaunchKernelGGL(k_eam_rho<8>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), dF_.get(), part_.get()); break;
+    }
+    prof_.end(stream_);
+    prof_.begin(K_EAM_FORCE, stream_);
+    double* p2 = part_.get() + (size_t)nb * kPartW;
+    switch (batch_force_) {
+        case 1: hipLaunchKernelGGL(k_eam_force<1>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_, P, bd, dF_.get(), f_.get(), p2); break;
+        case 2: hipLaunchKernelGGL(k_eam_force<2>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_, P, bd, dF_.get(), f_.get(), p2); break;
+        case 8: hipLaunchKernelGGL(k_eam_force<8>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_, P, bd, dF_.get(), f_.get(), p2); break;
+        default: hipLaunchKernelGGL(k_eam_force<4>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_, P, bd, dF_.get(), f_.get(), p2); break;
+    }
+    prof_.end(stream_);
     hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, stream_, 2 * nb, 7, part_.get(), red_.get());
 }
 
@@ -804,7 +880,9 @@ MDReport MDSystem::reduce_report(int)
 MDReport MDSystem::force_calc()
 {
     force_passes(true);
-    return reduce_report(7);
+    MDReport r = reduce_report(7);
+    prof_.flush();
+    return r;
 }
 
 double MDSystem::value()
@@ -814,8 +892,10 @@ double MDSystem::value()
     BoxDev bd;
     std::memcpy(bd.h, box_.h, sizeof bd.h);
     std::memcpy(bd.b, box_.b, sizeof bd.b);
+    prof_.begin(K_EAM_ENERGY, stream_);
     hipLaunchKernelGGL(k_eam_energy, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_,
                        P, bd, part_.get());
+    prof_.end(stream_);
     hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, stream_, nb, 1, part_.get(), red_.get());
     double e = 0.0;
     MAPP_HIP(hipMemcpyAsync(&e, red_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
@@ -1012,6 +1092,7 @@ int MDSystem::nvt_run(const double* masses, double kB, double T, double dt, int 
     const double skin_sq4 = 0.25 * skin_ * skin_;
     int flag = 0;
     for (int step = 0; step < nsteps; ++step) {
+        prof_.begin(K_INTEGRATE, stream_);
         hipLaunchKernelGGL(k_nhc, dim3(1), dim3(64), 0, stream_, nh, mvv, A, 1);
         hipLaunchKernelGGL(k_kick_drift, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), v_.get(), f_.get(), A, nh,
                            part_.get() + (size_t)2 * nb * kPartW);
@@ -1020,20 +1101,24 @@ int MDSystem::nvt_run(const double* masses, double kB, double T, double dt, int 
         MAPP_HIP(hipMemsetAsync(flags_.get() + 1, 0, sizeof(int), stream_));
         hipLaunchKernelGGL(k_shift_decide, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), x0_.get(), com,
                            1.0 / n_, skin_sq4, flags_.get());
+        prof_.end(stream_);
         MAPP_HIP(hipMemcpyAsync(&flag, flags_.get() + 1, sizeof(int), hipMemcpyDeviceToHost, stream_));
         MAPP_HIP(hipStreamSynchronize(stream_));
         if (flag) rebuild();
         force_passes(true);
+        prof_.begin(K_INTEGRATE, stream_);
         hipLaunchKernelGGL(k_kick2, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), v_.get(), f_.get(), A, 1,
                            part_.get() + (size_t)2 * nb * kPartW);
         hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, stream_, nb, 6,
                            part_.get() + (size_t)2 * nb * kPartW, mvv);
         hipLaunchKernelGGL(k_nhc, dim3(1), dim3(64), 0, stream_, nh, mvv, A, 2);
+        prof_.end(stream_);
     }
     hipLaunchKernelGGL(k_scale_v, dim3(nb), dim3(kBlock), 0, stream_, n_, v_.get(), nh);
     double nhh[NH_SIZE];
     MAPP_HIP(hipMemcpyAsync(nhh, nh, sizeof(double) * NH_SIZE, hipMemcpyDeviceToHost, stream_));
     MDReport r = reduce_report(7);
+    prof_.flush();
     if (rep) *rep = r;
     if (T_part) *T_part = nhh[NH_T];
     return nrebuild_ - rebuilds0;

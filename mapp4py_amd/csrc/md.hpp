@@ -77,6 +77,11 @@ public:
     // MDNVT::run; masses per element; returns rebuild count
     int nvt_run(const double* masses, double kB, double T, double dt, int nsteps, MDReport* rep, double* T_part);
     hipStream_t stream() const { return stream_; }
+    Profiler& profiler() { return prof_; }
+    // launch-shape knobs picked by the tuner: neighbors resolved per batch
+    void set_batch(int rho, int force);
+    int batch_rho() const { return batch_rho_; }
+    int batch_force() const { return batch_force_; }
     void sync();
     int last_rebuilds() const { return nrebuild_; }
 
@@ -118,6 +123,8 @@ private:
     DevBuf<double> part_;            // per-block partial sums
     DevBuf<double> red_;             // reduced values
     std::vector<double> masses_;
+    Profiler prof_;
+    int batch_rho_ = 8, batch_force_ = 8;
 };
 
 }  // namespace mapp

@@ -212,3 +212,14 @@ def test_nvt_long_run_forces_stay_exact(M, cu_pot):
     assert nvt.rebuilds >= 2
     o = O.OMD(O.OEam.from_knots(cu_pot), H, g.x, elem)
     check_forces(g, o, "after 300 NVT steps")
+
+
+def test_every_batch_shape_gives_the_oracle_forces(M, cu_pot):
+    x, H = pot.fcc(3.615, (10, 10, 10))
+    x = jitter(x, 0.05, 21)
+    elem = np.zeros(len(x), np.uint8)
+    g, o = both(M, cu_pot, H, x, elem)
+    for rb in (1, 4, 8):
+        for fb in (1, 2, 4, 8):
+            g.set_batch(rb, fb)
+            check_forces(g, o, f"batch rho={rb} force={fb}")
