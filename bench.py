@@ -114,7 +114,17 @@ def main():
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the plain copy kernel (256 MiB, 20 launches) used to calibrate HBM counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import ctypes as C
+        from mapp4py_amd import _lib
+        g = C.c_double()
+        _lib.check(_lib.lib().mapp_diag_copy(256 << 20, 20, C.byref(g)))
+        print(json.dumps({"copy_bytes_per_launch": 2 * (256 << 20), "read_bytes": 256 << 20, "write_bytes": 256 << 20,
+                          "launches": 21, "GBps": g.value}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)

@@ -107,6 +107,12 @@ mapp_status mapp_md_set_batch(mapp_md* md, int rho_batch, int force_batch);
 mapp_status mapp_create_T(int natms, const uint8_t* elem, const double* masses, double kB, double T, int seed,
                           double* v);
 
+/* Diagnostic (no counterpart in the reference): copies `bytes` between two
+ * device buffers `reps` times with a plain streaming kernel and reports the
+ * achieved bandwidth (read + write bytes per second / 1e9). Used to calibrate
+ * the HBM traffic counters and to measure the device's copy bandwidth. */
+mapp_status mapp_diag_copy(size_t bytes, int reps, double* gbps);
+
 #ifdef __cplusplus
 }
 #endif

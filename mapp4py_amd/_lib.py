@@ -53,6 +53,7 @@ SIGNATURES = {
     "mapp_md_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "mapp_md_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, dp, i64p]),
     "mapp_md_set_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "mapp_diag_copy": (C.c_int, [C.c_size_t, C.c_int, dp]),
     "mapp_create_T": (C.c_int, [C.c_int, u8p, dp, C.c_double, C.c_double, C.c_int, dp]),
 }
 

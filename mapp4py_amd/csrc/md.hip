@@ -53,14 +53,20 @@ __device__ inline void block_sum_store(double (&v)[K], double* out)
 }
 
 // Fixed-order reduction of rows of partial sums: out[k] = sum_r part[r*W + k].
+// Block b sums rows [b*chunk, (b+1)*chunk) and writes row b of out, so a
+// launch over G blocks followed by one over a single block is a fixed-order
+// (deterministic) two-level reduction.
 __global__ __launch_bounds__(kBlock) void k_reduce_rows(int nrows, int ncols, const double* __restrict__ part,
-                                                        double* __restrict__ out)
+                                                        double* __restrict__ out, int chunk)
 {
     __shared__ double sh[kBlock][kPartW];
     double acc[kPartW];
     for (int k = 0; k < kPartW; ++k) acc[k] = 0.0;
-    for (int r = threadIdx.x; r < nrows; r += kBlock)
+    const int r0 = blockIdx.x * chunk;
+    const int r1 = min(nrows, r0 + chunk);
+    for (int r = r0 + threadIdx.x; r < r1; r += kBlock)
         for (int k = 0; k < ncols; ++k) acc[k] += part[(size_t)r * kPartW + k];
+    out += (size_t)blockIdx.x * (gridDim.x > 1 ? kPartW : 0);
     for (int k = 0; k < kPartW; ++k) sh[threadIdx.x][k] = acc[k];
     __syncthreads();
     for (int s = kBlock / 2; s > 0; s >>= 1) {
@@ -250,13 +256,22 @@ struct PairGeo {
     bool in;
 };
 
-__device__ inline PairGeo pair_geo(const double4& xi, int ti, uint32_t e, const double4& xj, const EamArgs& P,
+// pass-2 record: position with (type, F'(rho)) packed into the fourth slot,
+// so one gather brings everything pass 2 needs about a neighbor
+__device__ inline double pack_type_dF(int t, float dF)
+{
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)t << 32) | __float_as_uint(dF)));
+}
+__device__ inline int rec_type(double w) { return (int)((unsigned long long)__double_as_longlong(w) >> 32); }
+__device__ inline float rec_dF(double w) { return __uint_as_float((unsigned)(__double_as_longlong(w) & 0xffffffffull)); }
+
+__device__ inline PairGeo pair_geo(const double4& xi, int ti, uint32_t e, const double4& xj, int tj, const EamArgs& P,
                                    const BoxDev& bd)
 {
     PairGeo g;
     pair_dx(xi, xj, e >> kIndexBits, bd.h, g.dx, g.dy, g.dz);
     const double rsq = g.dx * g.dx + g.dy * g.dy + g.dz * g.dz;
-    const int tt = ti * P.ne + (int)xj.w;
+    const int tt = ti * P.ne + tj;
     g.in = rsq < P.cut_sq[tt];
     g.r = root64(rsq);
     const int m = knot(g.r, P.dr_inv, P.nr, g.p);
@@ -264,34 +279,99 @@ __device__ inline PairGeo pair_geo(const double4& xi, int ti, uint32_t e, const 
     return g;
 }
 
+// 16-byte position record for the density pass: three 41-bit fixed-point
+// coordinates x = o + c q and the element in the top 5 bits. Differences of
+// two records are exact integers, so dx carries FP64 quality (q is about
+// 2e-10 A for a 500 A frame) from half the bytes of a double4 gather.
+constexpr unsigned long long kFxMask = (1ull << 41) - 1ull;
+
+__device__ inline uint4 fx_encode(const FxFrame& F, double x, double y, double z, int t, bool& ok)
+{
+    const double a = (x - F.o[0]) * F.inv_q, b = (y - F.o[1]) * F.inv_q, c = (z - F.o[2]) * F.inv_q;
+    const double lim = (double)kFxMask;
+    ok = a >= 0.0 && b >= 0.0 && c >= 0.0 && a <= lim && b <= lim && c <= lim;
+    const unsigned long long ca = ok ? (unsigned long long)rint(a) : 0ull;
+    const unsigned long long cb = ok ? (unsigned long long)rint(b) : 0ull;
+    const unsigned long long cc = ok ? (unsigned long long)rint(c) : 0ull;
+    const unsigned long long lo = (ca & kFxMask) | ((cb & kFxMask) << 41);
+    const unsigned long long hi = ((cb & kFxMask) >> 23) | ((cc & kFxMask) << 18) | ((unsigned long long)t << 59);
+    return make_uint4((unsigned)lo, (unsigned)(lo >> 32), (unsigned)hi, (unsigned)(hi >> 32));
+}
+
+__device__ inline void fx_decode(const uint4 r, long long& cx, long long& cy, long long& cz, int& t)
+{
+    const unsigned long long lo = (unsigned long long)r.x | ((unsigned long long)r.y << 32);
+    const unsigned long long hi = (unsigned long long)r.z | ((unsigned long long)r.w << 32);
+    cx = (long long)(lo & kFxMask);
+    cy = (long long)(((lo >> 41) | (hi << 23)) & kFxMask);
+    cz = (long long)((hi >> 18) & kFxMask);
+    t = (int)(hi >> 59);
+}
+
+// records of every atom; flags[2] is raised if a coordinate leaves the frame
+__global__ __launch_bounds__(kBlock) void k_make_records(int n, const double4* __restrict__ x, FxFrame F,
+                                                         uint4* __restrict__ rec, int* __restrict__ flags)
+{
+    const int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double4 xi = x[i];
+    bool ok;
+    rec[i] = fx_encode(F, xi.x, xi.y, xi.z, (int)xi.w, ok);
+    if (!ok) atomicOr(&flags[2], 1);
+}
+
 // pass 1: rho_i = sum_j rho_{j->i}(r_ij) (FP64 sum of FP32 spline values),
-// F(rho_i) and F'(rho_i); embedding energy reduced per workgroup
+// F(rho_i) and F'(rho_i); embedding energy reduced per workgroup. FX reads
+// neighbor positions from the fixed-point records (one 16-byte gather).
 template <int B>
-__global__ __launch_bounds__(kBlock) void k_eam_rho(int n, const double4* __restrict__ x, const int* __restrict__ nn,
+__global__ __launch_bounds__(kBlock) void k_eam_rho(int n, const double4* __restrict__ x, const uint4* __restrict__ rec,
+                                                    FxFrame F, const int* __restrict__ fx_bad, const int* __restrict__ nn,
                                                     const uint32_t* __restrict__ list, int ldl, EamArgs P, BoxDev bd,
-                                                    double* __restrict__ rho_out, float* __restrict__ dF_out,
+                                                    double* __restrict__ rho_out, double4* __restrict__ xf,
                                                     double* __restrict__ part)
 {
     const int blk = xcd_block();
     const int i = blk * kBlock + threadIdx.x;
     double acc[1] = {0.0};
+    // records are used unless a coordinate fell outside their frame (uniform branch)
+    const bool FX = fx_bad[0] == 0;
     if (i < n) {
         const double4 xi = x[i];
         const int ti = (int)xi.w;
         const int cnt = nn[i];
+        long long ci0 = 0, ci1 = 0, ci2 = 0;
+        if (FX) {
+            int t;
+            fx_decode(rec[i], ci0, ci1, ci2, t);
+        }
         double rho = 0.0;
         for (int k0 = 0; k0 < cnt; k0 += B) {
             uint32_t e[B];
             double4 xj[B];
+            uint4 rj[B];
             float4 c[B];
             PairGeo g[B];
 #pragma unroll
             for (int b = 0; b < B; ++b) e[b] = k0 + b < cnt ? list[(size_t)(k0 + b) * ldl + i] : (uint32_t)i | (kZeroImage << kIndexBits);
+            if (FX) {
 #pragma unroll
-            for (int b = 0; b < B; ++b) xj[b] = x[e[b] & kIndexMask];
+                for (int b = 0; b < B; ++b) rj[b] = rec[e[b] & kIndexMask];
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    long long c0, c1, c2;
+                    int tj;
+                    fx_decode(rj[b], c0, c1, c2, tj);
+                    // exact integer differences, re-based on x_i so pair_geo sees x_i - x_j
+                    xj[b] = make_double4(xi.x - (double)(ci0 - c0) * F.q, xi.y - (double)(ci1 - c1) * F.q,
+                                         xi.z - (double)(ci2 - c2) * F.q, (double)tj);
+                }
+            } else {
+#pragma unroll
+                for (int b = 0; b < B; ++b) xj[b] = x[e[b] & kIndexMask];
+            }
 #pragma unroll
             for (int b = 0; b < B; ++b) {
-                g[b] = pair_geo(xi, ti, e[b], xj[b], P, bd);
+                g[b] = pair_geo(xi, ti, e[b], xj[b], (int)xj[b].w, P, bd);
                 g[b].in = g[b].in && (k0 + b < cnt);
             }
 #pragma unroll
@@ -306,28 +386,34 @@ __global__ __launch_bounds__(kBlock) void k_eam_rho(int n, const double4* __rest
         double F, dF;
         embed(P, ti, rho, F, dF);
         rho_out[i] = rho;
-        dF_out[i] = (float)dF;
+        xf[i] = make_double4(xi.x, xi.y, xi.z, pack_type_dF(ti, (float)dF));
         acc[0] = F;
     }
     block_sum_store<1>(acc, part + (size_t)blk * kPartW);
 }
 
+// second half kick of velocity Verlet, fused into the force pass: the owner
+// of f_i updates v_i and adds m v (x) v to the kinetic tensor
+
 // pass 2: pair + embedding forces on the owner only (no scatter), half of
 // each full-list pair's energy and virial
-template <int B>
+template <int B, bool ONE>
 __global__ __launch_bounds__(kBlock) void k_eam_force(int n, const double4* __restrict__ x,
                                                       const int* __restrict__ nn, const uint32_t* __restrict__ list,
-                                                      int ldl, EamArgs P, BoxDev bd, const float* __restrict__ dF,
-                                                      float4* __restrict__ f_out, double* __restrict__ part)
+                                                      int ldl, EamArgs P, BoxDev bd,
+                                                      float4* __restrict__ f_out, double* __restrict__ part,
+                                                      KickArgs K, double4* __restrict__ v, double* __restrict__ part_mvv)
 {
+    // x holds the pass-2 records written by the density pass
     const int blk = xcd_block();
     const int i = blk * kBlock + threadIdx.x;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    double mvv[6] = {0, 0, 0, 0, 0, 0};
     if (i < n) {
         const double4 xi = x[i];
-        const int ti = (int)xi.w;
+        const int ti = rec_type(xi.w);
         const int cnt = nn[i];
-        const float dFi = dF[i];
+        const float dFi = rec_dF(xi.w);
         float fx = 0.f, fy = 0.f, fz = 0.f;
         for (int k0 = 0; k0 < cnt; k0 += B) {
             uint32_t e[B];
@@ -338,21 +424,28 @@ __global__ __launch_bounds__(kBlock) void k_eam_force(int n, const double4* __re
 #pragma unroll
             for (int b = 0; b < B; ++b) e[b] = k0 + b < cnt ? list[(size_t)(k0 + b) * ldl + i] : (uint32_t)i | (kZeroImage << kIndexBits);
 #pragma unroll
-            for (int b = 0; b < B; ++b) {
-                xj[b] = x[e[b] & kIndexMask];
-                dFj[b] = dF[e[b] & kIndexMask];
-            }
+            for (int b = 0; b < B; ++b) xj[b] = x[e[b] & kIndexMask];
 #pragma unroll
             for (int b = 0; b < B; ++b) {
-                g[b] = pair_geo(xi, ti, e[b], xj[b], P, bd);
+                g[b] = pair_geo(xi, ti, e[b], xj[b], rec_type(xj[b].w), P, bd);
                 g[b].in = g[b].in && (k0 + b < cnt);
+                dFj[b] = rec_dF(xj[b].w);
             }
 #pragma unroll
             for (int b = 0; b < B; ++b) {
-                const float4* c = P.tabB + (size_t)(g[b].in ? g[b].row : 0) * 3;
-                ca[b] = c[0];
-                cb[b] = c[1];
-                cp[b] = c[2];
+                if (ONE) {
+                    // one element: rho_{j->i} = rho_{i->j}; row = (rho c1..c3, rphi c0), (rphi c1..c3, -)
+                    const float4* c = P.tabC + (size_t)(g[b].in ? g[b].row : 0) * 2;
+                    const float4 u = c[0], v = c[1];
+                    ca[b] = make_float4(0.f, u.x, u.y, u.z);
+                    cb[b] = ca[b];
+                    cp[b] = make_float4(u.w, v.x, v.y, v.z);
+                } else {
+                    const float4* c = P.tabB + (size_t)(g[b].in ? g[b].row : 0) * 3;
+                    ca[b] = c[0];
+                    cb[b] = c[1];
+                    cp[b] = c[2];
+                }
             }
 #pragma unroll
             for (int b = 0; b < B; ++b) {
@@ -381,8 +474,26 @@ __global__ __launch_bounds__(kBlock) void k_eam_force(int n, const double4* __re
             }
         }
         f_out[i] = make_float4(fx, fy, fz, 0.f);
+        if (K.on) {
+            double4 vi = v[i];
+            const double m = K.mass[ti];
+            vi.x += (double)fx * K.dt2 / m;
+            vi.y += (double)fy * K.dt2 / m;
+            vi.z += (double)fz * K.dt2 / m;
+            v[i] = vi;
+            mvv[0] = vi.x * vi.x * m;
+            mvv[1] = vi.x * vi.y * m;
+            mvv[2] = vi.x * vi.z * m;
+            mvv[3] = vi.y * vi.y * m;
+            mvv[4] = vi.y * vi.z * m;
+            mvv[5] = vi.z * vi.z * m;
+        }
     }
     block_sum_store<7>(acc, part + (size_t)blk * kPartW);
+    if (K.on) {
+        __syncthreads();
+        block_sum_store<6>(mvv, part_mvv + (size_t)blk * kPartW);
+    }
 }
 
 // value(): rho and pair energy in one pass, then F(rho)
@@ -641,6 +752,18 @@ EamForceField::EamForceField(EamTables t) : t_(std::move(t))
                 B[(tt * nr + m) * 3 + 2] = f4(t_.rphi_at(a, b) + (size_t)m * 4);
             }
         }
+    // one element: the density pair is symmetric, so a row needs only the
+    // rho derivative coefficients and the four r*phi coefficients (32 B)
+    std::vector<float4> Cc(ne == 1 ? (size_t)nr * 2 : 1);
+    if (ne == 1)
+        for (int m = 0; m < nr; ++m) {
+            const double* r = t_.rho_at(0, 0) + (size_t)m * 4;
+            const double* p = t_.rphi_at(0, 0) + (size_t)m * 4;
+            Cc[(size_t)m * 2] = make_float4((float)r[1], (float)r[2], (float)r[3], (float)p[0]);
+            Cc[(size_t)m * 2 + 1] = make_float4((float)p[1], (float)p[2], (float)p[3], 0.f);
+        }
+    tabC_.reserve(Cc.size());
+    MAPP_HIP(hipMemcpy(tabC_.get(), Cc.data(), Cc.size() * sizeof(float4), hipMemcpyHostToDevice));
     std::vector<double> F((size_t)ne * nrho * 4);
     for (int a = 0; a < ne; ++a)
         for (int m = 0; m < nrho; ++m)
@@ -666,6 +789,7 @@ EamArgs EamForceField::args() const
     for (int k = 0; k < t_.ne * t_.ne; ++k) P.cut_sq[k] = t_.rc[k] * t_.rc[k];
     P.tabA = tabA_.get();
     P.tabB = tabB_.get();
+    P.tabC = tabC_.get();
     P.Ftab = Ftab_.get();
     return P;
 }
@@ -690,12 +814,13 @@ MDSystem::MDSystem(std::shared_ptr<EamForceField> ff, int natms, const double* H
     setup_grid();
     x_.reserve(n_); x_alt_.reserve(n_); s_.reserve(n_); x0_.reserve(n_);
     v_.reserve(n_); v_alt_.reserve(n_); id_.reserve(n_); id_alt_.reserve(n_); slot_of_.reserve(n_);
-    f_.reserve(n_); rho_.reserve(n_); dF_.reserve(n_);
+    f_.reserve(n_); rho_.reserve(n_); xf_.reserve(n_); rec_.reserve(n_);
     keys_.reserve(n_); keys_alt_.reserve(n_); vals_.reserve(n_); vals_alt_.reserve(n_); s_tmp_.reserve(n_);
     nn_.reserve(n_);
     flags_.reserve(4);
     part_.reserve((size_t)3 * grid_for(n_) * kPartW + 64);  // rho, force, integrator rows
     red_.reserve(64);
+    red2_.reserve(256 * kPartW);
     std::vector<double4> xh(n_);
     std::vector<int> idh(n_);
     for (int i = 0; i < n_; ++i) {
@@ -718,6 +843,15 @@ MDSystem::~MDSystem()
 }
 
 void MDSystem::sync() { MAPP_HIP(hipStreamSynchronize(stream_)); }
+
+// two-level fixed-order sum of partial rows: up to 256 blocks, then one
+void MDSystem::reduce_rows(const double* part, int nrows, int ncols, double* out)
+{
+    const int g = std::min(256, std::max(1, (nrows + kBlock - 1) / kBlock));
+    const int chunk = (nrows + g - 1) / g;
+    hipLaunchKernelGGL(k_reduce_rows, dim3(g), dim3(kBlock), 0, stream_, nrows, ncols, part, red2_.get(), chunk);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, stream_, g, ncols, red2_.get(), out, g);
+}
 
 void MDSystem::set_batch(int rho, int force)
 {
@@ -747,6 +881,25 @@ void MDSystem::setup_grid()
         ncells *= nc;
     }
     grid_.ncells = ncells;
+    // fixed-point frame covering the box's bounding box with a margin for
+    // atoms that drift between rebuilds
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int c = 0; c < 8; ++c) {
+        const double s[3] = {(double)(c & 1), (double)((c >> 1) & 1), (double)((c >> 2) & 1)};
+        for (int d = 0; d < 3; ++d) {
+            double v = 0.0;
+            for (int k = 0; k < 3; ++k) v += s[k] * box_.H[k][d];
+            lo[d] = std::min(lo[d], v);
+            hi[d] = std::max(hi[d], v);
+        }
+    }
+    const double margin = 4.0 * max_cut_ + 16.0;
+    double ext = 0.0;
+    for (int d = 0; d < 3; ++d) ext = std::max(ext, hi[d] - lo[d] + 2.0 * margin);
+    const double span = std::ldexp(1.0, (int)std::ceil(std::log2(ext)));
+    fx_.q = span / std::ldexp(1.0, 41);
+    fx_.inv_q = 1.0 / fx_.q;
+    for (int d = 0; d < 3; ++d) fx_.o[d] = lo[d] - margin;
     // Morton rank of every cell so that cell-sorted atoms are spatially compact
     std::vector<std::pair<uint32_t, int>> code(ncells);
     for (int c2 = 0, c = 0; c2 < grid_.nc[2]; ++c2)
@@ -843,30 +996,43 @@ void MDSystem::build_list()
     throw Error(ERR_STATE, "neighbor list capacity did not converge");
 }
 
-void MDSystem::force_passes(bool)
+void MDSystem::force_passes(const KickArgs* kick)
 {
     const int nb = grid_for(n_);
     const EamArgs P = ff_->args();
     BoxDev bd;
     std::memcpy(bd.h, box_.h, sizeof bd.h);
     std::memcpy(bd.b, box_.b, sizeof bd.b);
+    MAPP_HIP(hipMemsetAsync(flags_.get() + 2, 0, sizeof(int), stream_));
     prof_.begin(K_EAM_RHO, stream_);
+    hipLaunchKernelGGL(k_make_records, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), fx_, rec_.get(), flags_.get());
+    const int* bad = flags_.get() + 2;
     switch (batch_rho_) {
-        case 1: hipLaunchKernelGGL(k_eam_rho<1>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), dF_.get(), part_.get()); break;
-        case 4: hipLaunchKernelGGL(k_eam_rho<4>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), dF_.get(), part_.get()); break;
-        default: hipLaunchKernelGGL(k_eam_rho<8>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), dF_.get(), part_.get()); break;
+        case 1: hipLaunchKernelGGL(k_eam_rho<1>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), fx_, bad, nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), xf_.get(), part_.get()); break;
+        case 4: hipLaunchKernelGGL(k_eam_rho<4>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), fx_, bad, nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), xf_.get(), part_.get()); break;
+        default: hipLaunchKernelGGL(k_eam_rho<8>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), fx_, bad, nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), xf_.get(), part_.get()); break;
     }
     prof_.end(stream_);
     prof_.begin(K_EAM_FORCE, stream_);
     double* p2 = part_.get() + (size_t)nb * kPartW;
-    switch (batch_force_) {
-        case 1: hipLaunchKernelGGL(k_eam_force<1>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_, P, bd, dF_.get(), f_.get(), p2); break;
-        case 2: hipLaunchKernelGGL(k_eam_force<2>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_, P, bd, dF_.get(), f_.get(), p2); break;
-        case 8: hipLaunchKernelGGL(k_eam_force<8>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_, P, bd, dF_.get(), f_.get(), p2); break;
-        default: hipLaunchKernelGGL(k_eam_force<4>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_, P, bd, dF_.get(), f_.get(), p2); break;
+    KickArgs K{};
+    if (kick) {
+        K = *kick;
+        K.on = 1;
     }
+    double* p3 = part_.get() + (size_t)2 * nb * kPartW;
+#define MAPP_FORCE(B, ONE) hipLaunchKernelGGL((k_eam_force<B, ONE>), dim3(nb), dim3(kBlock), 0, stream_, n_, xf_.get(), \
+                                              nn_.get(), list_.get(), ldl_, P, bd, f_.get(), p2, K, v_.get(), p3)
+    const bool one = P.ne == 1;
+    switch (batch_force_) {
+        case 1: if (one) MAPP_FORCE(1, true); else MAPP_FORCE(1, false); break;
+        case 2: if (one) MAPP_FORCE(2, true); else MAPP_FORCE(2, false); break;
+        case 8: if (one) MAPP_FORCE(8, true); else MAPP_FORCE(8, false); break;
+        default: if (one) MAPP_FORCE(4, true); else MAPP_FORCE(4, false); break;
+    }
+#undef MAPP_FORCE
     prof_.end(stream_);
-    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, stream_, 2 * nb, 7, part_.get(), red_.get());
+    reduce_rows(part_.get(), 2 * nb, 7, red_.get());
 }
 
 MDReport MDSystem::reduce_report(int)
@@ -879,7 +1045,7 @@ MDReport MDSystem::reduce_report(int)
 
 MDReport MDSystem::force_calc()
 {
-    force_passes(true);
+    force_passes(nullptr);
     MDReport r = reduce_report(7);
     prof_.flush();
     return r;
@@ -896,7 +1062,7 @@ double MDSystem::value()
     hipLaunchKernelGGL(k_eam_energy, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_,
                        P, bd, part_.get());
     prof_.end(stream_);
-    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, stream_, nb, 1, part_.get(), red_.get());
+    reduce_rows(part_.get(), nb, 1, red_.get());
     double e = 0.0;
     MAPP_HIP(hipMemcpyAsync(&e, red_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
     MAPP_HIP(hipStreamSynchronize(stream_));
@@ -1083,21 +1249,22 @@ int MDSystem::nvt_run(const double* masses, double kB, double T, double dt, int 
     double* mvv = red_.get() + 8;
     MAPP_HIP(hipMemsetAsync(nh, 0, sizeof(double) * NH_SIZE, stream_));
     const int rebuilds0 = nrebuild_;
-    force_passes(true);
+    force_passes(nullptr);
     hipLaunchKernelGGL(k_kick2, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), v_.get(), f_.get(), A, 0,
                        part_.get() + (size_t)2 * nb * kPartW);
-    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, stream_, nb, 6, part_.get() + (size_t)2 * nb * kPartW,
-                       mvv);
+    reduce_rows(part_.get() + (size_t)2 * nb * kPartW, nb, 6, mvv);
     hipLaunchKernelGGL(k_nhc, dim3(1), dim3(64), 0, stream_, nh, mvv, A, 0);
     const double skin_sq4 = 0.25 * skin_ * skin_;
+    KickArgs kick{};
+    kick.dt2 = A.dt2;
+    for (int a = 0; a < t.ne; ++a) kick.mass[a] = A.mass[a];
     int flag = 0;
     for (int step = 0; step < nsteps; ++step) {
         prof_.begin(K_INTEGRATE, stream_);
         hipLaunchKernelGGL(k_nhc, dim3(1), dim3(64), 0, stream_, nh, mvv, A, 1);
         hipLaunchKernelGGL(k_kick_drift, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), v_.get(), f_.get(), A, nh,
                            part_.get() + (size_t)2 * nb * kPartW);
-        hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, stream_, nb, 3,
-                           part_.get() + (size_t)2 * nb * kPartW, com);
+        reduce_rows(part_.get() + (size_t)2 * nb * kPartW, nb, 3, com);
         MAPP_HIP(hipMemsetAsync(flags_.get() + 1, 0, sizeof(int), stream_));
         hipLaunchKernelGGL(k_shift_decide, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), x0_.get(), com,
                            1.0 / n_, skin_sq4, flags_.get());
@@ -1105,12 +1272,9 @@ int MDSystem::nvt_run(const double* masses, double kB, double T, double dt, int 
         MAPP_HIP(hipMemcpyAsync(&flag, flags_.get() + 1, sizeof(int), hipMemcpyDeviceToHost, stream_));
         MAPP_HIP(hipStreamSynchronize(stream_));
         if (flag) rebuild();
-        force_passes(true);
+        force_passes(&kick);
         prof_.begin(K_INTEGRATE, stream_);
-        hipLaunchKernelGGL(k_kick2, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), v_.get(), f_.get(), A, 1,
-                           part_.get() + (size_t)2 * nb * kPartW);
-        hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, stream_, nb, 6,
-                           part_.get() + (size_t)2 * nb * kPartW, mvv);
+        reduce_rows(part_.get() + (size_t)2 * nb * kPartW, nb, 6, mvv);
         hipLaunchKernelGGL(k_nhc, dim3(1), dim3(64), 0, stream_, nh, mvv, A, 2);
         prof_.end(stream_);
     }

@@ -22,6 +22,7 @@ struct EamArgs {
     double cut_sq[kMaxElems * kMaxElems];
     const float4* tabA;   // [ne*ne][nr]: density at i due to j (rho[tj][ti])
     const float4* tabB;   // [ne*ne][nr][3]: rho[tj][ti], rho[ti][tj], r*phi[ti][tj]
+    const float4* tabC;   // one element only: [nr][2] = rho c1..c3, r*phi c0 | r*phi c1..c3
     const double* Ftab;   // [ne][nrho][4]
 };
 
@@ -34,8 +35,14 @@ public:
     EamArgs args() const;
 private:
     EamTables t_;
-    DevBuf<float4> tabA_, tabB_;
+    DevBuf<float4> tabA_, tabB_, tabC_;
     DevBuf<double> Ftab_;
+};
+
+// Frame of the fixed-point position records: x = o + c q per coordinate.
+struct FxFrame {
+    double o[3];
+    double q, inv_q;
 };
 
 // Box, cell grid and search radius of one neighbor build.
@@ -44,6 +51,13 @@ struct CellGrid {
     int R[3];
     int ncells;
     double cut_s[3];
+};
+
+// second velocity-Verlet half kick carried out by the force pass
+struct KickArgs {
+    double dt2;
+    double mass[kMaxElems];
+    int on;
 };
 
 struct MDReport {
@@ -88,8 +102,9 @@ public:
 private:
     void setup_grid();
     void build_list();
-    void force_passes(bool energy_virial);
+    void force_passes(const KickArgs* kick);
     MDReport reduce_report(int nparts);
+    void reduce_rows(const double* part, int nrows, int ncols, double* out);
     bool decide();
 
     std::shared_ptr<EamForceField> ff_;
@@ -97,6 +112,7 @@ private:
     double skin_, max_cut_;
     Box box_;
     CellGrid grid_{};
+    FxFrame fx_{};
     hipStream_t stream_ = nullptr;
     int nrebuild_ = 0;
 
@@ -108,7 +124,8 @@ private:
     DevBuf<int> slot_of_;            // caller index -> slot
     DevBuf<float4> f_;               // forces (w unused)
     DevBuf<double> rho_;
-    DevBuf<float> dF_;
+    DevBuf<uint4> rec_;              // fixed-point position records (density pass)
+    DevBuf<double4> xf_;             // pass-2 records: x, y, z, (type, F'(rho))
     DevBuf<uint32_t> keys_, keys_alt_;
     DevBuf<int> vals_, vals_alt_;
     DevBuf<double4> s_tmp_;
@@ -122,6 +139,7 @@ private:
     DevBuf<int> flags_;              // [0] overflow max count, [1] rebuild needed
     DevBuf<double> part_;            // per-block partial sums
     DevBuf<double> red_;             // reduced values
+    DevBuf<double> red2_;            // first-level reduction rows
     std::vector<double> masses_;
     Profiler prof_;
     int batch_rho_ = 8, batch_force_ = 8;
