@@ -107,6 +107,56 @@ mapp_status mapp_md_set_batch(mapp_md* md, int rho_batch, int force_batch);
 mapp_status mapp_create_T(int natms, const uint8_t* elem, const double* masses, double kB, double T, int seed,
                           double* v);
 
+/* ---------------- DMD (Gaussian-smeared EAM) ----------------
+ * Replaces AtomsDMD + DynamicDMD::init (src/dynamic_dmd.cpp:34-57) with
+ * ForceFieldEAMDMD (src/ff_eam_dmd.cpp:17-106). ff must be built with dmd=1.
+ * Sites carry c_dim components: elem/alpha/c are [nsites][c_dim]; c < 0
+ * marks an absent component (src/atoms_dmd.cpp:20-24). nquad Hermite points
+ * (src/atoms_dmd.cpp:19). masses[ne] enter mu_0 = 1.5 kT (2 ln lambda - 1 -
+ * ln pi), lambda = hP / sqrt(2 pi m kT) (src/ff_dmd.cpp:166-168). */
+typedef struct mapp_dmd mapp_dmd;
+mapp_status mapp_dmd_create(mapp_eam* ff, int nsites, int c_dim, int nquad, const double* H, const double* x,
+                            const uint8_t* elem, const double* alpha, const double* c, double skin, double kB,
+                            double T, double hP, const double* masses, mapp_dmd** out);
+void mapp_dmd_destroy(mapp_dmd* d);
+mapp_status mapp_dmd_set_state(mapp_dmd* d, const double* x, const double* alpha, const double* c);
+mapp_status mapp_dmd_get_state(mapp_dmd* d, double* x, double* alpha, double* c);
+/* DynamicDMD::update (decide at src/dynamic_dmd.cpp:103-126) */
+mapp_status mapp_dmd_update(mapp_dmd* d, int* rebuilt);
+/* ForceFieldDMD::new_derivative<with_c> (src/ff_dmd.cpp:285-366) over
+ * ForceFieldEAMDMD::__force_calc (src/ff_eam_dmd.cpp:126-360).
+ * out8 = free energy, F_H Voigt[6], entropy sum S. f [nsites][3],
+ * f_alpha / mu [nsites][c_dim] (any may be NULL). With with_c == 0, mu is
+ * the raw f_c vector of __force_calc; with with_c != 0 it is the chemical
+ * potential and ext_mu[ne] enters the free energy. */
+mapp_status mapp_dmd_derivative(mapp_dmd* d, int with_c, const double* ext_mu, double* f, double* f_alpha,
+                                double* mu, double* out8);
+/* ForceFieldDMD::new_value<with_c> (src/ff_dmd.cpp:252-281) over
+ * ForceFieldEAMDMD::__energy_calc (src/ff_eam_dmd.cpp:835-953) */
+mapp_status mapp_dmd_value(mapp_dmd* d, int with_c, const double* ext_mu, double* fe);
+/* NeighborDMD::create_list half list (src/neighbor_dmd.cpp:102-194) as
+ * (component i, component j, image) triples in caller indices */
+mapp_status mapp_dmd_pairs_count(mapp_dmd* d, int64_t* npairs);
+mapp_status mapp_dmd_pairs(mapp_dmd* d, int32_t* i, int32_t* j, int8_t* img);
+/* Smeared pair quantities at npts (r, alpha_ij) points for elements (ei, ej):
+ * out[npts][9] = rho_phi[3], drho_phi_dr[3], drho_phi_dalpha[3] in the order
+ * r*phi[ei][ej], r*rho[ei][ej], r*rho[ej][ei] (src/ff_eam_dmd.cpp:195-253). */
+mapp_status mapp_dmd_smear(mapp_dmd* d, int ei, int ej, int npts, const double* r, const double* alpha,
+                           double* out);
+/* MinCGDMD over x and alpha with c fixed (src/min_cg_dmd.h:633-700) and the
+ * backtracking line search (src/ls.h:711-770). Stops on an energy decrement
+ * below e_tol (the reference's criterion) or when max |gradient component|
+ * < grad_tol (grad_tol <= 0 disables). out4 = fe0, fe, |grad|_2, |grad|_max;
+ * status: 1 zero gradient, 2 not downhill, 3 grad_tol reached, 4 line search
+ * failed, 5 e_tol reached, 0 iteration limit. */
+mapp_status mapp_dmd_min_cg(mapp_dmd* d, int max_iter, double e_tol, double max_dx, double max_dalpha,
+                            double grad_tol, int* iters, int* status, double* out4);
+mapp_status mapp_dmd_profile(mapp_dmd* d, int enable);
+mapp_status mapp_dmd_profile_read(mapp_dmd* d, const char* kernel, double* total_ms, int64_t* launches);
+/* XMath::quadrature_hg (src/xmath.cpp:440-589): n-point physicists'
+ * Gauss-Hermite nodes and weights, sum w = sqrt(pi) */
+mapp_status mapp_gauss_hermite(int n, double* x, double* w);
+
 /* Diagnostic (no counterpart in the reference): copies `bytes` between two
  * device buffers `reps` times with a plain streaming kernel and reports the
  * achieved bandwidth (read + write bytes per second / 1e9). Used to calibrate

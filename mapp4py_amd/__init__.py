@@ -17,7 +17,7 @@ import numpy as np
 
 from ._lib import MappError, check, f64, i8p, i32p, i64p, ip, lib, ptr, u8p
 
-__all__ = ["ForceFieldEAM", "AtomsMD", "NVT", "MappError", "create_temp", "lib"]
+__all__ = ["ForceFieldEAM", "AtomsMD", "AtomsDMD", "NVT", "MappError", "create_temp", "gauss_hermite", "lib"]
 
 
 class ForceFieldEAM:
@@ -198,3 +198,106 @@ def create_temp(elem, masses, T, seed, kB=8.617330350e-5):
     v = np.zeros((elem.shape[0], 3))
     check(lib().mapp_create_T(elem.shape[0], ptr(elem, u8p), ptr(masses), float(kB), float(T), int(seed), ptr(v)))
     return v
+
+
+class AtomsDMD:
+    """dmd.atoms with ForceFieldEAMDMD, NeighborDMD and DynamicDMD on the GPU.
+
+    elem / alpha / c are [nsites][c_dim]; c < 0 marks an absent component.
+    """
+
+    def __init__(self, ff: ForceFieldEAM, H, x, elem, alpha, c, nquad=3, skin=0.5, T=300.0,
+                 kB=8.617330350e-5, hP=None, masses=None):
+        from .potentials import HP
+
+        self.ff = ff
+        x = f64(x).reshape(-1, 3)
+        self.nsites = x.shape[0]
+        elem = np.ascontiguousarray(elem, dtype=np.uint8).reshape(self.nsites, -1)
+        self.c_dim = elem.shape[1]
+        alpha = f64(alpha).reshape(self.nsites, self.c_dim)
+        c = f64(c).reshape(self.nsites, self.c_dim)
+        self.H = f64(H).reshape(3, 3)
+        self.masses = f64(masses)
+        self.kB, self.T = float(kB), float(T)
+        h = C.c_void_p()
+        check(lib().mapp_dmd_create(ff._h, self.nsites, self.c_dim, int(nquad), ptr(self.H), ptr(x), ptr(elem, u8p),
+                                    ptr(alpha), ptr(c), float(skin), self.kB, self.T,
+                                    float(HP if hP is None else hP), ptr(self.masses), C.byref(h)))
+        self._h = h
+
+    def state(self):
+        x = np.zeros((self.nsites, 3))
+        a = np.zeros((self.nsites, self.c_dim))
+        c = np.zeros((self.nsites, self.c_dim))
+        check(lib().mapp_dmd_get_state(self._h, ptr(x), ptr(a), ptr(c)))
+        return x, a, c
+
+    def set_state(self, x, alpha, c):
+        x, alpha, c = f64(x), f64(alpha), f64(c)
+        check(lib().mapp_dmd_set_state(self._h, ptr(x), ptr(alpha), ptr(c)))
+
+    def update(self):
+        r = C.c_int()
+        check(lib().mapp_dmd_update(self._h, C.byref(r)))
+        return bool(r.value)
+
+    def derivative(self, with_c=False, ext_mu=None):
+        """Returns (f, f_alpha, mu, fe, F_H, S)."""
+        f = np.zeros((self.nsites, 3))
+        fa = np.zeros((self.nsites, self.c_dim))
+        mu = np.zeros((self.nsites, self.c_dim))
+        out = np.zeros(8)
+        em = None if ext_mu is None else f64(ext_mu)
+        check(lib().mapp_dmd_derivative(self._h, int(with_c), ptr(em) if em is not None else None, ptr(f), ptr(fa),
+                                        ptr(mu), ptr(out)))
+        return f, fa, mu, out[0], out[1:7].copy(), out[7]
+
+    def value(self, with_c=False, ext_mu=None):
+        e = C.c_double()
+        em = None if ext_mu is None else f64(ext_mu)
+        check(lib().mapp_dmd_value(self._h, int(with_c), ptr(em) if em is not None else None, C.byref(e)))
+        return e.value
+
+    def half_pairs(self):
+        n = C.c_int64()
+        check(lib().mapp_dmd_pairs_count(self._h, C.byref(n)))
+        i = np.zeros(n.value, np.int32)
+        j = np.zeros(n.value, np.int32)
+        img = np.zeros((n.value, 3), np.int8)
+        check(lib().mapp_dmd_pairs(self._h, ptr(i, i32p), ptr(j, i32p), ptr(img, i8p)))
+        return i, j, img
+
+    def smear(self, ei, ej, r, a):
+        r, a = f64(r), f64(a)
+        out = np.zeros((len(r), 9))
+        check(lib().mapp_dmd_smear(self._h, int(ei), int(ej), len(r), ptr(r), ptr(a), ptr(out)))
+        return out
+
+    def min_cg(self, max_iter=1000, e_tol=1e-8, max_dx=1.0, max_dalpha=0.1, grad_tol=0.0):
+        it, st = C.c_int(), C.c_int()
+        out = np.zeros(4)
+        check(lib().mapp_dmd_min_cg(self._h, int(max_iter), float(e_tol), float(max_dx), float(max_dalpha),
+                                    float(grad_tol), C.byref(it), C.byref(st), ptr(out)))
+        return {"iters": it.value, "status": st.value, "fe0": out[0], "fe": out[1], "grad_norm": out[2],
+                "grad_max": out[3]}
+
+    def profile(self, enable=True):
+        check(lib().mapp_dmd_profile(self._h, int(enable)))
+
+    def profile_read(self, kernel):
+        ms, n = C.c_double(), C.c_int64()
+        check(lib().mapp_dmd_profile_read(self._h, kernel.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mapp_dmd_destroy(self._h)
+            self._h = None
+
+
+def gauss_hermite(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    check(lib().mapp_gauss_hermite(int(n), ptr(x), ptr(w)))
+    return x, w

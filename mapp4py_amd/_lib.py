@@ -53,6 +53,21 @@ SIGNATURES = {
     "mapp_md_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "mapp_md_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, dp, i64p]),
     "mapp_md_set_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "mapp_dmd_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, dp, u8p, dp, dp, C.c_double,
+                                  C.c_double, C.c_double, C.c_double, dp, vpp]),
+    "mapp_dmd_destroy": (None, [C.c_void_p]),
+    "mapp_dmd_set_state": (C.c_int, [C.c_void_p, dp, dp, dp]),
+    "mapp_dmd_get_state": (C.c_int, [C.c_void_p, dp, dp, dp]),
+    "mapp_dmd_update": (C.c_int, [C.c_void_p, ip]),
+    "mapp_dmd_derivative": (C.c_int, [C.c_void_p, C.c_int, dp, dp, dp, dp, dp]),
+    "mapp_dmd_value": (C.c_int, [C.c_void_p, C.c_int, dp, dp]),
+    "mapp_dmd_pairs_count": (C.c_int, [C.c_void_p, i64p]),
+    "mapp_dmd_pairs": (C.c_int, [C.c_void_p, i32p, i32p, i8p]),
+    "mapp_dmd_smear": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, dp, dp]),
+    "mapp_dmd_min_cg": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, ip, ip, dp]),
+    "mapp_dmd_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "mapp_dmd_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, dp, i64p]),
+    "mapp_gauss_hermite": (C.c_int, [C.c_int, dp, dp]),
     "mapp_diag_copy": (C.c_int, [C.c_size_t, C.c_int, dp]),
     "mapp_create_T": (C.c_int, [C.c_int, u8p, dp, C.c_double, C.c_double, C.c_int, dp]),
 }

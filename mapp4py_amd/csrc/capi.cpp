@@ -5,6 +5,7 @@
 #include <string>
 
 #include "../../include/mapp_amd.h"
+#include "dmd.hpp"
 #include "md.hpp"
 #include "rng.hpp"
 
@@ -20,6 +21,9 @@ struct mapp_eam {
 };
 struct mapp_md {
     std::unique_ptr<mapp::MDSystem> sys;
+};
+struct mapp_dmd {
+    std::unique_ptr<mapp::DMDSystem> sys;
 };
 
 namespace {
@@ -250,6 +254,127 @@ mapp_status mapp_md_profile_read(mapp_md* md, const char* kernel, double* total_
 mapp_status mapp_md_set_batch(mapp_md* md, int rho_batch, int force_batch)
 {
     return guard([&] { NEED(md); md->sys->set_batch(rho_batch, force_batch); });
+}
+
+mapp_status mapp_dmd_create(mapp_eam* ff, int nsites, int c_dim, int nquad, const double* H, const double* x,
+                            const uint8_t* elem, const double* alpha, const double* c, double skin, double kB,
+                            double T, double hP, const double* masses, mapp_dmd** out)
+{
+    return guard([&] {
+        NEED(ff); NEED(H); NEED(x); NEED(elem); NEED(alpha); NEED(c); NEED(masses); NEED(out);
+        *out = new mapp_dmd{std::make_unique<mapp::DMDSystem>(ff->ff, nsites, c_dim, nquad, H, x, elem, alpha, c,
+                                                              skin, kB, T, hP, masses)};
+    });
+}
+
+void mapp_dmd_destroy(mapp_dmd* d) { delete d; }
+
+mapp_status mapp_dmd_set_state(mapp_dmd* d, const double* x, const double* alpha, const double* c)
+{
+    return guard([&] { NEED(d); NEED(x); NEED(alpha); NEED(c); d->sys->set_state(x, alpha, c); });
+}
+
+mapp_status mapp_dmd_get_state(mapp_dmd* d, double* x, double* alpha, double* c)
+{
+    return guard([&] { NEED(d); d->sys->get_state(x, alpha, c); });
+}
+
+mapp_status mapp_dmd_update(mapp_dmd* d, int* rebuilt)
+{
+    return guard([&] {
+        NEED(d);
+        const bool r = d->sys->update();
+        if (rebuilt) *rebuilt = r ? 1 : 0;
+    });
+}
+
+mapp_status mapp_dmd_derivative(mapp_dmd* d, int with_c, const double* ext_mu, double* f, double* f_alpha,
+                                double* mu, double* out8)
+{
+    return guard([&] {
+        NEED(d);
+        const mapp::DMDReport r = d->sys->derivative(with_c != 0, ext_mu);
+        if (out8) {
+            out8[0] = r.fe;
+            for (int k = 0; k < 6; ++k) out8[1 + k] = r.FH[k];
+            out8[7] = r.s;
+        }
+        if (f || f_alpha || mu) d->sys->get_forces(f, f_alpha, mu);
+    });
+}
+
+mapp_status mapp_dmd_value(mapp_dmd* d, int with_c, const double* ext_mu, double* fe)
+{
+    return guard([&] { NEED(d); NEED(fe); *fe = d->sys->value(with_c != 0, ext_mu); });
+}
+
+mapp_status mapp_dmd_pairs_count(mapp_dmd* d, int64_t* npairs)
+{
+    return guard([&] { NEED(d); NEED(npairs); *npairs = d->sys->half_pairs_count(); });
+}
+
+mapp_status mapp_dmd_pairs(mapp_dmd* d, int32_t* i, int32_t* j, int8_t* img)
+{
+    return guard([&] { NEED(d); NEED(i); NEED(j); NEED(img); d->sys->half_pairs(i, j, img); });
+}
+
+mapp_status mapp_dmd_smear(mapp_dmd* d, int ei, int ej, int npts, const double* r, const double* alpha,
+                           double* out)
+{
+    return guard([&] { NEED(d); NEED(r); NEED(alpha); NEED(out); d->sys->smear(ei, ej, npts, r, alpha, out); });
+}
+
+mapp_status mapp_dmd_min_cg(mapp_dmd* d, int max_iter, double e_tol, double max_dx, double max_dalpha,
+                            double grad_tol, int* iters, int* status, double* out4)
+{
+    return guard([&] {
+        NEED(d);
+        const mapp::CGReport r = d->sys->cg(max_iter, e_tol, max_dx, max_dalpha, grad_tol);
+        if (iters) *iters = r.iters;
+        if (status) *status = r.status;
+        if (out4) {
+            out4[0] = r.fe0;
+            out4[1] = r.fe;
+            out4[2] = r.grad_norm;
+            out4[3] = r.grad_max;
+        }
+    });
+}
+
+mapp_status mapp_dmd_profile(mapp_dmd* d, int enable)
+{
+    return guard([&] {
+        NEED(d);
+        auto& p = d->sys->profiler();
+        p.reset();
+        p.on = enable != 0;
+    });
+}
+
+mapp_status mapp_dmd_profile_read(mapp_dmd* d, const char* kernel, double* total_ms, int64_t* launches)
+{
+    return guard([&] {
+        NEED(d);
+        NEED(kernel);
+        auto& p = d->sys->profiler();
+        p.flush();
+        for (int k = 0; k < mapp::K_COUNT; ++k)
+            if (std::strcmp(kernel, mapp::kernel_name(k)) == 0) {
+                if (total_ms) *total_ms = p.total_ms[k];
+                if (launches) *launches = p.launches[k];
+                return;
+            }
+        throw mapp::Error(mapp::ERR_ARG, std::string("unknown kernel group ") + kernel);
+    });
+}
+
+mapp_status mapp_gauss_hermite(int n, double* x, double* w)
+{
+    return guard([&] {
+        NEED(x); NEED(w);
+        MAPP_REQUIRE(n >= 1 && n <= mapp::kMaxQuad, mapp::ERR_ARG, "1 to 16 points");
+        mapp::gauss_hermite(n, x, w);
+    });
 }
 
 mapp_status mapp_create_T(int natms, const uint8_t* elem, const double* masses, double kB, double T, int seed,

@@ -1,0 +1,1131 @@
+// dmd.hip -- diffusive MD on gfx950: per-component neighbor list with the
+// alpha-dependent cutoff, Gauss-Hermite smeared EAM in two passes (density /
+// free energy, then forces on x and alpha and the chemical-potential sums),
+// the skin check, and the conjugate-gradient free-energy minimiser.
+//
+// One thread per site loops over the site's components (c_dim <= 4), so the
+// site force is accumulated in registers with no atomics. The smeared
+// functions use FP64 tables and FP64 moments: the CG driver converges the
+// gradient to 1e-6, below the noise floor FP32 spline values would leave.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "dmd.hpp"
+#include "kern.hpp"
+#include "surrogate.hpp"
+
+namespace mapp {
+
+namespace {
+
+constexpr double kPiInSq = 0.564189583547756286948079451561;  // 1/sqrt(pi)
+
+struct DGrid {
+    int nc[3], R[3];
+    int ne, C;
+    double scl;
+    double cut_sk[kMaxElems * kMaxElems];  // r_c + skin (not squared: the DMD test is on r)
+};
+
+__device__ inline int dcell(double s, int nc)
+{
+    int c = (int)(s * (double)nc);
+    return c < nc - 1 ? c : nc - 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dcell_keys(int n, const double4* __restrict__ x, BoxDev bd, DGrid g,
+                                                       const int* __restrict__ cell_rank, double4* __restrict__ s_tmp,
+                                                       uint32_t* __restrict__ keys, int* __restrict__ vals)
+{
+    const int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double4 xi = x[i];
+    double s0, s1, s2;
+    x_to_s(bd.b, xi.x, xi.y, xi.z, s0, s1, s2);
+    s_tmp[i] = make_double4(s0, s1, s2, 0.0);
+    const int c = dcell(s0, g.nc[0]) + g.nc[0] * (dcell(s1, g.nc[1]) + g.nc[1] * dcell(s2, g.nc[2]));
+    keys[i] = (uint32_t)cell_rank[c];
+    vals[i] = i;
+}
+
+// sites into cell order; x re-expressed from the wrapped s (src/cell.h:208)
+__global__ __launch_bounds__(kBlock) void k_dpermute(int n, int C, const int* __restrict__ perm,
+                                                     const double4* __restrict__ s_tmp, BoxDev bd,
+                                                     const double* __restrict__ al_old, const double* __restrict__ c_old,
+                                                     const int* __restrict__ el_old, const int* __restrict__ id_old,
+                                                     double4* __restrict__ x_new, double4* __restrict__ s4,
+                                                     double4* __restrict__ x0, double* __restrict__ al_new,
+                                                     double* __restrict__ c_new, int* __restrict__ el_new,
+                                                     double* __restrict__ al0, int* __restrict__ id_new,
+                                                     int* __restrict__ slot_of)
+{
+    const int p = xcd_block() * kBlock + threadIdx.x;
+    if (p >= n) return;
+    const int o = perm[p];
+    const double4 s = s_tmp[o];
+    double a0, a1, a2;
+    s_to_x(bd.h, s.x, s.y, s.z, a0, a1, a2);
+    const double4 xn = make_double4(a0, a1, a2, 0.0);
+    x_new[p] = xn;
+    x0[p] = xn;
+    s4[p] = s;
+    for (int k = 0; k < C; ++k) {
+        al_new[p * C + k] = al_old[o * C + k];
+        al0[p * C + k] = al_old[o * C + k];
+        c_new[p * C + k] = c_old[o * C + k];
+        el_new[p * C + k] = el_old[o * C + k];
+    }
+    const int id = id_old[o];
+    id_new[p] = id;
+    slot_of[id] = p;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dcell_start(int n, int ncells, const uint32_t* __restrict__ keys,
+                                                        int* __restrict__ cell_start)
+{
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r > ncells) return;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid] < (uint32_t)r) lo = mid + 1; else hi = mid;
+    }
+    cell_start[r] = lo;
+}
+
+__device__ inline double dsqrt_exact(double v)
+{
+#pragma clang fp contract(off)
+    return sqrt(v);
+}
+
+// Per-component full list (src/neighbor_dmd.cpp:102-194 predicate): for
+// components ic of site i and jc of the image of site j, both present and
+// preceded only by present components, keep j*C+jc when
+// r - x_max sqrt(alpha_i^2 + alpha_j^2) < r_c + skin.
+__global__ __launch_bounds__(kBlock) void k_dbuild_list(int n, const double4* __restrict__ s4, BoxDev bd, DGrid g,
+                                                        const double* __restrict__ al, const double* __restrict__ cc,
+                                                        const int* __restrict__ el, const int* __restrict__ cell_rank,
+                                                        const int* __restrict__ cell_start, int maxn, int ldl,
+                                                        uint32_t* __restrict__ list, int* __restrict__ nn,
+                                                        int* __restrict__ flags)
+{
+#pragma clang fp contract(off)
+    const int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int C = g.C;
+    const double4 si = s4[i];
+    double xi0, xi1, xi2;
+    s_to_x(bd.h, si.x, si.y, si.z, xi0, xi1, xi2);
+    int npres_i = 0;
+    while (npres_i < C && cc[i * C + npres_i] >= 0.0) ++npres_i;
+    int cnt[kMaxComp] = {0, 0, 0, 0};
+    const int c0 = dcell(si.x, g.nc[0]), c1 = dcell(si.y, g.nc[1]), c2 = dcell(si.z, g.nc[2]);
+    if (npres_i > 0) {
+        for (int o2 = -g.R[2]; o2 <= g.R[2]; ++o2) {
+            const int u2 = c2 + o2, n2 = floordiv(u2, g.nc[2]), w2 = u2 - n2 * g.nc[2];
+            for (int o1 = -g.R[1]; o1 <= g.R[1]; ++o1) {
+                const int u1 = c1 + o1, n1 = floordiv(u1, g.nc[1]), w1 = u1 - n1 * g.nc[1];
+                for (int o0 = -g.R[0]; o0 <= g.R[0]; ++o0) {
+                    const int u0 = c0 + o0, n0 = floordiv(u0, g.nc[0]), w0 = u0 - n0 * g.nc[0];
+                    const int rank = cell_rank[w0 + g.nc[0] * (w1 + g.nc[1] * w2)];
+                    const int jb = cell_start[rank], je = cell_start[rank + 1];
+                    const bool self_img = (n0 | n1 | n2) == 0;
+                    const uint32_t code = (uint32_t)((n0 + 2) + 5 * (n1 + 2) + 25 * (n2 + 2)) << kIndexBits;
+                    for (int j = jb; j < je; ++j) {
+                        if (self_img && j == i) continue;
+                        const double4 sj = s4[j];
+                        double xj0, xj1, xj2;
+                        s_to_x(bd.h, shift_unit(sj.x, n0), shift_unit(sj.y, n1), shift_unit(sj.z, n2), xj0, xj1, xj2);
+                        const double r = dsqrt_exact(rsq_exact(xi0, xi1, xi2, xj0, xj1, xj2));
+                        for (int ic = 0; ic < npres_i; ++ic) {
+                            const double ai = al[i * C + ic];
+                            const int ei = el[i * C + ic];
+                            for (int jc = 0; jc < C && cc[j * C + jc] >= 0.0; ++jc) {
+                                const double aj = al[j * C + jc];
+                                const double aij = dsqrt_exact(ai * ai + aj * aj);
+                                if (r - g.scl * aij < g.cut_sk[ei * g.ne + el[j * C + jc]]) {
+                                    if (cnt[ic] < maxn)
+                                        list[(size_t)cnt[ic] * ldl + (i * C + ic)] = (uint32_t)(j * C + jc) | code;
+                                    ++cnt[ic];
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    int mx = 0;
+    for (int ic = 0; ic < C; ++ic) {
+        nn[i * C + ic] = cnt[ic];
+        mx = max(mx, cnt[ic]);
+    }
+    if (mx > maxn) atomicMax(&flags[0], mx);
+}
+
+__device__ inline void dpair_dx(const double4& xi, const double4& xj, uint32_t code, const double* __restrict__ h,
+                                double& dx, double& dy, double& dz)
+{
+    dx = xi.x - xj.x;
+    dy = xi.y - xj.y;
+    dz = xi.z - xj.z;
+    if (code != (uint32_t)kZeroImage) {
+        const int n2 = (int)(code / 25u) - 2;
+        const int rem = (int)(code % 25u);
+        const int n1 = rem / 5 - 2, n0 = rem % 5 - 2;
+        dx -= h[0] * n0 + h[1] * n1 + h[2] * n2;
+        dy -= h[3] * n1 + h[4] * n2;
+        dz -= h[5] * n2;
+    }
+}
+
+__device__ inline double hv(const double4& c, double p) { return ((c.w * p + c.z) * p + c.y) * p + c.x; }
+
+// Smeared r*f over the odd extension, N-point Gauss-Hermite
+// (src/ff_eam_dmd.cpp:195-253): rp = g, dr = (1/r) dg/dr, da = (1/a) dg/da.
+template <bool DERIV>
+__device__ inline void smear_quad(const DmdArgs& A, int tt, double r, double a, double (&rp)[3], double (&dr)[3],
+                                  double (&da)[3])
+{
+    double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int l = 0; l < A.N; ++l) {
+        const double xl = A.xi[l];
+        const double u = r - xl * a;
+        double p = fabs(u) * A.dr_inv;
+        int m = (int)p;
+        m = m < A.nr - 2 ? m : A.nr - 2;
+        p -= (double)m;
+        p = p < 1.0 ? p : 1.0;
+        const double4* row = A.tab + ((size_t)tt * A.nr + m) * 3;
+        const double sgn = u < 0.0 ? -A.wi[l] : A.wi[l];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double t = hv(row[k], p) * sgn;
+            H[k][0] += t;
+            if (DERIV) {
+                H[k][1] += t * xl;
+                H[k][2] += t * (xl * xl);
+            }
+        }
+    }
+    const double rinv = 1.0 / r;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        rp[k] = kPiInSq * rinv * H[k][0];
+        if (DERIV) {
+            const double ainv = 1.0 / a;
+            dr[k] = -(rinv * rinv) * (rp[k] + 2.0 * kPiInSq * ainv * H[k][1]);
+            da[k] = -(ainv * ainv) * (rp[k] - 2.0 * kPiInSq * rinv * H[k][2]);
+        }
+    }
+}
+
+__device__ inline void femb5(const DmdArgs& A, int e, double rho, double& F, double& dF)
+{
+    double p = rho * A.drho_inv;
+    int m = (int)p;
+    m = m < A.nrho - 2 ? m : A.nrho - 2;
+    m = m > 0 ? m : 0;
+    p -= (double)m;
+    p = p < 1.0 ? p : 1.0;
+    const double* c = A.F5 + ((size_t)e * A.nrho + m) * 5;
+    F = (((c[4] * p + c[3]) * p + c[2]) * p + c[1]) * p + c[0];
+    dF = (((4.0 * c[4] * p + 3.0 * c[3]) * p + 2.0 * c[2]) * p + c[1]) * A.drho_inv;
+    if (rho > A.rho_max) F += dF * (rho - A.rho_max);
+}
+
+__device__ inline double xlogx(double v) { return v == 0.0 ? 0.0 : v * log(v); }
+
+// pass 1: rho per component, F(rho), F'(rho); free energy and entropy sums.
+// energy_only mirrors __energy_calc (the c != 0 test on the site term).
+template <bool SURR>
+__global__ __launch_bounds__(kBlock) void k_dmd_pass1(int n, const double4* __restrict__ x, const double* __restrict__ al,
+                                                      const double* __restrict__ cc, const int* __restrict__ el,
+                                                      const int* __restrict__ nn, const uint32_t* __restrict__ list,
+                                                      int ldl, DmdArgs A, BoxDev bd, SurrogateArgs S, int energy_only,
+                                                      double* __restrict__ E, double* __restrict__ dE,
+                                                      double* __restrict__ part)
+{
+    const int blk = xcd_block();
+    const int i = blk * kBlock + threadIdx.x;
+    double acc[2] = {0.0, 0.0};  // free energy without the kT S term, S
+    if (i < n) {
+        const int C = A.C;
+        const double4 xi = x[i];
+        double cv = 1.0;
+        for (int ic = 0; ic < C; ++ic) {
+            const int ci_idx = i * C + ic;
+            const double ci = cc[ci_idx];
+            if (ci > 0.0) cv -= ci;
+            if (ci < 0.0) continue;
+            const double ai = al[ci_idx];
+            const int ei = el[ci_idx];
+            double rho = 0.0, ep = 0.0;
+            const int cnt = nn[ci_idx];
+            for (int k = 0; k < cnt; ++k) {
+                const uint32_t e = list[(size_t)k * ldl + ci_idx];
+                const int jc = (int)(e & kIndexMask);
+                const double4 xj = x[jc / C];
+                double dx, dy, dz;
+                dpair_dx(xi, xj, e >> kIndexBits, bd.h, dx, dy, dz);
+                const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                const double aj = al[jc];
+                const double a = sqrt(ai * ai + aj * aj);
+                const int ej = el[jc];
+                if (r - a * A.xmax >= A.cut[ei * A.ne + ej]) continue;
+                double rp[3], drr[3], dal[3];
+                if (!SURR || !surrogate_read<false>(S, (size_t)k * ldl + ci_idx, r, a, rp, drr, dal))
+                    smear_quad<false>(A, ei * A.ne + ej, r, a, rp, drr, dal);
+                rho += cc[jc] * rp[2];
+                ep += 0.5 * ci * cc[jc] * rp[0];
+            }
+            double F, dF;
+            femb5(A, ei, rho, F, dF);
+            E[ci_idx] = F;
+            dE[ci_idx] = dF;
+            if (!energy_only || ci != 0.0) acc[0] += ci * (F + A.mu0[ei] - 3.0 * A.kBT * log(ai));
+            acc[0] += ep;
+            acc[1] += xlogx(ci);
+        }
+        acc[1] += xlogx(cv);
+    }
+    block_sum_store<2>(acc, part + (size_t)blk * kPartW);
+}
+
+// pass 2: forces on x (site), on alpha and the chemical-potential sums per
+// component, virial; owner computes every pair from its own side
+template <bool SURR>
+__global__ __launch_bounds__(kBlock) void k_dmd_pass2(int n, const double4* __restrict__ x, const double* __restrict__ al,
+                                                      const double* __restrict__ cc, const int* __restrict__ el,
+                                                      const int* __restrict__ nn, const uint32_t* __restrict__ list,
+                                                      int ldl, DmdArgs A, BoxDev bd, SurrogateArgs S,
+                                                      const double* __restrict__ E, const double* __restrict__ dE,
+                                                      double4* __restrict__ f, double* __restrict__ fa,
+                                                      double* __restrict__ mu, double* __restrict__ part)
+{
+    const int blk = xcd_block();
+    const int i = blk * kBlock + threadIdx.x;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        const int C = A.C;
+        const double4 xi = x[i];
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int ic = 0; ic < C; ++ic) {
+            const int ci_idx = i * C + ic;
+            const double ci = cc[ci_idx];
+            if (ci < 0.0) {
+                fa[ci_idx] = 0.0;
+                mu[ci_idx] = 0.0;
+                continue;
+            }
+            const double ai = al[ci_idx];
+            const int ei = el[ci_idx];
+            const double dEi = dE[ci_idx];
+            double fai = 0.0, mui = 0.0;
+            const int cnt = nn[ci_idx];
+            for (int k = 0; k < cnt; ++k) {
+                const uint32_t e = list[(size_t)k * ldl + ci_idx];
+                const int jc = (int)(e & kIndexMask);
+                const double4 xj = x[jc / C];
+                double dx, dy, dz;
+                dpair_dx(xi, xj, e >> kIndexBits, bd.h, dx, dy, dz);
+                const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                const double aj = al[jc];
+                const double a = sqrt(ai * ai + aj * aj);
+                const int ej = el[jc];
+                if (r - a * A.xmax >= A.cut[ei * A.ne + ej]) continue;
+                double rp[3], drr[3], dal[3];
+                if (!SURR || !surrogate_read<true>(S, (size_t)k * ldl + ci_idx, r, a, rp, drr, dal))
+                    smear_quad<true>(A, ei * A.ne + ej, r, a, rp, drr, dal);
+                const double cj = cc[jc], dEj = dE[jc];
+                const double fpair = -(drr[2] * dEi + drr[1] * dEj + drr[0]) * ci * cj;
+                const double apair = -(dal[2] * dEi + dal[1] * dEj + dal[0]) * ci * cj;
+                mui += cj * (rp[0] + rp[1] * dEj);
+                fx += fpair * dx;
+                fy += fpair * dy;
+                fz += fpair * dz;
+                fai += ai * apair;
+                const double hf = 0.5 * fpair;
+                acc[0] += hf * dx * dx;
+                acc[1] += hf * dx * dy;
+                acc[2] += hf * dx * dz;
+                acc[3] += hf * dy * dy;
+                acc[4] += hf * dy * dz;
+                acc[5] += hf * dz * dz;
+            }
+            fa[ci_idx] = fai + 3.0 * A.kBT * ci / ai;
+            mu[ci_idx] = E[ci_idx] + mui;
+        }
+        f[i] = make_double4(fx, fy, fz, 0.0);
+    }
+    block_sum_store<6>(acc, part + (size_t)blk * kPartW);
+}
+
+// chemical potential of new_derivative<true> (src/ff_dmd.cpp:305-366) and
+// the sum of c * mu_ext
+__global__ __launch_bounds__(kBlock) void k_dmd_chem(int n, int C, const double* __restrict__ al,
+                                                     const double* __restrict__ cc, const int* __restrict__ el,
+                                                     double kBT, DmdArgs A, const double* __restrict__ ext_mu,
+                                                     double* __restrict__ mu, double* __restrict__ part)
+{
+    const int blk = xcd_block();
+    const int i = blk * kBlock + threadIdx.x;
+    double acc[1] = {0.0};
+    if (i < n) {
+        double cv = 1.0;
+        for (int k = 0; k < C; ++k)
+            if (cc[i * C + k] >= 0.0) cv -= cc[i * C + k];
+        const double logcv = cv != 0.0 ? log(cv) : log(2.220446049250313e-16);
+        for (int k = 0; k < C; ++k) {
+            const int q = i * C + k;
+            const double c = cc[q];
+            if (c < 0.0) continue;
+            const double logc = c != 0.0 ? log(c) : log(2.220446049250313e-16);
+            const int e = el[q];
+            mu[q] = -mu[q] - kBT * (logc - logcv - 3.0 * log(al[q])) - A.mu0[e] + ext_mu[e];
+            acc[0] += c * ext_mu[e];
+        }
+    }
+    block_sum_store<1>(acc, part + (size_t)blk * kPartW);
+}
+
+// DynamicDMD::decide (src/dynamic_dmd.cpp:103-126)
+__global__ __launch_bounds__(kBlock) void k_dmd_decide(int n, int C, const double4* __restrict__ x,
+                                                       const double4* __restrict__ x0, const double* __restrict__ al,
+                                                       const double* __restrict__ al0, double scale, double half_skin,
+                                                       int* __restrict__ flags)
+{
+    const int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double4 a = x0[i], b = x[i];
+    const double dr = sqrt(rsq_exact(a.x, a.y, a.z, b.x, b.y, b.z));
+    double da = al[i * C] - al0[i * C];
+    for (int k = 0; k < C; ++k) da = fmax(da, al[i * C + k] - al0[i * C + k]);
+    if (dr + da * scale > half_skin) atomicOr(&flags[1], 1);
+}
+
+// ---------------- CG vector kernels ----------------
+
+// partial dot products over the x part (slots) and the alpha part (components)
+__global__ __launch_bounds__(kBlock) void k_dot2(int n, int nc, const double4* __restrict__ ax,
+                                                 const double* __restrict__ aa, const double4* __restrict__ bx,
+                                                 const double* __restrict__ ba, double* __restrict__ part)
+{
+    const int blk = xcd_block();
+    const int i = blk * kBlock + threadIdx.x;
+    double acc[1] = {0.0};
+    const int stride = gridDim.x * kBlock;
+    for (int k = i; k < n; k += stride) {
+        const double4 u = ax[k], v = bx[k];
+        acc[0] += u.x * v.x + u.y * v.y + u.z * v.z;
+    }
+    for (int k = i; k < nc; k += stride) acc[0] += aa[k] * ba[k];
+    block_sum_store<1>(acc, part + (size_t)blk * kPartW);
+}
+
+// max |h_x|, max |h_alpha| and min over h_alpha<0 of -alpha/h_alpha
+__global__ __launch_bounds__(kBlock) void k_cg_limits(int n, int nc, const double4* __restrict__ hx,
+                                                      const double* __restrict__ ha, const double* __restrict__ al,
+                                                      const double* __restrict__ cc, double* __restrict__ out)
+{
+    __shared__ double sh[3][kBlock];
+    double mx = 0.0, ma = 0.0, mn = 1e300;
+    for (int k = threadIdx.x; k < n; k += kBlock) {
+        const double4 h = hx[k];
+        mx = fmax(mx, fmax(fabs(h.x), fmax(fabs(h.y), fabs(h.z))));
+    }
+    for (int k = threadIdx.x; k < nc; k += kBlock) {
+        if (cc[k] < 0.0) continue;
+        ma = fmax(ma, fabs(ha[k]));
+        if (ha[k] < 0.0) mn = fmin(mn, -al[k] / ha[k]);
+    }
+    sh[0][threadIdx.x] = mx;
+    sh[1][threadIdx.x] = ma;
+    sh[2][threadIdx.x] = mn;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sh[0][threadIdx.x] = fmax(sh[0][threadIdx.x], sh[0][threadIdx.x + s]);
+            sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + s]);
+            sh[2][threadIdx.x] = fmin(sh[2][threadIdx.x], sh[2][threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sh[0][0];
+        out[1] = sh[1][0];
+        out[2] = sh[2][0];
+    }
+}
+
+// x = x0 + t h (positions and alpha; absent components untouched)
+__global__ __launch_bounds__(kBlock) void k_cg_move(int n, int nc, double t, const double4* __restrict__ x0,
+                                                    const double4* __restrict__ hx, const double* __restrict__ a0,
+                                                    const double* __restrict__ ha, const double* __restrict__ cc,
+                                                    double4* __restrict__ x, double* __restrict__ al)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const double4 a = x0[i], h = hx[i];
+        x[i] = make_double4(a.x + t * h.x, a.y + t * h.y, a.z + t * h.z, 0.0);
+    }
+    if (i < nc && cc[i] >= 0.0) al[i] = a0[i] + t * ha[i];
+}
+
+// h = ratio h + f
+__global__ __launch_bounds__(kBlock) void k_cg_dir(int n, int nc, double ratio, const double4* __restrict__ fx,
+                                                   const double* __restrict__ fa, double4* __restrict__ hx,
+                                                   double* __restrict__ ha)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const double4 f = fx[i], h = hx[i];
+        hx[i] = make_double4(ratio * h.x + f.x, ratio * h.y + f.y, ratio * h.z + f.z, 0.0);
+    }
+    if (i < nc) ha[i] = ratio * ha[i] + fa[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_gmax(int n, int nc, const double4* __restrict__ fx,
+                                                 const double* __restrict__ fa, double* __restrict__ out)
+{
+    __shared__ double sh[kBlock];
+    double m = 0.0;
+    for (int k = threadIdx.x; k < n; k += kBlock) {
+        const double4 f = fx[k];
+        m = fmax(m, fmax(fabs(f.x), fmax(fabs(f.y), fabs(f.z))));
+    }
+    for (int k = threadIdx.x; k < nc; k += kBlock) m = fmax(m, fabs(fa[k]));
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// host <-> slot order
+__global__ __launch_bounds__(kBlock) void k_dscatter(int n, int C, const double* __restrict__ xh,
+                                                     const double* __restrict__ ah, const double* __restrict__ ch,
+                                                     const int* __restrict__ slot_of, double4* __restrict__ x,
+                                                     double* __restrict__ al, double* __restrict__ cc)
+{
+    const int id = blockIdx.x * kBlock + threadIdx.x;
+    if (id >= n) return;
+    const int s = slot_of[id];
+    x[s] = make_double4(xh[3 * (size_t)id], xh[3 * (size_t)id + 1], xh[3 * (size_t)id + 2], 0.0);
+    for (int k = 0; k < C; ++k) {
+        al[s * C + k] = ah[(size_t)id * C + k];
+        cc[s * C + k] = ch[(size_t)id * C + k];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dgather(int n, int C, const int* __restrict__ slot_of,
+                                                    const double4* __restrict__ x, const double* __restrict__ a,
+                                                    const double* __restrict__ b, double* __restrict__ xo,
+                                                    double* __restrict__ ao, double* __restrict__ bo)
+{
+    const int id = blockIdx.x * kBlock + threadIdx.x;
+    if (id >= n) return;
+    const int s = slot_of[id];
+    const double4 v = x[s];
+    xo[3 * (size_t)id] = v.x;
+    xo[3 * (size_t)id + 1] = v.y;
+    xo[3 * (size_t)id + 2] = v.z;
+    for (int k = 0; k < C; ++k) {
+        if (ao) ao[(size_t)id * C + k] = a[s * C + k];
+        if (bo) bo[(size_t)id * C + k] = b[s * C + k];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_smear_points(DmdArgs A, int tt, int npts, const double* __restrict__ r,
+                                                         const double* __restrict__ a, double* __restrict__ out)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= npts) return;
+    double rp[3], dr[3], da[3];
+    smear_quad<true>(A, tt, r[i], a[i], rp, dr, da);
+    for (int k = 0; k < 3; ++k) {
+        out[(size_t)i * 9 + k] = rp[k];
+        out[(size_t)i * 9 + 3 + k] = dr[k];
+        out[(size_t)i * 9 + 6 + k] = da[k];
+    }
+}
+
+}  // namespace
+
+// ---------------- DMDSystem ----------------
+
+// physicists' Gauss-Hermite nodes/weights by Newton on H_n(x) with the
+// three-term recurrence (Golub-Welsch is not needed for n <= 16)
+void gauss_hermite(int n, double* x, double* w)
+{
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < (n + 1) / 2; ++i) {
+        // initial guesses (Numerical Recipes style asymptotics)
+        double z;
+        if (i == 0) z = std::sqrt(2.0 * n + 1.0) - 1.85575 * std::pow(2.0 * n + 1.0, -0.16667);
+        else if (i == 1) z = x[n - 1] - 1.14 * std::pow((double)n, 0.426) / x[n - 1];
+        else if (i == 2) z = 1.86 * x[n - 2] - 0.86 * x[n - 1];
+        else if (i == 3) z = 1.91 * x[n - 3] - 0.91 * x[n - 2];
+        else z = 2.0 * x[n - i] - x[n - i + 1];
+        double pp = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            // orthonormal recurrence: p_k = sqrt(2/k) z p_{k-1} - sqrt((k-1)/k) p_{k-2}
+            double p1 = std::pow(pi, -0.25), p2 = 0.0;
+            for (int k = 1; k <= n; ++k) {
+                const double p3 = p2;
+                p2 = p1;
+                p1 = z * std::sqrt(2.0 / k) * p2 - std::sqrt((k - 1.0) / k) * p3;
+            }
+            pp = std::sqrt(2.0 * n) * p2;
+            const double z1 = z;
+            z = z1 - p1 / pp;
+            if (std::fabs(z - z1) <= 1e-15 * std::max(1.0, std::fabs(z))) break;
+        }
+        x[n - 1 - i] = z;
+        x[i] = -z;
+        w[n - 1 - i] = w[i] = 2.0 / (pp * pp);
+    }
+    if (n % 2 == 1) x[n / 2] = 0.0;
+}
+
+
+DMDSystem::DMDSystem(std::shared_ptr<EamForceField> ff, int nsites, int c_dim, int nquad, const double* H,
+                     const double* x, const uint8_t* elem, const double* alpha, const double* c, double skin,
+                     double kB, double T, double hP, const double* masses)
+    : ff_(std::move(ff)), n_(nsites), C_(c_dim), nc_(nsites * c_dim), skin_(skin), kB_(kB), T_(T)
+{
+    const EamTables& t = ff_->tables();
+    MAPP_REQUIRE(t.dmd, ERR_ARG, "the DMD system needs a force field built with dmd=1 (quartic F, r*rho tables)");
+    MAPP_REQUIRE(nsites >= 1 && c_dim >= 1 && c_dim <= kMaxComp, ERR_ARG, "1 to 4 components per site");
+    MAPP_REQUIRE(nquad >= 1 && nquad <= kMaxQuad, ERR_ARG, "1 to 16 Hermite points");
+    MAPP_REQUIRE((uint64_t)nsites * c_dim <= kIndexMask, ERR_CAPACITY, "too many components");
+    MAPP_REQUIRE(kB > 0.0 && T > 0.0 && hP > 0.0, ERR_ARG, "kB, T and hP must be positive");
+    for (int k = 0; k < nsites * c_dim; ++k) {
+        MAPP_REQUIRE(elem[k] < t.ne, ERR_ARG, "element index out of range");
+        MAPP_REQUIRE(c[k] < 0.0 || alpha[k] > 0.0, ERR_ARG, "present components need alpha > 0");
+    }
+    MAPP_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    box_ = make_box(H);
+    // quadrature and thermodynamic constants (src/ff_dmd.cpp:157-204, src/atoms_dmd.cpp:19)
+    A_.ne = t.ne;
+    A_.nr = t.nr;
+    A_.nrho = t.nrho;
+    A_.N = nquad;
+    A_.C = c_dim;
+    A_.dr_inv = 1.0 / t.dr;
+    A_.drho_inv = 1.0 / t.drho;
+    A_.rho_max = (double)t.nrho * t.drho;
+    A_.kBT = kB * T;
+    gauss_hermite(nquad, A_.xi, A_.wi);
+    A_.xmax = A_.xi[nquad - 1];
+    scl_ = A_.xmax;
+    for (int e = 0; e < t.ne; ++e) {
+        const double lambda = hP / std::sqrt(2.0 * M_PI * A_.kBT * masses[e]);
+        A_.mu0[e] = A_.kBT * 1.5 * (2.0 * std::log(lambda) - 1.0 - std::log(M_PI));
+    }
+    for (int k = 0; k < t.ne * t.ne; ++k) A_.cut[k] = t.rc[k];
+    // FP64 device tables
+    {
+        const int ne = t.ne, nr = t.nr;
+        std::vector<double4> T4((size_t)ne * ne * nr * 3);
+        auto d4 = [](const double* c) { return make_double4(c[0], c[1], c[2], c[3]); };
+        for (int a = 0; a < ne; ++a)
+            for (int b = 0; b < ne; ++b)
+                for (int m = 0; m < nr; ++m) {
+                    const size_t base = (((size_t)a * ne + b) * nr + m) * 3;
+                    T4[base] = d4(t.rphi_at(a, b) + (size_t)m * 4);
+                    T4[base + 1] = d4(t.rho_at(a, b) + (size_t)m * 4);
+                    T4[base + 2] = d4(t.rho_at(b, a) + (size_t)m * 4);
+                }
+        tab_.reserve(T4.size());
+        MAPP_HIP(hipMemcpy(tab_.get(), T4.data(), T4.size() * sizeof(double4), hipMemcpyHostToDevice));
+        F5_.reserve(t.F.size());
+        MAPP_HIP(hipMemcpy(F5_.get(), t.F.data(), t.F.size() * sizeof(double), hipMemcpyHostToDevice));
+        A_.tab = tab_.get();
+        A_.F5 = F5_.get();
+    }
+    double max_alpha = 0.0;
+    for (int k = 0; k < nsites * c_dim; ++k)
+        if (c[k] != -1.0) max_alpha = std::max(max_alpha, alpha[k]);
+    max_cut_ = t.max_rc() + skin_ + scl_ * std::sqrt(2.0) * max_alpha;
+    setup_grid();
+    x_.reserve(n_); x_alt_.reserve(n_); s_.reserve(n_); x0_.reserve(n_); s_tmp_.reserve(n_);
+    al_.reserve(nc_); al_alt_.reserve(nc_); c_.reserve(nc_); c_alt_.reserve(nc_); al0_.reserve(nc_);
+    el_.reserve(nc_); el_alt_.reserve(nc_);
+    id_.reserve(n_); id_alt_.reserve(n_); slot_of_.reserve(n_);
+    f_.reserve(n_); fa_.reserve(nc_); mu_.reserve(nc_); E_.reserve(nc_); dE_.reserve(nc_);
+    keys_.reserve(n_); keys_alt_.reserve(n_); vals_.reserve(n_); vals_alt_.reserve(n_);
+    nn_.reserve(nc_);
+    flags_.reserve(4);
+    part_.reserve((size_t)2 * grid_for(std::max(n_, nc_)) * kPartW + 64);
+    red_.reserve(64);
+    red2_.reserve(256 * kPartW);
+    std::vector<double4> xh(n_);
+    std::vector<int> idh(n_), elh(nc_);
+    for (int i = 0; i < n_; ++i) {
+        xh[i] = make_double4(x[3 * (size_t)i], x[3 * (size_t)i + 1], x[3 * (size_t)i + 2], 0.0);
+        idh[i] = i;
+    }
+    for (int k = 0; k < nc_; ++k) elh[k] = elem[k];
+    MAPP_HIP(hipMemcpyAsync(x_.get(), xh.data(), sizeof(double4) * n_, hipMemcpyHostToDevice, stream_));
+    MAPP_HIP(hipMemcpyAsync(id_.get(), idh.data(), sizeof(int) * n_, hipMemcpyHostToDevice, stream_));
+    MAPP_HIP(hipMemcpyAsync(al_.get(), alpha, sizeof(double) * nc_, hipMemcpyHostToDevice, stream_));
+    MAPP_HIP(hipMemcpyAsync(c_.get(), c, sizeof(double) * nc_, hipMemcpyHostToDevice, stream_));
+    MAPP_HIP(hipMemcpyAsync(el_.get(), elh.data(), sizeof(int) * nc_, hipMemcpyHostToDevice, stream_));
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    rebuild();
+}
+
+DMDSystem::~DMDSystem()
+{
+    if (stream_) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipStreamDestroy(stream_);
+    }
+}
+
+void DMDSystem::setup_grid()
+{
+    int ncells = 1;
+    for (int d = 0; d < 3; ++d) {
+        const double cs = max_cut_ * box_.depth_inv[d];
+        grid_.cut_s[d] = cs;
+        int nc = (int)(1.0 / cs);
+        if (nc < 1) nc = 1;
+        MAPP_REQUIRE(nc <= 1024, ERR_CAPACITY, "at most 1024 cells per dimension");
+        grid_.nc[d] = nc;
+        int R = (int)std::ceil(cs * nc - 1e-12);
+        if (R < 1) R = 1;
+        MAPP_REQUIRE(R <= 2, ERR_CAPACITY, "box narrower than half the list cutoff; replicate the cell first");
+        grid_.R[d] = R;
+        ncells *= nc;
+    }
+    grid_.ncells = ncells;
+    std::vector<std::pair<uint32_t, int>> code(ncells);
+    for (int c2 = 0, c = 0; c2 < grid_.nc[2]; ++c2)
+        for (int c1 = 0; c1 < grid_.nc[1]; ++c1)
+            for (int c0 = 0; c0 < grid_.nc[0]; ++c0, ++c) code[c] = {morton3(c0, c1, c2), c};
+    std::sort(code.begin(), code.end());
+    std::vector<int> rank(ncells);
+    for (int r = 0; r < ncells; ++r) rank[code[r].second] = r;
+    cell_rank_.reserve(ncells);
+    cell_start_.reserve(ncells + 1);
+    MAPP_HIP(hipMemcpy(cell_rank_.get(), rank.data(), sizeof(int) * ncells, hipMemcpyHostToDevice));
+    const double dens = n_ / box_.vol;
+    const double sphere = 4.0 / 3.0 * M_PI * max_cut_ * max_cut_ * max_cut_;
+    maxn_ = std::max(maxn_, (int)(1.25 * dens * sphere * C_) + 16);
+}
+
+void DMDSystem::reduce_rows(const double* part, int nrows, int ncols, double* out)
+{
+    const int g = std::min(256, std::max(1, (nrows + kBlock - 1) / kBlock));
+    const int chunk = (nrows + g - 1) / g;
+    hipLaunchKernelGGL(k_reduce_rows, dim3(g), dim3(kBlock), 0, stream_, nrows, ncols, part, red2_.get(), chunk);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, stream_, g, ncols, red2_.get(), out, g);
+}
+
+void DMDSystem::rebuild()
+{
+    const int nb = grid_for(n_);
+    DGrid g{};
+    for (int d = 0; d < 3; ++d) {
+        g.nc[d] = grid_.nc[d];
+        g.R[d] = grid_.R[d];
+    }
+    BoxDev bd;
+    std::memcpy(bd.h, box_.h, sizeof bd.h);
+    std::memcpy(bd.b, box_.b, sizeof bd.b);
+    prof_.begin(K_CELL_SORT, stream_);
+    hipLaunchKernelGGL(k_dcell_keys, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), bd, g, cell_rank_.get(),
+                       s_tmp_.get(), keys_.get(), vals_.get());
+    int end_bit = 1;
+    while ((1 << end_bit) < grid_.ncells + 1) ++end_bit;
+    size_t bytes = 0;
+    MAPP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys_.get(), keys_alt_.get(), vals_.get(),
+                                                vals_alt_.get(), n_, 0, end_bit, stream_));
+    if (bytes > sort_tmp_bytes_) {
+        sort_tmp_.reserve(bytes);
+        sort_tmp_bytes_ = bytes;
+    }
+    MAPP_HIP(hipcub::DeviceRadixSort::SortPairs(sort_tmp_.get(), bytes, keys_.get(), keys_alt_.get(), vals_.get(),
+                                                vals_alt_.get(), n_, 0, end_bit, stream_));
+    hipLaunchKernelGGL(k_dpermute, dim3(nb), dim3(kBlock), 0, stream_, n_, C_, vals_alt_.get(), s_tmp_.get(), bd,
+                       al_.get(), c_.get(), el_.get(), id_.get(), x_alt_.get(), s_.get(), x0_.get(), al_alt_.get(),
+                       c_alt_.get(), el_alt_.get(), al0_.get(), id_alt_.get(), slot_of_.get());
+    std::swap(x_.p, x_alt_.p);
+    std::swap(al_.p, al_alt_.p);
+    std::swap(c_.p, c_alt_.p);
+    std::swap(el_.p, el_alt_.p);
+    std::swap(id_.p, id_alt_.p);
+    hipLaunchKernelGGL(k_dcell_start, dim3((grid_.ncells + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n_,
+                       grid_.ncells, keys_alt_.get(), cell_start_.get());
+    prof_.end(stream_);
+    build_list();
+    ++nrebuild_;
+}
+
+void DMDSystem::build_list()
+{
+    const int nb = grid_for(n_);
+    DGrid g{};
+    for (int d = 0; d < 3; ++d) {
+        g.nc[d] = grid_.nc[d];
+        g.R[d] = grid_.R[d];
+    }
+    const EamTables& t = ff_->tables();
+    g.ne = t.ne;
+    g.C = C_;
+    g.scl = scl_;
+    for (int k = 0; k < t.ne * t.ne; ++k) g.cut_sk[k] = t.rc[k] + skin_;
+    BoxDev bd;
+    std::memcpy(bd.h, box_.h, sizeof bd.h);
+    std::memcpy(bd.b, box_.b, sizeof bd.b);
+    ldl_ = nc_;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        list_.reserve((size_t)maxn_ * ldl_);
+        MAPP_HIP(hipMemsetAsync(flags_.get(), 0, sizeof(int), stream_));
+        prof_.begin(K_BUILD_LIST, stream_);
+        hipLaunchKernelGGL(k_dbuild_list, dim3(nb), dim3(kBlock), 0, stream_, n_, s_.get(), bd, g, al_.get(),
+                           c_.get(), el_.get(), cell_rank_.get(), cell_start_.get(), maxn_, ldl_, list_.get(),
+                           nn_.get(), flags_.get());
+        prof_.end(stream_);
+        int over = 0;
+        MAPP_HIP(hipMemcpyAsync(&over, flags_.get(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+        MAPP_HIP(hipStreamSynchronize(stream_));
+        if (over <= maxn_) return;
+        maxn_ = over + over / 8 + 8;
+    }
+    throw Error(ERR_STATE, "DMD neighbor list capacity did not converge");
+}
+
+bool DMDSystem::decide()
+{
+    const int nb = grid_for(n_);
+    MAPP_HIP(hipMemsetAsync(flags_.get() + 1, 0, sizeof(int), stream_));
+    hipLaunchKernelGGL(k_dmd_decide, dim3(nb), dim3(kBlock), 0, stream_, n_, C_, x_.get(), x0_.get(), al_.get(),
+                       al0_.get(), scl_, 0.5 * skin_, flags_.get());
+    int flag = 0;
+    MAPP_HIP(hipMemcpyAsync(&flag, flags_.get() + 1, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    return flag == 0;
+}
+
+bool DMDSystem::update()
+{
+    if (decide()) return false;
+    rebuild();
+    return true;
+}
+
+void DMDSystem::set_surrogate(const SurrogateArgs* s) { surr_ = s; }
+
+void DMDSystem::pass1(bool energy_only)
+{
+    const int nb = grid_for(n_);
+    BoxDev bd;
+    std::memcpy(bd.h, box_.h, sizeof bd.h);
+    std::memcpy(bd.b, box_.b, sizeof bd.b);
+    SurrogateArgs S{};
+    if (surr_) S = *surr_;
+    prof_.begin(energy_only ? K_DMD_ENERGY : K_DMD_PASS1, stream_);
+    if (surr_)
+        hipLaunchKernelGGL(k_dmd_pass1<true>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
+                           el_.get(), nn_.get(), list_.get(), ldl_, A_, bd, S, energy_only ? 1 : 0, E_.get(),
+                           dE_.get(), part_.get());
+    else
+        hipLaunchKernelGGL(k_dmd_pass1<false>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
+                           el_.get(), nn_.get(), list_.get(), ldl_, A_, bd, S, energy_only ? 1 : 0, E_.get(),
+                           dE_.get(), part_.get());
+    prof_.end(stream_);
+    reduce_rows(part_.get(), nb, 2, red_.get());
+}
+
+void DMDSystem::pass2()
+{
+    const int nb = grid_for(n_);
+    BoxDev bd;
+    std::memcpy(bd.h, box_.h, sizeof bd.h);
+    std::memcpy(bd.b, box_.b, sizeof bd.b);
+    SurrogateArgs S{};
+    if (surr_) S = *surr_;
+    double* p2 = part_.get() + (size_t)nb * kPartW;
+    prof_.begin(K_DMD_PASS2, stream_);
+    if (surr_)
+        hipLaunchKernelGGL(k_dmd_pass2<true>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
+                           el_.get(), nn_.get(), list_.get(), ldl_, A_, bd, S, E_.get(), dE_.get(), f_.get(),
+                           fa_.get(), mu_.get(), p2);
+    else
+        hipLaunchKernelGGL(k_dmd_pass2<false>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
+                           el_.get(), nn_.get(), list_.get(), ldl_, A_, bd, S, E_.get(), dE_.get(), f_.get(),
+                           fa_.get(), mu_.get(), p2);
+    prof_.end(stream_);
+    reduce_rows(p2, nb, 6, red_.get() + 8);
+}
+
+DMDReport DMDSystem::derivative(bool with_c, const double* ext_mu)
+{
+    pass1(false);
+    pass2();
+    if (with_c) {
+        MAPP_REQUIRE(ext_mu != nullptr, ERR_ARG, "ext_mu is required with c degrees of freedom");
+        DevBuf<double> em;
+        em.reserve(kMaxElems);
+        MAPP_HIP(hipMemcpyAsync(em.get(), ext_mu, sizeof(double) * A_.ne, hipMemcpyHostToDevice, stream_));
+        const int nb = grid_for(n_);
+        double* p3 = part_.get() + (size_t)nb * kPartW;
+        hipLaunchKernelGGL(k_dmd_chem, dim3(nb), dim3(kBlock), 0, stream_, n_, C_, al_.get(), c_.get(), el_.get(),
+                           A_.kBT, A_, em.get(), mu_.get(), p3);
+        reduce_rows(p3, nb, 1, red_.get() + 16);
+        MAPP_HIP(hipStreamSynchronize(stream_));
+    }
+    double h[24];
+    MAPP_HIP(hipMemcpyAsync(h, red_.get(), sizeof(double) * 24, hipMemcpyDeviceToHost, stream_));
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    prof_.flush();
+    DMDReport r{};
+    r.s = h[1];
+    r.fe = h[0] + A_.kBT * h[1] - (with_c ? h[16] : 0.0);
+    for (int k = 0; k < 6; ++k) r.FH[k] = h[8 + k];
+    return r;
+}
+
+double DMDSystem::value(bool with_c, const double* ext_mu)
+{
+    pass1(true);
+    double h[2];
+    MAPP_HIP(hipMemcpyAsync(h, red_.get(), sizeof(double) * 2, hipMemcpyDeviceToHost, stream_));
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    prof_.flush();
+    double fe = h[0] + A_.kBT * h[1];
+    if (with_c) {
+        MAPP_REQUIRE(ext_mu != nullptr, ERR_ARG, "ext_mu is required with c degrees of freedom");
+        std::vector<double> c(nc_);
+        std::vector<int> e(nc_);
+        MAPP_HIP(hipMemcpy(c.data(), c_.get(), sizeof(double) * nc_, hipMemcpyDeviceToHost));
+        MAPP_HIP(hipMemcpy(e.data(), el_.get(), sizeof(int) * nc_, hipMemcpyDeviceToHost));
+        double s = 0.0;
+        for (int k = 0; k < nc_; ++k)
+            if (c[k] >= 0.0) s += c[k] * ext_mu[e[k]];
+        fe -= s;
+    }
+    return fe;
+}
+
+void DMDSystem::set_state(const double* x, const double* alpha, const double* c)
+{
+    DevBuf<double> tx, ta, tc;
+    tx.reserve((size_t)3 * n_);
+    ta.reserve(nc_);
+    tc.reserve(nc_);
+    MAPP_HIP(hipMemcpyAsync(tx.get(), x, sizeof(double) * 3 * n_, hipMemcpyHostToDevice, stream_));
+    MAPP_HIP(hipMemcpyAsync(ta.get(), alpha, sizeof(double) * nc_, hipMemcpyHostToDevice, stream_));
+    MAPP_HIP(hipMemcpyAsync(tc.get(), c, sizeof(double) * nc_, hipMemcpyHostToDevice, stream_));
+    hipLaunchKernelGGL(k_dscatter, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n_, C_, tx.get(),
+                       ta.get(), tc.get(), slot_of_.get(), x_.get(), al_.get(), c_.get());
+    MAPP_HIP(hipStreamSynchronize(stream_));
+}
+
+void DMDSystem::get_state(double* x, double* alpha, double* c)
+{
+    DevBuf<double> tx, ta, tc;
+    tx.reserve((size_t)3 * n_);
+    ta.reserve(nc_);
+    tc.reserve(nc_);
+    hipLaunchKernelGGL(k_dgather, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n_, C_, slot_of_.get(),
+                       x_.get(), al_.get(), c_.get(), tx.get(), ta.get(), tc.get());
+    if (x) MAPP_HIP(hipMemcpyAsync(x, tx.get(), sizeof(double) * 3 * n_, hipMemcpyDeviceToHost, stream_));
+    if (alpha) MAPP_HIP(hipMemcpyAsync(alpha, ta.get(), sizeof(double) * nc_, hipMemcpyDeviceToHost, stream_));
+    if (c) MAPP_HIP(hipMemcpyAsync(c, tc.get(), sizeof(double) * nc_, hipMemcpyDeviceToHost, stream_));
+    MAPP_HIP(hipStreamSynchronize(stream_));
+}
+
+void DMDSystem::get_forces(double* f, double* f_alpha, double* mu)
+{
+    DevBuf<double> tx, ta, tc;
+    tx.reserve((size_t)3 * n_);
+    ta.reserve(nc_);
+    tc.reserve(nc_);
+    hipLaunchKernelGGL(k_dgather, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n_, C_, slot_of_.get(),
+                       f_.get(), fa_.get(), mu_.get(), tx.get(), ta.get(), tc.get());
+    if (f) MAPP_HIP(hipMemcpyAsync(f, tx.get(), sizeof(double) * 3 * n_, hipMemcpyDeviceToHost, stream_));
+    if (f_alpha) MAPP_HIP(hipMemcpyAsync(f_alpha, ta.get(), sizeof(double) * nc_, hipMemcpyDeviceToHost, stream_));
+    if (mu) MAPP_HIP(hipMemcpyAsync(mu, tc.get(), sizeof(double) * nc_, hipMemcpyDeviceToHost, stream_));
+    MAPP_HIP(hipStreamSynchronize(stream_));
+}
+
+// (i, j, n) with n != 0 or site(i) < site(j) in caller indices
+// (src/neighbor_dmd.cpp:150-167 keeps i < j over local sites and every ghost)
+int64_t DMDSystem::half_pairs_count()
+{
+    std::vector<int> nn(nc_), id(n_);
+    std::vector<uint32_t> list((size_t)maxn_ * ldl_);
+    MAPP_HIP(hipMemcpy(nn.data(), nn_.get(), sizeof(int) * nc_, hipMemcpyDeviceToHost));
+    MAPP_HIP(hipMemcpy(id.data(), id_.get(), sizeof(int) * n_, hipMemcpyDeviceToHost));
+    MAPP_HIP(hipMemcpy(list.data(), list_.get(), sizeof(uint32_t) * list.size(), hipMemcpyDeviceToHost));
+    int64_t cnt = 0;
+    for (int q = 0; q < nc_; ++q)
+        for (int k = 0; k < nn[q]; ++k) {
+            const uint32_t e = list[(size_t)k * ldl_ + q];
+            const int jc = (int)(e & kIndexMask);
+            if ((e >> kIndexBits) != (uint32_t)kZeroImage || id[q / C_] < id[jc / C_]) ++cnt;
+        }
+    return cnt;
+}
+
+void DMDSystem::half_pairs(int32_t* pi, int32_t* pj, int8_t* img)
+{
+    std::vector<int> nn(nc_), id(n_);
+    std::vector<uint32_t> list((size_t)maxn_ * ldl_);
+    MAPP_HIP(hipMemcpy(nn.data(), nn_.get(), sizeof(int) * nc_, hipMemcpyDeviceToHost));
+    MAPP_HIP(hipMemcpy(id.data(), id_.get(), sizeof(int) * n_, hipMemcpyDeviceToHost));
+    MAPP_HIP(hipMemcpy(list.data(), list_.get(), sizeof(uint32_t) * list.size(), hipMemcpyDeviceToHost));
+    int64_t c = 0;
+    for (int q = 0; q < nc_; ++q)
+        for (int k = 0; k < nn[q]; ++k) {
+            const uint32_t e = list[(size_t)k * ldl_ + q];
+            const int jc = (int)(e & kIndexMask);
+            const uint32_t code = e >> kIndexBits;
+            if (code != (uint32_t)kZeroImage || id[q / C_] < id[jc / C_]) {
+                pi[c] = id[q / C_] * C_ + q % C_;
+                pj[c] = id[jc / C_] * C_ + jc % C_;
+                img[3 * c + 2] = (int8_t)((int)(code / 25) - 2);
+                img[3 * c + 1] = (int8_t)((int)((code % 25) / 5) - 2);
+                img[3 * c + 0] = (int8_t)((int)(code % 5) - 2);
+                ++c;
+            }
+        }
+}
+
+void DMDSystem::smear(int ei, int ej, int npts, const double* r, const double* a, double* out)
+{
+    MAPP_REQUIRE(ei >= 0 && ej >= 0 && ei < A_.ne && ej < A_.ne, ERR_ARG, "element index out of range");
+    DevBuf<double> dr, da, dout;
+    dr.reserve(npts);
+    da.reserve(npts);
+    dout.reserve((size_t)npts * 9);
+    MAPP_HIP(hipMemcpy(dr.get(), r, sizeof(double) * npts, hipMemcpyHostToDevice));
+    MAPP_HIP(hipMemcpy(da.get(), a, sizeof(double) * npts, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_smear_points, dim3((npts + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, A_,
+                       ei * A_.ne + ej, npts, dr.get(), da.get(), dout.get());
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    MAPP_HIP(hipMemcpy(out, dout.get(), sizeof(double) * npts * 9, hipMemcpyDeviceToHost));
+}
+
+// MinCGDMD::__run over x and alpha (src/min_cg_dmd.h:633-700) with the
+// backtracking line search (src/ls.h:711-770: c = 0.4, rho = 0.5, the
+// initial step from the previous decrease) and the step limits of
+// MinDMDHandler::ls_prep (max_dx, max_dalpha, alpha kept positive). Stops on
+// the energy decrement e_tol like the reference, or when the largest
+// gradient component drops below grad_tol (0 disables).
+CGReport DMDSystem::cg(int max_iter, double e_tol, double max_dx, double max_dalpha, double grad_tol)
+{
+    const int nb = grid_for(std::max(n_, nc_));
+    const int nbe = (std::max(n_, nc_) + kBlock - 1) / kBlock;
+    cg_x0_.reserve(n_); cg_h_.reserve(n_); cg_f0_.reserve(n_);
+    cg_a0_.reserve(nc_); cg_ha_.reserve(nc_); cg_fa0_.reserve(nc_);
+    double* part = part_.get();
+    auto dotv = [&](const double4* ax, const double* aa, const double4* bx, const double* ba) {
+        hipLaunchKernelGGL(k_dot2, dim3(nb), dim3(kBlock), 0, stream_, n_, nc_, ax, aa, bx, ba, part);
+        reduce_rows(part, nb, 1, red_.get() + 24);
+        double v;
+        MAPP_HIP(hipMemcpyAsync(&v, red_.get() + 24, sizeof(double), hipMemcpyDeviceToHost, stream_));
+        MAPP_HIP(hipStreamSynchronize(stream_));
+        return v;
+    };
+    auto gmax = [&]() {
+        hipLaunchKernelGGL(k_gmax, dim3(1), dim3(kBlock), 0, stream_, n_, nc_, f_.get(), fa_.get(), red_.get() + 28);
+        double v;
+        MAPP_HIP(hipMemcpyAsync(&v, red_.get() + 28, sizeof(double), hipMemcpyDeviceToHost, stream_));
+        MAPP_HIP(hipStreamSynchronize(stream_));
+        return v;
+    };
+    auto copy_state = [&](DevBuf<double4>& dx, DevBuf<double>& da, const double4* sx, const double* sa) {
+        MAPP_HIP(hipMemcpyAsync(dx.get(), sx, sizeof(double4) * n_, hipMemcpyDeviceToDevice, stream_));
+        MAPP_HIP(hipMemcpyAsync(da.get(), sa, sizeof(double) * nc_, hipMemcpyDeviceToDevice, stream_));
+    };
+    CGReport rep{};
+    const int reb0 = nrebuild_;
+    DMDReport d = derivative(false, nullptr);
+    rep.evals = 1;
+    rep.fe0 = d.fe;
+    double e_curr = d.fe;
+    copy_state(cg_h_, cg_ha_, f_.get(), fa_.get());
+    double f0f0 = dotv(f_.get(), fa_.get(), f_.get(), fa_.get());
+    double prev_val = -1.0;
+    int status = 0, it = 0;
+    for (; it < max_iter; ++it) {
+        if (f0f0 == 0.0) { status = 1; break; }
+        if (grad_tol > 0.0 && gmax() < grad_tol) { status = 3; break; }
+        copy_state(cg_x0_, cg_a0_, x_.get(), al_.get());
+        copy_state(cg_f0_, cg_fa0_, f_.get(), fa_.get());
+        const double e_prev = e_curr;
+        double fh = dotv(f_.get(), fa_.get(), cg_h_.get(), cg_ha_.get());
+        if (fh < 0.0) {
+            copy_state(cg_h_, cg_ha_, f_.get(), fa_.get());
+            fh = f0f0;
+        }
+        // ls_prep: largest step within max_dx, max_dalpha and alpha > 0
+        hipLaunchKernelGGL(k_cg_limits, dim3(1), dim3(kBlock), 0, stream_, n_, nc_, cg_h_.get(), cg_ha_.get(),
+                           al_.get(), c_.get(), red_.get() + 32);
+        double lim[3];
+        MAPP_HIP(hipMemcpyAsync(lim, red_.get() + 32, sizeof(double) * 3, hipMemcpyDeviceToHost, stream_));
+        MAPP_HIP(hipStreamSynchronize(stream_));
+        double max_a = std::numeric_limits<double>::infinity();
+        if (lim[0] > 0.0) max_a = std::min(max_a, std::fabs(max_dx / lim[0]));
+        if (lim[1] > 0.0) max_a = std::min(max_a, std::fabs(max_dalpha / lim[1]));
+        max_a = std::min(max_a, lim[2]);
+        const double dfa = -fh;
+        if (dfa >= 0.0 || !(max_a > 0.0)) { status = 2; break; }
+        if (prev_val > 0.0) max_a = std::min(max_a, -prev_val / dfa);
+        prev_val = -dfa;
+        double t = max_a;
+        bool ok = false;
+        for (int bt = 0; bt < 60; ++bt) {
+            const double ideal = e_curr + t * 0.4 * dfa;
+            hipLaunchKernelGGL(k_cg_move, dim3(nbe), dim3(kBlock), 0, stream_, n_, nc_, t, cg_x0_.get(), cg_h_.get(),
+                               cg_a0_.get(), cg_ha_.get(), c_.get(), x_.get(), al_.get());
+            update();
+            const double e = value(false, nullptr);
+            ++rep.evals;
+            if (e <= ideal) {
+                prev_val *= t;
+                e_curr = e;
+                ok = true;
+                break;
+            }
+            t *= 0.5;
+        }
+        if (!ok) {
+            hipLaunchKernelGGL(k_cg_move, dim3(nbe), dim3(kBlock), 0, stream_, n_, nc_, 0.0, cg_x0_.get(),
+                               cg_h_.get(), cg_a0_.get(), cg_ha_.get(), c_.get(), x_.get(), al_.get());
+            update();
+            status = 4;
+            break;
+        }
+        d = derivative(false, nullptr);
+        ++rep.evals;
+        e_curr = d.fe;
+        if (e_prev - e_curr < e_tol) { status = 5; ++it; break; }
+        const double ff = dotv(f_.get(), fa_.get(), f_.get(), fa_.get());
+        const double ff0 = dotv(f_.get(), fa_.get(), cg_f0_.get(), cg_fa0_.get());
+        const double ratio = (ff - ff0) / f0f0;
+        hipLaunchKernelGGL(k_cg_dir, dim3(nbe), dim3(kBlock), 0, stream_, n_, nc_, ratio, f_.get(), fa_.get(),
+                           cg_h_.get(), cg_ha_.get());
+        f0f0 = ff;
+    }
+    rep.iters = it;
+    rep.status = status;
+    rep.fe = e_curr;
+    rep.grad_norm = std::sqrt(dotv(f_.get(), fa_.get(), f_.get(), fa_.get()));
+    rep.grad_max = gmax();
+    rep.rebuilds = nrebuild_ - reb0;
+    return rep;
+}
+
+}  // namespace mapp

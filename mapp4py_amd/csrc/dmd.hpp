@@ -1,0 +1,112 @@
+// dmd.hpp -- device-resident diffusive-MD (DMD) system: Gaussian-smeared
+// EAM by Gauss-Hermite quadrature over per-site components.
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "md.hpp"
+
+namespace mapp {
+
+constexpr int kMaxQuad = 16;
+constexpr int kMaxComp = 4;
+
+// Kernel argument block of the smeared force field.
+struct DmdArgs {
+    int ne, nr, nrho, N, C;
+    double dr_inv, drho_inv, rho_max, kBT, xmax;
+    double xi[kMaxQuad], wi[kMaxQuad];
+    double cut[kMaxElems * kMaxElems];  // r_c (no skin)
+    double mu0[kMaxElems];
+    const double4* tab;  // [ne*ne][nr][3]: r*phi[a][b], r*rho[a][b], r*rho[b][a] (FP64)
+    const double* F5;    // [ne][nrho][5], quartic embedding
+};
+
+struct SurrogateArgs;
+
+// physicists' Gauss-Hermite rule (weight e^{-t^2}), n <= 16
+void gauss_hermite(int n, double* x, double* w);
+
+struct DMDReport {
+    double fe;      // free energy (ext_mu term included when with_c)
+    double FH[6];   // Voigt xx,xy,xz,yy,yz,zz
+    double s;       // configurational entropy sum (units of kB)
+};
+
+struct CGReport {
+    int iters, status, rebuilds, evals;
+    double fe0, fe, grad_norm, grad_max;
+};
+
+class DMDSystem {
+public:
+    DMDSystem(std::shared_ptr<EamForceField> ff, int nsites, int c_dim, int nquad, const double* H, const double* x,
+              const uint8_t* elem, const double* alpha, const double* c, double skin, double kB, double T,
+              double hP, const double* masses);
+    ~DMDSystem();
+
+    int nsites() const { return n_; }
+    int c_dim() const { return C_; }
+    void set_state(const double* x, const double* alpha, const double* c);
+    void get_state(double* x, double* alpha, double* c);
+    bool update();
+    void rebuild();
+    // ForceFieldDMD::new_derivative<with_c> / new_value<with_c>
+    DMDReport derivative(bool with_c, const double* ext_mu);
+    double value(bool with_c, const double* ext_mu);
+    void get_forces(double* f, double* f_alpha, double* mu);
+    int64_t half_pairs_count();
+    void half_pairs(int32_t* i, int32_t* j, int8_t* img);
+    // MinCGDMD over x and alpha (c fixed)
+    CGReport cg(int max_iter, double e_tol, double max_dx, double max_dalpha, double grad_tol);
+    // smeared pair functions at given (r, alpha_ij): out[k*9 + q], q = rho_phi[3], drho_phi_dr[3], drho_phi_dalpha[3]
+    void smear(int ei, int ej, int npts, const double* r, const double* a, double* out);
+    Profiler& profiler() { return prof_; }
+    void set_surrogate(const SurrogateArgs* s);
+    const DmdArgs& args() const { return A_; }
+    hipStream_t stream() const { return stream_; }
+
+private:
+    void setup_grid();
+    void build_list();
+    void pass1(bool energy_only);
+    void pass2();
+    bool decide();
+    void reduce_rows(const double* part, int nrows, int ncols, double* out);
+
+    std::shared_ptr<EamForceField> ff_;
+    int n_, C_, nc_;  // sites, components per site, total components
+    double skin_, max_cut_, scl_;
+    Box box_;
+    CellGrid grid_{};
+    DmdArgs A_{};
+    hipStream_t stream_ = nullptr;
+    int nrebuild_ = 0;
+    double kB_, T_;
+
+    DevBuf<double4> x_, x_alt_, s_, x0_, s_tmp_;
+    DevBuf<double> al_, al_alt_, c_, c_alt_, al0_;
+    DevBuf<int> el_, el_alt_;
+    DevBuf<int> id_, id_alt_, slot_of_;
+    DevBuf<double4> f_;
+    DevBuf<double> fa_, mu_, E_, dE_;
+    DevBuf<uint32_t> keys_, keys_alt_;
+    DevBuf<int> vals_, vals_alt_;
+    DevBuf<unsigned char> sort_tmp_;
+    size_t sort_tmp_bytes_ = 0;
+    DevBuf<int> cell_rank_, cell_start_;
+    DevBuf<int> nn_;
+    DevBuf<uint32_t> list_;
+    int maxn_ = 0, ldl_ = 0;
+    DevBuf<int> flags_;
+    DevBuf<double> part_, red_, red2_;
+    DevBuf<double4> tab_;
+    DevBuf<double> F5_;
+    // CG work vectors (x part as double4 slots, alpha part per component)
+    DevBuf<double4> cg_x0_, cg_h_, cg_f0_;
+    DevBuf<double> cg_a0_, cg_ha_, cg_fa0_;
+    Profiler prof_;
+    const SurrogateArgs* surr_ = nullptr;
+};
+
+}  // namespace mapp

@@ -227,6 +227,32 @@ def _lattice(basis, a, n):
     return pts.astype(np.float64), H
 
 
+def dmd_cuh_sites(cells, seed=4, a=3.615, alpha_lo=0.05, alpha_hi=0.30, vacancy=0.01):
+    """Cu-H diffusive-MD sites on an fcc lattice, two components per site
+    (component 0 = Cu, 1 = H): alpha log-uniform in [alpha_lo, alpha_hi] per
+    component, c_H ~ Beta(2, 5), c_Cu = max(1 - vacancy - c_H, 0)."""
+    x, H = fcc(a, cells)
+    rng = np.random.default_rng(seed)
+    n = len(x)
+    elem = np.tile(np.array([0, 1], np.uint8), (n, 1))
+    alpha = np.exp(rng.uniform(np.log(alpha_lo), np.log(alpha_hi), (n, 2)))
+    cH = rng.beta(2.0, 5.0, n)
+    c = np.stack([np.maximum(1.0 - vacancy - cH, 0.0), cH], axis=1)
+    return x, H, elem, alpha, c
+
+
+def gaussian_knots(s=1.5, nr=20000, dr=0.0005, nrho=2000, drho=0.01, r_zero=9.0):
+    """Single-element table whose density and r*phi are r*exp(-r^2/s^2) after
+    the DMD r-scaling, so the 3-D Gaussian smearing has a closed form:
+    g(r, a) = (s^2/(s^2+a^2))^(3/2) exp(-r^2/(s^2+a^2))."""
+    r = np.arange(nr) * dr
+    f = np.where(r < r_zero, np.exp(-r * r / (s * s)), 0.0)
+    F = np.zeros((1, nrho))
+    rho = f[None, None, :].copy()
+    r_phi = (r * f)[None, None, :].copy()
+    return EamKnots(["X"], [1.0], nr, dr, nrho, drho, F, rho, r_phi, False)
+
+
 def fcc(a, n):
     """fcc positions [4*n0*n1*n2][3] and the box H (rows = box vectors)."""
     return _lattice([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]], a, n)

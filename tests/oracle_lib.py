@@ -239,3 +239,56 @@ def brute_half_pairs(H, x, elem, cut_sk, kmax=3):
                         continue
                     out.add((int(i), int(j), a, b, c))
     return out
+
+
+class ODMD:
+    """Oracle DMD system (reference semantics: per-component half list over ghosts)."""
+
+    def __init__(self, ff: OEam, H, x, elem, alpha, c, nquad=3, skin=0.5, kB=8.617330350e-5, T=300.0, hP=None,
+                 masses=None):
+        from mapp4py_amd.potentials import HP
+
+        x = f64(x).reshape(-1, 3)
+        self.n = x.shape[0]
+        self.elem = np.ascontiguousarray(elem, dtype=np.uint8).reshape(self.n, -1)
+        self.C = self.elem.shape[1]
+        alpha = f64(alpha).reshape(self.n, self.C)
+        c = f64(c).reshape(self.n, self.C)
+        H = f64(H).reshape(3, 3)
+        masses = f64(masses)
+        self.ff = ff
+        self.h = L().o_dmd_create(ff.h, self.n, self.C, nquad, P(H), P(x), P(self.elem, u8p), P(alpha), P(c),
+                                  float(skin), float(kB), float(T), float(HP if hP is None else hP), P(masses))
+
+    def pairs(self):
+        n = L().o_dmd_npairs(self.h)
+        i = np.zeros(n, np.int32)
+        j = np.zeros(n, np.int32)
+        img = np.zeros((n, 3), np.int8)
+        L().o_dmd_pairs(self.h, P(i, i32p), P(j, i32p), P(img, i8p))
+        return i, j, img
+
+    def set_state(self, x, alpha, c):
+        x, alpha, c = f64(x), f64(alpha), f64(c)
+        L().o_dmd_set_state(self.h, P(x), P(alpha), P(c))
+
+    def force(self):
+        f = np.zeros((self.n, 3))
+        fa = np.zeros((self.n, self.C))
+        mu = np.zeros((self.n, self.C))
+        out = np.zeros(8)
+        L().o_dmd_force(self.h, P(f), P(fa), P(mu), P(out))
+        return f, fa, mu, out[0], out[1:7], out[7]
+
+    def energy(self):
+        return L().o_dmd_energy(self.h)
+
+    def pair_eval(self, ei, ej, r, a):
+        out = np.zeros(9)
+        L().o_dmd_pair_eval(self.h, ei, ej, float(r), float(a), P(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            L().o_dmd_free(self.h)
+            self.h = None

@@ -1,0 +1,140 @@
+"""GPU parity of the DMD (Gaussian-smeared EAM) path against the oracle.
+
+Bars: per-component neighbor sets bit-exact; the DMD kernels run FP64, so
+forces on x and alpha, the chemical-potential sums, free energy, virial and
+entropy agree with the FP64 oracle to 1e-9 relative (summation order only).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from mapp4py_amd import potentials as pot
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-9
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def M():
+    import mapp4py_amd
+
+    return mapp4py_amd
+
+
+@pytest.fixture(scope="module")
+def cuh():
+    return pot.johnson_cuh(nr=5000, nrho=5000, rc=5.5)
+
+
+def sites(cells, seed, tilt=(0, 0, 0)):
+    x, H, elem, alpha, c = pot.dmd_cuh_sites(cells, seed=seed)
+    H = H.copy()
+    H[1, 0], H[2, 0], H[2, 1] = tilt
+    x = x + np.random.default_rng(seed + 100).normal(0, 0.05, x.shape)
+    return x, H, elem, alpha, c
+
+
+def both(M, p, x, H, elem, alpha, c, nquad=3):
+    g = M.AtomsDMD(M.ForceFieldEAM.from_knots(p, dmd=True), H, x, elem, alpha, c, nquad=nquad, masses=p.masses)
+    o = O.ODMD(O.OEam.from_knots(p, dmd=True), H, x, elem, alpha, c, nquad=nquad, masses=p.masses)
+    return g, o
+
+
+def triples(i, j, img):
+    return {(int(a), int(b), *map(int, m)) for a, b, m in zip(i, j, img)}
+
+
+def close(a, b, tag):
+    a, b = np.asarray(a), np.asarray(b)
+    err = np.abs(a - b).max() / max(1e-300, np.abs(b).max())
+    assert err < RTOL, (tag, err)
+    return err
+
+
+@pytest.mark.parametrize("cells,tilt,seed", [((3, 3, 3), (0, 0, 0), 1), ((3, 4, 3), (0.6, -0.4, 0.5), 2),
+                                             ((2, 2, 2), (0, 0, 0), 3)])
+def test_dmd_lists_and_derivative_match_oracle(M, cuh, cells, tilt, seed):
+    x, H, elem, alpha, c = sites(cells, seed, tilt)
+    g, o = both(M, cuh, x, H, elem, alpha, c)
+    assert triples(*g.half_pairs()) == triples(*o.pairs())
+    f, fa, mu, fe, FH, S = g.derivative()
+    fo, fao, muo, feo, FHo, So = o.force()
+    errs = [close(f, fo, "f_x"), close(fa, fao, "f_alpha"), close(mu, muo, "mu"), close([fe], [feo], "fe"),
+            close(FH, FHo, "F_H"), close([S], [So], "S")]
+    say(f"DMD {cells} tilt {tilt}: rel errors f {errs[0]:.1e} f_a {errs[1]:.1e} mu {errs[2]:.1e} "
+        f"fe {errs[3]:.1e} F_H {errs[4]:.1e} S {errs[5]:.1e}")
+    close([g.value()], [o.energy()], "value")
+
+
+def test_chemical_potential_transform(M, cuh):
+    x, H, elem, alpha, c = sites((3, 3, 3), 5)
+    g, _ = both(M, cuh, x, H, elem, alpha, c)
+    ext = np.array([-3.2, -2.3])
+    _, _, raw, fe0, _, _ = g.derivative(False)
+    _, _, chem, fe1, _, _ = g.derivative(True, ext)
+    kT = pot.KB_EV * 300.0
+    lam = pot.HP / np.sqrt(2 * np.pi * kT * np.asarray(cuh.masses))
+    mu0 = kT * 1.5 * (2 * np.log(lam) - 1 - np.log(np.pi))
+    cv = 1.0 - c.sum(1, keepdims=True)
+    want = -raw - kT * (np.log(c) - np.log(cv) - 3 * np.log(alpha)) - mu0[elem] + ext[elem]
+    np.testing.assert_allclose(chem, want, rtol=1e-12, atol=1e-12)
+    assert abs((fe0 - fe1) - (c * ext[elem]).sum()) < 1e-9 * abs(fe0)
+    assert abs(g.value(True, ext) - fe1) < 1e-9 * abs(fe1)
+
+
+def test_smear_matches_oracle_pair_eval(M, cuh):
+    x, H, elem, alpha, c = sites((2, 2, 2), 7)
+    g, o = both(M, cuh, x, H, elem, alpha, c)
+    rng = np.random.default_rng(3)
+    r = rng.uniform(1.5, 6.0, 64)
+    a = rng.uniform(0.07, 0.42, 64)
+    for ei in range(2):
+        for ej in range(2):
+            out = g.smear(ei, ej, r, a)
+            ref = np.array([o.pair_eval(ei, ej, rr, aa) for rr, aa in zip(r, a)])
+            np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-13)
+
+
+def test_dmd_update_and_rebuild(M, cuh):
+    x, H, elem, alpha, c = sites((3, 3, 3), 9)
+    g, _ = both(M, cuh, x, H, elem, alpha, c)
+    xw, aw, cw = g.state()
+    g.set_state(xw + 0.05, aw, cw)
+    assert g.update() is False
+    a2 = aw.copy()
+    a2[3, 1] += 0.3  # alpha growth times x_max beyond skin/2
+    g.set_state(xw + 0.05, a2, cw)
+    assert g.update() is True
+    xs, as_, cs = g.state()
+    o = O.ODMD(O.OEam.from_knots(cuh, dmd=True), H, xs, elem, as_, cs, nquad=3, masses=cuh.masses)
+    assert triples(*g.half_pairs()) == triples(*o.pairs())
+    f, fa, mu, fe, FH, S = g.derivative()
+    fo, fao, muo, feo, FHo, So = o.force()
+    close(f, fo, "f after rebuild")
+    close(fa, fao, "f_alpha after rebuild")
+
+
+def test_cg_lowers_free_energy_and_gradient(M, cuh):
+    x, H, elem, alpha, c = sites((3, 3, 3), 11)
+    g, _ = both(M, cuh, x, H, elem, alpha, c)
+    r = g.min_cg(max_iter=60, e_tol=1e-12, grad_tol=1e-6)
+    f, fa, mu, fe, FH, S = g.derivative()
+    say(f"CG: {r['iters']} iterations status {r['status']} fe {r['fe0']:.6f} -> {r['fe']:.6f} "
+        f"max|grad| {r['grad_max']:.2e}")
+    assert r["fe"] < r["fe0"]
+    assert abs(fe - r["fe"]) < 1e-9 * abs(fe)
+    assert max(np.abs(f).max(), np.abs(fa).max()) == pytest.approx(r["grad_max"], rel=1e-12)
+    # the result is a stationary point of the oracle's free energy too
+    xs, as_, cs = g.state()
+    o = O.ODMD(O.OEam.from_knots(cuh, dmd=True), H, xs, elem, as_, cs, nquad=3, masses=cuh.masses)
+    fo, fao, _, feo, _, _ = o.force()
+    assert abs(feo - r["fe"]) < 1e-9 * abs(feo)
+    assert max(np.abs(fo).max(), np.abs(fao).max()) < 10 * r["grad_max"] + 1e-9

@@ -1,0 +1,171 @@
+"""Pins the oracle's DMD restatement: closed-form Gaussian smearing, the
+alpha -> 0 limit, finite differences of the free energy against f_x,
+f_alpha and the chemical potential, and the list against brute force."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from mapp4py_amd import potentials as pot
+
+KB = pot.KB_EV
+
+
+def closed_form(s, r, a):
+    q = s * s + a * a
+    g = (s * s / q) ** 1.5 * math.exp(-r * r / q)
+    dr = g * (-2.0 / q)                      # (1/r) dg/dr
+    da = g * (-3.0 / q + 2.0 * r * r / q / q)  # (1/a) dg/da
+    return g, dr, da
+
+
+@pytest.fixture(scope="module")
+def gauss16():
+    p = pot.gaussian_knots(s=1.5)
+    ff = O.OEam.from_knots(p, dmd=True)
+    x, H = pot.fcc(8.0, (2, 2, 2))
+    n = len(x)
+    d = O.ODMD(ff, H, x, np.zeros((n, 1), np.uint8), np.full((n, 1), 0.1), np.full((n, 1), 0.5), nquad=16,
+               masses=[1.0])
+    return d
+
+
+@pytest.mark.parametrize("r,a", [(0.7, 0.05), (1.3, 0.2), (2.5, 0.3), (3.4, 0.15), (1.0, 0.45)])
+def test_smearing_matches_3d_gaussian_convolution(gauss16, r, a):
+    out = gauss16.pair_eval(0, 0, r, a)
+    g, dr, da = closed_form(1.5, r, a)
+    for k in range(3):  # phi, rho_ij, rho_ji all hold r*exp(-r^2/s^2)
+        assert abs(out[k] - g) < 1e-8 * max(1.0, abs(g)), (k, out[k], g)
+        assert abs(out[3 + k] - dr) < 1e-7 * max(1.0, abs(dr)), (k, out[3 + k], dr)
+        assert abs(out[6 + k] - da) < 1e-6 * max(1.0, abs(da)), (k, out[6 + k], da)
+
+
+def test_zero_width_recovers_the_table():
+    p = pot.johnson_cuh(nr=2000, nrho=2000)
+    ff_d = O.OEam.from_knots(p, dmd=True)
+    x, H = pot.fcc(3.615, (2, 2, 2))
+    n = len(x)
+    d = O.ODMD(ff_d, H, x, np.zeros((n, 1), np.uint8), np.full((n, 1), 0.1), np.full((n, 1), 0.9), nquad=3,
+               masses=p.masses)
+    for r in (2.2, 2.9, 3.7):
+        out = d.pair_eval(0, 1, r, 1e-7)
+        phi = p.analytic["phi"][0][1](r)
+        rho_01 = p.analytic["rho"][0][0](r)   # setfl: density of a Cu atom
+        rho_10 = p.analytic["rho"][1][0](r)   # density of an H atom
+        assert abs(out[0] - phi) < 1e-6
+        assert abs(out[1] - rho_01) < 1e-6
+        assert abs(out[2] - rho_10) < 1e-6
+
+
+@pytest.fixture(scope="module")
+def cuh_small():
+    p = pot.johnson_cuh(nr=3000, nrho=3000)
+    ff = O.OEam.from_knots(p, dmd=True)
+    x, H, elem, alpha, c = pot.dmd_cuh_sites((2, 2, 2), seed=3)
+    x = x + np.random.default_rng(1).normal(0, 0.05, x.shape)
+    # wrapped into the box, as a list build leaves them, so set_state keeps
+    # the image shifts of the list valid
+    s = x @ np.linalg.inv(H)
+    s -= np.floor(s)
+    x = s @ H
+    return p, ff, x, H, elem, alpha, c
+
+
+def make(ff, p, x, H, elem, alpha, c, nquad=12):
+    # The reference differentiates the smeared integral (Gaussian
+    # integration by parts turns d/dr and d/dalpha into moments), not the
+    # N-point sum, so forces equal -grad FE only as far as the rule is exact:
+    # the gradient checks use a 12-point rule.
+    return O.ODMD(ff, H, x, elem, alpha, c, nquad=nquad, masses=p.masses)
+
+
+def test_forces_are_free_energy_gradients(cuh_small):
+    p, ff, x, H, elem, alpha, c = cuh_small
+    d = make(ff, p, x, H, elem, alpha, c)
+    f, fa, mu, fe, FH, S = d.force()
+    assert abs(d.energy() - fe) < 1e-10 * abs(fe)
+    h = 1e-6
+    rng = np.random.default_rng(0)
+    for k in rng.choice(len(x), 4, replace=False):
+        for q in range(3):
+            xp = x.copy(); xp[k, q] += h
+            xm = x.copy(); xm[k, q] -= h
+            d.set_state(xp, alpha, c); ep = d.energy()
+            d.set_state(xm, alpha, c); em = d.energy()
+            fd = -(ep - em) / (2 * h)
+            assert abs(fd - f[k, q]) < 1e-5 * max(1.0, abs(f[k, q])), (k, q, fd, f[k, q])
+        for q in range(2):
+            ap = alpha.copy(); ap[k, q] += h
+            am = alpha.copy(); am[k, q] -= h
+            d.set_state(x, ap, c); ep = d.energy()
+            d.set_state(x, am, c); em = d.energy()
+            fd = -(ep - em) / (2 * h)
+            assert abs(fd - fa[k, q]) < 1e-5 * max(1.0, abs(fa[k, q])), (k, q, fd, fa[k, q])
+    d.set_state(x, alpha, c)
+
+
+def test_raw_f_c_plus_site_terms_is_dFE_dc(cuh_small):
+    # dFE/dc_i = f_c + mu_0 - 3 kT ln(alpha_i) + kT (ln c_i - ln c_v)
+    p, ff, x, H, elem, alpha, c = cuh_small
+    d = make(ff, p, x, H, elem, alpha, c)
+    f, fa, mu, fe, FH, S = d.force()
+    kT = KB * 300.0
+    lam = pot.HP / np.sqrt(2 * np.pi * kT * np.asarray(p.masses))
+    mu0 = kT * 1.5 * (2 * np.log(lam) - 1 - np.log(np.pi))
+    h = 1e-7
+    for k in (0, 5, 17):
+        for q in range(2):
+            cp = c.copy(); cp[k, q] += h
+            cm = c.copy(); cm[k, q] -= h
+            d.set_state(x, alpha, cp); ep = d.energy()
+            d.set_state(x, alpha, cm); em = d.energy()
+            fd = (ep - em) / (2 * h)
+            cv = 1.0 - c[k].sum()
+            an = mu[k, q] + mu0[elem[k, q]] - 3 * kT * np.log(alpha[k, q]) + kT * (np.log(c[k, q]) - np.log(cv))
+            assert abs(fd - an) < 1e-5 * max(1.0, abs(an)), (k, q, fd, an)
+    d.set_state(x, alpha, c)
+
+
+def test_virial_is_minus_strain_derivative(cuh_small):
+    p, ff, x, H, elem, alpha, c = cuh_small
+    _, _, _, fe, FH, _ = make(ff, p, x, H, elem, alpha, c).force()
+    h = 1e-6
+    for a, b, k in [(0, 0, 0), (1, 1, 3), (2, 2, 5), (1, 0, 1)]:
+        eps = np.zeros((3, 3))
+        eps[a, b] = h
+        ep = make(ff, p, x @ (np.eye(3) + eps), H @ (np.eye(3) + eps), elem, alpha, c).energy()
+        em = make(ff, p, x @ (np.eye(3) - eps), H @ (np.eye(3) - eps), elem, alpha, c).energy()
+        assert abs(-(ep - em) / (2 * h) - FH[k]) < 1e-5 * max(1.0, abs(FH[k]))
+
+
+def test_dmd_half_list_matches_brute_force(cuh_small):
+    p, ff, x, H, elem, alpha, c = cuh_small
+    d = make(ff, p, x, H, elem, alpha, c, nquad=3)
+    i, j, img = d.pairs()
+    got = {(int(a), int(b), *map(int, m)) for a, b, m in zip(i, j, img)}
+    assert len(got) == len(i)
+    xg, wg = O.gauss_hermite(3)
+    xmax = xg[-1]
+    cut_sk = ff.cut() + 0.5
+    B = np.linalg.inv(H)
+    s = x @ B
+    s -= np.floor(s)
+    want = set()
+    n = len(x)
+    for a0 in range(-2, 3):
+        for a1 in range(-2, 3):
+            for a2 in range(-2, 3):
+                xj = (s + np.array([a0, a1, a2])) @ H
+                xi = s @ H
+                r = np.sqrt(((xi[:, None, :] - xj[None, :, :]) ** 2).sum(-1))
+                for si in range(n):
+                    for sj in range(n):
+                        if (a0, a1, a2) == (0, 0, 0) and si >= sj:
+                            continue
+                        for ic in range(2):
+                            for jc in range(2):
+                                aij = math.sqrt(alpha[si, ic] ** 2 + alpha[sj, jc] ** 2)
+                                if r[si, sj] - xmax * aij < cut_sk[elem[si, ic], elem[sj, jc]]:
+                                    want.add((si * 2 + ic, sj * 2 + jc, a0, a1, a2))
+    assert got == want
