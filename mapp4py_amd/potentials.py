@@ -20,11 +20,14 @@ forces against closed forms.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 __all__ = [
     "EamKnots", "johnson_cu", "johnson_feni_fs", "johnson_cuh", "write_setfl", "write_fs",
-    "fcc", "bcc", "MASS",
+    "fcc", "bcc", "dmd_cuh_sites", "gaussian_knots", "gcmc_nih_cells", "NIALH_JEA", "NIALH_ELEMS",
+    "MASS", "KB_EV", "HP", "FS",
 ]
 
 MASS = {"Cu": 63.546, "Fe": 55.845, "Ni": 58.6934, "H": 1.00794, "Al": 26.9815}
@@ -260,3 +263,53 @@ def fcc(a, n):
 
 def bcc(a, n):
     return _lattice([[0, 0, 0], [0.5, 0.5, 0.5]], a, n)
+
+
+NIALH_JEA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "NiAlH_jea.eam.alloy")
+NIALH_ELEMS = ("Ni", "Al", "H")
+
+
+def _fcc_shapes(nmin=108, nmax=4000, kmin=3):
+    """Every fcc block k0 <= k1 <= k2 (k >= kmin) with 4 k0 k1 k2 atoms in
+    [nmin, nmax], as (natoms, spread, (k0, k1, k2)) sorted by size."""
+    out = []
+    k = kmin
+    while 4 * k ** 3 <= nmax:
+        for k1 in range(k, nmax):
+            if 4 * k * k1 * k1 > nmax:
+                break
+            for k2 in range(k1, nmax):
+                n = 4 * k * k1 * k2
+                if n > nmax:
+                    break
+                if n >= nmin:
+                    out.append((n, k2 - k, (k, k1, k2)))
+        k += 1
+    return sorted(out)
+
+
+def gcmc_nih_cells(ncells, seed=6, nmin=108, nmax=4000, a=3.52, jitter=0.03, h_frac=0.02):
+    """Batch of periodic fcc Ni cells for grand-canonical H exchange: sizes
+    drawn log-uniform in [nmin, nmax] atoms (each rounded to the nearest fcc
+    block of at least 3 cells a side, most cubic first), Ni positions
+    jittered N(0, jitter), H on a random h_frac of the octahedral sites.
+    Element codes follow NiAlH_jea.eam.alloy: Ni = 0, H = 2.
+    Returns a list of (x [n][3], H [3][3], elem [n] uint8)."""
+    rng = np.random.default_rng(seed)
+    shapes = _fcc_shapes(nmin, nmax)
+    sizes = np.array([s[0] for s in shapes], dtype=np.float64)
+    cells = []
+    for _ in range(ncells):
+        target = np.exp(rng.uniform(np.log(nmin), np.log(nmax)))
+        best = min(range(len(shapes)), key=lambda q: (abs(sizes[q] - target), shapes[q][1]))
+        k = shapes[best][2]
+        x, H = fcc(a, k)
+        x = x + rng.normal(0.0, jitter, x.shape)
+        octa, _ = _lattice([[0.5, 0, 0], [0, 0.5, 0], [0, 0, 0.5], [0.5, 0.5, 0.5]], a, k)
+        nh = int(round(h_frac * len(octa)))
+        pick = rng.choice(len(octa), nh, replace=False) if nh else np.zeros(0, np.int64)
+        xh = octa[np.sort(pick)] + rng.normal(0.0, jitter, (nh, 3))
+        xa = np.concatenate([x, xh])
+        elem = np.concatenate([np.zeros(len(x), np.uint8), np.full(nh, 2, np.uint8)])
+        cells.append((xa, H, elem))
+    return cells

@@ -48,6 +48,10 @@ double o_rsq(const double* a, const double* b);
 void o_cell_do(const o_ghosts* g, const o_box* bx, double max_cut,
                void (*pre)(void*, int), void (*visit)(void*, int, int), void (*post)(void*, int), void* ctx);
 
+/* read-only view of an MD system's atoms (locals then ghosts) and half list */
+void o_md_view(const o_md* md, const double** x, const double** s, const uint8_t** elem,
+               const int64_t** row, const int32_t** col);
+
 /* EAM spline evaluation at r (p clamped as src/ff_eam.cpp:117-128) */
 static inline void o_spline_pm(double v, double inv, int n, size_t* m, double* p)
 {

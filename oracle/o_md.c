@@ -659,3 +659,13 @@ void o_create_T(int natms, const uint8_t* elem, const double* masses, double kB,
         for (int d = 0; d < 3; d++) v[3 * i + d] = o_rng_gaussian(&r) * fac;
     }
 }
+
+void o_md_view(const o_md* md, const double** x, const double** s, const uint8_t** elem,
+               const int64_t** row, const int32_t** col)
+{
+    *x = md->g.x;
+    *s = md->g.s;
+    *elem = md->elem;
+    *row = md->row;
+    *col = md->col;
+}

@@ -120,15 +120,26 @@ double o_dmd_energy(o_dmd*);
 void o_dmd_pair_eval(const o_dmd*, int ei, int ej, double r, double alpha_ij, double* out9);
 
 /* ---------------- GCMC (src/sgcmc.cpp, src/ff_eam.cpp:323-589) ---------------- */
-/* Runs nattempts SGCMC insert/delete attempts of element gas_elem on one
- * periodic cell (sites given by x/elem). Reproduces the RNG draw order of
- * SGCMC::attmpt. decisions[k] = 1 accepted insertion, 2 accepted deletion,
- * -1 rejected insertion, -2 rejected deletion; dU[k] = energy delta.
- * Returns the final atom count; x/elem must have room for natms+nattempts. */
-int o_gcmc_run(const o_eam* ff, int natms, double* x, uint8_t* elem,
-               const double* H, int gas_elem, double mu, double T, double kB,
-               double hP, const double* masses, int seed, int nattempts,
-               int* decisions, double* dU);
+/* One periodic cell under single-process SGCMC with EAM exchange energies.
+ * cap bounds the atom count (inserted atoms included). */
+typedef struct o_gcmc o_gcmc;
+o_gcmc* o_gcmc_create(const o_eam* ff, int natms, const double* H, const double* x,
+                      const uint8_t* elem, int gas_elem, double mu, double T, double kB,
+                      double hP, const double* masses, int seed, int cap);
+void o_gcmc_free(o_gcmc*);
+/* nattempts SGCMC::attmpt calls. decisions[k]: 1 accepted insertion, -1
+ * rejected insertion, 2 accepted deletion, -2 rejected deletion, 0 deletion
+ * drawn with no gas atom (no further draws); dU[k] = delta_u (0 when 0).
+ * Returns 0, or -1 when an insertion would exceed cap. */
+int o_gcmc_attempts(o_gcmc*, int nattempts, int* decisions, double* dU);
+int o_gcmc_natms(const o_gcmc*);
+int o_gcmc_ngas(const o_gcmc*);
+int o_gcmc_rng_seed(const o_gcmc*);
+/* out8: z_fac, vol, beta, lambda, sigma, cut, cut_s[0], cell_size[0];
+ * iout4: max trial images, cells per dimension (3) */
+void o_gcmc_info(const o_gcmc*, double* out8, int* iout4);
+void o_gcmc_state(const o_gcmc*, double* x, double* s, uint8_t* elem, double* rho,
+                  double* F, double* v);
 
 #ifdef __cplusplus
 }

@@ -67,6 +67,15 @@ def L():
             "o_dmd_force": (None, [C.c_void_p, dp, dp, dp, dp]),
             "o_dmd_energy": (C.c_double, [C.c_void_p]),
             "o_dmd_pair_eval": (None, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, dp]),
+            "o_gcmc_create": (C.c_void_p, [C.c_void_p, C.c_int, dp, dp, u8p, C.c_int, C.c_double, C.c_double,
+                                           C.c_double, C.c_double, dp, C.c_int, C.c_int]),
+            "o_gcmc_free": (None, [C.c_void_p]),
+            "o_gcmc_attempts": (C.c_int, [C.c_void_p, C.c_int, i32p, dp]),
+            "o_gcmc_natms": (C.c_int, [C.c_void_p]),
+            "o_gcmc_ngas": (C.c_int, [C.c_void_p]),
+            "o_gcmc_rng_seed": (C.c_int, [C.c_void_p]),
+            "o_gcmc_info": (None, [C.c_void_p, dp, i32p]),
+            "o_gcmc_state": (None, [C.c_void_p, dp, dp, u8p, dp, dp, dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -291,4 +300,59 @@ class ODMD:
     def __del__(self):
         if getattr(self, "h", None):
             L().o_dmd_free(self.h)
+            self.h = None
+
+
+class OGCMC:
+    """Oracle single-cell SGCMC with EAM exchange energies."""
+
+    def __init__(self, ff: OEam, H, x, elem, gas, mu, T, masses, seed, cap=None, kB=8.617330350e-5, hP=None):
+        from mapp4py_amd.potentials import HP
+
+        x = f64(x).reshape(-1, 3)
+        n = x.shape[0]
+        self.ff = ff
+        elem = np.ascontiguousarray(elem, dtype=np.uint8)
+        cap = n + 4096 if cap is None else cap
+        masses = f64(masses)
+        self.h = L().o_gcmc_create(ff.h, n, P(f64(H)), P(x), P(elem, u8p), int(gas), float(mu), float(T),
+                                   float(kB), float(HP if hP is None else hP), P(masses), int(seed), int(cap))
+
+    def attempts(self, k):
+        d = np.zeros(k, np.int32)
+        du = np.zeros(k)
+        rc = L().o_gcmc_attempts(self.h, k, P(d, i32p), P(du))
+        if rc != 0:
+            raise RuntimeError("oracle GCMC cell capacity exceeded")
+        return d, du
+
+    @property
+    def natms(self):
+        return L().o_gcmc_natms(self.h)
+
+    @property
+    def ngas(self):
+        return L().o_gcmc_ngas(self.h)
+
+    @property
+    def rng_seed(self):
+        return L().o_gcmc_rng_seed(self.h)
+
+    def info(self):
+        o = np.zeros(8)
+        i = np.zeros(4, np.int32)
+        L().o_gcmc_info(self.h, P(o), P(i, i32p))
+        return dict(z_fac=o[0], vol=o[1], beta=o[2], lam=o[3], sigma=o[4], cut=o[5], cut_s0=o[6], cell_size0=o[7],
+                    max_trials=int(i[0]), ncell=tuple(int(v) for v in i[1:]))
+
+    def state(self):
+        n = self.natms
+        x, s, rho, F, v = (np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n), np.zeros(n), np.zeros((n, 3)))
+        elem = np.zeros(n, np.uint8)
+        L().o_gcmc_state(self.h, P(x), P(s), P(elem, u8p), P(rho), P(F), P(v))
+        return dict(x=x, s=s, elem=elem, rho=rho, F=F, v=v)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            L().o_gcmc_free(self.h)
             self.h = None
