@@ -157,6 +157,48 @@ mapp_status mapp_dmd_profile_read(mapp_dmd* d, const char* kernel, double* total
  * Gauss-Hermite nodes and weights, sum w = sqrt(pi) */
 mapp_status mapp_gauss_hermite(int n, double* x, double* w);
 
+/* ---------------- batched grand-canonical exchange (GCMC) ----------------
+ * Many independent periodic cells, each running its own chain of SGCMC
+ * insert/delete attempts of one gas element with EAM exchange energies:
+ * SGCMC::xchng's attempt loop (src/sgcmc.cpp:145-230) with SGCMC::attmpt
+ * (src/sgcmc.cpp:255-327) over ForceFieldEAM::init_xchng / pre_xchng_energy /
+ * xchng_energy / post_xchng_energy (src/ff_eam.cpp:323-578); constants of
+ * GCMC::init (src/gcmc.cpp:86-116). Cell c holds natms[c] atoms (x and elem
+ * concatenated over cells, H[c] a row-major lower-triangular 3x3) and room
+ * for cap[c] (cap NULL: natms + max(32, natms/8)); seeds[c] starts its
+ * Park-Miller stream (src/random.cpp:7-68). masses[ne]; only the gas mass
+ * enters (lambda, sigma). */
+typedef struct mapp_gcmc mapp_gcmc;
+mapp_status mapp_gcmc_create(mapp_eam* ff, int ncells, const int32_t* natms, const double* H, const double* x,
+                             const uint8_t* elem, const int32_t* cap, int gas_elem, double mu, double T, double kB,
+                             double hP, const double* masses, const int32_t* seeds, mapp_gcmc** out);
+void mapp_gcmc_destroy(mapp_gcmc* g);
+/* nattempts attempts in every cell, all cells in one persistent launch.
+ * decisions/dU are [ncells][nattempts] (either may be NULL): 1 accepted
+ * insertion, -1 rejected insertion, 2 accepted deletion, -2 rejected deletion,
+ * 0 deletion drawn with no gas atom (src/sgcmc.cpp:280-281; dU 0). An
+ * insertion accepted in a full cell stops that cell's chain and returns
+ * MAPP_ERR_CAPACITY. */
+mapp_status mapp_gcmc_attempts(mapp_gcmc* g, int nattempts, int8_t* decisions, double* dU);
+/* Tile tuner (no counterpart in the reference): per cell-size class
+ * ([0,256) [256,512) [512,1024) [1024,2048) [2048,inf) atoms), times the
+ * class's cells on a scratch copy with one wave and with four waves per
+ * cell and keeps the faster shape. waves[5] and ms[5][2] report the choice and
+ * the timings (0 where a shape was not timed). The chains are not advanced. */
+mapp_status mapp_gcmc_tune(mapp_gcmc* g, int nattempts, int32_t* waves, double* ms);
+/* sets the per-class shape directly (1 or 4 waves per cell); results do not depend on it */
+mapp_status mapp_gcmc_set_waves(mapp_gcmc* g, const int32_t* waves);
+/* on != 0: the attempt launch is captured once in a HIP graph and replayed */
+mapp_status mapp_gcmc_use_graph(mapp_gcmc* g, int on);
+mapp_status mapp_gcmc_counts(mapp_gcmc* g, int32_t* natms, int32_t* ngas);
+/* one cell's atoms in the reference's order (appended on insertion, last
+ * atom moved into the hole on deletion): Cartesian x [n][3], elem, rho, F
+ * (the exchange bookkeeping of ForceFieldEAM), v, and its stream's seed */
+mapp_status mapp_gcmc_get_cell(mapp_gcmc* g, int cell, double* x, uint8_t* elem, double* rho, double* F, double* v,
+                               int32_t* rng_seed);
+mapp_status mapp_gcmc_profile(mapp_gcmc* g, int enable);
+mapp_status mapp_gcmc_profile_read(mapp_gcmc* g, const char* kernel, double* total_ms, int64_t* launches);
+
 /* Diagnostic (no counterpart in the reference): copies `bytes` between two
  * device buffers `reps` times with a plain streaming kernel and reports the
  * achieved bandwidth (read + write bytes per second / 1e9). Used to calibrate

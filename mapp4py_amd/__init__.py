@@ -8,6 +8,9 @@ The Python layer is a thin mirror of the reference's objects over the C ABI
 * :class:`AtomsMD`        -- md.atoms with its ForceField, NeighborMD and DynamicMD
   (force_calc / value / update of src/ff_md.cpp:125-179, src/dynamic_md.h:38-68)
 * :class:`NVT`            -- md.nvt(T, dt).run(sim, nsteps) (src/md_nvt.cpp:383-480)
+* :class:`AtomsDMD`       -- dmd.atoms with ForceFieldEAMDMD (src/ff_eam_dmd.cpp, src/ff_dmd.cpp)
+* :class:`GCMCBatch`      -- many independent cells under SGCMC exchange (src/sgcmc.cpp,
+  ForceFieldEAM's exchange hooks at src/ff_eam.cpp:323-578)
 """
 from __future__ import annotations
 
@@ -17,7 +20,8 @@ import numpy as np
 
 from ._lib import MappError, check, f64, i8p, i32p, i64p, ip, lib, ptr, u8p
 
-__all__ = ["ForceFieldEAM", "AtomsMD", "AtomsDMD", "NVT", "MappError", "create_temp", "gauss_hermite", "lib"]
+__all__ = ["ForceFieldEAM", "AtomsMD", "AtomsDMD", "NVT", "GCMCBatch", "MappError", "create_temp", "gauss_hermite",
+           "lib"]
 
 
 class ForceFieldEAM:
@@ -301,3 +305,88 @@ def gauss_hermite(n):
     w = np.zeros(n)
     check(lib().mapp_gauss_hermite(int(n), ptr(x), ptr(w)))
     return x, w
+
+
+class GCMCBatch:
+    """Independent periodic cells, each with its own SGCMC chain of insert /
+    delete attempts of one gas element (md.muvt's exchange step,
+    src/sgcmc.cpp:145-327), all advanced together on the GPU.
+
+    cells: list of (x [n][3], H [3][3], elem [n]); seeds: one stream seed per
+    cell. Decisions: 1 / -1 accepted / rejected insertion, 2 / -2 deletion,
+    0 deletion drawn with no gas atom.
+    """
+
+    def __init__(self, ff: ForceFieldEAM, cells, gas, mu, T, masses, seeds, cap=None, kB=8.617330350e-5, hP=None):
+        from .potentials import HP
+
+        self.ff = ff
+        self.ncells = len(cells)
+        natms = np.array([len(e) for _, _, e in cells], dtype=np.int32)
+        x = f64(np.concatenate([np.asarray(c[0], dtype=np.float64).reshape(-1, 3) for c in cells]))
+        elem = np.ascontiguousarray(np.concatenate([np.asarray(c[2], dtype=np.uint8) for c in cells]))
+        H = f64(np.stack([np.asarray(c[1], dtype=np.float64).reshape(3, 3) for c in cells]))
+        seeds = np.ascontiguousarray(seeds, dtype=np.int32)
+        if len(seeds) != self.ncells:
+            raise ValueError("one seed per cell")
+        capp = None
+        if cap is not None:
+            cap = np.ascontiguousarray(cap, dtype=np.int32)
+            capp = ptr(cap, i32p)
+        self.masses = f64(masses)
+        h = C.c_void_p()
+        check(lib().mapp_gcmc_create(ff._h, self.ncells, ptr(natms, i32p), ptr(H), ptr(x), ptr(elem, u8p), capp,
+                                     int(gas), float(mu), float(T), float(kB), float(HP if hP is None else hP),
+                                     ptr(self.masses), ptr(seeds, i32p), C.byref(h)))
+        self._h = h
+
+    def attempts(self, k):
+        d = np.zeros((self.ncells, k), np.int8)
+        du = np.zeros((self.ncells, k))
+        check(lib().mapp_gcmc_attempts(self._h, int(k), ptr(d, i8p), ptr(du)))
+        return d, du
+
+    def run(self, k):
+        """k attempts per cell without copying decisions back"""
+        check(lib().mapp_gcmc_attempts(self._h, int(k), None, None))
+
+    def tune(self, k=16):
+        w = np.zeros(5, np.int32)
+        ms = np.zeros((5, 2))
+        check(lib().mapp_gcmc_tune(self._h, int(k), ptr(w, i32p), ptr(ms)))
+        return w, ms
+
+    def set_waves(self, waves):
+        w = np.ascontiguousarray(waves, dtype=np.int32)
+        check(lib().mapp_gcmc_set_waves(self._h, ptr(w, i32p)))
+
+    def use_graph(self, on=True):
+        check(lib().mapp_gcmc_use_graph(self._h, int(on)))
+
+    def counts(self):
+        n = np.zeros(self.ncells, np.int32)
+        g = np.zeros(self.ncells, np.int32)
+        check(lib().mapp_gcmc_counts(self._h, ptr(n, i32p), ptr(g, i32p)))
+        return n, g
+
+    def cell(self, c):
+        n = int(self.counts()[0][c])
+        x, rho, F, v = np.zeros((n, 3)), np.zeros(n), np.zeros(n), np.zeros((n, 3))
+        elem = np.zeros(n, np.uint8)
+        seed = C.c_int32()
+        check(lib().mapp_gcmc_get_cell(self._h, int(c), ptr(x), ptr(elem, u8p), ptr(rho), ptr(F), ptr(v),
+                                       C.byref(seed)))
+        return dict(x=x, elem=elem, rho=rho, F=F, v=v, seed=seed.value)
+
+    def profile(self, enable=True):
+        check(lib().mapp_gcmc_profile(self._h, int(enable)))
+
+    def profile_read(self, kernel="gcmc"):
+        ms, n = C.c_double(), C.c_int64()
+        check(lib().mapp_gcmc_profile_read(self._h, kernel.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mapp_gcmc_destroy(self._h)
+            self._h = None

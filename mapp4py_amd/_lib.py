@@ -68,6 +68,17 @@ SIGNATURES = {
     "mapp_dmd_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "mapp_dmd_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, dp, i64p]),
     "mapp_gauss_hermite": (C.c_int, [C.c_int, dp, dp]),
+    "mapp_gcmc_create": (C.c_int, [C.c_void_p, C.c_int, i32p, dp, dp, u8p, i32p, C.c_int, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, dp, i32p, vpp]),
+    "mapp_gcmc_destroy": (None, [C.c_void_p]),
+    "mapp_gcmc_attempts": (C.c_int, [C.c_void_p, C.c_int, i8p, dp]),
+    "mapp_gcmc_tune": (C.c_int, [C.c_void_p, C.c_int, i32p, dp]),
+    "mapp_gcmc_set_waves": (C.c_int, [C.c_void_p, i32p]),
+    "mapp_gcmc_use_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "mapp_gcmc_counts": (C.c_int, [C.c_void_p, i32p, i32p]),
+    "mapp_gcmc_get_cell": (C.c_int, [C.c_void_p, C.c_int, dp, u8p, dp, dp, dp, i32p]),
+    "mapp_gcmc_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "mapp_gcmc_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, dp, i64p]),
     "mapp_diag_copy": (C.c_int, [C.c_size_t, C.c_int, dp]),
     "mapp_create_T": (C.c_int, [C.c_int, u8p, dp, C.c_double, C.c_double, C.c_int, dp]),
 }

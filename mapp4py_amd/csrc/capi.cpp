@@ -6,6 +6,7 @@
 
 #include "../../include/mapp_amd.h"
 #include "dmd.hpp"
+#include "gcmc.hpp"
 #include "md.hpp"
 #include "rng.hpp"
 
@@ -24,6 +25,9 @@ struct mapp_md {
 };
 struct mapp_dmd {
     std::unique_ptr<mapp::DMDSystem> sys;
+};
+struct mapp_gcmc {
+    std::unique_ptr<mapp::GcmcBatch> sys;
 };
 
 namespace {
@@ -389,6 +393,106 @@ mapp_status mapp_create_T(int natms, const uint8_t* elem, const double* masses, 
             const double fac = std::sqrt(kB * T / masses[elem[i]]);
             for (int d = 0; d < 3; ++d) v[3 * (size_t)i + d] = rng.gaussian() * fac;
         }
+    });
+}
+
+
+/* ---------------- batched grand-canonical exchange ---------------- */
+mapp_status mapp_gcmc_create(mapp_eam* ff, int ncells, const int32_t* natms, const double* H, const double* x,
+                             const uint8_t* elem, const int32_t* cap, int gas_elem, double mu, double T, double kB,
+                             double hP, const double* masses, const int32_t* seeds, mapp_gcmc** out)
+{
+    return guard([&] {
+        NEED(ff);
+        NEED(natms);
+        NEED(H);
+        NEED(masses);
+        NEED(seeds);
+        NEED(out);
+        *out = nullptr;
+        int64_t tot = 0;
+        for (int c = 0; c < ncells; ++c) tot += natms[c];
+        MAPP_REQUIRE(tot == 0 || (x && elem), mapp::ERR_ARG, "x and elem must not be NULL");
+        auto g = std::make_unique<mapp_gcmc>();
+        g->sys = std::make_unique<mapp::GcmcBatch>(ff->ff, ncells, natms, H, x, elem, cap, gas_elem, mu, T, kB, hP,
+                                                   masses, seeds);
+        *out = g.release();
+    });
+}
+
+void mapp_gcmc_destroy(mapp_gcmc* g) { delete g; }
+
+mapp_status mapp_gcmc_attempts(mapp_gcmc* g, int nattempts, int8_t* decisions, double* dU)
+{
+    return guard([&] { NEED(g); g->sys->attempts(nattempts, decisions, dU); });
+}
+
+mapp_status mapp_gcmc_tune(mapp_gcmc* g, int nattempts, int32_t* waves, double* ms)
+{
+    return guard([&] {
+        NEED(g);
+        const mapp::GcmcTuneReport r = g->sys->tune(nattempts);
+        for (int k = 0; k < mapp::kGcmcClasses; ++k) {
+            if (waves) waves[k] = r.waves[k];
+            if (ms) {
+                ms[2 * k] = r.ms[k][0];
+                ms[2 * k + 1] = r.ms[k][1];
+            }
+        }
+    });
+}
+
+mapp_status mapp_gcmc_set_waves(mapp_gcmc* g, const int32_t* waves)
+{
+    return guard([&] {
+        NEED(g);
+        NEED(waves);
+        int w[mapp::kGcmcClasses];
+        for (int k = 0; k < mapp::kGcmcClasses; ++k) w[k] = waves[k];
+        g->sys->set_waves(w);
+    });
+}
+
+mapp_status mapp_gcmc_use_graph(mapp_gcmc* g, int on)
+{
+    return guard([&] { NEED(g); g->sys->use_graph(on != 0); });
+}
+
+mapp_status mapp_gcmc_counts(mapp_gcmc* g, int32_t* natms, int32_t* ngas)
+{
+    return guard([&] { NEED(g); g->sys->counts(natms, ngas); });
+}
+
+mapp_status mapp_gcmc_get_cell(mapp_gcmc* g, int cell, double* x, uint8_t* elem, double* rho, double* F, double* v,
+                               int32_t* rng_seed)
+{
+    return guard([&] { NEED(g); g->sys->get_cell(cell, x, elem, rho, F, v, rng_seed); });
+}
+
+mapp_status mapp_gcmc_profile(mapp_gcmc* g, int enable)
+{
+    return guard([&] {
+        NEED(g);
+        auto& p = g->sys->profiler();
+        p.reset();
+        p.on = enable != 0;
+    });
+}
+
+mapp_status mapp_gcmc_profile_read(mapp_gcmc* g, const char* kernel, double* total_ms, int64_t* launches)
+{
+    return guard([&] {
+        NEED(g);
+        NEED(kernel);
+        auto& p = g->sys->profiler();
+        p.flush();
+        for (int k = 0; k < mapp::K_COUNT; ++k)
+            if (std::strcmp(kernel, mapp::kernel_name(k)) == 0) {
+                if (total_ms) *total_ms = p.total_ms[k];
+                if (launches) *launches = p.launches[k];
+                return;
+            }
+        throw mapp::Error(mapp::ERR_ARG, std::string("unknown kernel group ") + kernel);
     });
 }
 
