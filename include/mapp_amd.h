@@ -180,11 +180,13 @@ void mapp_gcmc_destroy(mapp_gcmc* g);
  * insertion accepted in a full cell stops that cell's chain and returns
  * MAPP_ERR_CAPACITY. */
 mapp_status mapp_gcmc_attempts(mapp_gcmc* g, int nattempts, int8_t* decisions, double* dU);
-/* Tile tuner (no counterpart in the reference): per cell-size class
- * ([0,256) [256,512) [512,1024) [1024,2048) [2048,inf) atoms), times the
- * class's cells on a scratch copy with one wave and with four waves per
- * cell and keeps the faster shape. waves[5] and ms[5][2] report the choice and
- * the timings (0 where a shape was not timed). The chains are not advanced. */
+/* Tile tuner (no counterpart in the reference): cells of the size classes
+ * [0,256) [256,512) [512,1024) [1024,2048) [2048,inf) atoms run on one wave
+ * below a threshold class and on four waves (one workgroup) from it up. Each
+ * threshold 0..5 is timed with nattempts attempts on a scratch copy of the
+ * whole batch and the fastest is kept; the chains are not advanced.
+ * waves[5] reports the chosen shape per class, ms[6] the launch time of each
+ * threshold. */
 mapp_status mapp_gcmc_tune(mapp_gcmc* g, int nattempts, int32_t* waves, double* ms);
 /* sets the per-class shape directly (1 or 4 waves per cell); results do not depend on it */
 mapp_status mapp_gcmc_set_waves(mapp_gcmc* g, const int32_t* waves);

@@ -352,7 +352,7 @@ class GCMCBatch:
 
     def tune(self, k=16):
         w = np.zeros(5, np.int32)
-        ms = np.zeros((5, 2))
+        ms = np.zeros(6)
         check(lib().mapp_gcmc_tune(self._h, int(k), ptr(w, i32p), ptr(ms)))
         return w, ms
 

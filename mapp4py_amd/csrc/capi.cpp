@@ -432,13 +432,10 @@ mapp_status mapp_gcmc_tune(mapp_gcmc* g, int nattempts, int32_t* waves, double* 
     return guard([&] {
         NEED(g);
         const mapp::GcmcTuneReport r = g->sys->tune(nattempts);
-        for (int k = 0; k < mapp::kGcmcClasses; ++k) {
+        for (int k = 0; k < mapp::kGcmcClasses; ++k)
             if (waves) waves[k] = r.waves[k];
-            if (ms) {
-                ms[2 * k] = r.ms[k][0];
-                ms[2 * k + 1] = r.ms[k][1];
-            }
-        }
+        for (int k = 0; k <= mapp::kGcmcClasses; ++k)
+            if (ms) ms[k] = r.ms[k];
     });
 }
 

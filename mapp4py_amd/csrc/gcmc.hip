@@ -667,16 +667,23 @@ void GcmcBatch::set_waves(const int* waves)
 }
 
 // Cells of each shape, largest first (longest chains start first).
-void GcmcBatch::build_queues()
+void GcmcBatch::split_cells(const int* waves, std::vector<int>& q4, std::vector<int>& q1) const
 {
     std::vector<int> order(nc_);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n0_[a] > n0_[b]; });
-    std::vector<int> q4, q1;
+    q4.clear();
+    q1.clear();
     for (int c : order) {
-        const int wv = host_cells_[c].cap > kGcmcWaveCap ? 4 : waves_[host_cells_[c].cls];
+        const int wv = host_cells_[c].cap > kGcmcWaveCap ? 4 : waves[host_cells_[c].cls];
         (wv == 4 ? q4 : q1).push_back(c);
     }
+}
+
+void GcmcBatch::build_queues()
+{
+    std::vector<int> q4, q1;
+    split_cells(waves_, q4, q1);
     n4_ = (int)q4.size();
     n1_ = (int)q1.size();
     q4_.reserve(std::max(1, n4_));
@@ -775,10 +782,10 @@ void GcmcBatch::copy_state(GcmcState& dst, const GcmcState& src)
     MAPP_HIP(hipMemcpyAsync(dst.el.get(), src.el.get(), total_cap_, hipMemcpyDeviceToDevice, stream_));
 }
 
-// Tile tuner: for every size class, time the class's cells alone with one
-// wave per cell and with four, on a scratch copy of the batch (the
-// production chains and their random streams are not advanced), and keep
-// the faster shape.
+// Tile tuner: the launch shape is one wave per cell for the small size
+// classes and four waves per cell from a threshold class up. Every threshold
+// is timed on the whole batch (a scratch copy, so the production chains and
+// their random streams are not advanced) and the fastest is kept.
 GcmcTuneReport GcmcBatch::tune(int nattempts)
 {
     MAPP_REQUIRE(nattempts > 0, ERR_ARG, "nattempts must be positive");
@@ -786,41 +793,34 @@ GcmcTuneReport GcmcBatch::tune(int nattempts)
     hipEvent_t e0, e1;
     MAPP_HIP(hipEventCreate(&e0));
     MAPP_HIP(hipEventCreate(&e1));
-    std::vector<int> cls_cells[kGcmcClasses];
-    for (int c = 0; c < nc_; ++c) cls_cells[host_cells_[c].cls].push_back(c);
-    DevBuf<int> q;
-    q.reserve(nc_);
-    for (int k = 0; k < kGcmcClasses; ++k) {
-        rep.waves[k] = waves_[k];
-        auto& v = cls_cells[k];
-        if (v.empty()) continue;
-        std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return n0_[a] > n0_[b]; });
-        bool wave_ok = true;
-        for (int c : v) wave_ok = wave_ok && host_cells_[c].cap <= kGcmcWaveCap;
-        MAPP_HIP(hipMemcpy(q.get(), v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
-        double best = 1e300;
-        for (int shape = 0; shape < 2; ++shape) {
-            const int wv = shape == 0 ? 1 : 4;
-            if (wv == 1 && !wave_ok) continue;
-            copy_state(scratch_, st_);
-            MAPP_HIP(hipEventRecord(e0, stream_));
-            if (wv == 4)
-                launch(scratch_, nattempts, nullptr, nullptr, q.get(), (int)v.size(), q.get(), 0);
-            else
-                launch(scratch_, nattempts, nullptr, nullptr, q.get(), 0, q.get(), (int)v.size());
-            MAPP_HIP(hipEventRecord(e1, stream_));
-            MAPP_HIP(hipEventSynchronize(e1));
-            float ms = 0.f;
-            MAPP_HIP(hipEventElapsedTime(&ms, e0, e1));
-            rep.ms[k][shape] = ms;
-            if (ms < best) {
-                best = ms;
-                rep.waves[k] = wv;
-            }
+    DevBuf<int> d4, d1;
+    d4.reserve(nc_);
+    d1.reserve(nc_);
+    double best = 1e300;
+    int best_t = kGcmcClasses;
+    std::vector<int> q4, q1;
+    for (int th = 0; th <= kGcmcClasses; ++th) {
+        int wv[kGcmcClasses];
+        for (int k = 0; k < kGcmcClasses; ++k) wv[k] = k < th ? 1 : 4;
+        split_cells(wv, q4, q1);
+        if (!q4.empty()) MAPP_HIP(hipMemcpy(d4.get(), q4.data(), q4.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!q1.empty()) MAPP_HIP(hipMemcpy(d1.get(), q1.data(), q1.size() * sizeof(int), hipMemcpyHostToDevice));
+        copy_state(scratch_, st_);
+        MAPP_HIP(hipEventRecord(e0, stream_));
+        launch(scratch_, nattempts, nullptr, nullptr, d4.get(), (int)q4.size(), d1.get(), (int)q1.size());
+        MAPP_HIP(hipEventRecord(e1, stream_));
+        MAPP_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        MAPP_HIP(hipEventElapsedTime(&ms, e0, e1));
+        rep.ms[th] = ms;
+        if (ms < best) {
+            best = ms;
+            best_t = th;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    for (int k = 0; k < kGcmcClasses; ++k) rep.waves[k] = k < best_t ? 1 : 4;
     set_waves(rep.waves);
     return rep;
 }

@@ -50,7 +50,7 @@ struct GcmcState {
 
 struct GcmcTuneReport {
     int waves[kGcmcClasses];
-    double ms[kGcmcClasses][2];  // launch time with 1 and 4 waves per cell (0 when not timed)
+    double ms[kGcmcClasses + 1];  // whole-batch launch time for threshold class 0..5
 };
 
 class GcmcBatch {
@@ -71,6 +71,7 @@ public:
     Profiler& profiler() { return prof_; }
 
 private:
+    void split_cells(const int* waves, std::vector<int>& q4, std::vector<int>& q1) const;
     void build_queues();
     void launch(GcmcState& st, int nattempts, int8_t* dec, double* du, const int* q4, int n4, const int* q1, int n1);
     void copy_state(GcmcState& dst, const GcmcState& src);
