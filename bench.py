@@ -1,4 +1,6 @@
-"""Benchmark: EAM NVT atom-steps/s on one MI355X.
+"""Benchmark: EAM NVT atom-steps/s on one MI355X (the headline value), with
+the DMD free-energy evaluation rate and the batched GCMC attempt rate
+reported beside it.
 
 Workload (the metric's "8M atoms" case): fcc Cu, a = 3.615 A, 128^3 cells =
 8,388,608 atoms, Johnson-form EAM tabulated to setfl with Nr = Nrho = 10000
@@ -100,6 +102,113 @@ def cpu_baseline(seconds):
                       f"at 1 fs, same table, 1 thread, {dt:.1f} s"}
 
 
+DMD_BYTES_PER_SITE = 132  # x 16, alpha 8, c 16 (read twice), f 12, f_alpha 8, mu 16, E/dE write + read 32
+
+
+def dmd_run(evals=10):
+    """Cu-H DMD, 262,144 sites (fcc 32x32x64 cells), two components per site,
+    N = 3 Gauss-Hermite points, r_c = 5.5 A: full free-energy + gradient
+    evaluations (ForceFieldDMD::derivative) per second."""
+    import mapp4py_amd as M
+    from mapp4py_amd import potentials as pot
+
+    p = pot.johnson_cuh(nr=5000, nrho=5000, rc=5.5)
+    x, H, elem, alpha, c = pot.dmd_cuh_sites((32, 32, 64), seed=4)
+    x = x + np.random.default_rng(5).normal(0, 0.05, x.shape)
+    d = M.AtomsDMD(M.ForceFieldEAM.from_knots(p, dmd=True), H, x, elem, alpha, c, nquad=3, masses=p.masses)
+    d.derivative()
+    d.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(evals):
+        d.derivative()
+    dt = time.perf_counter() - t0
+    k1, k2 = d.profile_read("dmd_pass1"), d.profile_read("dmd_pass2")
+    d.profile(False)
+    n = len(x)
+    ms = dt / evals * 1e3
+    return {"workload": "DMD Cu-H, 262,144 fcc sites x 2 components, N=3 Gauss-Hermite, r_c=5.5 A, "
+                        "alpha log-uniform [0.05,0.30] A, c_H ~ Beta(2,5)",
+            "metric": "DMD site-evals/s (free energy + gradient)", "value": n * evals / dt, "unit": "site-evals/s",
+            "ms_per_eval": ms, "pass1_ms": k1[0] / max(k1[1], 1), "pass2_ms": k2[0] / max(k2[1], 1),
+            "dtype": "f64", "half_pairs": int(d.half_pairs()[0].shape[0]),
+            "hbm_fraction": DMD_BYTES_PER_SITE * n / (ms / 1e3) / HBM_PEAK}
+
+
+def dmd_cpu_baseline(seconds):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from mapp4py_amd import potentials as pot
+
+    p = pot.johnson_cuh(nr=5000, nrho=5000, rc=5.5)
+    x, H, elem, alpha, c = pot.dmd_cuh_sites((4, 4, 8), seed=4)
+    x = x + np.random.default_rng(5).normal(0, 0.05, x.shape)
+    o = O.ODMD(O.OEam.from_knots(p, dmd=True), H, x, elem, alpha, c, nquad=3, masses=p.masses)
+    t0 = time.perf_counter()
+    o.force()
+    per = time.perf_counter() - t0
+    k = max(2, int(seconds / max(per, 1e-6)))
+    t0 = time.perf_counter()
+    for _ in range(k):
+        o.force()
+    dt = time.perf_counter() - t0
+    return {"value": len(x) * k / dt, "unit": "site-evals/s", "cores": 1, "kind": "port",
+            "sample": f"oracle, same table and site statistics, fcc 4x4x8 cells = {len(x)} sites, {k} evaluations, "
+                      f"{dt:.1f} s"}
+
+
+def gcmc_run(launches=10, attempts=64):
+    """4096 periodic Ni-H cells (108-4000 atoms, log-uniform), NiAlH_jea EAM,
+    H exchange at mu = -2.3 eV and 300 K: one persistent HIP-graph launch per
+    64 attempts in every cell, tile shape from the tuner."""
+    import mapp4py_amd as M
+    from mapp4py_amd import potentials as pot
+
+    cells = pot.gcmc_nih_cells(4096, seed=6)
+    masses = [pot.MASS["Ni"], pot.MASS["Al"], pot.MASS["H"]]
+    gb = M.GCMCBatch(M.ForceFieldEAM.setfl(pot.NIALH_JEA), cells, 2, -2.3, 300.0, masses,
+                     np.arange(4096, dtype=np.int32) + 1000)
+    waves, _ = gb.tune(attempts)
+    gb.use_graph(True)
+    gb.run(attempts)
+    gb.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(launches):
+        gb.run(attempts)
+    dt = time.perf_counter() - t0
+    ms_k, nk = gb.profile_read("gcmc")
+    gb.profile(False)
+    n, ng = gb.counts()
+    total = 4096 * attempts * launches
+    return {"workload": "4096 periodic Ni-H cells, 108-4000 atoms log-uniform, NiAlH_jea EAM, mu=-2.3 eV, 300 K",
+            "metric": "GCMC trial-move energy deltas/s", "value": total / dt, "unit": "attempts/s",
+            "cells_per_s": 4096 * launches / dt, "ms_per_launch": ms_k / max(nk, 1), "attempts_per_launch": attempts,
+            "waves_per_class": [int(w) for w in waves], "atoms": int(n.sum()), "gas_atoms": int(ng.sum()),
+            "dtype": "f64"}
+
+
+def gcmc_cpu_baseline(seconds):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from mapp4py_amd import potentials as pot
+
+    cells = pot.gcmc_nih_cells(4096, seed=6)
+    masses = [pot.MASS["Ni"], pot.MASS["Al"], pot.MASS["H"]]
+    ff = O.OEam.read(pot.NIALH_JEA)
+    done, dt, atoms = 0, 0.0, 0
+    for c, (x, H, e) in enumerate(cells):
+        g = O.OGCMC(ff, H, x, e, 2, -2.3, 300.0, masses, seed=1000 + c)
+        t0 = time.perf_counter()
+        g.attempts(512)
+        dt += time.perf_counter() - t0
+        done += 512
+        atoms += len(e)
+        if dt > seconds:
+            break
+    return {"value": done / dt, "unit": "attempts/s", "cores": 1, "kind": "port",
+            "sample": f"oracle SGCMC, first {c + 1} cells of the same batch ({atoms} atoms), 512 attempts each, "
+                      f"attempts only, {dt:.1f} s"}
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "eam_force_traffic.json")
     if not os.path.exists(path):
@@ -116,6 +225,8 @@ def main():
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-dmd", action="store_true", help="skip the DMD measurement")
+    ap.add_argument("--no-gcmc", action="store_true", help="skip the batched GCMC measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel (256 MiB, 20 launches) used to calibrate HBM counters")
     args = ap.parse_args()
@@ -147,15 +258,28 @@ def main():
         log(f"[bench] cpu baseline (~{args.cpu_seconds:.0f} s of oracle work)")
         cpu = cpu_baseline(args.cpu_seconds)
         log(f"[bench] cpu baseline {cpu['value']:.3e} atom-steps/s")
+    dmd = gcmc = None
+    if not args.no_dmd:
+        log("[bench] DMD")
+        dmd = dmd_run()
+        if not args.no_cpu:
+            dmd["cpu_baseline"] = dmd_cpu_baseline(args.cpu_seconds / 3)
+        log(f"[bench] DMD {dmd['value']:.3e} site-evals/s")
+    if not args.no_gcmc:
+        log("[bench] GCMC")
+        gcmc = gcmc_run()
+        if not args.no_cpu:
+            gcmc["cpu_baseline"] = gcmc_cpu_baseline(args.cpu_seconds / 3)
+        log(f"[bench] GCMC {gcmc['value']:.3e} attempts/s")
     line = {
-        "metric": "EAM atom-steps/s @8M atoms", "value": value, "unit": "atom-steps/s", "n_gpus": 1,
+        "metric": "EAM atom-steps/s @8M atoms; DMD sites/s surrogate vs quadrature @1e-4 eV/A", "value": value, "unit": "atom-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic (Johnson-form Cu EAM table, jittered fcc lattice, Maxwell velocities)",
         "config": {"workload": f"EAM NVT, fcc Cu {args.cells}^3 cells ({n} atoms), Johnson setfl Nr=Nrho=10000, "
                                f"r_c=4.95 A, skin 0.5 A, dt 1 fs, 300 K", "natoms": n, "rebuilds": nvt.rebuilds,
                    "pe_per_atom_eV": out[0] / n, "T_K": nvt.T_part},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "dmd": dmd, "gcmc": gcmc,
     }
     print(json.dumps(line), flush=True)
 
