@@ -157,6 +157,42 @@ mapp_status mapp_dmd_profile_read(mapp_dmd* d, const char* kernel, double* total
  * Gauss-Hermite nodes and weights, sum w = sqrt(pi) */
 mapp_status mapp_gauss_hermite(int n, double* x, double* w);
 
+/* ---------------- learned surrogate of the smeared functions ----------------
+ * No counterpart in the reference: one MLP per ordered element pair (a, b),
+ * 3 -> 64 -> 64 -> 6 with tanh, maps (r, alpha_a, alpha_b) to what
+ * ForceFieldEAMDMD's quadrature computes for that pair
+ * (src/ff_eam_dmd.cpp:195-253): (g_phi, g_rho_ab, dg_phi/dr, dg_rho/dr,
+ * dg_phi/dalpha, dg_rho/dalpha), alpha = sqrt(alpha_a^2 + alpha_b^2).
+ * box4 = r_lo, r_hi, alpha_lo, alpha_hi: the training box; pairs outside it
+ * keep the quadrature. Training and inference run on BF16 MFMA with FP32
+ * accumulation. Network index = a * ne + b. */
+typedef struct mapp_surr mapp_surr;
+mapp_status mapp_surr_create(int ne, const float* box4, int seed, mapp_surr** out);
+void mapp_surr_destroy(mapp_surr* s);
+/* nsamples quadrature samples per network from d's force field and rule,
+ * then `steps` Adam steps of `batch` samples per network (cosine rate lr ->
+ * lr/20). loss2 = mean squared error of the normalized outputs at the first
+ * and the last step. */
+mapp_status mapp_surr_train(mapp_surr* s, mapp_dmd* d, int nsamples, int steps, int batch, double lr,
+                            double* loss2);
+/* MSE loss and its gradient (flat parameter order W1[64][4] (r, alpha_a,
+ * alpha_b, bias), W2[64][64], b2[64], W3[6][64], b3[6]) for normalized inputs
+ * x[n][3] and targets t[n][6], n a multiple of 64, through the training kernel */
+mapp_status mapp_surr_loss_grad(mapp_surr* s, int net, int n, const float* x, const float* t, float* grad,
+                                double* loss);
+mapp_status mapp_surr_get_params(mapp_surr* s, int net, float* p);
+mapp_status mapp_surr_set_params(mapp_surr* s, int net, const float* p);
+/* output de-normalization y = net * scale + shift, per output */
+mapp_status mapp_surr_get_norm(mapp_surr* s, int net, float* scale, float* shift);
+mapp_status mapp_surr_set_norm(mapp_surr* s, int net, const float* scale, const float* shift);
+/* outputs [n][6] for raw inputs rin[n][3] = (r, alpha_a, alpha_b), inference kernel */
+mapp_status mapp_surr_infer(mapp_surr* s, int net, int n, const float* rin, float* out);
+/* DMD evaluations use s (NULL: the quadrature again). Kernel group "surr_infer"
+ * of mapp_dmd_profile_read times the request staging and inference. */
+mapp_status mapp_dmd_use_surrogate(mapp_dmd* d, mapp_surr* s);
+/* network evaluations and quadrature-covered slots of the last evaluation */
+mapp_status mapp_dmd_surrogate_coverage(mapp_dmd* d, int64_t* net_evals, int64_t* quadrature_slots);
+
 /* ---------------- batched grand-canonical exchange (GCMC) ----------------
  * Many independent periodic cells, each running its own chain of SGCMC
  * insert/delete attempts of one gas element with EAM exchange energies:

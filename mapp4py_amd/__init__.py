@@ -18,10 +18,10 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import MappError, check, f64, i8p, i32p, i64p, ip, lib, ptr, u8p
+from ._lib import MappError, check, f64, fp, i8p, i32p, i64p, ip, lib, ptr, u8p
 
-__all__ = ["ForceFieldEAM", "AtomsMD", "AtomsDMD", "NVT", "GCMCBatch", "MappError", "create_temp", "gauss_hermite",
-           "lib"]
+__all__ = ["ForceFieldEAM", "AtomsMD", "AtomsDMD", "NVT", "GCMCBatch", "Surrogate", "MappError", "create_temp",
+           "gauss_hermite", "lib"]
 
 
 class ForceFieldEAM:
@@ -286,6 +286,17 @@ class AtomsDMD:
         return {"iters": it.value, "status": st.value, "fe0": out[0], "fe": out[1], "grad_norm": out[2],
                 "grad_max": out[3]}
 
+    def use_surrogate(self, s=None):
+        """Evaluate with the networks of s inside their training box (None: quadrature only)."""
+        self._surr = s
+        check(lib().mapp_dmd_use_surrogate(self._h, s._h if s is not None else None))
+
+    def surrogate_coverage(self):
+        """(network evaluations, slots left to the quadrature) of the last evaluation"""
+        a, b = C.c_int64(), C.c_int64()
+        check(lib().mapp_dmd_surrogate_coverage(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def profile(self, enable=True):
         check(lib().mapp_dmd_profile(self._h, int(enable)))
 
@@ -389,4 +400,65 @@ class GCMCBatch:
     def __del__(self):
         if getattr(self, "_h", None):
             lib().mapp_gcmc_destroy(self._h)
+            self._h = None
+
+
+class Surrogate:
+    """Per-element-pair MLPs (3 -> 64 -> 64 -> 6, tanh) standing in for the
+    DMD Gauss-Hermite smearing; trained on the GPU from the quadrature of an
+    AtomsDMD's force field. box = (r_lo, r_hi, alpha_lo, alpha_hi)."""
+
+    NPARAMS = 4806
+
+    def __init__(self, ne, box=(1.8, 6.7, 0.05, 0.30), seed=1):
+        self.ne = ne
+        self.box = tuple(float(v) for v in box)
+        b = np.ascontiguousarray(self.box, dtype=np.float32)
+        h = C.c_void_p()
+        check(lib().mapp_surr_create(int(ne), ptr(b, fp), int(seed), C.byref(h)))
+        self._h = h
+
+    def train(self, dmd, nsamples=2_000_000, steps=4000, batch=4096, lr=2e-3):
+        out = np.zeros(2)
+        check(lib().mapp_surr_train(self._h, dmd._h, int(nsamples), int(steps), int(batch), float(lr), ptr(out)))
+        return out[0], out[1]
+
+    def loss_grad(self, net, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.NPARAMS, np.float32)
+        loss = C.c_double()
+        check(lib().mapp_surr_loss_grad(self._h, int(net), len(x), ptr(x, fp), ptr(t, fp), ptr(g, fp),
+                                        C.byref(loss)))
+        return loss.value, g
+
+    def params(self, net):
+        p = np.zeros(self.NPARAMS, np.float32)
+        check(lib().mapp_surr_get_params(self._h, int(net), ptr(p, fp)))
+        return p
+
+    def set_params(self, net, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(lib().mapp_surr_set_params(self._h, int(net), ptr(p, fp)))
+
+    def norm(self, net):
+        sc, sh = np.zeros(6, np.float32), np.zeros(6, np.float32)
+        check(lib().mapp_surr_get_norm(self._h, int(net), ptr(sc, fp), ptr(sh, fp)))
+        return sc, sh
+
+    def set_norm(self, net, scale, shift):
+        sc = np.ascontiguousarray(scale, dtype=np.float32)
+        sh = np.ascontiguousarray(shift, dtype=np.float32)
+        check(lib().mapp_surr_set_norm(self._h, int(net), ptr(sc, fp), ptr(sh, fp)))
+
+    def infer(self, net, rin):
+        """outputs [n][6] for raw inputs rin[n][3] = (r, alpha_a, alpha_b)"""
+        rin = np.ascontiguousarray(rin, dtype=np.float32).reshape(-1, 3)
+        out = np.zeros((len(rin), 6), np.float32)
+        check(lib().mapp_surr_infer(self._h, int(net), len(rin), ptr(rin, fp), ptr(out, fp)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mapp_surr_destroy(self._h)
             self._h = None

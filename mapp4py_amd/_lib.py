@@ -68,6 +68,17 @@ SIGNATURES = {
     "mapp_dmd_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "mapp_dmd_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, dp, i64p]),
     "mapp_gauss_hermite": (C.c_int, [C.c_int, dp, dp]),
+    "mapp_surr_create": (C.c_int, [C.c_int, fp, C.c_int, vpp]),
+    "mapp_surr_destroy": (None, [C.c_void_p]),
+    "mapp_surr_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, dp]),
+    "mapp_surr_loss_grad": (C.c_int, [C.c_void_p, C.c_int, C.c_int, fp, fp, fp, dp]),
+    "mapp_surr_get_params": (C.c_int, [C.c_void_p, C.c_int, fp]),
+    "mapp_surr_set_params": (C.c_int, [C.c_void_p, C.c_int, fp]),
+    "mapp_surr_get_norm": (C.c_int, [C.c_void_p, C.c_int, fp, fp]),
+    "mapp_surr_set_norm": (C.c_int, [C.c_void_p, C.c_int, fp, fp]),
+    "mapp_surr_infer": (C.c_int, [C.c_void_p, C.c_int, C.c_int, fp, fp]),
+    "mapp_dmd_use_surrogate": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mapp_dmd_surrogate_coverage": (C.c_int, [C.c_void_p, i64p, i64p]),
     "mapp_gcmc_create": (C.c_int, [C.c_void_p, C.c_int, i32p, dp, dp, u8p, i32p, C.c_int, C.c_double, C.c_double,
                                    C.c_double, C.c_double, dp, i32p, vpp]),
     "mapp_gcmc_destroy": (None, [C.c_void_p]),

@@ -7,6 +7,7 @@
 #include "../../include/mapp_amd.h"
 #include "dmd.hpp"
 #include "gcmc.hpp"
+#include "surrogate.hpp"
 #include "md.hpp"
 #include "rng.hpp"
 
@@ -25,6 +26,9 @@ struct mapp_md {
 };
 struct mapp_dmd {
     std::unique_ptr<mapp::DMDSystem> sys;
+};
+struct mapp_surr {
+    std::shared_ptr<mapp::Surrogate> sys;
 };
 struct mapp_gcmc {
     std::unique_ptr<mapp::GcmcBatch> sys;
@@ -491,6 +495,83 @@ mapp_status mapp_gcmc_profile_read(mapp_gcmc* g, const char* kernel, double* tot
             }
         throw mapp::Error(mapp::ERR_ARG, std::string("unknown kernel group ") + kernel);
     });
+}
+
+/* ---------------- surrogate ---------------- */
+mapp_status mapp_surr_create(int ne, const float* box4, int seed, mapp_surr** out)
+{
+    return guard([&] {
+        NEED(box4);
+        NEED(out);
+        *out = nullptr;
+        auto s = std::make_unique<mapp_surr>();
+        s->sys = std::make_shared<mapp::Surrogate>(ne, mapp::SurrBox{box4[0], box4[1], box4[2], box4[3]}, seed);
+        *out = s.release();
+    });
+}
+
+void mapp_surr_destroy(mapp_surr* s) { delete s; }
+
+mapp_status mapp_surr_train(mapp_surr* s, mapp_dmd* d, int nsamples, int steps, int batch, double lr,
+                            double* loss2)
+{
+    return guard([&] {
+        NEED(s);
+        NEED(d);
+        const mapp::SurrTrainReport r = s->sys->train(*d->sys, nsamples, steps, batch, lr);
+        if (loss2) {
+            loss2[0] = r.loss0;
+            loss2[1] = r.loss;
+        }
+    });
+}
+
+mapp_status mapp_surr_loss_grad(mapp_surr* s, int net, int n, const float* x, const float* t, float* grad,
+                                double* loss)
+{
+    return guard([&] {
+        NEED(s);
+        NEED(x);
+        NEED(t);
+        NEED(grad);
+        const double l = s->sys->loss_grad(net, n, x, t, grad);
+        if (loss) *loss = l;
+    });
+}
+
+mapp_status mapp_surr_get_params(mapp_surr* s, int net, float* p)
+{
+    return guard([&] { NEED(s); NEED(p); s->sys->get_params(net, p); });
+}
+
+mapp_status mapp_surr_set_params(mapp_surr* s, int net, const float* p)
+{
+    return guard([&] { NEED(s); NEED(p); s->sys->set_params(net, p); });
+}
+
+mapp_status mapp_surr_get_norm(mapp_surr* s, int net, float* scale, float* shift)
+{
+    return guard([&] { NEED(s); NEED(scale); NEED(shift); s->sys->get_norm(net, scale, shift); });
+}
+
+mapp_status mapp_surr_set_norm(mapp_surr* s, int net, const float* scale, const float* shift)
+{
+    return guard([&] { NEED(s); NEED(scale); NEED(shift); s->sys->set_norm(net, scale, shift); });
+}
+
+mapp_status mapp_surr_infer(mapp_surr* s, int net, int n, const float* rin, float* out)
+{
+    return guard([&] { NEED(s); NEED(rin); NEED(out); s->sys->infer(net, n, rin, out); });
+}
+
+mapp_status mapp_dmd_use_surrogate(mapp_dmd* d, mapp_surr* s)
+{
+    return guard([&] { NEED(d); d->sys->use_surrogate(s ? s->sys : nullptr); });
+}
+
+mapp_status mapp_dmd_surrogate_coverage(mapp_dmd* d, int64_t* net_evals, int64_t* quadrature_slots)
+{
+    return guard([&] { NEED(d); d->sys->surrogate_coverage(net_evals, quadrature_slots); });
 }
 
 }  // extern "C"

@@ -170,6 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_dbuild_list(int n, const double4* __
 __device__ inline void dpair_dx(const double4& xi, const double4& xj, uint32_t code, const double* __restrict__ h,
                                 double& dx, double& dy, double& dz)
 {
+#pragma clang fp contract(off)
     dx = xi.x - xj.x;
     dy = xi.y - xj.y;
     dz = xi.z - xj.z;
@@ -181,6 +182,26 @@ __device__ inline void dpair_dx(const double4& xi, const double4& xj, uint32_t c
         dy -= h[3] * n1 + h[4] * n2;
         dz -= h[5] * n2;
     }
+}
+
+// Separation, distance, pair width and the cut test of one listed pair,
+// rounded the same way in every kernel (contraction off): the passes and the
+// surrogate staging must take identical decisions pair by pair.
+struct PairGeo {
+    double dx, dy, dz, r, a;
+    bool live;
+};
+
+__device__ inline PairGeo pair_geo(const double4& xi, const double4& xj, uint32_t code, const double* __restrict__ h,
+                                   double ai, double aj, double xmax, double cut)
+{
+#pragma clang fp contract(off)
+    PairGeo g;
+    dpair_dx(xi, xj, code, h, g.dx, g.dy, g.dz);
+    g.r = sqrt(g.dx * g.dx + g.dy * g.dy + g.dz * g.dz);
+    g.a = sqrt(ai * ai + aj * aj);
+    g.live = g.r - g.a * xmax < cut;
+    return g;
 }
 
 __device__ inline double hv(const double4& c, double p) { return ((c.w * p + c.z) * p + c.y) * p + c.x; }
@@ -257,6 +278,8 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass1(int n, const double4* __re
         const int C = A.C;
         const double4 xi = x[i];
         double cv = 1.0;
+        SurrCursor cur;
+        if (SURR) cur.begin(S, i, A.ne * A.ne);
         for (int ic = 0; ic < C; ++ic) {
             const int ci_idx = i * C + ic;
             const double ci = cc[ci_idx];
@@ -270,15 +293,16 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass1(int n, const double4* __re
                 const uint32_t e = list[(size_t)k * ldl + ci_idx];
                 const int jc = (int)(e & kIndexMask);
                 const double4 xj = x[jc / C];
-                double dx, dy, dz;
-                dpair_dx(xi, xj, e >> kIndexBits, bd.h, dx, dy, dz);
-                const double r = sqrt(dx * dx + dy * dy + dz * dz);
                 const double aj = al[jc];
-                const double a = sqrt(ai * ai + aj * aj);
                 const int ej = el[jc];
-                if (r - a * A.xmax >= A.cut[ei * A.ne + ej]) continue;
+                const PairGeo pg = pair_geo(xi, xj, e >> kIndexBits, bd.h, ai, aj, A.xmax, A.cut[ei * A.ne + ej]);
+                if (!pg.live) continue;
+                const double r = pg.r, a = pg.a;
                 double rp[3], drr[3], dal[3];
-                if (!SURR || !surrogate_read<false>(S, (size_t)k * ldl + ci_idx, r, a, rp, drr, dal))
+                int qin, qout;
+                if (SURR && cur.next(S, A.ne, ei, ej, r, ai, aj, qin, qout))
+                    surrogate_values<false>(S, qin, qout, r, a, rp, drr, dal);
+                else
                     smear_quad<false>(A, ei * A.ne + ej, r, a, rp, drr, dal);
                 rho += cc[jc] * rp[2];
                 ep += 0.5 * ci * cc[jc] * rp[0];
@@ -314,6 +338,8 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass2(int n, const double4* __re
         const int C = A.C;
         const double4 xi = x[i];
         double fx = 0.0, fy = 0.0, fz = 0.0;
+        SurrCursor cur;
+        if (SURR) cur.begin(S, i, A.ne * A.ne);
         for (int ic = 0; ic < C; ++ic) {
             const int ci_idx = i * C + ic;
             const double ci = cc[ci_idx];
@@ -331,15 +357,16 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass2(int n, const double4* __re
                 const uint32_t e = list[(size_t)k * ldl + ci_idx];
                 const int jc = (int)(e & kIndexMask);
                 const double4 xj = x[jc / C];
-                double dx, dy, dz;
-                dpair_dx(xi, xj, e >> kIndexBits, bd.h, dx, dy, dz);
-                const double r = sqrt(dx * dx + dy * dy + dz * dz);
                 const double aj = al[jc];
-                const double a = sqrt(ai * ai + aj * aj);
                 const int ej = el[jc];
-                if (r - a * A.xmax >= A.cut[ei * A.ne + ej]) continue;
+                const PairGeo pg = pair_geo(xi, xj, e >> kIndexBits, bd.h, ai, aj, A.xmax, A.cut[ei * A.ne + ej]);
+                if (!pg.live) continue;
+                const double r = pg.r, a = pg.a, dx = pg.dx, dy = pg.dy, dz = pg.dz;
                 double rp[3], drr[3], dal[3];
-                if (!SURR || !surrogate_read<true>(S, (size_t)k * ldl + ci_idx, r, a, rp, drr, dal))
+                int qin, qout;
+                if (SURR && cur.next(S, A.ne, ei, ej, r, ai, aj, qin, qout))
+                    surrogate_values<true>(S, qin, qout, r, a, rp, drr, dal);
+                else
                     smear_quad<true>(A, ei * A.ne + ej, r, a, rp, drr, dal);
                 const double cj = cc[jc], dEj = dE[jc];
                 const double fpair = -(drr[2] * dEi + drr[1] * dEj + drr[0]) * ci * cj;
@@ -554,6 +581,65 @@ __global__ __launch_bounds__(kBlock) void k_smear_points(DmdArgs A, int tt, int 
         out[(size_t)i * 9 + 3 + k] = dr[k];
         out[(size_t)i * 9 + 6 + k] = da[k];
     }
+}
+
+// Surrogate requests of every pair the passes will evaluate, in the order the
+// passes meet them (SurrCursor). COUNT: the number of requests per (network,
+// site) and, in the last segment, of pairs left to the quadrature; else: the
+// requests themselves at the scanned offsets.
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_surr_stage(int n, const double4* __restrict__ x,
+                                                       const double* __restrict__ al, const double* __restrict__ cc,
+                                                       const int* __restrict__ el, const int* __restrict__ nn,
+                                                       const uint32_t* __restrict__ list, int ldl, DmdArgs A,
+                                                       BoxDev bd, SurrBox box, int* __restrict__ cnt,
+                                                       const int* __restrict__ offs, SurrRequest* __restrict__ req)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int C = A.C, ne = A.ne, nt = ne * ne;
+    int pos[kMaxSurrNets + 1];  // requests per network, then the slots left to the quadrature
+    for (int t = 0; t <= nt; ++t) pos[t] = COUNT ? 0 : offs[(size_t)t * n + i];
+    const double sr = 2.0 / ((double)box.r_hi - box.r_lo), sa = 2.0 / ((double)box.a_hi - box.a_lo);
+    const double4 xi = x[i];
+    for (int ic = 0; ic < C; ++ic) {
+        const int ci_idx = i * C + ic;
+        if (cc[ci_idx] < 0.0) continue;
+        const double ai = al[ci_idx];
+        const int ei = el[ci_idx];
+        const int cntk = nn[ci_idx];
+        for (int k = 0; k < cntk; ++k) {
+            const uint32_t e = list[(size_t)k * ldl + ci_idx];
+            const int jc = (int)(e & kIndexMask);
+            const double4 xj = x[jc / C];
+            const double aj = al[jc];
+            const int ej = el[jc];
+            const PairGeo pg = pair_geo(xi, xj, e >> kIndexBits, bd.h, ai, aj, A.xmax, A.cut[ei * ne + ej]);
+            if (!pg.live) continue;  // the passes skip it too
+            const double r = pg.r;
+            if (!surrogate_in_box(box, r, ai, aj)) {
+                ++pos[nt];
+                continue;
+            }
+            const float xr = (float)((r - box.r_lo) * sr - 1.0);
+            const float xa = (float)((ai - box.a_lo) * sa - 1.0), xb = (float)((aj - box.a_lo) * sa - 1.0);
+            if (ei == ej) {
+                const int t = ei * ne + ei;
+                if (!COUNT) req[pos[t]] = SurrRequest{xr, xa, xb, 0.0f};
+                ++pos[t];
+            } else {
+                const int t_in = ej * ne + ei, t_out = ei * ne + ej;
+                if (!COUNT) {
+                    req[pos[t_in]] = SurrRequest{xr, xb, xa, 0.0f};
+                    req[pos[t_out]] = SurrRequest{xr, xa, xb, 0.0f};
+                }
+                ++pos[t_in];
+                ++pos[t_out];
+            }
+        }
+    }
+    if (COUNT)
+        for (int t = 0; t <= nt; ++t) cnt[(size_t)t * n + i] = pos[t];
 }
 
 }  // namespace
@@ -823,7 +909,70 @@ bool DMDSystem::update()
     return true;
 }
 
-void DMDSystem::set_surrogate(const SurrogateArgs* s) { surr_ = s; }
+void DMDSystem::use_surrogate(std::shared_ptr<Surrogate> s)
+{
+    if (s) MAPP_REQUIRE(s->nnets() == A_.ne * A_.ne, ERR_ARG, "the surrogate was built for another element count");
+    MAPP_REQUIRE(A_.ne * A_.ne <= kMaxSurrNets, ERR_CAPACITY, "too many elements for the surrogate");
+    surr_ = std::move(s);
+}
+
+void DMDSystem::surrogate_coverage(int64_t* by_net, int64_t* by_quadrature)
+{
+    if (by_net) *by_net = s_net_;
+    if (by_quadrature) *by_quadrature = s_quad_;
+}
+
+// Requests for every slot, scanned into one array grouped by network, then
+// one inference launch per network writing the per-slot outputs.
+void DMDSystem::stage_surrogate()
+{
+    const int nt = A_.ne * A_.ne;
+    MAPP_REQUIRE((size_t)ldl_ * maxn_ < (1ull << 30), ERR_CAPACITY, "too many list slots for 32-bit request offsets");
+    const size_t m = (size_t)(nt + 1) * n_;
+    s_cnt_.reserve(m);
+    s_off_.reserve(m);
+    BoxDev bd;
+    std::memcpy(bd.h, box_.h, sizeof bd.h);
+    std::memcpy(bd.b, box_.b, sizeof bd.b);
+    const SurrBox sb = surr_->box();
+    const int nb = (n_ + kBlock - 1) / kBlock;
+    prof_.begin(K_SURR_INFER, stream_);
+    hipLaunchKernelGGL(k_surr_stage<true>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
+                       el_.get(), nn_.get(), list_.get(), ldl_, A_, bd, sb, s_cnt_.get(), s_off_.get(),
+                       s_req_.get());
+    size_t bytes = 0;
+    MAPP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, s_cnt_.get(), s_off_.get(), (int)m, stream_));
+    if (bytes > s_scan_bytes_) {
+        s_scan_tmp_.reserve(bytes);
+        s_scan_bytes_ = bytes;
+    }
+    MAPP_HIP(hipcub::DeviceScan::ExclusiveSum(s_scan_tmp_.get(), bytes, s_cnt_.get(), s_off_.get(), (int)m,
+                                              stream_));
+    std::vector<int> first(nt + 1), last_off(nt + 1), last_cnt(nt + 1);
+    for (int t = 0; t <= nt; ++t) {
+        MAPP_HIP(hipMemcpyAsync(&first[t], s_off_.get() + (size_t)t * n_, sizeof(int), hipMemcpyDeviceToHost,
+                                stream_));
+        MAPP_HIP(hipMemcpyAsync(&last_off[t], s_off_.get() + (size_t)t * n_ + n_ - 1, sizeof(int),
+                                hipMemcpyDeviceToHost, stream_));
+        MAPP_HIP(hipMemcpyAsync(&last_cnt[t], s_cnt_.get() + (size_t)t * n_ + n_ - 1, sizeof(int),
+                                hipMemcpyDeviceToHost, stream_));
+    }
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    const int total = last_off[nt - 1] + last_cnt[nt - 1];  // requests end where the quadrature count starts
+    s_req_.reserve(std::max(1, total));
+    s_y_.reserve((size_t)std::max(1, total) * kSurrOut);
+    hipLaunchKernelGGL(k_surr_stage<false>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
+                       el_.get(), nn_.get(), list_.get(), ldl_, A_, bd, sb, s_cnt_.get(), s_off_.get(),
+                       s_req_.get());
+    s_net_ = 0;
+    for (int t = 0; t < nt; ++t) {
+        const int cnt = last_off[t] + last_cnt[t] - first[t];
+        s_net_ += cnt;
+        surr_->infer_dev(t, cnt, s_req_.get() + first[t], s_y_.get() + (size_t)first[t] * kSurrOut, stream_);
+    }
+    s_quad_ = last_off[nt] + last_cnt[nt] - first[nt];
+    prof_.end(stream_);
+}
 
 void DMDSystem::pass1(bool energy_only)
 {
@@ -832,7 +981,10 @@ void DMDSystem::pass1(bool energy_only)
     std::memcpy(bd.h, box_.h, sizeof bd.h);
     std::memcpy(bd.b, box_.b, sizeof bd.b);
     SurrogateArgs S{};
-    if (surr_) S = *surr_;
+    if (surr_) {
+        stage_surrogate();
+        S = SurrogateArgs{s_y_.get(), s_off_.get(), n_, surr_->box()};
+    }
     prof_.begin(energy_only ? K_DMD_ENERGY : K_DMD_PASS1, stream_);
     if (surr_)
         hipLaunchKernelGGL(k_dmd_pass1<true>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
@@ -853,7 +1005,7 @@ void DMDSystem::pass2()
     std::memcpy(bd.h, box_.h, sizeof bd.h);
     std::memcpy(bd.b, box_.b, sizeof bd.b);
     SurrogateArgs S{};
-    if (surr_) S = *surr_;
+    if (surr_) S = SurrogateArgs{s_y_.get(), s_off_.get(), n_, surr_->box()};
     double* p2 = part_.get() + (size_t)nb * kPartW;
     prof_.begin(K_DMD_PASS2, stream_);
     if (surr_)
@@ -1000,6 +1152,16 @@ void DMDSystem::half_pairs(int32_t* pi, int32_t* pj, int8_t* img)
                 ++c;
             }
         }
+}
+
+void DMDSystem::smear_dev(int tt, int npts, const double* r, const double* a, double* out)
+{
+    MAPP_REQUIRE(tt >= 0 && tt < A_.ne * A_.ne, ERR_ARG, "table index out of range");
+    if (npts <= 0) return;
+    hipLaunchKernelGGL(k_smear_points, dim3((npts + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, A_, tt, npts,
+                       r, a, out);
+    MAPP_HIP(hipGetLastError());
+    MAPP_HIP(hipStreamSynchronize(stream_));
 }
 
 void DMDSystem::smear(int ei, int ej, int npts, const double* r, const double* a, double* out)

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "md.hpp"
+#include "surrogate.hpp"
 
 namespace mapp {
 
@@ -22,7 +23,6 @@ struct DmdArgs {
     const double* F5;    // [ne][nrho][5], quartic embedding
 };
 
-struct SurrogateArgs;
 
 // physicists' Gauss-Hermite rule (weight e^{-t^2}), n <= 16
 void gauss_hermite(int n, double* x, double* w);
@@ -61,8 +61,14 @@ public:
     CGReport cg(int max_iter, double e_tol, double max_dx, double max_dalpha, double grad_tol);
     // smeared pair functions at given (r, alpha_ij): out[k*9 + q], q = rho_phi[3], drho_phi_dr[3], drho_phi_dalpha[3]
     void smear(int ei, int ej, int npts, const double* r, const double* a, double* out);
+    // the same for device arrays and a table index tt = ei * ne + ej (synchronous)
+    void smear_dev(int tt, int npts, const double* r, const double* a, double* out);
     Profiler& profiler() { return prof_; }
-    void set_surrogate(const SurrogateArgs* s);
+    // evaluations use the surrogate networks for pairs inside their training
+    // box and the quadrature elsewhere; nullptr returns to the quadrature
+    void use_surrogate(std::shared_ptr<Surrogate> s);
+    // slots the last evaluation gave to the networks / to the quadrature
+    void surrogate_coverage(int64_t* by_net, int64_t* by_quadrature);
     const DmdArgs& args() const { return A_; }
     hipStream_t stream() const { return stream_; }
 
@@ -106,7 +112,14 @@ private:
     DevBuf<double4> cg_x0_, cg_h_, cg_f0_;
     DevBuf<double> cg_a0_, cg_ha_, cg_fa0_;
     Profiler prof_;
-    const SurrogateArgs* surr_ = nullptr;
+    void stage_surrogate();
+    std::shared_ptr<Surrogate> surr_;
+    DevBuf<float> s_y_;
+    DevBuf<int> s_cnt_, s_off_;
+    DevBuf<SurrRequest> s_req_;
+    DevBuf<unsigned char> s_scan_tmp_;
+    size_t s_scan_bytes_ = 0;
+    int64_t s_net_ = 0, s_quad_ = 0;
 };
 
 }  // namespace mapp
