@@ -124,14 +124,44 @@ def dmd_run(evals=10):
     dt = time.perf_counter() - t0
     k1, k2 = d.profile_read("dmd_pass1"), d.profile_read("dmd_pass2")
     d.profile(False)
+    f0, fa0, _, fe0, _, _ = d.derivative()
     n = len(x)
     ms = dt / evals * 1e3
+    # the learned surrogate (config 4): trained on the GPU from this rule, then the same evaluation
+    s = M.Surrogate(2, box=(2.2, 6.1, 0.05, 0.30), seed=1)
+    t0 = time.perf_counter()
+    l0, l1 = s.train(d, nsamples=2_000_000, steps=8000, batch=4096, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    d.use_surrogate(s)
+    f1, fa1, _, fe1, _, _ = d.derivative()
+    t0 = time.perf_counter()
+    for _ in range(evals):
+        d.derivative()
+    dts = time.perf_counter() - t0
+    nets, quad = d.surrogate_coverage()
+    d.use_surrogate(None)
+    d16 = M.AtomsDMD(M.ForceFieldEAM.from_knots(p, dmd=True), H, x, elem, alpha, c, nquad=16, masses=p.masses)
+    d16.derivative()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        d16.derivative()
+    ms16 = (time.perf_counter() - t0) / 3 * 1e3
+    surr = {"value": n * evals / dts, "unit": "site-evals/s", "ms_per_eval": dts / evals * 1e3,
+            "speed_vs_quadrature_N3": dt / dts, "quadrature_N16_ms_per_eval": ms16,
+            "speed_vs_quadrature_N16": ms16 / (dts / evals * 1e3),
+            "max_abs_df_x_eV_per_A": float(np.abs(f1 - f0).max()),
+            "rms_df_x_eV_per_A": float(np.sqrt(((f1 - f0) ** 2).mean())),
+            "max_abs_df_alpha_eV_per_A": float(np.abs(fa1 - fa0).max()),
+            "free_energy_error_eV_per_site": abs(fe1 - fe0) / n, "network_calls": nets, "quadrature_pairs": quad,
+            "train": {"samples_per_net": 2_000_000, "steps": 8000, "batch": 4096, "loss0": l0, "loss": l1,
+                      "seconds": t_train, "dtype": "bf16 MFMA, fp32 accumulate"},
+            "target": ">= 5x the N=3 quadrature at <= 1e-4 eV/A (not met; see DESIGN.md)"}
     return {"workload": "DMD Cu-H, 262,144 fcc sites x 2 components, N=3 Gauss-Hermite, r_c=5.5 A, "
                         "alpha log-uniform [0.05,0.30] A, c_H ~ Beta(2,5)",
             "metric": "DMD site-evals/s (free energy + gradient)", "value": n * evals / dt, "unit": "site-evals/s",
             "ms_per_eval": ms, "pass1_ms": k1[0] / max(k1[1], 1), "pass2_ms": k2[0] / max(k2[1], 1),
             "dtype": "f64", "half_pairs": int(d.half_pairs()[0].shape[0]),
-            "hbm_fraction": DMD_BYTES_PER_SITE * n / (ms / 1e3) / HBM_PEAK}
+            "hbm_fraction": DMD_BYTES_PER_SITE * n / (ms / 1e3) / HBM_PEAK, "surrogate": surr}
 
 
 def dmd_cpu_baseline(seconds):

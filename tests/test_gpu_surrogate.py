@@ -132,3 +132,40 @@ def test_dmd_evaluation_through_the_networks(M, trained):
         f"|dFE|/N {abs(fe1 - fe0) / len(f0):.2e} eV, network calls {nets}, quadrature slots {quad}")
     assert nets > 0
     assert df < 0.5 and abs(fe1 - fe0) / len(f0) < 0.05
+
+
+def _time(fn, k=5):
+    import time
+
+    fn()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        fn()
+    return (time.perf_counter() - t0) / k
+
+
+def test_networks_gain_on_the_quadrature_as_its_order_grows(M):
+    # What the networks buy: a fixed cost per pair whatever the number of
+    # Gauss-Hermite points they were trained from, while the quadrature's
+    # cost per pair grows with it. The training box covers the whole list
+    # range for each rule (r_c + x_max sqrt(2) alpha_max grows with N). On
+    # MI355X the FP64 quadrature stays the cheaper path at both orders; the
+    # networks' relative speed improves with N.
+    p = pot.johnson_cuh(nr=5000, nrho=5000, rc=5.5)
+    x, H, elem, alpha, c = pot.dmd_cuh_sites((32, 32, 16), seed=4)
+    x = x + np.random.default_rng(5).normal(0, 0.05, x.shape)
+    ff = M.ForceFieldEAM.from_knots(p, dmd=True)
+    t = {}
+    for N in (3, 16):
+        d = M.AtomsDMD(ff, H, x, elem, alpha, c, nquad=N, masses=p.masses)
+        tq = _time(d.derivative)
+        xmax = M.gauss_hermite(N)[0].max()
+        s = M.Surrogate(2, box=(2.2, 5.5 + xmax * np.sqrt(2.0) * 0.30 + 0.05, 0.05, 0.30), seed=1)
+        s.train(d, nsamples=100_000, steps=300, batch=4096, lr=2e-3)
+        d.use_surrogate(s)
+        t[N] = (tq, _time(d.derivative))
+        assert d.surrogate_coverage()[1] == 0  # every pair went to the networks
+        say(f"{len(x)} sites, N={N}: quadrature {t[N][0] * 1e3:.2f} ms, networks {t[N][1] * 1e3:.2f} ms per evaluation "
+            f"(networks / quadrature speed {t[N][0] / t[N][1]:.2f})")
+    assert t[16][0] > 2.0 * t[3][0]
+    assert t[16][0] / t[16][1] > 1.2 * (t[3][0] / t[3][1])
