@@ -203,6 +203,24 @@ __device__ inline PairGeo pair_geo(const double4& xi, int ti, uint32_t e, const 
     return g;
 }
 
+// separation of two 16-byte pass-2 records with the image shift of the list entry
+__device__ inline void rec16_dx(const uint4& ri, const uint4& rj, uint32_t code, const Rec16Frame& R, double& dx,
+                                double& dy, double& dz)
+{
+    unsigned sx = 0u, sy = 0u, sz = 0u;
+    if (code != (uint32_t)kZeroImage) {
+        const int n2 = (int)(code / 25u) - 2;
+        const int rem = (int)(code % 25u);
+        const unsigned n1 = (unsigned)(rem / 5 - 2), n0 = (unsigned)(rem % 5 - 2), m2 = (unsigned)n2;
+        sx = n0 * R.cH[0] + n1 * R.cH[3] + m2 * R.cH[6];
+        sy = n1 * R.cH[4] + m2 * R.cH[7];
+        sz = m2 * R.cH[8];
+    }
+    dx = (double)(int)(ri.x - rj.x - sx) * R.q;
+    dy = (double)(int)(ri.y - rj.y - sy) * R.q;
+    dz = (double)(int)(ri.z - rj.z - sz) * R.q;
+}
+
 // 16-byte position record for the density pass: three 41-bit fixed-point
 // coordinates x = o + c q and the element in the top 5 bits. Differences of
 // two records are exact integers, so dx carries FP64 quality (q is about
@@ -252,6 +270,7 @@ __global__ __launch_bounds__(kBlock) void k_eam_rho(int n, const double4* __rest
                                                     FxFrame F, const int* __restrict__ fx_bad, const int* __restrict__ nn,
                                                     const uint32_t* __restrict__ list, int ldl, EamArgs P, BoxDev bd,
                                                     double* __restrict__ rho_out, double4* __restrict__ xf,
+                                                    uint4* __restrict__ xr, Rec16Frame R16,
                                                     double* __restrict__ part)
 {
     const int blk = xcd_block();
@@ -310,7 +329,11 @@ __global__ __launch_bounds__(kBlock) void k_eam_rho(int n, const double4* __rest
         double F, dF;
         embed(P, ti, rho, F, dF);
         rho_out[i] = rho;
-        xf[i] = make_double4(xi.x, xi.y, xi.z, pack_type_dF(ti, (float)dF));
+        if (xr)
+            xr[i] = make_uint4((unsigned)__double2ll_rn(xi.x * R16.inv_q), (unsigned)__double2ll_rn(xi.y * R16.inv_q),
+                               (unsigned)__double2ll_rn(xi.z * R16.inv_q), __float_as_uint((float)dF));
+        else
+            xf[i] = make_double4(xi.x, xi.y, xi.z, pack_type_dF(ti, (float)dF));
         acc[0] = F;
     }
     block_sum_store<1>(acc, part + (size_t)blk * kPartW);
@@ -321,39 +344,65 @@ __global__ __launch_bounds__(kBlock) void k_eam_rho(int n, const double4* __rest
 
 // pass 2: pair + embedding forces on the owner only (no scatter), half of
 // each full-list pair's energy and virial
-template <int B, bool ONE>
+template <int B, bool ONE, bool R16>
 __global__ __launch_bounds__(kBlock) void k_eam_force(int n, const double4* __restrict__ x,
+                                                      const uint4* __restrict__ xr, Rec16Frame RF,
                                                       const int* __restrict__ nn, const uint32_t* __restrict__ list,
                                                       int ldl, EamArgs P, BoxDev bd,
                                                       float4* __restrict__ f_out, double* __restrict__ part,
                                                       KickArgs K, double4* __restrict__ v, double* __restrict__ part_mvv)
 {
-    // x holds the pass-2 records written by the density pass
+    // x (or xr, for single-element tables) holds the pass-2 records written by the density pass
     const int blk = xcd_block();
     const int i = blk * kBlock + threadIdx.x;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     double mvv[6] = {0, 0, 0, 0, 0, 0};
     if (i < n) {
-        const double4 xi = x[i];
-        const int ti = rec_type(xi.w);
+        double4 xi = make_double4(0.0, 0.0, 0.0, 0.0);
+        uint4 ri = make_uint4(0u, 0u, 0u, 0u);
+        int ti = 0;
+        float dFi;
+        if (R16) {
+            ri = xr[i];
+            dFi = __uint_as_float(ri.w);
+        } else {
+            xi = x[i];
+            ti = rec_type(xi.w);
+            dFi = rec_dF(xi.w);
+        }
         const int cnt = nn[i];
-        const float dFi = rec_dF(xi.w);
         float fx = 0.f, fy = 0.f, fz = 0.f;
         for (int k0 = 0; k0 < cnt; k0 += B) {
             uint32_t e[B];
             double4 xj[B];
+            uint4 rj[B];
             float dFj[B];
             float4 ca[B], cb[B], cp[B];
             PairGeo g[B];
 #pragma unroll
             for (int b = 0; b < B; ++b) e[b] = k0 + b < cnt ? list[(size_t)(k0 + b) * ldl + i] : (uint32_t)i | (kZeroImage << kIndexBits);
+            if (R16) {
 #pragma unroll
-            for (int b = 0; b < B; ++b) xj[b] = x[e[b] & kIndexMask];
+                for (int b = 0; b < B; ++b) rj[b] = xr[e[b] & kIndexMask];
 #pragma unroll
-            for (int b = 0; b < B; ++b) {
-                g[b] = pair_geo(xi, ti, e[b], xj[b], rec_type(xj[b].w), P, bd);
-                g[b].in = g[b].in && (k0 + b < cnt);
-                dFj[b] = rec_dF(xj[b].w);
+                for (int b = 0; b < B; ++b) {
+                    rec16_dx(ri, rj[b], e[b] >> kIndexBits, RF, g[b].dx, g[b].dy, g[b].dz);
+                    const double rsq = g[b].dx * g[b].dx + g[b].dy * g[b].dy + g[b].dz * g[b].dz;
+                    g[b].in = rsq < P.cut_sq[0] && (k0 + b < cnt);
+                    g[b].r = root64(rsq);
+                    const int m = knot(g[b].r, P.dr_inv, P.nr, g[b].p);
+                    g[b].row = g[b].in ? m : 0;
+                    dFj[b] = __uint_as_float(rj[b].w);
+                }
+            } else {
+#pragma unroll
+                for (int b = 0; b < B; ++b) xj[b] = x[e[b] & kIndexMask];
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    g[b] = pair_geo(xi, ti, e[b], xj[b], rec_type(xj[b].w), P, bd);
+                    g[b].in = g[b].in && (k0 + b < cnt);
+                    dFj[b] = rec_dF(xj[b].w);
+                }
             }
 #pragma unroll
             for (int b = 0; b < B; ++b) {
@@ -726,7 +775,7 @@ MDSystem::MDSystem(std::shared_ptr<EamForceField> ff, int natms, const double* H
     setup_grid();
     x_.reserve(n_); x_alt_.reserve(n_); s_.reserve(n_); x0_.reserve(n_);
     v_.reserve(n_); v_alt_.reserve(n_); id_.reserve(n_); id_alt_.reserve(n_); slot_of_.reserve(n_);
-    f_.reserve(n_); rho_.reserve(n_); xf_.reserve(n_); rec_.reserve(n_);
+    f_.reserve(n_); rho_.reserve(n_); xf_.reserve(n_); xr_.reserve(n_); rec_.reserve(n_);
     keys_.reserve(n_); keys_alt_.reserve(n_); vals_.reserve(n_); vals_alt_.reserve(n_); s_tmp_.reserve(n_);
     nn_.reserve(n_);
     flags_.reserve(4);
@@ -812,6 +861,12 @@ void MDSystem::setup_grid()
     fx_.q = span / std::ldexp(1.0, 41);
     fx_.inv_q = 1.0 / fx_.q;
     for (int d = 0; d < 3; ++d) fx_.o[d] = lo[d] - margin;
+    // 16-byte pass-2 records: separations up to 4 (cut + skin) stay below 2^31 q
+    r16_.q = std::ldexp(1.0, (int)std::ceil(std::log2(4.0 * max_cut_ / std::ldexp(1.0, 31))));
+    r16_.inv_q = 1.0 / r16_.q;
+    for (int k = 0; k < 3; ++k)
+        for (int d = 0; d < 3; ++d)
+            r16_.cH[3 * k + d] = (unsigned)(unsigned long long)std::llrint(box_.H[k][d] * r16_.inv_q);
     // Morton rank of every cell so that cell-sorted atoms are spatially compact
     std::vector<std::pair<uint32_t, int>> code(ncells);
     for (int c2 = 0, c = 0; c2 < grid_.nc[2]; ++c2)
@@ -918,11 +973,13 @@ void MDSystem::force_passes(const KickArgs* kick)
     MAPP_HIP(hipMemsetAsync(flags_.get() + 2, 0, sizeof(int), stream_));
     prof_.begin(K_EAM_RHO, stream_);
     hipLaunchKernelGGL(k_make_records, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), fx_, rec_.get(), flags_.get());
+    const bool one = P.ne == 1;
+    uint4* xr = one ? xr_.get() : nullptr;
     const int* bad = flags_.get() + 2;
     switch (batch_rho_) {
-        case 1: hipLaunchKernelGGL(k_eam_rho<1>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), fx_, bad, nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), xf_.get(), part_.get()); break;
-        case 4: hipLaunchKernelGGL(k_eam_rho<4>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), fx_, bad, nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), xf_.get(), part_.get()); break;
-        default: hipLaunchKernelGGL(k_eam_rho<8>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), fx_, bad, nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), xf_.get(), part_.get()); break;
+        case 1: hipLaunchKernelGGL(k_eam_rho<1>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), fx_, bad, nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), xf_.get(), xr, r16_, part_.get()); break;
+        case 4: hipLaunchKernelGGL(k_eam_rho<4>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), fx_, bad, nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), xf_.get(), xr, r16_, part_.get()); break;
+        default: hipLaunchKernelGGL(k_eam_rho<8>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), fx_, bad, nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), xf_.get(), xr, r16_, part_.get()); break;
     }
     prof_.end(stream_);
     prof_.begin(K_EAM_FORCE, stream_);
@@ -933,9 +990,9 @@ void MDSystem::force_passes(const KickArgs* kick)
         K.on = 1;
     }
     double* p3 = part_.get() + (size_t)2 * nb * kPartW;
-#define MAPP_FORCE(B, ONE) hipLaunchKernelGGL((k_eam_force<B, ONE>), dim3(nb), dim3(kBlock), 0, stream_, n_, xf_.get(), \
-                                              nn_.get(), list_.get(), ldl_, P, bd, f_.get(), p2, K, v_.get(), p3)
-    const bool one = P.ne == 1;
+#define MAPP_FORCE(B, ONE) hipLaunchKernelGGL((k_eam_force<B, ONE, ONE>), dim3(nb), dim3(kBlock), 0, stream_, n_, \
+                                              xf_.get(), xr_.get(), r16_, nn_.get(), list_.get(), ldl_, P, bd, \
+                                              f_.get(), p2, K, v_.get(), p3)
     switch (batch_force_) {
         case 1: if (one) MAPP_FORCE(1, true); else MAPP_FORCE(1, false); break;
         case 2: if (one) MAPP_FORCE(2, true); else MAPP_FORCE(2, false); break;

@@ -53,6 +53,16 @@ struct CellGrid {
     double cut_s[3];
 };
 
+// 16-byte pass-2 records of single-element force fields: coordinates as
+// 32-bit integers modulo 2^32 on a grid of q (a power of two, small enough
+// that every listed separation stays below 2^31 q), F'(rho) in the fourth
+// word. Differences of two records, less the integer image shift n . cH, are
+// exact integers, so separations carry about 1e-9 A.
+struct Rec16Frame {
+    double q, inv_q;
+    unsigned cH[9];  // rows of H on the grid, modulo 2^32: cH[3 * row + dim]
+};
+
 // second velocity-Verlet half kick carried out by the force pass
 struct KickArgs {
     double dt2;
@@ -126,6 +136,8 @@ private:
     DevBuf<double> rho_;
     DevBuf<uint4> rec_;              // fixed-point position records (density pass)
     DevBuf<double4> xf_;             // pass-2 records: x, y, z, (type, F'(rho))
+    DevBuf<uint4> xr_;               // 16-byte pass-2 records (single-element tables)
+    Rec16Frame r16_{};
     DevBuf<uint32_t> keys_, keys_alt_;
     DevBuf<int> vals_, vals_alt_;
     DevBuf<double4> s_tmp_;
