@@ -221,101 +221,55 @@ __device__ inline void rec16_dx(const uint4& ri, const uint4& rj, uint32_t code,
     dz = (double)(int)(ri.z - rj.z - sz) * R.q;
 }
 
-// 16-byte position record for the density pass: three 41-bit fixed-point
-// coordinates x = o + c q and the element in the top 5 bits. Differences of
-// two records are exact integers, so dx carries FP64 quality (q is about
-// 2e-10 A for a 500 A frame) from half the bytes of a double4 gather.
-constexpr unsigned long long kFxMask = (1ull << 41) - 1ull;
-
-__device__ inline uint4 fx_encode(const FxFrame& F, double x, double y, double z, int t, bool& ok)
-{
-    const double a = (x - F.o[0]) * F.inv_q, b = (y - F.o[1]) * F.inv_q, c = (z - F.o[2]) * F.inv_q;
-    const double lim = (double)kFxMask;
-    ok = a >= 0.0 && b >= 0.0 && c >= 0.0 && a <= lim && b <= lim && c <= lim;
-    const unsigned long long ca = ok ? (unsigned long long)rint(a) : 0ull;
-    const unsigned long long cb = ok ? (unsigned long long)rint(b) : 0ull;
-    const unsigned long long cc = ok ? (unsigned long long)rint(c) : 0ull;
-    const unsigned long long lo = (ca & kFxMask) | ((cb & kFxMask) << 41);
-    const unsigned long long hi = ((cb & kFxMask) >> 23) | ((cc & kFxMask) << 18) | ((unsigned long long)t << 59);
-    return make_uint4((unsigned)lo, (unsigned)(lo >> 32), (unsigned)hi, (unsigned)(hi >> 32));
-}
-
-__device__ inline void fx_decode(const uint4 r, long long& cx, long long& cy, long long& cz, int& t)
-{
-    const unsigned long long lo = (unsigned long long)r.x | ((unsigned long long)r.y << 32);
-    const unsigned long long hi = (unsigned long long)r.z | ((unsigned long long)r.w << 32);
-    cx = (long long)(lo & kFxMask);
-    cy = (long long)(((lo >> 41) | (hi << 23)) & kFxMask);
-    cz = (long long)((hi >> 18) & kFxMask);
-    t = (int)(hi >> 59);
-}
-
-// records of every atom; flags[2] is raised if a coordinate leaves the frame
-__global__ __launch_bounds__(kBlock) void k_make_records(int n, const double4* __restrict__ x, FxFrame F,
-                                                         uint4* __restrict__ rec, int* __restrict__ flags)
+// 16-byte position records for the density pass: the coordinates on the
+// grid of the pass-2 records (32-bit integers modulo 2^32) and the element
+// in the fourth word. Record differences are exact integers (rec16_dx).
+__global__ __launch_bounds__(kBlock) void k_make_records(int n, const double4* __restrict__ x, Rec16Frame R,
+                                                         uint4* __restrict__ rec)
 {
     const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     const double4 xi = x[i];
-    bool ok;
-    rec[i] = fx_encode(F, xi.x, xi.y, xi.z, (int)xi.w, ok);
-    if (!ok) atomicOr(&flags[2], 1);
+    rec[i] = make_uint4((unsigned)__double2ll_rn(xi.x * R.inv_q), (unsigned)__double2ll_rn(xi.y * R.inv_q),
+                        (unsigned)__double2ll_rn(xi.z * R.inv_q), (unsigned)(int)xi.w);
 }
 
 // pass 1: rho_i = sum_j rho_{j->i}(r_ij) (FP64 sum of FP32 spline values),
-// F(rho_i) and F'(rho_i); embedding energy reduced per workgroup. FX reads
-// neighbor positions from the fixed-point records (one 16-byte gather).
+// F(rho_i) and F'(rho_i); embedding energy reduced per workgroup. Neighbor
+// positions come from the 16-byte records (one gather per neighbor).
 template <int B>
 __global__ __launch_bounds__(kBlock) void k_eam_rho(int n, const double4* __restrict__ x, const uint4* __restrict__ rec,
-                                                    FxFrame F, const int* __restrict__ fx_bad, const int* __restrict__ nn,
-                                                    const uint32_t* __restrict__ list, int ldl, EamArgs P, BoxDev bd,
-                                                    double* __restrict__ rho_out, double4* __restrict__ xf,
-                                                    uint4* __restrict__ xr, Rec16Frame R16,
+                                                    const int* __restrict__ nn, const uint32_t* __restrict__ list,
+                                                    int ldl, EamArgs P, double* __restrict__ rho_out,
+                                                    double4* __restrict__ xf, uint4* __restrict__ xr, Rec16Frame R16,
                                                     double* __restrict__ part)
 {
     const int blk = xcd_block();
     const int i = blk * kBlock + threadIdx.x;
     double acc[1] = {0.0};
-    // records are used unless a coordinate fell outside their frame (uniform branch)
-    const bool FX = fx_bad[0] == 0;
     if (i < n) {
-        const double4 xi = x[i];
-        const int ti = (int)xi.w;
+        const uint4 ri = rec[i];
+        const int ti = (int)ri.w;
         const int cnt = nn[i];
-        long long ci0 = 0, ci1 = 0, ci2 = 0;
-        if (FX) {
-            int t;
-            fx_decode(rec[i], ci0, ci1, ci2, t);
-        }
         double rho = 0.0;
         for (int k0 = 0; k0 < cnt; k0 += B) {
             uint32_t e[B];
-            double4 xj[B];
             uint4 rj[B];
             float4 c[B];
             PairGeo g[B];
 #pragma unroll
             for (int b = 0; b < B; ++b) e[b] = k0 + b < cnt ? list[(size_t)(k0 + b) * ldl + i] : (uint32_t)i | (kZeroImage << kIndexBits);
-            if (FX) {
 #pragma unroll
-                for (int b = 0; b < B; ++b) rj[b] = rec[e[b] & kIndexMask];
-#pragma unroll
-                for (int b = 0; b < B; ++b) {
-                    long long c0, c1, c2;
-                    int tj;
-                    fx_decode(rj[b], c0, c1, c2, tj);
-                    // exact integer differences, re-based on x_i so pair_geo sees x_i - x_j
-                    xj[b] = make_double4(xi.x - (double)(ci0 - c0) * F.q, xi.y - (double)(ci1 - c1) * F.q,
-                                         xi.z - (double)(ci2 - c2) * F.q, (double)tj);
-                }
-            } else {
-#pragma unroll
-                for (int b = 0; b < B; ++b) xj[b] = x[e[b] & kIndexMask];
-            }
+            for (int b = 0; b < B; ++b) rj[b] = rec[e[b] & kIndexMask];
 #pragma unroll
             for (int b = 0; b < B; ++b) {
-                g[b] = pair_geo(xi, ti, e[b], xj[b], (int)xj[b].w, P, bd);
-                g[b].in = g[b].in && (k0 + b < cnt);
+                rec16_dx(ri, rj[b], e[b] >> kIndexBits, R16, g[b].dx, g[b].dy, g[b].dz);
+                const double rsq = g[b].dx * g[b].dx + g[b].dy * g[b].dy + g[b].dz * g[b].dz;
+                const int tt = ti * P.ne + (int)rj[b].w;
+                g[b].in = rsq < P.cut_sq[tt] && (k0 + b < cnt);
+                g[b].r = root64(rsq);
+                const int m = knot(g[b].r, P.dr_inv, P.nr, g[b].p);
+                g[b].row = g[b].in ? tt * P.nr + m : 0;
             }
 #pragma unroll
             for (int b = 0; b < B; ++b) c[b] = P.tabA[g[b].in ? g[b].row : 0];
@@ -329,11 +283,12 @@ __global__ __launch_bounds__(kBlock) void k_eam_rho(int n, const double4* __rest
         double F, dF;
         embed(P, ti, rho, F, dF);
         rho_out[i] = rho;
-        if (xr)
-            xr[i] = make_uint4((unsigned)__double2ll_rn(xi.x * R16.inv_q), (unsigned)__double2ll_rn(xi.y * R16.inv_q),
-                               (unsigned)__double2ll_rn(xi.z * R16.inv_q), __float_as_uint((float)dF));
-        else
+        if (xr) {
+            xr[i] = make_uint4(ri.x, ri.y, ri.z, __float_as_uint((float)dF));
+        } else {
+            const double4 xi = x[i];
             xf[i] = make_double4(xi.x, xi.y, xi.z, pack_type_dF(ti, (float)dF));
+        }
         acc[0] = F;
     }
     block_sum_store<1>(acc, part + (size_t)blk * kPartW);
@@ -842,26 +797,7 @@ void MDSystem::setup_grid()
         ncells *= nc;
     }
     grid_.ncells = ncells;
-    // fixed-point frame covering the box's bounding box with a margin for
-    // atoms that drift between rebuilds
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int c = 0; c < 8; ++c) {
-        const double s[3] = {(double)(c & 1), (double)((c >> 1) & 1), (double)((c >> 2) & 1)};
-        for (int d = 0; d < 3; ++d) {
-            double v = 0.0;
-            for (int k = 0; k < 3; ++k) v += s[k] * box_.H[k][d];
-            lo[d] = std::min(lo[d], v);
-            hi[d] = std::max(hi[d], v);
-        }
-    }
-    const double margin = 4.0 * max_cut_ + 16.0;
-    double ext = 0.0;
-    for (int d = 0; d < 3; ++d) ext = std::max(ext, hi[d] - lo[d] + 2.0 * margin);
-    const double span = std::ldexp(1.0, (int)std::ceil(std::log2(ext)));
-    fx_.q = span / std::ldexp(1.0, 41);
-    fx_.inv_q = 1.0 / fx_.q;
-    for (int d = 0; d < 3; ++d) fx_.o[d] = lo[d] - margin;
-    // 16-byte pass-2 records: separations up to 4 (cut + skin) stay below 2^31 q
+    // 16-byte position records: separations up to 4 (cut + skin) stay below 2^31 q
     r16_.q = std::ldexp(1.0, (int)std::ceil(std::log2(4.0 * max_cut_ / std::ldexp(1.0, 31))));
     r16_.inv_q = 1.0 / r16_.q;
     for (int k = 0; k < 3; ++k)
@@ -972,14 +908,13 @@ void MDSystem::force_passes(const KickArgs* kick)
     std::memcpy(bd.b, box_.b, sizeof bd.b);
     MAPP_HIP(hipMemsetAsync(flags_.get() + 2, 0, sizeof(int), stream_));
     prof_.begin(K_EAM_RHO, stream_);
-    hipLaunchKernelGGL(k_make_records, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), fx_, rec_.get(), flags_.get());
+    hipLaunchKernelGGL(k_make_records, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), r16_, rec_.get());
     const bool one = P.ne == 1;
     uint4* xr = one ? xr_.get() : nullptr;
-    const int* bad = flags_.get() + 2;
     switch (batch_rho_) {
-        case 1: hipLaunchKernelGGL(k_eam_rho<1>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), fx_, bad, nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), xf_.get(), xr, r16_, part_.get()); break;
-        case 4: hipLaunchKernelGGL(k_eam_rho<4>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), fx_, bad, nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), xf_.get(), xr, r16_, part_.get()); break;
-        default: hipLaunchKernelGGL(k_eam_rho<8>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), fx_, bad, nn_.get(), list_.get(), ldl_, P, bd, rho_.get(), xf_.get(), xr, r16_, part_.get()); break;
+        case 1: hipLaunchKernelGGL(k_eam_rho<1>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), nn_.get(), list_.get(), ldl_, P, rho_.get(), xf_.get(), xr, r16_, part_.get()); break;
+        case 4: hipLaunchKernelGGL(k_eam_rho<4>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), nn_.get(), list_.get(), ldl_, P, rho_.get(), xf_.get(), xr, r16_, part_.get()); break;
+        default: hipLaunchKernelGGL(k_eam_rho<8>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), nn_.get(), list_.get(), ldl_, P, rho_.get(), xf_.get(), xr, r16_, part_.get()); break;
     }
     prof_.end(stream_);
     prof_.begin(K_EAM_FORCE, stream_);

@@ -39,12 +39,6 @@ private:
     DevBuf<double> Ftab_;
 };
 
-// Frame of the fixed-point position records: x = o + c q per coordinate.
-struct FxFrame {
-    double o[3];
-    double q, inv_q;
-};
-
 // Box, cell grid and search radius of one neighbor build.
 struct CellGrid {
     int nc[3];
@@ -53,11 +47,12 @@ struct CellGrid {
     double cut_s[3];
 };
 
-// 16-byte pass-2 records of single-element force fields: coordinates as
-// 32-bit integers modulo 2^32 on a grid of q (a power of two, small enough
-// that every listed separation stays below 2^31 q), F'(rho) in the fourth
-// word. Differences of two records, less the integer image shift n . cH, are
-// exact integers, so separations carry about 1e-9 A.
+// 16-byte position records: coordinates as 32-bit integers modulo 2^32 on a
+// grid of q (a power of two, small enough that every listed separation stays
+// below 2^31 q), the element (density pass) or F'(rho) (force pass of
+// single-element tables) in the fourth word. Differences of two records, less
+// the integer image shift n . cH, are exact integers, so separations carry
+// about 1e-8 A.
 struct Rec16Frame {
     double q, inv_q;
     unsigned cH[9];  // rows of H on the grid, modulo 2^32: cH[3 * row + dim]
@@ -122,7 +117,6 @@ private:
     double skin_, max_cut_;
     Box box_;
     CellGrid grid_{};
-    FxFrame fx_{};
     hipStream_t stream_ = nullptr;
     int nrebuild_ = 0;
 
