@@ -99,9 +99,16 @@ mapp_status mapp_md_nvt_run(mapp_md* md, const double* masses, double kB, double
 mapp_status mapp_md_profile(mapp_md* md, int enable);
 mapp_status mapp_md_profile_read(mapp_md* md, const char* kernel, double* total_ms, int64_t* launches);
 /* Launch-shape knobs (no counterpart in the reference): neighbors resolved
- * per batch in the density pass (1, 4, 8) and the force pass (1, 2, 4, 8).
- * Any value gives the same results; the tuner picks the fastest. */
+ * per batch in the density pass (1, 2, 4, 8) and the force pass (1, 2, 4, 8;
+ * the LDS force pass uses at most 2). Any value gives the same results to
+ * rounding; the tuner picks the fastest. */
 mapp_status mapp_md_set_batch(mapp_md* md, int rho_batch, int force_batch);
+/* Where the passes of a one-element table read their spline rows (no
+ * counterpart in the reference): on != 0 keeps a window of rows in LDS and
+ * runs one persistent 1024-thread workgroup per CU (the default when the
+ * device has the LDS); 0 reads the global tables. *in_use (may be NULL)
+ * reports the mode in effect. */
+mapp_status mapp_md_set_lds(mapp_md* md, int on, int* in_use);
 /* AtomsMD::create_T (src/atoms_md.cpp:130-160): Maxwell velocities from the
  * Park-Miller / Marsaglia stream of src/random.cpp:7-68 seeded with seed. */
 mapp_status mapp_create_T(int natms, const uint8_t* elem, const double* masses, double kB, double T, int seed,

@@ -159,6 +159,13 @@ class AtomsMD:
     def set_batch(self, rho, force):
         check(lib().mapp_md_set_batch(self._h, int(rho), int(force)))
 
+    def set_lds(self, on):
+        """Spline rows of a one-element table in LDS (True) or read from the
+        global tables (False); returns the mode in effect."""
+        used = C.c_int(0)
+        check(lib().mapp_md_set_lds(self._h, int(bool(on)), C.byref(used)))
+        return bool(used.value)
+
     def profile(self, enable=True):
         check(lib().mapp_md_profile(self._h, int(enable)))
 

@@ -264,6 +264,15 @@ mapp_status mapp_md_set_batch(mapp_md* md, int rho_batch, int force_batch)
     return guard([&] { NEED(md); md->sys->set_batch(rho_batch, force_batch); });
 }
 
+mapp_status mapp_md_set_lds(mapp_md* md, int on, int* in_use)
+{
+    return guard([&] {
+        NEED(md);
+        const bool used = md->sys->set_lds(on != 0);
+        if (in_use) *in_use = used ? 1 : 0;
+    });
+}
+
 mapp_status mapp_dmd_create(mapp_eam* ff, int nsites, int c_dim, int nquad, const double* H, const double* x,
                             const uint8_t* elem, const double* alpha, const double* c, double skin, double kB,
                             double T, double hP, const double* masses, mapp_dmd** out)

@@ -684,7 +684,7 @@ void gauss_hermite(int n, double* x, double* w)
 DMDSystem::DMDSystem(std::shared_ptr<EamForceField> ff, int nsites, int c_dim, int nquad, const double* H,
                      const double* x, const uint8_t* elem, const double* alpha, const double* c, double skin,
                      double kB, double T, double hP, const double* masses)
-    : ff_(std::move(ff)), n_(nsites), C_(c_dim), nc_(nsites * c_dim), skin_(skin), kB_(kB), T_(T)
+    : ff_(std::move(ff)), n_(nsites), C_(c_dim), nc_(nsites * c_dim), skin_(skin)
 {
     const EamTables& t = ff_->tables();
     MAPP_REQUIRE(t.dmd, ERR_ARG, "the DMD system needs a force field built with dmd=1 (quartic F, r*rho tables)");

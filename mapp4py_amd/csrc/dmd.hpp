@@ -88,7 +88,6 @@ private:
     DmdArgs A_{};
     hipStream_t stream_ = nullptr;
     int nrebuild_ = 0;
-    double kB_, T_;
 
     DevBuf<double4> x_, x_alt_, s_, x0_, s_tmp_;
     DevBuf<double> al_, al_alt_, c_, c_alt_, al0_;

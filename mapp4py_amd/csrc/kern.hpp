@@ -23,12 +23,13 @@ __device__ inline int xcd_block()
     return (blockIdx.x % kXcd) * q + blockIdx.x / kXcd;
 }
 
-template <int K>
+// K FP64 sums over a workgroup of NT threads, written as one row of kPartW
+template <int K, int NT = kBlock>
 __device__ inline void block_sum_store(double (&v)[K], double* out)
 {
     for (int off = 32; off > 0; off >>= 1)
         for (int k = 0; k < K; ++k) v[k] += __shfl_xor(v[k], off, 64);
-    __shared__ double sh[kBlock / 64][K];
+    __shared__ double sh[NT / 64][K];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0)
         for (int k = 0; k < K; ++k) sh[w][k] = v[k];
@@ -37,7 +38,7 @@ __device__ inline void block_sum_store(double (&v)[K], double* out)
     if (threadIdx.x < kPartW) {
         double s = 0.0;
         if ((int)threadIdx.x < K)
-            for (int q = 0; q < kBlock / 64; ++q) s += sh[q][threadIdx.x];
+            for (int q = 0; q < NT / 64; ++q) s += sh[q][threadIdx.x];
         out[threadIdx.x] = s;
     }
 }

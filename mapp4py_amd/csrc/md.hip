@@ -146,10 +146,10 @@ __device__ inline void pair_dx(const double4& xi, const double4& xj, uint32_t co
 // knot interval and offset with the reference's clamps (src/ff_eam.cpp:117-128)
 __device__ inline int knot(double v, double inv, int nknots, float& p)
 {
+    // v >= 0 here (a distance), so the reference's lower clamp m >= 0 holds
     const double pm = v * inv;
     int m = (int)pm;
     m = m < nknots - 2 ? m : nknots - 2;
-    m = m > 0 ? m : 0;
     const double f = pm - (double)m;
     p = (float)(f < 1.0 ? f : 1.0);
     return m;
@@ -169,61 +169,76 @@ __device__ inline void embed(const EamArgs& P, int t, double rho, double& F, dou
     if (rho > P.rho_max) F += dF * (rho - P.rho_max);
 }
 
-// A batch of B neighbor entries is resolved in three stages -- all list
-// entries, then all neighbor positions, then all spline rows -- so each thread
-// has B independent loads in flight per stage instead of one dependent chain
-// per neighbor (the passes are latency bound, not bandwidth bound).
-struct PairGeo {
-    double dx, dy, dz, r;
-    int row;   // spline row (pair type * nr + knot), 0 when outside the cutoff
+// A batch of B list entries is resolved in three stages -- all list entries,
+// then all neighbor records, then all spline rows -- so each thread has B
+// independent loads in flight per stage instead of one dependent chain per
+// neighbor. Every lane runs whole batches without branches: entries past the
+// end of its row resolve to the atom itself and their contributions are
+// selected away, and the passes are bound by instruction issue, so the pair
+// arithmetic is kept to what the result needs (FP32 splines and reciprocal
+// root, FP64 where a sum or the spline abscissa needs it).
+
+// list entry k of slot i; the load is unconditional (a row past maxn is never
+// read) and entries past the row end become the atom itself, zero image
+__device__ inline uint32_t list_entry(const uint32_t* __restrict__ list, int ldl, int maxn, int k, int i, int cnt)
+{
+    const uint32_t raw = list[(size_t)(k < maxn ? k : maxn - 1) * ldl + i];
+    return k < cnt ? raw : ((uint32_t)i | ((uint32_t)kZeroImage << kIndexBits));
+}
+
+// whether any lane of the wave holds a shifted periodic image among B entries
+template <int B>
+__device__ inline bool wave_has_image(const uint32_t (&e)[B])
+{
+    bool img = false;
+#pragma unroll
+    for (int b = 0; b < B; ++b) img |= (e[b] >> kIndexBits) != (uint32_t)kZeroImage;
+    return __ballot(img) != 0ull;
+}
+
+// Geometry of one list entry on the record grid: the separation in grid
+// units (exact integers), 1/r (FP32), and the spline interval and offset of r.
+// Every length is that of Angstrom scaled by the power of two 1/q, so the
+// cut-off test and the interval are those of the Angstrom computation.
+struct Geo16 {
+    double dx, dy, dz;
+    float rinv;
     float p;
+    int m;
     bool in;
 };
 
-// pass-2 record: position with (type, F'(rho)) packed into the fourth slot,
-// so one gather brings everything pass 2 needs about a neighbor
-__device__ inline double pack_type_dF(int t, float dF)
+// any_img: some lane of the wave holds a shifted image in this batch (a
+// wave-uniform branch; most waves of a large box never take it)
+__device__ inline Geo16 geo16(const uint4& ri, const uint4& rj, uint32_t e, bool any_img, const Rec16Frame& R,
+                              double cut_sq_u, int nr)
 {
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)t << 32) | __float_as_uint(dF)));
-}
-__device__ inline int rec_type(double w) { return (int)((unsigned long long)__double_as_longlong(w) >> 32); }
-__device__ inline float rec_dF(double w) { return __uint_as_float((unsigned)(__double_as_longlong(w) & 0xffffffffull)); }
-
-__device__ inline PairGeo pair_geo(const double4& xi, int ti, uint32_t e, const double4& xj, int tj, const EamArgs& P,
-                                   const BoxDev& bd)
-{
-    PairGeo g;
-    pair_dx(xi, xj, e >> kIndexBits, bd.h, g.dx, g.dy, g.dz);
+    Geo16 g;
+    unsigned sx = 0u, sy = 0u, sz = 0u;
+    if (any_img) {
+        // entry 62 of the table is the zero shift
+        const uint4 s = R.shift[e >> kIndexBits];
+        sx = s.x;
+        sy = s.y;
+        sz = s.z;
+    }
+    g.dx = (double)(int)(ri.x - rj.x - sx);
+    g.dy = (double)(int)(ri.y - rj.y - sy);
+    g.dz = (double)(int)(ri.z - rj.z - sz);
     const double rsq = g.dx * g.dx + g.dy * g.dy + g.dz * g.dz;
-    const int tt = ti * P.ne + tj;
-    g.in = rsq < P.cut_sq[tt];
-    g.r = root64(rsq);
-    const int m = knot(g.r, P.dr_inv, P.nr, g.p);
-    g.row = g.in ? tt * P.nr + m : 0;
+    g.in = rsq < cut_sq_u;
+    // r to ~1e-14: FP32 reciprocal root, one FP64 Newton step for the root
+    const float sf = (float)rsq;
+    g.rinv = __builtin_amdgcn_rsqf(sf);
+    const double r0 = (double)(sf * g.rinv);
+    const double r = fma(fma(-r0, r0, rsq), (double)(0.5f * g.rinv), r0);
+    g.m = knot(r, R.knot_u, nr, g.p);
     return g;
 }
 
-// separation of two 16-byte pass-2 records with the image shift of the list entry
-__device__ inline void rec16_dx(const uint4& ri, const uint4& rj, uint32_t code, const Rec16Frame& R, double& dx,
-                                double& dy, double& dz)
-{
-    unsigned sx = 0u, sy = 0u, sz = 0u;
-    if (code != (uint32_t)kZeroImage) {
-        const int n2 = (int)(code / 25u) - 2;
-        const int rem = (int)(code % 25u);
-        const unsigned n1 = (unsigned)(rem / 5 - 2), n0 = (unsigned)(rem % 5 - 2), m2 = (unsigned)n2;
-        sx = n0 * R.cH[0] + n1 * R.cH[3] + m2 * R.cH[6];
-        sy = n1 * R.cH[4] + m2 * R.cH[7];
-        sz = m2 * R.cH[8];
-    }
-    dx = (double)(int)(ri.x - rj.x - sx) * R.q;
-    dy = (double)(int)(ri.y - rj.y - sy) * R.q;
-    dz = (double)(int)(ri.z - rj.z - sz) * R.q;
-}
-
 // 16-byte position records for the density pass: the coordinates on the
-// grid of the pass-2 records (32-bit integers modulo 2^32) and the element
-// in the fourth word. Record differences are exact integers (rec16_dx).
+// record grid (32-bit integers modulo 2^32) and the element in the fourth
+// word. Record differences are exact integers.
 __global__ __launch_bounds__(kBlock) void k_make_records(int n, const double4* __restrict__ x, Rec16Frame R,
                                                          uint4* __restrict__ rec)
 {
@@ -234,14 +249,243 @@ __global__ __launch_bounds__(kBlock) void k_make_records(int n, const double4* _
                         (unsigned)__double2ll_rn(xi.z * R.inv_q), (unsigned)(int)xi.w);
 }
 
-// pass 1: rho_i = sum_j rho_{j->i}(r_ij) (FP64 sum of FP32 spline values),
-// F(rho_i) and F'(rho_i); embedding energy reduced per workgroup. Neighbor
-// positions come from the 16-byte records (one gather per neighbor).
-template <int B>
-__global__ __launch_bounds__(kBlock) void k_eam_rho(int n, const double4* __restrict__ x, const uint4* __restrict__ rec,
-                                                    const int* __restrict__ nn, const uint32_t* __restrict__ list,
-                                                    int ldl, EamArgs P, double* __restrict__ rho_out,
-                                                    double4* __restrict__ xf, uint4* __restrict__ xr, Rec16Frame R16,
+// pass 1 per atom: rho_i = sum_j rho_{j->i}(r_ij), an FP64 sum of FP32 spline
+// values. Tab supplies the spline rows (global table or an LDS window).
+template <int B, bool ONE, class Tab>
+__device__ inline double rho_atom(int i, const uint4& ri, int cnt, const uint4* __restrict__ rec,
+                                  const uint32_t* __restrict__ list, int ldl, int maxn, const EamArgs& P,
+                                  const Rec16Frame& R, const Tab& T)
+{
+    const int ti = ONE ? 0 : (int)ri.w;
+    double rho = 0.0;
+    for (int k0 = 0; k0 < cnt; k0 += B) {
+        uint32_t e[B];
+        uint4 rj[B];
+        int row[B];
+        float p[B];
+        bool in[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) e[b] = list_entry(list, ldl, maxn, k0 + b, i, cnt);
+#pragma unroll
+        for (int b = 0; b < B; ++b) rj[b] = rec[e[b] & kIndexMask];
+        const bool any_img = wave_has_image(e);
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int tt = ONE ? 0 : ti * P.ne + (int)rj[b].w;
+            const Geo16 g = geo16(ri, rj[b], e[b], any_img, R, R.cut_sq_u[tt], P.nr);
+            in[b] = g.in && k0 + b < cnt;
+            row[b] = in[b] ? tt * P.nr + g.m : 0;
+            p[b] = g.p;
+        }
+        float4 c[B];
+        T.fetch(row, in, c);
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const float v = ((c[b].w * p[b] + c[b].z) * p[b] + c[b].y) * p[b] + c[b].x;
+            rho += (double)(in[b] ? v : 0.f);
+        }
+    }
+    return rho;
+}
+
+// density spline rows from the global table [ne*ne][nr]
+struct RhoRowsGlobal {
+    const float4* tab;
+    template <int B>
+    __device__ void fetch(const int (&row)[B], const bool (&)[B], float4 (&c)[B]) const
+    {
+#pragma unroll
+        for (int b = 0; b < B; ++b) c[b] = *(const float4*)((const char*)tab + (size_t)((uint32_t)row[b] << 4));
+    }
+};
+
+// density spline rows m_lo.. held in LDS (one element); rows below the window,
+// rare in a condensed phase, come from the global table
+struct RhoRowsLds {
+    const float4* tab;
+    const float4* win;
+    int m_lo;
+    template <int B>
+    __device__ void fetch(const int (&row)[B], const bool (&in)[B], float4 (&c)[B]) const
+    {
+        bool low[B], any = false;
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int r = row[b] - m_lo;
+            low[b] = in[b] && r < 0;
+            any |= low[b];
+            c[b] = win[r < 0 ? 0 : r];
+        }
+        if (__ballot(any) != 0ull) {
+#pragma unroll
+            for (int b = 0; b < B; ++b)
+                if (low[b]) c[b] = tab[row[b]];
+        }
+    }
+};
+
+// pass 2 per atom: pair + embedding forces on the owner (no scatter), half
+// of each full-list pair's energy (ep) and virial (w, in grid units: u dx dy).
+// One element: neighbor records are the 16-byte pass-2 records (coordinates,
+// F'); otherwise the density records (coordinates, element) and F' from dF.
+template <int B, bool ONE, class Tab>
+__device__ inline void force_atom(int i, const uint4& ri, int ti, float dFi, int cnt, const uint4* __restrict__ rec,
+                                  const uint4* __restrict__ xr, const float* __restrict__ dF,
+                                  const uint32_t* __restrict__ list, int ldl, int maxn, const EamArgs& P,
+                                  const Rec16Frame& R, const Tab& T, float& fx, float& fy, float& fz, double& ep,
+                                  double (&w)[6])
+{
+    for (int k0 = 0; k0 < cnt; k0 += B) {
+        uint32_t e[B];
+        uint4 rj[B];
+        float dFj[B];
+        Geo16 g[B];
+        int row[B];
+        bool in[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) e[b] = list_entry(list, ldl, maxn, k0 + b, i, cnt);
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const uint32_t j = e[b] & kIndexMask;
+            if (ONE) {
+                rj[b] = xr[j];
+            } else {
+                rj[b] = rec[j];
+                dFj[b] = dF[j];
+            }
+        }
+        const bool any_img = wave_has_image(e);
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int tt = ONE ? 0 : ti * P.ne + (int)rj[b].w;
+            g[b] = geo16(ri, rj[b], e[b], any_img, R, R.cut_sq_u[tt], P.nr);
+            in[b] = g[b].in && k0 + b < cnt;
+            row[b] = in[b] ? tt * P.nr + g[b].m : 0;
+            if (ONE) dFj[b] = __uint_as_float(rj[b].w);
+        }
+        // spline rows: (rho' coefficients at i, r*phi c0), (r*phi c1..c3),
+        // and with several elements (rho' coefficients at j)
+        float4 ca[B], cz[B], cb[B];
+        T.fetch(row, in, ca, cz, cb);
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const float p = g[b].p;
+            const float d_in = (ca[b].z * p + ca[b].y) * p + ca[b].x;
+            const float d_out = ONE ? d_in : (cb[b].z * p + cb[b].y) * p + cb[b].x;
+            // r*phi and its derivative in p by synthetic division
+            const float b1 = cz[b].z * p + cz[b].y;
+            const float b0 = b1 * p + cz[b].x;
+            const float z = b0 * p + ca[b].w;
+            const float zp = ((cz[b].z * p + b1) * p + b0) * P.dr_invf;
+            const float rinv = g[b].rinv * R.inv_qf;  // 1/r in 1/Angstrom
+            const float phi = z * rinv;
+            const float dphi = (zp - phi) * rinv;
+            // u = f_pair * q, so that the force is u times the grid separation
+            float u = -(dphi + dFi * d_in + dFj[b] * d_out) * g[b].rinv;
+            u = in[b] ? u : 0.f;
+            ep += (double)(in[b] ? phi : 0.f);
+            fx += u * (float)g[b].dx;
+            fy += u * (float)g[b].dy;
+            fz += u * (float)g[b].dz;
+            const double ud = (double)u;
+            const double ux = ud * g[b].dx, uy = ud * g[b].dy, uz = ud * g[b].dz;
+            w[0] += ux * g[b].dx;
+            w[1] += ux * g[b].dy;
+            w[2] += ux * g[b].dz;
+            w[3] += uy * g[b].dy;
+            w[4] += uy * g[b].dz;
+            w[5] += uz * g[b].dz;
+        }
+    }
+}
+
+// force spline rows from the global tables: [nr][2] (one element) or
+// [ne*ne][nr][3]
+template <bool ONE>
+struct ForceRowsGlobal {
+    const float4* tab;
+    template <int B>
+    __device__ void fetch(const int (&row)[B], const bool (&)[B], float4 (&ca)[B], float4 (&cz)[B],
+                          float4 (&cb)[B]) const
+    {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            if (ONE) {
+                const float4* c = (const float4*)((const char*)tab + (size_t)((uint32_t)row[b] << 5));
+                ca[b] = c[0];
+                cz[b] = c[1];
+            } else {
+                const float4* c = (const float4*)((const char*)tab + (size_t)((uint32_t)row[b] * 48u));
+                ca[b] = c[0];
+                cz[b] = c[1];
+                cb[b] = c[2];
+            }
+        }
+    }
+};
+
+// one-element force rows m_lo.. held in LDS as 28 bytes: (rho' d1..d3, r*phi
+// c0) [nwin] float4, (c1, c2) [nwin] float2, c3 [nwin] float; rows below the
+// window come from the global table
+struct ForceRowsLds {
+    const float4* tab;
+    const float4* w4;
+    const float2* w2;
+    const float* w1;
+    int m_lo;
+    template <int B>
+    __device__ void fetch(const int (&row)[B], const bool (&in)[B], float4 (&ca)[B], float4 (&cz)[B],
+                          float4 (&)[B]) const
+    {
+        bool low[B], any = false;
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            int r = row[b] - m_lo;
+            low[b] = in[b] && r < 0;
+            any |= low[b];
+            r = r < 0 ? 0 : r;
+            ca[b] = w4[r];
+            const float2 t = w2[r];
+            cz[b] = make_float4(t.x, t.y, w1[r], 0.f);
+        }
+        if (__ballot(any) != 0ull) {
+#pragma unroll
+            for (int b = 0; b < B; ++b)
+                if (low[b]) {
+                    ca[b] = tab[(size_t)row[b] * 2];
+                    cz[b] = tab[(size_t)row[b] * 2 + 1];
+                }
+        }
+    }
+};
+
+// second velocity-Verlet half kick of the owner of f_i, and m v (x) v
+__device__ inline void kick_atom(int i, int ti, float fx, float fy, float fz, const KickArgs& K,
+                                 double4* __restrict__ v, double (&mvv)[6])
+{
+    double4 vi = v[i];
+    const double m = K.mass[ti];
+    vi.x += (double)fx * K.dt2 / m;
+    vi.y += (double)fy * K.dt2 / m;
+    vi.z += (double)fz * K.dt2 / m;
+    v[i] = vi;
+    mvv[0] += vi.x * vi.x * m;
+    mvv[1] += vi.x * vi.y * m;
+    mvv[2] += vi.x * vi.z * m;
+    mvv[3] += vi.y * vi.y * m;
+    mvv[4] += vi.y * vi.z * m;
+    mvv[5] += vi.z * vi.z * m;
+}
+
+// pass 1 (one thread per atom, 256-thread workgroups): rho_i, F(rho_i) and
+// F'(rho_i); embedding energy reduced per workgroup. With one element,
+// F'(rho_i) goes into the fourth word of the pass-2 record (xr); otherwise
+// into dF.
+template <int B, bool ONE>
+__global__ __launch_bounds__(kBlock) void k_eam_rho(int n, const uint4* __restrict__ rec, const int* __restrict__ nn,
+                                                    const uint32_t* __restrict__ list, int ldl, int maxn, EamArgs P,
+                                                    Rec16Frame R, double* __restrict__ rho_out,
+                                                    uint4* __restrict__ xr, float* __restrict__ dF_out,
                                                     double* __restrict__ part)
 {
     const int blk = xcd_block();
@@ -249,178 +493,144 @@ __global__ __launch_bounds__(kBlock) void k_eam_rho(int n, const double4* __rest
     double acc[1] = {0.0};
     if (i < n) {
         const uint4 ri = rec[i];
-        const int ti = (int)ri.w;
-        const int cnt = nn[i];
-        double rho = 0.0;
-        for (int k0 = 0; k0 < cnt; k0 += B) {
-            uint32_t e[B];
-            uint4 rj[B];
-            float4 c[B];
-            PairGeo g[B];
-#pragma unroll
-            for (int b = 0; b < B; ++b) e[b] = k0 + b < cnt ? list[(size_t)(k0 + b) * ldl + i] : (uint32_t)i | (kZeroImage << kIndexBits);
-#pragma unroll
-            for (int b = 0; b < B; ++b) rj[b] = rec[e[b] & kIndexMask];
-#pragma unroll
-            for (int b = 0; b < B; ++b) {
-                rec16_dx(ri, rj[b], e[b] >> kIndexBits, R16, g[b].dx, g[b].dy, g[b].dz);
-                const double rsq = g[b].dx * g[b].dx + g[b].dy * g[b].dy + g[b].dz * g[b].dz;
-                const int tt = ti * P.ne + (int)rj[b].w;
-                g[b].in = rsq < P.cut_sq[tt] && (k0 + b < cnt);
-                g[b].r = root64(rsq);
-                const int m = knot(g[b].r, P.dr_inv, P.nr, g[b].p);
-                g[b].row = g[b].in ? tt * P.nr + m : 0;
-            }
-#pragma unroll
-            for (int b = 0; b < B; ++b) c[b] = P.tabA[g[b].in ? g[b].row : 0];
-#pragma unroll
-            for (int b = 0; b < B; ++b) {
-                const float p = g[b].p;
-                const float v = ((c[b].w * p + c[b].z) * p + c[b].y) * p + c[b].x;
-                if (g[b].in) rho += (double)v;
-            }
-        }
+        const int ti = ONE ? 0 : (int)ri.w;
+        const double rho = rho_atom<B, ONE>(i, ri, nn[i], rec, list, ldl, maxn, P, R, RhoRowsGlobal{P.tabA});
         double F, dF;
         embed(P, ti, rho, F, dF);
         rho_out[i] = rho;
-        if (xr) {
+        if (ONE)
             xr[i] = make_uint4(ri.x, ri.y, ri.z, __float_as_uint((float)dF));
-        } else {
-            const double4 xi = x[i];
-            xf[i] = make_double4(xi.x, xi.y, xi.z, pack_type_dF(ti, (float)dF));
-        }
+        else
+            dF_out[i] = (float)dF;
         acc[0] = F;
     }
     block_sum_store<1>(acc, part + (size_t)blk * kPartW);
 }
 
-// second half kick of velocity Verlet, fused into the force pass: the owner
-// of f_i updates v_i and adds m v (x) v to the kinetic tensor
-
-// pass 2: pair + embedding forces on the owner only (no scatter), half of
-// each full-list pair's energy and virial
-template <int B, bool ONE, bool R16>
-__global__ __launch_bounds__(kBlock) void k_eam_force(int n, const double4* __restrict__ x,
-                                                      const uint4* __restrict__ xr, Rec16Frame RF,
-                                                      const int* __restrict__ nn, const uint32_t* __restrict__ list,
-                                                      int ldl, EamArgs P, BoxDev bd,
+// pass 2 (one thread per atom, 256-thread workgroups) with the fused second
+// half kick when K.on
+template <int B, bool ONE>
+__global__ __launch_bounds__(kBlock) void k_eam_force(int n, const uint4* __restrict__ rec,
+                                                      const uint4* __restrict__ xr, const float* __restrict__ dF,
+                                                      Rec16Frame R, const int* __restrict__ nn,
+                                                      const uint32_t* __restrict__ list, int ldl, int maxn, EamArgs P,
                                                       float4* __restrict__ f_out, double* __restrict__ part,
-                                                      KickArgs K, double4* __restrict__ v, double* __restrict__ part_mvv)
+                                                      KickArgs K, double4* __restrict__ v,
+                                                      double* __restrict__ part_mvv)
 {
-    // x (or xr, for single-element tables) holds the pass-2 records written by the density pass
     const int blk = xcd_block();
     const int i = blk * kBlock + threadIdx.x;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     double mvv[6] = {0, 0, 0, 0, 0, 0};
     if (i < n) {
-        double4 xi = make_double4(0.0, 0.0, 0.0, 0.0);
-        uint4 ri = make_uint4(0u, 0u, 0u, 0u);
-        int ti = 0;
-        float dFi;
-        if (R16) {
-            ri = xr[i];
-            dFi = __uint_as_float(ri.w);
-        } else {
-            xi = x[i];
-            ti = rec_type(xi.w);
-            dFi = rec_dF(xi.w);
-        }
-        const int cnt = nn[i];
+        const uint4 ri = ONE ? xr[i] : rec[i];
+        const int ti = ONE ? 0 : (int)ri.w;
+        const float dFi = ONE ? __uint_as_float(ri.w) : dF[i];
         float fx = 0.f, fy = 0.f, fz = 0.f;
-        for (int k0 = 0; k0 < cnt; k0 += B) {
-            uint32_t e[B];
-            double4 xj[B];
-            uint4 rj[B];
-            float dFj[B];
-            float4 ca[B], cb[B], cp[B];
-            PairGeo g[B];
-#pragma unroll
-            for (int b = 0; b < B; ++b) e[b] = k0 + b < cnt ? list[(size_t)(k0 + b) * ldl + i] : (uint32_t)i | (kZeroImage << kIndexBits);
-            if (R16) {
-#pragma unroll
-                for (int b = 0; b < B; ++b) rj[b] = xr[e[b] & kIndexMask];
-#pragma unroll
-                for (int b = 0; b < B; ++b) {
-                    rec16_dx(ri, rj[b], e[b] >> kIndexBits, RF, g[b].dx, g[b].dy, g[b].dz);
-                    const double rsq = g[b].dx * g[b].dx + g[b].dy * g[b].dy + g[b].dz * g[b].dz;
-                    g[b].in = rsq < P.cut_sq[0] && (k0 + b < cnt);
-                    g[b].r = root64(rsq);
-                    const int m = knot(g[b].r, P.dr_inv, P.nr, g[b].p);
-                    g[b].row = g[b].in ? m : 0;
-                    dFj[b] = __uint_as_float(rj[b].w);
-                }
-            } else {
-#pragma unroll
-                for (int b = 0; b < B; ++b) xj[b] = x[e[b] & kIndexMask];
-#pragma unroll
-                for (int b = 0; b < B; ++b) {
-                    g[b] = pair_geo(xi, ti, e[b], xj[b], rec_type(xj[b].w), P, bd);
-                    g[b].in = g[b].in && (k0 + b < cnt);
-                    dFj[b] = rec_dF(xj[b].w);
-                }
-            }
-#pragma unroll
-            for (int b = 0; b < B; ++b) {
-                if (ONE) {
-                    // one element: rho_{j->i} = rho_{i->j}; row = (rho c1..c3, rphi c0), (rphi c1..c3, -)
-                    const float4* c = P.tabC + (size_t)(g[b].in ? g[b].row : 0) * 2;
-                    const float4 u = c[0], v = c[1];
-                    ca[b] = make_float4(0.f, u.x, u.y, u.z);
-                    cb[b] = ca[b];
-                    cp[b] = make_float4(u.w, v.x, v.y, v.z);
-                } else {
-                    const float4* c = P.tabB + (size_t)(g[b].in ? g[b].row : 0) * 3;
-                    ca[b] = c[0];
-                    cb[b] = c[1];
-                    cp[b] = c[2];
-                }
-            }
-#pragma unroll
-            for (int b = 0; b < B; ++b) {
-                if (!g[b].in) continue;
-                const float p = g[b].p;
-                const float rinv = 1.0f / (float)g[b].r;
-                const float d_in = ((3.f * ca[b].w * p + 2.f * ca[b].z) * p + ca[b].y) * P.dr_invf;
-                const float d_out = ((3.f * cb[b].w * p + 2.f * cb[b].z) * p + cb[b].y) * P.dr_invf;
-                const float z = ((cp[b].w * p + cp[b].z) * p + cp[b].y) * p + cp[b].x;
-                const float zp = ((3.f * cp[b].w * p + 2.f * cp[b].z) * p + cp[b].y) * P.dr_invf;
-                const float phi = z * rinv;
-                const float dphi = (zp - phi) * rinv;
-                const float fpair = -(dphi + dFi * d_in + dFj[b] * d_out) * rinv;
-                const double dx = g[b].dx, dy = g[b].dy, dz = g[b].dz;
-                fx += fpair * (float)dx;
-                fy += fpair * (float)dy;
-                fz += fpair * (float)dz;
-                const double hf = 0.5 * (double)fpair;
-                acc[0] += 0.5 * (double)phi;
-                acc[1] += hf * dx * dx;
-                acc[2] += hf * dx * dy;
-                acc[3] += hf * dx * dz;
-                acc[4] += hf * dy * dy;
-                acc[5] += hf * dy * dz;
-                acc[6] += hf * dz * dz;
-            }
-        }
+        double ep = 0.0, w[6] = {0, 0, 0, 0, 0, 0};
+        force_atom<B, ONE>(i, ri, ti, dFi, nn[i], rec, xr, dF, list, ldl, maxn, P, R,
+                           ForceRowsGlobal<ONE>{ONE ? P.tabC : P.tabB}, fx, fy, fz, ep, w);
         f_out[i] = make_float4(fx, fy, fz, 0.f);
-        if (K.on) {
-            double4 vi = v[i];
-            const double m = K.mass[ti];
-            vi.x += (double)fx * K.dt2 / m;
-            vi.y += (double)fy * K.dt2 / m;
-            vi.z += (double)fz * K.dt2 / m;
-            v[i] = vi;
-            mvv[0] = vi.x * vi.x * m;
-            mvv[1] = vi.x * vi.y * m;
-            mvv[2] = vi.x * vi.z * m;
-            mvv[3] = vi.y * vi.y * m;
-            mvv[4] = vi.y * vi.z * m;
-            mvv[5] = vi.z * vi.z * m;
-        }
+        acc[0] = 0.5 * ep;
+        const double hq = 0.5 * R.q;  // f_pair dx_A dy_A / 2 = (q/2) u dx dy
+        for (int k = 0; k < 6; ++k) acc[1 + k] = hq * w[k];
+        if (K.on) kick_atom(i, ti, fx, fy, fz, K, v, mvv);
     }
     block_sum_store<7>(acc, part + (size_t)blk * kPartW);
     if (K.on) {
         __syncthreads();
         block_sum_store<6>(mvv, part_mvv + (size_t)blk * kPartW);
+    }
+}
+
+// Persistent passes for one element with the spline rows in LDS: one
+// 1024-thread workgroup per CU loads the row window once and walks chunks of
+// kBigBlock atoms. The chunks form 8 contiguous ranges, one per XCD (where the
+// workgroup runs), so neighbor records stay in that XCD's L2.
+constexpr int kBigBlock = 1024;
+
+struct ChunkWalk {
+    int c, end, step;
+};
+__device__ inline ChunkWalk chunk_walk(int n)
+{
+    const int nchunks = (n + kBigBlock - 1) / kBigBlock;
+    const int per = (nchunks + kXcd - 1) / kXcd;
+    const int x = blockIdx.x % kXcd;
+    ChunkWalk w;
+    w.c = x * per + (int)blockIdx.x / kXcd;
+    w.end = min((x + 1) * per, nchunks);
+    w.step = gridDim.x / kXcd;
+    return w;
+}
+
+template <int B>
+__global__ __launch_bounds__(kBigBlock) void k_eam_rho_lds(int n, const uint4* __restrict__ rec,
+                                                           const int* __restrict__ nn,
+                                                           const uint32_t* __restrict__ list, int ldl, int maxn,
+                                                           EamArgs P, Rec16Frame R, int m_lo, int nwin,
+                                                           double* __restrict__ rho_out, uint4* __restrict__ xr,
+                                                           double* __restrict__ part)
+{
+    extern __shared__ float4 lds_rows[];
+    for (int r = threadIdx.x; r < nwin; r += kBigBlock) lds_rows[r] = P.tabA[m_lo + r];
+    __syncthreads();
+    const RhoRowsLds T{P.tabA, lds_rows, m_lo};
+    double acc[1] = {0.0};
+    for (ChunkWalk w = chunk_walk(n); w.c < w.end; w.c += w.step) {
+        const int i = w.c * kBigBlock + threadIdx.x;
+        if (i >= n) continue;
+        const uint4 ri = rec[i];
+        const double rho = rho_atom<B, true>(i, ri, nn[i], rec, list, ldl, maxn, P, R, T);
+        double F, dF;
+        embed(P, 0, rho, F, dF);
+        rho_out[i] = rho;
+        xr[i] = make_uint4(ri.x, ri.y, ri.z, __float_as_uint((float)dF));
+        acc[0] += F;
+    }
+    block_sum_store<1, kBigBlock>(acc, part + (size_t)blockIdx.x * kPartW);
+}
+
+template <int B>
+__global__ __launch_bounds__(kBigBlock) void k_eam_force_lds(int n, const uint4* __restrict__ xr, Rec16Frame R,
+                                                             const int* __restrict__ nn,
+                                                             const uint32_t* __restrict__ list, int ldl, int maxn,
+                                                             EamArgs P, int m_lo, int nwin,
+                                                             float4* __restrict__ f_out, double* __restrict__ part,
+                                                             KickArgs K, double4* __restrict__ v,
+                                                             double* __restrict__ part_mvv)
+{
+    extern __shared__ float4 lds_rows[];
+    float2* w2 = (float2*)(lds_rows + nwin);
+    float* w1 = (float*)(w2 + nwin);
+    for (int r = threadIdx.x; r < nwin; r += kBigBlock) {
+        const float4 a = P.tabC[(size_t)(m_lo + r) * 2], c = P.tabC[(size_t)(m_lo + r) * 2 + 1];
+        lds_rows[r] = a;
+        w2[r] = make_float2(c.x, c.y);
+        w1[r] = c.z;
+    }
+    __syncthreads();
+    const ForceRowsLds T{P.tabC, lds_rows, w2, w1, m_lo};
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    double mvv[6] = {0, 0, 0, 0, 0, 0};
+    const double hq = 0.5 * R.q;  // f_pair dx_A dy_A / 2 = (q/2) u dx dy
+    for (ChunkWalk cw = chunk_walk(n); cw.c < cw.end; cw.c += cw.step) {
+        const int i = cw.c * kBigBlock + threadIdx.x;
+        if (i >= n) continue;
+        const uint4 ri = xr[i];
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        double ep = 0.0, w[6] = {0, 0, 0, 0, 0, 0};
+        force_atom<B, true>(i, ri, 0, __uint_as_float(ri.w), nn[i], nullptr, xr, nullptr, list, ldl, maxn, P, R, T,
+                            fx, fy, fz, ep, w);
+        f_out[i] = make_float4(fx, fy, fz, 0.f);
+        acc[0] += 0.5 * ep;
+        for (int k = 0; k < 6; ++k) acc[1 + k] += hq * w[k];
+        if (K.on) kick_atom(i, 0, fx, fy, fz, K, v, mvv);
+    }
+    block_sum_store<7, kBigBlock>(acc, part + (size_t)blockIdx.x * kPartW);
+    if (K.on) {
+        __syncthreads();
+        block_sum_store<6, kBigBlock>(mvv, part_mvv + (size_t)blockIdx.x * kPartW);
     }
 }
 
@@ -448,10 +658,11 @@ __global__ __launch_bounds__(kBlock) void k_eam_energy(int n, const double4* __r
                 const double r = root64(rsq);
                 float p;
                 const int m = knot(r, P.dr_inv, P.nr, p);
+                const float4 ca = P.tabA[(size_t)tt * P.nr + m];
                 const float4* c = P.tabB + ((size_t)tt * P.nr + m) * 3;
-                const float4 ca = c[0], cp = c[2];
+                const float4 c0 = c[0], cp = c[1];
                 rho += (double)(((ca.w * p + ca.z) * p + ca.y) * p + ca.x);
-                ep += 0.5 * (double)((((cp.w * p + cp.z) * p + cp.y) * p + cp.x) / (float)r);
+                ep += 0.5 * (double)((((cp.z * p + cp.y) * p + cp.x) * p + c0.w) / (float)r);
             }
         }
         double F, dF;
@@ -656,16 +867,23 @@ EamForceField::EamForceField(EamTables t) : t_(std::move(t))
     MAPP_REQUIRE(ne >= 1 && ne <= kMaxElems, ERR_ARG, "1 to 8 elements are supported");
     std::vector<float4> A((size_t)ne * ne * nr), B((size_t)ne * ne * nr * 3);
     auto f4 = [](const double* c) { return make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[3]); };
+    // derivative coefficients of a density spline, d/dr in 1/Angstrom: the
+    // factors 1, 2, 3 and 1/dr applied in FP64 before the rounding to FP32
+    const double dri = 1.0 / t_.dr;
+    auto dcoef = [dri](const double* c, float w) {
+        return make_float4((float)(c[1] * dri), (float)(2.0 * c[2] * dri), (float)(3.0 * c[3] * dri), w);
+    };
     for (int a = 0; a < ne; ++a)
         for (int b = 0; b < ne; ++b) {
             const size_t tt = (size_t)a * ne + b;
             for (int m = 0; m < nr; ++m) {
-                const float4 in = f4(t_.rho_at(b, a) + (size_t)m * 4);   // density at a due to b
-                const float4 out = f4(t_.rho_at(a, b) + (size_t)m * 4);  // density at b due to a
-                A[tt * nr + m] = in;
-                B[(tt * nr + m) * 3 + 0] = in;
-                B[(tt * nr + m) * 3 + 1] = out;
-                B[(tt * nr + m) * 3 + 2] = f4(t_.rphi_at(a, b) + (size_t)m * 4);
+                const double* in = t_.rho_at(b, a) + (size_t)m * 4;   // density at a due to b
+                const double* out = t_.rho_at(a, b) + (size_t)m * 4;  // density at b due to a
+                const double* rp = t_.rphi_at(a, b) + (size_t)m * 4;
+                A[tt * nr + m] = f4(in);
+                B[(tt * nr + m) * 3 + 0] = dcoef(in, (float)rp[0]);
+                B[(tt * nr + m) * 3 + 1] = make_float4((float)rp[1], (float)rp[2], (float)rp[3], 0.f);
+                B[(tt * nr + m) * 3 + 2] = dcoef(out, 0.f);
             }
         }
     // one element: the density pair is symmetric, so a row needs only the
@@ -673,10 +891,8 @@ EamForceField::EamForceField(EamTables t) : t_(std::move(t))
     std::vector<float4> Cc(ne == 1 ? (size_t)nr * 2 : 1);
     if (ne == 1)
         for (int m = 0; m < nr; ++m) {
-            const double* r = t_.rho_at(0, 0) + (size_t)m * 4;
-            const double* p = t_.rphi_at(0, 0) + (size_t)m * 4;
-            Cc[(size_t)m * 2] = make_float4((float)r[1], (float)r[2], (float)r[3], (float)p[0]);
-            Cc[(size_t)m * 2 + 1] = make_float4((float)p[1], (float)p[2], (float)p[3], 0.f);
+            Cc[(size_t)m * 2] = B[(size_t)m * 3];
+            Cc[(size_t)m * 2 + 1] = B[(size_t)m * 3 + 1];
         }
     tabC_.reserve(Cc.size());
     MAPP_HIP(hipMemcpy(tabC_.get(), Cc.data(), Cc.size() * sizeof(float4), hipMemcpyHostToDevice));
@@ -730,7 +946,7 @@ MDSystem::MDSystem(std::shared_ptr<EamForceField> ff, int natms, const double* H
     setup_grid();
     x_.reserve(n_); x_alt_.reserve(n_); s_.reserve(n_); x0_.reserve(n_);
     v_.reserve(n_); v_alt_.reserve(n_); id_.reserve(n_); id_alt_.reserve(n_); slot_of_.reserve(n_);
-    f_.reserve(n_); rho_.reserve(n_); xf_.reserve(n_); xr_.reserve(n_); rec_.reserve(n_);
+    f_.reserve(n_); rho_.reserve(n_); dF_.reserve(n_); xr_.reserve(n_); rec_.reserve(n_);
     keys_.reserve(n_); keys_alt_.reserve(n_); vals_.reserve(n_); vals_alt_.reserve(n_); s_tmp_.reserve(n_);
     nn_.reserve(n_);
     flags_.reserve(4);
@@ -747,6 +963,7 @@ MDSystem::MDSystem(std::shared_ptr<EamForceField> ff, int natms, const double* H
     MAPP_HIP(hipMemcpyAsync(id_.get(), idh.data(), sizeof(int) * n_, hipMemcpyHostToDevice, stream_));
     MAPP_HIP(hipMemsetAsync(v_.get(), 0, sizeof(double4) * n_, stream_));
     MAPP_HIP(hipMemsetAsync(f_.get(), 0, sizeof(float4) * n_, stream_));
+    plan_lds();
     rebuild();
 }
 
@@ -769,9 +986,50 @@ void MDSystem::reduce_rows(const double* part, int nrows, int ncols, double* out
     hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, stream_, g, ncols, red2_.get(), out, g);
 }
 
+// LDS row windows of the persistent one-element passes: the rows of the
+// largest r that fit the device's workgroup LDS (16 B per density row, 28 B
+// per force row), leaving room for the static reduction arrays
+void MDSystem::plan_lds()
+{
+    lds_ok_ = use_lds_ = false;
+    const EamArgs P = ff_->args();
+    if (P.ne != 1 || P.nr < 2) return;
+    int dev = 0, lds_max = 0, ncu = 0;
+    MAPP_HIP(hipGetDevice(&dev));
+    MAPP_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    MAPP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const long budget = (long)lds_max - 4096;
+    if (budget < 16384 || ncu < 1) return;
+    const int rows = P.nr - 1;  // spline intervals 0 .. nr-2
+    rho_nwin_ = (int)std::min<long>(rows, budget / 16);
+    rho_m_lo_ = rows - rho_nwin_;
+    f_nwin_ = (int)std::min<long>(rows, budget / 28);
+    f_m_lo_ = rows - f_nwin_;
+    rho_lds_bytes_ = (size_t)rho_nwin_ * 16;
+    f_lds_bytes_ = (size_t)f_nwin_ * 28;
+    lds_grid_ = std::max(1, ncu / kXcd) * kXcd;
+    const void* fns[] = {(const void*)k_eam_rho_lds<1>, (const void*)k_eam_rho_lds<2>, (const void*)k_eam_rho_lds<4>,
+                         (const void*)k_eam_rho_lds<8>, (const void*)k_eam_force_lds<1>,
+                         (const void*)k_eam_force_lds<2>};
+    for (int k = 0; k < 6; ++k) {
+        const size_t bytes = k < 4 ? rho_lds_bytes_ : f_lds_bytes_;
+        if (hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return;
+        }
+    }
+    lds_ok_ = use_lds_ = true;
+}
+
+bool MDSystem::set_lds(bool on)
+{
+    use_lds_ = on && lds_ok_;
+    return use_lds_;
+}
+
 void MDSystem::set_batch(int rho, int force)
 {
-    MAPP_REQUIRE(rho == 1 || rho == 4 || rho == 8, ERR_ARG, "rho batch must be 1, 4 or 8");
+    MAPP_REQUIRE(rho == 1 || rho == 2 || rho == 4 || rho == 8, ERR_ARG, "rho batch must be 1, 2, 4 or 8");
     MAPP_REQUIRE(force == 1 || force == 2 || force == 4 || force == 8, ERR_ARG, "force batch must be 1, 2, 4 or 8");
     batch_rho_ = rho;
     batch_force_ = force;
@@ -803,6 +1061,25 @@ void MDSystem::setup_grid()
     for (int k = 0; k < 3; ++k)
         for (int d = 0; d < 3; ++d)
             r16_.cH[3 * k + d] = (unsigned)(unsigned long long)std::llrint(box_.H[k][d] * r16_.inv_q);
+    // grid shift of every image code (n0+2) + 5 (n1+2) + 25 (n2+2)
+    {
+        std::vector<uint4> sh(128, make_uint4(0u, 0u, 0u, 0u));  // 7-bit codes; 125..127 unused
+        for (int c = 0; c < 125; ++c) {
+            const unsigned n0 = (unsigned)(c % 5 - 2), n1 = (unsigned)(c / 5 % 5 - 2), n2 = (unsigned)(c / 25 - 2);
+            sh[c] = make_uint4(n0 * r16_.cH[0] + n1 * r16_.cH[3] + n2 * r16_.cH[6],
+                               n1 * r16_.cH[4] + n2 * r16_.cH[7], n2 * r16_.cH[8], 0u);
+        }
+        shift_.reserve(sh.size());
+        MAPP_HIP(hipMemcpy(shift_.get(), sh.data(), sizeof(uint4) * sh.size(), hipMemcpyHostToDevice));
+        r16_.shift = shift_.get();
+    }
+    // q is a power of two: the scaled cut-offs and knot spacing are exact
+    {
+        const EamArgs P = ff_->args();
+        for (int k = 0; k < P.ne * P.ne; ++k) r16_.cut_sq_u[k] = P.cut_sq[k] * r16_.inv_q * r16_.inv_q;
+        r16_.knot_u = r16_.q * P.dr_inv;
+        r16_.inv_qf = (float)r16_.inv_q;
+    }
     // Morton rank of every cell so that cell-sorted atoms are spatially compact
     std::vector<std::pair<uint32_t, int>> code(ncells);
     for (int c2 = 0, c = 0; c2 < grid_.nc[2]; ++c2)
@@ -903,31 +1180,62 @@ void MDSystem::force_passes(const KickArgs* kick)
 {
     const int nb = grid_for(n_);
     const EamArgs P = ff_->args();
-    BoxDev bd;
-    std::memcpy(bd.h, box_.h, sizeof bd.h);
-    std::memcpy(bd.b, box_.b, sizeof bd.b);
-    MAPP_HIP(hipMemsetAsync(flags_.get() + 2, 0, sizeof(int), stream_));
     prof_.begin(K_EAM_RHO, stream_);
     hipLaunchKernelGGL(k_make_records, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), r16_, rec_.get());
     const bool one = P.ne == 1;
-    uint4* xr = one ? xr_.get() : nullptr;
-    switch (batch_rho_) {
-        case 1: hipLaunchKernelGGL(k_eam_rho<1>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), nn_.get(), list_.get(), ldl_, P, rho_.get(), xf_.get(), xr, r16_, part_.get()); break;
-        case 4: hipLaunchKernelGGL(k_eam_rho<4>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), nn_.get(), list_.get(), ldl_, P, rho_.get(), xf_.get(), xr, r16_, part_.get()); break;
-        default: hipLaunchKernelGGL(k_eam_rho<8>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), rec_.get(), nn_.get(), list_.get(), ldl_, P, rho_.get(), xf_.get(), xr, r16_, part_.get()); break;
-    }
-    prof_.end(stream_);
-    prof_.begin(K_EAM_FORCE, stream_);
-    double* p2 = part_.get() + (size_t)nb * kPartW;
+    const bool lds = one && use_lds_;
     KickArgs K{};
     if (kick) {
         K = *kick;
         K.on = 1;
     }
     double* p3 = part_.get() + (size_t)2 * nb * kPartW;
-#define MAPP_FORCE(B, ONE) hipLaunchKernelGGL((k_eam_force<B, ONE, ONE>), dim3(nb), dim3(kBlock), 0, stream_, n_, \
-                                              xf_.get(), xr_.get(), r16_, nn_.get(), list_.get(), ldl_, P, bd, \
-                                              f_.get(), p2, K, v_.get(), p3)
+    if (lds) {
+        // at most one workgroup per CU, and never more rows than the partial-sum buffer has per pass
+        const int G = std::min(lds_grid_, nb);
+#define MAPP_RHO(B) hipLaunchKernelGGL((k_eam_rho_lds<B>), dim3(G), dim3(kBigBlock), rho_lds_bytes_, stream_, n_, \
+                                       rec_.get(), nn_.get(), list_.get(), ldl_, maxn_, P, r16_, rho_m_lo_, rho_nwin_, \
+                                       rho_.get(), xr_.get(), part_.get())
+        switch (batch_rho_) {
+            case 1: MAPP_RHO(1); break;
+            case 2: MAPP_RHO(2); break;
+            case 4: MAPP_RHO(4); break;
+            default: MAPP_RHO(8); break;
+        }
+#undef MAPP_RHO
+        prof_.end(stream_);
+        prof_.begin(K_EAM_FORCE, stream_);
+        double* p2 = part_.get() + (size_t)G * kPartW;
+#define MAPP_FORCE(B) hipLaunchKernelGGL((k_eam_force_lds<B>), dim3(G), dim3(kBigBlock), f_lds_bytes_, stream_, n_, \
+                                         xr_.get(), r16_, nn_.get(), list_.get(), ldl_, maxn_, P, f_m_lo_, f_nwin_, \
+                                         f_.get(), p2, K, v_.get(), p3)
+        // 1024-thread workgroups leave 128 VGPRs per lane: batches above 2 would spill
+        if (batch_force_ == 1)
+            MAPP_FORCE(1);
+        else
+            MAPP_FORCE(2);
+#undef MAPP_FORCE
+        prof_.end(stream_);
+        mvv_rows_ = G;
+        reduce_rows(part_.get(), 2 * G, 7, red_.get());
+        return;
+    }
+#define MAPP_RHO(B, ONE) hipLaunchKernelGGL((k_eam_rho<B, ONE>), dim3(nb), dim3(kBlock), 0, stream_, n_, rec_.get(), \
+                                            nn_.get(), list_.get(), ldl_, maxn_, P, r16_, rho_.get(), xr_.get(), \
+                                            dF_.get(), part_.get())
+    switch (batch_rho_) {
+        case 1: if (one) MAPP_RHO(1, true); else MAPP_RHO(1, false); break;
+        case 2: if (one) MAPP_RHO(2, true); else MAPP_RHO(2, false); break;
+        case 4: if (one) MAPP_RHO(4, true); else MAPP_RHO(4, false); break;
+        default: if (one) MAPP_RHO(8, true); else MAPP_RHO(8, false); break;
+    }
+#undef MAPP_RHO
+    prof_.end(stream_);
+    prof_.begin(K_EAM_FORCE, stream_);
+    double* p2 = part_.get() + (size_t)nb * kPartW;
+#define MAPP_FORCE(B, ONE) hipLaunchKernelGGL((k_eam_force<B, ONE>), dim3(nb), dim3(kBlock), 0, stream_, n_, \
+                                              rec_.get(), xr_.get(), dF_.get(), r16_, nn_.get(), list_.get(), ldl_, \
+                                              maxn_, P, f_.get(), p2, K, v_.get(), p3)
     switch (batch_force_) {
         case 1: if (one) MAPP_FORCE(1, true); else MAPP_FORCE(1, false); break;
         case 2: if (one) MAPP_FORCE(2, true); else MAPP_FORCE(2, false); break;
@@ -936,6 +1244,7 @@ void MDSystem::force_passes(const KickArgs* kick)
     }
 #undef MAPP_FORCE
     prof_.end(stream_);
+    mvv_rows_ = nb;
     reduce_rows(part_.get(), 2 * nb, 7, red_.get());
 }
 
@@ -1178,7 +1487,7 @@ int MDSystem::nvt_run(const double* masses, double kB, double T, double dt, int 
         if (flag) rebuild();
         force_passes(&kick);
         prof_.begin(K_INTEGRATE, stream_);
-        reduce_rows(part_.get() + (size_t)2 * nb * kPartW, nb, 6, mvv);
+        reduce_rows(part_.get() + (size_t)2 * nb * kPartW, mvv_rows_, 6, mvv);
         hipLaunchKernelGGL(k_nhc, dim3(1), dim3(64), 0, stream_, nh, mvv, A, 2);
         prof_.end(stream_);
     }

@@ -21,8 +21,8 @@ struct EamArgs {
     float dr_invf;
     double cut_sq[kMaxElems * kMaxElems];
     const float4* tabA;   // [ne*ne][nr]: density at i due to j (rho[tj][ti])
-    const float4* tabB;   // [ne*ne][nr][3]: rho[tj][ti], rho[ti][tj], r*phi[ti][tj]
-    const float4* tabC;   // one element only: [nr][2] = rho c1..c3, r*phi c0 | r*phi c1..c3
+    const float4* tabB;   // [ne*ne][nr][3]: (rho'[tj][ti] d1..d3, r*phi c0), (r*phi c1..c3, 0), (rho'[ti][tj] d1..d3, 0)
+    const float4* tabC;   // one element only: [nr][2] = (rho' d1..d3, r*phi c0), (r*phi c1..c3, 0)
     const double* Ftab;   // [ne][nrho][4]
 };
 
@@ -56,6 +56,10 @@ struct CellGrid {
 struct Rec16Frame {
     double q, inv_q;
     unsigned cH[9];  // rows of H on the grid, modulo 2^32: cH[3 * row + dim]
+    double cut_sq_u[kMaxElems * kMaxElems];  // squared cut-offs in grid units (cut^2 / q^2)
+    double knot_u;                           // spline knots per grid unit (q / dr)
+    float inv_qf;                            // 1/q
+    const uint4* shift;                      // [128] image code -> grid shift (x, y, z) modulo 2^32
 };
 
 // second velocity-Verlet half kick carried out by the force pass
@@ -99,6 +103,10 @@ public:
     Profiler& profiler() { return prof_; }
     // launch-shape knobs picked by the tuner: neighbors resolved per batch
     void set_batch(int rho, int force);
+    // one-element tables: spline rows in LDS (persistent passes) or read from
+    // the global tables; returns whether the LDS passes are in use
+    bool set_lds(bool on);
+    bool lds() const { return use_lds_; }
     int batch_rho() const { return batch_rho_; }
     int batch_force() const { return batch_force_; }
     void sync();
@@ -128,10 +136,11 @@ private:
     DevBuf<int> slot_of_;            // caller index -> slot
     DevBuf<float4> f_;               // forces (w unused)
     DevBuf<double> rho_;
-    DevBuf<uint4> rec_;              // fixed-point position records (density pass)
-    DevBuf<double4> xf_;             // pass-2 records: x, y, z, (type, F'(rho))
-    DevBuf<uint4> xr_;               // 16-byte pass-2 records (single-element tables)
+    DevBuf<uint4> rec_;              // 16-byte position records (coordinates on the grid, element)
+    DevBuf<uint4> xr_;               // 16-byte pass-2 records (coordinates, F'(rho)), one element
+    DevBuf<float> dF_;               // F'(rho) per slot, several elements
     Rec16Frame r16_{};
+    DevBuf<uint4> shift_;            // image shifts of r16_
     DevBuf<uint32_t> keys_, keys_alt_;
     DevBuf<int> vals_, vals_alt_;
     DevBuf<double4> s_tmp_;
@@ -149,6 +158,13 @@ private:
     std::vector<double> masses_;
     Profiler prof_;
     int batch_rho_ = 8, batch_force_ = 8;
+    // persistent one-element passes: LDS row windows [m_lo, m_lo + nwin)
+    bool lds_ok_ = false, use_lds_ = false;
+    int lds_grid_ = 0;
+    int rho_m_lo_ = 0, rho_nwin_ = 0, f_m_lo_ = 0, f_nwin_ = 0;
+    size_t rho_lds_bytes_ = 0, f_lds_bytes_ = 0;
+    int mvv_rows_ = 0;               // rows of kinetic partial sums written by the last force pass
+    void plan_lds();
 };
 
 }  // namespace mapp

@@ -219,7 +219,38 @@ def test_every_batch_shape_gives_the_oracle_forces(M, cu_pot):
     x = jitter(x, 0.05, 21)
     elem = np.zeros(len(x), np.uint8)
     g, o = both(M, cu_pot, H, x, elem)
-    for rb in (1, 4, 8):
-        for fb in (1, 2, 4, 8):
-            g.set_batch(rb, fb)
-            check_forces(g, o, f"batch rho={rb} force={fb}")
+    assert g.set_lds(True), "an MI355X holds the one-element spline rows in LDS"
+    for lds in (True, False):
+        assert g.set_lds(lds) == lds
+        for rb in (1, 2, 4, 8):
+            for fb in (1, 2, 4, 8):
+                g.set_batch(rb, fb)
+                check_forces(g, o, f"lds={lds} batch rho={rb} force={fb}")
+
+
+def test_spline_rows_below_the_lds_window(M, cu_pot):
+    # The LDS force pass holds the rows of the largest r that fit (about
+    # 5,700 of the 9,999 intervals of this table, r > ~2.15 A); a compressed
+    # lattice (nearest neighbors near 2.05 A) sends the first-shell pairs to
+    # the global-table rows below the window. Both modes give the oracle's
+    # forces, and the same forces to FP32 rounding. At this density the pair
+    # terms are large and cancel, so energy and virial are held to FP32
+    # rounding relative to the largest pair term rather than per atom.
+    x, H = pot.fcc(2.9, (6, 6, 6))
+    x = jitter(x, 0.05, 22)
+    elem = np.zeros(len(x), np.uint8)
+    g, o = both(M, cu_pot, H, x, elem)
+    fo, eo, FHo, rho_o = o.force()
+    out = {}
+    for lds in (True, False):
+        assert g.set_lds(lds) == lds
+        f, e, FH = g.force_calc()
+        df = np.abs(f - fo).max()
+        say(f"compressed fcc a=2.9, lds={lds}: max|df|={df:.2e} eV/A  dE/E={abs(e - eo) / abs(eo):.1e}  "
+            f"max|dF_H|/|F_H|max={np.abs(FH - FHo).max() / np.abs(FHo).max():.1e}")
+        assert df < F_TOL
+        assert abs(e - eo) < 1e-6 * abs(eo)
+        assert np.abs(FH - FHo).max() < 1e-6 * np.abs(FHo).max()
+        np.testing.assert_allclose(g.rho(), rho_o, rtol=2e-7)
+        out[lds] = f
+    assert np.abs(out[True] - out[False]).max() < 1e-5
