@@ -102,6 +102,38 @@ def cpu_baseline(seconds):
                       f"at 1 fs, same table, 1 thread, {dt:.1f} s"}
 
 
+def feni_run(steps=1000, warmup=20):
+    """Binary Fe-Ni eam/fs alloy: bcc 64^3 x 2 = 1,048,576 atoms, Ni on a random
+    20% of sites, Nr = Nrho = 5000, r_c = 5.3 A, NVT at 600 K and 1 fs with
+    on-device rebuilds (the multi-element passes: global spline tables)."""
+    import mapp4py_amd as M
+    from mapp4py_amd import potentials as pot
+
+    p = pot.johnson_feni_fs(nr=5000, nrho=5000)
+    x, H = pot.bcc(2.86, (64, 64, 64))
+    x = x + np.random.default_rng(31).normal(0.0, 0.05, x.shape)
+    elem = (np.random.default_rng(32).random(len(x)) < 0.2).astype(np.uint8)
+    masses = [pot.MASS["Fe"], pot.MASS["Ni"]]
+    md = M.AtomsMD(M.ForceFieldEAM.from_knots(p), H, x, elem, skin=0.5)
+    md.x_d = M.create_temp(elem, masses, 600.0, 33)
+    nvt = M.NVT(600.0, 1.0 * pot.FS)
+    nvt.run(md, warmup, masses)
+    md.profile(True)
+    t0 = time.perf_counter()
+    out = nvt.run(md, steps, masses)
+    wall = time.perf_counter() - t0
+    kern = {k: md.profile_read(k) for k in ("eam_rho", "eam_force", "build_list", "integrate")}
+    md.profile(False)
+    n = len(x)
+    maxn, nfull, _ = md.list_stats()
+    return {"workload": "EAM/FS Fe-Ni NVT, bcc 64^3 x 2 = 1,048,576 atoms (20% Ni), Nr=Nrho=5000, r_c=5.3 A, "
+                        "skin 0.5 A, dt 1 fs, 600 K",
+            "metric": "EAM atom-steps/s", "value": n * steps / wall, "unit": "atom-steps/s", "steps": steps,
+            "ms_per_step": wall / steps * 1e3, "rebuilds": nvt.rebuilds, "T_K": nvt.T_part,
+            "pe_per_atom_eV": out[0] / n, "list_per_atom": nfull / n,
+            "kernel_ms": {k: ms / max(c, 1) for k, (ms, c) in kern.items()}, "dtype": "f32+f64"}
+
+
 DMD_BYTES_PER_SITE = 132  # x 16, alpha 8, c 16 (read twice), f 12, f_alpha 8, mu 16, E/dE write + read 32
 
 
@@ -257,6 +289,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dmd", action="store_true", help="skip the DMD measurement")
     ap.add_argument("--no-gcmc", action="store_true", help="skip the batched GCMC measurement")
+    ap.add_argument("--no-feni", action="store_true", help="skip the Fe-Ni (two-element) MD measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel (256 MiB, 20 launches) used to calibrate HBM counters")
     args = ap.parse_args()
@@ -288,7 +321,12 @@ def main():
         log(f"[bench] cpu baseline (~{args.cpu_seconds:.0f} s of oracle work)")
         cpu = cpu_baseline(args.cpu_seconds)
         log(f"[bench] cpu baseline {cpu['value']:.3e} atom-steps/s")
-    dmd = gcmc = None
+    dmd = gcmc = feni = None
+    if not args.no_feni:
+        log("[bench] Fe-Ni")
+        feni = feni_run()
+        log(f"[bench] Fe-Ni {feni['value']:.3e} atom-steps/s ({feni['ms_per_step']:.3f} ms/step, "
+            f"{feni['rebuilds']} rebuilds, kernels {feni['kernel_ms']})")
     if not args.no_dmd:
         log("[bench] DMD")
         dmd = dmd_run()
@@ -309,7 +347,7 @@ def main():
         "config": {"workload": f"EAM NVT, fcc Cu {args.cells}^3 cells ({n} atoms), Johnson setfl Nr=Nrho=10000, "
                                f"r_c=4.95 A, skin 0.5 A, dt 1 fs, 300 K", "natoms": n, "rebuilds": nvt.rebuilds,
                    "pe_per_atom_eV": out[0] / n, "T_K": nvt.T_part},
-        "roofline": roof, "cpu_baseline": cpu, "dmd": dmd, "gcmc": gcmc,
+        "roofline": roof, "cpu_baseline": cpu, "feni": feni, "dmd": dmd, "gcmc": gcmc,
     }
     print(json.dumps(line), flush=True)
 

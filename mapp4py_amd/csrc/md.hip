@@ -82,6 +82,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_start(int n, int ncells, const 
 
 // Full neighbor list with the reference's FP64 predicate: x_i = s_i H and the
 // image of j at (s_j + n) H, rsq = dx0^2 + (dx1^2 + dx2^2) < (r_c + skin)^2.
+// With a 3x3x3 stencil of distinct cells (the usual case) the cells are
+// visited in increasing Morton rank, so every row lists j in increasing slot
+// order: the k-th entries of a wave's 64 spatially adjacent atoms then lie
+// close together, and the passes' neighbor gathers touch few cache lines.
 __global__ __launch_bounds__(kBlock) void k_build_list(int n, const double4* __restrict__ s4, BoxDev bd, GridArgs g,
                                                        const int* __restrict__ cell_rank,
                                                        const int* __restrict__ cell_start, int maxn, int ldl,
@@ -96,29 +100,55 @@ __global__ __launch_bounds__(kBlock) void k_build_list(int n, const double4* __r
     s_to_x(bd.h, si.x, si.y, si.z, xi0, xi1, xi2);
     const int c0 = cell_coord(si.x, g.nc[0]), c1 = cell_coord(si.y, g.nc[1]), c2 = cell_coord(si.z, g.nc[2]);
     int cnt = 0;
-    for (int o2 = -g.R[2]; o2 <= g.R[2]; ++o2) {
+    // candidates of the stencil cell at offset (o0, o1, o2)
+    auto scan = [&](int o0, int o1, int o2) {
         const int u2 = c2 + o2, n2 = floordiv(u2, g.nc[2]), w2 = u2 - n2 * g.nc[2];
-        for (int o1 = -g.R[1]; o1 <= g.R[1]; ++o1) {
-            const int u1 = c1 + o1, n1 = floordiv(u1, g.nc[1]), w1 = u1 - n1 * g.nc[1];
-            for (int o0 = -g.R[0]; o0 <= g.R[0]; ++o0) {
-                const int u0 = c0 + o0, n0 = floordiv(u0, g.nc[0]), w0 = u0 - n0 * g.nc[0];
-                const int rank = cell_rank[w0 + g.nc[0] * (w1 + g.nc[1] * w2)];
-                const int jb = cell_start[rank], je = cell_start[rank + 1];
-                const bool self_img = (n0 | n1 | n2) == 0;
-                const uint32_t code = (uint32_t)((n0 + 2) + 5 * (n1 + 2) + 25 * (n2 + 2)) << kIndexBits;
-                for (int j = jb; j < je; ++j) {
-                    if (self_img && j == i) continue;
-                    const double4 sj = s4[j];
-                    double xj0, xj1, xj2;
-                    s_to_x(bd.h, shift_unit(sj.x, n0), shift_unit(sj.y, n1), shift_unit(sj.z, n2), xj0, xj1, xj2);
-                    const double rsq = rsq_exact(xi0, xi1, xi2, xj0, xj1, xj2);
-                    if (rsq < g.cut_sk_sq[ti * g.ne + (int)sj.w]) {
-                        if (cnt < maxn) list[(size_t)cnt * ldl + i] = (uint32_t)j | code;
-                        ++cnt;
-                    }
-                }
+        const int u1 = c1 + o1, n1 = floordiv(u1, g.nc[1]), w1 = u1 - n1 * g.nc[1];
+        const int u0 = c0 + o0, n0 = floordiv(u0, g.nc[0]), w0 = u0 - n0 * g.nc[0];
+        const int rank = cell_rank[w0 + g.nc[0] * (w1 + g.nc[1] * w2)];
+        const int jb = cell_start[rank], je = cell_start[rank + 1];
+        const bool self_img = (n0 | n1 | n2) == 0;
+        const uint32_t code = (uint32_t)((n0 + 2) + 5 * (n1 + 2) + 25 * (n2 + 2)) << kIndexBits;
+        for (int j = jb; j < je; ++j) {
+            if (self_img && j == i) continue;
+            const double4 sj = s4[j];
+            double xj0, xj1, xj2;
+            s_to_x(bd.h, shift_unit(sj.x, n0), shift_unit(sj.y, n1), shift_unit(sj.z, n2), xj0, xj1, xj2);
+            const double rsq = rsq_exact(xi0, xi1, xi2, xj0, xj1, xj2);
+            if (rsq < g.cut_sk_sq[ti * g.ne + (int)sj.w]) {
+                if (cnt < maxn) list[(size_t)cnt * ldl + i] = (uint32_t)j | code;
+                ++cnt;
             }
         }
+    };
+    const bool ordered = g.R[0] == 1 && g.R[1] == 1 && g.R[2] == 1 && g.nc[0] >= 3 && g.nc[1] >= 3 &&
+                         g.nc[2] >= 3 && (long long)g.nc[0] * g.nc[1] * g.nc[2] < (1ll << 27);
+    if (ordered) {
+        // key = rank * 32 + stencil index; 27 rounds of "smallest key above the last"
+        uint32_t key[27];
+#pragma unroll
+        for (int t = 0; t < 27; ++t) {
+            const int w0 = (c0 + t % 3 - 1 + g.nc[0]) % g.nc[0];
+            const int w1 = (c1 + t / 3 % 3 - 1 + g.nc[1]) % g.nc[1];
+            const int w2 = (c2 + t / 9 - 1 + g.nc[2]) % g.nc[2];
+            key[t] = ((uint32_t)cell_rank[w0 + g.nc[0] * (w1 + g.nc[1] * w2)] << 5) | (uint32_t)t;
+        }
+        uint32_t last = 0u;
+        for (int round = 0; round < 27; ++round) {
+            uint32_t best = 0xffffffffu;
+#pragma unroll
+            for (int t = 0; t < 27; ++t) {
+                const uint32_t k = key[t];
+                best = ((round == 0 || k > last) && k < best) ? k : best;
+            }
+            last = best;
+            const int t = (int)(best & 31u);
+            scan(t % 3 - 1, t / 3 % 3 - 1, t / 9 - 1);
+        }
+    } else {
+        for (int o2 = -g.R[2]; o2 <= g.R[2]; ++o2)
+            for (int o1 = -g.R[1]; o1 <= g.R[1]; ++o1)
+                for (int o0 = -g.R[0]; o0 <= g.R[0]; ++o0) scan(o0, o1, o2);
     }
     nn[i] = cnt;
     if (cnt > maxn) atomicMax(&flags[0], cnt);
@@ -335,6 +365,11 @@ __device__ inline void force_atom(int i, const uint4& ri, int ti, float dFi, int
                                   const Rec16Frame& R, const Tab& T, float& fx, float& fy, float& fz, double& ep,
                                   double (&w)[6])
 {
+    // the list entries of the next batch are loaded one batch ahead (this
+    // pass's batches are short, so one memory round trip less per batch pays)
+    uint32_t en[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) en[b] = list_entry(list, ldl, maxn, b, i, cnt);
     for (int k0 = 0; k0 < cnt; k0 += B) {
         uint32_t e[B];
         uint4 rj[B];
@@ -343,7 +378,7 @@ __device__ inline void force_atom(int i, const uint4& ri, int ti, float dFi, int
         int row[B];
         bool in[B];
 #pragma unroll
-        for (int b = 0; b < B; ++b) e[b] = list_entry(list, ldl, maxn, k0 + b, i, cnt);
+        for (int b = 0; b < B; ++b) e[b] = en[b];
 #pragma unroll
         for (int b = 0; b < B; ++b) {
             const uint32_t j = e[b] & kIndexMask;
@@ -354,6 +389,8 @@ __device__ inline void force_atom(int i, const uint4& ri, int ti, float dFi, int
                 dFj[b] = dF[j];
             }
         }
+#pragma unroll
+        for (int b = 0; b < B; ++b) en[b] = list_entry(list, ldl, maxn, k0 + B + b, i, cnt);
         const bool any_img = wave_has_image(e);
 #pragma unroll
         for (int b = 0; b < B; ++b) {
