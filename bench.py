@@ -75,6 +75,7 @@ def gpu_run(args):
     md.profile(False)
     wall = t1 - t0
     maxn, nfull, _ = md.list_stats()
+    log(f"[bench] LDS row windows: density from interval {md.lds_window()[0]}, force from {md.lds_window()[1]}")
     log(f"[bench] timed {args.steps} steps in {wall:.3f}s, rebuilds {nvt.rebuilds}, T {nvt.T_part:.2f} K, "
         f"PE/atom {out[0] / n:.6f} eV, list {nfull / n:.1f}/atom (max {maxn})")
     for k, (ms, c) in kern.items():
@@ -149,14 +150,21 @@ def dmd_run(evals=10):
     x = x + np.random.default_rng(5).normal(0, 0.05, x.shape)
     d = M.AtomsDMD(M.ForceFieldEAM.from_knots(p, dmd=True), H, x, elem, alpha, c, nquad=3, masses=p.masses)
     d.derivative()
+    # device-resident evaluations: f, f_alpha and mu stay in HBM
     d.profile(True)
     t0 = time.perf_counter()
     for _ in range(evals):
-        d.derivative()
+        d.evaluate()
     dt = time.perf_counter() - t0
     k1, k2 = d.profile_read("dmd_pass1"), d.profile_read("dmd_pass2")
     d.profile(False)
+    # the same with f, f_alpha and mu copied to host arrays every time (PCIe included)
+    t0 = time.perf_counter()
+    for _ in range(evals):
+        d.derivative()
+    dt_host = time.perf_counter() - t0
     f0, fa0, _, fe0, _, _ = d.derivative()
+    half_pairs = int(d.half_pairs()[0].shape[0])
     n = len(x)
     ms = dt / evals * 1e3
     # the learned surrogate (config 4): trained on the GPU from this rule, then the same evaluation
@@ -168,7 +176,7 @@ def dmd_run(evals=10):
     f1, fa1, _, fe1, _, _ = d.derivative()
     t0 = time.perf_counter()
     for _ in range(evals):
-        d.derivative()
+        d.evaluate()
     dts = time.perf_counter() - t0
     nets, quad = d.surrogate_coverage()
     d.use_surrogate(None)
@@ -176,8 +184,13 @@ def dmd_run(evals=10):
     d16.derivative()
     t0 = time.perf_counter()
     for _ in range(3):
-        d16.derivative()
+        d16.evaluate()
     ms16 = (time.perf_counter() - t0) / 3 * 1e3
+    del d16
+    # config 3's solver: CG on the free energy from the same state (x and alpha)
+    t0 = time.perf_counter()
+    cg = d.min_cg(max_iter=400, e_tol=0.0, grad_tol=1e-6)
+    cg_s = time.perf_counter() - t0
     surr = {"value": n * evals / dts, "unit": "site-evals/s", "ms_per_eval": dts / evals * 1e3,
             "speed_vs_quadrature_N3": dt / dts, "quadrature_N16_ms_per_eval": ms16,
             "speed_vs_quadrature_N16": ms16 / (dts / evals * 1e3),
@@ -192,8 +205,14 @@ def dmd_run(evals=10):
                         "alpha log-uniform [0.05,0.30] A, c_H ~ Beta(2,5)",
             "metric": "DMD site-evals/s (free energy + gradient)", "value": n * evals / dt, "unit": "site-evals/s",
             "ms_per_eval": ms, "pass1_ms": k1[0] / max(k1[1], 1), "pass2_ms": k2[0] / max(k2[1], 1),
-            "dtype": "f64", "half_pairs": int(d.half_pairs()[0].shape[0]),
-            "hbm_fraction": DMD_BYTES_PER_SITE * n / (ms / 1e3) / HBM_PEAK, "surrogate": surr}
+            "host_copy_inclusive": {"value": n * evals / dt_host, "ms_per_eval": dt_host / evals * 1e3,
+                                    "note": "f, f_alpha, mu gathered to caller order and copied to host each time"},
+            "dtype": "f64", "half_pairs": half_pairs,
+            "hbm_fraction": DMD_BYTES_PER_SITE * n / (ms / 1e3) / HBM_PEAK,
+            "cg": {"solver": "CG on free energy over x and alpha (MinCGDMD), grad tol 1e-6, max 400 iterations",
+                   "iters": cg["iters"], "status": cg["status"], "fe0": cg["fe0"], "fe": cg["fe"],
+                   "grad_norm": cg["grad_norm"], "grad_max": cg["grad_max"], "seconds": cg_s},
+            "surrogate": surr}
 
 
 def dmd_cpu_baseline(seconds):

@@ -109,6 +109,10 @@ mapp_status mapp_md_set_batch(mapp_md* md, int rho_batch, int force_batch);
  * device has the LDS); 0 reads the global tables. *in_use (may be NULL)
  * reports the mode in effect. */
 mapp_status mapp_md_set_lds(mapp_md* md, int on, int* in_use);
+/* First spline interval of the LDS row windows of the density and force
+ * passes (rows from there to nr-2 are in LDS; -1 when the LDS passes are not
+ * available). No counterpart in the reference. */
+mapp_status mapp_md_lds_window(mapp_md* md, int* rho_first_row, int* force_first_row);
 /* AtomsMD::create_T (src/atoms_md.cpp:130-160): Maxwell velocities from the
  * Park-Miller / Marsaglia stream of src/random.cpp:7-68 seeded with seed. */
 mapp_status mapp_create_T(int natms, const uint8_t* elem, const double* masses, double kB, double T, int seed,
@@ -140,6 +144,10 @@ mapp_status mapp_dmd_derivative(mapp_dmd* d, int with_c, const double* ext_mu, d
                                 double* mu, double* out8);
 /* ForceFieldDMD::new_value<with_c> (src/ff_dmd.cpp:252-281) over
  * ForceFieldEAMDMD::__energy_calc (src/ff_eam_dmd.cpp:835-953) */
+/* Launch-shape knob (no counterpart in the reference): list entries per batch
+ * of the quadrature passes, 1 (one pair at a time), 2 or 4 (default). Any
+ * value gives the same results to rounding. */
+mapp_status mapp_dmd_set_batch(mapp_dmd* d, int batch);
 mapp_status mapp_dmd_value(mapp_dmd* d, int with_c, const double* ext_mu, double* fe);
 /* NeighborDMD::create_list half list (src/neighbor_dmd.cpp:102-194) as
  * (component i, component j, image) triples in caller indices */

@@ -159,6 +159,12 @@ class AtomsMD:
     def set_batch(self, rho, force):
         check(lib().mapp_md_set_batch(self._h, int(rho), int(force)))
 
+    def lds_window(self):
+        """First spline interval held in LDS by the density and force passes (-1: none)."""
+        a, b = C.c_int(), C.c_int()
+        check(lib().mapp_md_lds_window(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def set_lds(self, on):
         """Spline rows of a one-element table in LDS (True) or read from the
         global tables (False); returns the mode in effect."""
@@ -263,6 +269,18 @@ class AtomsDMD:
         check(lib().mapp_dmd_derivative(self._h, int(with_c), ptr(em) if em is not None else None, ptr(f), ptr(fa),
                                         ptr(mu), ptr(out)))
         return f, fa, mu, out[0], out[1:7].copy(), out[7]
+
+    def evaluate(self, with_c=False, ext_mu=None):
+        """derivative() leaving f, f_alpha and mu on the device; returns (fe, F_H, S)."""
+        out = np.zeros(8)
+        em = None if ext_mu is None else f64(ext_mu)
+        check(lib().mapp_dmd_derivative(self._h, int(with_c), ptr(em) if em is not None else None, None, None, None,
+                                        ptr(out)))
+        return out[0], out[1:7].copy(), out[7]
+
+    def set_batch(self, b):
+        """List entries per batch of the quadrature passes (1, 2 or 4)."""
+        check(lib().mapp_dmd_set_batch(self._h, int(b)))
 
     def value(self, with_c=False, ext_mu=None):
         e = C.c_double()

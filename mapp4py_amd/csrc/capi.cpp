@@ -264,6 +264,16 @@ mapp_status mapp_md_set_batch(mapp_md* md, int rho_batch, int force_batch)
     return guard([&] { NEED(md); md->sys->set_batch(rho_batch, force_batch); });
 }
 
+mapp_status mapp_md_lds_window(mapp_md* md, int* rho_first_row, int* force_first_row)
+{
+    return guard([&] {
+        NEED(md);
+        NEED(rho_first_row);
+        NEED(force_first_row);
+        md->sys->lds_window(rho_first_row, force_first_row);
+    });
+}
+
 mapp_status mapp_md_set_lds(mapp_md* md, int on, int* in_use)
 {
     return guard([&] {
@@ -318,6 +328,11 @@ mapp_status mapp_dmd_derivative(mapp_dmd* d, int with_c, const double* ext_mu, d
         }
         if (f || f_alpha || mu) d->sys->get_forces(f, f_alpha, mu);
     });
+}
+
+mapp_status mapp_dmd_set_batch(mapp_dmd* d, int batch)
+{
+    return guard([&] { NEED(d); d->sys->set_batch(batch); });
 }
 
 mapp_status mapp_dmd_value(mapp_dmd* d, int with_c, const double* ext_mu, double* fe)

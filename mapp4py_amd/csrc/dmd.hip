@@ -392,6 +392,246 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass2(int n, const double4* __re
     block_sum_store<6>(acc, part + (size_t)blk * kPartW);
 }
 
+// ---------------- quadrature-only passes, batched ----------------
+//
+// Without the surrogate every listed pair takes the quadrature, so the
+// passes resolve B list entries per batch with their loads staged across the
+// batch: all list entries, then the neighbor data, then for each Hermite
+// point the table rows of every pair of the batch. A wave keeps many
+// independent loads in flight instead of one dependent chain per pair (the
+// passes are bound by load latency, not FP64 arithmetic). Entries past the
+// end of a row, and pairs beyond the cut, read table row 0 and are selected
+// away, so a batch has no branches. The arithmetic per pair is that of
+// smear_quad, point by point in the same order.
+
+// list entry k of component ci (entries past the row end: ci itself, zero image)
+__device__ inline uint32_t dlist_entry(const uint32_t* __restrict__ list, int ldl, int maxn, int k, int ci, int cnt)
+{
+    const uint32_t raw = list[(size_t)(k < maxn ? k : maxn - 1) * ldl + ci];
+    return k < cnt ? raw : ((uint32_t)ci | ((uint32_t)kZeroImage << kIndexBits));
+}
+
+// spline interval and offset of |u| (src/ff_eam_dmd.cpp:209-217 clamps)
+__device__ inline int dknot(const DmdArgs& A, double u, double& p)
+{
+    double pm = fabs(u) * A.dr_inv;
+    int m = (int)pm;
+    m = m < A.nr - 2 ? m : A.nr - 2;
+    pm -= (double)m;
+    p = pm < 1.0 ? pm : 1.0;
+    return m;
+}
+
+// pass 1: rho of every component (smeared r*rho_ba) and the pair energy
+// (smeared r*phi); only table functions 0 and 2 are read
+template <int B>
+__global__ __launch_bounds__(kBlock) void k_dmd_pass1q(int n, const double4* __restrict__ x, const double* __restrict__ al,
+                                                       const double* __restrict__ cc, const int* __restrict__ el,
+                                                       const int* __restrict__ nn, const uint32_t* __restrict__ list,
+                                                       int ldl, int maxn, DmdArgs A, BoxDev bd, int energy_only,
+                                                       double* __restrict__ E, double* __restrict__ dE,
+                                                       double* __restrict__ part)
+{
+    const int blk = xcd_block();
+    const int i = blk * kBlock + threadIdx.x;
+    double acc[2] = {0.0, 0.0};  // free energy without the kT S term, S
+    if (i < n) {
+        const int C = A.C;
+        const double4 xi = x[i];
+        double cv = 1.0;
+        for (int ic = 0; ic < C; ++ic) {
+            const int ci_idx = i * C + ic;
+            const double ci = cc[ci_idx];
+            if (ci > 0.0) cv -= ci;
+            if (ci < 0.0) continue;
+            const double ai = al[ci_idx];
+            const int ei = el[ci_idx];
+            double rho = 0.0, ep = 0.0;
+            const int cnt = nn[ci_idx];
+            for (int k0 = 0; k0 < cnt; k0 += B) {
+                uint32_t e[B];
+                double4 xj[B];
+                double aj[B], cj[B];
+                int tt[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) e[b] = dlist_entry(list, ldl, maxn, k0 + b, ci_idx, cnt);
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const int jc = (int)(e[b] & kIndexMask);
+                    xj[b] = x[jc / C];
+                    aj[b] = al[jc];
+                    cj[b] = cc[jc];
+                    tt[b] = ei * A.ne + el[jc];
+                }
+                PairGeo pg[B];
+                bool live[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    pg[b] = pair_geo(xi, xj[b], e[b] >> kIndexBits, bd.h, ai, aj[b], A.xmax, A.cut[tt[b]]);
+                    live[b] = pg[b].live && k0 + b < cnt;
+                }
+                double H0[B], H2[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) H0[b] = H2[b] = 0.0;
+                for (int l = 0; l < A.N; ++l) {
+                    const double xl = A.xi[l], wl = A.wi[l];
+                    double4 c0[B], c2[B];
+                    double p[B], s[B];
+#pragma unroll
+                    for (int b = 0; b < B; ++b) {
+                        const double u = pg[b].r - xl * pg[b].a;
+                        const int m = dknot(A, u, p[b]);
+                        s[b] = u < 0.0 ? -wl : wl;
+                        const size_t row = live[b] ? ((size_t)tt[b] * A.nr + m) * 3 : 0;
+                        c0[b] = A.tab[row];
+                        c2[b] = A.tab[row + 2];
+                    }
+#pragma unroll
+                    for (int b = 0; b < B; ++b) {
+                        H0[b] += hv(c0[b], p[b]) * s[b];
+                        H2[b] += hv(c2[b], p[b]) * s[b];
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const double rinv = 1.0 / pg[b].r;
+                    const double rp0 = kPiInSq * rinv * H0[b], rp2 = kPiInSq * rinv * H2[b];
+                    rho += live[b] ? cj[b] * rp2 : 0.0;
+                    ep += live[b] ? 0.5 * ci * cj[b] * rp0 : 0.0;
+                }
+            }
+            double F, dF;
+            femb5(A, ei, rho, F, dF);
+            E[ci_idx] = F;
+            dE[ci_idx] = dF;
+            if (!energy_only || ci != 0.0) acc[0] += ci * (F + A.mu0[ei] - 3.0 * A.kBT * log(ai));
+            acc[0] += ep;
+            acc[1] += xlogx(ci);
+        }
+        acc[1] += xlogx(cv);
+    }
+    block_sum_store<2>(acc, part + (size_t)blk * kPartW);
+}
+
+// pass 2: forces on x and alpha, chemical-potential sums, virial
+template <int B>
+__global__ __launch_bounds__(kBlock) void k_dmd_pass2q(int n, const double4* __restrict__ x, const double* __restrict__ al,
+                                                       const double* __restrict__ cc, const int* __restrict__ el,
+                                                       const int* __restrict__ nn, const uint32_t* __restrict__ list,
+                                                       int ldl, int maxn, DmdArgs A, BoxDev bd,
+                                                       const double* __restrict__ E, const double* __restrict__ dE,
+                                                       double4* __restrict__ f, double* __restrict__ fa,
+                                                       double* __restrict__ mu, double* __restrict__ part)
+{
+    const int blk = xcd_block();
+    const int i = blk * kBlock + threadIdx.x;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        const int C = A.C;
+        const double4 xi = x[i];
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int ic = 0; ic < C; ++ic) {
+            const int ci_idx = i * C + ic;
+            const double ci = cc[ci_idx];
+            if (ci < 0.0) {
+                fa[ci_idx] = 0.0;
+                mu[ci_idx] = 0.0;
+                continue;
+            }
+            const double ai = al[ci_idx];
+            const int ei = el[ci_idx];
+            const double dEi = dE[ci_idx];
+            double fai = 0.0, mui = 0.0;
+            const int cnt = nn[ci_idx];
+            for (int k0 = 0; k0 < cnt; k0 += B) {
+                uint32_t e[B];
+                double4 xj[B];
+                double aj[B], cj[B], dEj[B];
+                int tt[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) e[b] = dlist_entry(list, ldl, maxn, k0 + b, ci_idx, cnt);
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const int jc = (int)(e[b] & kIndexMask);
+                    xj[b] = x[jc / C];
+                    aj[b] = al[jc];
+                    cj[b] = cc[jc];
+                    dEj[b] = dE[jc];
+                    tt[b] = ei * A.ne + el[jc];
+                }
+                PairGeo pg[B];
+                bool live[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    pg[b] = pair_geo(xi, xj[b], e[b] >> kIndexBits, bd.h, ai, aj[b], A.xmax, A.cut[tt[b]]);
+                    live[b] = pg[b].live && k0 + b < cnt;
+                }
+                double H[B][3][3];
+#pragma unroll
+                for (int b = 0; b < B; ++b)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) H[b][k][0] = H[b][k][1] = H[b][k][2] = 0.0;
+                for (int l = 0; l < A.N; ++l) {
+                    const double xl = A.xi[l], wl = A.wi[l];
+                    double4 c[B][3];
+                    double p[B], s[B];
+#pragma unroll
+                    for (int b = 0; b < B; ++b) {
+                        const double u = pg[b].r - xl * pg[b].a;
+                        const int m = dknot(A, u, p[b]);
+                        s[b] = u < 0.0 ? -wl : wl;
+                        const size_t row = live[b] ? ((size_t)tt[b] * A.nr + m) * 3 : 0;
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) c[b][k] = A.tab[row + k];
+                    }
+#pragma unroll
+                    for (int b = 0; b < B; ++b)
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            const double t = hv(c[b][k], p[b]) * s[b];
+                            H[b][k][0] += t;
+                            H[b][k][1] += t * xl;
+                            H[b][k][2] += t * (xl * xl);
+                        }
+                }
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const double r = pg[b].r, a = pg[b].a;
+                    const double rinv = 1.0 / r, ainv = 1.0 / a;
+                    double rp[3], drr[3], dal[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        rp[k] = kPiInSq * rinv * H[b][k][0];
+                        drr[k] = -(rinv * rinv) * (rp[k] + 2.0 * kPiInSq * ainv * H[b][k][1]);
+                        dal[k] = -(ainv * ainv) * (rp[k] - 2.0 * kPiInSq * rinv * H[b][k][2]);
+                    }
+                    const double fpair = -(drr[2] * dEi + drr[1] * dEj[b] + drr[0]) * ci * cj[b];
+                    const double apair = -(dal[2] * dEi + dal[1] * dEj[b] + dal[0]) * ci * cj[b];
+                    const double du = cj[b] * (rp[0] + rp[1] * dEj[b]);
+                    const double fp = live[b] ? fpair : 0.0;
+                    mui += live[b] ? du : 0.0;
+                    fai += live[b] ? ai * apair : 0.0;
+                    const double dx = pg[b].dx, dy = pg[b].dy, dz = pg[b].dz;
+                    fx += fp * dx;
+                    fy += fp * dy;
+                    fz += fp * dz;
+                    const double hf = 0.5 * fp;
+                    acc[0] += hf * dx * dx;
+                    acc[1] += hf * dx * dy;
+                    acc[2] += hf * dx * dz;
+                    acc[3] += hf * dy * dy;
+                    acc[4] += hf * dy * dz;
+                    acc[5] += hf * dz * dz;
+                }
+            }
+            fa[ci_idx] = fai + 3.0 * A.kBT * ci / ai;
+            mu[ci_idx] = E[ci_idx] + mui;
+        }
+        f[i] = make_double4(fx, fy, fz, 0.0);
+    }
+    block_sum_store<6>(acc, part + (size_t)blk * kPartW);
+}
+
 // chemical potential of new_derivative<true> (src/ff_dmd.cpp:305-366) and
 // the sum of c * mu_ext
 __global__ __launch_bounds__(kBlock) void k_dmd_chem(int n, int C, const double* __restrict__ al,
@@ -990,9 +1230,17 @@ void DMDSystem::pass1(bool energy_only)
         hipLaunchKernelGGL(k_dmd_pass1<true>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
                            el_.get(), nn_.get(), list_.get(), ldl_, A_, bd, S, energy_only ? 1 : 0, E_.get(),
                            dE_.get(), part_.get());
-    else
+    else if (batch_ == 1)
         hipLaunchKernelGGL(k_dmd_pass1<false>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
                            el_.get(), nn_.get(), list_.get(), ldl_, A_, bd, S, energy_only ? 1 : 0, E_.get(),
+                           dE_.get(), part_.get());
+    else if (batch_ == 2)
+        hipLaunchKernelGGL(k_dmd_pass1q<2>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
+                           el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, energy_only ? 1 : 0, E_.get(),
+                           dE_.get(), part_.get());
+    else
+        hipLaunchKernelGGL(k_dmd_pass1q<4>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
+                           el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, energy_only ? 1 : 0, E_.get(),
                            dE_.get(), part_.get());
     prof_.end(stream_);
     reduce_rows(part_.get(), nb, 2, red_.get());
@@ -1012,12 +1260,26 @@ void DMDSystem::pass2()
         hipLaunchKernelGGL(k_dmd_pass2<true>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
                            el_.get(), nn_.get(), list_.get(), ldl_, A_, bd, S, E_.get(), dE_.get(), f_.get(),
                            fa_.get(), mu_.get(), p2);
-    else
+    else if (batch_ == 1)
         hipLaunchKernelGGL(k_dmd_pass2<false>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
                            el_.get(), nn_.get(), list_.get(), ldl_, A_, bd, S, E_.get(), dE_.get(), f_.get(),
                            fa_.get(), mu_.get(), p2);
+    else if (batch_ == 2)
+        hipLaunchKernelGGL(k_dmd_pass2q<2>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
+                           el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, E_.get(), dE_.get(), f_.get(),
+                           fa_.get(), mu_.get(), p2);
+    else
+        hipLaunchKernelGGL(k_dmd_pass2q<4>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
+                           el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, E_.get(), dE_.get(), f_.get(),
+                           fa_.get(), mu_.get(), p2);
     prof_.end(stream_);
     reduce_rows(p2, nb, 6, red_.get() + 8);
+}
+
+void DMDSystem::set_batch(int b)
+{
+    MAPP_REQUIRE(b == 1 || b == 2 || b == 4, ERR_ARG, "DMD batch must be 1, 2 or 4");
+    batch_ = b;
 }
 
 DMDReport DMDSystem::derivative(bool with_c, const double* ext_mu)

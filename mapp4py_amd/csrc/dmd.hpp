@@ -43,6 +43,8 @@ public:
     DMDSystem(std::shared_ptr<EamForceField> ff, int nsites, int c_dim, int nquad, const double* H, const double* x,
               const uint8_t* elem, const double* alpha, const double* c, double skin, double kB, double T,
               double hP, const double* masses);
+    // list entries per batch of the quadrature passes (1 = one pair at a time)
+    void set_batch(int b);
     ~DMDSystem();
 
     int nsites() const { return n_; }
@@ -103,6 +105,7 @@ private:
     DevBuf<int> nn_;
     DevBuf<uint32_t> list_;
     int maxn_ = 0, ldl_ = 0;
+    int batch_ = 4;                  // quadrature pass batch: 1 (per-pair loop), 2 or 4 list entries
     DevBuf<int> flags_;
     DevBuf<double> part_, red_, red2_;
     DevBuf<double4> tab_;

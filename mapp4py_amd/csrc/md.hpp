@@ -107,6 +107,12 @@ public:
     // the global tables; returns whether the LDS passes are in use
     bool set_lds(bool on);
     bool lds() const { return use_lds_; }
+    // first spline interval held in LDS by the density and force passes (-1: no LDS passes)
+    void lds_window(int* rho_first, int* force_first) const
+    {
+        *rho_first = lds_ok_ ? rho_m_lo_ : -1;
+        *force_first = lds_ok_ ? f_m_lo_ : -1;
+    }
     int batch_rho() const { return batch_rho_; }
     int batch_force() const { return batch_force_; }
     void sync();

@@ -65,13 +65,17 @@ def test_dmd_lists_and_derivative_match_oracle(M, cuh, cells, tilt, seed):
     x, H, elem, alpha, c = sites(cells, seed, tilt)
     g, o = both(M, cuh, x, H, elem, alpha, c)
     assert triples(*g.half_pairs()) == triples(*o.pairs())
-    f, fa, mu, fe, FH, S = g.derivative()
     fo, fao, muo, feo, FHo, So = o.force()
-    errs = [close(f, fo, "f_x"), close(fa, fao, "f_alpha"), close(mu, muo, "mu"), close([fe], [feo], "fe"),
-            close(FH, FHo, "F_H"), close([S], [So], "S")]
-    say(f"DMD {cells} tilt {tilt}: rel errors f {errs[0]:.1e} f_a {errs[1]:.1e} mu {errs[2]:.1e} "
-        f"fe {errs[3]:.1e} F_H {errs[4]:.1e} S {errs[5]:.1e}")
-    close([g.value()], [o.energy()], "value")
+    for batch in (1, 2, 4):  # per-pair loop, batched passes
+        g.set_batch(batch)
+        f, fa, mu, fe, FH, S = g.derivative()
+        errs = [close(f, fo, "f_x"), close(fa, fao, "f_alpha"), close(mu, muo, "mu"), close([fe], [feo], "fe"),
+                close(FH, FHo, "F_H"), close([S], [So], "S")]
+        say(f"DMD {cells} tilt {tilt} batch {batch}: rel errors f {errs[0]:.1e} f_a {errs[1]:.1e} "
+            f"mu {errs[2]:.1e} fe {errs[3]:.1e} F_H {errs[4]:.1e} S {errs[5]:.1e}")
+        close([g.value()], [o.energy()], "value")
+        fe_r, FH_r, S_r = g.evaluate()
+        assert fe_r == fe and np.array_equal(FH_r, FH) and S_r == S
 
 
 def test_chemical_potential_transform(M, cuh):

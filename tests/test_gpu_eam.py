@@ -240,6 +240,9 @@ def test_spline_rows_below_the_lds_window(M, cu_pot):
     x = jitter(x, 0.05, 22)
     elem = np.zeros(len(x), np.uint8)
     g, o = both(M, cu_pot, H, x, elem)
+    rho_lo, f_lo = g.lds_window()
+    say(f"LDS windows: density rows from {rho_lo}, force rows from {f_lo} (r > {f_lo * cu_pot.dr:.3f} A)")
+    assert f_lo * cu_pot.dr > 2.9 / np.sqrt(2.0)  # the first shell (2.05 A) starts below the window
     fo, eo, FHo, rho_o = o.force()
     out = {}
     for lds in (True, False):
