@@ -697,15 +697,18 @@ __global__ __launch_bounds__(kBlock) void k_dot2(int n, int nc, const double4* _
 // max |h_x|, max |h_alpha| and min over h_alpha<0 of -alpha/h_alpha
 __global__ __launch_bounds__(kBlock) void k_cg_limits(int n, int nc, const double4* __restrict__ hx,
                                                       const double* __restrict__ ha, const double* __restrict__ al,
-                                                      const double* __restrict__ cc, double* __restrict__ out)
+                                                      const double* __restrict__ cc, double* __restrict__ part)
 {
+    // grid-stride partial max |h_x|, max |h_alpha|, min -alpha/h_alpha; one
+    // row per workgroup (max and min are exact, so any order gives the same)
     __shared__ double sh[3][kBlock];
     double mx = 0.0, ma = 0.0, mn = 1e300;
-    for (int k = threadIdx.x; k < n; k += kBlock) {
+    const int stride = gridDim.x * kBlock;
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += stride) {
         const double4 h = hx[k];
         mx = fmax(mx, fmax(fabs(h.x), fmax(fabs(h.y), fabs(h.z))));
     }
-    for (int k = threadIdx.x; k < nc; k += kBlock) {
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < nc; k += stride) {
         if (cc[k] < 0.0) continue;
         ma = fmax(ma, fabs(ha[k]));
         if (ha[k] < 0.0) mn = fmin(mn, -al[k] / ha[k]);
@@ -722,10 +725,31 @@ __global__ __launch_bounds__(kBlock) void k_cg_limits(int n, int nc, const doubl
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        out[0] = sh[0][0];
-        out[1] = sh[1][0];
-        out[2] = sh[2][0];
+    if (threadIdx.x < 3) part[(size_t)blockIdx.x * kPartW + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// rows of partial extrema -> out: columns [0, nmax) by max, [nmax, nmax + nmin) by min
+__global__ __launch_bounds__(kBlock) void k_extrema_rows(int nrows, int nmax, int nmin, const double* __restrict__ part,
+                                                         double* __restrict__ out)
+{
+    __shared__ double sh[kBlock];
+    for (int c = 0; c < nmax + nmin; ++c) {
+        const bool is_max = c < nmax;
+        double v = is_max ? -1e300 : 1e300;
+        for (int r = threadIdx.x; r < nrows; r += kBlock) {
+            const double u = part[(size_t)r * kPartW + c];
+            v = is_max ? fmax(v, u) : fmin(v, u);
+        }
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s)
+                sh[threadIdx.x] = is_max ? fmax(sh[threadIdx.x], sh[threadIdx.x + s])
+                                         : fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[c] = sh[0];
+        __syncthreads();
     }
 }
 
@@ -756,23 +780,25 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(int n, int nc, double ratio, 
     if (i < nc) ha[i] = ratio * ha[i] + fa[i];
 }
 
+// grid-stride partial max |f_x|, |f_alpha|: one row per workgroup
 __global__ __launch_bounds__(kBlock) void k_gmax(int n, int nc, const double4* __restrict__ fx,
-                                                 const double* __restrict__ fa, double* __restrict__ out)
+                                                 const double* __restrict__ fa, double* __restrict__ part)
 {
     __shared__ double sh[kBlock];
     double m = 0.0;
-    for (int k = threadIdx.x; k < n; k += kBlock) {
+    const int stride = gridDim.x * kBlock;
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += stride) {
         const double4 f = fx[k];
         m = fmax(m, fmax(fabs(f.x), fmax(fabs(f.y), fabs(f.z))));
     }
-    for (int k = threadIdx.x; k < nc; k += kBlock) m = fmax(m, fabs(fa[k]));
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < nc; k += stride) m = fmax(m, fabs(fa[k]));
     sh[threadIdx.x] = m;
     __syncthreads();
     for (int s = kBlock / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = sh[0];
+    if (threadIdx.x == 0) part[(size_t)blockIdx.x * kPartW] = sh[0];
 }
 
 // host <-> slot order
@@ -1462,8 +1488,11 @@ CGReport DMDSystem::cg(int max_iter, double e_tol, double max_dx, double max_dal
         MAPP_HIP(hipStreamSynchronize(stream_));
         return v;
     };
+    // extrema over the grid: at most 256 workgroups, each row of the partial buffer once
+    const int gx = std::min(256, grid_for(std::max(n_, nc_)));
     auto gmax = [&]() {
-        hipLaunchKernelGGL(k_gmax, dim3(1), dim3(kBlock), 0, stream_, n_, nc_, f_.get(), fa_.get(), red_.get() + 28);
+        hipLaunchKernelGGL(k_gmax, dim3(gx), dim3(kBlock), 0, stream_, n_, nc_, f_.get(), fa_.get(), part);
+        hipLaunchKernelGGL(k_extrema_rows, dim3(1), dim3(kBlock), 0, stream_, gx, 1, 0, part, red_.get() + 28);
         double v;
         MAPP_HIP(hipMemcpyAsync(&v, red_.get() + 28, sizeof(double), hipMemcpyDeviceToHost, stream_));
         MAPP_HIP(hipStreamSynchronize(stream_));
@@ -1495,8 +1524,9 @@ CGReport DMDSystem::cg(int max_iter, double e_tol, double max_dx, double max_dal
             fh = f0f0;
         }
         // ls_prep: largest step within max_dx, max_dalpha and alpha > 0
-        hipLaunchKernelGGL(k_cg_limits, dim3(1), dim3(kBlock), 0, stream_, n_, nc_, cg_h_.get(), cg_ha_.get(),
-                           al_.get(), c_.get(), red_.get() + 32);
+        hipLaunchKernelGGL(k_cg_limits, dim3(gx), dim3(kBlock), 0, stream_, n_, nc_, cg_h_.get(), cg_ha_.get(),
+                           al_.get(), c_.get(), part);
+        hipLaunchKernelGGL(k_extrema_rows, dim3(1), dim3(kBlock), 0, stream_, gx, 2, 1, part, red_.get() + 32);
         double lim[3];
         MAPP_HIP(hipMemcpyAsync(lim, red_.get() + 32, sizeof(double) * 3, hipMemcpyDeviceToHost, stream_));
         MAPP_HIP(hipStreamSynchronize(stream_));
