@@ -23,6 +23,10 @@ struct GridArgs {
     int R[3];
     int ne;
     double cut_sk_sq[kMaxElems * kMaxElems];
+    // on the record grid: pairs with rsq below rec_in_sq are inside the list
+    // cut-off for certain, above rec_out_sq outside (the band between holds
+    // every pair the record quantisation could misplace)
+    double rec_in_sq[kMaxElems * kMaxElems], rec_out_sq[kMaxElems * kMaxElems];
 };
 // ---------------- neighbor build ----------------
 
@@ -46,9 +50,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(int n, const double4* __re
 // the wrapped s exactly as a rebuild does (src/cell.h:208)
 __global__ __launch_bounds__(kBlock) void k_permute(int n, const int* __restrict__ perm, const double4* __restrict__ s_tmp,
                                                     const double4* __restrict__ v_old, const int* __restrict__ id_old,
-                                                    BoxDev bd, double4* __restrict__ x_new, double4* __restrict__ v_new,
-                                                    int* __restrict__ id_new, double4* __restrict__ s4,
-                                                    double4* __restrict__ x0, int* __restrict__ slot_of)
+                                                    BoxDev bd, Rec16Frame R, double4* __restrict__ x_new,
+                                                    double4* __restrict__ v_new, int* __restrict__ id_new,
+                                                    double4* __restrict__ s4, double4* __restrict__ x0,
+                                                    int* __restrict__ slot_of, uint4* __restrict__ rec)
 {
     const int p = xcd_block() * kBlock + threadIdx.x;
     if (p >= n) return;
@@ -60,6 +65,9 @@ __global__ __launch_bounds__(kBlock) void k_permute(int n, const int* __restrict
     x_new[p] = xn;
     x0[p] = xn;
     s4[p] = s;
+    // the 16-byte record of the new slot (coordinates on the record grid, element)
+    rec[p] = make_uint4((unsigned)__double2ll_rn(xn.x * R.inv_q), (unsigned)__double2ll_rn(xn.y * R.inv_q),
+                        (unsigned)__double2ll_rn(xn.z * R.inv_q), (unsigned)(int)xn.w);
     v_new[p] = v_old[o];
     const int id = id_old[o];
     id_new[p] = id;
@@ -86,8 +94,13 @@ __global__ __launch_bounds__(kBlock) void k_cell_start(int n, int ncells, const 
 // visited in increasing Morton rank, so every row lists j in increasing slot
 // order: the k-th entries of a wave's 64 spatially adjacent atoms then lie
 // close together, and the passes' neighbor gathers touch few cache lines.
-__global__ __launch_bounds__(kBlock) void k_build_list(int n, const double4* __restrict__ s4, BoxDev bd, GridArgs g,
-                                                       const int* __restrict__ cell_rank,
+// Candidates are screened on the 16-byte records: one gather each, exact
+// integer separations. Only a pair within the quantisation band of the
+// cut-off (a few 1e-8 A wide) is decided by the FP64 predicate on s, so the
+// neighbor set is that of the predicate for every pair.
+__global__ __launch_bounds__(kBlock) void k_build_list(int n, const double4* __restrict__ s4,
+                                                       const uint4* __restrict__ rec, Rec16Frame R, BoxDev bd,
+                                                       GridArgs g, const int* __restrict__ cell_rank,
                                                        const int* __restrict__ cell_start, int maxn, int ldl,
                                                        uint32_t* __restrict__ list, int* __restrict__ nn,
                                                        int* __restrict__ flags)
@@ -95,6 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_build_list(int n, const double4* __r
     const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     const double4 si = s4[i];
+    const uint4 ri = rec[i];
     const int ti = (int)si.w;
     double xi0, xi1, xi2;
     s_to_x(bd.h, si.x, si.y, si.z, xi0, xi1, xi2);
@@ -108,14 +122,25 @@ __global__ __launch_bounds__(kBlock) void k_build_list(int n, const double4* __r
         const int rank = cell_rank[w0 + g.nc[0] * (w1 + g.nc[1] * w2)];
         const int jb = cell_start[rank], je = cell_start[rank + 1];
         const bool self_img = (n0 | n1 | n2) == 0;
-        const uint32_t code = (uint32_t)((n0 + 2) + 5 * (n1 + 2) + 25 * (n2 + 2)) << kIndexBits;
+        const uint32_t img = (uint32_t)((n0 + 2) + 5 * (n1 + 2) + 25 * (n2 + 2));
+        const uint32_t code = img << kIndexBits;
+        const uint4 sh = R.shift[img];
+        const uint4 rs = make_uint4(ri.x - sh.x, ri.y - sh.y, ri.z - sh.z, 0u);
         for (int j = jb; j < je; ++j) {
             if (self_img && j == i) continue;
-            const double4 sj = s4[j];
-            double xj0, xj1, xj2;
-            s_to_x(bd.h, shift_unit(sj.x, n0), shift_unit(sj.y, n1), shift_unit(sj.z, n2), xj0, xj1, xj2);
-            const double rsq = rsq_exact(xi0, xi1, xi2, xj0, xj1, xj2);
-            if (rsq < g.cut_sk_sq[ti * g.ne + (int)sj.w]) {
+            const uint4 rj = rec[j];
+            const double dx = (double)(int)(rs.x - rj.x), dy = (double)(int)(rs.y - rj.y),
+                         dz = (double)(int)(rs.z - rj.z);
+            const double rsq_u = dx * dx + dy * dy + dz * dz;
+            const int tt = ti * g.ne + (int)rj.w;
+            bool in = rsq_u < g.rec_in_sq[tt];
+            if (!in && rsq_u <= g.rec_out_sq[tt]) {
+                const double4 sj = s4[j];
+                double xj0, xj1, xj2;
+                s_to_x(bd.h, shift_unit(sj.x, n0), shift_unit(sj.y, n1), shift_unit(sj.z, n2), xj0, xj1, xj2);
+                in = rsq_exact(xi0, xi1, xi2, xj0, xj1, xj2) < g.cut_sk_sq[tt];
+            }
+            if (in) {
                 if (cnt < maxn) list[(size_t)cnt * ldl + i] = (uint32_t)j | code;
                 ++cnt;
             }
@@ -1167,7 +1192,8 @@ void MDSystem::rebuild()
     MAPP_HIP(hipcub::DeviceRadixSort::SortPairs(sort_tmp_.get(), bytes, keys_.get(), keys_alt_.get(), vals_.get(),
                                                 vals_alt_.get(), n_, 0, end_bit, stream_));
     hipLaunchKernelGGL(k_permute, dim3(nb), dim3(kBlock), 0, stream_, n_, vals_alt_.get(), s_tmp_.get(), v_.get(),
-                       id_.get(), bd, x_alt_.get(), v_alt_.get(), id_alt_.get(), s_.get(), x0_.get(), slot_of_.get());
+                       id_.get(), bd, r16_, x_alt_.get(), v_alt_.get(), id_alt_.get(), s_.get(), x0_.get(),
+                       slot_of_.get(), rec_.get());
     std::swap(x_.p, x_alt_.p);
     std::swap(v_.p, v_alt_.p);
     std::swap(id_.p, id_alt_.p);
@@ -1192,6 +1218,13 @@ void MDSystem::build_list()
         for (int b = 0; b < t.ne; ++b) {
             const double c = t.rc[(size_t)a * t.ne + b] + skin_;
             g.cut_sk_sq[a * t.ne + b] = c * c;
+            // Record separations differ from the exact ones by at most 4 q
+            // per coordinate (q/2 rounding of each position, q/2 per H row
+            // for up to two box vectors in each of three rows), so at most
+            // 4 sqrt(3) q < 7 q in length; the band takes 16 q either side.
+            const double cu = c * r16_.inv_q;
+            g.rec_in_sq[a * t.ne + b] = (cu - 16.0) * (cu - 16.0);
+            g.rec_out_sq[a * t.ne + b] = (cu + 16.0) * (cu + 16.0);
         }
     BoxDev bd;
     std::memcpy(bd.h, box_.h, sizeof bd.h);
@@ -1201,8 +1234,8 @@ void MDSystem::build_list()
         list_.reserve((size_t)maxn_ * ldl_);
         MAPP_HIP(hipMemsetAsync(flags_.get(), 0, sizeof(int), stream_));
         prof_.begin(K_BUILD_LIST, stream_);
-        hipLaunchKernelGGL(k_build_list, dim3(nb), dim3(kBlock), 0, stream_, n_, s_.get(), bd, g, cell_rank_.get(),
-                           cell_start_.get(), maxn_, ldl_, list_.get(), nn_.get(), flags_.get());
+        hipLaunchKernelGGL(k_build_list, dim3(nb), dim3(kBlock), 0, stream_, n_, s_.get(), rec_.get(), r16_, bd, g,
+                           cell_rank_.get(), cell_start_.get(), maxn_, ldl_, list_.get(), nn_.get(), flags_.get());
         prof_.end(stream_);
         int over = 0;
         MAPP_HIP(hipMemcpyAsync(&over, flags_.get(), sizeof(int), hipMemcpyDeviceToHost, stream_));
