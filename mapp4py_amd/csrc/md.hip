@@ -608,8 +608,10 @@ __global__ __launch_bounds__(kBlock) void k_eam_force(int n, const uint4* __rest
 // Persistent passes for one element with the spline rows in LDS: one
 // 1024-thread workgroup per CU loads the row window once and walks chunks of
 // kBigBlock atoms. The chunks form 8 contiguous ranges, one per XCD (where the
-// workgroup runs), and each workgroup walks a contiguous run of its XCD's
-// range, so consecutive chunks share their neighbor halo in that XCD's L2.
+// workgroup runs); the workgroups of an XCD take consecutive chunks side by
+// side and stride through the range together, so the chunks in flight on an
+// XCD form one compact region whose neighbor halos are shared in its L2 (a
+// contiguous run per workgroup measured 15-25% more HBM fetch).
 constexpr int kBigBlock = 1024;
 
 struct ChunkWalk {
@@ -619,12 +621,11 @@ __device__ inline ChunkWalk chunk_walk(int n)
 {
     const int nchunks = (n + kBigBlock - 1) / kBigBlock;
     const int per = (nchunks + kXcd - 1) / kXcd;
-    const int x = blockIdx.x % kXcd, g = gridDim.x / kXcd, k = (int)blockIdx.x / kXcd;
-    const int sub = (per + g - 1) / g;
+    const int x = blockIdx.x % kXcd;
     ChunkWalk w;
-    w.c = x * per + k * sub;
-    w.end = min(min(x * per + (k + 1) * sub, (x + 1) * per), nchunks);
-    w.step = 1;
+    w.c = x * per + (int)blockIdx.x / kXcd;
+    w.end = min((x + 1) * per, nchunks);
+    w.step = gridDim.x / kXcd;
     return w;
 }
 
