@@ -330,7 +330,7 @@ def main():
     avg_s = ms_f / max(c_f, 1) / 1e3
     achieved = PASS2_BYTES_PER_ATOM * n / avg_s if avg_s > 0 else 0.0
     traffic = load_traffic()
-    roof = {"kernel": "k_eam_force (EAM pass 2 + second half-kick)", "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+    roof = {"kernel": "k_eam_force_lds (EAM pass 2 + second half-kick, spline rows in LDS)", "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
             "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
             "avg_launch_ms": avg_s * 1e3, "launches": c_f,
             "algorithmic_bytes_per_launch": PASS2_BYTES_PER_ATOM * n,
