@@ -778,10 +778,12 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift(int n, double4* __restric
 }
 
 // remove the centre-of-mass drift of the step, then the skin check of
-// DynamicMD::decide (|x - x0|^2 > skin^2 / 4 anywhere -> rebuild)
+// DynamicMD::decide (|x - x0|^2 > skin^2 / 4 anywhere -> rebuild); with rec
+// set, also the 16-byte position record of the shifted position (the force
+// passes then skip their own record pass; a rebuild rewrites the records)
 __global__ __launch_bounds__(kBlock) void k_shift_decide(int n, double4* __restrict__ x, const double4* __restrict__ x0,
                                                          const double* __restrict__ com, double inv_n, double skin_sq4,
-                                                         int* __restrict__ flags)
+                                                         int* __restrict__ flags, Rec16Frame R, uint4* __restrict__ rec)
 {
     const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
@@ -790,6 +792,9 @@ __global__ __launch_bounds__(kBlock) void k_shift_decide(int n, double4* __restr
     xi.y -= com[1] * inv_n;
     xi.z -= com[2] * inv_n;
     x[i] = xi;
+    if (rec)
+        rec[i] = make_uint4((unsigned)__double2ll_rn(xi.x * R.inv_q), (unsigned)__double2ll_rn(xi.y * R.inv_q),
+                            (unsigned)__double2ll_rn(xi.z * R.inv_q), (unsigned)(int)xi.w);
     const double4 a = x0[i];
     if (rsq_exact(a.x, a.y, a.z, xi.x, xi.y, xi.z) > skin_sq4) atomicOr(&flags[1], 1);
 }
@@ -1249,12 +1254,13 @@ void MDSystem::build_list()
     throw Error(ERR_STATE, "neighbor list capacity did not converge");
 }
 
-void MDSystem::force_passes(const KickArgs* kick)
+void MDSystem::force_passes(const KickArgs* kick, bool records_fresh)
 {
     const int nb = grid_for(n_);
     const EamArgs P = ff_->args();
     prof_.begin(K_EAM_RHO, stream_);
-    hipLaunchKernelGGL(k_make_records, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), r16_, rec_.get());
+    if (!records_fresh)
+        hipLaunchKernelGGL(k_make_records, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), r16_, rec_.get());
     const bool one = P.ne == 1;
     const bool lds = one && use_lds_;
     KickArgs K{};
@@ -1362,7 +1368,7 @@ bool MDSystem::decide()
     // no centre-of-mass shift outside the integrator: com sums are zero
     MAPP_HIP(hipMemsetAsync(red_.get() + 16, 0, sizeof(double) * 3, stream_));
     hipLaunchKernelGGL(k_shift_decide, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), x0_.get(), red_.get() + 16,
-                       1.0 / n_, 0.25 * skin_ * skin_, flags_.get());
+                       1.0 / n_, 0.25 * skin_ * skin_, flags_.get(), r16_, (uint4*)nullptr);
     int flag = 0;
     MAPP_HIP(hipMemcpyAsync(&flag, flags_.get() + 1, sizeof(int), hipMemcpyDeviceToHost, stream_));
     MAPP_HIP(hipStreamSynchronize(stream_));
@@ -1553,12 +1559,12 @@ int MDSystem::nvt_run(const double* masses, double kB, double T, double dt, int 
         reduce_rows(part_.get() + (size_t)2 * nb * kPartW, nb, 3, com);
         MAPP_HIP(hipMemsetAsync(flags_.get() + 1, 0, sizeof(int), stream_));
         hipLaunchKernelGGL(k_shift_decide, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), x0_.get(), com,
-                           1.0 / n_, skin_sq4, flags_.get());
+                           1.0 / n_, skin_sq4, flags_.get(), r16_, rec_.get());
         prof_.end(stream_);
         MAPP_HIP(hipMemcpyAsync(&flag, flags_.get() + 1, sizeof(int), hipMemcpyDeviceToHost, stream_));
         MAPP_HIP(hipStreamSynchronize(stream_));
-        if (flag) rebuild();
-        force_passes(&kick);
+        if (flag) rebuild();  // the permute rewrites the records
+        force_passes(&kick, true);
         prof_.begin(K_INTEGRATE, stream_);
         reduce_rows(part_.get() + (size_t)2 * nb * kPartW, mvv_rows_, 6, mvv);
         hipLaunchKernelGGL(k_nhc, dim3(1), dim3(64), 0, stream_, nh, mvv, A, 2);

@@ -121,7 +121,8 @@ public:
 private:
     void setup_grid();
     void build_list();
-    void force_passes(const KickArgs* kick);
+    // records_fresh: rec_ already holds the current positions (written by the skin check or a rebuild)
+    void force_passes(const KickArgs* kick, bool records_fresh = false);
     MDReport reduce_report(int nparts);
     void reduce_rows(const double* part, int nrows, int ncols, double* out);
     bool decide();
