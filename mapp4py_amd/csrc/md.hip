@@ -234,7 +234,8 @@ __device__ inline void embed(const EamArgs& P, int t, double rho, double& F, dou
 // root, FP64 where a sum or the spline abscissa needs it).
 
 // list entry k of slot i; the load is unconditional (a row past maxn is never
-// read) and entries past the row end become the atom itself, zero image
+// read; the list always has maxn >= 16 rows) and entries past the row end
+// become the atom itself, zero image
 __device__ inline uint32_t list_entry(const uint32_t* __restrict__ list, int ldl, int maxn, int k, int i, int cnt)
 {
     const uint32_t raw = list[(size_t)(k < maxn ? k : maxn - 1) * ldl + i];

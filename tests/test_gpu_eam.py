@@ -257,3 +257,22 @@ def test_spline_rows_below_the_lds_window(M, cu_pot):
         np.testing.assert_allclose(g.rho(), rho_o, rtol=2e-7)
         out[lds] = f
     assert np.abs(out[True] - out[False]).max() < 1e-5
+
+
+@pytest.mark.parametrize("natoms", [1, 7])
+def test_isolated_atoms_have_empty_rows(M, cu_pot, natoms):
+    # atoms further apart than r_c + skin in a large periodic box: every row
+    # is empty, forces vanish and the energy is n F(0), in both pass modes
+    H = np.diag([40.0, 41.0, 42.0])
+    x = np.array([[2.0 + 9.5 * k, 3.0 + 7.0 * (k % 2), 5.0 + 8.0 * (k % 3)] for k in range(natoms)])
+    elem = np.zeros(natoms, np.uint8)
+    g, o = both(M, cu_pot, H, x, elem)
+    assert g.list_stats()[1] == 0
+    fo, eo, FHo, rho_o = o.force()
+    for lds in (True, False):
+        g.set_lds(lds)
+        f, e, FH = g.force_calc()
+        say(f"{natoms} isolated atoms, lds={lds}: E {e:.9f} vs {eo:.9f}, max|f| {np.abs(f).max():.1e}")
+        assert np.abs(f).max() == 0.0 and np.abs(FH).max() == 0.0
+        assert abs(e - eo) < 1e-9 * max(1.0, abs(eo))
+        np.testing.assert_array_equal(g.rho(), np.zeros(natoms))
