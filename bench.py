@@ -32,6 +32,9 @@ HBM_PEAK = 8.0e12  # B/s, MI355X
 # half-kick fused into it reads and writes the FP64 velocity (24 + 24 B).
 # Neighbor gathers are taken as cache-served and the list as overhead.
 PASS2_BYTES_PER_ATOM = 80
+# algorithmic HBM bytes of a whole NVT step per atom: pass 1 28 B, pass 2 32 B,
+# integrator 128 B (positions, velocities, forces, types, position records)
+STEP_BYTES_PER_ATOM = 188
 
 
 def log(*a):
@@ -334,7 +337,10 @@ def main():
             "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
             "avg_launch_ms": avg_s * 1e3, "launches": c_f,
             "algorithmic_bytes_per_launch": PASS2_BYTES_PER_ATOM * n,
-            "list_inclusive_bytes_per_launch": PASS2_BYTES_PER_ATOM * n + 4 * nfull}
+            "list_inclusive_bytes_per_launch": PASS2_BYTES_PER_ATOM * n + 4 * nfull,
+            "whole_step": {"algorithmic_bytes_per_atom_step": STEP_BYTES_PER_ATOM,
+                           "achieved_GBps": value * STEP_BYTES_PER_ATOM / 1e9,
+                           "frac": value * STEP_BYTES_PER_ATOM / HBM_PEAK}}
     cpu = None
     if not args.no_cpu:
         log(f"[bench] cpu baseline (~{args.cpu_seconds:.0f} s of oracle work)")
