@@ -148,6 +148,15 @@ mapp_status mapp_dmd_derivative(mapp_dmd* d, int with_c, const double* ext_mu, d
  * of the quadrature passes, 1 (one pair at a time), 2 or 4 (default). Any
  * value gives the same results to rounding. */
 mapp_status mapp_dmd_set_batch(mapp_dmd* d, int batch);
+/* ForceFieldDMD::c_d_calc (src/ff_dmd.cpp:449-456) with
+ * ForceFieldEAMDMD::__c_d_calc (src/ff_eam_dmd.cpp:1359-1473): the
+ * concentration rates dc/dt of every component at the current state, over the
+ * second list of NeighborDMD::create_2nd_list (src/neighbor_dmd.cpp:224-273:
+ * same element, r < r_crd[e]). r_crd and C (the exchange width, zeta) are the
+ * per-element parameters of ff_eam_setfl/fs (src/ff_eam_dmd.cpp:1734-1790).
+ * c_d [natms][c_dim] in caller order (may be NULL: kept on the device);
+ * c_d_norm = max |c_d| (may be NULL). */
+mapp_status mapp_dmd_c_d(mapp_dmd* d, const double* r_crd, const double* C, double* c_d, double* c_d_norm);
 mapp_status mapp_dmd_value(mapp_dmd* d, int with_c, const double* ext_mu, double* fe);
 /* NeighborDMD::create_list half list (src/neighbor_dmd.cpp:102-194) as
  * (component i, component j, image) triples in caller indices */

@@ -278,6 +278,15 @@ class AtomsDMD:
                                         ptr(out)))
         return out[0], out[1:7].copy(), out[7]
 
+    def c_d(self, r_crd, zeta, fetch=True):
+        """ForceFieldDMD::c_d_calc: (c_d [nsites][c_dim] or None, max |c_d|) for the
+        exchange radii r_crd and widths zeta (the reference's C) per element."""
+        out = np.zeros((self.nsites, self.c_dim)) if fetch else None
+        norm = C.c_double()
+        rc, zt = f64(r_crd), f64(zeta)
+        check(lib().mapp_dmd_c_d(self._h, ptr(rc), ptr(zt), ptr(out) if fetch else None, C.byref(norm)))
+        return out, norm.value
+
     def set_batch(self, b):
         """List entries per batch of the quadrature passes (1, 2 or 4)."""
         check(lib().mapp_dmd_set_batch(self._h, int(b)))

@@ -335,6 +335,17 @@ mapp_status mapp_dmd_set_batch(mapp_dmd* d, int batch)
     return guard([&] { NEED(d); d->sys->set_batch(batch); });
 }
 
+mapp_status mapp_dmd_c_d(mapp_dmd* d, const double* r_crd, const double* C, double* c_d, double* c_d_norm)
+{
+    return guard([&] {
+        NEED(d);
+        NEED(r_crd);
+        NEED(C);
+        const double v = d->sys->c_d(r_crd, C, c_d);
+        if (c_d_norm) *c_d_norm = v;
+    });
+}
+
 mapp_status mapp_dmd_value(mapp_dmd* d, int with_c, const double* ext_mu, double* fe)
 {
     return guard([&] { NEED(d); NEED(fe); *fe = d->sys->value(with_c != 0, ext_mu); });

@@ -82,6 +82,7 @@ enum KernelId : int {
     K_SURR_INFER,
     K_SURR_TRAIN,
     K_GCMC,
+    K_DMD_CD,
     K_COUNT
 };
 
@@ -89,7 +90,8 @@ inline const char* kernel_name(int k)
 {
     static const char* names[K_COUNT] = {"eam_rho",    "eam_force",  "build_list", "cell_sort",
                                          "integrate",  "eam_energy", "dmd_pass1",  "dmd_pass2",
-                                         "dmd_energy", "surr_infer", "surr_train", "gcmc"};
+                                         "dmd_energy", "surr_infer", "surr_train", "gcmc",
+                                         "dmd_cd"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "";
 }
 

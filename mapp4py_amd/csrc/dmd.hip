@@ -632,6 +632,96 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass2q(int n, const double4* __r
     block_sum_store<6>(acc, part + (size_t)blk * kPartW);
 }
 
+// ---------------- concentration rates ----------------
+
+// ForceFieldEAMDMD::calc_Q with theta (src/ff_eam_dmd.cpp:1611-1633): the
+// exchange barrier between two sites in units of kT, the inverse squared
+// width alpha_Q_sq and theta (the Jacobian's derivative term)
+__device__ inline void exchange_q(double gamma_i, double gamma_j, double mu_ji, double& Q, double& alpha_Q_sq,
+                                  double& theta)
+{
+    const double a = 5.0 * (gamma_i - gamma_j - 6.0 * mu_ji);
+    const double xi = -2.0 * (gamma_i + gamma_j);
+    const double dsq = sqrt((xi - a) * (xi - a) - 8.0 * a * gamma_i);
+    const double z0 = -4.0 * gamma_i / ((xi - a) - dsq);
+    const double dz0 = 30.0 * z0 * (1.0 - z0) / dsq;
+    const double dQ = z0 * z0 * z0 * (6.0 * z0 * z0 - 15.0 * z0 + 10.0);
+    Q = z0 * z0 * (1.0 - z0) * (1.0 - z0) * ((1.0 - z0) * gamma_i + z0 * gamma_j) + dQ * mu_ji;
+    alpha_Q_sq = 1.0 / ((1.0 - z0) * gamma_i + z0 * gamma_j);
+    theta = alpha_Q_sq * (gamma_i - gamma_j) * dz0 - dQ;
+}
+
+struct CdArgs {
+    double rsq_crd[kMaxElems], zeta[kMaxElems], c1[kMaxElems];
+    double beta;
+};
+
+// ForceFieldEAMDMD::__c_d_calc (src/ff_eam_dmd.cpp:1359-1473) over the second
+// list of NeighborDMD::create_2nd_list (src/neighbor_dmd.cpp:224-273): the
+// entries of the component's list with the same element and r^2 < r_crd^2.
+// One thread per component; the owner sums every exchange of its component
+// from its own side (the rate of a pair seen from the other side is minus
+// this one: the barrier there is Q - mu_ji). Partial max |c_d| per workgroup.
+__global__ __launch_bounds__(kBlock) void k_dmd_cd(int nc, const double4* __restrict__ x, const double* __restrict__ al,
+                                                   const double* __restrict__ cc, const int* __restrict__ el,
+                                                   const int* __restrict__ nn, const uint32_t* __restrict__ list,
+                                                   int ldl, int C, BoxDev bd, CdArgs P,
+                                                   const double* __restrict__ mu, double* __restrict__ cd,
+                                                   double* __restrict__ part)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[kBlock];
+    const int blk = xcd_block();
+    const int ic = blk * kBlock + threadIdx.x;
+    double dci = 0.0;
+    if (ic < nc) {
+        const double ci = cc[ic];
+        if (ci >= 0.0) {
+            const int site = ic / C, ei = el[ic];
+            const double4 xi = x[site];
+            const double alpha_i = al[ic], alpha_sq_i = alpha_i * alpha_i, mu_i = mu[ic];
+            double cv_i = 1.0;
+            for (int k = 0; k < C; ++k) {
+                const double v = cc[site * C + k];
+                if (v > 0.0) cv_i -= v;
+            }
+            const double zeta = P.zeta[ei], c1 = P.c1[ei], rsq_crd = P.rsq_crd[ei];
+            const int cnt = nn[ic];
+            for (int k = 0; k < cnt; ++k) {
+                const uint32_t e = list[(size_t)k * ldl + ic];
+                const int jc = (int)(e & kIndexMask);
+                if (el[jc] != ei) continue;
+                double dx, dy, dz;
+                dpair_dx(xi, x[jc / C], e >> kIndexBits, bd.h, dx, dy, dz);
+                const double rsq = dx * dx + (dy * dy + dz * dz);
+                if (rsq >= rsq_crd) continue;
+                const double alpha_j = al[jc], alpha_sq_j = alpha_j * alpha_j;
+                const double gamma_i = rsq / (zeta * alpha_sq_i), gamma_j = rsq / (zeta * alpha_sq_j);
+                const double mu_ji = P.beta * (mu[jc] - mu_i);
+                double Q, gamma_ij_inv, theta;
+                exchange_q(gamma_i, gamma_j, mu_ji, Q, gamma_ij_inv, theta);
+                const double coef0 = -c1 * rsq * gamma_ij_inv * exp(-Q + 0.5 * mu_ji);
+                double cv_j = 1.0;
+                const int sj = jc / C;
+                for (int q = 0; q < C; ++q) {
+                    const double v = cc[sj * C + q];
+                    if (v > 0.0) cv_j -= v;
+                }
+                dci += coef0 * (ci * cv_j * exp(-0.5 * mu_ji) / (alpha_sq_i * alpha_i) -
+                                cc[jc] * cv_i * exp(0.5 * mu_ji) / (alpha_sq_j * alpha_j));
+            }
+        }
+        cd[ic] = dci;
+    }
+    sh[threadIdx.x] = fabs(dci);
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)blk * kPartW] = sh[0];
+}
+
 // chemical potential of new_derivative<true> (src/ff_dmd.cpp:305-366) and
 // the sum of c * mu_ext
 __global__ __launch_bounds__(kBlock) void k_dmd_chem(int n, int C, const double* __restrict__ al,
@@ -835,6 +925,16 @@ __global__ __launch_bounds__(kBlock) void k_dgather(int n, int C, const int* __r
     }
 }
 
+// per-component array from slot order to caller order
+__global__ __launch_bounds__(kBlock) void k_dgather_comp(int n, int C, const int* __restrict__ slot_of,
+                                                         const double* __restrict__ a, double* __restrict__ ao)
+{
+    const int id = blockIdx.x * kBlock + threadIdx.x;
+    if (id >= n) return;
+    const int s = slot_of[id];
+    for (int k = 0; k < C; ++k) ao[(size_t)id * C + k] = a[s * C + k];
+}
+
 __global__ __launch_bounds__(kBlock) void k_smear_points(DmdArgs A, int tt, int npts, const double* __restrict__ r,
                                                          const double* __restrict__ a, double* __restrict__ out)
 {
@@ -981,6 +1081,7 @@ DMDSystem::DMDSystem(std::shared_ptr<EamForceField> ff, int nsites, int c_dim, i
         const double lambda = hP / std::sqrt(2.0 * M_PI * A_.kBT * masses[e]);
         A_.mu0[e] = A_.kBT * 1.5 * (2.0 * std::log(lambda) - 1.0 - std::log(M_PI));
     }
+    masses_.assign(masses, masses + t.ne);
     for (int k = 0; k < t.ne * t.ne; ++k) A_.cut[k] = t.rc[k];
     // FP64 device tables
     {
@@ -1306,6 +1407,47 @@ void DMDSystem::set_batch(int b)
 {
     MAPP_REQUIRE(b == 1 || b == 2 || b == 4, ERR_ARG, "DMD batch must be 1, 2 or 4");
     batch_ = b;
+}
+
+double DMDSystem::c_d(const double* r_crd, const double* zeta, double* c_d_host)
+{
+    const EamTables& t = ff_->tables();
+    CdArgs P{};
+    for (int e = 0; e < t.ne; ++e) {
+        MAPP_REQUIRE(r_crd[e] > 0.0 && zeta[e] > 0.0, ERR_ARG, "r_crd and C must be positive");
+        MAPP_REQUIRE(r_crd[e] <= t.rc[(size_t)e * t.ne + e], ERR_ARG, "r_crd must not exceed r_c of the element");
+        P.rsq_crd[e] = r_crd[e] * r_crd[e];
+        P.zeta[e] = zeta[e];
+        // c_1 of ForceFieldEAMDMD::set_temp (src/ff_eam_dmd.cpp:1062-1066)
+        P.c1[e] = std::sqrt(0.5 * A_.kBT / masses_[e]) / M_PI;
+    }
+    P.beta = 1.0 / A_.kBT;
+    // calc_mu: rho, E, dE and the raw chemical potentials of the passes
+    pass1(false);
+    pass2();
+    cd_.reserve(nc_);
+    BoxDev bd;
+    std::memcpy(bd.h, box_.h, sizeof bd.h);
+    std::memcpy(bd.b, box_.b, sizeof bd.b);
+    const int nb = grid_for(nc_);
+    prof_.begin(K_DMD_CD, stream_);
+    hipLaunchKernelGGL(k_dmd_cd, dim3(nb), dim3(kBlock), 0, stream_, nc_, x_.get(), al_.get(), c_.get(), el_.get(),
+                       nn_.get(), list_.get(), ldl_, C_, bd, P, mu_.get(), cd_.get(), part_.get());
+    hipLaunchKernelGGL(k_extrema_rows, dim3(1), dim3(kBlock), 0, stream_, nb, 1, 0, part_.get(), red_.get() + 40);
+    prof_.end(stream_);
+    double norm = 0.0;
+    MAPP_HIP(hipMemcpyAsync(&norm, red_.get() + 40, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    if (c_d_host) {
+        DevBuf<double> tc;
+        tc.reserve(nc_);
+        hipLaunchKernelGGL(k_dgather_comp, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n_, C_,
+                           slot_of_.get(), cd_.get(), tc.get());
+        MAPP_HIP(hipMemcpyAsync(c_d_host, tc.get(), sizeof(double) * nc_, hipMemcpyDeviceToHost, stream_));
+        MAPP_HIP(hipStreamSynchronize(stream_));
+    }
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    prof_.flush();
+    return norm;
 }
 
 DMDReport DMDSystem::derivative(bool with_c, const double* ext_mu)

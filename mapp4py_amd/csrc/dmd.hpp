@@ -57,6 +57,10 @@ public:
     DMDReport derivative(bool with_c, const double* ext_mu);
     double value(bool with_c, const double* ext_mu);
     void get_forces(double* f, double* f_alpha, double* mu);
+    // ForceFieldDMD::c_d_calc: concentration rates over the second list
+    // (same element, r < r_crd[e]) with exchange widths zeta[e]; c_d in caller
+    // order [site][c_dim] when c_d_host is not null; returns max |c_d|
+    double c_d(const double* r_crd, const double* zeta, double* c_d_host);
     int64_t half_pairs_count();
     void half_pairs(int32_t* i, int32_t* j, int8_t* img);
     // MinCGDMD over x and alpha (c fixed)
@@ -97,6 +101,8 @@ private:
     DevBuf<int> id_, id_alt_, slot_of_;
     DevBuf<double4> f_;
     DevBuf<double> fa_, mu_, E_, dE_;
+    DevBuf<double> cd_;              // concentration rates per component
+    std::vector<double> masses_;
     DevBuf<uint32_t> keys_, keys_alt_;
     DevBuf<int> vals_, vals_alt_;
     DevBuf<unsigned char> sort_tmp_;

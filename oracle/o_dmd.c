@@ -397,3 +397,85 @@ double o_dmd_energy(o_dmd* d)
     free(rho);
     return en;
 }
+
+/* ForceFieldEAMDMD::calc_Q with theta (src/ff_eam_dmd.cpp:1611-1633): the
+ * barrier of the exchange path between two sites, z0 its position, alpha_Q_sq
+ * the inverse width, theta the derivative term the Jacobian uses. */
+void o_dmd_calc_q(double gamma_i, double gamma_j, double mu_ji, double* Q, double* alpha_Q_sq, double* theta)
+{
+    double a = 5.0 * (gamma_i - gamma_j - 6.0 * mu_ji);
+    double xi = -2.0 * (gamma_i + gamma_j);
+    double delta_sqrt = sqrt((xi - a) * (xi - a) - 8.0 * a * gamma_i);
+    double z0 = -4.0 * gamma_i / ((xi - a) - delta_sqrt);
+    double dz0 = 30.0 * z0 * (1.0 - z0) / delta_sqrt;
+    double dQ = z0 * z0 * z0 * (6.0 * z0 * z0 - 15.0 * z0 + 10.0);
+    double q = z0 * z0 * (1.0 - z0) * (1.0 - z0) * ((1.0 - z0) * gamma_i + z0 * gamma_j);
+    q += dQ * mu_ji;
+    *Q = q;
+    *alpha_Q_sq = 1.0 / ((1.0 - z0) * gamma_i + z0 * gamma_j);
+    *theta = *alpha_Q_sq * (gamma_i - gamma_j) * dz0 - dQ;
+}
+
+/* ForceFieldDMD::c_d_calc (src/ff_dmd.cpp:449-456) with
+ * ForceFieldEAMDMD::__c_d_calc (src/ff_eam_dmd.cpp:1359-1473): calc_mu's
+ * chemical potentials (the same sums as o_dmd_force), the second list of
+ * NeighborDMD::create_2nd_list (src/neighbor_dmd.cpp:224-273: same element,
+ * RSQ < r_crd^2, from the half list), c_1 = sqrt(kT / 2m) / pi
+ * (src/ff_eam_dmd.cpp:1065) and beta = 1/kT (src/ff_dmd.cpp:166). c_d
+ * [n*c_dim] of the local components; returns max |c_d|. */
+double o_dmd_c_d(o_dmd* d, const double* r_crd, const double* zeta, const double* masses, double* c_d)
+{
+    int cd = d->c_dim, ne = d->ff->ne;
+    int n = d->n * cd;
+    int nall = d->g.nall * cd;
+    const double* x = d->g.x;
+    const double* c = d->c;
+    const double* al = d->alpha;
+    const uint8_t* el = d->elem;
+    double* fx = (double*)calloc((size_t)3 * d->n + 1, sizeof(double));
+    double* fa = (double*)calloc((size_t)n + 1, sizeof(double));
+    double* mu = (double*)calloc((size_t)nall + 1, sizeof(double));
+    double* cv = (double*)calloc((size_t)d->g.nall + 1, sizeof(double));
+    double out8[8];
+    o_dmd_force(d, fx, fa, mu, out8);
+    /* ghost update of mu (src/ff_eam_dmd.cpp:1279, update(f_c)) */
+    for (int k = d->n; k < d->g.nall; k++)
+        for (int ic = 0; ic < cd; ic++) mu[k * cd + ic] = mu[d->g.src[k] * cd + ic];
+    for (int k = 0; k < d->g.nall; k++) {
+        double v = 1.0;
+        for (int ic = 0; ic < cd; ic++)
+            if (c[k * cd + ic] > 0.0) v -= c[k * cd + ic];
+        cv[k] = v;
+    }
+    double beta = 1.0 / d->kBT;
+    double c1[8];
+    for (int e = 0; e < ne && e < 8; e++) c1[e] = sqrt(0.5 * d->kBT / masses[e]) / M_PI;
+    for (int i = 0; i < n; i++) c_d[i] = 0.0;
+    double norm = 0.0;
+    for (int i = 0; i < n; i++) {
+        if (c[i] < 0.0) continue;
+        int ei = el[i];
+        double rsq_crd = r_crd[ei] * r_crd[ei];
+        double alpha_i = al[i], alpha_sq_i = alpha_i * alpha_i, mu_i = mu[i];
+        for (int64_t k = d->row[i]; k < d->row[i + 1]; k++) {
+            int j = d->col[k];
+            if (el[j] != ei) continue;
+            double rsq = o_rsq(x + 3 * (i / cd), x + 3 * (j / cd));
+            if (rsq >= rsq_crd) continue;
+            double alpha_j = al[j], alpha_sq_j = alpha_j * alpha_j;
+            double gamma_i = rsq / (zeta[ei] * alpha_sq_i);
+            double gamma_j = rsq / (zeta[ei] * alpha_sq_j);
+            double mu_ji = beta * (mu[j] - mu_i);
+            double Q, gamma_ij_inv, theta;
+            o_dmd_calc_q(gamma_i, gamma_j, mu_ji, &Q, &gamma_ij_inv, &theta);
+            double coef0 = -c1[ei] * rsq * gamma_ij_inv * exp(-Q + 0.5 * mu_ji);
+            double dc = coef0 * (c[i] * cv[j / cd] * exp(-0.5 * mu_ji) / (alpha_sq_i * alpha_i) -
+                                 c[j] * cv[i / cd] * exp(0.5 * mu_ji) / (alpha_sq_j * alpha_j));
+            c_d[i] += dc;
+            if (j < n) c_d[j] -= dc;
+        }
+        if (fabs(c_d[i]) > norm) norm = fabs(c_d[i]);
+    }
+    free(fx); free(fa); free(mu); free(cv);
+    return norm;
+}

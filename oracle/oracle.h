@@ -118,6 +118,11 @@ double o_dmd_energy(o_dmd*);
  * drho_phi_dr[3], drho_phi_dalpha[3] in the order phi(ei,ej), rho(ei,ej),
  * rho(ej,ei) (src/ff_eam_dmd.cpp:195-253). */
 void o_dmd_pair_eval(const o_dmd*, int ei, int ej, double r, double alpha_ij, double* out9);
+/* ForceFieldEAMDMD::calc_Q with theta (src/ff_eam_dmd.cpp:1611-1633) */
+void o_dmd_calc_q(double gamma_i, double gamma_j, double mu_ji, double* Q, double* alpha_Q_sq, double* theta);
+/* ForceFieldDMD::c_d_calc + ForceFieldEAMDMD::__c_d_calc (src/ff_dmd.cpp:449-456,
+ * src/ff_eam_dmd.cpp:1359-1473): c_d [natms*c_dim]; returns max |c_d| */
+double o_dmd_c_d(o_dmd*, const double* r_crd, const double* zeta, const double* masses, double* c_d);
 
 /* ---------------- GCMC (src/sgcmc.cpp, src/ff_eam.cpp:323-589) ---------------- */
 /* One periodic cell under single-process SGCMC with EAM exchange energies.

@@ -67,6 +67,8 @@ def L():
             "o_dmd_force": (None, [C.c_void_p, dp, dp, dp, dp]),
             "o_dmd_energy": (C.c_double, [C.c_void_p]),
             "o_dmd_pair_eval": (None, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, dp]),
+            "o_dmd_calc_q": (None, [C.c_double, C.c_double, C.c_double, dp, dp, dp]),
+            "o_dmd_c_d": (C.c_double, [C.c_void_p, dp, dp, dp, dp]),
             "o_gcmc_create": (C.c_void_p, [C.c_void_p, C.c_int, dp, dp, u8p, C.c_int, C.c_double, C.c_double,
                                            C.c_double, C.c_double, dp, C.c_int, C.c_int]),
             "o_gcmc_free": (None, [C.c_void_p]),
@@ -250,6 +252,13 @@ def brute_half_pairs(H, x, elem, cut_sk, kmax=3):
     return out
 
 
+def calc_q(gamma_i, gamma_j, mu_ji):
+    """ForceFieldEAMDMD::calc_Q with theta: (Q, alpha_Q_sq, theta)."""
+    q, a, t = C.c_double(), C.c_double(), C.c_double()
+    L().o_dmd_calc_q(float(gamma_i), float(gamma_j), float(mu_ji), C.byref(q), C.byref(a), C.byref(t))
+    return q.value, a.value, t.value
+
+
 class ODMD:
     """Oracle DMD system (reference semantics: per-component half list over ghosts)."""
 
@@ -296,6 +305,12 @@ class ODMD:
         out = np.zeros(9)
         L().o_dmd_pair_eval(self.h, ei, ej, float(r), float(a), P(out))
         return out
+
+    def c_d(self, r_crd, zeta, masses):
+        """ForceFieldDMD::c_d_calc: (c_d [n][c_dim], max |c_d|)."""
+        out = np.zeros((self.n, self.C))
+        norm = L().o_dmd_c_d(self.h, P(f64(r_crd)), P(f64(zeta)), P(f64(masses)), P(out))
+        return out, norm
 
     def __del__(self):
         if getattr(self, "h", None):

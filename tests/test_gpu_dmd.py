@@ -142,3 +142,21 @@ def test_cg_lowers_free_energy_and_gradient(M, cuh):
     fo, fao, _, feo, _, _ = o.force()
     assert abs(feo - r["fe"]) < 1e-9 * abs(feo)
     assert max(np.abs(fo).max(), np.abs(fao).max()) < 10 * r["grad_max"] + 1e-9
+
+
+@pytest.mark.parametrize("cells,tilt,seed", [((3, 3, 3), (0, 0, 0), 11), ((3, 4, 3), (0.6, -0.4, 0.5), 12)])
+def test_concentration_rates_match_oracle(M, cuh, cells, tilt, seed):
+    # ForceFieldDMD::c_d_calc over the second list (same element, r < r_crd)
+    x, H, elem, alpha, c = sites(cells, seed, tilt)
+    g, o = both(M, cuh, x, H, elem, alpha, c)
+    r_crd, zeta = [3.5, 3.5], [1.7, 1.7]
+    cd, norm = g.c_d(r_crd, zeta)
+    cdo, normo = o.c_d(r_crd, zeta, cuh.masses)
+    err = np.abs(cd - cdo).max() / normo
+    say(f"c_d {cells} tilt {tilt}: max|c_d| {norm:.6e} vs {normo:.6e}, rel error {err:.1e}")
+    assert normo > 0.0 and err < RTOL
+    assert abs(norm - normo) < RTOL * normo
+    for e in (0, 1):  # exchange conserves each element
+        assert abs(cd[elem == e].sum()) < 1e-11 * np.abs(cd[elem == e]).sum()
+    _, norm2 = g.c_d(r_crd, zeta, fetch=False)
+    assert norm2 == norm

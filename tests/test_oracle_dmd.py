@@ -169,3 +169,58 @@ def test_dmd_half_list_matches_brute_force(cuh_small):
                                 if r[si, sj] - xmax * aij < cut_sk[elem[si, ic], elem[sj, jc]]:
                                     want.add((si * 2 + ic, sj * 2 + jc, a0, a1, a2))
     assert got == want
+
+
+# ---- concentration rates (ForceFieldEAMDMD::__c_d_calc) ----
+
+R_CRD = [3.5, 3.5]  # first fcc shell only, as examples/diff/in_Cu_diff.py
+ZETA = [1.7, 1.7]
+
+
+def test_exchange_barrier_symmetric_closed_form():
+    # equal widths and no chemical-potential difference: the barrier sits
+    # half way (z0 = 1/2), Q = gamma/16, alpha_Q^2 = 1/gamma, theta = -1/2
+    for g in (4.0, 64.0, 913.0):
+        Q, aQ, th = O.calc_q(g, g, 0.0)
+        assert abs(Q - g / 16.0) < 1e-14 * g and abs(aQ * g - 1.0) < 1e-14 and abs(th + 0.5) < 1e-14
+
+
+def test_exchange_barrier_detailed_balance():
+    # seen from the other site the barrier is Q - mu_ji (so forward and
+    # backward rates carry the same prefactor exp(-Q + mu_ji/2)), the width
+    # is the same and the two theta add up to -1
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        gi, gj = np.exp(rng.uniform(np.log(20.0), np.log(3000.0), 2))
+        mu = rng.uniform(-0.05, 0.05) * min(gi, gj)
+        q1, a1, t1 = O.calc_q(gi, gj, mu)
+        q2, a2, t2 = O.calc_q(gj, gi, -mu)
+        assert abs(q2 - (q1 - mu)) < 1e-12 * (abs(q1) + abs(mu))
+        assert abs(a2 - a1) < 1e-13 * a1
+        assert abs(t1 + t2 + 1.0) < 1e-12
+
+
+def test_concentration_rates_conserve_mass_per_element(cuh_small):
+    p, ff, x, H, elem, alpha, c = cuh_small
+    d = make(ff, p, x, H, elem, alpha, c, nquad=3)
+    cd, norm = d.c_d(R_CRD, ZETA, p.masses)
+    assert norm == np.abs(cd).max() and norm > 0.0
+    for e in (0, 1):
+        s = cd[elem == e]
+        assert abs(s.sum()) < 1e-12 * np.abs(s).sum(), e
+
+
+def test_concentration_rates_vanish_at_uniform_state(cuh_small):
+    # every site equivalent (perfect lattice, one alpha and c per element):
+    # equal chemical potentials and equal terms, no net exchange
+    p, ff, _, _, _, _, _ = cuh_small
+    x, H = pot.fcc(3.615, (3, 3, 3))
+    n = len(x)
+    elem = np.tile(np.array([0, 1], np.uint8), (n, 1))
+    alpha = np.tile([0.12, 0.2], (n, 1))
+    c = np.tile([0.7, 0.2], (n, 1))
+    d = make(ff, p, x, H, elem, alpha, c, nquad=3)
+    cd, norm = d.c_d(R_CRD, ZETA, p.masses)
+    d2 = make(ff, p, x + np.random.default_rng(2).normal(0, 0.05, x.shape), H, elem, alpha, c, nquad=3)
+    scale = d2.c_d(R_CRD, ZETA, p.masses)[1]
+    assert scale > 0.0 and norm < 1e-9 * scale
