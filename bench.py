@@ -190,6 +190,18 @@ def dmd_run(evals=10):
         d16.evaluate()
     ms16 = (time.perf_counter() - t0) / 3 * 1e3
     del d16
+    # concentration rates (ForceFieldDMD::c_d_calc) at the same state: both
+    # passes for the chemical potentials, then the exchange sums over the
+    # second list (first fcc shell, r_crd 3.5 A, C 1.7 as examples/diff)
+    r_crd, zeta = [3.5, 3.5], [1.7, 1.7]
+    d.c_d(r_crd, zeta, fetch=False)
+    d.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(evals):
+        _, cd_norm = d.c_d(r_crd, zeta, fetch=False)
+    dt_cd = time.perf_counter() - t0
+    kcd = d.profile_read("dmd_cd")
+    d.profile(False)
     # config 3's solver: CG on the free energy from the same state (x and alpha)
     t0 = time.perf_counter()
     cg = d.min_cg(max_iter=400, e_tol=0.0, grad_tol=1e-6)
@@ -212,6 +224,9 @@ def dmd_run(evals=10):
                                     "note": "f, f_alpha, mu gathered to caller order and copied to host each time"},
             "dtype": "f64", "half_pairs": half_pairs,
             "hbm_fraction": DMD_BYTES_PER_SITE * n / (ms / 1e3) / HBM_PEAK,
+            "c_d": {"metric": "concentration-rate evaluations (c_d_calc) site-evals/s", "value": n * evals / dt_cd,
+                    "ms_per_eval": dt_cd / evals * 1e3, "exchange_kernel_ms": kcd[0] / max(kcd[1], 1),
+                    "max_abs_c_d": cd_norm, "r_crd_A": r_crd, "C": zeta},
             "cg": {"solver": "CG on free energy over x and alpha (MinCGDMD), grad tol 1e-6, max 400 iterations",
                    "iters": cg["iters"], "status": cg["status"], "fe0": cg["fe0"], "fe": cg["fe"],
                    "grad_norm": cg["grad_norm"], "grad_max": cg["grad_max"], "seconds": cg_s},
@@ -238,6 +253,29 @@ def dmd_cpu_baseline(seconds):
     return {"value": len(x) * k / dt, "unit": "site-evals/s", "cores": 1, "kind": "port",
             "sample": f"oracle, same table and site statistics, fcc 4x4x8 cells = {len(x)} sites, {k} evaluations, "
                       f"{dt:.1f} s"}
+
+
+def dmd_cd_cpu_baseline(seconds):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from mapp4py_amd import potentials as pot
+
+    p = pot.johnson_cuh(nr=5000, nrho=5000, rc=5.5)
+    x, H, elem, alpha, c = pot.dmd_cuh_sites((4, 4, 8), seed=4)
+    x = x + np.random.default_rng(5).normal(0, 0.05, x.shape)
+    o = O.ODMD(O.OEam.from_knots(p, dmd=True), H, x, elem, alpha, c, nquad=3, masses=p.masses)
+    r_crd, zeta = [3.5, 3.5], [1.7, 1.7]
+    t0 = time.perf_counter()
+    o.c_d(r_crd, zeta, p.masses)
+    per = time.perf_counter() - t0
+    k = max(2, int(seconds / max(per, 1e-6)))
+    t0 = time.perf_counter()
+    for _ in range(k):
+        o.c_d(r_crd, zeta, p.masses)
+    dt = time.perf_counter() - t0
+    return {"value": len(x) * k / dt, "unit": "site-evals/s", "cores": 1, "kind": "port",
+            "sample": f"oracle c_d_calc (both passes + exchange sums), fcc 4x4x8 cells = {len(x)} sites, {k} "
+                      f"evaluations, {dt:.1f} s"}
 
 
 def gcmc_run(launches=10, attempts=64):
@@ -357,6 +395,7 @@ def main():
         dmd = dmd_run()
         if not args.no_cpu:
             dmd["cpu_baseline"] = dmd_cpu_baseline(args.cpu_seconds / 3)
+            dmd["c_d"]["cpu_baseline"] = dmd_cd_cpu_baseline(args.cpu_seconds / 5)
         log(f"[bench] DMD {dmd['value']:.3e} site-evals/s")
     if not args.no_gcmc:
         log("[bench] GCMC")
