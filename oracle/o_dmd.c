@@ -416,66 +416,230 @@ void o_dmd_calc_q(double gamma_i, double gamma_j, double mu_ji, double* Q, doubl
     *theta = *alpha_Q_sq * (gamma_i - gamma_j) * dz0 - dQ;
 }
 
-/* ForceFieldDMD::c_d_calc (src/ff_dmd.cpp:449-456) with
- * ForceFieldEAMDMD::__c_d_calc (src/ff_eam_dmd.cpp:1359-1473): calc_mu's
- * chemical potentials (the same sums as o_dmd_force), the second list of
- * NeighborDMD::create_2nd_list (src/neighbor_dmd.cpp:224-273: same element,
- * RSQ < r_crd^2, from the half list), c_1 = sqrt(kT / 2m) / pi
- * (src/ff_eam_dmd.cpp:1065) and beta = 1/kT (src/ff_dmd.cpp:166). c_d
- * [n*c_dim] of the local components; returns max |c_d|. */
-double o_dmd_c_d(o_dmd* d, const double* r_crd, const double* zeta, const double* masses, double* c_d)
+/* The state ForceFieldEAMDMD::calc_mu leaves (src/ff_eam_dmd.cpp:1082-1282)
+ * for c_d_calc and __J: rho_phi of every half-list pair (the smeared r*phi,
+ * rho at j, rho at i; zero for pairs beyond the cut), dE and ddE = F''(rho),
+ * the raw chemical potentials of local and ghost components and the vacancy
+ * concentration cv of every site. */
+typedef struct {
+    double* rp;   /* [npairs*3] */
+    double* dE;   /* [nall*c_dim] */
+    double* ddE;  /* [nall*c_dim] */
+    double* mu;   /* [nall*c_dim] */
+    double* cv;   /* [nall sites] */
+} dmd_static;
+
+static void dmd_static_free(dmd_static* s)
+{
+    free(s->rp); free(s->dE); free(s->ddE); free(s->mu); free(s->cv);
+}
+
+static void dmd_static_build(o_dmd* d, dmd_static* s)
 {
     int cd = d->c_dim, ne = d->ff->ne;
-    int n = d->n * cd;
-    int nall = d->g.nall * cd;
+    int n = d->n * cd, nall = d->g.nall * cd;
+    const o_eam* e = d->ff;
     const double* x = d->g.x;
     const double* c = d->c;
     const double* al = d->alpha;
     const uint8_t* el = d->elem;
-    double* fx = (double*)calloc((size_t)3 * d->n + 1, sizeof(double));
-    double* fa = (double*)calloc((size_t)n + 1, sizeof(double));
-    double* mu = (double*)calloc((size_t)nall + 1, sizeof(double));
-    double* cv = (double*)calloc((size_t)d->g.nall + 1, sizeof(double));
-    double out8[8];
-    o_dmd_force(d, fx, fa, mu, out8);
-    /* ghost update of mu (src/ff_eam_dmd.cpp:1279, update(f_c)) */
+    s->rp = (double*)calloc((size_t)d->npairs * 3 + 1, sizeof(double));
+    s->dE = (double*)calloc((size_t)nall + 1, sizeof(double));
+    s->ddE = (double*)calloc((size_t)nall + 1, sizeof(double));
+    s->mu = (double*)calloc((size_t)nall + 1, sizeof(double));
+    s->cv = (double*)calloc((size_t)d->g.nall + 1, sizeof(double));
+    double* rho = (double*)calloc((size_t)n + 1, sizeof(double));
+    double dr9[9];
+    for (int i = 0; i < n; i++) {
+        if (c[i] < 0.0) continue;
+        int ei = el[i];
+        for (int64_t k = d->row[i]; k < d->row[i + 1]; k++) {
+            int j = d->col[k];
+            int ej = el[j];
+            double r = sqrt(o_rsq(x + 3 * (i / cd), x + 3 * (j / cd)));
+            double a = sqrt(al[i] * al[i] + al[j] * al[j]);
+            if (r - a * d->xi[d->N - 1] >= d->cut[ei * ne + ej]) continue;
+            smear(d, ei, ej, r, a, s->rp + 3 * k, dr9, dr9 + 3);
+        }
+    }
+    /* rho, E, dE, ddE (src/ff_eam_dmd.cpp:1145-1180) */
+    double drho_inv = 1.0 / e->drho, rho_max = (double)e->nrho * e->drho;
+    for (int i = 0; i < n; i++) {
+        if (c[i] < 0.0) continue;
+        for (int64_t k = d->row[i]; k < d->row[i + 1]; k++) {
+            int j = d->col[k];
+            rho[i] += c[j] * s->rp[3 * k + 2];
+            if (j < n) rho[j] += c[i] * s->rp[3 * k + 1];
+        }
+        size_t m;
+        double p;
+        o_spline_pm(rho[i], drho_inv, e->nrho, &m, &p);
+        const double* cf = OF(e, el[i]) + m * 5;
+        double E = (((cf[4] * p + cf[3]) * p + cf[2]) * p + cf[1]) * p + cf[0];
+        double dE = (((4.0 * cf[4] * p + 3.0 * cf[3]) * p + 2.0 * cf[2]) * p + cf[1]) * drho_inv;
+        s->ddE[i] = ((12.0 * cf[4] * p + 6.0 * cf[3]) * p + 2.0 * cf[2]) * drho_inv * drho_inv;
+        if (rho[i] > rho_max) E += dE * (rho[i] - rho_max);
+        s->mu[i] = E;
+        s->dE[i] = dE;
+    }
+    /* ghost copies (update(dE_ptr), src/ff_eam_dmd.cpp:1182) */
     for (int k = d->n; k < d->g.nall; k++)
-        for (int ic = 0; ic < cd; ic++) mu[k * cd + ic] = mu[d->g.src[k] * cd + ic];
+        for (int ic = 0; ic < cd; ic++) {
+            s->dE[k * cd + ic] = s->dE[d->g.src[k] * cd + ic];
+            s->ddE[k * cd + ic] = s->ddE[d->g.src[k] * cd + ic];
+        }
+    /* mu_i += c_j (rho_phi_0 + rho_phi_1 dE_j), mu_j += c_i (rho_phi_0 + rho_phi_2 dE_i) */
+    for (int i = 0; i < n; i++)
+        for (int64_t k = d->row[i]; k < d->row[i + 1]; k++) {
+            int j = d->col[k];
+            s->mu[i] += c[j] * (s->rp[3 * k] + s->rp[3 * k + 1] * s->dE[j]);
+            if (j < n) s->mu[j] += c[i] * (s->rp[3 * k] + s->rp[3 * k + 2] * s->dE[i]);
+        }
+    /* update(f_c) */
+    for (int k = d->n; k < d->g.nall; k++)
+        for (int ic = 0; ic < cd; ic++) s->mu[k * cd + ic] = s->mu[d->g.src[k] * cd + ic];
     for (int k = 0; k < d->g.nall; k++) {
         double v = 1.0;
         for (int ic = 0; ic < cd; ic++)
             if (c[k * cd + ic] > 0.0) v -= c[k * cd + ic];
-        cv[k] = v;
+        s->cv[k] = v;
     }
-    double beta = 1.0 / d->kBT;
-    double c1[8];
-    for (int e = 0; e < ne && e < 8; e++) c1[e] = sqrt(0.5 * d->kBT / masses[e]) / M_PI;
+    free(rho);
+}
+
+/* The exchange terms of one second-list pair (src/ff_eam_dmd.cpp:1397-1462):
+ * dc_ij and M_IJ[0..3]; returns 0 when (i, j) is not in the second list. */
+static int dmd_exchange(const o_dmd* d, const dmd_static* s, int i, int j, const double* rsq_crd,
+                        const double* zeta, const double* c1, double beta, double* dc, double* M)
+{
+    int cd = d->c_dim;
+    const double* x = d->g.x;
+    const double* c = d->c;
+    const double* al = d->alpha;
+    int ei = d->elem[i];
+    if (d->elem[j] != ei) return 0;
+    double rsq = o_rsq(x + 3 * (i / cd), x + 3 * (j / cd));
+    if (rsq >= rsq_crd[ei]) return 0;
+    double alpha_i = al[i], alpha_sq_i = alpha_i * alpha_i;
+    double alpha_j = al[j], alpha_sq_j = alpha_j * alpha_j;
+    double gamma_i = rsq / (zeta[ei] * alpha_sq_i);
+    double gamma_j = rsq / (zeta[ei] * alpha_sq_j);
+    double mu_ji = beta * (s->mu[j] - s->mu[i]);
+    double Q, gamma_ij_inv, theta_i;
+    o_dmd_calc_q(gamma_i, gamma_j, mu_ji, &Q, &gamma_ij_inv, &theta_i);
+    double coef0 = -c1[ei] * rsq * gamma_ij_inv * exp(-Q + 0.5 * mu_ji);
+    double theta_j = theta_i + 1.0;
+    double cvi = s->cv[i / cd], cvj = s->cv[j / cd];
+    *dc = coef0 * (c[i] * cvj * exp(-0.5 * mu_ji) / (alpha_sq_i * alpha_i) -
+                   c[j] * cvi * exp(0.5 * mu_ji) / (alpha_sq_j * alpha_j));
+    double d_i = exp(-0.5 * mu_ji) / (alpha_sq_i * alpha_i);
+    double d_j = exp(0.5 * mu_ji) / (alpha_sq_j * alpha_j);
+    M[0] = beta * (theta_i * c[i] * cvj * d_i - theta_j * c[j] * cvi * d_j);
+    M[1] = d_i;
+    M[2] = d_j;
+    M[3] = coef0;
+    return 1;
+}
+
+static void dmd_exchange_params(const o_dmd* d, const double* r_crd, const double* masses, double* rsq_crd,
+                                double* c1, double* beta)
+{
+    for (int e = 0; e < d->ff->ne && e < 8; e++) {
+        rsq_crd[e] = r_crd[e] * r_crd[e];
+        c1[e] = sqrt(0.5 * d->kBT / masses[e]) / M_PI;  /* set_temp (src/ff_eam_dmd.cpp:1062-1066) */
+    }
+    *beta = 1.0 / d->kBT;  /* src/ff_dmd.cpp:166 */
+}
+
+/* ForceFieldDMD::c_d_calc (src/ff_dmd.cpp:449-456) with
+ * ForceFieldEAMDMD::__c_d_calc (src/ff_eam_dmd.cpp:1359-1473): calc_mu's
+ * chemical potentials, the second list of NeighborDMD::create_2nd_list
+ * (src/neighbor_dmd.cpp:224-273: same element, RSQ < r_crd^2, from the half
+ * list) and the exchange rates. c_d [n*c_dim] of the local components;
+ * returns max |c_d|. */
+double o_dmd_c_d(o_dmd* d, const double* r_crd, const double* zeta, const double* masses, double* c_d)
+{
+    int cd = d->c_dim;
+    int n = d->n * cd;
+    dmd_static s;
+    dmd_static_build(d, &s);
+    double rsq_crd[8], c1[8], beta, M[4];
+    dmd_exchange_params(d, r_crd, masses, rsq_crd, c1, &beta);
     for (int i = 0; i < n; i++) c_d[i] = 0.0;
     double norm = 0.0;
     for (int i = 0; i < n; i++) {
-        if (c[i] < 0.0) continue;
-        int ei = el[i];
-        double rsq_crd = r_crd[ei] * r_crd[ei];
-        double alpha_i = al[i], alpha_sq_i = alpha_i * alpha_i, mu_i = mu[i];
+        if (d->c[i] < 0.0) continue;
         for (int64_t k = d->row[i]; k < d->row[i + 1]; k++) {
             int j = d->col[k];
-            if (el[j] != ei) continue;
-            double rsq = o_rsq(x + 3 * (i / cd), x + 3 * (j / cd));
-            if (rsq >= rsq_crd) continue;
-            double alpha_j = al[j], alpha_sq_j = alpha_j * alpha_j;
-            double gamma_i = rsq / (zeta[ei] * alpha_sq_i);
-            double gamma_j = rsq / (zeta[ei] * alpha_sq_j);
-            double mu_ji = beta * (mu[j] - mu_i);
-            double Q, gamma_ij_inv, theta;
-            o_dmd_calc_q(gamma_i, gamma_j, mu_ji, &Q, &gamma_ij_inv, &theta);
-            double coef0 = -c1[ei] * rsq * gamma_ij_inv * exp(-Q + 0.5 * mu_ji);
-            double dc = coef0 * (c[i] * cv[j / cd] * exp(-0.5 * mu_ji) / (alpha_sq_i * alpha_i) -
-                                 c[j] * cv[i / cd] * exp(0.5 * mu_ji) / (alpha_sq_j * alpha_j));
+            double dc;
+            if (!dmd_exchange(d, &s, i, j, rsq_crd, zeta, c1, beta, &dc, M)) continue;
             c_d[i] += dc;
             if (j < n) c_d[j] -= dc;
         }
         if (fabs(c_d[i]) > norm) norm = fabs(c_d[i]);
     }
-    free(fx); free(fa); free(mu); free(cv);
+    dmd_static_free(&s);
     return norm;
+}
+
+/* ForceFieldEAMDMD::__J(x, Ax) (src/ff_eam_dmd.cpp:1475-1575) at the current
+ * state: the derivative of the rates c_d with respect to the concentrations,
+ * applied to xc [n*c_dim] (ghost values follow their owners: update(x_ptr)). */
+void o_dmd_J(o_dmd* d, const double* r_crd, const double* zeta, const double* masses, const double* xc, double* Ax)
+{
+    int cd = d->c_dim;
+    int n = d->n * cd, nall = d->g.nall * cd;
+    const double* c = d->c;
+    dmd_static s;
+    dmd_static_build(d, &s);
+    double* xa = (double*)calloc((size_t)nall + 1, sizeof(double));
+    double* A1 = (double*)calloc((size_t)nall + 1, sizeof(double));
+    double* b0 = (double*)calloc((size_t)nall + 1, sizeof(double));
+    double* xv = (double*)calloc((size_t)d->g.nall + 1, sizeof(double));
+    memcpy(xa, xc, sizeof(double) * n);
+    for (int k = d->n; k < d->g.nall; k++)
+        for (int ic = 0; ic < cd; ic++) xa[k * cd + ic] = xa[d->g.src[k] * cd + ic];
+    /* A1_i = c_i ddE_i sum_j rho_{j->i} x_j: the change of F'(rho_i), times c_i */
+    for (int i = 0; i < n; i++) {
+        double ci_ddEi = c[i] * s.ddE[i];
+        for (int64_t k = d->row[i]; k < d->row[i + 1]; k++) {
+            int j = d->col[k];
+            A1[i] += ci_ddEi * s.rp[3 * k + 2] * xa[j];
+            if (j < n) A1[j] += c[j] * s.ddE[j] * s.rp[3 * k + 1] * xa[i];
+        }
+    }
+    for (int k = d->n; k < d->g.nall; k++)
+        for (int ic = 0; ic < cd; ic++) A1[k * cd + ic] = A1[d->g.src[k] * cd + ic];
+    /* b0_i: the change of the chemical potential mu_i */
+    for (int i = 0; i < n; i++)
+        for (int64_t k = d->row[i]; k < d->row[i + 1]; k++) {
+            int j = d->col[k];
+            double t0 = s.rp[3 * k] + s.dE[i] * s.rp[3 * k + 2] + s.dE[j] * s.rp[3 * k + 1];
+            b0[i] += s.rp[3 * k + 1] * A1[j] + t0 * xa[j];
+            if (j < n) b0[j] += s.rp[3 * k + 2] * A1[i] + t0 * xa[i];
+        }
+    for (int k = d->n; k < d->g.nall; k++)
+        for (int ic = 0; ic < cd; ic++) b0[k * cd + ic] = b0[d->g.src[k] * cd + ic];
+    for (int k = 0; k < d->g.nall; k++)
+        for (int ic = 0; ic < cd; ic++)
+            if (c[k * cd + ic] >= 0.0) xv[k] += xa[k * cd + ic];
+    double rsq_crd[8], c1[8], beta, M[4];
+    dmd_exchange_params(d, r_crd, masses, rsq_crd, c1, &beta);
+    for (int i = 0; i < n; i++) Ax[i] = 0.0;
+    for (int i = 0; i < n; i++) {
+        if (c[i] < 0.0) continue;
+        for (int64_t k = d->row[i]; k < d->row[i + 1]; k++) {
+            int j = d->col[k];
+            double dc;
+            if (!dmd_exchange(d, &s, i, j, rsq_crd, zeta, c1, beta, &dc, M)) continue;
+            double t = M[0] * (b0[j] - b0[i]);
+            t += M[1] * (s.cv[j / cd] * xa[i] - c[i] * xv[j / cd]);
+            t -= M[2] * (s.cv[i / cd] * xa[j] - c[j] * xv[i / cd]);
+            t *= M[3];
+            Ax[i] += t;
+            if (j < n) Ax[j] -= t;
+        }
+    }
+    free(xa); free(A1); free(b0); free(xv);
+    dmd_static_free(&s);
 }

@@ -123,6 +123,8 @@ void o_dmd_calc_q(double gamma_i, double gamma_j, double mu_ji, double* Q, doubl
 /* ForceFieldDMD::c_d_calc + ForceFieldEAMDMD::__c_d_calc (src/ff_dmd.cpp:449-456,
  * src/ff_eam_dmd.cpp:1359-1473): c_d [natms*c_dim]; returns max |c_d| */
 double o_dmd_c_d(o_dmd*, const double* r_crd, const double* zeta, const double* masses, double* c_d);
+/* ForceFieldEAMDMD::__J (src/ff_eam_dmd.cpp:1475-1575): Ax = (d c_d / d c) x */
+void o_dmd_J(o_dmd*, const double* r_crd, const double* zeta, const double* masses, const double* x, double* Ax);
 
 /* ---------------- GCMC (src/sgcmc.cpp, src/ff_eam.cpp:323-589) ---------------- */
 /* One periodic cell under single-process SGCMC with EAM exchange energies.

@@ -69,6 +69,7 @@ def L():
             "o_dmd_pair_eval": (None, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, dp]),
             "o_dmd_calc_q": (None, [C.c_double, C.c_double, C.c_double, dp, dp, dp]),
             "o_dmd_c_d": (C.c_double, [C.c_void_p, dp, dp, dp, dp]),
+            "o_dmd_J": (None, [C.c_void_p, dp, dp, dp, dp, dp]),
             "o_gcmc_create": (C.c_void_p, [C.c_void_p, C.c_int, dp, dp, u8p, C.c_int, C.c_double, C.c_double,
                                            C.c_double, C.c_double, dp, C.c_int, C.c_int]),
             "o_gcmc_free": (None, [C.c_void_p]),
@@ -311,6 +312,13 @@ class ODMD:
         out = np.zeros((self.n, self.C))
         norm = L().o_dmd_c_d(self.h, P(f64(r_crd)), P(f64(zeta)), P(f64(masses)), P(out))
         return out, norm
+
+    def J(self, r_crd, zeta, masses, xc):
+        """ForceFieldEAMDMD::__J: (d c_d / d c) applied to xc [n][c_dim]."""
+        out = np.zeros((self.n, self.C))
+        L().o_dmd_J(self.h, P(f64(r_crd)), P(f64(zeta)), P(f64(masses)), P(f64(np.asarray(xc).reshape(self.n, self.C))),
+                    P(out))
+        return out
 
     def __del__(self):
         if getattr(self, "h", None):

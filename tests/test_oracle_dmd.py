@@ -224,3 +224,27 @@ def test_concentration_rates_vanish_at_uniform_state(cuh_small):
     d2 = make(ff, p, x + np.random.default_rng(2).normal(0, 0.05, x.shape), H, elem, alpha, c, nquad=3)
     scale = d2.c_d(R_CRD, ZETA, p.masses)[1]
     assert scale > 0.0 and norm < 1e-9 * scale
+
+
+def test_jacobian_product_is_the_derivative_of_the_rates(cuh_small):
+    # __J(x) is d c_d / d c applied to x: central differences of c_d along x
+    # (through the chemical potentials, F'' and the vacancy concentrations)
+    # agree to the differencing error
+    p, ff, x, H, elem, alpha, c = cuh_small
+    d = make(ff, p, x, H, elem, alpha, c, nquad=3)
+    v = np.random.default_rng(5).normal(0.0, 1.0, c.shape)
+    Jv = d.J(R_CRD, ZETA, p.masses, v)
+    h = 1e-6
+    d.set_state(x, alpha, c + h * v)
+    cp = d.c_d(R_CRD, ZETA, p.masses)[0]
+    d.set_state(x, alpha, c - h * v)
+    cm = d.c_d(R_CRD, ZETA, p.masses)[0]
+    fd = (cp - cm) / (2.0 * h)
+    assert np.abs(fd).max() > 1e-4
+    assert np.abs(fd - Jv).max() < 1e-7 * np.abs(fd).max()
+    # linear in x
+    w = np.random.default_rng(6).normal(0.0, 1.0, c.shape)
+    d.set_state(x, alpha, c)
+    Jw = d.J(R_CRD, ZETA, p.masses, w)
+    Jvw = d.J(R_CRD, ZETA, p.masses, 2.0 * v - 3.0 * w)
+    assert np.abs(Jvw - (2.0 * Jv - 3.0 * Jw)).max() < 1e-12 * np.abs(Jvw).max()
