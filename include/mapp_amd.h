@@ -157,6 +157,13 @@ mapp_status mapp_dmd_set_batch(mapp_dmd* d, int batch);
  * c_d [natms][c_dim] in caller order (may be NULL: kept on the device);
  * c_d_norm = max |c_d| (may be NULL). */
 mapp_status mapp_dmd_c_d(mapp_dmd* d, const double* r_crd, const double* C, double* c_d, double* c_d_norm);
+/* ForceFieldEAMDMD::__J (src/ff_eam_dmd.cpp:1475-1575): Ax = (d c_d / d c) x,
+ * the Jacobian of the concentration rates applied to x, at the state and
+ * r_crd / C of the last mapp_dmd_c_d call (as __J reads the rho_phi, ddE and
+ * chemical potentials c_d_calc left). x and Ax [natms][c_dim] in caller
+ * order. MAPP_ERR_STATE if x, alpha or c changed since that call, or if it
+ * ran through the surrogate networks. */
+mapp_status mapp_dmd_J(mapp_dmd* d, const double* x, double* Ax);
 mapp_status mapp_dmd_value(mapp_dmd* d, int with_c, const double* ext_mu, double* fe);
 /* NeighborDMD::create_list half list (src/neighbor_dmd.cpp:102-194) as
  * (component i, component j, image) triples in caller indices */

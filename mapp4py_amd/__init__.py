@@ -287,6 +287,14 @@ class AtomsDMD:
         check(lib().mapp_dmd_c_d(self._h, ptr(rc), ptr(zt), ptr(out) if fetch else None, C.byref(norm)))
         return out, norm.value
 
+    def J(self, x):
+        """ForceFieldEAMDMD::__J: (d c_d / d c) x [nsites][c_dim] at the state and
+        exchange parameters of the last c_d call."""
+        xv = f64(np.asarray(x).reshape(self.nsites, self.c_dim))
+        out = np.zeros((self.nsites, self.c_dim))
+        check(lib().mapp_dmd_J(self._h, ptr(xv), ptr(out)))
+        return out
+
     def set_batch(self, b):
         """List entries per batch of the quadrature passes (1, 2 or 4)."""
         check(lib().mapp_dmd_set_batch(self._h, int(b)))

@@ -58,6 +58,7 @@ SIGNATURES = {
     "mapp_dmd_set_batch": (C.c_int, [C.c_void_p, C.c_int]),
     "mapp_dmd_c_d": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                C.POINTER(C.c_double)]),
+    "mapp_dmd_J": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mapp_dmd_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, dp, u8p, dp, dp, C.c_double,
                                   C.c_double, C.c_double, C.c_double, dp, vpp]),
     "mapp_dmd_destroy": (None, [C.c_void_p]),

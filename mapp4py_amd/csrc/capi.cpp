@@ -346,6 +346,16 @@ mapp_status mapp_dmd_c_d(mapp_dmd* d, const double* r_crd, const double* C, doub
     });
 }
 
+mapp_status mapp_dmd_J(mapp_dmd* d, const double* x, double* Ax)
+{
+    return guard([&] {
+        NEED(d);
+        NEED(x);
+        NEED(Ax);
+        d->sys->J(x, Ax);
+    });
+}
+
 mapp_status mapp_dmd_value(mapp_dmd* d, int with_c, const double* ext_mu, double* fe)
 {
     return guard([&] { NEED(d); NEED(fe); *fe = d->sys->value(with_c != 0, ext_mu); });

@@ -651,11 +651,6 @@ __device__ inline void exchange_q(double gamma_i, double gamma_j, double mu_ji, 
     theta = alpha_Q_sq * (gamma_i - gamma_j) * dz0 - dQ;
 }
 
-struct CdArgs {
-    double rsq_crd[kMaxElems], zeta[kMaxElems], c1[kMaxElems];
-    double beta;
-};
-
 // ForceFieldEAMDMD::__c_d_calc (src/ff_eam_dmd.cpp:1359-1473) over the second
 // list of NeighborDMD::create_2nd_list (src/neighbor_dmd.cpp:224-273): the
 // entries of the component's list with the same element and r^2 < r_crd^2.
@@ -720,6 +715,216 @@ __global__ __launch_bounds__(kBlock) void k_dmd_cd(int nc, const double4* __rest
         __syncthreads();
     }
     if (threadIdx.x == 0) part[(size_t)blk * kPartW] = sh[0];
+}
+
+// ---------------- Jacobian of the concentration rates ----------------
+
+// The state ForceFieldEAMDMD::__J reads (src/ff_eam_dmd.cpp:1475-1575):
+// rho_phi of every list slot -- the smeared r*phi, the density at j from i
+// and the density at i from j, zero past the cut -- stored slot-major like
+// the list, and ddE = F''(rho) of every component (src/ff_eam_dmd.cpp:1177).
+template <int B>
+__global__ __launch_bounds__(kBlock) void k_dmd_jrp(int n, const double4* __restrict__ x, const double* __restrict__ al,
+                                                    const double* __restrict__ cc, const int* __restrict__ el,
+                                                    const int* __restrict__ nn, const uint32_t* __restrict__ list,
+                                                    int ldl, int maxn, DmdArgs A, BoxDev bd,
+                                                    double* __restrict__ rp0, double* __restrict__ rp1,
+                                                    double* __restrict__ rp2, double* __restrict__ ddE)
+{
+    const int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int C = A.C;
+    const double4 xi = x[i];
+    for (int ic = 0; ic < C; ++ic) {
+        const int ci_idx = i * C + ic;
+        const double ci = cc[ci_idx];
+        if (ci < 0.0) {
+            ddE[ci_idx] = 0.0;
+            continue;
+        }
+        const double ai = al[ci_idx];
+        const int ei = el[ci_idx];
+        double rho = 0.0;
+        const int cnt = nn[ci_idx];
+        for (int k0 = 0; k0 < cnt; k0 += B) {
+            uint32_t e[B];
+            double4 xj[B];
+            double aj[B], cj[B];
+            int tt[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) e[b] = dlist_entry(list, ldl, maxn, k0 + b, ci_idx, cnt);
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int jc = (int)(e[b] & kIndexMask);
+                xj[b] = x[jc / C];
+                aj[b] = al[jc];
+                cj[b] = cc[jc];
+                tt[b] = ei * A.ne + el[jc];
+            }
+            PairGeo pg[B];
+            bool live[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                pg[b] = pair_geo(xi, xj[b], e[b] >> kIndexBits, bd.h, ai, aj[b], A.xmax, A.cut[tt[b]]);
+                live[b] = pg[b].live && k0 + b < cnt;
+            }
+            double H[B][3];
+#pragma unroll
+            for (int b = 0; b < B; ++b) H[b][0] = H[b][1] = H[b][2] = 0.0;
+            for (int l = 0; l < A.N; ++l) {
+                const double xl = A.xi[l], wl = A.wi[l];
+                double4 c[B][3];
+                double p[B], sg[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const double u = pg[b].r - xl * pg[b].a;
+                    const int m = dknot(A, u, p[b]);
+                    sg[b] = u < 0.0 ? -wl : wl;
+                    const size_t row = live[b] ? ((size_t)tt[b] * A.nr + m) * 3 : 0;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) c[b][k] = A.tab[row + k];
+                }
+#pragma unroll
+                for (int b = 0; b < B; ++b)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) H[b][k] += hv(c[b][k], p[b]) * sg[b];
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                if (k0 + b >= cnt) continue;
+                const double rinv = 1.0 / pg[b].r;
+                const double v0 = live[b] ? kPiInSq * rinv * H[b][0] : 0.0;
+                const double v1 = live[b] ? kPiInSq * rinv * H[b][1] : 0.0;
+                const double v2 = live[b] ? kPiInSq * rinv * H[b][2] : 0.0;
+                const size_t slot = (size_t)(k0 + b) * ldl + ci_idx;
+                rp0[slot] = v0;
+                rp1[slot] = v1;
+                rp2[slot] = v2;
+                rho += cj[b] * v2;
+            }
+        }
+        // F''(rho) on the clamped interval of femb5 (no second derivative past rho_max)
+        double p = rho * A.drho_inv;
+        int m = (int)p;
+        m = m < A.nrho - 2 ? m : A.nrho - 2;
+        m = m > 0 ? m : 0;
+        p -= (double)m;
+        p = p < 1.0 ? p : 1.0;
+        const double* cf = A.F5 + ((size_t)ei * A.nrho + m) * 5;
+        ddE[ci_idx] = ((12.0 * cf[4] * p + 6.0 * cf[3]) * p + 2.0 * cf[2]) * A.drho_inv * A.drho_inv;
+    }
+}
+
+// __J, first sum: A1_i = c_i F''(rho_i) sum_j rho_{j->i} x_j, the change of
+// F'(rho_i) times c_i. Every component sums its own list (the reference's
+// half list adds the j side of each pair to A1_j in the same loop).
+__global__ __launch_bounds__(kBlock) void k_dmd_ja1(int nc, const double* __restrict__ cc, const int* __restrict__ nn,
+                                                    const uint32_t* __restrict__ list, int ldl,
+                                                    const double* __restrict__ rp2, const double* __restrict__ ddE,
+                                                    const double* __restrict__ xv, double* __restrict__ a1)
+{
+    const int ic = xcd_block() * kBlock + threadIdx.x;
+    if (ic >= nc) return;
+    const double ci = cc[ic];
+    double s = 0.0;
+    if (ci >= 0.0) {
+        const int cnt = nn[ic];
+        for (int k = 0; k < cnt; ++k) {
+            const size_t slot = (size_t)k * ldl + ic;
+            const int jc = (int)(list[slot] & kIndexMask);
+            s += rp2[slot] * xv[jc];
+        }
+        s *= ci * ddE[ic];
+    }
+    a1[ic] = s;
+}
+
+// __J, second sum: b0_i = sum_j rho_{i->j} A1_j + (r phi + dE_i rho_{j->i}
+// + dE_j rho_{i->j}) x_j, the change of the chemical potential mu_i
+__global__ __launch_bounds__(kBlock) void k_dmd_jb0(int nc, const double* __restrict__ cc, const int* __restrict__ nn,
+                                                    const uint32_t* __restrict__ list, int ldl,
+                                                    const double* __restrict__ rp0, const double* __restrict__ rp1,
+                                                    const double* __restrict__ rp2, const double* __restrict__ dE,
+                                                    const double* __restrict__ a1, const double* __restrict__ xv,
+                                                    double* __restrict__ b0)
+{
+    const int ic = xcd_block() * kBlock + threadIdx.x;
+    if (ic >= nc) return;
+    double s = 0.0;
+    if (cc[ic] >= 0.0) {
+        const double dEi = dE[ic];
+        const int cnt = nn[ic];
+        for (int k = 0; k < cnt; ++k) {
+            const size_t slot = (size_t)k * ldl + ic;
+            const int jc = (int)(list[slot] & kIndexMask);
+            const double r1 = rp1[slot];
+            s += r1 * a1[jc] + (rp0[slot] + dEi * rp2[slot] + dE[jc] * r1) * xv[jc];
+        }
+    }
+    b0[ic] = s;
+}
+
+// __J, the product over the second list (src/ff_eam_dmd.cpp:1536-1572 with
+// the M_IJ coefficients of src/ff_eam_dmd.cpp:1430-1462): each component sums
+// its own exchanges; seen from j the term is minus this one (coef0 and M0 are
+// symmetric in i and j, M1 and M2 trade places).
+__global__ __launch_bounds__(kBlock) void k_dmd_jax(int nc, const double4* __restrict__ x, const double* __restrict__ al,
+                                                    const double* __restrict__ cc, const int* __restrict__ el,
+                                                    const int* __restrict__ nn, const uint32_t* __restrict__ list,
+                                                    int ldl, int C, BoxDev bd, CdArgs P,
+                                                    const double* __restrict__ mu, const double* __restrict__ b0,
+                                                    const double* __restrict__ xv, double* __restrict__ ax)
+{
+#pragma clang fp contract(off)
+    const int ic = xcd_block() * kBlock + threadIdx.x;
+    if (ic >= nc) return;
+    double acc = 0.0;
+    const double ci = cc[ic];
+    if (ci >= 0.0) {
+        const int site = ic / C, ei = el[ic];
+        const double4 xi = x[site];
+        const double alpha_i = al[ic], alpha_sq_i = alpha_i * alpha_i, mu_i = mu[ic], b0_i = b0[ic], x_i = xv[ic];
+        double cv_i = 1.0, xs_i = 0.0;
+        for (int k = 0; k < C; ++k) {
+            const double v = cc[site * C + k];
+            if (v > 0.0) cv_i -= v;
+            if (v >= 0.0) xs_i += xv[site * C + k];
+        }
+        const double zeta = P.zeta[ei], c1 = P.c1[ei], rsq_crd = P.rsq_crd[ei];
+        const int cnt = nn[ic];
+        for (int k = 0; k < cnt; ++k) {
+            const uint32_t e = list[(size_t)k * ldl + ic];
+            const int jc = (int)(e & kIndexMask);
+            if (el[jc] != ei) continue;
+            double dx, dy, dz;
+            dpair_dx(xi, x[jc / C], e >> kIndexBits, bd.h, dx, dy, dz);
+            const double rsq = dx * dx + (dy * dy + dz * dz);
+            if (rsq >= rsq_crd) continue;
+            const double alpha_j = al[jc], alpha_sq_j = alpha_j * alpha_j;
+            const double gamma_i = rsq / (zeta * alpha_sq_i), gamma_j = rsq / (zeta * alpha_sq_j);
+            const double mu_ji = P.beta * (mu[jc] - mu_i);
+            double Q, gamma_ij_inv, theta_i;
+            exchange_q(gamma_i, gamma_j, mu_ji, Q, gamma_ij_inv, theta_i);
+            const double coef0 = -c1 * rsq * gamma_ij_inv * exp(-Q + 0.5 * mu_ji);
+            const double theta_j = theta_i + 1.0;
+            const int sj = jc / C;
+            double cv_j = 1.0, xs_j = 0.0;
+            for (int q = 0; q < C; ++q) {
+                const double v = cc[sj * C + q];
+                if (v > 0.0) cv_j -= v;
+                if (v >= 0.0) xs_j += xv[sj * C + q];
+            }
+            const double cj = cc[jc];
+            const double d_i = exp(-0.5 * mu_ji) / (alpha_sq_i * alpha_i);
+            const double d_j = exp(0.5 * mu_ji) / (alpha_sq_j * alpha_j);
+            const double M0 = P.beta * (theta_i * ci * cv_j * d_i - theta_j * cj * cv_i * d_j);
+            double t = M0 * (b0[jc] - b0_i);
+            t += d_i * (cv_j * x_i - ci * xs_j);
+            t -= d_j * (cv_i * xv[jc] - cj * xs_i);
+            acc += coef0 * t;
+        }
+    }
+    ax[ic] = acc;
 }
 
 // chemical potential of new_derivative<true> (src/ff_dmd.cpp:305-366) and
@@ -933,6 +1138,16 @@ __global__ __launch_bounds__(kBlock) void k_dgather_comp(int n, int C, const int
     if (id >= n) return;
     const int s = slot_of[id];
     for (int k = 0; k < C; ++k) ao[(size_t)id * C + k] = a[s * C + k];
+}
+
+// per-component array from caller order to slot order
+__global__ __launch_bounds__(kBlock) void k_dscatter_comp(int n, int C, const int* __restrict__ slot_of,
+                                                          const double* __restrict__ a, double* __restrict__ ao)
+{
+    const int id = blockIdx.x * kBlock + threadIdx.x;
+    if (id >= n) return;
+    const int s = slot_of[id];
+    for (int k = 0; k < C; ++k) ao[s * C + k] = a[(size_t)id * C + k];
 }
 
 __global__ __launch_bounds__(kBlock) void k_smear_points(DmdArgs A, int tt, int npts, const double* __restrict__ r,
@@ -1185,6 +1400,7 @@ void DMDSystem::reduce_rows(const double* part, int nrows, int ncols, double* ou
 
 void DMDSystem::rebuild()
 {
+    ++ver_;
     const int nb = grid_for(n_);
     DGrid g{};
     for (int d = 0; d < 3; ++d) {
@@ -1425,6 +1641,15 @@ double DMDSystem::c_d(const double* r_crd, const double* zeta, double* c_d_host)
     // calc_mu: rho, E, dE and the raw chemical potentials of the passes
     pass1(false);
     pass2();
+    // what J reads later at this state: the parameters, mu and dE (the
+    // slot values and F'' follow on the first J call)
+    jmu_.reserve(nc_);
+    jdE_.reserve(nc_);
+    MAPP_HIP(hipMemcpyAsync(jmu_.get(), mu_.get(), sizeof(double) * nc_, hipMemcpyDeviceToDevice, stream_));
+    MAPP_HIP(hipMemcpyAsync(jdE_.get(), dE_.get(), sizeof(double) * nc_, hipMemcpyDeviceToDevice, stream_));
+    jP_ = P;
+    jver_ = surr_ ? -1 : ver_;
+    jrp_ver_ = -1;
     cd_.reserve(nc_);
     BoxDev bd;
     std::memcpy(bd.h, box_.h, sizeof bd.h);
@@ -1448,6 +1673,60 @@ double DMDSystem::c_d(const double* r_crd, const double* zeta, double* c_d_host)
     MAPP_HIP(hipStreamSynchronize(stream_));
     prof_.flush();
     return norm;
+}
+
+void DMDSystem::J(const double* x_host, double* ax_host)
+{
+    MAPP_REQUIRE(jver_ >= 0 && jver_ == ver_, ERR_STATE,
+                 "J needs c_d at the current state (quadrature, no surrogate) first");
+    BoxDev bd;
+    std::memcpy(bd.h, box_.h, sizeof bd.h);
+    std::memcpy(bd.b, box_.b, sizeof bd.b);
+    const int nb = grid_for(n_);
+    if (jrp_ver_ != ver_) {
+        const size_t slots = (size_t)std::max(maxn_, 1) * ldl_;
+        jrp0_.reserve(slots);
+        jrp1_.reserve(slots);
+        jrp2_.reserve(slots);
+        jddE_.reserve(nc_);
+        prof_.begin(K_DMD_JSTATE, stream_);
+        hipLaunchKernelGGL(k_dmd_jrp<4>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
+                           el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, jrp0_.get(), jrp1_.get(),
+                           jrp2_.get(), jddE_.get());
+        prof_.end(stream_);
+        jrp_ver_ = ver_;
+    }
+    jx_.reserve(nc_);
+    ja1_.reserve(nc_);
+    jb0_.reserve(nc_);
+    jax_.reserve(nc_);
+    DevBuf<double> th;
+    th.reserve(nc_);
+    MAPP_HIP(hipMemcpyAsync(th.get(), x_host, sizeof(double) * nc_, hipMemcpyHostToDevice, stream_));
+    hipLaunchKernelGGL(k_dscatter_comp, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n_, C_,
+                       slot_of_.get(), th.get(), jx_.get());
+    J_dev();
+    hipLaunchKernelGGL(k_dgather_comp, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n_, C_,
+                       slot_of_.get(), jax_.get(), th.get());
+    MAPP_HIP(hipMemcpyAsync(ax_host, th.get(), sizeof(double) * nc_, hipMemcpyDeviceToHost, stream_));
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    prof_.flush();
+}
+
+void DMDSystem::J_dev()
+{
+    BoxDev bd;
+    std::memcpy(bd.h, box_.h, sizeof bd.h);
+    std::memcpy(bd.b, box_.b, sizeof bd.b);
+    const int nbc = grid_for(nc_);
+    prof_.begin(K_DMD_J, stream_);
+    hipLaunchKernelGGL(k_dmd_ja1, dim3(nbc), dim3(kBlock), 0, stream_, nc_, c_.get(), nn_.get(), list_.get(), ldl_,
+                       jrp2_.get(), jddE_.get(), jx_.get(), ja1_.get());
+    hipLaunchKernelGGL(k_dmd_jb0, dim3(nbc), dim3(kBlock), 0, stream_, nc_, c_.get(), nn_.get(), list_.get(), ldl_,
+                       jrp0_.get(), jrp1_.get(), jrp2_.get(), jdE_.get(), ja1_.get(), jx_.get(), jb0_.get());
+    hipLaunchKernelGGL(k_dmd_jax, dim3(nbc), dim3(kBlock), 0, stream_, nc_, x_.get(), al_.get(), c_.get(), el_.get(),
+                       nn_.get(), list_.get(), ldl_, C_, bd, jP_, jmu_.get(), jb0_.get(), jx_.get(), jax_.get());
+    prof_.end(stream_);
 }
 
 DMDReport DMDSystem::derivative(bool with_c, const double* ext_mu)
@@ -1501,6 +1780,7 @@ double DMDSystem::value(bool with_c, const double* ext_mu)
 
 void DMDSystem::set_state(const double* x, const double* alpha, const double* c)
 {
+    ++ver_;
     DevBuf<double> tx, ta, tc;
     tx.reserve((size_t)3 * n_);
     ta.reserve(nc_);
@@ -1617,6 +1897,7 @@ void DMDSystem::smear(int ei, int ej, int npts, const double* r, const double* a
 // gradient component drops below grad_tol (0 disables).
 CGReport DMDSystem::cg(int max_iter, double e_tol, double max_dx, double max_dalpha, double grad_tol)
 {
+    ++ver_;
     const int nb = grid_for(std::max(n_, nc_));
     const int nbe = (std::max(n_, nc_) + kBlock - 1) / kBlock;
     cg_x0_.reserve(n_); cg_h_.reserve(n_); cg_f0_.reserve(n_);

@@ -33,6 +33,13 @@ struct DMDReport {
     double s;       // configurational entropy sum (units of kB)
 };
 
+// exchange parameters of the concentration rates: r_crd^2, the exchange
+// width zeta and c_1 = sqrt(kT / 2m) / pi per element, beta = 1 / kT
+struct CdArgs {
+    double rsq_crd[kMaxElems], zeta[kMaxElems], c1[kMaxElems];
+    double beta;
+};
+
 struct CGReport {
     int iters, status, rebuilds, evals;
     double fe0, fe, grad_norm, grad_max;
@@ -61,6 +68,10 @@ public:
     // (same element, r < r_crd[e]) with exchange widths zeta[e]; c_d in caller
     // order [site][c_dim] when c_d_host is not null; returns max |c_d|
     double c_d(const double* r_crd, const double* zeta, double* c_d_host);
+    // ForceFieldEAMDMD::__J: Ax = (d c_d / d c) x at the state and exchange
+    // parameters of the last c_d call (an error if the state changed since);
+    // x and Ax in caller order [site][c_dim]
+    void J(const double* x, double* ax);
     int64_t half_pairs_count();
     void half_pairs(int32_t* i, int32_t* j, int8_t* img);
     // MinCGDMD over x and alpha (c fixed)
@@ -85,6 +96,7 @@ private:
     void pass2();
     bool decide();
     void reduce_rows(const double* part, int nrows, int ncols, double* out);
+    void J_dev();
 
     std::shared_ptr<EamForceField> ff_;
     int n_, C_, nc_;  // sites, components per site, total components
@@ -102,6 +114,11 @@ private:
     DevBuf<double4> f_;
     DevBuf<double> fa_, mu_, E_, dE_;
     DevBuf<double> cd_;              // concentration rates per component
+    // Jacobian state: bumped by every change of x, alpha, c or the slot order
+    int64_t ver_ = 0, jver_ = -1, jrp_ver_ = -1;
+    CdArgs jP_{};
+    DevBuf<double> jmu_, jdE_, jddE_, jrp0_, jrp1_, jrp2_;  // rho_phi of every list slot
+    DevBuf<double> jx_, ja1_, jb0_, jax_;
     std::vector<double> masses_;
     DevBuf<uint32_t> keys_, keys_alt_;
     DevBuf<int> vals_, vals_alt_;

@@ -160,3 +160,35 @@ def test_concentration_rates_match_oracle(M, cuh, cells, tilt, seed):
         assert abs(cd[elem == e].sum()) < 1e-11 * np.abs(cd[elem == e]).sum()
     _, norm2 = g.c_d(r_crd, zeta, fetch=False)
     assert norm2 == norm
+
+
+@pytest.mark.parametrize("cells,tilt,seed", [((3, 3, 3), (0, 0, 0), 11), ((3, 4, 3), (0.6, -0.4, 0.5), 12)])
+def test_jacobian_product_matches_oracle(M, cuh, cells, tilt, seed):
+    # ForceFieldEAMDMD::__J at the state of the last c_d call
+    x, H, elem, alpha, c = sites(cells, seed, tilt)
+    g, o = both(M, cuh, x, H, elem, alpha, c)
+    r_crd, zeta = [3.5, 3.5], [1.7, 1.7]
+    with pytest.raises(RuntimeError):
+        g.J(np.zeros_like(c))  # no c_d at this state yet
+    g.c_d(r_crd, zeta, fetch=False)
+    rng = np.random.default_rng(seed)
+    v, w = rng.normal(0.0, 1.0, c.shape), rng.normal(0.0, 1.0, c.shape)
+    Jv = g.J(v)
+    Jvo = o.J(r_crd, zeta, cuh.masses, v)
+    scale = np.abs(Jvo).max()
+    err = np.abs(Jv - Jvo).max() / scale
+    say(f"J {cells} tilt {tilt}: max|Jx| {scale:.6e}, rel error vs oracle {err:.1e}")
+    assert scale > 0.0 and err < RTOL
+    # a second product at the same state reuses the slot values; linear in x
+    Jw = g.J(w)
+    Jvw = g.J(2.0 * v - 3.0 * w)
+    assert np.abs(Jvw - (2.0 * Jv - 3.0 * Jw)).max() < 1e-12 * np.abs(Jvw).max()
+    for e in (0, 1):  # exchange conserves each element, also to first order
+        assert abs(Jv[elem == e].sum()) < 1e-11 * np.abs(Jv[elem == e]).sum()
+    # a new state needs a new c_d (here the same values written again)
+    xs, as_, cs = g.state()
+    g.set_state(xs, as_, cs)
+    with pytest.raises(RuntimeError):
+        g.J(v)
+    g.c_d(r_crd, zeta, fetch=False)
+    np.testing.assert_allclose(g.J(v), Jv, rtol=0, atol=1e-13 * scale)
