@@ -138,6 +138,13 @@ def feni_run(steps=1000, warmup=20):
             "kernel_ms": {k: ms / max(c, 1) for k, (ms, c) in kern.items()}, "dtype": "f32+f64"}
 
 
+# algorithmic bytes of one __J product: per live list entry the partner
+# index and rho_{j->i} (first sum) and the index, rho_{i->j} and t0 (second);
+# per second-list entry the index and three coefficients; per component the
+# counts, F'', S0..S2, c, x, A1, b0, xs and Ax once each
+J_BYTES_PER_LIVE = 32
+J_BYTES_PER_EXCHANGE = 28
+J_BYTES_PER_COMPONENT = 120
 DMD_BYTES_PER_SITE = 132  # x 16, alpha 8, c 16 (read twice), f 12, f_alpha 8, mu 16, E/dE write + read 32
 
 
@@ -201,7 +208,20 @@ def dmd_run(evals=10):
         _, cd_norm = d.c_d(r_crd, zeta, fetch=False)
     dt_cd = time.perf_counter() - t0
     kcd = d.profile_read("dmd_cd")
+    # Jacobian products (ForceFieldEAMDMD::__J) at that state: the first one
+    # stores the smeared values of every list slot and F''; later ones are
+    # three gathers over the list. Caller-order x in, Ax out (host copies).
+    v = np.random.default_rng(7).normal(0.0, 1.0, c.shape)
+    d.J(v)
+    kjs = d.profile_read("dmd_j_state")
+    t0 = time.perf_counter()
+    for _ in range(evals):
+        d.J(v)
+    dt_j = time.perf_counter() - t0
+    kj = d.profile_read("dmd_j")
     d.profile(False)
+    j_live, j_exch = d.J_entries()
+    j_bytes = J_BYTES_PER_LIVE * j_live + J_BYTES_PER_EXCHANGE * j_exch + J_BYTES_PER_COMPONENT * c.size
     # config 3's solver: CG on the free energy from the same state (x and alpha)
     t0 = time.perf_counter()
     cg = d.min_cg(max_iter=400, e_tol=0.0, grad_tol=1e-6)
@@ -227,6 +247,11 @@ def dmd_run(evals=10):
             "c_d": {"metric": "concentration-rate evaluations (c_d_calc) site-evals/s", "value": n * evals / dt_cd,
                     "ms_per_eval": dt_cd / evals * 1e3, "exchange_kernel_ms": kcd[0] / max(kcd[1], 1),
                     "max_abs_c_d": cd_norm, "r_crd_A": r_crd, "C": zeta},
+            "J": {"metric": "Jacobian products (__J) site-evals/s, device kernels", "value": n / (kj[0] / kj[1] / 1e3),
+                  "kernels_ms": kj[0] / kj[1], "host_copy_inclusive_ms": dt_j / evals * 1e3,
+                  "slot_state_ms": kjs[0] / max(kjs[1], 1),
+                  "live_entries": j_live, "exchange_entries": j_exch, "algorithmic_bytes": j_bytes,
+                  "hbm_fraction": j_bytes / (kj[0] / kj[1] / 1e3) / HBM_PEAK},
             "cg": {"solver": "CG on free energy over x and alpha (MinCGDMD), grad tol 1e-6, max 400 iterations",
                    "iters": cg["iters"], "status": cg["status"], "fe0": cg["fe0"], "fe": cg["fe"],
                    "grad_norm": cg["grad_norm"], "grad_max": cg["grad_max"], "seconds": cg_s},
@@ -276,6 +301,30 @@ def dmd_cd_cpu_baseline(seconds):
     return {"value": len(x) * k / dt, "unit": "site-evals/s", "cores": 1, "kind": "port",
             "sample": f"oracle c_d_calc (both passes + exchange sums), fcc 4x4x8 cells = {len(x)} sites, {k} "
                       f"evaluations, {dt:.1f} s"}
+
+
+def dmd_j_cpu_baseline(seconds):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from mapp4py_amd import potentials as pot
+
+    p = pot.johnson_cuh(nr=5000, nrho=5000, rc=5.5)
+    x, H, elem, alpha, c = pot.dmd_cuh_sites((4, 4, 8), seed=4)
+    x = x + np.random.default_rng(5).normal(0, 0.05, x.shape)
+    o = O.ODMD(O.OEam.from_knots(p, dmd=True), H, x, elem, alpha, c, nquad=3, masses=p.masses)
+    r_crd, zeta = [3.5, 3.5], [1.7, 1.7]
+    v = np.random.default_rng(7).normal(0.0, 1.0, c.shape)
+    t0 = time.perf_counter()
+    o.J(r_crd, zeta, p.masses, v)
+    per = time.perf_counter() - t0
+    k = max(2, int(seconds / max(per, 1e-6)))
+    t0 = time.perf_counter()
+    for _ in range(k):
+        o.J(r_crd, zeta, p.masses, v)
+    dt = time.perf_counter() - t0
+    return {"value": len(x) * k / dt, "unit": "site-evals/s", "cores": 1, "kind": "port",
+            "sample": f"oracle __J, state rebuilt per call (calc_mu + rate coefficients), fcc 4x4x8 cells = "
+                      f"{len(x)} sites, {k} products, {dt:.1f} s"}
 
 
 def gcmc_run(launches=10, attempts=64):
@@ -396,6 +445,7 @@ def main():
         if not args.no_cpu:
             dmd["cpu_baseline"] = dmd_cpu_baseline(args.cpu_seconds / 3)
             dmd["c_d"]["cpu_baseline"] = dmd_cd_cpu_baseline(args.cpu_seconds / 5)
+            dmd["J"]["cpu_baseline"] = dmd_j_cpu_baseline(args.cpu_seconds / 5)
         log(f"[bench] DMD {dmd['value']:.3e} site-evals/s")
     if not args.no_gcmc:
         log("[bench] GCMC")

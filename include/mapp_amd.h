@@ -164,6 +164,10 @@ mapp_status mapp_dmd_c_d(mapp_dmd* d, const double* r_crd, const double* C, doub
  * order. MAPP_ERR_STATE if x, alpha or c changed since that call, or if it
  * ran through the surrogate networks. */
 mapp_status mapp_dmd_J(mapp_dmd* d, const double* x, double* Ax);
+/* Diagnostic (no counterpart in the reference): the entries the state of the
+ * last mapp_dmd_J call holds, list entries inside the smeared cut (live) and
+ * second-list entries (exchange), summed over components. */
+mapp_status mapp_dmd_J_entries(mapp_dmd* d, int64_t* live, int64_t* exchange);
 mapp_status mapp_dmd_value(mapp_dmd* d, int with_c, const double* ext_mu, double* fe);
 /* NeighborDMD::create_list half list (src/neighbor_dmd.cpp:102-194) as
  * (component i, component j, image) triples in caller indices */

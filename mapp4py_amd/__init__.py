@@ -295,6 +295,12 @@ class AtomsDMD:
         check(lib().mapp_dmd_J(self._h, ptr(xv), ptr(out)))
         return out
 
+    def J_entries(self):
+        """(live list entries, second-list entries) of the last J state."""
+        a, b = C.c_int64(), C.c_int64()
+        check(lib().mapp_dmd_J_entries(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def set_batch(self, b):
         """List entries per batch of the quadrature passes (1, 2 or 4)."""
         check(lib().mapp_dmd_set_batch(self._h, int(b)))

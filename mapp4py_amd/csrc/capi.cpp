@@ -356,6 +356,16 @@ mapp_status mapp_dmd_J(mapp_dmd* d, const double* x, double* Ax)
     });
 }
 
+mapp_status mapp_dmd_J_entries(mapp_dmd* d, int64_t* live, int64_t* exchange)
+{
+    return guard([&] {
+        NEED(d);
+        NEED(live);
+        NEED(exchange);
+        d->sys->J_entries(live, exchange);
+    });
+}
+
 mapp_status mapp_dmd_value(mapp_dmd* d, int with_c, const double* ext_mu, double* fe)
 {
     return guard([&] { NEED(d); NEED(fe); *fe = d->sys->value(with_c != 0, ext_mu); });

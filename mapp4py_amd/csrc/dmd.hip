@@ -719,17 +719,27 @@ __global__ __launch_bounds__(kBlock) void k_dmd_cd(int nc, const double4* __rest
 
 // ---------------- Jacobian of the concentration rates ----------------
 
-// The state ForceFieldEAMDMD::__J reads (src/ff_eam_dmd.cpp:1475-1575):
-// rho_phi of every list slot -- the smeared r*phi, the density at j from i
-// and the density at i from j, zero past the cut -- stored slot-major like
-// the list, and ddE = F''(rho) of every component (src/ff_eam_dmd.cpp:1177).
+// Products with the Jacobian of the rates (ForceFieldEAMDMD::__J,
+// src/ff_eam_dmd.cpp:1475-1575) at a fixed state. The state kernels below
+// run once per state and keep what every product reads, so a product is
+// three gathers:
+//  * the live entries of every component's list (pairs inside the smeared
+//    cut; the rest have zero rho_phi), compacted slot-major: the partner,
+//    rho_{j->i}, rho_{i->j} and t0 = r phi + dE_i rho_{j->i} + dE_j rho_{i->j};
+//  * the second list (same element, r < r_crd) with the exchange
+//    coefficients of src/ff_eam_dmd.cpp:1430-1462 folded in: per entry
+//    f0 = coef0 M0, f1 = coef0 d_i c_i, f2 = coef0 d_j cv_i, and per component
+//    S0 = sum coef0 M0, S1 = sum coef0 d_i cv_j, S2 = sum coef0 d_j c_j, so that
+//    Ax_i = S1 x_i + S2 xs_i - S0 b0_i + sum_j (f0 b0_j - f1 xs_j - f2 x_j)
+//    with xs the sum of x over a site's present components.
+
+// quadrature values of every live list entry, rho and F''(rho)
 template <int B>
 __global__ __launch_bounds__(kBlock) void k_dmd_jrp(int n, const double4* __restrict__ x, const double* __restrict__ al,
                                                     const double* __restrict__ cc, const int* __restrict__ el,
                                                     const int* __restrict__ nn, const uint32_t* __restrict__ list,
                                                     int ldl, int maxn, DmdArgs A, BoxDev bd,
-                                                    double* __restrict__ rp0, double* __restrict__ rp1,
-                                                    double* __restrict__ rp2, double* __restrict__ ddE)
+                                                    const double* __restrict__ dE, JState J)
 {
     const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
@@ -739,12 +749,14 @@ __global__ __launch_bounds__(kBlock) void k_dmd_jrp(int n, const double4* __rest
         const int ci_idx = i * C + ic;
         const double ci = cc[ci_idx];
         if (ci < 0.0) {
-            ddE[ci_idx] = 0.0;
+            J.ddE[ci_idx] = 0.0;
+            J.cnt[ci_idx] = 0;
             continue;
         }
-        const double ai = al[ci_idx];
+        const double ai = al[ci_idx], dEi = dE[ci_idx];
         const int ei = el[ci_idx];
         double rho = 0.0;
+        int live_n = 0;
         const int cnt = nn[ci_idx];
         for (int k0 = 0; k0 < cnt; k0 += B) {
             uint32_t e[B];
@@ -791,19 +803,23 @@ __global__ __launch_bounds__(kBlock) void k_dmd_jrp(int n, const double4* __rest
             }
 #pragma unroll
             for (int b = 0; b < B; ++b) {
-                if (k0 + b >= cnt) continue;
+                if (!live[b]) continue;
+                const int jc = (int)(e[b] & kIndexMask);
                 const double rinv = 1.0 / pg[b].r;
-                const double v0 = live[b] ? kPiInSq * rinv * H[b][0] : 0.0;
-                const double v1 = live[b] ? kPiInSq * rinv * H[b][1] : 0.0;
-                const double v2 = live[b] ? kPiInSq * rinv * H[b][2] : 0.0;
-                const size_t slot = (size_t)(k0 + b) * ldl + ci_idx;
-                rp0[slot] = v0;
-                rp1[slot] = v1;
-                rp2[slot] = v2;
+                const double v0 = kPiInSq * rinv * H[b][0];
+                const double v1 = kPiInSq * rinv * H[b][1];
+                const double v2 = kPiInSq * rinv * H[b][2];
+                const size_t slot = (size_t)live_n * ldl + ci_idx;
+                J.idx[slot] = (uint32_t)jc;
+                J.r2[slot] = v2;
+                J.r1[slot] = v1;
+                J.t0[slot] = v0 + dEi * v2 + dE[jc] * v1;
+                ++live_n;
                 rho += cj[b] * v2;
             }
         }
-        // F''(rho) on the clamped interval of femb5 (no second derivative past rho_max)
+        J.cnt[ci_idx] = live_n;
+        // F''(rho) on the interval femb5 uses (no second derivative past rho_max)
         double p = rho * A.drho_inv;
         int m = (int)p;
         m = m < A.nrho - 2 ? m : A.nrho - 2;
@@ -811,84 +827,33 @@ __global__ __launch_bounds__(kBlock) void k_dmd_jrp(int n, const double4* __rest
         p -= (double)m;
         p = p < 1.0 ? p : 1.0;
         const double* cf = A.F5 + ((size_t)ei * A.nrho + m) * 5;
-        ddE[ci_idx] = ((12.0 * cf[4] * p + 6.0 * cf[3]) * p + 2.0 * cf[2]) * A.drho_inv * A.drho_inv;
+        J.ddE[ci_idx] = ((12.0 * cf[4] * p + 6.0 * cf[3]) * p + 2.0 * cf[2]) * A.drho_inv * A.drho_inv;
     }
 }
 
-// __J, first sum: A1_i = c_i F''(rho_i) sum_j rho_{j->i} x_j, the change of
-// F'(rho_i) times c_i. Every component sums its own list (the reference's
-// half list adds the j side of each pair to A1_j in the same loop).
-__global__ __launch_bounds__(kBlock) void k_dmd_ja1(int nc, const double* __restrict__ cc, const int* __restrict__ nn,
-                                                    const uint32_t* __restrict__ list, int ldl,
-                                                    const double* __restrict__ rp2, const double* __restrict__ ddE,
-                                                    const double* __restrict__ xv, double* __restrict__ a1)
-{
-    const int ic = xcd_block() * kBlock + threadIdx.x;
-    if (ic >= nc) return;
-    const double ci = cc[ic];
-    double s = 0.0;
-    if (ci >= 0.0) {
-        const int cnt = nn[ic];
-        for (int k = 0; k < cnt; ++k) {
-            const size_t slot = (size_t)k * ldl + ic;
-            const int jc = (int)(list[slot] & kIndexMask);
-            s += rp2[slot] * xv[jc];
-        }
-        s *= ci * ddE[ic];
-    }
-    a1[ic] = s;
-}
-
-// __J, second sum: b0_i = sum_j rho_{i->j} A1_j + (r phi + dE_i rho_{j->i}
-// + dE_j rho_{i->j}) x_j, the change of the chemical potential mu_i
-__global__ __launch_bounds__(kBlock) void k_dmd_jb0(int nc, const double* __restrict__ cc, const int* __restrict__ nn,
-                                                    const uint32_t* __restrict__ list, int ldl,
-                                                    const double* __restrict__ rp0, const double* __restrict__ rp1,
-                                                    const double* __restrict__ rp2, const double* __restrict__ dE,
-                                                    const double* __restrict__ a1, const double* __restrict__ xv,
-                                                    double* __restrict__ b0)
-{
-    const int ic = xcd_block() * kBlock + threadIdx.x;
-    if (ic >= nc) return;
-    double s = 0.0;
-    if (cc[ic] >= 0.0) {
-        const double dEi = dE[ic];
-        const int cnt = nn[ic];
-        for (int k = 0; k < cnt; ++k) {
-            const size_t slot = (size_t)k * ldl + ic;
-            const int jc = (int)(list[slot] & kIndexMask);
-            const double r1 = rp1[slot];
-            s += r1 * a1[jc] + (rp0[slot] + dEi * rp2[slot] + dE[jc] * r1) * xv[jc];
-        }
-    }
-    b0[ic] = s;
-}
-
-// __J, the product over the second list (src/ff_eam_dmd.cpp:1536-1572 with
-// the M_IJ coefficients of src/ff_eam_dmd.cpp:1430-1462): each component sums
-// its own exchanges; seen from j the term is minus this one (coef0 and M0 are
-// symmetric in i and j, M1 and M2 trade places).
-__global__ __launch_bounds__(kBlock) void k_dmd_jax(int nc, const double4* __restrict__ x, const double* __restrict__ al,
+// the second list with the exchange coefficients of the state. Each
+// component keeps its own exchanges; seen from j a term is minus this one
+// (coef0 and M0 are symmetric in i and j, d_i and d_j trade places).
+__global__ __launch_bounds__(kBlock) void k_dmd_jex(int nc, const double4* __restrict__ x, const double* __restrict__ al,
                                                     const double* __restrict__ cc, const int* __restrict__ el,
                                                     const int* __restrict__ nn, const uint32_t* __restrict__ list,
                                                     int ldl, int C, BoxDev bd, CdArgs P,
-                                                    const double* __restrict__ mu, const double* __restrict__ b0,
-                                                    const double* __restrict__ xv, double* __restrict__ ax)
+                                                    const double* __restrict__ mu, JState J)
 {
 #pragma clang fp contract(off)
     const int ic = xcd_block() * kBlock + threadIdx.x;
     if (ic >= nc) return;
-    double acc = 0.0;
     const double ci = cc[ic];
+    double S0 = 0.0, S1 = 0.0, S2 = 0.0;
+    int m = 0;
     if (ci >= 0.0) {
         const int site = ic / C, ei = el[ic];
         const double4 xi = x[site];
-        const double alpha_i = al[ic], alpha_sq_i = alpha_i * alpha_i, mu_i = mu[ic], b0_i = b0[ic], x_i = xv[ic];
-        double cv_i = 1.0, xs_i = 0.0;
+        const double alpha_i = al[ic], alpha_sq_i = alpha_i * alpha_i, mu_i = mu[ic];
+        double cv_i = 1.0;
         for (int k = 0; k < C; ++k) {
             const double v = cc[site * C + k];
             if (v > 0.0) cv_i -= v;
-            if (v >= 0.0) xs_i += xv[site * C + k];
         }
         const double zeta = P.zeta[ei], c1 = P.c1[ei], rsq_crd = P.rsq_crd[ei];
         const int cnt = nn[ic];
@@ -908,23 +873,91 @@ __global__ __launch_bounds__(kBlock) void k_dmd_jax(int nc, const double4* __res
             const double coef0 = -c1 * rsq * gamma_ij_inv * exp(-Q + 0.5 * mu_ji);
             const double theta_j = theta_i + 1.0;
             const int sj = jc / C;
-            double cv_j = 1.0, xs_j = 0.0;
+            double cv_j = 1.0;
             for (int q = 0; q < C; ++q) {
                 const double v = cc[sj * C + q];
                 if (v > 0.0) cv_j -= v;
-                if (v >= 0.0) xs_j += xv[sj * C + q];
             }
             const double cj = cc[jc];
             const double d_i = exp(-0.5 * mu_ji) / (alpha_sq_i * alpha_i);
             const double d_j = exp(0.5 * mu_ji) / (alpha_sq_j * alpha_j);
             const double M0 = P.beta * (theta_i * ci * cv_j * d_i - theta_j * cj * cv_i * d_j);
-            double t = M0 * (b0[jc] - b0_i);
-            t += d_i * (cv_j * x_i - ci * xs_j);
-            t -= d_j * (cv_i * xv[jc] - cj * xs_i);
-            acc += coef0 * t;
+            const size_t slot = (size_t)m * ldl + ic;
+            J.idx2[slot] = (uint32_t)jc;
+            J.f0[slot] = coef0 * M0;
+            J.f1[slot] = coef0 * d_i * ci;
+            J.f2[slot] = coef0 * d_j * cv_i;
+            S0 += coef0 * M0;
+            S1 += coef0 * d_i * cv_j;
+            S2 += coef0 * d_j * cj;
+            ++m;
         }
     }
-    ax[ic] = acc;
+    J.cnt2[ic] = m;
+    J.S[(size_t)ic * 3] = S0;
+    J.S[(size_t)ic * 3 + 1] = S1;
+    J.S[(size_t)ic * 3 + 2] = S2;
+}
+
+// product, first sum: A1_i = c_i F''(rho_i) sum_j rho_{j->i} x_j (the change
+// of F'(rho_i), times c_i); every component sums its own list (the
+// reference's half list adds the j side of each pair in the same loop). The
+// thread of a site's first component also writes xs of the site.
+__global__ __launch_bounds__(kBlock) void k_dmd_ja1(int nc, int C, const double* __restrict__ cc, int ldl, JState J,
+                                                    const double* __restrict__ xv, double* __restrict__ a1,
+                                                    double* __restrict__ xs)
+{
+    const int ic = xcd_block() * kBlock + threadIdx.x;
+    if (ic >= nc) return;
+    const double ci = cc[ic];
+    double s = 0.0;
+    const int cnt = J.cnt[ic];
+    for (int k = 0; k < cnt; ++k) {
+        const size_t slot = (size_t)k * ldl + ic;
+        s += J.r2[slot] * xv[J.idx[slot]];
+    }
+    a1[ic] = ci >= 0.0 ? s * (ci * J.ddE[ic]) : 0.0;
+    if (ic % C == 0) {
+        double t = 0.0;
+        for (int q = 0; q < C; ++q)
+            if (cc[ic + q] >= 0.0) t += xv[ic + q];
+        xs[ic / C] = t;
+    }
+}
+
+// product, second sum: b0_i = sum_j rho_{i->j} A1_j + t0 x_j, the change of
+// the chemical potential mu_i
+__global__ __launch_bounds__(kBlock) void k_dmd_jb0(int nc, int ldl, JState J, const double* __restrict__ a1,
+                                                    const double* __restrict__ xv, double* __restrict__ b0)
+{
+    const int ic = xcd_block() * kBlock + threadIdx.x;
+    if (ic >= nc) return;
+    double s = 0.0;
+    const int cnt = J.cnt[ic];
+    for (int k = 0; k < cnt; ++k) {
+        const size_t slot = (size_t)k * ldl + ic;
+        const uint32_t jc = J.idx[slot];
+        s += J.r1[slot] * a1[jc] + J.t0[slot] * xv[jc];
+    }
+    b0[ic] = s;
+}
+
+// product over the second list (src/ff_eam_dmd.cpp:1536-1572)
+__global__ __launch_bounds__(kBlock) void k_dmd_jax(int nc, int C, int ldl, JState J, const double* __restrict__ b0,
+                                                    const double* __restrict__ xv, const double* __restrict__ xs,
+                                                    double* __restrict__ ax)
+{
+    const int ic = xcd_block() * kBlock + threadIdx.x;
+    if (ic >= nc) return;
+    double acc = 0.0;
+    const int cnt = J.cnt2[ic];
+    for (int k = 0; k < cnt; ++k) {
+        const size_t slot = (size_t)k * ldl + ic;
+        const uint32_t jc = J.idx2[slot];
+        acc += J.f0[slot] * b0[jc] - J.f1[slot] * xs[jc / C] - J.f2[slot] * xv[jc];
+    }
+    const double* S = J.S + (size_t)ic * 3;
+    ax[ic] = (S[1] * xv[ic] + S[2] * xs[ic / C] - S[0] * b0[ic]) + acc;
 }
 
 // chemical potential of new_derivative<true> (src/ff_dmd.cpp:305-366) and
@@ -1675,24 +1708,38 @@ double DMDSystem::c_d(const double* r_crd, const double* zeta, double* c_d_host)
     return norm;
 }
 
+JState DMDSystem::jstate()
+{
+    return JState{jidx_.get(), jr2_.get(), jr1_.get(), jt0_.get(), jcnt_.get(), jddE_.get(),
+                  jidx2_.get(), jf0_.get(), jf1_.get(), jf2_.get(), jS_.get(), jcnt2_.get()};
+}
+
 void DMDSystem::J(const double* x_host, double* ax_host)
 {
     MAPP_REQUIRE(jver_ >= 0 && jver_ == ver_, ERR_STATE,
                  "J needs c_d at the current state (quadrature, no surrogate) first");
-    BoxDev bd;
-    std::memcpy(bd.h, box_.h, sizeof bd.h);
-    std::memcpy(bd.b, box_.b, sizeof bd.b);
-    const int nb = grid_for(n_);
     if (jrp_ver_ != ver_) {
+        BoxDev bd;
+        std::memcpy(bd.h, box_.h, sizeof bd.h);
+        std::memcpy(bd.b, box_.b, sizeof bd.b);
         const size_t slots = (size_t)std::max(maxn_, 1) * ldl_;
-        jrp0_.reserve(slots);
-        jrp1_.reserve(slots);
-        jrp2_.reserve(slots);
+        jidx_.reserve(slots);
+        jr2_.reserve(slots);
+        jr1_.reserve(slots);
+        jt0_.reserve(slots);
+        jidx2_.reserve(slots);
+        jf0_.reserve(slots);
+        jf1_.reserve(slots);
+        jf2_.reserve(slots);
+        jcnt_.reserve(nc_);
+        jcnt2_.reserve(nc_);
         jddE_.reserve(nc_);
+        jS_.reserve((size_t)3 * nc_);
         prof_.begin(K_DMD_JSTATE, stream_);
-        hipLaunchKernelGGL(k_dmd_jrp<4>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
-                           el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, jrp0_.get(), jrp1_.get(),
-                           jrp2_.get(), jddE_.get());
+        hipLaunchKernelGGL(k_dmd_jrp<4>, dim3(grid_for(n_)), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(),
+                           c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, jdE_.get(), jstate());
+        hipLaunchKernelGGL(k_dmd_jex, dim3(grid_for(nc_)), dim3(kBlock), 0, stream_, nc_, x_.get(), al_.get(),
+                           c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, C_, bd, jP_, jmu_.get(), jstate());
         prof_.end(stream_);
         jrp_ver_ = ver_;
     }
@@ -1700,6 +1747,7 @@ void DMDSystem::J(const double* x_host, double* ax_host)
     ja1_.reserve(nc_);
     jb0_.reserve(nc_);
     jax_.reserve(nc_);
+    jxs_.reserve(n_);
     DevBuf<double> th;
     th.reserve(nc_);
     MAPP_HIP(hipMemcpyAsync(th.get(), x_host, sizeof(double) * nc_, hipMemcpyHostToDevice, stream_));
@@ -1713,19 +1761,31 @@ void DMDSystem::J(const double* x_host, double* ax_host)
     prof_.flush();
 }
 
+void DMDSystem::J_entries(int64_t* live, int64_t* exchange)
+{
+    MAPP_REQUIRE(jrp_ver_ >= 0 && jrp_ver_ == ver_, ERR_STATE, "J_entries needs a J call at the current state");
+    std::vector<int> a(nc_), b(nc_);
+    MAPP_HIP(hipMemcpyAsync(a.data(), jcnt_.get(), sizeof(int) * nc_, hipMemcpyDeviceToHost, stream_));
+    MAPP_HIP(hipMemcpyAsync(b.data(), jcnt2_.get(), sizeof(int) * nc_, hipMemcpyDeviceToHost, stream_));
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    *live = *exchange = 0;
+    for (int i = 0; i < nc_; ++i) {
+        *live += a[i];
+        *exchange += b[i];
+    }
+}
+
 void DMDSystem::J_dev()
 {
-    BoxDev bd;
-    std::memcpy(bd.h, box_.h, sizeof bd.h);
-    std::memcpy(bd.b, box_.b, sizeof bd.b);
     const int nbc = grid_for(nc_);
+    const JState js = jstate();
     prof_.begin(K_DMD_J, stream_);
-    hipLaunchKernelGGL(k_dmd_ja1, dim3(nbc), dim3(kBlock), 0, stream_, nc_, c_.get(), nn_.get(), list_.get(), ldl_,
-                       jrp2_.get(), jddE_.get(), jx_.get(), ja1_.get());
-    hipLaunchKernelGGL(k_dmd_jb0, dim3(nbc), dim3(kBlock), 0, stream_, nc_, c_.get(), nn_.get(), list_.get(), ldl_,
-                       jrp0_.get(), jrp1_.get(), jrp2_.get(), jdE_.get(), ja1_.get(), jx_.get(), jb0_.get());
-    hipLaunchKernelGGL(k_dmd_jax, dim3(nbc), dim3(kBlock), 0, stream_, nc_, x_.get(), al_.get(), c_.get(), el_.get(),
-                       nn_.get(), list_.get(), ldl_, C_, bd, jP_, jmu_.get(), jb0_.get(), jx_.get(), jax_.get());
+    hipLaunchKernelGGL(k_dmd_ja1, dim3(nbc), dim3(kBlock), 0, stream_, nc_, C_, c_.get(), ldl_, js, jx_.get(),
+                       ja1_.get(), jxs_.get());
+    hipLaunchKernelGGL(k_dmd_jb0, dim3(nbc), dim3(kBlock), 0, stream_, nc_, ldl_, js, ja1_.get(), jx_.get(),
+                       jb0_.get());
+    hipLaunchKernelGGL(k_dmd_jax, dim3(nbc), dim3(kBlock), 0, stream_, nc_, C_, ldl_, js, jb0_.get(), jx_.get(),
+                       jxs_.get(), jax_.get());
     prof_.end(stream_);
 }
 

@@ -45,6 +45,20 @@ struct CGReport {
     double fe0, fe, grad_norm, grad_max;
 };
 
+// Arrays of the Jacobian-product state, as the kernels take them
+struct JState {
+    uint32_t* idx;   // [maxn][ldl] partner component of a live entry
+    double* r2;      // rho_{j->i}
+    double* r1;      // rho_{i->j}
+    double* t0;      // r phi + dE_i rho_{j->i} + dE_j rho_{i->j}
+    int* cnt;        // [nc] live entries
+    double* ddE;     // [nc] F''(rho)
+    uint32_t* idx2;  // [maxn][ldl] second-list partner
+    double *f0, *f1, *f2;  // exchange coefficients of each second-list entry
+    double* S;       // [nc][3] their sums S0, S1, S2 per component
+    int* cnt2;       // [nc]
+};
+
 class DMDSystem {
 public:
     DMDSystem(std::shared_ptr<EamForceField> ff, int nsites, int c_dim, int nquad, const double* H, const double* x,
@@ -72,6 +86,8 @@ public:
     // parameters of the last c_d call (an error if the state changed since);
     // x and Ax in caller order [site][c_dim]
     void J(const double* x, double* ax);
+    // entries the last J state holds: live list entries and second-list entries
+    void J_entries(int64_t* live, int64_t* exchange);
     int64_t half_pairs_count();
     void half_pairs(int32_t* i, int32_t* j, int8_t* img);
     // MinCGDMD over x and alpha (c fixed)
@@ -97,6 +113,7 @@ private:
     bool decide();
     void reduce_rows(const double* part, int nrows, int ncols, double* out);
     void J_dev();
+    JState jstate();
 
     std::shared_ptr<EamForceField> ff_;
     int n_, C_, nc_;  // sites, components per site, total components
@@ -117,8 +134,11 @@ private:
     // Jacobian state: bumped by every change of x, alpha, c or the slot order
     int64_t ver_ = 0, jver_ = -1, jrp_ver_ = -1;
     CdArgs jP_{};
-    DevBuf<double> jmu_, jdE_, jddE_, jrp0_, jrp1_, jrp2_;  // rho_phi of every list slot
-    DevBuf<double> jx_, ja1_, jb0_, jax_;
+    DevBuf<double> jmu_, jdE_, jddE_;
+    DevBuf<uint32_t> jidx_, jidx2_;   // live list entries / second list, slot-major
+    DevBuf<double> jr2_, jr1_, jt0_, jf0_, jf1_, jf2_, jS_;
+    DevBuf<int> jcnt_, jcnt2_;
+    DevBuf<double> jx_, ja1_, jb0_, jax_, jxs_;
     std::vector<double> masses_;
     DevBuf<uint32_t> keys_, keys_alt_;
     DevBuf<int> vals_, vals_alt_;

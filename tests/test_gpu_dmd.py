@@ -185,6 +185,8 @@ def test_jacobian_product_matches_oracle(M, cuh, cells, tilt, seed):
     assert np.abs(Jvw - (2.0 * Jv - 3.0 * Jw)).max() < 1e-12 * np.abs(Jvw).max()
     for e in (0, 1):  # exchange conserves each element, also to first order
         assert abs(Jv[elem == e].sum()) < 1e-11 * np.abs(Jv[elem == e]).sum()
+    live, exch = g.J_entries()  # every pair from both sides
+    assert 0 < exch <= live <= 2 * len(g.half_pairs()[0]) and exch % 2 == 0
     # a new state needs a new c_d (here the same values written again)
     xs, as_, cs = g.state()
     g.set_state(xs, as_, cs)
