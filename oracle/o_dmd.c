@@ -643,3 +643,233 @@ void o_dmd_J(o_dmd* d, const double* r_crd, const double* zeta, const double* ma
     free(xa); free(A1); free(b0); free(xv);
     dmd_static_free(&s);
 }
+
+/* __GMRES (src/gmres.h:40-60, src/gmres.cpp:32-211): restarted-free GMRES with
+ * one classical Gram-Schmidt pass and Givens rotations, zero initial guess.
+ * The matrix enters as a callback y = A(x). */
+typedef void (*o_matvec)(void* ctx, const double* x, double* y);
+
+typedef struct {
+    int m, n;
+    double* Q;       /* [m+1][n] Krylov basis */
+    double* H;       /* packed columns: column i has i+1 entries at H + i(i+1)/2 */
+    double* b_hat;   /* [m+1] rotated right-hand side */
+    double* cs;      /* [m+1][2] rotations */
+    double* y;       /* [m+1] */
+} o_gmres;
+
+static void gmres_init(o_gmres* g, int m, int n)
+{
+    g->m = m;
+    g->n = n;
+    g->Q = (double*)calloc((size_t)(m + 1) * n + 1, sizeof(double));
+    g->H = (double*)calloc((size_t)m * (m + 1) / 2 + 1, sizeof(double));
+    g->b_hat = (double*)calloc(m + 2, sizeof(double));
+    g->cs = (double*)calloc(2 * (m + 2), sizeof(double));
+    g->y = (double*)calloc(m + 2, sizeof(double));
+}
+
+static void gmres_free(o_gmres* g)
+{
+    free(g->Q); free(g->H); free(g->b_hat); free(g->cs); free(g->y);
+}
+
+/* back substitution and x = sum y_k q_k (src/gmres.cpp:113-211); returns |y| */
+static double gmres_solve_y(o_gmres* g, int nvecs, double* x)
+{
+    for (int i = nvecs - 1; i > -1; i--) {
+        g->y[i] = g->b_hat[i];
+        for (int j = i + 1; j < nvecs; j++) g->y[i] -= g->H[(size_t)j * (j + 1) / 2 + i] * g->y[j];
+        g->y[i] /= g->H[(size_t)i * (i + 1) / 2 + i];
+    }
+    for (int i = 0; i < g->n; i++) {
+        x[i] = 0.0;
+        for (int k = 0; k < nvecs; k++) x[i] += g->y[k] * g->Q[(size_t)k * g->n + i];
+    }
+    double s = 0.0;
+    for (int k = 0; k < nvecs; k++) s += g->y[k] * g->y[k];
+    return sqrt(s);
+}
+
+/* GMRES::solve: x ~ A^-1 b within |residual| < tol in at most m products;
+ * b is overwritten (it holds the products). Returns 1 when the tolerance was
+ * met; *norm = |x|. */
+static int gmres_solve(o_gmres* g, o_matvec A, void* ctx, double* b, double tol, double* norm, double* x)
+{
+    int n = g->n, m = g->m;
+    /* calc(Ax, x): first basis vector (src/gmres.cpp:32-46) */
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += b[i] * b[i];
+    double bn = sqrt(s), inv = 1.0 / bn;
+    for (int i = 0; i < n; i++) x[i] = g->Q[i] = inv * b[i];
+    g->b_hat[0] = bn;
+    for (int it = 0; it < m; it++) {
+        A(ctx, x, b);
+        /* calc(iter, Ax, x) (src/gmres.cpp:50-112) */
+        double* h = g->H + (size_t)it * (it + 1) / 2;
+        int iv = it + 1;
+        for (int j = 0; j < iv; j++) h[j] = 0.0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < iv; j++) h[j] += b[i] * g->Q[(size_t)j * n + i];
+        double* qn = g->Q + (size_t)iv * n;
+        double ns = 0.0;
+        for (int i = 0; i < n; i++) {
+            qn[i] = b[i];
+            for (int j = 0; j < iv; j++) qn[i] -= h[j] * g->Q[(size_t)j * n + i];
+            ns += qn[i] * qn[i];
+        }
+        double nrm = sqrt(ns), ninv = 1.0 / nrm;
+        for (int i = 0; i < n; i++) x[i] = qn[i] *= ninv;
+        for (int i = 0; i < it; i++) {
+            double t = g->cs[2 * i] * h[i] - g->cs[2 * i + 1] * h[i + 1];
+            h[i + 1] = g->cs[2 * i + 1] * h[i] + g->cs[2 * i] * h[i + 1];
+            h[i] = t;
+        }
+        double t = sqrt(h[it] * h[it] + nrm * nrm);
+        g->cs[2 * it] = h[it] / t;
+        g->cs[2 * it + 1] = -nrm / t;
+        h[it] = g->cs[2 * it] * h[it] - g->cs[2 * it + 1] * nrm;
+        g->b_hat[it + 1] = g->cs[2 * it + 1] * g->b_hat[it];
+        g->b_hat[it] *= g->cs[2 * it];
+        if (fabs(g->b_hat[it + 1]) < tol) {
+            *norm = gmres_solve_y(g, it + 1, x);
+            return 1;
+        }
+    }
+    *norm = gmres_solve_y(g, m, x);
+    return 0;
+}
+
+typedef struct { const double* A; int n; } dense_ctx;
+
+static void dense_mv(void* v, const double* x, double* y)
+{
+    const dense_ctx* c = (const dense_ctx*)v;
+    for (int i = 0; i < c->n; i++) {
+        double s = 0.0;
+        for (int j = 0; j < c->n; j++) s += c->A[(size_t)i * c->n + j] * x[j];
+        y[i] = s;
+    }
+}
+
+int o_gmres_dense(int n, int m, const double* A, const double* b, double tol, double* x, double* norm)
+{
+    o_gmres g;
+    gmres_init(&g, m, n);
+    double* bb = (double*)malloc(sizeof(double) * n + 1);
+    memcpy(bb, b, sizeof(double) * n);
+    dense_ctx c = {A, n};
+    int ok = gmres_solve(&g, dense_mv, &c, bb, tol, norm, x);
+    free(bb);
+    gmres_free(&g);
+    return ok;
+}
+
+typedef struct {
+    o_dmd* d;
+    const double *r_crd, *zeta, *masses;
+    double beta;
+} newton_ctx;
+
+/* the Jacobian of F = c / beta + a - c_d (src/dae_imp.cpp:163-172) */
+static void newton_mv(void* v, const double* x, double* y)
+{
+    newton_ctx* t = (newton_ctx*)v;
+    int n = t->d->n * t->d->c_dim;
+    o_dmd_J(t->d, t->r_crd, t->zeta, t->masses, x, y);
+    double bi = 1.0 / t->beta;
+    for (int i = 0; i < n; i++) y[i] = -y[i] + bi * x[i];
+}
+
+static void newton_set_c(o_dmd* d, const double* c)
+{
+    memcpy(d->c_lcl, c, sizeof(double) * d->n * d->c_dim);
+    dmd_sync_ghost_state(d);
+}
+
+/* DAEImplicit::newton (src/dae_imp.cpp:146-206) with update_c
+ * (src/dae_imp.cpp:89-138) for F(c) = c / beta + a - c_d(c) = 0 from the
+ * current c, GMRES of at most max_gmres products per step. Tolerances from
+ * a_tol sqrt(nc_dof) (src/dae.cpp:128; nc_dof = natms c_dim) and err_fac.
+ * On success c holds the solution; otherwise c is restored. Returns 1 on
+ * convergence; *iters = Newton steps counted as the reference counts them. */
+int o_dmd_newton(o_dmd* d, const double* r_crd, const double* zeta, const double* masses, double beta,
+                 const double* a, double a_tol, double err_fac, int max_newton, int max_gmres, int* iters)
+{
+    int n = d->n * d->c_dim;
+    double a_tol_sqrt = a_tol * sqrt((double)n);
+    double* c0 = (double*)malloc(sizeof(double) * n + 1);
+    double* c = (double*)malloc(sizeof(double) * n + 1);
+    double* F = (double*)malloc(sizeof(double) * n + 1);
+    double* cd = (double*)malloc(sizeof(double) * n + 1);
+    double* del = (double*)malloc(sizeof(double) * n + 1);
+    memcpy(c0, d->c_lcl, sizeof(double) * n);
+    memcpy(c, d->c_lcl, sizeof(double) * n);
+    o_gmres g;
+    gmres_init(&g, max_gmres, n);
+    newton_ctx ctx = {d, r_crd, zeta, masses, beta};
+    double beta_inv = 1.0 / beta;
+    /* F_norm */
+#define F_NORM()                                                   \
+    do {                                                           \
+        o_dmd_c_d(d, r_crd, zeta, masses, cd);                     \
+        for (int i_ = 0; i_ < n; i_++) F[i_] = beta_inv * c[i_] + a[i_] - cd[i_]; \
+    } while (0)
+    double res_tol = 0.005 * a_tol_sqrt / err_fac;
+    double denom = 0.1 * a_tol_sqrt / err_fac;
+    double norm = 1.0, delta = 0.0, delta_prev = 0.0, ratio = 1.0, R = 1.0;
+    int iter = 0, converge = 0, diverge = 0;
+    F_NORM();
+    while (iter < max_newton && !converge && !diverge) {
+        gmres_solve(&g, newton_mv, &ctx, F, res_tol, &norm, del);
+        /* update_c: largest r <= 1 keeping 0 <= c - r del <= 1 */
+        double r = 1.0;
+        for (int i = 0; i < n; i++) {
+            if (c[i] >= 0.0) {
+                double tmp = c[i] - r * del[i];
+                if (tmp > 1.0) {
+                    r = (c[i] - 1.0) / del[i];
+                    while (c[i] - r * del[i] > 1.0) r = nextafter(r, 0.0);
+                }
+                if (tmp < 0.0) {
+                    r = c[i] / del[i];
+                    while (c[i] - r * del[i] < 0.0) r = nextafter(r, 0.0);
+                }
+            } else
+                del[i] = 0.0;
+        }
+        for (int i = 0; i < n; i++) {
+            volatile double v = c[i] - r * del[i];
+            v += 1.0;
+            v -= 1.0;
+            c[i] = v;
+        }
+        ratio = r;
+        newton_set_c(d, c);
+        delta = fabs(ratio * norm / denom);
+        if (iter) R = fmax(0.3 * R, delta / delta_prev);
+        if (iter > 1 && R * delta < 1.0) {
+            F_NORM();
+            converge = 1;
+            continue;
+        }
+        if (iter > 1 && delta / delta_prev > 2.0) {
+            diverge = 1;
+            continue;
+        }
+        delta_prev = delta;
+        F_NORM();
+        iter++;
+    }
+#undef F_NORM
+    if (!converge) newton_set_c(d, c0);
+    *iters = iter;
+    gmres_free(&g);
+    free(c0); free(c); free(F); free(cd); free(del);
+    return converge;
+}
+
+void o_dmd_get_c(const o_dmd* d, double* c)
+{
+    memcpy(c, d->c_lcl, sizeof(double) * d->n * d->c_dim);
+}

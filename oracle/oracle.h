@@ -125,6 +125,15 @@ void o_dmd_calc_q(double gamma_i, double gamma_j, double mu_ji, double* Q, doubl
 double o_dmd_c_d(o_dmd*, const double* r_crd, const double* zeta, const double* masses, double* c_d);
 /* ForceFieldEAMDMD::__J (src/ff_eam_dmd.cpp:1475-1575): Ax = (d c_d / d c) x */
 void o_dmd_J(o_dmd*, const double* r_crd, const double* zeta, const double* masses, const double* x, double* Ax);
+/* the local concentrations [natms][c_dim] */
+void o_dmd_get_c(const o_dmd*, double* c);
+/* __GMRES::solve (src/gmres.h:40-60) on a dense row-major A [n][n]: returns 1
+ * when |residual| < tol within m products; *norm = |x| */
+int o_gmres_dense(int n, int m, const double* A, const double* b, double tol, double* x, double* norm);
+/* DAEImplicit::newton (src/dae_imp.cpp:146-206): solve c / beta + a - c_d(c) = 0
+ * from the current c; 1 on convergence (c holds the solution), else c restored */
+int o_dmd_newton(o_dmd*, const double* r_crd, const double* zeta, const double* masses, double beta,
+                 const double* a, double a_tol, double err_fac, int max_newton, int max_gmres, int* iters);
 
 /* ---------------- GCMC (src/sgcmc.cpp, src/ff_eam.cpp:323-589) ---------------- */
 /* One periodic cell under single-process SGCMC with EAM exchange energies.

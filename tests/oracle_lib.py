@@ -70,6 +70,10 @@ def L():
             "o_dmd_calc_q": (None, [C.c_double, C.c_double, C.c_double, dp, dp, dp]),
             "o_dmd_c_d": (C.c_double, [C.c_void_p, dp, dp, dp, dp]),
             "o_dmd_J": (None, [C.c_void_p, dp, dp, dp, dp, dp]),
+            "o_dmd_get_c": (None, [C.c_void_p, dp]),
+            "o_gmres_dense": (C.c_int, [C.c_int, C.c_int, dp, dp, C.c_double, dp, dp]),
+            "o_dmd_newton": (C.c_int, [C.c_void_p, dp, dp, dp, C.c_double, dp, C.c_double, C.c_double, C.c_int,
+                                       C.c_int, C.POINTER(C.c_int)]),
             "o_gcmc_create": (C.c_void_p, [C.c_void_p, C.c_int, dp, dp, u8p, C.c_int, C.c_double, C.c_double,
                                            C.c_double, C.c_double, dp, C.c_int, C.c_int]),
             "o_gcmc_free": (None, [C.c_void_p]),
@@ -253,6 +257,16 @@ def brute_half_pairs(H, x, elem, cut_sk, kmax=3):
     return out
 
 
+def gmres_dense(A, b, m, tol):
+    """__GMRES::solve on a dense matrix: (converged, x, |x|)."""
+    A = f64(A)
+    n = A.shape[0]
+    x = np.zeros(n)
+    nrm = C.c_double()
+    ok = L().o_gmres_dense(n, m, P(A), P(f64(b)), float(tol), P(x), C.byref(nrm))
+    return bool(ok), x, nrm.value
+
+
 def calc_q(gamma_i, gamma_j, mu_ji):
     """ForceFieldEAMDMD::calc_Q with theta: (Q, alpha_Q_sq, theta)."""
     q, a, t = C.c_double(), C.c_double(), C.c_double()
@@ -319,6 +333,19 @@ class ODMD:
         L().o_dmd_J(self.h, P(f64(r_crd)), P(f64(zeta)), P(f64(masses)), P(f64(np.asarray(xc).reshape(self.n, self.C))),
                     P(out))
         return out
+
+    def state_c(self):
+        out = np.zeros((self.n, self.C))
+        L().o_dmd_get_c(self.h, P(out))
+        return out
+
+    def newton(self, r_crd, zeta, masses, beta, a, a_tol=1e-8, err_fac=1.0, max_newton=5, max_gmres=5):
+        """DAEImplicit::newton from the current c: (converged, iterations, c)."""
+        it = C.c_int()
+        ok = L().o_dmd_newton(self.h, P(f64(r_crd)), P(f64(zeta)), P(f64(masses)), float(beta),
+                              P(f64(np.asarray(a).reshape(self.n, self.C))), float(a_tol), float(err_fac),
+                              int(max_newton), int(max_gmres), C.byref(it))
+        return bool(ok), it.value, self.state_c()
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -248,3 +248,52 @@ def test_jacobian_product_is_the_derivative_of_the_rates(cuh_small):
     Jw = d.J(R_CRD, ZETA, p.masses, w)
     Jvw = d.J(R_CRD, ZETA, p.masses, 2.0 * v - 3.0 * w)
     assert np.abs(Jvw - (2.0 * Jv - 3.0 * Jw)).max() < 1e-12 * np.abs(Jvw).max()
+
+
+def test_gmres_minimizes_the_residual_over_the_krylov_space():
+    # __GMRES::solve from a zero guess: after k products x is the minimiser
+    # of |b - A x| over span(b, Ab, ..., A^(k-1) b); with k = n it solves A x = b
+    rng = np.random.default_rng(1)
+    n = 10
+    A = 3.0 * np.eye(n) + rng.normal(0.0, 0.5, (n, n))
+    b = rng.normal(0.0, 1.0, n)
+    ok, x, nrm = O.gmres_dense(A, b, n, 1e-13)
+    assert ok and np.abs(x - np.linalg.solve(A, b)).max() < 1e-12
+    assert abs(nrm - np.linalg.norm(x)) < 1e-13  # |x| from the orthonormal basis
+    for m in (1, 2, 4, 6):
+        ok, x, _ = O.gmres_dense(A, b, m, 1e-30)
+        K = np.stack([np.linalg.matrix_power(A, k) @ b for k in range(m)], 1)
+        xr = K @ np.linalg.lstsq(A @ K, b, rcond=None)[0]
+        assert not ok and np.abs(x - xr).max() < 1e-11 * np.abs(xr).max(), m
+    # the tolerance stops it early, at the first k whose residual is below it
+    res = [np.linalg.norm(b - A @ O.gmres_dense(A, b, m, 1e-30)[1]) for m in range(1, n + 1)]
+    k = next(i for i, r in enumerate(res) if r < 0.05) + 1
+    ok, x, _ = O.gmres_dense(A, b, n, 0.05)
+    assert ok and np.abs(x - O.gmres_dense(A, b, k, 1e-30)[1]).max() < 1e-14
+
+
+@pytest.mark.parametrize("dt", [30.0, 300.0])
+def test_implicit_step_solves_backward_euler(cuh_small, dt):
+    # DAEImplicit::newton on c - c_n = dt c_d(c) (beta = dt, a = -c_n / dt):
+    # the result is a root of F to well inside a_tol, keeps 0 <= c <= 1 and
+    # conserves each element
+    p, ff, x, H, elem, alpha, c = cuh_small
+    d = make(ff, p, x, H, elem, alpha, c, nquad=3)
+    a_tol = 1e-8
+    ok, it, cn = d.newton(R_CRD, ZETA, p.masses, dt, -c / dt, a_tol=a_tol)
+    assert ok and it >= 2
+    assert np.abs(cn - c).max() > 1e-3
+    d.set_state(x, alpha, cn)
+    F = cn / dt - c / dt - d.c_d(R_CRD, ZETA, p.masses)[0]
+    assert np.linalg.norm(F) < 0.01 * a_tol * np.sqrt(c.size)
+    assert cn.min() >= 0.0 and cn.max() <= 1.0
+    for e in (0, 1):
+        assert abs((cn - c)[elem == e].sum()) < 1e-13
+
+
+def test_implicit_step_that_does_not_converge_restores_c(cuh_small):
+    p, ff, x, H, elem, alpha, c = cuh_small
+    d = make(ff, p, x, H, elem, alpha, c, nquad=3)
+    ok, it, cn = d.newton(R_CRD, ZETA, p.masses, 3000.0, -c / 3000.0, a_tol=1e-8)
+    assert not ok
+    np.testing.assert_array_equal(cn, c)
