@@ -164,6 +164,20 @@ mapp_status mapp_dmd_c_d(mapp_dmd* d, const double* r_crd, const double* C, doub
  * order. MAPP_ERR_STATE if x, alpha or c changed since that call, or if it
  * ran through the surrogate networks. */
 mapp_status mapp_dmd_J(mapp_dmd* d, const double* x, double* Ax);
+/* DAEImplicit::newton (src/dae_imp.cpp:146-206) with update_c
+ * (src/dae_imp.cpp:89-138) and __GMRES (src/gmres.h:40-60, src/gmres.cpp):
+ * solve c / beta + a - c_d(c) = 0 for c from the current c (the implicit
+ * equation of one BDF step: beta is the step's coefficient, a its history
+ * term, [natms][c_dim] in caller order), Newton steps of GMRES over
+ * mapp_dmd_J with at most max_ngmres_iters (1..8) products each. Tolerances
+ * from a_tol sqrt(natms c_dim) and err_fac as DAEImplicit sets them.
+ * *converged = 1: c holds the solution (read it with mapp_dmd_get_state);
+ * 0: c is restored to its value at the call. *iters = Newton steps as the
+ * reference counts them. An exactly zero residual gives no step (the
+ * reference divides by zero there). */
+mapp_status mapp_dmd_newton(mapp_dmd* d, const double* r_crd, const double* C, double beta, const double* a,
+                            double a_tol, double err_fac, int max_nnewton_iters, int max_ngmres_iters,
+                            int* converged, int* iters);
 /* Diagnostic (no counterpart in the reference): the entries the state of the
  * last mapp_dmd_J call holds, list entries inside the smeared cut (live) and
  * second-list entries (exchange), summed over components. */

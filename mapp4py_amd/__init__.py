@@ -295,6 +295,16 @@ class AtomsDMD:
         check(lib().mapp_dmd_J(self._h, ptr(xv), ptr(out)))
         return out
 
+    def newton(self, r_crd, zeta, beta, a, a_tol=1e-8, err_fac=1.0, max_nnewton_iters=5, max_ngmres_iters=5):
+        """DAEImplicit::newton: solve c / beta + a - c_d(c) = 0 from the current c.
+        Returns (converged, Newton iterations); c stays on the device (state())."""
+        av = f64(np.asarray(a).reshape(self.nsites, self.c_dim))
+        rc, zt = f64(r_crd), f64(zeta)
+        ok, it = C.c_int(), C.c_int()
+        check(lib().mapp_dmd_newton(self._h, ptr(rc), ptr(zt), float(beta), ptr(av), float(a_tol), float(err_fac),
+                                    int(max_nnewton_iters), int(max_ngmres_iters), C.byref(ok), C.byref(it)))
+        return bool(ok.value), it.value
+
     def J_entries(self):
         """(live list entries, second-list entries) of the last J state."""
         a, b = C.c_int64(), C.c_int64()

@@ -59,6 +59,9 @@ SIGNATURES = {
     "mapp_dmd_c_d": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                C.POINTER(C.c_double)]),
     "mapp_dmd_J": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "mapp_dmd_newton": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double,
+                                  C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int, C.c_int,
+                                  C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "mapp_dmd_J_entries": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mapp_dmd_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, dp, u8p, dp, dp, C.c_double,
                                   C.c_double, C.c_double, C.c_double, dp, vpp]),

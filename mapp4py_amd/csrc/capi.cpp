@@ -356,6 +356,20 @@ mapp_status mapp_dmd_J(mapp_dmd* d, const double* x, double* Ax)
     });
 }
 
+mapp_status mapp_dmd_newton(mapp_dmd* d, const double* r_crd, const double* C, double beta, const double* a,
+                            double a_tol, double err_fac, int max_nnewton_iters, int max_ngmres_iters,
+                            int* converged, int* iters)
+{
+    return guard([&] {
+        NEED(d);
+        NEED(r_crd);
+        NEED(C);
+        NEED(a);
+        const bool ok = d->sys->newton(r_crd, C, beta, a, a_tol, err_fac, max_nnewton_iters, max_ngmres_iters, iters);
+        if (converged) *converged = ok ? 1 : 0;
+    });
+}
+
 mapp_status mapp_dmd_J_entries(mapp_dmd* d, int64_t* live, int64_t* exchange)
 {
     return guard([&] {

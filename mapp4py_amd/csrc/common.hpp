@@ -85,6 +85,7 @@ enum KernelId : int {
     K_DMD_CD,
     K_DMD_JSTATE,
     K_DMD_J,
+    K_DMD_NEWTON,
     K_COUNT
 };
 
@@ -93,7 +94,8 @@ inline const char* kernel_name(int k)
     static const char* names[K_COUNT] = {"eam_rho",    "eam_force",  "build_list", "cell_sort",
                                          "integrate",  "eam_energy", "dmd_pass1",  "dmd_pass2",
                                          "dmd_energy", "surr_infer", "surr_train", "gcmc",
-                                         "dmd_cd",     "dmd_j_state", "dmd_j"};
+                                         "dmd_cd",     "dmd_j_state", "dmd_j",
+                                         "dmd_newton"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "";
 }
 

@@ -960,6 +960,133 @@ __global__ __launch_bounds__(kBlock) void k_dmd_jax(int nc, int C, int ldl, JSta
     ax[ic] = (S[1] * xv[ic] + S[2] * xs[ic / C] - S[0] * b0[ic]) + acc;
 }
 
+// ---------------- implicit concentration step ----------------
+
+// F = c / beta + a - c_d, the residual of DAEImplicit::newton
+// (src/dae_imp.cpp:148-161)
+__global__ __launch_bounds__(kBlock) void k_newton_f(int nc, double beta_inv, const double* __restrict__ c,
+                                                     const double* __restrict__ a, const double* __restrict__ cd,
+                                                     double* __restrict__ F)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < nc) F[i] = beta_inv * c[i] + a[i] - cd[i];
+}
+
+// y = -y + x / beta: the Jacobian of F applied to x, from y = J x
+// (src/dae_imp.cpp:163-172)
+__global__ __launch_bounds__(kBlock) void k_newton_jac(int nc, double beta_inv, const double* __restrict__ x,
+                                                       double* __restrict__ y)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < nc) y[i] = -y[i] + beta_inv * x[i];
+}
+
+// GMRES: partial sums of v . Q_j for j < nv (Q_j at Q + j * ldq)
+__global__ __launch_bounds__(kBlock) void k_gm_dots(int nc, int nv, const double* __restrict__ v,
+                                                    const double* __restrict__ Q, size_t ldq,
+                                                    double* __restrict__ part)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double acc[kPartW];
+#pragma unroll
+    for (int k = 0; k < kPartW; ++k) acc[k] = 0.0;
+    if (i < nc) {
+        const double vi = v[i];
+#pragma unroll
+        for (int k = 0; k < kPartW; ++k)
+            if (k < nv) acc[k] = vi * Q[(size_t)k * ldq + i];
+    }
+    block_sum_store<kPartW>(acc, part + (size_t)blockIdx.x * kPartW);
+}
+
+// GMRES, one classical Gram-Schmidt pass (src/gmres.cpp:66-76):
+// qn = v - sum_j h_j Q_j with h on the device; partial sums of qn^2
+__global__ __launch_bounds__(kBlock) void k_gm_orth(int nc, int nv, const double* __restrict__ v,
+                                                    const double* __restrict__ Q, size_t ldq,
+                                                    const double* __restrict__ h, double* __restrict__ qn,
+                                                    double* __restrict__ part)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double acc[1] = {0.0};
+    if (i < nc) {
+        double q = v[i];
+        for (int k = 0; k < nv; ++k) q -= h[k] * Q[(size_t)k * ldq + i];
+        qn[i] = q;
+        acc[0] = q * q;
+    }
+    block_sum_store<1>(acc, part + (size_t)blockIdx.x * kPartW);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gm_scale(int nc, double s, const double* __restrict__ in,
+                                                     double* __restrict__ out)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < nc) out[i] = s * in[i];
+}
+
+struct GmY {
+    double y[kPartW];
+};
+
+// x = sum_k y_k Q_k (src/gmres.cpp:183-189)
+__global__ __launch_bounds__(kBlock) void k_gm_combine(int nc, int nv, GmY y, const double* __restrict__ Q,
+                                                       size_t ldq, double* __restrict__ x)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nc) return;
+    double s = 0.0;
+    for (int k = 0; k < nv; ++k) s += y.y[k] * Q[(size_t)k * ldq + i];
+    x[i] = s;
+}
+
+// update_c (src/dae_imp.cpp:89-117): the largest step fraction r <= 1 that
+// keeps every present component in [0, 1], per component (the nextafter
+// walk makes the bound hold in FP64), minimum per workgroup; absent
+// components get no step
+__global__ __launch_bounds__(kBlock) void k_newton_r(int nc, const double* __restrict__ c, double* __restrict__ del,
+                                                     double* __restrict__ part)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[kBlock];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double r = 1.0;
+    if (i < nc) {
+        const double ci = c[i];
+        if (ci >= 0.0) {
+            const double di = del[i];
+            const double tmp = ci - di;
+            if (tmp > 1.0) {
+                r = (ci - 1.0) / di;
+                while (ci - r * di > 1.0) r = nextafter(r, 0.0);
+            }
+            if (tmp < 0.0) {
+                r = ci / di;
+                while (ci - r * di < 0.0) r = nextafter(r, 0.0);
+            }
+        } else {
+            del[i] = 0.0;
+        }
+    }
+    sh[threadIdx.x] = r;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)blockIdx.x * kPartW] = sh[0];
+}
+
+// c = (c - r del) rounded through 1 + (.) - 1, as src/dae_imp.cpp:119-124
+__global__ __launch_bounds__(kBlock) void k_newton_apply(int nc, double r, const double* __restrict__ del,
+                                                         double* __restrict__ c)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nc) return;
+    const double v = c[i] - r * del[i];  // no contraction: one rounding for r del, one for the difference
+    c[i] = (v + 1.0) - 1.0;              // strict IEEE: not folded to v
+}
+
 // chemical potential of new_derivative<true> (src/ff_dmd.cpp:305-366) and
 // the sum of c * mu_ext
 __global__ __launch_bounds__(kBlock) void k_dmd_chem(int n, int C, const double* __restrict__ al,
@@ -1714,46 +1841,50 @@ JState DMDSystem::jstate()
                   jidx2_.get(), jf0_.get(), jf1_.get(), jf2_.get(), jS_.get(), jcnt2_.get()};
 }
 
-void DMDSystem::J(const double* x_host, double* ax_host)
+void DMDSystem::J_prepare()
 {
     MAPP_REQUIRE(jver_ >= 0 && jver_ == ver_, ERR_STATE,
                  "J needs c_d at the current state (quadrature, no surrogate) first");
-    if (jrp_ver_ != ver_) {
-        BoxDev bd;
-        std::memcpy(bd.h, box_.h, sizeof bd.h);
-        std::memcpy(bd.b, box_.b, sizeof bd.b);
-        const size_t slots = (size_t)std::max(maxn_, 1) * ldl_;
-        jidx_.reserve(slots);
-        jr2_.reserve(slots);
-        jr1_.reserve(slots);
-        jt0_.reserve(slots);
-        jidx2_.reserve(slots);
-        jf0_.reserve(slots);
-        jf1_.reserve(slots);
-        jf2_.reserve(slots);
-        jcnt_.reserve(nc_);
-        jcnt2_.reserve(nc_);
-        jddE_.reserve(nc_);
-        jS_.reserve((size_t)3 * nc_);
-        prof_.begin(K_DMD_JSTATE, stream_);
-        hipLaunchKernelGGL(k_dmd_jrp<4>, dim3(grid_for(n_)), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(),
-                           c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, jdE_.get(), jstate());
-        hipLaunchKernelGGL(k_dmd_jex, dim3(grid_for(nc_)), dim3(kBlock), 0, stream_, nc_, x_.get(), al_.get(),
-                           c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, C_, bd, jP_, jmu_.get(), jstate());
-        prof_.end(stream_);
-        jrp_ver_ = ver_;
-    }
-    jx_.reserve(nc_);
+    if (jrp_ver_ == ver_) return;
+    BoxDev bd;
+    std::memcpy(bd.h, box_.h, sizeof bd.h);
+    std::memcpy(bd.b, box_.b, sizeof bd.b);
+    const size_t slots = (size_t)std::max(maxn_, 1) * ldl_;
+    jidx_.reserve(slots);
+    jr2_.reserve(slots);
+    jr1_.reserve(slots);
+    jt0_.reserve(slots);
+    jidx2_.reserve(slots);
+    jf0_.reserve(slots);
+    jf1_.reserve(slots);
+    jf2_.reserve(slots);
+    jcnt_.reserve(nc_);
+    jcnt2_.reserve(nc_);
+    jddE_.reserve(nc_);
+    jS_.reserve((size_t)3 * nc_);
     ja1_.reserve(nc_);
     jb0_.reserve(nc_);
-    jax_.reserve(nc_);
     jxs_.reserve(n_);
+    prof_.begin(K_DMD_JSTATE, stream_);
+    hipLaunchKernelGGL(k_dmd_jrp<4>, dim3(grid_for(n_)), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
+                       el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, jdE_.get(), jstate());
+    hipLaunchKernelGGL(k_dmd_jex, dim3(grid_for(nc_)), dim3(kBlock), 0, stream_, nc_, x_.get(), al_.get(), c_.get(),
+                       el_.get(), nn_.get(), list_.get(), ldl_, C_, bd, jP_, jmu_.get(), jstate());
+    prof_.end(stream_);
+    jrp_ver_ = ver_;
+}
+
+void DMDSystem::J(const double* x_host, double* ax_host)
+{
+    J_prepare();
+    jx_.reserve(nc_);
+    jax_.reserve(nc_);
     DevBuf<double> th;
     th.reserve(nc_);
     MAPP_HIP(hipMemcpyAsync(th.get(), x_host, sizeof(double) * nc_, hipMemcpyHostToDevice, stream_));
     hipLaunchKernelGGL(k_dscatter_comp, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n_, C_,
                        slot_of_.get(), th.get(), jx_.get());
-    J_dev();
+    J_dev(jx_.get(), jax_.get());
     hipLaunchKernelGGL(k_dgather_comp, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n_, C_,
                        slot_of_.get(), jax_.get(), th.get());
     MAPP_HIP(hipMemcpyAsync(ax_host, th.get(), sizeof(double) * nc_, hipMemcpyDeviceToHost, stream_));
@@ -1775,18 +1906,170 @@ void DMDSystem::J_entries(int64_t* live, int64_t* exchange)
     }
 }
 
-void DMDSystem::J_dev()
+void DMDSystem::J_dev(const double* x, double* ax)
 {
     const int nbc = grid_for(nc_);
     const JState js = jstate();
     prof_.begin(K_DMD_J, stream_);
-    hipLaunchKernelGGL(k_dmd_ja1, dim3(nbc), dim3(kBlock), 0, stream_, nc_, C_, c_.get(), ldl_, js, jx_.get(),
-                       ja1_.get(), jxs_.get());
-    hipLaunchKernelGGL(k_dmd_jb0, dim3(nbc), dim3(kBlock), 0, stream_, nc_, ldl_, js, ja1_.get(), jx_.get(),
-                       jb0_.get());
-    hipLaunchKernelGGL(k_dmd_jax, dim3(nbc), dim3(kBlock), 0, stream_, nc_, C_, ldl_, js, jb0_.get(), jx_.get(),
-                       jxs_.get(), jax_.get());
+    hipLaunchKernelGGL(k_dmd_ja1, dim3(nbc), dim3(kBlock), 0, stream_, nc_, C_, c_.get(), ldl_, js, x, ja1_.get(),
+                       jxs_.get());
+    hipLaunchKernelGGL(k_dmd_jb0, dim3(nbc), dim3(kBlock), 0, stream_, nc_, ldl_, js, ja1_.get(), x, jb0_.get());
+    hipLaunchKernelGGL(k_dmd_jax, dim3(nbc), dim3(kBlock), 0, stream_, nc_, C_, ldl_, js, jb0_.get(), x, jxs_.get(),
+                       ax);
     prof_.end(stream_);
+}
+
+bool DMDSystem::newton(const double* r_crd, const double* zeta, double beta, const double* a_host, double a_tol,
+                       double err_fac, int max_newton, int max_gmres, int* iters)
+{
+    MAPP_REQUIRE(beta > 0.0 && a_tol > 0.0 && err_fac > 0.0, ERR_ARG, "beta, a_tol and err_fac must be positive");
+    MAPP_REQUIRE(max_newton >= 0, ERR_ARG, "max_nnewton_iters must be >= 0");
+    MAPP_REQUIRE(max_gmres >= 1 && max_gmres <= kPartW, ERR_ARG, "max_ngmres_iters must be in [1, 8]");
+    MAPP_REQUIRE(!surr_, ERR_STATE, "the implicit step runs on the quadrature (no surrogate)");
+    const int m = max_gmres;
+    const int nb = (nc_ + kBlock - 1) / kBlock;
+    const size_t ld = nc_;
+    nt_a_.reserve(nc_);
+    nt_F_.reserve(nc_);
+    nt_del_.reserve(nc_);
+    nt_c0_.reserve(nc_);
+    nt_v_.reserve(nc_);
+    nt_Q_.reserve((size_t)(m + 1) * nc_);
+    nt_red_.reserve(16);
+    double* part = part_.get();
+    {
+        DevBuf<double> th;
+        th.reserve(nc_);
+        MAPP_HIP(hipMemcpyAsync(th.get(), a_host, sizeof(double) * nc_, hipMemcpyHostToDevice, stream_));
+        hipLaunchKernelGGL(k_dscatter_comp, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n_, C_,
+                           slot_of_.get(), th.get(), nt_a_.get());
+        MAPP_HIP(hipStreamSynchronize(stream_));
+    }
+    MAPP_HIP(hipMemcpyAsync(nt_c0_.get(), c_.get(), sizeof(double) * nc_, hipMemcpyDeviceToDevice, stream_));
+    const double beta_inv = 1.0 / beta;
+    const double a_tol_sqrt = a_tol * std::sqrt((double)nc_);  // nc_dof = natms c_dim (src/ff_dmd.cpp:75)
+    auto F_norm = [&] {
+        c_d(r_crd, zeta, nullptr);
+        hipLaunchKernelGGL(k_newton_f, dim3(nb), dim3(kBlock), 0, stream_, nc_, beta_inv, c_.get(), nt_a_.get(),
+                           cd_.get(), nt_F_.get());
+    };
+    auto fetch = [&](int count, double* out) {
+        MAPP_HIP(hipMemcpyAsync(out, nt_red_.get(), sizeof(double) * count, hipMemcpyDeviceToHost, stream_));
+        MAPP_HIP(hipStreamSynchronize(stream_));
+    };
+    // __GMRES::solve (src/gmres.h:40-60) of (I / beta - J) del = F from zero;
+    // the Krylov basis stays on the device, the Hessenberg columns on the host
+    std::vector<double> H((size_t)m * (m + 1) / 2), b_hat(m + 1), cs(2 * (m + 1));
+    auto solve_y = [&](int nv, double& norm) {
+        GmY y{};
+        for (int i = nv - 1; i > -1; --i) {
+            y.y[i] = b_hat[i];
+            for (int j = i + 1; j < nv; ++j) y.y[i] -= H[(size_t)j * (j + 1) / 2 + i] * y.y[j];
+            y.y[i] /= H[(size_t)i * (i + 1) / 2 + i];
+        }
+        hipLaunchKernelGGL(k_gm_combine, dim3(nb), dim3(kBlock), 0, stream_, nc_, nv, y, nt_Q_.get(), ld,
+                           nt_del_.get());
+        double s = 0.0;
+        for (int k = 0; k < nv; ++k) s += y.y[k] * y.y[k];
+        norm = std::sqrt(s);
+    };
+    auto gmres = [&](double tol, double& norm) -> bool {
+        double* Q = nt_Q_.get();
+        prof_.begin(K_DMD_NEWTON, stream_);
+        hipLaunchKernelGGL(k_gm_dots, dim3(nb), dim3(kBlock), 0, stream_, nc_, 1, nt_F_.get(), nt_F_.get(), ld, part);
+        reduce_rows(part, nb, 1, nt_red_.get());
+        prof_.end(stream_);
+        double bb;
+        fetch(1, &bb);
+        const double bn = std::sqrt(bb);
+        if (!(bn > 0.0)) {  // already a root: no step (the reference would divide by zero here)
+            hipLaunchKernelGGL(k_gm_scale, dim3(nb), dim3(kBlock), 0, stream_, nc_, 0.0, nt_F_.get(), nt_del_.get());
+            norm = 0.0;
+            return true;
+        }
+        hipLaunchKernelGGL(k_gm_scale, dim3(nb), dim3(kBlock), 0, stream_, nc_, 1.0 / bn, nt_F_.get(), Q);
+        b_hat[0] = bn;
+        for (int it = 0; it < m; ++it) {
+            const int iv = it + 1;
+            J_prepare();
+            J_dev(Q + (size_t)it * ld, nt_v_.get());
+            prof_.begin(K_DMD_NEWTON, stream_);
+            hipLaunchKernelGGL(k_newton_jac, dim3(nb), dim3(kBlock), 0, stream_, nc_, beta_inv, Q + (size_t)it * ld,
+                               nt_v_.get());
+            hipLaunchKernelGGL(k_gm_dots, dim3(nb), dim3(kBlock), 0, stream_, nc_, iv, nt_v_.get(), Q, ld, part);
+            reduce_rows(part, nb, iv, nt_red_.get());
+            hipLaunchKernelGGL(k_gm_orth, dim3(nb), dim3(kBlock), 0, stream_, nc_, iv, nt_v_.get(), Q, ld,
+                               nt_red_.get(), Q + (size_t)iv * ld, part);
+            reduce_rows(part, nb, 1, nt_red_.get() + kPartW);
+            prof_.end(stream_);
+            double hr[2 * kPartW];
+            fetch(2 * kPartW, hr);
+            double* h = H.data() + (size_t)it * (it + 1) / 2;
+            for (int j = 0; j < iv; ++j) h[j] = hr[j];
+            const double nrm = std::sqrt(hr[kPartW]);
+            hipLaunchKernelGGL(k_gm_scale, dim3(nb), dim3(kBlock), 0, stream_, nc_, 1.0 / nrm, Q + (size_t)iv * ld,
+                               Q + (size_t)iv * ld);
+            for (int i = 0; i < it; ++i) {
+                const double t = cs[2 * i] * h[i] - cs[2 * i + 1] * h[i + 1];
+                h[i + 1] = cs[2 * i + 1] * h[i] + cs[2 * i] * h[i + 1];
+                h[i] = t;
+            }
+            const double t = std::sqrt(h[it] * h[it] + nrm * nrm);
+            cs[2 * it] = h[it] / t;
+            cs[2 * it + 1] = -nrm / t;
+            h[it] = cs[2 * it] * h[it] - cs[2 * it + 1] * nrm;
+            b_hat[it + 1] = cs[2 * it + 1] * b_hat[it];
+            b_hat[it] *= cs[2 * it];
+            if (std::fabs(b_hat[it + 1]) < tol) {
+                solve_y(it + 1, norm);
+                return true;
+            }
+        }
+        solve_y(m, norm);
+        return false;
+    };
+    // DAEImplicit::newton (src/dae_imp.cpp:146-206)
+    const double res_tol = 0.005 * a_tol_sqrt / err_fac;
+    const double denom = 0.1 * a_tol_sqrt / err_fac;
+    double norm = 1.0, delta = 0.0, delta_prev = 0.0, ratio = 1.0, R = 1.0;
+    int iter = 0;
+    bool converge = false, diverge = false;
+    F_norm();
+    while (iter < max_newton && !converge && !diverge) {
+        gmres(res_tol, norm);
+        // update_c
+        prof_.begin(K_DMD_NEWTON, stream_);
+        hipLaunchKernelGGL(k_newton_r, dim3(nb), dim3(kBlock), 0, stream_, nc_, c_.get(), nt_del_.get(), part);
+        hipLaunchKernelGGL(k_extrema_rows, dim3(1), dim3(kBlock), 0, stream_, nb, 0, 1, part, nt_red_.get());
+        prof_.end(stream_);
+        double r;
+        fetch(1, &r);
+        hipLaunchKernelGGL(k_newton_apply, dim3(nb), dim3(kBlock), 0, stream_, nc_, r, nt_del_.get(), c_.get());
+        ++ver_;
+        ratio = r;
+        delta = std::fabs(ratio * norm / denom);
+        if (iter) R = std::max(0.3 * R, delta / delta_prev);
+        if (iter > 1 && R * delta < 1.0) {
+            F_norm();
+            converge = true;
+            continue;
+        }
+        if (iter > 1 && delta / delta_prev > 2.0) {
+            diverge = true;
+            continue;
+        }
+        delta_prev = delta;
+        F_norm();
+        iter++;
+    }
+    if (!converge) {
+        MAPP_HIP(hipMemcpyAsync(c_.get(), nt_c0_.get(), sizeof(double) * nc_, hipMemcpyDeviceToDevice, stream_));
+        ++ver_;
+    }
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    prof_.flush();
+    if (iters) *iters = iter;
+    return converge;
 }
 
 DMDReport DMDSystem::derivative(bool with_c, const double* ext_mu)

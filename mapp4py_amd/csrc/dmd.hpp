@@ -86,6 +86,12 @@ public:
     // parameters of the last c_d call (an error if the state changed since);
     // x and Ax in caller order [site][c_dim]
     void J(const double* x, double* ax);
+    // DAEImplicit::newton: solve c / beta + a - c_d(c) = 0 from the current c
+    // by Newton steps with GMRES over J (at most max_gmres <= 8 products per
+    // step); a in caller order. True on convergence (c holds the solution);
+    // otherwise c is restored. iters = Newton steps as the reference counts them.
+    bool newton(const double* r_crd, const double* zeta, double beta, const double* a, double a_tol,
+                double err_fac, int max_newton, int max_gmres, int* iters);
     // entries the last J state holds: live list entries and second-list entries
     void J_entries(int64_t* live, int64_t* exchange);
     int64_t half_pairs_count();
@@ -112,7 +118,8 @@ private:
     void pass2();
     bool decide();
     void reduce_rows(const double* part, int nrows, int ncols, double* out);
-    void J_dev();
+    void J_prepare();
+    void J_dev(const double* x, double* ax);
     JState jstate();
 
     std::shared_ptr<EamForceField> ff_;
@@ -139,6 +146,8 @@ private:
     DevBuf<double> jr2_, jr1_, jt0_, jf0_, jf1_, jf2_, jS_;
     DevBuf<int> jcnt_, jcnt2_;
     DevBuf<double> jx_, ja1_, jb0_, jax_, jxs_;
+    // implicit-step vectors (slot order) and the Krylov basis [m+1][nc]
+    DevBuf<double> nt_a_, nt_F_, nt_del_, nt_c0_, nt_v_, nt_Q_, nt_red_;
     std::vector<double> masses_;
     DevBuf<uint32_t> keys_, keys_alt_;
     DevBuf<int> vals_, vals_alt_;

@@ -701,6 +701,11 @@ static int gmres_solve(o_gmres* g, o_matvec A, void* ctx, double* b, double tol,
     double s = 0.0;
     for (int i = 0; i < n; i++) s += b[i] * b[i];
     double bn = sqrt(s), inv = 1.0 / bn;
+    if (!(bn > 0.0)) { /* b = 0: x = 0 (the reference divides by zero here) */
+        for (int i = 0; i < n; i++) x[i] = 0.0;
+        *norm = 0.0;
+        return 1;
+    }
     for (int i = 0; i < n; i++) x[i] = g->Q[i] = inv * b[i];
     g->b_hat[0] = bn;
     for (int it = 0; it < m; it++) {

@@ -194,3 +194,37 @@ def test_jacobian_product_matches_oracle(M, cuh, cells, tilt, seed):
         g.J(v)
     g.c_d(r_crd, zeta, fetch=False)
     np.testing.assert_allclose(g.J(v), Jv, rtol=0, atol=1e-13 * scale)
+
+
+@pytest.mark.parametrize("dt", [30.0, 300.0])
+def test_implicit_step_matches_oracle(M, cuh, dt):
+    # DAEImplicit::newton for one backward-Euler step c - c_n = dt c_d(c)
+    x, H, elem, alpha, c = sites((3, 3, 3), 11)
+    g, o = both(M, cuh, x, H, elem, alpha, c)
+    r_crd, zeta = [3.5, 3.5], [1.7, 1.7]
+    ok, it = g.newton(r_crd, zeta, dt, -c / dt, a_tol=1e-8)
+    oko, ito, co = o.newton(r_crd, zeta, cuh.masses, dt, -c / dt, a_tol=1e-8)
+    cg = g.state()[2]
+    err = np.abs(cg - co).max() / np.abs(co - c).max()
+    say(f"implicit step dt {dt}: converged {ok}/{oko}, iterations {it}/{ito}, max|dc| {np.abs(co - c).max():.3e}, "
+        f"rel diff vs oracle {err:.1e}")
+    assert ok and oko and it == ito
+    assert err < 1e-6
+    for e in (0, 1):
+        assert abs((cg - c)[elem == e].sum()) < 1e-12
+    # the solution is a root of F: the rates there agree with the oracle's
+    g.c_d(r_crd, zeta, fetch=False)
+    cdg = g.c_d(r_crd, zeta)[0]
+    F = cg / dt - c / dt - cdg
+    assert np.linalg.norm(F) < 0.01 * 1e-8 * np.sqrt(c.size)
+
+
+def test_implicit_step_that_does_not_converge_restores_c(M, cuh):
+    x, H, elem, alpha, c = sites((3, 3, 3), 11)
+    g, o = both(M, cuh, x, H, elem, alpha, c)
+    ok, _ = g.newton([3.5, 3.5], [1.7, 1.7], 3000.0, -c / 3000.0, a_tol=1e-8)
+    oko, _, _ = o.newton([3.5, 3.5], [1.7, 1.7], cuh.masses, 3000.0, -c / 3000.0, a_tol=1e-8)
+    assert not ok and not oko
+    np.testing.assert_array_equal(g.state()[2], c)
+    with pytest.raises(RuntimeError):
+        g.newton([3.5, 3.5], [1.7, 1.7], 30.0, -c / 30.0, max_ngmres_iters=9)
