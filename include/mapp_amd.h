@@ -178,6 +178,35 @@ mapp_status mapp_dmd_J(mapp_dmd* d, const double* x, double* Ax);
 mapp_status mapp_dmd_newton(mapp_dmd* d, const double* r_crd, const double* C, double beta, const double* a,
                             double a_tol, double err_fac, int max_nnewton_iters, int max_ngmres_iters,
                             int* converged, int* iters);
+/* Extended CFG (AtomEye) files of DMD systems. mapp_cfg_dmd_read replaces
+ * ImportCFGDMD::operator() (src/import_cfg.cpp:669-854) with
+ * ImportCFG::read_header / read_body_ext (src/import_cfg.cpp:32-356): sites
+ * with fractional positions (wrapped into [0, 1)), one alpha and one c per
+ * element (c = -1: absent), checked as the reference checks them, the
+ * present elements of a site compacted into c_dim components (absent
+ * components: element 0, c = -1, alpha = 0), the box in lower-triangular
+ * form. mapp_cfg_dmd_get fills H [3][3] (rows = box vectors), x
+ * [natms][3] (Cartesian), elem / alpha / c [natms][c_dim] and masses
+ * [nelems]; mapp_cfg_dmd_element gives the symbol of element i (NULL if out
+ * of range). */
+typedef struct mapp_cfg mapp_cfg;
+mapp_status mapp_cfg_dmd_read(const char* path, mapp_cfg** out);
+mapp_status mapp_cfg_dmd_info(const mapp_cfg* cfg, int* natms, int* c_dim, int* nelems);
+mapp_status mapp_cfg_dmd_get(const mapp_cfg* cfg, double* H, double* x, uint8_t* elem, double* alpha, double* c,
+                             double* masses);
+const char* mapp_cfg_dmd_element(const mapp_cfg* cfg, int i);
+void mapp_cfg_dmd_free(mapp_cfg* cfg);
+/* ExportCFGDMD::write_header / write_body (src/export_cfg_dmd.cpp:25-170) for
+ * host arrays laid out as mapp_cfg_dmd_get gives them (x Cartesian, H lower
+ * triangular); names holds nelems symbols. */
+mapp_status mapp_cfg_dmd_write(const char* path, int natms, int c_dim, const double* H, const double* x,
+                               const uint8_t* elem, const double* alpha, const double* c, int nelems,
+                               const double* masses, const char* const* names);
+/* ExportCFGDMD::write_header / write_body (src/export_cfg_dmd.cpp:25-170): the
+ * current state of d as an extended CFG file, sites in caller order, with
+ * the masses and symbols of the force field's nelems elements. */
+mapp_status mapp_dmd_export_cfg(mapp_dmd* d, const char* path, int nelems, const double* masses,
+                                const char* const* names);
 /* Diagnostic (no counterpart in the reference): the entries the state of the
  * last mapp_dmd_J call holds, list entries inside the smeared cut (live) and
  * second-list entries (exchange), summed over components. */

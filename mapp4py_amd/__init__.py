@@ -305,6 +305,14 @@ class AtomsDMD:
                                     int(max_nnewton_iters), int(max_ngmres_iters), C.byref(ok), C.byref(it)))
         return bool(ok.value), it.value
 
+    def export_cfg(self, path, names, masses=None):
+        """dmd.export_cfg: the current state as an extended CFG file (caller order);
+        names and masses of the force field's elements (masses default to the
+        system's)."""
+        m = f64(self.masses if masses is None else masses)
+        arr = (C.c_char_p * len(names))(*[str(n).encode() for n in names])
+        check(lib().mapp_dmd_export_cfg(self._h, str(path).encode(), len(names), ptr(m), arr))
+
     def J_entries(self):
         """(live list entries, second-list entries) of the last J state."""
         a, b = C.c_int64(), C.c_int64()
@@ -367,6 +375,40 @@ class AtomsDMD:
         if getattr(self, "_h", None):
             lib().mapp_dmd_destroy(self._h)
             self._h = None
+
+
+def import_cfg_dmd(path):
+    """dmd.atoms.import_cfg: an extended CFG file of a DMD system as a dict with
+    H [3][3], x [natms][3], elem / alpha / c [natms][c_dim], masses and names
+    (the element symbols in file order)."""
+    h = C.c_void_p()
+    check(lib().mapp_cfg_dmd_read(str(path).encode(), C.byref(h)))
+    try:
+        n, cd, ne = C.c_int(), C.c_int(), C.c_int()
+        check(lib().mapp_cfg_dmd_info(h, C.byref(n), C.byref(cd), C.byref(ne)))
+        n, cd, ne = n.value, max(cd.value, 1), ne.value
+        out = {"H": np.zeros((3, 3)), "x": np.zeros((n, 3)), "elem": np.zeros((n, cd), np.uint8),
+               "alpha": np.zeros((n, cd)), "c": np.zeros((n, cd)), "masses": np.zeros(max(ne, 1))}
+        check(lib().mapp_cfg_dmd_get(h, ptr(out["H"]), ptr(out["x"]), ptr(out["elem"], u8p), ptr(out["alpha"]),
+                                     ptr(out["c"]), ptr(out["masses"])))
+        out["masses"] = out["masses"][:ne]
+        out["names"] = [lib().mapp_cfg_dmd_element(h, i).decode() for i in range(ne)]
+        return out
+    finally:
+        lib().mapp_cfg_dmd_free(h)
+
+
+def write_cfg_dmd(path, H, x, elem, alpha, c, names, masses):
+    """dmd.export_cfg for host arrays: x Cartesian [natms][3], H lower triangular,
+    elem / alpha / c [natms][c_dim]."""
+    x = f64(x).reshape(-1, 3)
+    n = x.shape[0]
+    elem = np.ascontiguousarray(elem, dtype=np.uint8).reshape(n, -1)
+    cd = elem.shape[1]
+    alpha, c, H, m = f64(alpha).reshape(n, cd), f64(c).reshape(n, cd), f64(H).reshape(3, 3), f64(masses)
+    arr = (C.c_char_p * len(names))(*[str(v).encode() for v in names])
+    check(lib().mapp_cfg_dmd_write(str(path).encode(), n, cd, ptr(H), ptr(x), ptr(elem, u8p), ptr(alpha), ptr(c),
+                                   len(names), ptr(m), arr))
 
 
 def gauss_hermite(n):

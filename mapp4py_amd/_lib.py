@@ -62,6 +62,14 @@ SIGNATURES = {
     "mapp_dmd_newton": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double,
                                   C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int, C.c_int,
                                   C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "mapp_cfg_dmd_read": (C.c_int, [C.c_char_p, vpp]),
+    "mapp_cfg_dmd_info": (C.c_int, [C.c_void_p, ip, ip, ip]),
+    "mapp_cfg_dmd_get": (C.c_int, [C.c_void_p, dp, dp, u8p, dp, dp, dp]),
+    "mapp_cfg_dmd_element": (C.c_char_p, [C.c_void_p, C.c_int]),
+    "mapp_cfg_dmd_free": (None, [C.c_void_p]),
+    "mapp_cfg_dmd_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, dp, dp, u8p, dp, dp, C.c_int, dp,
+                                     C.POINTER(C.c_char_p)]),
+    "mapp_dmd_export_cfg": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, dp, C.POINTER(C.c_char_p)]),
     "mapp_dmd_J_entries": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mapp_dmd_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, dp, u8p, dp, dp, C.c_double,
                                   C.c_double, C.c_double, C.c_double, dp, vpp]),

@@ -5,6 +5,7 @@
 #include <string>
 
 #include "../../include/mapp_amd.h"
+#include "cfg.hpp"
 #include "dmd.hpp"
 #include "gcmc.hpp"
 #include "surrogate.hpp"
@@ -29,6 +30,9 @@ struct mapp_dmd {
 };
 struct mapp_surr {
     std::shared_ptr<mapp::Surrogate> sys;
+};
+struct mapp_cfg {
+    mapp::CfgDMD data;
 };
 struct mapp_gcmc {
     std::unique_ptr<mapp::GcmcBatch> sys;
@@ -367,6 +371,95 @@ mapp_status mapp_dmd_newton(mapp_dmd* d, const double* r_crd, const double* C, d
         NEED(a);
         const bool ok = d->sys->newton(r_crd, C, beta, a, a_tol, err_fac, max_nnewton_iters, max_ngmres_iters, iters);
         if (converged) *converged = ok ? 1 : 0;
+    });
+}
+
+mapp_status mapp_cfg_dmd_read(const char* path, mapp_cfg** out)
+{
+    return guard([&] {
+        NEED(path);
+        NEED(out);
+        *out = nullptr;
+        auto h = std::make_unique<mapp_cfg>();
+        h->data = mapp::read_cfg_dmd(path);
+        *out = h.release();
+    });
+}
+
+mapp_status mapp_cfg_dmd_info(const mapp_cfg* cfg, int* natms, int* c_dim, int* nelems)
+{
+    return guard([&] {
+        NEED(cfg);
+        if (natms) *natms = cfg->data.natms;
+        if (c_dim) *c_dim = cfg->data.c_dim;
+        if (nelems) *nelems = cfg->data.nelems;
+    });
+}
+
+mapp_status mapp_cfg_dmd_get(const mapp_cfg* cfg, double* H, double* x, uint8_t* elem, double* alpha, double* c,
+                             double* masses)
+{
+    return guard([&] {
+        NEED(cfg);
+        const mapp::CfgDMD& d = cfg->data;
+        if (H) std::memcpy(H, d.H, sizeof d.H);
+        if (x) std::memcpy(x, d.x.data(), sizeof(double) * d.x.size());
+        if (elem) std::memcpy(elem, d.elem.data(), d.elem.size());
+        if (alpha) std::memcpy(alpha, d.alpha.data(), sizeof(double) * d.alpha.size());
+        if (c) std::memcpy(c, d.c.data(), sizeof(double) * d.c.size());
+        if (masses) std::memcpy(masses, d.masses.data(), sizeof(double) * d.masses.size());
+    });
+}
+
+const char* mapp_cfg_dmd_element(const mapp_cfg* cfg, int i)
+{
+    if (!cfg || i < 0 || i >= cfg->data.nelems) return nullptr;
+    return cfg->data.names[i].c_str();
+}
+
+void mapp_cfg_dmd_free(mapp_cfg* cfg) { delete cfg; }
+
+mapp_status mapp_cfg_dmd_write(const char* path, int natms, int c_dim, const double* H, const double* x,
+                               const uint8_t* elem, const double* alpha, const double* c, int nelems,
+                               const double* masses, const char* const* names)
+{
+    return guard([&] {
+        NEED(path);
+        NEED(H);
+        NEED(masses);
+        NEED(names);
+        MAPP_REQUIRE(natms >= 0 && c_dim >= 1 && c_dim <= 4, mapp::ERR_ARG, "natms >= 0 and 1 <= c_dim <= 4");
+        if (natms) {
+            NEED(x);
+            NEED(elem);
+            NEED(alpha);
+            NEED(c);
+        }
+        for (int i = 0; i < (natms * c_dim); ++i)
+            MAPP_REQUIRE(elem[i] < nelems, mapp::ERR_ARG, "element index out of range");
+        std::vector<std::string> nm;
+        for (int i = 0; i < nelems; ++i) {
+            NEED(names[i]);
+            nm.emplace_back(names[i]);
+        }
+        mapp::write_cfg_dmd(path, natms, c_dim, H, x, elem, alpha, c, nelems, masses, nm);
+    });
+}
+
+mapp_status mapp_dmd_export_cfg(mapp_dmd* d, const char* path, int nelems, const double* masses,
+                                const char* const* names)
+{
+    return guard([&] {
+        NEED(d);
+        NEED(path);
+        NEED(masses);
+        NEED(names);
+        std::vector<std::string> nm;
+        for (int i = 0; i < nelems; ++i) {
+            NEED(names[i]);
+            nm.emplace_back(names[i]);
+        }
+        d->sys->export_cfg(path, nelems, masses, nm);
     });
 }
 

@@ -14,6 +14,7 @@
 #include <cstring>
 
 #include "dmd.hpp"
+#include "cfg.hpp"
 #include "kern.hpp"
 #include "surrogate.hpp"
 
@@ -1433,6 +1434,7 @@ DMDSystem::DMDSystem(std::shared_ptr<EamForceField> ff, int nsites, int c_dim, i
     MAPP_REQUIRE(nquad >= 1 && nquad <= kMaxQuad, ERR_ARG, "1 to 16 Hermite points");
     MAPP_REQUIRE((uint64_t)nsites * c_dim <= kIndexMask, ERR_CAPACITY, "too many components");
     MAPP_REQUIRE(kB > 0.0 && T > 0.0 && hP > 0.0, ERR_ARG, "kB, T and hP must be positive");
+    elem_host_.assign(elem, elem + (size_t)nsites * c_dim);
     for (int k = 0; k < nsites * c_dim; ++k) {
         MAPP_REQUIRE(elem[k] < t.ne, ERR_ARG, "element index out of range");
         MAPP_REQUIRE(c[k] < 0.0 || alpha[k] > 0.0, ERR_ARG, "present components need alpha > 0");
@@ -2134,6 +2136,17 @@ void DMDSystem::set_state(const double* x, const double* alpha, const double* c)
     hipLaunchKernelGGL(k_dscatter, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n_, C_, tx.get(),
                        ta.get(), tc.get(), slot_of_.get(), x_.get(), al_.get(), c_.get());
     MAPP_HIP(hipStreamSynchronize(stream_));
+}
+
+void DMDSystem::export_cfg(const char* path, int nelems, const double* masses, const std::vector<std::string>& names)
+{
+    MAPP_REQUIRE(nelems >= ff_->tables().ne, ERR_ARG, "one name and mass per element of the force field");
+    std::vector<double> x((size_t)3 * n_), a(nc_), c(nc_);
+    get_state(x.data(), a.data(), c.data());
+    double H[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) H[i * 3 + j] = box_.H[i][j];
+    write_cfg_dmd(path, n_, C_, H, x.data(), elem_host_.data(), a.data(), c.data(), nelems, masses, names);
 }
 
 void DMDSystem::get_state(double* x, double* alpha, double* c)

@@ -2,6 +2,7 @@
 // EAM by Gauss-Hermite quadrature over per-site components.
 #pragma once
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "md.hpp"
@@ -78,6 +79,8 @@ public:
     DMDReport derivative(bool with_c, const double* ext_mu);
     double value(bool with_c, const double* ext_mu);
     void get_forces(double* f, double* f_alpha, double* mu);
+    // ExportCFGDMD: the current state as an extended CFG file (caller order)
+    void export_cfg(const char* path, int nelems, const double* masses, const std::vector<std::string>& names);
     // ForceFieldDMD::c_d_calc: concentration rates over the second list
     // (same element, r < r_crd[e]) with exchange widths zeta[e]; c_d in caller
     // order [site][c_dim] when c_d_host is not null; returns max |c_d|
@@ -149,6 +152,7 @@ private:
     // implicit-step vectors (slot order) and the Krylov basis [m+1][nc]
     DevBuf<double> nt_a_, nt_F_, nt_del_, nt_c0_, nt_v_, nt_Q_, nt_red_;
     std::vector<double> masses_;
+    std::vector<uint8_t> elem_host_;  // [site][c_dim] in caller order
     DevBuf<uint32_t> keys_, keys_alt_;
     DevBuf<int> vals_, vals_alt_;
     DevBuf<unsigned char> sort_tmp_;
