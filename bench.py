@@ -221,6 +221,19 @@ def dmd_run(evals=10):
     kj = d.profile_read("dmd_j")
     d.profile(False)
     j_live, j_exch = d.J_entries()
+    # one implicit backward-Euler step of the concentrations (DAEImplicit::newton,
+    # GMRES over J): c - c_n = dt c_d(c) from c_n, dt = 30 (time units of
+    # the table: sqrt(amu/eV) A), then c restored for the CG leg
+    xs0, as0, cs0 = d.state()
+    dt_be = 30.0
+    d.profile(True)
+    t0 = time.perf_counter()
+    nt_ok, nt_it = d.newton(r_crd, zeta, dt_be, -cs0 / dt_be, a_tol=1e-8)
+    dt_nt = time.perf_counter() - t0
+    nt_k = {k: d.profile_read(k) for k in ("dmd_pass1", "dmd_pass2", "dmd_cd", "dmd_j_state", "dmd_j", "dmd_newton")}
+    d.profile(False)
+    nt_dc = float(np.abs(d.state()[2] - cs0).max())
+    d.set_state(xs0, as0, cs0)
     j_bytes = J_BYTES_PER_LIVE * j_live + J_BYTES_PER_EXCHANGE * j_exch + J_BYTES_PER_COMPONENT * c.size
     # config 3's solver: CG on the free energy from the same state (x and alpha)
     t0 = time.perf_counter()
@@ -252,6 +265,11 @@ def dmd_run(evals=10):
                   "slot_state_ms": kjs[0] / max(kjs[1], 1),
                   "live_entries": j_live, "exchange_entries": j_exch, "algorithmic_bytes": j_bytes,
                   "hbm_fraction": j_bytes / (kj[0] / kj[1] / 1e3) / HBM_PEAK},
+            "implicit_step": {"solver": "DAEImplicit::newton, GMRES(5) over __J, a_tol 1e-8, backward Euler dt 30",
+                              "converged": nt_ok, "newton_iters": nt_it, "ms": dt_nt * 1e3, "max_abs_dc": nt_dc,
+                              "site_steps_per_s": n / dt_nt,
+                              "kernels_ms": {k: v[0] for k, v in nt_k.items()},
+                              "launches": {k: v[1] for k, v in nt_k.items()}},
             "cg": {"solver": "CG on free energy over x and alpha (MinCGDMD), grad tol 1e-6, max 400 iterations",
                    "iters": cg["iters"], "status": cg["status"], "fe0": cg["fe0"], "fe": cg["fe"],
                    "grad_norm": cg["grad_norm"], "grad_max": cg["grad_max"], "seconds": cg_s},
@@ -325,6 +343,122 @@ def dmd_j_cpu_baseline(seconds):
     return {"value": len(x) * k / dt, "unit": "site-evals/s", "cores": 1, "kind": "port",
             "sample": f"oracle __J, state rebuilt per call (calc_mu + rate coefficients), fcc 4x4x8 cells = "
                       f"{len(x)} sites, {k} products, {dt:.1f} s"}
+
+
+def dmd_newton_cpu_baseline(seconds):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from mapp4py_amd import potentials as pot
+
+    p = pot.johnson_cuh(nr=5000, nrho=5000, rc=5.5)
+    x, H, elem, alpha, c = pot.dmd_cuh_sites((4, 4, 8), seed=4)
+    x = x + np.random.default_rng(5).normal(0, 0.05, x.shape)
+    o = O.ODMD(O.OEam.from_knots(p, dmd=True), H, x, elem, alpha, c, nquad=3, masses=p.masses)
+    r_crd, zeta = [3.5, 3.5], [1.7, 1.7]
+    k, t0 = 0, time.perf_counter()
+    while True:
+        o.set_state(x, alpha, c)
+        o.newton(r_crd, zeta, p.masses, 30.0, -c / 30.0, a_tol=1e-8)
+        k += 1
+        if time.perf_counter() - t0 > seconds and k >= 2:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": len(x) * k / dt, "unit": "site-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle DAEImplicit::newton (its __J rebuilds the state per product), backward Euler dt 30, "
+                      f"fcc 4x4x8 cells = {len(x)} sites, {k} steps, {dt:.1f} s"}
+
+
+def example_replay(cpu_seconds):
+    """The reference's diffusion example up to its first printed line:
+    examples/diff/configs/Cu-DMD.cfg (16,384 sites, one vacancy) through the
+    library's CFG importer, x and alpha relaxed at fixed c (CG on the GPU, then
+    Newton-Krylov on the GPU gradient), FE and S against step 0 of
+    examples/diff/output_cu."""
+    import gzip
+    import shutil
+    import tempfile
+
+    import mapp4py_amd as M
+    from scipy.optimize import newton_krylov
+
+    import math
+
+    GOLDEN = os.path.join(ROOT, "tests", "golden")
+    HP_EXAMPLE = 4.13566766225 * 0.1 * math.sqrt(1.60217656535 / 1.66053904020)  # examples/diff/in_Cu_diff.py
+    KB_EXAMPLE = 8.617330350e-5
+    with open(os.path.join(GOLDEN, "cu_dmd_step0.json")) as fj:
+        gd = json.load(fj)
+    tmp = tempfile.mkdtemp()
+    try:
+        files = {}
+        for name in ("Cu-DMD.cfg.gz", "Cu_mishin.eam.alloy.gz"):
+            files[name] = os.path.join(tmp, name[:-3])
+            with gzip.open(os.path.join(GOLDEN, name), "rb") as f, open(files[name], "wb") as g:
+                shutil.copyfileobj(f, g)
+        t0 = time.perf_counter()
+        d = M.import_cfg_dmd(files["Cu-DMD.cfg.gz"])
+        t_imp = time.perf_counter() - t0
+        ff = M.ForceFieldEAM.setfl(files["Cu_mishin.eam.alloy.gz"], ["Cu"], dmd=True)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    g = M.AtomsDMD(ff, d["H"], d["x"], d["elem"], d["alpha"], d["c"], nquad=5, T=300.0, kB=KB_EXAMPLE,
+                   hP=HP_EXAMPLE, masses=d["masses"])
+    n = len(d["c"])
+    g.derivative()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        g.derivative()
+    gpu_eval = (time.perf_counter() - t0) / 20
+    t0 = time.perf_counter()
+    r = g.min_cg(max_iter=2000, e_tol=0.0, grad_tol=1e-7)
+    t_cg = time.perf_counter() - t0
+    xs, as_, cs = g.state()
+
+    def grad(v):
+        g.set_state(v[:3 * n].reshape(n, 3), v[3 * n:].reshape(n, 1), cs)
+        g.update()
+        f, fa, _, _, _, _ = g.derivative()
+        return -np.concatenate([f.ravel(), fa.ravel()])
+
+    t0 = time.perf_counter()
+    v = newton_krylov(grad, np.concatenate([xs.ravel(), as_.ravel()]), f_tol=1e-9, maxiter=30, inner_maxiter=20,
+                      rdiff=1e-7)
+    t_nk = time.perf_counter() - t0
+    g.set_state(v[:3 * n].reshape(n, 3), v[3 * n:].reshape(n, 1), cs)
+    g.update()
+    f, fa, _, fe, FH, _ = g.derivative()
+    S = -np.asarray(FH) / np.linalg.det(d["H"])
+    out = {"workload": "examples/diff Cu-DMD.cfg, 16,384 sites, Cu_mishin setfl, N=5, 300 K",
+           "import_ms": t_imp * 1e3, "import_sites_per_s": n / t_imp,
+           "gpu_evals_per_s": 1.0 / gpu_eval, "gpu_ms_per_eval": gpu_eval * 1e3,
+           "cg": {"iters": r["iters"], "seconds": t_cg, "fe": r["fe"], "grad_max": r["grad_max"]},
+           "newton_krylov_seconds": t_nk, "fe": fe, "S_diag": [S[0], S[3], S[5]],
+           "max_grad": float(max(np.abs(f).max(), np.abs(fa).max())),
+           "reference_output_step0": {"FE": gd["FE"], "S_diag": gd["S_diag"][0]},
+           "abs_err_fe": abs(fe - gd["FE"]), "abs_err_S": float(np.abs(S[[0, 3, 5]] - gd["S_diag"][0]).max())}
+    if cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+
+        tmp = tempfile.mkdtemp()
+        try:
+            pot = os.path.join(tmp, "Cu_mishin.eam.alloy")
+            with gzip.open(os.path.join(GOLDEN, "Cu_mishin.eam.alloy.gz"), "rb") as fsrc, open(pot, "wb") as fdst:
+                shutil.copyfileobj(fsrc, fdst)
+            oe = O.OEam.read(pot, 0, dmd=True)
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        o = O.ODMD(oe, d["H"], d["x"], d["elem"], d["alpha"], d["c"], nquad=5, kB=KB_EXAMPLE, T=300.0,
+                   hP=HP_EXAMPLE, masses=d["masses"])
+        k, t0 = 0, time.perf_counter()
+        while k < 2 or time.perf_counter() - t0 < cpu_seconds:
+            o.force()
+            k += 1
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": k / dt, "unit": "evals/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle free energy + gradient of the same 16,384-site state, {k} "
+                                         f"evaluations, {dt:.1f} s"}
+    return out
 
 
 def gcmc_run(launches=10, attempts=64):
@@ -442,10 +576,14 @@ def main():
     if not args.no_dmd:
         log("[bench] DMD")
         dmd = dmd_run()
+        dmd["example_replay"] = example_replay(0.0 if args.no_cpu else args.cpu_seconds / 5)
+        log(f"[bench] examples/diff replay: FE {dmd['example_replay']['fe']:.6f} "
+            f"(|err| {dmd['example_replay']['abs_err_fe']:.1e}), S err {dmd['example_replay']['abs_err_S']:.1e}")
         if not args.no_cpu:
             dmd["cpu_baseline"] = dmd_cpu_baseline(args.cpu_seconds / 3)
             dmd["c_d"]["cpu_baseline"] = dmd_cd_cpu_baseline(args.cpu_seconds / 5)
             dmd["J"]["cpu_baseline"] = dmd_j_cpu_baseline(args.cpu_seconds / 5)
+            dmd["implicit_step"]["cpu_baseline"] = dmd_newton_cpu_baseline(args.cpu_seconds / 5)
         log(f"[bench] DMD {dmd['value']:.3e} site-evals/s")
     if not args.no_gcmc:
         log("[bench] GCMC")
