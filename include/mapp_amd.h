@@ -45,6 +45,12 @@ const char* mapp_version(void);
  * ForceFieldEAMDMD (src/ff_eam_dmd.cpp:17-106). */
 mapp_status mapp_eam_create_setfl(const char* path, const char* const* elems, int nelems, int dmd, mapp_eam** out);
 mapp_status mapp_eam_create_fs(const char* path, const char* const* elems, int nelems, int dmd, mapp_eam** out);
+/* ff_eam_funcfl (src/ff_eam.cpp:597-650) with ImportEAM::funcfl
+ * (src/import_eam.h:68-344): one DYNAMO funcfl file per element, element i =
+ * paths[i]; the files are resampled onto the largest spacing over the longest
+ * range, r phi_ij = 27.2 * 0.529 Z_i Z_j. Files whose r table is shorter than
+ * that grid are rejected (the reference reads past them). */
+mapp_status mapp_eam_create_funcfl(const char* const* paths, int nfiles, int dmd, mapp_eam** out);
 /* From knot arrays: F[ne][nrho], rho[ne][ne][nr] (rho[a][b] = density
  * function listed in element a's block at position b), r_phi[ne][ne][nr]
  * symmetric; fs == 0 uses rho[a][0] for every b as setfl does. */

@@ -15,6 +15,7 @@ The Python layer is a thin mirror of the reference's objects over the C ABI
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -50,6 +51,15 @@ class ForceFieldEAM:
         ne = C.c_int()
         check(lib().mapp_eam_cutoff(h, C.byref(ne), None))
         return cls(h, ne.value)
+
+    @classmethod
+    def funcfl(cls, paths, dmd=False):
+        """ff_eam_funcfl: one DYNAMO funcfl file per element, in element order."""
+        paths = [paths] if isinstance(paths, (str, bytes, os.PathLike)) else list(paths)
+        arr = (C.c_char_p * len(paths))(*[os.fsencode(p) for p in paths])
+        h = C.c_void_p()
+        check(lib().mapp_eam_create_funcfl(arr, len(paths), int(dmd), C.byref(h)))
+        return cls(h, len(paths))
 
     @classmethod
     def from_knots(cls, pot, dmd=False):

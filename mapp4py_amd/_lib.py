@@ -31,6 +31,7 @@ SIGNATURES = {
     "mapp_version": (C.c_char_p, []),
     "mapp_eam_create_setfl": (C.c_int, [C.c_char_p, C.POINTER(C.c_char_p), C.c_int, C.c_int, vpp]),
     "mapp_eam_create_fs": (C.c_int, [C.c_char_p, C.POINTER(C.c_char_p), C.c_int, C.c_int, vpp]),
+    "mapp_eam_create_funcfl": (C.c_int, [C.POINTER(C.c_char_p), C.c_int, C.c_int, vpp]),
     "mapp_eam_create_knots": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dp, dp, dp,
                                         C.c_int, C.c_int, vpp]),
     "mapp_eam_destroy": (None, [C.c_void_p]),

@@ -96,6 +96,21 @@ mapp_status mapp_eam_create_fs(const char* path, const char* const* elems, int n
     });
 }
 
+mapp_status mapp_eam_create_funcfl(const char* const* paths, int nfiles, int dmd, mapp_eam** out)
+{
+    return guard([&] {
+        NEED(paths);
+        NEED(out);
+        std::vector<std::string> p;
+        for (int i = 0; i < nfiles; ++i) {
+            NEED(paths[i]);
+            p.emplace_back(paths[i]);
+        }
+        auto t = mapp::read_funcfl(p, dmd != 0);
+        *out = new mapp_eam{std::make_shared<mapp::EamForceField>(std::move(t))};
+    });
+}
+
 mapp_status mapp_eam_create_knots(int ne, int nr, int nrho, double dr, double drho, const double* F,
                                   const double* rho, const double* r_phi, int fs, int dmd, mapp_eam** out)
 {

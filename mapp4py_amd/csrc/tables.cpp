@@ -6,6 +6,7 @@
 // the grid line is ignored and r_c is derived from the last non-zero knot.
 #include "tables.hpp"
 
+#include <algorithm>
 #include <cctype>
 #include <cstdlib>
 #include <cstring>
@@ -201,6 +202,118 @@ EamTables read_dynamo(const std::string& path, int fmt, const std::vector<std::s
         }
     finalize(t);
     return t;
+}
+
+namespace {
+
+// Value at offset p of interval k of the cubic through the knots with
+// 5-point slopes (one-sided near the ends, linear on the last interval):
+// ImportEAM::interpolate(arr, n, p, k), src/import_eam.cpp:9-52
+double knot_cubic(const std::vector<double>& a, size_t n, double p, size_t k)
+{
+    const double y0 = a[k];
+    double d1, d2, d3;
+    if (k + 1 == n) return y0 + (a[n - 1] - a[n - 2]) * p;
+    if (k == 0) {
+        d1 = a[1] - a[0];
+        d2 = (-a[0] + 2.0 * a[1] - a[2]) / 2.0;
+        d3 = (a[0] - 2.0 * a[1] + a[2]) / 2.0;
+    } else if (k == 1) {
+        d1 = 0.5 * (a[2] - a[0]);
+        d2 = (11.0 * a[0] - 28.0 * a[1] + 24.0 * a[2] - 8.0 * a[3] + a[4]) / 12.0;
+        d3 = (-5.0 * a[0] + 16.0 * a[1] - 18.0 * a[2] + 8.0 * a[3] - a[4]) / 12.0;
+    } else if (k + 2 == n) {
+        d1 = 0.5 * (a[n - 1] - a[n - 3]);
+        d2 = a[n - 3] - 3.0 * a[n - 2] + 2.0 * a[n - 1];
+        d3 = 0.5 * (-a[n - 3] + 3.0 * a[n - 2] - 2.0 * a[n - 1]);
+    } else {
+        d1 = ((a[k - 2] - a[k + 2]) + 8.0 * (a[k + 1] - a[k - 1])) / 12.0;
+        d2 = (-2.0 * a[k - 2] + 15.0 * a[k - 1] - 28.0 * a[k] + 21.0 * a[k + 1] - 6.0 * a[k + 2]) / 12.0;
+        d3 = (a[k - 2] - 7.0 * a[k - 1] + 16.0 * a[k] - 17.0 * a[k + 1] + 7.0 * a[k + 2]) / 12.0;
+    }
+    return ((d3 * p + d2) * p + d1) * p + y0;
+}
+
+// interval and offset of x on a grid of n knots spaced d (the last interval
+// takes everything beyond it, the offset is clamped to 1)
+size_t knot_at(double x, double d, size_t n, double& p)
+{
+    double q = x / d;
+    size_t l = (size_t)q;
+    if (l > n - 2) l = n - 2;
+    q -= (double)l;
+    p = q < 1.0 ? q : 1.0;
+    return l;
+}
+
+}  // namespace
+
+EamTables read_funcfl(const std::vector<std::string>& paths, bool dmd)
+{
+    const int ne = (int)paths.size();
+    MAPP_REQUIRE(ne >= 1 && ne <= 8, ERR_ARG, "1 to 8 funcfl files (one per element)");
+    struct One {
+        size_t nrho = 0, nr = 0;
+        double drho = 0.0, dr = 0.0;
+        std::vector<double> F, Z, rho;
+    };
+    std::vector<One> f(ne);
+    for (int i = 0; i < ne; ++i) {
+        LineReader rd(paths[i]);
+        std::string line;
+        rd.need(line);  // comment
+        rd.need(line);  // atomic number, mass, lattice constant, lattice
+        rd.need(line);
+        double rc_hdr = 0.0;
+        std::istringstream is(line);
+        MAPP_REQUIRE(bool(is >> f[i].nrho >> f[i].drho >> f[i].nr >> f[i].dr >> rc_hdr), ERR_FILE,
+                     "invalid line in " + paths[i] + ": " + line);
+        MAPP_REQUIRE(f[i].nrho >= 5 && f[i].nr >= 5, ERR_FILE, "nr and nrho in " + paths[i] + " should be larger than 5");
+        MAPP_REQUIRE(f[i].drho > 0.0 && f[i].dr > 0.0, ERR_FILE,
+                     "dr and drho in " + paths[i] + " should be larger than 0.0");
+        std::vector<double> v;
+        rd.numbers(v, f[i].nrho + 2 * f[i].nr);
+        f[i].F.assign(v.begin(), v.begin() + f[i].nrho);
+        f[i].Z.assign(v.begin() + f[i].nrho, v.begin() + f[i].nrho + f[i].nr);
+        f[i].rho.assign(v.begin() + f[i].nrho + f[i].nr, v.end());
+    }
+    // common grid: the largest spacing over the longest range (src/import_eam.h:214-231)
+    double rmax = 0.0, rhomax = 0.0, dr = 0.0, drho = 0.0;
+    for (const One& o : f) {
+        rmax = std::max(rmax, (double)(o.nr - 1) * o.dr);
+        rhomax = std::max(rhomax, (double)(o.nrho - 1) * o.drho);
+        dr = std::max(dr, o.dr);
+        drho = std::max(drho, o.drho);
+    }
+    const size_t nr = (size_t)(rmax / dr + 0.5), nrho = (size_t)(rhomax / drho + 0.5);
+    for (int i = 0; i < ne; ++i)  // the reference would read past this file's Z table
+        MAPP_REQUIRE(nr <= f[i].nr, ERR_FILE, "funcfl: " + paths[i] + " has fewer r knots than the common grid");
+    std::vector<double> F((size_t)ne * nrho), rho((size_t)ne * ne * nr, 0.0), rphi((size_t)ne * ne * nr);
+    double p;
+    // r phi_ij = 27.2 * 0.529 Z_i Z_j (Hartree * Bohr to eV * A); Z is read on
+    // interval k of the common grid with the offset of the file's own
+    // interval, as src/import_eam.h:286-305 does
+    for (int i = 0; i < ne; ++i)
+        for (int j = 0; j <= i; ++j)
+            for (size_t k = 0; k < nr; ++k) {
+                const double r = (double)k * dr;
+                knot_at(r, f[i].dr, f[i].nr, p);
+                const double zi = knot_cubic(f[i].Z, f[i].nr, p, k);
+                knot_at(r, f[j].dr, f[j].nr, p);
+                const double zj = knot_cubic(f[j].Z, f[j].nr, p, k);
+                rphi[((size_t)i * ne + j) * nr + k] = rphi[((size_t)j * ne + i) * nr + k] = 27.2 * 0.529 * zi * zj;
+            }
+    for (int i = 0; i < ne; ++i)
+        for (size_t k = 0; k < nr; ++k) {
+            const size_t l = knot_at((double)k * dr, f[i].dr, f[i].nr, p);
+            rho[(size_t)i * ne * nr + k] = knot_cubic(f[i].rho, f[i].nr, p, l);
+        }
+    for (int i = 0; i < ne; ++i)
+        for (size_t k = 0; k < nrho; ++k) {
+            const size_t l = knot_at((double)k * drho, f[i].drho, f[i].nrho, p);
+            F[(size_t)i * nrho + k] = knot_cubic(f[i].F, f[i].nrho, p, l);
+        }
+    return tables_from_knots(ne, (int)nr, (int)nrho, dr, drho, F.data(), rho.data(), rphi.data(), true, dmd);
 }
 
 EamTables tables_from_knots(int ne, int nr, int nrho, double dr, double drho, const double* F,

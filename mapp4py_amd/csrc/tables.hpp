@@ -37,6 +37,10 @@ void spline_coefficients(int n, double* s, int stride);
 // file's elements (empty: all, in file order).
 EamTables read_dynamo(const std::string& path, int fmt, const std::vector<std::string>& elems, bool dmd);
 
+// ImportEAM::funcfl (src/import_eam.h:68-344): one DYNAMO funcfl file per
+// element (F, Z, rho), resampled onto a common grid, r phi = 27.2*0.529 Z_i Z_j.
+EamTables read_funcfl(const std::vector<std::string>& paths, bool dmd);
+
 // Knots given directly: F[ne][nrho], rho[ne][ne][nr], r_phi[ne][ne][nr].
 // setfl_alias makes every rho[a][b] equal rho[a][0] as the setfl format does.
 EamTables tables_from_knots(int ne, int nr, int nrho, double dr, double drho, const double* F,

@@ -204,6 +204,20 @@ def write_setfl(path, pot: EamKnots, rc_header=None, lattice=(3.615, "FCC")):
                 fh.write(_fmt_block(pot.r_phi[a, b]) + "\n")
 
 
+def write_funcfl(path, F, Z, rho, drho, dr, mass, Znum=29, lattice=(3.615, "FCC"), comment="synthetic funcfl"):
+    """DYNAMO funcfl (one element): comment, 'Z mass a lattice', 'nrho drho nr dr rc', then F [nrho],
+    Z [nr] and rho [nr]; r*phi = 27.2*0.529*Z(r)^2 (Hartree*Bohr to eV*A)."""
+    F, Z, rho = np.asarray(F, float), np.asarray(Z, float), np.asarray(rho, float)
+    nr = len(Z)
+    with open(path, "w") as fh:
+        fh.write(f"# {comment}\n")
+        fh.write(f"{Znum} {mass!r} {lattice[0]} {lattice[1]}\n")
+        fh.write(f"{len(F)} {drho!r} {nr} {dr!r} {(nr - 1) * dr!r}\n")
+        fh.write(_fmt_block(F) + "\n")
+        fh.write(_fmt_block(Z) + "\n")
+        fh.write(_fmt_block(rho) + "\n")
+
+
 def write_fs(path, pot: EamKnots, rc_header=None, lattice=(2.86, "BCC")):
     """DYNAMO eam/fs: like setfl but element a's block lists ne density functions."""
     ne = pot.ne

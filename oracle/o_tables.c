@@ -362,3 +362,145 @@ double o_rng_gaussian(o_rng* r)
     r->reserved = v * f;
     return u * f;
 }
+
+/* ImportEAM::interpolate(arr, n, p, k) (src/import_eam.cpp:9-52): the cubic of
+ * interval k through the knots with 5-point slopes (one-sided at the ends,
+ * linear past the last knot), at offset p */
+static double funcfl_interp(const double* a, size_t n, double p, size_t k)
+{
+    double c0 = a[k], c1, c2, c3;
+    if (k == 0) {
+        c1 = a[1] - a[0];
+        c2 = (-a[0] + 2.0 * a[1] - a[2]) / 2.0;
+        c3 = (a[0] - 2.0 * a[1] + a[2]) / 2.0;
+    } else if (k == 1) {
+        c1 = 0.5 * (a[2] - a[0]);
+        c2 = (11.0 * a[0] - 28.0 * a[1] + 24.0 * a[2] - 8.0 * a[3] + a[4]) / 12.0;
+        c3 = (-5.0 * a[0] + 16.0 * a[1] - 18.0 * a[2] + 8.0 * a[3] - a[4]) / 12.0;
+    } else if (k == n - 2) {
+        c1 = 0.5 * (a[n - 1] - a[n - 3]);
+        c2 = a[n - 3] - 3.0 * a[n - 2] + 2.0 * a[n - 1];
+        c3 = 0.5 * (-a[n - 3] + 3.0 * a[n - 2] - 2.0 * a[n - 1]);
+    } else if (k == n - 1) {
+        return c0 + (a[n - 1] - a[n - 2]) * p;
+    } else {
+        c1 = ((a[k - 2] - a[k + 2]) + 8.0 * (a[k + 1] - a[k - 1])) / 12.0;
+        c2 = (-2.0 * a[k - 2] + 15.0 * a[k - 1] - 28.0 * a[k] + 21.0 * a[k + 1] - 6.0 * a[k + 2]) / 12.0;
+        c3 = (a[k - 2] - 7.0 * a[k - 1] + 16.0 * a[k] - 17.0 * a[k + 1] + 7.0 * a[k + 2]) / 12.0;
+    }
+    return ((c3 * p + c2) * p + c1) * p + c0;
+}
+
+/* position of r on a grid of n knots spaced d: interval l, offset p
+ * (src/import_eam.h:289-296) */
+static size_t funcfl_pos(double r, double d, size_t n, double* p)
+{
+    double q = r / d;
+    size_t l = (size_t)q;
+    if (l > n - 2) l = n - 2;
+    q -= (double)l;
+    *p = q < 1.0 ? q : 1.0;
+    return l;
+}
+
+/* ImportEAM::funcfl (src/import_eam.h:68-344): one DYNAMO funcfl file per
+ * element. Each file: a comment line, the Z / mass / lattice line, then
+ * "nrho drho nr dr rc" and F [nrho], Z [nr], rho [nr] in free format. The
+ * files are resampled onto the coarsest spacing over the longest range
+ * (nr = (n_r - 1) dr / dr_max rounded, likewise nrho), r phi_ij =
+ * 27.2 * 0.529 Z_i Z_j, rho_ij = rho_i. As the reference does, Z is read at
+ * interval k of the output grid while p is the offset within the file's own
+ * interval; on one grid the two coincide. */
+o_eam* o_eam_read_funcfl(const char* const* paths, int nfiles, int dmd, char* err, int errlen)
+{
+    if (nfiles < 1 || nfiles > 8) { snprintf(err, errlen, "1 to 8 funcfl files"); return NULL; }
+    size_t nrs[8], nrhos[8];
+    double drs[8], drhos[8];
+    double* Fv[8] = {0};
+    double* Zv[8] = {0};
+    double* Rv[8] = {0};
+    o_eam* e = NULL;
+    double *F = NULL, *rho = NULL, *rphi = NULL;
+    int ok = 0;
+    for (int i = 0; i < nfiles; i++) {
+        FILE* fp = fopen(paths[i], "rb");
+        if (!fp) { snprintf(err, errlen, "cannot open %s", paths[i]); goto done; }
+        fseek(fp, 0, SEEK_END);
+        long sz = ftell(fp);
+        fseek(fp, 0, SEEK_SET);
+        reader r;
+        r.buf = (char*)malloc((size_t)sz + 1);
+        size_t got = fread(r.buf, 1, (size_t)sz, fp);
+        fclose(fp);
+        r.buf[got] = '\0';
+        r.cur = r.buf; r.line_no = 0;
+        char* l = NULL;
+        if (!next_line(&r) || !next_line(&r) || !(l = next_line(&r))) {
+            snprintf(err, errlen, "%s file ended immaturely", paths[i]); free(r.buf); goto done;
+        }
+        double rc_hdr;
+        if (sscanf(l, " %zu %lf %zu %lf %lf", &nrhos[i], &drhos[i], &nrs[i], &drs[i], &rc_hdr) != 5) {
+            snprintf(err, errlen, "invalid line in %s", paths[i]); free(r.buf); goto done;
+        }
+        if (nrhos[i] < 5 || nrs[i] < 5 || drhos[i] <= 0.0 || drs[i] <= 0.0) {
+            snprintf(err, errlen, "invalid grid in %s", paths[i]); free(r.buf); goto done;
+        }
+        size_t nv = nrhos[i] + 2 * nrs[i];
+        double* buf = (double*)malloc(sizeof(double) * nv);
+        if (read_doubles(&r, buf, nv, err, errlen)) { free(buf); free(r.buf); goto done; }
+        free(r.buf);
+        Fv[i] = buf;
+        Zv[i] = buf + nrhos[i];
+        Rv[i] = buf + nrhos[i] + nrs[i];
+    }
+    {
+        double maxr = 0.0, maxrho = 0.0, maxdr = 0.0, maxdrho = 0.0;
+        for (int i = 0; i < nfiles; i++) {
+            double a = (double)(nrs[i] - 1) * drs[i], b = (double)(nrhos[i] - 1) * drhos[i];
+            if (a > maxr) maxr = a;
+            if (b > maxrho) maxrho = b;
+            if (drs[i] > maxdr) maxdr = drs[i];
+            if (drhos[i] > maxdrho) maxdrho = drhos[i];
+        }
+        size_t nr = (size_t)(maxr / maxdr + 0.5), nrho = (size_t)(maxrho / maxdrho + 0.5);
+        double dr = maxdr, drho = maxdrho;
+        int ne = nfiles;
+        for (int i = 0; i < ne; i++)
+            if (nr > nrs[i]) {  /* the reference would read past this file's Z table */
+                snprintf(err, errlen, "funcfl: %s has fewer r knots than the common grid", paths[i]);
+                goto done;
+            }
+        F = (double*)calloc((size_t)ne * nrho, sizeof(double));
+        rho = (double*)calloc((size_t)ne * ne * nr, sizeof(double));
+        rphi = (double*)calloc((size_t)ne * ne * nr, sizeof(double));
+        double p;
+        for (int i = 0; i < ne; i++)
+            for (int j = 0; j <= i; j++)
+                for (size_t k = 0; k < nr; k++) {
+                    double r = (double)k * dr;
+                    funcfl_pos(r, drs[i], nrs[i], &p);
+                    double t0 = funcfl_interp(Zv[i], nrs[i], p, k);
+                    funcfl_pos(r, drs[j], nrs[j], &p);
+                    double t1 = funcfl_interp(Zv[j], nrs[j], p, k);
+                    double v = 27.2 * 0.529 * t0 * t1;
+                    rphi[((size_t)i * ne + j) * nr + k] = v;
+                    rphi[((size_t)j * ne + i) * nr + k] = v;
+                }
+        for (int i = 0; i < ne; i++)
+            for (size_t k = 0; k < nr; k++) {
+                size_t l = funcfl_pos((double)k * dr, drs[i], nrs[i], &p);
+                rho[((size_t)i * ne) * nr + k] = funcfl_interp(Rv[i], nrs[i], p, l);
+            }
+        for (int i = 0; i < ne; i++)
+            for (size_t k = 0; k < nrho; k++) {
+                size_t l = funcfl_pos((double)k * drho, drhos[i], nrhos[i], &p);
+                F[(size_t)i * nrho + k] = funcfl_interp(Fv[i], nrhos[i], p, l);
+            }
+        e = o_eam_from_knots(ne, (int)nr, (int)nrho, dr, drho, F, rho, rphi, 1, dmd);
+        ok = 1;
+    }
+done:
+    for (int i = 0; i < nfiles; i++) free(Fv[i]);
+    free(F); free(rho); free(rphi);
+    return ok ? e : NULL;
+}

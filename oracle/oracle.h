@@ -50,6 +50,8 @@ o_eam* o_eam_read(const char* path, int fmt, int dmd, char* err, int errlen);
 o_eam* o_eam_from_knots(int ne, int nr, int nrho, double dr, double drho,
                         const double* F, const double* rho, const double* r_phi,
                         int setfl_alias, int dmd);
+/* ImportEAM::funcfl (src/import_eam.h:68-344): one funcfl file per element */
+o_eam* o_eam_read_funcfl(const char* const* paths, int nfiles, int dmd, char* err, int errlen);
 void o_eam_free(o_eam*);
 void o_eam_info(const o_eam*, int* ne, int* nr, int* nrho, double* dr, double* drho);
 void o_eam_cut(const o_eam*, double* rc /* [ne*ne] */);

@@ -71,6 +71,7 @@ def L():
             "o_dmd_c_d": (C.c_double, [C.c_void_p, dp, dp, dp, dp]),
             "o_dmd_J": (None, [C.c_void_p, dp, dp, dp, dp, dp]),
             "o_dmd_get_c": (None, [C.c_void_p, dp]),
+            "o_eam_read_funcfl": (C.c_void_p, [C.POINTER(C.c_char_p), C.c_int, C.c_int, C.c_char_p, C.c_int]),
             "o_gmres_dense": (C.c_int, [C.c_int, C.c_int, dp, dp, C.c_double, dp, dp]),
             "o_dmd_newton": (C.c_int, [C.c_void_p, dp, dp, dp, C.c_double, dp, C.c_double, C.c_double, C.c_int,
                                        C.c_int, C.POINTER(C.c_int)]),
@@ -112,6 +113,22 @@ class OEam:
     def read(cls, path, fmt=0, dmd=False):
         err = C.create_string_buffer(256)
         h = L().o_eam_read(str(path).encode(), fmt, int(dmd), err, 256)
+        if not h:
+            raise RuntimeError(err.value.decode())
+        return cls(h)
+
+    def info(self):
+        """(ne, nr, nrho, dr, drho) of the tables."""
+        ne, nr, nrho = C.c_int(), C.c_int(), C.c_int()
+        dr, drho = C.c_double(), C.c_double()
+        L().o_eam_info(self.h, C.byref(ne), C.byref(nr), C.byref(nrho), C.byref(dr), C.byref(drho))
+        return ne.value, nr.value, nrho.value, dr.value, drho.value
+
+    @classmethod
+    def funcfl(cls, paths, dmd=False):
+        err = C.create_string_buffer(256)
+        arr = (C.c_char_p * len(paths))(*[str(p).encode() for p in paths])
+        h = L().o_eam_read_funcfl(arr, len(paths), int(dmd), err, 256)
         if not h:
             raise RuntimeError(err.value.decode())
         return cls(h)
