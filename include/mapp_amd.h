@@ -184,6 +184,30 @@ mapp_status mapp_dmd_J(mapp_dmd* d, const double* x, double* Ax);
 mapp_status mapp_dmd_newton(mapp_dmd* d, const double* r_crd, const double* C, double beta, const double* a,
                             double a_tol, double err_fac, int max_nnewton_iters, int max_ngmres_iters,
                             int* converged, int* iters);
+/* DAEBDF::run (src/dae_bdf.cpp:103-250, the coefficient algebra of
+ * src/dae_bdf.h:103-546): variable-order (1..5) BDF integration of the
+ * concentrations from the current state over t_tot, each step's implicit
+ * equation solved by mapp_dmd_newton, order and step chosen as the reference
+ * chooses them. Positions and widths stay fixed between relaxations; with
+ * nreset > 0 they are relaxed at the start, every nreset steps when the
+ * gradient error exceeds a_tol, and whenever it doubles, by the GPU CG
+ * (DAE::min_error_false's CG stage; its second-derivative Newton stage is not
+ * built). nreset = 0: no relaxation (the reference divides by nreset). With
+ * ntally > 0, rows (step, time, free energy) at step 0 and every ntally steps
+ * go to tally [max_tally_rows][3]; *tally_rows counts the rows written. */
+typedef struct {
+    double a_tol, min_dt;
+    int max_nsteps, max_nnewton_iters, max_ngmres_iters, nreset, ntally, relax_max_iter;
+} mapp_bdf_params;
+typedef struct {
+    int nsteps, max_q;
+    double t, max_dt, min_dt;
+    int nonlin_acc, nonlin_rej, intpol_acc, intpol_rej, integ_acc, integ_rej, err_mins;
+} mapp_bdf_report;
+mapp_status mapp_dmd_bdf_run(mapp_dmd* d, const double* r_crd, const double* C, double t_tot,
+                             const mapp_bdf_params* params, mapp_bdf_report* report, double* tally,
+                             int max_tally_rows, int* tally_rows);
+
 /* Extended CFG (AtomEye) files of DMD systems. mapp_cfg_dmd_read replaces
  * ImportCFGDMD::operator() (src/import_cfg.cpp:669-854) with
  * ImportCFG::read_header / read_body_ext (src/import_cfg.cpp:32-356): sites

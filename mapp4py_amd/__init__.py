@@ -323,6 +323,24 @@ class AtomsDMD:
         arr = (C.c_char_p * len(names))(*[str(n).encode() for n in names])
         check(lib().mapp_dmd_export_cfg(self._h, str(path).encode(), len(names), ptr(m), arr))
 
+    def bdf_run(self, r_crd, zeta, t_tot, a_tol=1e-8, min_dt=2.220446049250313e-16, max_nsteps=1000,
+                max_nnewton_iters=5, max_ngmres_iters=5, nreset=0, ntally=0, relax_max_iter=1000):
+        """dmd.bdf().run: variable-order BDF of the concentrations over t_tot.
+        Returns (report dict, tally rows [k][3] of (step, time, free energy))."""
+        from ._lib import BdfParams, BdfReport
+
+        p = BdfParams(float(a_tol), float(min_dt), int(max_nsteps), int(max_nnewton_iters), int(max_ngmres_iters),
+                      int(nreset), int(ntally), int(relax_max_iter))
+        r = BdfReport()
+        cap = (int(max_nsteps) // int(ntally) + 2) if ntally else 0
+        tally = np.zeros((max(cap, 1), 3))
+        rows = C.c_int()
+        rc, zt = f64(r_crd), f64(zeta)
+        check(lib().mapp_dmd_bdf_run(self._h, ptr(rc), ptr(zt), float(t_tot), C.byref(p), C.byref(r), ptr(tally),
+                                     cap, C.byref(rows)))
+        rep = {k: getattr(r, k) for k, _ in BdfReport._fields_}
+        return rep, tally[:rows.value]
+
     def J_entries(self):
         """(live list entries, second-list entries) of the last J state."""
         a, b = C.c_int64(), C.c_int64()

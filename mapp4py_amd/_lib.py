@@ -63,6 +63,8 @@ SIGNATURES = {
     "mapp_dmd_newton": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double,
                                   C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int, C.c_int,
                                   C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "mapp_dmd_bdf_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double,
+                                   C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int)]),
     "mapp_cfg_dmd_read": (C.c_int, [C.c_char_p, vpp]),
     "mapp_cfg_dmd_info": (C.c_int, [C.c_void_p, ip, ip, ip]),
     "mapp_cfg_dmd_get": (C.c_int, [C.c_void_p, dp, dp, u8p, dp, dp, dp]),
@@ -144,6 +146,18 @@ def check(status):
 
 def ptr(a, t=dp):
     return a.ctypes.data_as(t)
+
+
+class BdfParams(C.Structure):
+    _fields_ = [("a_tol", C.c_double), ("min_dt", C.c_double), ("max_nsteps", C.c_int),
+                ("max_nnewton_iters", C.c_int), ("max_ngmres_iters", C.c_int), ("nreset", C.c_int),
+                ("ntally", C.c_int), ("relax_max_iter", C.c_int)]
+
+
+class BdfReport(C.Structure):
+    _fields_ = [("nsteps", C.c_int), ("max_q", C.c_int), ("t", C.c_double), ("max_dt", C.c_double),
+                ("min_dt", C.c_double), ("nonlin_acc", C.c_int), ("nonlin_rej", C.c_int), ("intpol_acc", C.c_int),
+                ("intpol_rej", C.c_int), ("integ_acc", C.c_int), ("integ_rej", C.c_int), ("err_mins", C.c_int)]
 
 
 def f64(a, shape=None):

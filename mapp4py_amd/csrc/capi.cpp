@@ -5,6 +5,7 @@
 #include <string>
 
 #include "../../include/mapp_amd.h"
+#include "bdf.hpp"
 #include "cfg.hpp"
 #include "dmd.hpp"
 #include "gcmc.hpp"
@@ -386,6 +387,46 @@ mapp_status mapp_dmd_newton(mapp_dmd* d, const double* r_crd, const double* C, d
         NEED(a);
         const bool ok = d->sys->newton(r_crd, C, beta, a, a_tol, err_fac, max_nnewton_iters, max_ngmres_iters, iters);
         if (converged) *converged = ok ? 1 : 0;
+    });
+}
+
+mapp_status mapp_dmd_bdf_run(mapp_dmd* d, const double* r_crd, const double* C, double t_tot,
+                             const mapp_bdf_params* params, mapp_bdf_report* report, double* tally,
+                             int max_tally_rows, int* tally_rows)
+{
+    return guard([&] {
+        NEED(d);
+        NEED(r_crd);
+        NEED(C);
+        NEED(params);
+        mapp::BdfParams p;
+        p.a_tol = params->a_tol;
+        p.min_dt = params->min_dt;
+        p.max_nsteps = params->max_nsteps;
+        p.max_nnewton_iters = params->max_nnewton_iters;
+        p.max_ngmres_iters = params->max_ngmres_iters;
+        p.nreset = params->nreset;
+        p.ntally = params->ntally;
+        p.relax_max_iter = params->relax_max_iter;
+        std::vector<double> rows;
+        const mapp::BdfReport r = mapp::bdf_run(*d->sys, r_crd, C, t_tot, p, &rows);
+        const int nrows = (int)(rows.size() / 3);
+        if (tally) std::memcpy(tally, rows.data(), sizeof(double) * 3 * std::min(nrows, std::max(max_tally_rows, 0)));
+        if (tally_rows) *tally_rows = tally ? std::min(nrows, std::max(max_tally_rows, 0)) : 0;
+        if (report) {
+            report->nsteps = r.nsteps;
+            report->max_q = r.max_q;
+            report->t = r.t;
+            report->max_dt = r.max_dt;
+            report->min_dt = r.min_dt;
+            report->nonlin_acc = r.nonlin_acc;
+            report->nonlin_rej = r.nonlin_rej;
+            report->intpol_acc = r.intpol_acc;
+            report->intpol_rej = r.intpol_rej;
+            report->integ_acc = r.integ_acc;
+            report->integ_rej = r.integ_rej;
+            report->err_mins = r.err_mins;
+        }
     });
 }
 

@@ -2149,6 +2149,29 @@ void DMDSystem::export_cfg(const char* path, int nelems, const double* masses, c
     write_cfg_dmd(path, n_, C_, H, x.data(), elem_host_.data(), a.data(), c.data(), nelems, masses, names);
 }
 
+void DMDSystem::set_c(const double* c)
+{
+    ++ver_;
+    DevBuf<double> tc;
+    tc.reserve(nc_);
+    MAPP_HIP(hipMemcpyAsync(tc.get(), c, sizeof(double) * nc_, hipMemcpyHostToDevice, stream_));
+    hipLaunchKernelGGL(k_dscatter_comp, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n_, C_,
+                       slot_of_.get(), tc.get(), c_.get());
+    MAPP_HIP(hipStreamSynchronize(stream_));
+}
+
+double DMDSystem::grad_err_sq()
+{
+    const int nb = grid_for(std::max(n_, nc_));
+    hipLaunchKernelGGL(k_dot2, dim3(nb), dim3(kBlock), 0, stream_, n_, nc_, f_.get(), fa_.get(), f_.get(), fa_.get(),
+                       part_.get());
+    reduce_rows(part_.get(), nb, 1, red_.get() + 24);
+    double v = 0.0;
+    MAPP_HIP(hipMemcpyAsync(&v, red_.get() + 24, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    return v;
+}
+
 void DMDSystem::get_state(double* x, double* alpha, double* c)
 {
     DevBuf<double> tx, ta, tc;

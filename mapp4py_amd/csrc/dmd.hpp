@@ -72,6 +72,10 @@ public:
     int nsites() const { return n_; }
     int c_dim() const { return C_; }
     void set_state(const double* x, const double* alpha, const double* c);
+    // c alone, in caller order
+    void set_c(const double* c);
+    // sum of f_x^2 and f_alpha^2 of the last evaluation (ForceFieldDMD::norm_sq)
+    double grad_err_sq();
     void get_state(double* x, double* alpha, double* c);
     bool update();
     void rebuild();
