@@ -237,6 +237,11 @@ mapp_status mapp_cfg_dmd_write(const char* path, int natms, int c_dim, const dou
  * the masses and symbols of the force field's nelems elements. */
 mapp_status mapp_dmd_export_cfg(mapp_dmd* d, const char* path, int nelems, const double* masses,
                                 const char* const* names);
+/* The gradient per unit c that ForceFieldEAMDMD::calc_mu leaves in f and
+ * f_alpha (src/ff_eam_dmd.cpp:1189-1278) at the state of the last
+ * mapp_dmd_c_d call: g [natms][3], g_alpha [natms][c_dim] (either may be NULL);
+ * it is what DAE::calc_err measures. MAPP_ERR_STATE if the state changed. */
+mapp_status mapp_dmd_unit_gradient(mapp_dmd* d, double* g, double* g_alpha);
 /* Diagnostic (no counterpart in the reference): the entries the state of the
  * last mapp_dmd_J call holds, list entries inside the smeared cut (live) and
  * second-list entries (exchange), summed over components. */

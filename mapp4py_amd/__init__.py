@@ -341,6 +341,14 @@ class AtomsDMD:
         rep = {k: getattr(r, k) for k, _ in BdfReport._fields_}
         return rep, tally[:rows.value]
 
+    def unit_gradient(self):
+        """calc_mu's gradient per unit c at the state of the last c_d call:
+        (g [nsites][3], g_alpha [nsites][c_dim])."""
+        g = np.zeros((self.nsites, 3))
+        ga = np.zeros((self.nsites, self.c_dim))
+        check(lib().mapp_dmd_unit_gradient(self._h, ptr(g), ptr(ga)))
+        return g, ga
+
     def J_entries(self):
         """(live list entries, second-list entries) of the last J state."""
         a, b = C.c_int64(), C.c_int64()

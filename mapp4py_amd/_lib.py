@@ -73,6 +73,7 @@ SIGNATURES = {
     "mapp_cfg_dmd_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, dp, dp, u8p, dp, dp, C.c_int, dp,
                                      C.POINTER(C.c_char_p)]),
     "mapp_dmd_export_cfg": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, dp, C.POINTER(C.c_char_p)]),
+    "mapp_dmd_unit_gradient": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mapp_dmd_J_entries": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mapp_dmd_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, dp, u8p, dp, dp, C.c_double,
                                   C.c_double, C.c_double, C.c_double, dp, vpp]),

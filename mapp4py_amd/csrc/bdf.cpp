@@ -47,11 +47,13 @@ private:
     double* z(size_t i) { return z_.data() + i * (kMaxQ + 1); }
     void pull_c() { d_.get_state(nullptr, nullptr, c_.data()); }
 
-    // DAE::calc_err without box dofs (src/dae.cpp:151-164), from the last passes
+    // DAE::calc_err without box dofs (src/dae.cpp:151-164): the gradient per
+    // unit c that c_d_calc leaves (ForceFieldDMD::norm_sq of calc_mu's f and
+    // f_alpha) at the state of the last c_d call
     double calc_err()
     {
         const double n = 3.0 * d_.nsites() + (double)nc_;
-        return std::sqrt(d_.grad_err_sq()) / std::sqrt(n);
+        return std::sqrt(d_.unit_grad_err_sq()) / std::sqrt(n);
     }
 
     void min_error()
@@ -59,7 +61,6 @@ private:
         // DAE::min_error_false's CG stage (src/dae.cpp:274-330): MinCGDMD over x
         // and alpha at fixed c with e_tol 1e-9, max_dx 1, max_dalpha 0.1
         d_.cg(p_.relax_max_iter, 1e-9, 1.0, 0.1, 0.0);
-        d_.derivative(false, nullptr);
         ++rep_.err_mins;
     }
 
@@ -323,7 +324,7 @@ BdfReport Bdf::run(double t_tot, std::vector<double>* tally)
     t_cur_ = 0.0;
     t_fin_ = t_tot;
     if (p_.nreset) min_error();
-    d_.derivative(false, nullptr);
+    d_.c_d(r_crd_, zeta_, nullptr);
     double fe_err = calc_err(), prev_err = fe_err;
     reset();
     nconst_q_ = nconst_dt_ = 0;
@@ -352,6 +353,7 @@ BdfReport Bdf::run(double t_tot, std::vector<double>* tally)
         const bool periodic = p_.nreset && (istep + 1) % p_.nreset == 0 && fe_err / p_.a_tol > 1.0;
         if (p_.nreset && (periodic || fe_err > 2.0 * prev_err)) {
             min_error();
+            d_.c_d(r_crd_, zeta_, nullptr);
             fe_err = calc_err();
             prev_err = fe_err;
             reset();

@@ -519,6 +519,14 @@ mapp_status mapp_dmd_export_cfg(mapp_dmd* d, const char* path, int nelems, const
     });
 }
 
+mapp_status mapp_dmd_unit_gradient(mapp_dmd* d, double* g, double* g_alpha)
+{
+    return guard([&] {
+        NEED(d);
+        d->sys->unit_gradient(g, g_alpha);
+    });
+}
+
 mapp_status mapp_dmd_J_entries(mapp_dmd* d, int64_t* live, int64_t* exchange)
 {
     return guard([&] {

@@ -514,15 +514,20 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass1q(int n, const double4* __r
     block_sum_store<2>(acc, part + (size_t)blk * kPartW);
 }
 
-// pass 2: forces on x and alpha, chemical-potential sums, virial
-template <int B>
+// pass 2: forces on x and alpha, chemical-potential sums, virial. With UNIT
+// it also writes the gradient per unit c that ForceFieldEAMDMD::calc_mu
+// leaves in f and f_alpha (src/ff_eam_dmd.cpp:1189-1278): every pair weighted
+// by c_j and the component's fcoef (src/ff_eam_dmd.cpp:995-1046) instead of
+// c_i c_j, and d/dalpha with -3 kT / alpha, defined also where c_i = 0.
+template <int B, bool UNIT>
 __global__ __launch_bounds__(kBlock) void k_dmd_pass2q(int n, const double4* __restrict__ x, const double* __restrict__ al,
                                                        const double* __restrict__ cc, const int* __restrict__ el,
                                                        const int* __restrict__ nn, const uint32_t* __restrict__ list,
                                                        int ldl, int maxn, DmdArgs A, BoxDev bd,
                                                        const double* __restrict__ E, const double* __restrict__ dE,
                                                        double4* __restrict__ f, double* __restrict__ fa,
-                                                       double* __restrict__ mu, double* __restrict__ part)
+                                                       double* __restrict__ mu, double* __restrict__ part,
+                                                       double4* __restrict__ gu, double* __restrict__ gau)
 {
     const int blk = xcd_block();
     const int i = blk * kBlock + threadIdx.x;
@@ -531,18 +536,29 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass2q(int n, const double4* __r
         const int C = A.C;
         const double4 xi = x[i];
         double fx = 0.0, fy = 0.0, fz = 0.0;
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        double c_tot = 0.0;
+        if (UNIT)
+            for (int q = 0; q < C; ++q) {
+                const double v = cc[i * C + q];
+                if (v > 0.0) c_tot += v;
+            }
         for (int ic = 0; ic < C; ++ic) {
             const int ci_idx = i * C + ic;
             const double ci = cc[ci_idx];
             if (ci < 0.0) {
                 fa[ci_idx] = 0.0;
                 mu[ci_idx] = 0.0;
+                if (UNIT) gau[ci_idx] = 0.0;
                 continue;
             }
             const double ai = al[ci_idx];
             const int ei = el[ci_idx];
             const double dEi = dE[ci_idx];
-            double fai = 0.0, mui = 0.0;
+            // fcoef: 1 with one component; else c_i / sum of positive c, or
+            // 1 for the zero components of an empty site
+            const double fcoef = C == 1 ? 1.0 : (c_tot == 0.0 ? (ci == 0.0 ? 1.0 : 0.0) : ci / c_tot);
+            double fai = 0.0, mui = 0.0, gai = 0.0;
             const int cnt = nn[ci_idx];
             for (int k0 = 0; k0 < cnt; k0 += B) {
                 uint32_t e[B];
@@ -616,6 +632,13 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass2q(int n, const double4* __r
                     fx += fp * dx;
                     fy += fp * dy;
                     fz += fp * dz;
+                    if (UNIT) {
+                        const double gp = live[b] ? (drr[2] * dEi + drr[1] * dEj[b] + drr[0]) * cj[b] * fcoef : 0.0;
+                        gx += gp * dx;
+                        gy += gp * dy;
+                        gz += gp * dz;
+                        gai += live[b] ? ai * (dal[2] * dEi + dal[1] * dEj[b] + dal[0]) * cj[b] : 0.0;
+                    }
                     const double hf = 0.5 * fp;
                     acc[0] += hf * dx * dx;
                     acc[1] += hf * dx * dy;
@@ -627,8 +650,10 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass2q(int n, const double4* __r
             }
             fa[ci_idx] = fai + 3.0 * A.kBT * ci / ai;
             mu[ci_idx] = E[ci_idx] + mui;
+            if (UNIT) gau[ci_idx] = gai - 3.0 * A.kBT / ai;
         }
         f[i] = make_double4(fx, fy, fz, 0.0);
+        if (UNIT) gu[i] = make_double4(gx, gy, gz, 0.0);
     }
     block_sum_store<6>(acc, part + (size_t)blk * kPartW);
 }
@@ -1751,7 +1776,7 @@ void DMDSystem::pass1(bool energy_only)
     reduce_rows(part_.get(), nb, 2, red_.get());
 }
 
-void DMDSystem::pass2()
+void DMDSystem::pass2(bool unit)
 {
     const int nb = grid_for(n_);
     BoxDev bd;
@@ -1769,14 +1794,21 @@ void DMDSystem::pass2()
         hipLaunchKernelGGL(k_dmd_pass2<false>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
                            el_.get(), nn_.get(), list_.get(), ldl_, A_, bd, S, E_.get(), dE_.get(), f_.get(),
                            fa_.get(), mu_.get(), p2);
-    else if (batch_ == 2)
-        hipLaunchKernelGGL(k_dmd_pass2q<2>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
-                           el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, E_.get(), dE_.get(), f_.get(),
-                           fa_.get(), mu_.get(), p2);
+    else if (unit) {
+        gu_.reserve(n_);
+        gau_.reserve(nc_);
+        hipLaunchKernelGGL((k_dmd_pass2q<4, true>), dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(),
+                           c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, E_.get(), dE_.get(),
+                           f_.get(), fa_.get(), mu_.get(), p2, gu_.get(), gau_.get());
+        gu_ver_ = ver_;
+    } else if (batch_ == 2)
+        hipLaunchKernelGGL((k_dmd_pass2q<2, false>), dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(),
+                           c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, E_.get(), dE_.get(),
+                           f_.get(), fa_.get(), mu_.get(), p2, nullptr, nullptr);
     else
-        hipLaunchKernelGGL(k_dmd_pass2q<4>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
-                           el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, E_.get(), dE_.get(), f_.get(),
-                           fa_.get(), mu_.get(), p2);
+        hipLaunchKernelGGL((k_dmd_pass2q<4, false>), dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(),
+                           c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, E_.get(), dE_.get(),
+                           f_.get(), fa_.get(), mu_.get(), p2, nullptr, nullptr);
     prof_.end(stream_);
     reduce_rows(p2, nb, 6, red_.get() + 8);
 }
@@ -1800,9 +1832,9 @@ double DMDSystem::c_d(const double* r_crd, const double* zeta, double* c_d_host)
         P.c1[e] = std::sqrt(0.5 * A_.kBT / masses_[e]) / M_PI;
     }
     P.beta = 1.0 / A_.kBT;
-    // calc_mu: rho, E, dE and the raw chemical potentials of the passes
+    // calc_mu: rho, E, dE, the raw chemical potentials and the gradient per unit c
     pass1(false);
-    pass2();
+    pass2(!surr_);
     // what J reads later at this state: the parameters, mu and dE (the
     // slot values and F'' follow on the first J call)
     jmu_.reserve(nc_);
@@ -2157,6 +2189,32 @@ void DMDSystem::set_c(const double* c)
     MAPP_HIP(hipMemcpyAsync(tc.get(), c, sizeof(double) * nc_, hipMemcpyHostToDevice, stream_));
     hipLaunchKernelGGL(k_dscatter_comp, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n_, C_,
                        slot_of_.get(), tc.get(), c_.get());
+    MAPP_HIP(hipStreamSynchronize(stream_));
+}
+
+double DMDSystem::unit_grad_err_sq()
+{
+    MAPP_REQUIRE(gu_ver_ >= 0 && gu_ver_ == ver_, ERR_STATE, "the gradient per unit c needs c_d at the current state");
+    const int nb = grid_for(std::max(n_, nc_));
+    hipLaunchKernelGGL(k_dot2, dim3(nb), dim3(kBlock), 0, stream_, n_, nc_, gu_.get(), gau_.get(), gu_.get(),
+                       gau_.get(), part_.get());
+    reduce_rows(part_.get(), nb, 1, red_.get() + 24);
+    double v = 0.0;
+    MAPP_HIP(hipMemcpyAsync(&v, red_.get() + 24, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    return v;
+}
+
+void DMDSystem::unit_gradient(double* g_host, double* ga_host)
+{
+    MAPP_REQUIRE(gu_ver_ >= 0 && gu_ver_ == ver_, ERR_STATE, "the gradient per unit c needs c_d at the current state");
+    DevBuf<double> tx, ta;
+    tx.reserve((size_t)3 * n_);
+    ta.reserve(nc_);
+    hipLaunchKernelGGL(k_dgather, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n_, C_, slot_of_.get(),
+                       gu_.get(), gau_.get(), (const double*)nullptr, tx.get(), ta.get(), (double*)nullptr);
+    if (g_host) MAPP_HIP(hipMemcpyAsync(g_host, tx.get(), sizeof(double) * 3 * n_, hipMemcpyDeviceToHost, stream_));
+    if (ga_host) MAPP_HIP(hipMemcpyAsync(ga_host, ta.get(), sizeof(double) * nc_, hipMemcpyDeviceToHost, stream_));
     MAPP_HIP(hipStreamSynchronize(stream_));
 }
 

@@ -76,6 +76,10 @@ public:
     void set_c(const double* c);
     // sum of f_x^2 and f_alpha^2 of the last evaluation (ForceFieldDMD::norm_sq)
     double grad_err_sq();
+    // the gradient per unit c of the last c_d call (ForceFieldEAMDMD::calc_mu's
+    // f and f_alpha): its squared norm, or the arrays in caller order
+    double unit_grad_err_sq();
+    void unit_gradient(double* g, double* g_alpha);
     void get_state(double* x, double* alpha, double* c);
     bool update();
     void rebuild();
@@ -122,7 +126,7 @@ private:
     void setup_grid();
     void build_list();
     void pass1(bool energy_only);
-    void pass2();
+    void pass2(bool unit = false);
     bool decide();
     void reduce_rows(const double* part, int nrows, int ncols, double* out);
     void J_prepare();
@@ -148,6 +152,9 @@ private:
     // Jacobian state: bumped by every change of x, alpha, c or the slot order
     int64_t ver_ = 0, jver_ = -1, jrp_ver_ = -1;
     CdArgs jP_{};
+    DevBuf<double4> gu_;   // gradient per unit c of the last c_d, per site
+    DevBuf<double> gau_;   // and per component (alpha)
+    int64_t gu_ver_ = -1;
     DevBuf<double> jmu_, jdE_, jddE_;
     DevBuf<uint32_t> jidx_, jidx2_;   // live list entries / second list, slot-major
     DevBuf<double> jr2_, jr1_, jt0_, jf0_, jf1_, jf2_, jS_;

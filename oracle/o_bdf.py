@@ -104,10 +104,11 @@ class OBdf:
         return cd.ravel()
 
     def _calc_err(self):
-        """DAE::calc_err (src/dae.cpp:151-164) without box dofs: the gradient
-        norm of the free energy per x and alpha dof at the current state."""
-        f, fa, _, _, _, _ = self.d.force()
-        return np.sqrt((f * f).sum() + (fa * fa).sum()) / np.sqrt(f.size + fa.size)
+        """DAE::calc_err (src/dae.cpp:151-164) without box dofs: the norm of the
+        gradient per unit c (calc_mu's f and f_alpha) per x and alpha dof at the
+        current state."""
+        g, ga = self.d.unit_grad()
+        return np.sqrt((g * g).sum() + (ga * ga).sum()) / np.sqrt(g.size + ga.size)
 
     def _reset(self):
         """DAEBDF::reset (src/dae_bdf.cpp:560-604)."""

@@ -878,3 +878,62 @@ void o_dmd_get_c(const o_dmd* d, double* c)
 {
     memcpy(c, d->c_lcl, sizeof(double) * d->n * d->c_dim);
 }
+
+/* The gradient per unit c that ForceFieldEAMDMD::calc_mu leaves in f and
+ * f_alpha (src/ff_eam_dmd.cpp:1189-1278; fcoef of src/ff_eam_dmd.cpp:995-1046):
+ * over the half list, pair terms (1/r dphi/dr + dE_i (1/r) drho_ji/dr + dE_j
+ * (1/r) drho_ij/dr) weighted by c_j fcoef_i on i and by -c_i fcoef_j on a local
+ * j; alpha_i (d/dalpha terms) c_j on both, then -3 kT / alpha_i per present
+ * component. g [n][3] per site, ga [n*c_dim]. */
+void o_dmd_unit_grad(o_dmd* d, double* g, double* ga)
+{
+    int cd = d->c_dim, ne = d->ff->ne;
+    int n = d->n * cd;
+    const double* x = d->g.x;
+    const double* c = d->c;
+    const double* al = d->alpha;
+    dmd_static s;
+    dmd_static_build(d, &s);
+    double* fcoef = (double*)calloc((size_t)n + 1, sizeof(double));
+    for (int site = 0; site < d->n; site++) {
+        double ct = 0.0;
+        for (int q = 0; q < cd; q++)
+            if (c[site * cd + q] > 0.0) ct += c[site * cd + q];
+        for (int q = 0; q < cd; q++) {
+            int k = site * cd + q;
+            if (cd == 1) fcoef[k] = c[k] >= 0.0 ? 1.0 : 0.0;
+            else if (ct == 0.0) fcoef[k] = c[k] == 0.0 ? 1.0 : 0.0;
+            else fcoef[k] = c[k] >= 0.0 ? c[k] / ct : 0.0;
+        }
+    }
+    for (int k = 0; k < d->n * 3; k++) g[k] = 0.0;
+    for (int k = 0; k < n; k++) ga[k] = 0.0;
+    double rp[3], dr[3], da[3];
+    for (int i = 0; i < n; i++) {
+        if (c[i] < 0.0) continue;
+        int ei = d->elem[i];
+        const double* xi = x + 3 * (i / cd);
+        for (int64_t k = d->row[i]; k < d->row[i + 1]; k++) {
+            int j = d->col[k];
+            int ej = d->elem[j];
+            const double* xj = x + 3 * (j / cd);
+            double r = sqrt(o_rsq(xi, xj));
+            double a = sqrt(al[i] * al[i] + al[j] * al[j]);
+            if (r - a * d->xi[d->N - 1] >= d->cut[ei * ne + ej]) continue;
+            smear(d, ei, ej, r, a, rp, dr, da);
+            double gp = dr[2] * s.dE[i] + dr[1] * s.dE[j] + dr[0];
+            double ap = da[2] * s.dE[i] + da[1] * s.dE[j] + da[0];
+            double dx[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+            for (int q = 0; q < 3; q++) g[3 * (i / cd) + q] += gp * c[j] * fcoef[i] * dx[q];
+            ga[i] += al[i] * ap * c[j];
+            if (j < n) {
+                for (int q = 0; q < 3; q++) g[3 * (j / cd) + q] -= gp * c[i] * fcoef[j] * dx[q];
+                ga[j] += al[j] * ap * c[i];
+            }
+        }
+    }
+    for (int i = 0; i < n; i++)
+        if (c[i] >= 0.0) ga[i] -= 3.0 * d->kBT / al[i];
+    free(fcoef);
+    dmd_static_free(&s);
+}

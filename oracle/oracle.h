@@ -127,6 +127,8 @@ void o_dmd_calc_q(double gamma_i, double gamma_j, double mu_ji, double* Q, doubl
 double o_dmd_c_d(o_dmd*, const double* r_crd, const double* zeta, const double* masses, double* c_d);
 /* ForceFieldEAMDMD::__J (src/ff_eam_dmd.cpp:1475-1575): Ax = (d c_d / d c) x */
 void o_dmd_J(o_dmd*, const double* r_crd, const double* zeta, const double* masses, const double* x, double* Ax);
+/* ForceFieldEAMDMD::calc_mu's f and f_alpha: the gradient per unit c */
+void o_dmd_unit_grad(o_dmd*, double* g /* [natms][3] */, double* ga /* [natms*c_dim] */);
 /* the local concentrations [natms][c_dim] */
 void o_dmd_get_c(const o_dmd*, double* c);
 /* __GMRES::solve (src/gmres.h:40-60) on a dense row-major A [n][n]: returns 1

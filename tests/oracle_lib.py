@@ -71,6 +71,7 @@ def L():
             "o_dmd_c_d": (C.c_double, [C.c_void_p, dp, dp, dp, dp]),
             "o_dmd_J": (None, [C.c_void_p, dp, dp, dp, dp, dp]),
             "o_dmd_get_c": (None, [C.c_void_p, dp]),
+            "o_dmd_unit_grad": (None, [C.c_void_p, dp, dp]),
             "o_eam_read_funcfl": (C.c_void_p, [C.POINTER(C.c_char_p), C.c_int, C.c_int, C.c_char_p, C.c_int]),
             "o_gmres_dense": (C.c_int, [C.c_int, C.c_int, dp, dp, C.c_double, dp, dp]),
             "o_dmd_newton": (C.c_int, [C.c_void_p, dp, dp, dp, C.c_double, dp, C.c_double, C.c_double, C.c_int,
@@ -350,6 +351,13 @@ class ODMD:
         L().o_dmd_J(self.h, P(f64(r_crd)), P(f64(zeta)), P(f64(masses)), P(f64(np.asarray(xc).reshape(self.n, self.C))),
                     P(out))
         return out
+
+    def unit_grad(self):
+        """calc_mu's gradient per unit c: (g [n][3], g_alpha [n][c_dim])."""
+        g = np.zeros((self.n, 3))
+        ga = np.zeros((self.n, self.C))
+        L().o_dmd_unit_grad(self.h, P(g), P(ga))
+        return g, ga
 
     def state_c(self):
         out = np.zeros((self.n, self.C))

@@ -228,3 +228,20 @@ def test_implicit_step_that_does_not_converge_restores_c(M, cuh):
     np.testing.assert_array_equal(g.state()[2], c)
     with pytest.raises(RuntimeError):
         g.newton([3.5, 3.5], [1.7, 1.7], 30.0, -c / 30.0, max_ngmres_iters=9)
+
+
+def test_unit_gradient_matches_oracle(M, cuh):
+    # calc_mu's f and f_alpha (per unit c, fcoef-weighted) after c_d
+    x, H, elem, alpha, c = sites((3, 3, 3), 11)
+    c = c.copy()
+    c[0] = 0.0  # an empty site: fcoef 1 on its zero components
+    g, o = both(M, cuh, x, H, elem, alpha, c)
+    g.c_d([3.5, 3.5], [1.7, 1.7], fetch=False)
+    gu, gau = g.unit_gradient()
+    ou, oau = o.unit_grad()
+    err = max(np.abs(gu - ou).max() / np.abs(ou).max(), np.abs(gau - oau).max() / np.abs(oau).max())
+    say(f"unit gradient: max|g| {np.abs(ou).max():.3e}, max|g_alpha| {np.abs(oau).max():.3e}, rel error {err:.1e}")
+    assert err < RTOL
+    g.set_state(*g.state())
+    with pytest.raises(RuntimeError):
+        g.unit_gradient()

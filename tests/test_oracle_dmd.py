@@ -297,3 +297,39 @@ def test_implicit_step_that_does_not_converge_restores_c(cuh_small):
     ok, it, cn = d.newton(R_CRD, ZETA, p.masses, 3000.0, -c / 3000.0, a_tol=1e-8)
     assert not ok
     np.testing.assert_array_equal(cn, c)
+
+
+def one_component(seed=7, eps=None):
+    p = pot.johnson_cuh(nr=3000, nrho=3000)
+    ff = O.OEam.from_knots(p, dmd=True)
+    x, H = pot.fcc(3.615, (2, 2, 2))
+    rng = np.random.default_rng(seed)
+    x = x + rng.normal(0, 0.05, x.shape)
+    s = x @ np.linalg.inv(H)
+    s -= np.floor(s)
+    x = s @ H
+    n = len(x)
+    alpha = rng.uniform(0.08, 0.2, (n, 1))
+    c = rng.uniform(0.3, 1.0, (n, 1))
+    c[0, 0] = 0.0 if eps is None else eps  # a vacancy
+    return p, ff, x, H, np.zeros((n, 1), np.uint8), alpha, c
+
+
+def test_unit_gradient_is_the_force_per_unit_c():
+    # calc_mu's f and f_alpha: with one component, c_i times them is minus the
+    # free-energy force on that site; on the vacancy (c = 0), where the force
+    # vanishes, they are the limit of -force / c as c -> 0
+    p, ff, x, H, elem, alpha, c = one_component()
+    d = O.ODMD(ff, H, x, elem, alpha, c, nquad=3, masses=p.masses)
+    g, ga = d.unit_grad()
+    f, fa, _, _, _, _ = d.force()
+    live = c[:, 0] > 0.0
+    np.testing.assert_allclose((g * c)[live], -f[live], rtol=0, atol=1e-12 * np.abs(f).max())
+    np.testing.assert_allclose((ga * c)[live], -fa[live], rtol=0, atol=1e-12 * np.abs(fa).max())
+    assert np.abs(g[0]).max() > 1e-3 and np.all(f[0] == 0.0)
+    eps = 1e-7
+    p, ff, x, H, elem, alpha, ce = one_component(eps=eps)
+    de = O.ODMD(ff, H, x, elem, alpha, ce, nquad=3, masses=p.masses)
+    fe_, fae, _, _, _, _ = de.force()
+    assert np.abs(-fe_[0] / eps - g[0]).max() < 1e-5 * np.abs(g[0]).max()
+    assert abs(-fae[0, 0] / eps - ga[0, 0]) < 1e-5 * abs(ga[0, 0])
