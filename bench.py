@@ -138,11 +138,11 @@ def feni_run(steps=1000, warmup=20):
             "kernel_ms": {k: ms / max(c, 1) for k, (ms, c) in kern.items()}, "dtype": "f32+f64"}
 
 
-# algorithmic bytes of one __J product: per live list entry the partner
-# index and rho_{j->i} (first sum) and the index, rho_{i->j} and t0 (second);
-# per second-list entry the index and three coefficients; per component the
+# algorithmic bytes of one __J product: per list entry the partner index and
+# rho_{j->i} (first sum) and the index, rho_{i->j} and t0 (second); per
+# second-list entry the index and three coefficients; per component the
 # counts, F'', S0..S2, c, x, A1, b0, xs and Ax once each
-J_BYTES_PER_LIVE = 32
+J_BYTES_PER_ENTRY = 32
 J_BYTES_PER_EXCHANGE = 28
 J_BYTES_PER_COMPONENT = 120
 DMD_BYTES_PER_SITE = 132  # x 16, alpha 8, c 16 (read twice), f 12, f_alpha 8, mu 16, E/dE write + read 32
@@ -220,7 +220,7 @@ def dmd_run(evals=10):
     dt_j = time.perf_counter() - t0
     kj = d.profile_read("dmd_j")
     d.profile(False)
-    j_live, j_exch = d.J_entries()
+    j_entries, j_exch = d.J_entries()
     # one implicit backward-Euler step of the concentrations (DAEImplicit::newton,
     # GMRES over J): c - c_n = dt c_d(c) from c_n, dt = 30 (time units of
     # the table: sqrt(amu/eV) A), then c restored for the CG leg
@@ -234,7 +234,16 @@ def dmd_run(evals=10):
     d.profile(False)
     nt_dc = float(np.abs(d.state()[2] - cs0).max())
     d.set_state(xs0, as0, cs0)
-    j_bytes = J_BYTES_PER_LIVE * j_live + J_BYTES_PER_EXCHANGE * j_exch + J_BYTES_PER_COMPONENT * c.size
+    # 20 variable-order BDF steps of the concentrations (DAEBDF), x and alpha fixed
+    d.profile(True)
+    t0 = time.perf_counter()
+    bdf_rep, _ = d.bdf_run(r_crd, zeta, 1e6, a_tol=1e-8, max_nsteps=20)
+    dt_bdf = time.perf_counter() - t0
+    bdf_k = {k: d.profile_read(k) for k in ("dmd_pass1", "dmd_pass2", "dmd_cd", "dmd_j_state", "dmd_j", "dmd_newton")}
+    d.profile(False)
+    bdf_dc = float(np.abs(d.state()[2] - cs0).max())
+    d.set_state(xs0, as0, cs0)
+    j_bytes = J_BYTES_PER_ENTRY * j_entries + J_BYTES_PER_EXCHANGE * j_exch + J_BYTES_PER_COMPONENT * c.size
     # config 3's solver: CG on the free energy from the same state (x and alpha)
     t0 = time.perf_counter()
     cg = d.min_cg(max_iter=400, e_tol=0.0, grad_tol=1e-6)
@@ -263,13 +272,18 @@ def dmd_run(evals=10):
             "J": {"metric": "Jacobian products (__J) site-evals/s, device kernels", "value": n / (kj[0] / kj[1] / 1e3),
                   "kernels_ms": kj[0] / kj[1], "host_copy_inclusive_ms": dt_j / evals * 1e3,
                   "slot_state_ms": kjs[0] / max(kjs[1], 1),
-                  "live_entries": j_live, "exchange_entries": j_exch, "algorithmic_bytes": j_bytes,
+                  "list_entries": j_entries, "exchange_entries": j_exch, "algorithmic_bytes": j_bytes,
                   "hbm_fraction": j_bytes / (kj[0] / kj[1] / 1e3) / HBM_PEAK},
             "implicit_step": {"solver": "DAEImplicit::newton, GMRES(5) over __J, a_tol 1e-8, backward Euler dt 30",
                               "converged": nt_ok, "newton_iters": nt_it, "ms": dt_nt * 1e3, "max_abs_dc": nt_dc,
                               "site_steps_per_s": n / dt_nt,
                               "kernels_ms": {k: v[0] for k, v in nt_k.items()},
                               "launches": {k: v[1] for k, v in nt_k.items()}},
+            "bdf": {"solver": "DAEBDF, a_tol 1e-8, 20 steps from the same state, x and alpha fixed",
+                    "steps": bdf_rep["nsteps"], "t_reached": bdf_rep["t"], "max_order": bdf_rep["max_q"],
+                    "rejected": bdf_rep["integ_rej"], "seconds": dt_bdf, "ms_per_step": dt_bdf / max(bdf_rep["nsteps"], 1) * 1e3,
+                    "site_steps_per_s": n * bdf_rep["nsteps"] / dt_bdf, "max_abs_dc": bdf_dc,
+                    "kernels_ms": {k: v[0] for k, v in bdf_k.items()}},
             "cg": {"solver": "CG on free energy over x and alpha (MinCGDMD), grad tol 1e-6, max 400 iterations",
                    "iters": cg["iters"], "status": cg["status"], "fe0": cg["fe0"], "fe": cg["fe"],
                    "grad_norm": cg["grad_norm"], "grad_max": cg["grad_max"], "seconds": cg_s},
@@ -461,6 +475,26 @@ def example_replay(cpu_seconds):
     return out
 
 
+def dmd_bdf_cpu_baseline():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_lib as O
+    from mapp4py_amd import potentials as pot
+    from o_bdf import OBdf
+
+    p = pot.johnson_cuh(nr=5000, nrho=5000, rc=5.5)
+    x, H, elem, alpha, c = pot.dmd_cuh_sites((4, 4, 8), seed=4)
+    x = x + np.random.default_rng(5).normal(0, 0.05, x.shape)
+    o = O.ODMD(O.OEam.from_knots(p, dmd=True), H, x, elem, alpha, c, nquad=3, masses=p.masses)
+    b = OBdf(o, [3.5, 3.5], [1.7, 1.7], p.masses, a_tol=1e-8, max_nsteps=20, x=x, alpha=alpha)
+    t0 = time.perf_counter()
+    k = b.run(1e6)
+    dt = time.perf_counter() - t0
+    return {"value": len(x) * k / dt, "unit": "site-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle DAEBDF (numpy control over the C oracle; its __J rebuilds the state per product), "
+                      f"fcc 4x4x8 cells = {len(x)} sites, {k} steps, {dt:.1f} s"}
+
+
 def gcmc_run(launches=10, attempts=64):
     """4096 periodic Ni-H cells (108-4000 atoms, log-uniform), NiAlH_jea EAM,
     H exchange at mu = -2.3 eV and 300 K: one persistent HIP-graph launch per
@@ -584,6 +618,7 @@ def main():
             dmd["c_d"]["cpu_baseline"] = dmd_cd_cpu_baseline(args.cpu_seconds / 5)
             dmd["J"]["cpu_baseline"] = dmd_j_cpu_baseline(args.cpu_seconds / 5)
             dmd["implicit_step"]["cpu_baseline"] = dmd_newton_cpu_baseline(args.cpu_seconds / 5)
+            dmd["bdf"]["cpu_baseline"] = dmd_bdf_cpu_baseline()
         log(f"[bench] DMD {dmd['value']:.3e} site-evals/s")
     if not args.no_gcmc:
         log("[bench] GCMC")

@@ -243,9 +243,9 @@ mapp_status mapp_dmd_export_cfg(mapp_dmd* d, const char* path, int nelems, const
  * it is what DAE::calc_err measures. MAPP_ERR_STATE if the state changed. */
 mapp_status mapp_dmd_unit_gradient(mapp_dmd* d, double* g, double* g_alpha);
 /* Diagnostic (no counterpart in the reference): the entries the state of the
- * last mapp_dmd_J call holds, list entries inside the smeared cut (live) and
- * second-list entries (exchange), summed over components. */
-mapp_status mapp_dmd_J_entries(mapp_dmd* d, int64_t* live, int64_t* exchange);
+ * last mapp_dmd_J call holds, list entries (those past the smeared cut hold
+ * zeros) and second-list entries (exchange), summed over components. */
+mapp_status mapp_dmd_J_entries(mapp_dmd* d, int64_t* entries, int64_t* exchange);
 mapp_status mapp_dmd_value(mapp_dmd* d, int with_c, const double* ext_mu, double* fe);
 /* NeighborDMD::create_list half list (src/neighbor_dmd.cpp:102-194) as
  * (component i, component j, image) triples in caller indices */

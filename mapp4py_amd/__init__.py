@@ -350,7 +350,7 @@ class AtomsDMD:
         return g, ga
 
     def J_entries(self):
-        """(live list entries, second-list entries) of the last J state."""
+        """(list entries, second-list entries) of the last J state."""
         a, b = C.c_int64(), C.c_int64()
         check(lib().mapp_dmd_J_entries(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value

@@ -260,6 +260,19 @@ __device__ inline void femb5(const DmdArgs& A, int e, double rho, double& F, dou
     if (rho > A.rho_max) F += dF * (rho - A.rho_max);
 }
 
+// F''(rho) on the interval femb5 uses (no second derivative past rho_max)
+__device__ inline double ddemb5(const DmdArgs& A, int e, double rho)
+{
+    double p = rho * A.drho_inv;
+    int m = (int)p;
+    m = m < A.nrho - 2 ? m : A.nrho - 2;
+    m = m > 0 ? m : 0;
+    p -= (double)m;
+    p = p < 1.0 ? p : 1.0;
+    const double* cf = A.F5 + ((size_t)e * A.nrho + m) * 5;
+    return ((12.0 * cf[4] * p + 6.0 * cf[3]) * p + 2.0 * cf[2]) * A.drho_inv * A.drho_inv;
+}
+
 __device__ inline double xlogx(double v) { return v == 0.0 ? 0.0 : v * log(v); }
 
 // pass 1: rho per component, F(rho), F'(rho); free energy and entropy sums.
@@ -527,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass2q(int n, const double4* __r
                                                        const double* __restrict__ E, const double* __restrict__ dE,
                                                        double4* __restrict__ f, double* __restrict__ fa,
                                                        double* __restrict__ mu, double* __restrict__ part,
-                                                       double4* __restrict__ gu, double* __restrict__ gau)
+                                                       double4* __restrict__ gu, double* __restrict__ gau, JState J)
 {
     const int blk = xcd_block();
     const int i = blk * kBlock + threadIdx.x;
@@ -549,7 +562,11 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass2q(int n, const double4* __r
             if (ci < 0.0) {
                 fa[ci_idx] = 0.0;
                 mu[ci_idx] = 0.0;
-                if (UNIT) gau[ci_idx] = 0.0;
+                if (UNIT) {
+                    gau[ci_idx] = 0.0;
+                    J.cnt[ci_idx] = 0;
+                    J.ddE[ci_idx] = 0.0;
+                }
                 continue;
             }
             const double ai = al[ci_idx];
@@ -558,7 +575,7 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass2q(int n, const double4* __r
             // fcoef: 1 with one component; else c_i / sum of positive c, or
             // 1 for the zero components of an empty site
             const double fcoef = C == 1 ? 1.0 : (c_tot == 0.0 ? (ci == 0.0 ? 1.0 : 0.0) : ci / c_tot);
-            double fai = 0.0, mui = 0.0, gai = 0.0;
+            double fai = 0.0, mui = 0.0, gai = 0.0, rho = 0.0;
             const int cnt = nn[ci_idx];
             for (int k0 = 0; k0 < cnt; k0 += B) {
                 uint32_t e[B];
@@ -632,6 +649,14 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass2q(int n, const double4* __r
                     fx += fp * dx;
                     fy += fp * dy;
                     fz += fp * dz;
+                    if (UNIT && k0 + b < cnt) {  // the Jacobian's slot values (as k_dmd_jrp)
+                        const size_t slot = (size_t)(k0 + b) * ldl + ci_idx;
+                        J.idx[slot] = e[b] & kIndexMask;
+                        J.r2[slot] = live[b] ? rp[2] : 0.0;
+                        J.r1[slot] = live[b] ? rp[1] : 0.0;
+                        J.t0[slot] = live[b] ? rp[0] + dEi * rp[2] + dEj[b] * rp[1] : 0.0;
+                        rho += live[b] ? cj[b] * rp[2] : 0.0;
+                    }
                     if (UNIT) {
                         const double gp = live[b] ? (drr[2] * dEi + drr[1] * dEj[b] + drr[0]) * cj[b] * fcoef : 0.0;
                         gx += gp * dx;
@@ -650,7 +675,11 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass2q(int n, const double4* __r
             }
             fa[ci_idx] = fai + 3.0 * A.kBT * ci / ai;
             mu[ci_idx] = E[ci_idx] + mui;
-            if (UNIT) gau[ci_idx] = gai - 3.0 * A.kBT / ai;
+            if (UNIT) {
+                gau[ci_idx] = gai - 3.0 * A.kBT / ai;
+                J.cnt[ci_idx] = cnt;
+                J.ddE[ci_idx] = ddemb5(A, ei, rho);
+            }
         }
         f[i] = make_double4(fx, fy, fz, 0.0);
         if (UNIT) gu[i] = make_double4(gx, gy, gz, 0.0);
@@ -749,9 +778,10 @@ __global__ __launch_bounds__(kBlock) void k_dmd_cd(int nc, const double4* __rest
 // src/ff_eam_dmd.cpp:1475-1575) at a fixed state. The state kernels below
 // run once per state and keep what every product reads, so a product is
 // three gathers:
-//  * the live entries of every component's list (pairs inside the smeared
-//    cut; the rest have zero rho_phi), compacted slot-major: the partner,
-//    rho_{j->i}, rho_{i->j} and t0 = r phi + dE_i rho_{j->i} + dE_j rho_{i->j};
+//  * every entry of every component's list, slot-major at its list position
+//    (so a wave stores one row at a time): the partner, rho_{j->i},
+//    rho_{i->j} and t0 = r phi + dE_i rho_{j->i} + dE_j rho_{i->j}, zero past
+//    the smeared cut. c_d's second pass writes them; k_dmd_jrp otherwise;
 //  * the second list (same element, r < r_crd) with the exchange
 //    coefficients of src/ff_eam_dmd.cpp:1430-1462 folded in: per entry
 //    f0 = coef0 M0, f1 = coef0 d_i c_i, f2 = coef0 d_j cv_i, and per component
@@ -759,7 +789,7 @@ __global__ __launch_bounds__(kBlock) void k_dmd_cd(int nc, const double4* __rest
 //    Ax_i = S1 x_i + S2 xs_i - S0 b0_i + sum_j (f0 b0_j - f1 xs_j - f2 x_j)
 //    with xs the sum of x over a site's present components.
 
-// quadrature values of every live list entry, rho and F''(rho)
+// quadrature values of every list entry, rho and F''(rho)
 template <int B>
 __global__ __launch_bounds__(kBlock) void k_dmd_jrp(int n, const double4* __restrict__ x, const double* __restrict__ al,
                                                     const double* __restrict__ cc, const int* __restrict__ el,
@@ -782,7 +812,6 @@ __global__ __launch_bounds__(kBlock) void k_dmd_jrp(int n, const double4* __rest
         const double ai = al[ci_idx], dEi = dE[ci_idx];
         const int ei = el[ci_idx];
         double rho = 0.0;
-        int live_n = 0;
         const int cnt = nn[ci_idx];
         for (int k0 = 0; k0 < cnt; k0 += B) {
             uint32_t e[B];
@@ -829,22 +858,21 @@ __global__ __launch_bounds__(kBlock) void k_dmd_jrp(int n, const double4* __rest
             }
 #pragma unroll
             for (int b = 0; b < B; ++b) {
-                if (!live[b]) continue;
+                if (k0 + b >= cnt) continue;
                 const int jc = (int)(e[b] & kIndexMask);
                 const double rinv = 1.0 / pg[b].r;
-                const double v0 = kPiInSq * rinv * H[b][0];
-                const double v1 = kPiInSq * rinv * H[b][1];
-                const double v2 = kPiInSq * rinv * H[b][2];
-                const size_t slot = (size_t)live_n * ldl + ci_idx;
+                const double v0 = live[b] ? kPiInSq * rinv * H[b][0] : 0.0;
+                const double v1 = live[b] ? kPiInSq * rinv * H[b][1] : 0.0;
+                const double v2 = live[b] ? kPiInSq * rinv * H[b][2] : 0.0;
+                const size_t slot = (size_t)(k0 + b) * ldl + ci_idx;
                 J.idx[slot] = (uint32_t)jc;
                 J.r2[slot] = v2;
                 J.r1[slot] = v1;
                 J.t0[slot] = v0 + dEi * v2 + dE[jc] * v1;
-                ++live_n;
                 rho += cj[b] * v2;
             }
         }
-        J.cnt[ci_idx] = live_n;
+        J.cnt[ci_idx] = cnt;
         // F''(rho) on the interval femb5 uses (no second derivative past rho_max)
         double p = rho * A.drho_inv;
         int m = (int)p;
@@ -1790,25 +1818,27 @@ void DMDSystem::pass2(bool unit)
         hipLaunchKernelGGL(k_dmd_pass2<true>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
                            el_.get(), nn_.get(), list_.get(), ldl_, A_, bd, S, E_.get(), dE_.get(), f_.get(),
                            fa_.get(), mu_.get(), p2);
-    else if (batch_ == 1)
-        hipLaunchKernelGGL(k_dmd_pass2<false>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
-                           el_.get(), nn_.get(), list_.get(), ldl_, A_, bd, S, E_.get(), dE_.get(), f_.get(),
-                           fa_.get(), mu_.get(), p2);
     else if (unit) {
         gu_.reserve(n_);
         gau_.reserve(nc_);
+        j_reserve();
         hipLaunchKernelGGL((k_dmd_pass2q<4, true>), dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(),
                            c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, E_.get(), dE_.get(),
-                           f_.get(), fa_.get(), mu_.get(), p2, gu_.get(), gau_.get());
+                           f_.get(), fa_.get(), mu_.get(), p2, gu_.get(), gau_.get(), jstate());
         gu_ver_ = ver_;
-    } else if (batch_ == 2)
+        jslot_ver_ = ver_;
+    } else if (batch_ == 1)
+        hipLaunchKernelGGL(k_dmd_pass2<false>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
+                           el_.get(), nn_.get(), list_.get(), ldl_, A_, bd, S, E_.get(), dE_.get(), f_.get(),
+                           fa_.get(), mu_.get(), p2);
+    else if (batch_ == 2)
         hipLaunchKernelGGL((k_dmd_pass2q<2, false>), dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(),
                            c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, E_.get(), dE_.get(),
-                           f_.get(), fa_.get(), mu_.get(), p2, nullptr, nullptr);
+                           f_.get(), fa_.get(), mu_.get(), p2, nullptr, nullptr, JState{});
     else
         hipLaunchKernelGGL((k_dmd_pass2q<4, false>), dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(),
                            c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, E_.get(), dE_.get(),
-                           f_.get(), fa_.get(), mu_.get(), p2, nullptr, nullptr);
+                           f_.get(), fa_.get(), mu_.get(), p2, nullptr, nullptr, JState{});
     prof_.end(stream_);
     reduce_rows(p2, nb, 6, red_.get() + 8);
 }
@@ -1875,14 +1905,8 @@ JState DMDSystem::jstate()
                   jidx2_.get(), jf0_.get(), jf1_.get(), jf2_.get(), jS_.get(), jcnt2_.get()};
 }
 
-void DMDSystem::J_prepare()
+void DMDSystem::j_reserve()
 {
-    MAPP_REQUIRE(jver_ >= 0 && jver_ == ver_, ERR_STATE,
-                 "J needs c_d at the current state (quadrature, no surrogate) first");
-    if (jrp_ver_ == ver_) return;
-    BoxDev bd;
-    std::memcpy(bd.h, box_.h, sizeof bd.h);
-    std::memcpy(bd.b, box_.b, sizeof bd.b);
     const size_t slots = (size_t)std::max(maxn_, 1) * ldl_;
     jidx_.reserve(slots);
     jr2_.reserve(slots);
@@ -1899,9 +1923,21 @@ void DMDSystem::J_prepare()
     ja1_.reserve(nc_);
     jb0_.reserve(nc_);
     jxs_.reserve(n_);
+}
+
+void DMDSystem::J_prepare()
+{
+    MAPP_REQUIRE(jver_ >= 0 && jver_ == ver_, ERR_STATE,
+                 "J needs c_d at the current state (quadrature, no surrogate) first");
+    if (jrp_ver_ == ver_) return;
+    BoxDev bd;
+    std::memcpy(bd.h, box_.h, sizeof bd.h);
+    std::memcpy(bd.b, box_.b, sizeof bd.b);
+    j_reserve();
     prof_.begin(K_DMD_JSTATE, stream_);
-    hipLaunchKernelGGL(k_dmd_jrp<4>, dim3(grid_for(n_)), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
-                       el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, jdE_.get(), jstate());
+    if (jslot_ver_ != ver_)  // the slot values come with c_d's second pass; else a pass of their own
+        hipLaunchKernelGGL(k_dmd_jrp<4>, dim3(grid_for(n_)), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(),
+                           c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, jdE_.get(), jstate());
     hipLaunchKernelGGL(k_dmd_jex, dim3(grid_for(nc_)), dim3(kBlock), 0, stream_, nc_, x_.get(), al_.get(), c_.get(),
                        el_.get(), nn_.get(), list_.get(), ldl_, C_, bd, jP_, jmu_.get(), jstate());
     prof_.end(stream_);

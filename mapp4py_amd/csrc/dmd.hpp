@@ -48,11 +48,11 @@ struct CGReport {
 
 // Arrays of the Jacobian-product state, as the kernels take them
 struct JState {
-    uint32_t* idx;   // [maxn][ldl] partner component of a live entry
+    uint32_t* idx;   // [maxn][ldl] partner component of a list entry
     double* r2;      // rho_{j->i}
     double* r1;      // rho_{i->j}
     double* t0;      // r phi + dE_i rho_{j->i} + dE_j rho_{i->j}
-    int* cnt;        // [nc] live entries
+    int* cnt;        // [nc] list entries
     double* ddE;     // [nc] F''(rho)
     uint32_t* idx2;  // [maxn][ldl] second-list partner
     double *f0, *f1, *f2;  // exchange coefficients of each second-list entry
@@ -103,7 +103,7 @@ public:
     // otherwise c is restored. iters = Newton steps as the reference counts them.
     bool newton(const double* r_crd, const double* zeta, double beta, const double* a, double a_tol,
                 double err_fac, int max_newton, int max_gmres, int* iters);
-    // entries the last J state holds: live list entries and second-list entries
+    // entries the last J state holds: list entries and second-list entries
     void J_entries(int64_t* live, int64_t* exchange);
     int64_t half_pairs_count();
     void half_pairs(int32_t* i, int32_t* j, int8_t* img);
@@ -130,6 +130,7 @@ private:
     bool decide();
     void reduce_rows(const double* part, int nrows, int ncols, double* out);
     void J_prepare();
+    void j_reserve();
     void J_dev(const double* x, double* ax);
     JState jstate();
 
@@ -150,13 +151,13 @@ private:
     DevBuf<double> fa_, mu_, E_, dE_;
     DevBuf<double> cd_;              // concentration rates per component
     // Jacobian state: bumped by every change of x, alpha, c or the slot order
-    int64_t ver_ = 0, jver_ = -1, jrp_ver_ = -1;
+    int64_t ver_ = 0, jver_ = -1, jrp_ver_ = -1, jslot_ver_ = -1;
     CdArgs jP_{};
     DevBuf<double4> gu_;   // gradient per unit c of the last c_d, per site
     DevBuf<double> gau_;   // and per component (alpha)
     int64_t gu_ver_ = -1;
     DevBuf<double> jmu_, jdE_, jddE_;
-    DevBuf<uint32_t> jidx_, jidx2_;   // live list entries / second list, slot-major
+    DevBuf<uint32_t> jidx_, jidx2_;   // list entries / second list, slot-major
     DevBuf<double> jr2_, jr1_, jt0_, jf0_, jf1_, jf2_, jS_;
     DevBuf<int> jcnt_, jcnt2_;
     DevBuf<double> jx_, ja1_, jb0_, jax_, jxs_;
