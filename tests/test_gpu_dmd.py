@@ -245,3 +245,40 @@ def test_unit_gradient_matches_oracle(M, cuh):
     g.set_state(*g.state())
     with pytest.raises(RuntimeError):
         g.unit_gradient()
+
+
+def test_absent_components_match_oracle(M, cuh):
+    # a quarter of the sites without their H component (c = -1): no list rows
+    # or entries for it, zero rates, Jacobian rows and gradients there, and the
+    # implicit step leaves it at -1
+    x, H, elem, alpha, c = sites((3, 3, 3), 21)
+    c, alpha = c.copy(), alpha.copy()
+    gone = np.random.default_rng(22).uniform(size=len(c)) < 0.25
+    c[gone, 1] = -1.0
+    alpha[gone, 1] = 0.0
+    g, o = both(M, cuh, x, H, elem, alpha, c)
+    assert triples(*g.half_pairs()) == triples(*o.pairs())
+    f, fa, mu, fe, FH, S = g.derivative()
+    fo, fao, muo, feo, FHo, So = o.force()
+    # absent components: the reference adds 3 kT c / alpha to their f_alpha
+    # (-inf for the importer's alpha = 0); the GPU leaves 0 there
+    assert np.all(fa[gone, 1] == 0.0) and np.all(np.isinf(fao[gone, 1]))
+    fao = np.where(np.isinf(fao), 0.0, fao)
+    errs = [close(f, fo, "f"), close(fa, fao, "f_alpha"), close(mu, muo, "mu"), close([fe], [feo], "fe")]
+    r_crd, zeta = [3.5, 3.5], [1.7, 1.7]
+    cd, _ = g.c_d(r_crd, zeta)
+    cdo, normo = o.c_d(r_crd, zeta, cuh.masses)
+    assert np.abs(cd - cdo).max() < RTOL * normo and np.all(cd[gone, 1] == 0.0)
+    gu, gau = g.unit_gradient()
+    ou, oau = o.unit_grad()
+    assert np.abs(gu - ou).max() < RTOL * np.abs(ou).max() and np.abs(gau - oau).max() < RTOL * np.abs(oau).max()
+    v = np.random.default_rng(23).normal(0.0, 1.0, c.shape)
+    Jv, Jvo = g.J(v), o.J(r_crd, zeta, cuh.masses, v)
+    assert np.abs(Jv - Jvo).max() < RTOL * np.abs(Jvo).max() and np.all(Jv[gone, 1] == 0.0)
+    ok, it = g.newton(r_crd, zeta, 30.0, -c / 30.0, a_tol=1e-8)
+    oko, ito, co = o.newton(r_crd, zeta, cuh.masses, 30.0, -c / 30.0, a_tol=1e-8)
+    cg = g.state()[2]
+    say(f"absent components: f/f_a/mu/fe {max(errs):.1e}, c_d, unit gradient and J within {RTOL}; implicit step "
+        f"{ok}/{oko} in {it}/{ito}, max|dc| vs oracle {np.abs(cg - co).max():.1e}")
+    assert ok and oko and it == ito
+    assert np.abs(cg - co).max() < 1e-6 * np.abs(co - c).max() and np.all(cg[gone, 1] == -1.0)
