@@ -1184,6 +1184,35 @@ __global__ __launch_bounds__(kBlock) void k_dmd_decide(int n, int C, const doubl
     if (dr + da * scale > half_skin) atomicOr(&flags[1], 1);
 }
 
+// CG vectors after a rebuild inside a minimisation: slot p now holds the site
+// that was at slot perm[p]. Directions and forces move with the site; the
+// line-search origin x0 also takes the site's wrap into the box (the
+// difference of its new and old position), so x0 + t h stays consistent with
+// the new list's image codes.
+__global__ __launch_bounds__(kBlock) void k_dcg_follow(int n, int C, const int* __restrict__ perm,
+                                                       const double4* __restrict__ x_new,
+                                                       const double4* __restrict__ x_old,
+                                                       const double4* __restrict__ x0, const double4* __restrict__ h,
+                                                       const double4* __restrict__ f0, const double* __restrict__ a0,
+                                                       const double* __restrict__ ha, const double* __restrict__ fa0,
+                                                       double4* __restrict__ x0_o, double4* __restrict__ h_o,
+                                                       double4* __restrict__ f0_o, double* __restrict__ a0_o,
+                                                       double* __restrict__ ha_o, double* __restrict__ fa0_o)
+{
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    const int o = perm[p];
+    const double4 xn = x_new[p], xo = x_old[o], b = x0[o];
+    x0_o[p] = make_double4(b.x + (xn.x - xo.x), b.y + (xn.y - xo.y), b.z + (xn.z - xo.z), b.w);
+    h_o[p] = h[o];
+    f0_o[p] = f0[o];
+    for (int k = 0; k < C; ++k) {
+        a0_o[p * C + k] = a0[o * C + k];
+        ha_o[p * C + k] = ha[o * C + k];
+        fa0_o[p * C + k] = fa0[o * C + k];
+    }
+}
+
 // ---------------- CG vector kernels ----------------
 
 // partial dot products over the x part (slots) and the alpha part (components)
@@ -2398,6 +2427,26 @@ CGReport DMDSystem::cg(int max_iter, double e_tol, double max_dx, double max_dal
         MAPP_HIP(hipMemcpyAsync(dx.get(), sx, sizeof(double4) * n_, hipMemcpyDeviceToDevice, stream_));
         MAPP_HIP(hipMemcpyAsync(da.get(), sa, sizeof(double) * nc_, hipMemcpyDeviceToDevice, stream_));
     };
+    // the list update of a trial point; after a rebuild the CG vectors follow
+    // their sites into the new slot order
+    auto move_update = [&]() {
+        if (!update()) return;
+        DevBuf<double4> x0n, hn, f0n;
+        DevBuf<double> a0n, han, fa0n;
+        x0n.reserve(n_); hn.reserve(n_); f0n.reserve(n_);
+        a0n.reserve(nc_); han.reserve(nc_); fa0n.reserve(nc_);
+        hipLaunchKernelGGL(k_dcg_follow, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n_, C_,
+                           vals_alt_.get(), x_.get(), x_alt_.get(), cg_x0_.get(), cg_h_.get(), cg_f0_.get(),
+                           cg_a0_.get(), cg_ha_.get(), cg_fa0_.get(), x0n.get(), hn.get(), f0n.get(), a0n.get(),
+                           han.get(), fa0n.get());
+        auto take = [](auto& a, auto& b) {
+            std::swap(a.p, b.p);
+            std::swap(a.n, b.n);
+        };
+        take(cg_x0_, x0n); take(cg_h_, hn); take(cg_f0_, f0n);
+        take(cg_a0_, a0n); take(cg_ha_, han); take(cg_fa0_, fa0n);
+        MAPP_HIP(hipStreamSynchronize(stream_));
+    };
     CGReport rep{};
     const int reb0 = nrebuild_;
     DMDReport d = derivative(false, nullptr);
@@ -2440,7 +2489,7 @@ CGReport DMDSystem::cg(int max_iter, double e_tol, double max_dx, double max_dal
             const double ideal = e_curr + t * 0.4 * dfa;
             hipLaunchKernelGGL(k_cg_move, dim3(nbe), dim3(kBlock), 0, stream_, n_, nc_, t, cg_x0_.get(), cg_h_.get(),
                                cg_a0_.get(), cg_ha_.get(), c_.get(), x_.get(), al_.get());
-            update();
+            move_update();
             const double e = value(false, nullptr);
             ++rep.evals;
             if (e <= ideal) {
@@ -2454,7 +2503,7 @@ CGReport DMDSystem::cg(int max_iter, double e_tol, double max_dx, double max_dal
         if (!ok) {
             hipLaunchKernelGGL(k_cg_move, dim3(nbe), dim3(kBlock), 0, stream_, n_, nc_, 0.0, cg_x0_.get(),
                                cg_h_.get(), cg_a0_.get(), cg_ha_.get(), c_.get(), x_.get(), al_.get());
-            update();
+            move_update();
             status = 4;
             break;
         }

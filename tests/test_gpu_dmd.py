@@ -282,3 +282,28 @@ def test_absent_components_match_oracle(M, cuh):
         f"{ok}/{oko} in {it}/{ito}, max|dc| vs oracle {np.abs(cg - co).max():.1e}")
     assert ok and oko and it == ito
     assert np.abs(cg - co).max() < 1e-6 * np.abs(co - c).max() and np.all(cg[gone, 1] == -1.0)
+
+
+def test_cg_through_list_rebuilds(M, cuh):
+    # with a tiny skin the CG's trial points trigger many rebuilds, which
+    # reorder the sites; the minimisation must follow them and land where a run
+    # with few rebuilds (large skin; alpha growth still triggers some) lands
+    x, H, elem, alpha, c = sites((3, 3, 3), 31)
+    x = x + np.random.default_rng(32).normal(0, 0.12, x.shape)
+    out = []
+    for skin in (0.02, 1.0):
+        g = M.AtomsDMD(M.ForceFieldEAM.from_knots(cuh, dmd=True), H, x, elem, alpha, c, nquad=3, skin=skin,
+                       masses=cuh.masses)
+        g.profile(True)
+        r = g.min_cg(max_iter=40, e_tol=0.0, max_dx=0.1, grad_tol=0.0)
+        rebuilds = g.profile_read("cell_sort")[1]
+        g.profile(False)
+        out.append((r, rebuilds, g.state()))
+    (ra, na, sa), (rb, nb, sb) = out
+    say(f"CG 40 iterations: skin 0.02 {na} rebuilds fe {ra['fe']:.10f}; skin 1.0 {nb} rebuilds fe {rb['fe']:.10f}")
+    assert na > 3 * max(nb, 1)
+    assert ra["iters"] == rb["iters"] and abs(ra["fe"] - rb["fe"]) < 1e-8 * abs(rb["fe"])
+    assert ra["fe"] < ra["fe0"] - 1.0
+    ds = (sa[0] - sb[0]) @ np.linalg.inv(H)
+    ds -= np.round(ds)
+    assert np.abs(ds @ H).max() < 1e-6 and np.abs(sa[1] - sb[1]).max() < 1e-7
