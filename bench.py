@@ -138,6 +138,83 @@ def feni_run(steps=1000, warmup=20):
             "kernel_ms": {k: ms / max(c, 1) for k, (ms, c) in kern.items()}, "dtype": "f32+f64"}
 
 
+# minimiser vector kernels, bytes per atom as laid out (double4 rows of 32 B,
+# float4 forces of 16 B): one L-BFGS direction with two stored pairs (begin
+# 176, four two-loop passes 144 + 112 + 144 + 112), the pair update 144, and
+# 128 per line-search trial point (x0, h, build positions read, x written)
+MIN_VEC_BYTES_PER_ATOM_STEP = 832
+MIN_VEC_BYTES_PER_ATOM_EVAL = 128
+
+
+def min_run(steps=30):
+    """MinLBFGS (m = 2, back-tracking line search, e_tol 0 so every step is
+    taken) on fcc Cu 64^3 cells less 0.1% vacancies (1,047,528 atoms),
+    positions jittered by 0.1 A, Johnson setfl Nr = Nrho = 10000, skin 0.5 A."""
+    import mapp4py_amd as M
+    from mapp4py_amd import potentials as pot
+
+    p = pot.johnson_cu(nr=10000, nrho=10000)
+    x, H = pot.fcc(3.615, (64, 64, 64))
+    rng = np.random.default_rng(41)
+    x = np.delete(x, rng.choice(len(x), len(x) // 1000, replace=False), axis=0)
+    x = x + rng.normal(0.0, 0.1, x.shape)
+    elem = np.zeros(len(x), np.uint8)
+    md = M.AtomsMD(M.ForceFieldEAM.from_knots(p), H, x, elem, skin=0.5)
+    M.MinLBFGS(m=2, e_tol=0.0).run(md, 3)  # warm-up: list, tables, code objects
+    md.x = x
+    md.profile(True)
+    mn = M.MinLBFGS(m=2, e_tol=0.0)
+    t0 = time.perf_counter()
+    r = mn.run(md, steps)
+    wall = time.perf_counter() - t0
+    groups = ("eam_rho", "eam_force", "eam_energy", "build_list", "cell_sort", "md_min_vec")
+    kern = {k: md.profile_read(k) for k in groups}
+    md.profile(False)
+    n = len(x)
+    vec_ms = kern["md_min_vec"][0]
+    vec_bytes = n * (MIN_VEC_BYTES_PER_ATOM_STEP * r["steps"] + MIN_VEC_BYTES_PER_ATOM_EVAL * r["evals"])
+    vec_gbps = vec_bytes / (vec_ms / 1e3) / 1e9 if vec_ms > 0 else 0.0
+    return {"workload": f"MinLBFGS m=2 (ls_bt c=0.4 rho=0.5), fcc Cu 64^3 cells less 0.1% vacancies ({n} atoms), "
+                        "0.1 A jitter, Johnson setfl Nr=Nrho=10000, skin 0.5 A, e_tol 0",
+            "metric": "minimiser atom-steps/s", "value": n * r["steps"] / wall, "unit": "atom-steps/s",
+            "steps": r["steps"], "ms_per_step": wall / max(r["steps"], 1) * 1e3, "energy_evals": r["evals"],
+            "force_calcs": r["force_calcs"], "rebuilds": r["rebuilds"], "status": r["message"],
+            "e0_per_atom_eV": r["e0"] / n, "e_per_atom_eV": r["e"] / n, "f_max_eV_per_A": r["f_max"],
+            "kernel_ms_total": {k: ms for k, (ms, c) in kern.items()},
+            "kernel_launches": {k: c for k, (ms, c) in kern.items()},
+            "vector_kernels": {"bound": "hbm", "achieved": vec_gbps, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                               "frac": vec_gbps * 1e9 / HBM_PEAK, "algorithmic_bytes": vec_bytes},
+            "dtype": "f64 vectors, f32 forces"}
+
+
+def min_cpu_baseline(seconds):
+    """The oracle's MinLBFGS (oracle/o_min.py over the FP64 C oracle's EAM) on
+    fcc Cu 20^3 cells less 0.1% vacancies, 1 thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import o_min
+    import oracle_lib as O
+    from mapp4py_amd import potentials as pot
+
+    p = pot.johnson_cu(nr=10000, nrho=10000)
+    x, H = pot.fcc(3.615, (20, 20, 20))
+    rng = np.random.default_rng(41)
+    x = np.delete(x, rng.choice(len(x), len(x) // 1000, replace=False), axis=0)
+    x = x + rng.normal(0.0, 0.1, x.shape)
+    elem = np.zeros(len(x), np.uint8)
+    o = O.OMD(O.OEam.from_knots(p), H, x, elem, skin=0.5)
+    t0 = time.perf_counter()
+    o.force()
+    per = time.perf_counter() - t0
+    steps = max(2, int(seconds / (3.0 * max(per, 1e-6))))
+    t0 = time.perf_counter()
+    r = o_min.min_lbfgs(o, steps, m=2, e_tol=0.0)
+    dt = time.perf_counter() - t0
+    return {"value": len(x) * r["steps"] / dt, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle MinLBFGS m=2 over the FP64 C restatement, fcc Cu 20^3 cells less 0.1% vacancies "
+                      f"= {len(x)} atoms, {r['steps']} steps, 1 thread, {dt:.1f} s"}
+
+
 # algorithmic bytes of one __J product: per list entry the partner index and
 # rho_{j->i} (first sum) and the index, rho_{i->j} and t0 (second); per
 # second-list entry the index and three coefficients; per component the
@@ -567,6 +644,7 @@ def main():
     ap.add_argument("--no-dmd", action="store_true", help="skip the DMD measurement")
     ap.add_argument("--no-gcmc", action="store_true", help="skip the batched GCMC measurement")
     ap.add_argument("--no-feni", action="store_true", help="skip the Fe-Ni (two-element) MD measurement")
+    ap.add_argument("--no-min", action="store_true", help="skip the L-BFGS minimisation measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel (256 MiB, 20 launches) used to calibrate HBM counters")
     args = ap.parse_args()
@@ -601,7 +679,14 @@ def main():
         log(f"[bench] cpu baseline (~{args.cpu_seconds:.0f} s of oracle work)")
         cpu = cpu_baseline(args.cpu_seconds)
         log(f"[bench] cpu baseline {cpu['value']:.3e} atom-steps/s")
-    dmd = gcmc = feni = None
+    dmd = gcmc = feni = mini = None
+    if not args.no_min:
+        log("[bench] L-BFGS minimisation")
+        mini = min_run()
+        if not args.no_cpu:
+            mini["cpu_baseline"] = min_cpu_baseline(args.cpu_seconds / 3)
+        log(f"[bench] L-BFGS {mini['value']:.3e} atom-steps/s ({mini['ms_per_step']:.3f} ms/step, "
+            f"{mini['energy_evals']} energy evals, {mini['rebuilds']} rebuilds)")
     if not args.no_feni:
         log("[bench] Fe-Ni")
         feni = feni_run()
@@ -634,7 +719,7 @@ def main():
         "config": {"workload": f"EAM NVT, fcc Cu {args.cells}^3 cells ({n} atoms), Johnson setfl Nr=Nrho=10000, "
                                f"r_c=4.95 A, skin 0.5 A, dt 1 fs, 300 K", "natoms": n, "rebuilds": nvt.rebuilds,
                    "pe_per_atom_eV": out[0] / n, "T_K": nvt.T_part},
-        "roofline": roof, "cpu_baseline": cpu, "feni": feni, "dmd": dmd, "gcmc": gcmc,
+        "roofline": roof, "cpu_baseline": cpu, "feni": feni, "minimize": mini, "dmd": dmd, "gcmc": gcmc,
     }
     print(json.dumps(line), flush=True)
 

@@ -97,6 +97,28 @@ mapp_status mapp_md_list_stats(mapp_md* md, int* max_neighbors, int64_t* full_en
  * each step, neighbor rebuild on the skin criterion. masses[ne]. */
 mapp_status mapp_md_nvt_run(mapp_md* md, const double* masses, double kB, double T, double dt, int nsteps,
                             double* out7, double* T_part, int* rebuilds);
+/* Energy minimisation of the positions: MinCG::__run (src/min_cg.h:453-533,
+ * lbfgs = 0) or MinLBFGS::__run (src/min_l-bfgs.h:114-217, lbfgs = 1, memory
+ * m), each with LineSearchBackTrack::min (src/ls.h:711-769; c, rho,
+ * min_alpha as in src/ls.cpp:456-461) and MinMDHandler::ls_prep
+ * (src/min_cg.h:231-257: no step component larger than max_dx). The box stays
+ * fixed (H_dof all False) and displacements are per atom (affine = False).
+ * status follows the reference's err_msgs order (src/min.cpp:7-25): 1 not
+ * downhill, 2 zero gradient, 3 minimum alpha, 4 energy tolerance, 5 maximum
+ * iterations. Rows (step, energy), the start and every accepted step, go to
+ * tally [max_tally_rows][2]; *tally_rows counts the rows written. */
+typedef struct {
+    int lbfgs, m;
+    double e_tol, max_dx;
+    int max_nsteps;
+    double ls_c, ls_rho, ls_min_alpha;
+} mapp_min_params;
+typedef struct {
+    int steps, status, evals, force_calcs, rebuilds;
+    double e0, e, f_norm, f_max;
+} mapp_min_report;
+mapp_status mapp_md_minimize(mapp_md* md, const mapp_min_params* params, mapp_min_report* report, double* tally,
+                             int max_tally_rows, int* tally_rows);
 /* Kernel timing with HIP events on the system's stream (no counterpart in
  * the reference, which has no profiler). enable != 0 starts recording and
  * clears the totals; read returns the summed duration and launch count of a

@@ -8,6 +8,8 @@ The Python layer is a thin mirror of the reference's objects over the C ABI
 * :class:`AtomsMD`        -- md.atoms with its ForceField, NeighborMD and DynamicMD
   (force_calc / value / update of src/ff_md.cpp:125-179, src/dynamic_md.h:38-68)
 * :class:`NVT`            -- md.nvt(T, dt).run(sim, nsteps) (src/md_nvt.cpp:383-480)
+* :class:`MinCG`, :class:`MinLBFGS`, :class:`LineSearchBackTrack` -- md.min_cg, md.min_lbfgs
+  and md.ls_bt (src/min_cg.h, src/min_l-bfgs.h, src/ls.h)
 * :class:`AtomsDMD`       -- dmd.atoms with ForceFieldEAMDMD (src/ff_eam_dmd.cpp, src/ff_dmd.cpp)
 * :class:`GCMCBatch`      -- many independent cells under SGCMC exchange (src/sgcmc.cpp,
   ForceFieldEAM's exchange hooks at src/ff_eam.cpp:323-578)
@@ -21,7 +23,7 @@ import numpy as np
 
 from ._lib import MappError, check, f64, fp, i8p, i32p, i64p, ip, lib, ptr, u8p
 
-__all__ = ["ForceFieldEAM", "AtomsMD", "AtomsDMD", "NVT", "GCMCBatch", "Surrogate", "MappError", "create_temp",
+__all__ = ["ForceFieldEAM", "AtomsMD", "AtomsDMD", "NVT", "MinCG", "MinLBFGS", "LineSearchBackTrack", "GCMCBatch", "Surrogate", "MappError", "create_temp",
            "gauss_hermite", "lib"]
 
 
@@ -216,6 +218,77 @@ class NVT:
         atoms.pe = out[0]
         atoms.F_H = out[1:].copy()
         return out
+
+
+class LineSearchBackTrack:
+    """md.ls_bt: back-tracking line search (src/ls.cpp:456-461 defaults,
+    LineSearchBackTrack::min at src/ls.h:711-769)."""
+
+    def __init__(self, c=0.4, rho=0.5, min_alpha=0.0):
+        if not 0.0 < c < 1.0 or not 0.0 < rho < 1.0 or min_alpha < 0.0:
+            raise ValueError("ls_bt needs 0 < c < 1, 0 < rho < 1 and min_alpha >= 0")
+        self.c, self.rho, self.min_alpha = float(c), float(rho), float(min_alpha)
+
+
+_MIN_MSGS = {1: "line search failed: not downhill direction",
+             2: "line search failed: gradient is zero",
+             3: "line search failed: minimum alpha reached",
+             4: "minimization finished: energy tolerance reached",
+             5: "minimization finished: maximum iteration reached"}
+
+
+class MinCG:
+    """md.min_cg(e_tol, H_dof, affine, max_dx, ls).run(atoms, max_nsteps)
+    (src/min_cg.cpp:180, MinCG::__run at src/min_cg.h:453-533) on the GPU.
+
+    Only the positions are degrees of freedom: H_dof must be all False and
+    affine False (box relaxation is not built)."""
+
+    _lbfgs = 0
+
+    def __init__(self, e_tol=1.0e-8, H_dof=None, affine=False, max_dx=1.0, ls=None):
+        if H_dof is not None and np.any(np.asarray([v for row in H_dof for v in row], dtype=bool)):
+            raise NotImplementedError("box degrees of freedom (H_dof) are not built")
+        if affine:
+            raise NotImplementedError("affine displacements are not built")
+        if ls is None:
+            ls = LineSearchBackTrack()
+        if not isinstance(ls, LineSearchBackTrack):
+            raise NotImplementedError("only the back-tracking line search (ls_bt) is built")
+        if not e_tol >= 0.0 or not max_dx > 0.0:
+            raise ValueError("e_tol must be >= 0 and max_dx > 0")
+        self.e_tol, self.max_dx, self.ls, self.m = float(e_tol), float(max_dx), ls, 0
+        self.ntally = 0
+        self.report = None
+        self.tally = None
+
+    def run(self, atoms: "AtomsMD", max_nsteps):
+        from ._lib import MinParams, MinReport
+        p = MinParams(self._lbfgs, int(self.m), self.e_tol, self.max_dx, int(max_nsteps), self.ls.c, self.ls.rho,
+                      self.ls.min_alpha)
+        r = MinReport()
+        cap = int(max_nsteps) + 1
+        tally = np.zeros((cap, 2))
+        rows = C.c_int()
+        check(lib().mapp_md_minimize(atoms._h, C.byref(p), C.byref(r), ptr(tally), cap, C.byref(rows)))
+        self.tally = tally[:rows.value]
+        self.report = {k: getattr(r, k) for k, _ in MinReport._fields_}
+        self.report["message"] = _MIN_MSGS.get(r.status, "")
+        atoms.pe = r.e
+        return self.report
+
+
+class MinLBFGS(MinCG):
+    """md.min_lbfgs(m, e_tol, H_dof, affine, max_dx, ls).run(atoms, max_nsteps)
+    (src/min_l-bfgs.cpp:162, MinLBFGS::__run at src/min_l-bfgs.h:114-217)."""
+
+    _lbfgs = 1
+
+    def __init__(self, m=2, e_tol=1.0e-8, H_dof=None, affine=False, max_dx=1.0, ls=None):
+        super().__init__(e_tol, H_dof, affine, max_dx, ls)
+        if not 0 <= int(m) <= 64:
+            raise ValueError("m must lie in [0, 64]")
+        self.m = int(m)
 
 
 def create_temp(elem, masses, T, seed, kB=8.617330350e-5):

@@ -51,6 +51,7 @@ SIGNATURES = {
     "mapp_md_pairs": (C.c_int, [C.c_void_p, i32p, i32p, i8p]),
     "mapp_md_list_stats": (C.c_int, [C.c_void_p, ip, i64p, ip]),
     "mapp_md_nvt_run": (C.c_int, [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, ip]),
+    "mapp_md_minimize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, dp, C.c_int, ip]),
     "mapp_md_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "mapp_md_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, dp, i64p]),
     "mapp_md_set_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
@@ -147,6 +148,18 @@ def check(status):
 
 def ptr(a, t=dp):
     return a.ctypes.data_as(t)
+
+
+class MinParams(C.Structure):
+    _fields_ = [("lbfgs", C.c_int), ("m", C.c_int), ("e_tol", C.c_double), ("max_dx", C.c_double),
+                ("max_nsteps", C.c_int), ("ls_c", C.c_double), ("ls_rho", C.c_double),
+                ("ls_min_alpha", C.c_double)]
+
+
+class MinReport(C.Structure):
+    _fields_ = [("steps", C.c_int), ("status", C.c_int), ("evals", C.c_int), ("force_calcs", C.c_int),
+                ("rebuilds", C.c_int), ("e0", C.c_double), ("e", C.c_double), ("f_norm", C.c_double),
+                ("f_max", C.c_double)]
 
 
 class BdfParams(C.Structure):

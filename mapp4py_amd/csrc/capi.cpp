@@ -252,6 +252,41 @@ mapp_status mapp_md_nvt_run(mapp_md* md, const double* masses, double kB, double
     });
 }
 
+mapp_status mapp_md_minimize(mapp_md* md, const mapp_min_params* params, mapp_min_report* report, double* tally,
+                             int max_tally_rows, int* tally_rows)
+{
+    return guard([&] {
+        NEED(md);
+        NEED(params);
+        MAPP_REQUIRE(max_tally_rows >= 0 && (tally || max_tally_rows == 0), mapp::ERR_ARG, "invalid tally buffer");
+        mapp::MinParams p;
+        p.lbfgs = params->lbfgs;
+        p.m = params->m;
+        p.e_tol = params->e_tol;
+        p.max_dx = params->max_dx;
+        p.max_nsteps = params->max_nsteps;
+        p.ls_c = params->ls_c;
+        p.ls_rho = params->ls_rho;
+        p.ls_min_alpha = params->ls_min_alpha;
+        std::vector<double> rows;
+        const mapp::MinReport r = md->sys->minimize(p, tally ? &rows : nullptr);
+        if (report) {
+            report->steps = r.steps;
+            report->status = r.status;
+            report->evals = r.evals;
+            report->force_calcs = r.force_calcs;
+            report->rebuilds = r.rebuilds;
+            report->e0 = r.e0;
+            report->e = r.e;
+            report->f_norm = r.f_norm;
+            report->f_max = r.f_max;
+        }
+        const int nrows = std::min((int)(rows.size() / 2), max_tally_rows);
+        if (tally && nrows) std::memcpy(tally, rows.data(), sizeof(double) * 2 * nrows);
+        if (tally_rows) *tally_rows = nrows;
+    });
+}
+
 mapp_status mapp_md_profile(mapp_md* md, int enable)
 {
     return guard([&] {

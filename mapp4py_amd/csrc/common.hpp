@@ -86,6 +86,7 @@ enum KernelId : int {
     K_DMD_JSTATE,
     K_DMD_J,
     K_DMD_NEWTON,
+    K_MD_MIN,
     K_COUNT
 };
 
@@ -95,7 +96,7 @@ inline const char* kernel_name(int k)
                                          "integrate",  "eam_energy", "dmd_pass1",  "dmd_pass2",
                                          "dmd_energy", "surr_infer", "surr_train", "gcmc",
                                          "dmd_cd",     "dmd_j_state", "dmd_j",
-                                         "dmd_newton"};
+                                         "dmd_newton", "md_min_vec"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "";
 }
 

@@ -73,6 +73,26 @@ struct MDReport {
     double e[7];   // energy, F_H Voigt (xx,xy,xz,yy,yz,zz)
 };
 
+// MinCG / MinLBFGS with LineSearchBackTrack over the atom positions (no box
+// degrees of freedom, affine = False)
+struct MinParams {
+    int lbfgs = 1;        // 0: conjugate gradients, 1: L-BFGS
+    int m = 2;            // L-BFGS memory (pairs of s, y)
+    double e_tol = 1e-8;
+    double max_dx = 1.0;
+    int max_nsteps = 1000;
+    double ls_c = 0.4, ls_rho = 0.5, ls_min_alpha = 0.0;
+};
+
+// status: 0 line search succeeded (not returned), 1 not a downhill direction,
+// 2 zero gradient, 3 minimum step length reached, 4 energy tolerance
+// reached, 5 maximum number of steps (the reference's err_msgs order)
+struct MinReport {
+    int steps = 0, status = 0, evals = 0, force_calcs = 0, rebuilds = 0;
+    double e0 = 0.0, e = 0.0, f_norm = 0.0, f_max = 0.0;
+};
+constexpr int kMaxLbfgsM = 64;
+
 // AtomsMD + NeighborMD + DynamicMD + ForceFieldEAM state resident in HBM.
 // Atoms are kept in cell order ("slots"); id[slot] is the caller's index.
 class MDSystem {
@@ -99,6 +119,9 @@ public:
     void list_stats(int* maxn, int64_t* nfull);
     // MDNVT::run; masses per element; returns rebuild count
     int nvt_run(const double* masses, double kB, double T, double dt, int nsteps, MDReport* rep, double* T_part);
+    // MinCG::run / MinLBFGS::run (src/min_cg.h:453-533, src/min_l-bfgs.h:114-217);
+    // tally rows (step, energy): the start and every accepted step
+    MinReport minimize(const MinParams& p, std::vector<double>* tally);
     hipStream_t stream() const { return stream_; }
     Profiler& profiler() { return prof_; }
     // launch-shape knobs picked by the tuner: neighbors resolved per batch
@@ -171,6 +194,13 @@ private:
     int rho_m_lo_ = 0, rho_nwin_ = 0, f_m_lo_ = 0, f_nwin_ = 0;
     size_t rho_lds_bytes_ = 0, f_lds_bytes_ = 0;
     int mvv_rows_ = 0;               // rows of kinetic partial sums written by the last force pass
+    // minimiser vectors in slot order: line-search origin, direction, forces
+    // at the origin, L-BFGS pairs; a spare for permutations
+    DevBuf<double4> mn_x0_, mn_h_, mn_f0_, mn_tmp_;
+    std::vector<std::unique_ptr<DevBuf<double4>>> mn_s_, mn_y_;
+    DevBuf<double> mn_part_, mn_red_;
+    // the minimiser's vectors follow their atoms into a rebuilt slot order
+    void min_follow(int nvec, DevBuf<double4>** vecs, int nshift);
     void plan_lds();
 };
 

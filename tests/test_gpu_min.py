@@ -1,0 +1,148 @@
+"""GPU energy minimisation (MinCG, MinLBFGS with the back-tracking line
+search) against the oracle's restatement (oracle/o_min.py), through the C ABI.
+
+GPU energies carry the FP32 rounding of the pair spline values (bar of the
+EAM tests: 5e-8 eV per atom + 1e-6 eV) and the forces 1e-5 eV/A, so the two
+runs take the same accepted steps but not bit-equal ones: the bars are the
+same per step energy bar, the same step count and status, and positions
+within 1e-5 A after 8 steps. The FP32 force rounding (about 1e-7 relative)
+tilts every direction slightly and the soft modes around the vacancies
+amplify it, so after 25-30 steps the bar is 5e-4 A (seen up to 1.4e-4 A). Runs to convergence land on the
+oracle's minimum: the oracle's forces at the GPU's final positions are below
+1e-3 eV/A.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+import o_min  # noqa: E402
+import oracle_lib as O  # noqa: E402
+from mapp4py_amd import potentials as pot  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+E_ATOM = 5e-8
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def M():
+    import mapp4py_amd
+
+    return mapp4py_amd
+
+
+@pytest.fixture(scope="module")
+def cu_pot():
+    return pot.johnson_cu(nr=10000, nrho=10000)
+
+
+def cu_sites(cells, seed, sigma):
+    x, H = pot.fcc(3.615, (cells, cells, cells))
+    x = np.delete(x, [5, 77], axis=0)
+    x = x + np.random.default_rng(seed).normal(0.0, sigma, x.shape)
+    return H, x, np.zeros(len(x), np.uint8)
+
+
+def wrapped_diff(a, b, H):
+    L = np.diag(H)
+    return float(np.abs((a - b + 0.5 * L) % L - 0.5 * L).max())
+
+
+def run_gpu(M, which, m, atoms, nsteps, e_tol, max_dx=1.0):
+    mn = M.MinCG(e_tol=e_tol, max_dx=max_dx) if which == "cg" else M.MinLBFGS(m=m, e_tol=e_tol, max_dx=max_dx)
+    return mn.run(atoms, nsteps), mn.tally
+
+
+def run_oracle(which, m, o, nsteps, e_tol, max_dx=1.0):
+    if which == "cg":
+        return o_min.min_cg(o, nsteps, e_tol=e_tol, max_dx=max_dx)
+    return o_min.min_lbfgs(o, nsteps, m=m, e_tol=e_tol, max_dx=max_dx)
+
+
+@pytest.mark.parametrize("which,m", [("lbfgs", 2), ("lbfgs", 5), ("lbfgs", 0), ("cg", 0)])
+def test_steps_follow_the_oracle(M, cu_pot, which, m):
+    H, x, elem = cu_sites(6, 11, 0.1)
+    n = len(x)
+    say(f"minimiser {which} m={m}: {n} atoms, 25 steps")
+    g = M.AtomsMD(M.ForceFieldEAM.from_knots(cu_pot), H, x, elem, skin=0.5)
+    o = O.OMD(O.OEam.from_knots(cu_pot), H, x, elem, skin=0.5)
+    bar = E_ATOM * n + 1e-6
+    g8 = M.AtomsMD(M.ForceFieldEAM.from_knots(cu_pot), H, x, elem, skin=0.5)
+    o8 = O.OMD(O.OEam.from_knots(cu_pot), H, x, elem, skin=0.5)
+    run_gpu(M, which, m, g8, 8, 0.0)
+    run_oracle(which, m, o8, 8, 0.0)
+    dx8 = wrapped_diff(g8.x, o8.x, H)
+    say(f"  after 8 steps max |dx| {dx8:.2e} A")
+    assert dx8 < 1e-5
+    rg, tg = run_gpu(M, which, m, g, 25, 0.0)
+    ro = run_oracle(which, m, o, 25, 0.0)
+    say(f"  gpu steps {rg['steps']} status {rg['status']} e {rg['e']:.10f}; oracle steps {ro['steps']} "
+        f"status {ro['status']} e {ro['e']:.10f}; evals {rg['evals']} rebuilds {rg['rebuilds']}")
+    assert rg["steps"] == ro["steps"] and rg["status"] == ro["status"]
+    assert len(tg) == len(ro["tally"])
+    de = np.abs(tg[:, 1] - ro["tally"][:, 1]).max()
+    dx = wrapped_diff(g.x, o.x, H)
+    say(f"  max |dE| per step {de:.2e} eV (bar {bar:.1e}), max |dx| {dx:.2e} A")
+    assert de < bar
+    assert dx < 5e-4
+    assert rg["e"] < rg["e0"] - 1.0
+
+
+@pytest.mark.parametrize("which", ["lbfgs", "cg"])
+def test_converges_to_the_oracle_minimum(M, cu_pot, which):
+    H, x, elem = cu_sites(5, 12, 0.08)
+    n = len(x)
+    say(f"minimiser {which} to convergence: {n} atoms")
+    g = M.AtomsMD(M.ForceFieldEAM.from_knots(cu_pot), H, x, elem, skin=0.5)
+    o = O.OMD(O.OEam.from_knots(cu_pot), H, x, elem, skin=0.5)
+    rg, _ = run_gpu(M, which, 5, g, 2000, 1e-9)
+    ro = run_oracle(which, 5, o, 2000, 1e-12)
+    o2 = O.OMD(O.OEam.from_knots(cu_pot), H, g.x, elem, skin=0.5)
+    fo = o2.force()[0]
+    say(f"  gpu {rg['steps']} steps status {rg['status']} e {rg['e']:.10f} |f|max {rg['f_max']:.2e}; "
+        f"oracle {ro['steps']} steps e {ro['e']:.10f}; oracle |f|max at gpu x {np.abs(fo).max():.2e}")
+    assert rg["status"] in (3, 4)
+    assert abs(rg["e"] - ro["e"]) < E_ATOM * n + 1e-6
+    assert np.abs(fo).max() < 1e-3
+    assert wrapped_diff(g.x, o.x, H) < 1e-3
+
+
+def test_vectors_follow_atoms_through_rebuilds(M, cu_pot):
+    H, x, elem = cu_sites(6, 13, 0.15)
+    n = len(x)
+    runs = {}
+    for skin in (0.05, 1.0):
+        g = M.AtomsMD(M.ForceFieldEAM.from_knots(cu_pot), H, x, elem, skin=skin)
+        r, t = run_gpu(M, "lbfgs", 3, g, 30, 0.0)
+        runs[skin] = (r, t, g.x)
+        say(f"  skin {skin}: {r['rebuilds']} rebuilds, e {r['e']:.10f}, status {r['status']}")
+    o = O.OMD(O.OEam.from_knots(cu_pot), H, x, elem, skin=0.5)
+    ro = run_oracle("lbfgs", 3, o, 30, 0.0)
+    assert runs[0.05][0]["rebuilds"] > 3 * max(runs[1.0][0]["rebuilds"], 1)
+    for skin, (r, t, xg) in runs.items():
+        assert r["steps"] == ro["steps"]
+        assert np.abs(t[:, 1] - ro["tally"][:, 1]).max() < E_ATOM * n + 1e-6, skin
+        assert wrapped_diff(xg, o.x, H) < 5e-4, skin
+
+
+def test_zero_steps_and_limits(M, cu_pot):
+    H, x, elem = cu_sites(4, 14, 0.05)
+    g = M.AtomsMD(M.ForceFieldEAM.from_knots(cu_pot), H, x, elem, skin=0.5)
+    r, t = run_gpu(M, "lbfgs", 2, g, 0, 1e-8)
+    assert r["status"] == 5 and r["steps"] == 0 and len(t) == 1
+    # the run's fresh list wraps the positions into the box (x -> s -> x)
+    assert float(wrapped_diff(g.x, x, H)) < 1e-12
+    # max_dx bounds the first step's largest component
+    o = O.OMD(O.OEam.from_knots(cu_pot), H, x, elem, skin=0.5)
+    r, t = run_gpu(M, "cg", 0, g, 3, 0.0, max_dx=1e-3)
+    ro = run_oracle("cg", 0, o, 3, 0.0, max_dx=1e-3)
+    assert r["steps"] == ro["steps"] == 3
+    assert np.abs(t[:, 1] - ro["tally"][:, 1]).max() < E_ATOM * len(x) + 1e-6
+    assert wrapped_diff(g.x, x, H) <= 3e-3 * (1 + 1e-12)
