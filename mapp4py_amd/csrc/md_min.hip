@@ -108,10 +108,18 @@ __global__ __launch_bounds__(kBlock) void k_min_begin(int n, int use_ratio, doub
 }
 
 // h = (h + a u) g (u optional). Sums: v.h (v optional), f.h, h.h, f.f; max |h|.
-__global__ __launch_bounds__(kBlock) void k_min_axpy(int n, double a, const double4* __restrict__ u, double g,
+// With d_dot set the coefficient comes from reduced sums already on the
+// device (column 0 of earlier rows), so the two-loop recursion runs without a
+// host round trip: a = rho (d_al[0] if given, else 0) - rho d_dot[0], i.e.
+// alpha_i = -rho_i (s_i.h) in the first loop and -alpha_i - rho_i (y_i.h) in
+// the second (src/min_l-bfgs.h:150-162).
+__global__ __launch_bounds__(kBlock) void k_min_axpy(int n, double a, const double* __restrict__ d_dot,
+                                                     const double* __restrict__ d_al, double rho,
+                                                     const double4* __restrict__ u, double g,
                                                      double4* __restrict__ h, const double4* __restrict__ v,
                                                      const float4* __restrict__ f, double* __restrict__ part)
 {
+    if (d_dot) a = (d_al ? rho * d_al[0] : 0.0) - rho * d_dot[0];
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     double mx = 0.0;
     const int stride = gridDim.x * kBlock;
@@ -168,13 +176,16 @@ __global__ __launch_bounds__(kBlock) void k_min_pair(int n, double t, const doub
 __global__ __launch_bounds__(kBlock) void k_min_move(int n, double t, const double4* __restrict__ x0,
                                                      const double4* __restrict__ h, double4* __restrict__ x,
                                                      const double4* __restrict__ xb, double skin_sq4,
-                                                     int* __restrict__ flags)
+                                                     int* __restrict__ flags, Rec16Frame R, uint4* __restrict__ rec)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const double4 a = x0[i], d = h[i];
     const double4 xi = make_double4(a.x + t * d.x, a.y + t * d.y, a.z + t * d.z, a.w);
     x[i] = xi;
+    // the 16-byte record of the new position (the force passes skip their record pass)
+    rec[i] = make_uint4((unsigned)__double2ll_rn(xi.x * R.inv_q), (unsigned)__double2ll_rn(xi.y * R.inv_q),
+                        (unsigned)__double2ll_rn(xi.z * R.inv_q), (unsigned)(int)xi.w);
     const double4 b = xb[i];
     if (rsq_exact(b.x, b.y, b.z, xi.x, xi.y, xi.z) > skin_sq4) atomicOr(&flags[1], 1);
 }
@@ -238,14 +249,22 @@ MinReport MDSystem::minimize(const MinParams& P, std::vector<double>* tally)
         mn_y_[i]->reserve(n_);
     }
     mn_part_.reserve((size_t)kMinRows * kPartW);
-    mn_red_.reserve(kPartW);
+    mn_red_.reserve((size_t)(2 * kMaxLbfgsM + 4) * kPartW);
     const int G = std::min((n_ + kBlock - 1) / kBlock, kMinRows);
     const dim3 gmove((n_ + kBlock - 1) / kBlock);
     double red[kPartW];
-    auto rows = [&](int max_col) {
-        hipLaunchKernelGGL(k_min_rows, dim3(1), dim3(kBlock), 0, stream_, G, max_col, mn_part_.get(), mn_red_.get());
-        MAPP_HIP(hipMemcpyAsync(red, mn_red_.get(), sizeof red, hipMemcpyDeviceToHost, stream_));
+    // reduce the partial sums into device row r of mn_red_; fetch copies it to red
+    auto rows_dev = [&](int max_col, int r) {
+        hipLaunchKernelGGL(k_min_rows, dim3(1), dim3(kBlock), 0, stream_, G, max_col, mn_part_.get(),
+                           mn_red_.get() + (size_t)r * kPartW);
+    };
+    auto fetch = [&](int r) {
+        MAPP_HIP(hipMemcpyAsync(red, mn_red_.get() + (size_t)r * kPartW, sizeof red, hipMemcpyDeviceToHost, stream_));
         MAPP_HIP(hipStreamSynchronize(stream_));
+    };
+    auto rows = [&](int max_col) {
+        rows_dev(max_col, 0);
+        fetch(0);
     };
     // sums of the begin/axpy kernels
     struct Sums {
@@ -263,12 +282,25 @@ MinReport MDSystem::minimize(const MinParams& P, std::vector<double>* tally)
         prof_.end(stream_);
         return sums4();
     };
-    auto axpy = [&](double a, const DevBuf<double4>* u, double g, const DevBuf<double4>* v) {
+    // h = (h + a u) g with a from device rows (d_dot >= 0) or the host value;
+    // the sums go to device row r
+    auto axpy_dev = [&](double a, int d_dot, int d_al, double rho_i, const DevBuf<double4>* u, double g,
+                        const DevBuf<double4>* v, int r) {
+        const double* red_d = mn_red_.get();
         prof_.begin(K_MD_MIN, stream_);
-        hipLaunchKernelGGL(k_min_axpy, dim3(G), dim3(kBlock), 0, stream_, n_, a, u ? u->get() : nullptr, g,
+        hipLaunchKernelGGL(k_min_axpy, dim3(G), dim3(kBlock), 0, stream_, n_, a,
+                           d_dot >= 0 ? red_d + (size_t)d_dot * kPartW : nullptr,
+                           d_al >= 0 ? red_d + (size_t)d_al * kPartW : nullptr, rho_i, u ? u->get() : nullptr, g,
                            mn_h_.get(), v ? v->get() : nullptr, f_.get(), mn_part_.get());
         prof_.end(stream_);
-        return sums4();
+        rows_dev(4, r);
+    };
+    auto begin_dev = [&](const DevBuf<double4>* v, int r) {
+        prof_.begin(K_MD_MIN, stream_);
+        hipLaunchKernelGGL(k_min_begin, dim3(G), dim3(kBlock), 0, stream_, n_, 0, 0.0, f_.get(), mn_h_.get(),
+                           x_.get(), mn_x0_.get(), mn_f0_.get(), v ? v->get() : nullptr, mn_part_.get());
+        prof_.end(stream_);
+        rows_dev(4, r);
     };
     auto pair = [&](double t, DevBuf<double4>* s, DevBuf<double4>* y) {
         prof_.begin(K_MD_MIN, stream_);
@@ -285,15 +317,27 @@ MinReport MDSystem::minimize(const MinParams& P, std::vector<double>* tally)
     MinReport rep{};
     const int reb0 = nrebuild_;
     // MinMDHandler::F (src/min_cg.h:262-269) and F_reset (:273-278)
+    // The force passes of a trial point are queued behind the move before the
+    // skin check is read back (one host round trip per trial); when the check
+    // asks for a rebuild their result is discarded and they run again on the
+    // new list. On a stale list they read valid slots only.
     auto F = [&](double t, bool energy) {
         MAPP_HIP(hipMemsetAsync(flags_.get() + 1, 0, sizeof(int), stream_));
         prof_.begin(K_MD_MIN, stream_);
         hipLaunchKernelGGL(k_min_move, gmove, dim3(kBlock), 0, stream_, n_, t, mn_x0_.get(), mn_h_.get(), x_.get(),
-                           x0_.get(), 0.25 * skin_ * skin_, flags_.get());
+                           x0_.get(), 0.25 * skin_ * skin_, flags_.get(), r16_, rec_.get());
         prof_.end(stream_);
+        if (energy) force_passes(nullptr, true);
         int flag = 0;
+        double e = 0.0;
         MAPP_HIP(hipMemcpyAsync(&flag, flags_.get() + 1, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        if (energy) MAPP_HIP(hipMemcpyAsync(&e, red_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
         MAPP_HIP(hipStreamSynchronize(stream_));
+        f_fresh = false;
+        if (energy) {
+            ++rep.evals;
+            ++rep.force_calcs;
+        }
         if (flag) {
             rebuild();
             DevBuf<double4>* vecs[3 + 2 * kMaxLbfgsM];
@@ -306,12 +350,15 @@ MinReport MDSystem::minimize(const MinParams& P, std::vector<double>* tally)
                 vecs[nv++] = mn_y_[i].get();
             }
             min_follow(nv, vecs, 1);
+            if (energy) {
+                force_passes(nullptr, true);  // the rebuild wrote the records
+                MAPP_HIP(hipMemcpyAsync(&e, red_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
+                MAPP_HIP(hipStreamSynchronize(stream_));
+                ++rep.force_calcs;
+            }
         }
-        f_fresh = false;
-        if (!energy) return 0.0;
-        ++rep.evals;
-        ++rep.force_calcs;
-        return force_calc().e[0];
+        prof_.flush();
+        return e;
     };
     auto force = [&]() {
         const MDReport r = force_calc();
@@ -377,26 +424,35 @@ MinReport MDSystem::minimize(const MinParams& P, std::vector<double>* tally)
     int err = P.max_nsteps == 0 ? MIN_F_MAX_ITER : LS_S;
     int istep = 0;
     if (lbfgs) {
-        std::vector<double> rho(m, 0.0), al(m, 0.0);
+        std::vector<double> rho(m, 0.0);
+        std::vector<int> dot_row(m, 0);
         std::vector<int> ord(m);  // ord[i]: buffer holding the i-th most recent pair
         for (int i = 0; i < m; ++i) ord[i] = i;
         int k = 0;
         double gamma = 1.0;
         for (; err == LS_S; ++istep) {
-            // two-loop recursion: h = H f from the k stored pairs
-            Sums S = begin(0, 0.0, k > 0 ? mn_s_[ord[0]].get() : nullptr);
+            // two-loop recursion h = H f over the k stored pairs, queued with
+            // its coefficients computed on the device; dot_row[i] holds s_i.h
+            int cur = 0;
+            begin_dev(k > 0 ? mn_s_[ord[0]].get() : nullptr, cur);
             for (int i = 0; i < k; ++i) {
-                al[i] = -rho[i] * S.vh;
-                if (i + 1 < k)
-                    S = axpy(al[i], mn_y_[ord[i]].get(), 1.0, mn_s_[ord[i + 1]].get());
-                else
-                    S = axpy(al[i], mn_y_[ord[i]].get(), gamma, mn_y_[ord[k - 1]].get());
+                dot_row[i] = cur;
+                const bool last = i + 1 == k;
+                axpy_dev(0.0, cur, -1, rho[i], mn_y_[ord[i]].get(), last ? gamma : 1.0,
+                         last ? mn_y_[ord[k - 1]].get() : mn_s_[ord[i + 1]].get(), cur + 1);
+                ++cur;
             }
-            if (k == 0 && gamma != 1.0) S = axpy(0.0, nullptr, gamma, nullptr);
+            if (k == 0 && gamma != 1.0) {
+                axpy_dev(0.0, -1, -1, 0.0, nullptr, gamma, nullptr, cur + 1);
+                ++cur;
+            }
             for (int i = k - 1; i > -1; --i) {
-                const double scl0 = -al[i] - rho[i] * S.vh;
-                S = axpy(scl0, mn_s_[ord[i]].get(), 1.0, i > 0 ? mn_y_[ord[i - 1]].get() : nullptr);
+                axpy_dev(0.0, cur, dot_row[i], rho[i], mn_s_[ord[i]].get(), 1.0,
+                         i > 0 ? mn_y_[ord[i - 1]].get() : nullptr, cur + 1);
+                ++cur;
             }
+            fetch(cur);
+            Sums S{red[0], red[1], red[2], red[3], red[4]};
             e_prev = e_curr;
             double f_h = S.fh;
             if (f_h < 0.0) {
