@@ -99,19 +99,25 @@ mapp_status mapp_md_nvt_run(mapp_md* md, const double* masses, double kB, double
                             double* out7, double* T_part, int* rebuilds);
 /* Energy minimisation of the positions: MinCG::__run (src/min_cg.h:453-533,
  * lbfgs = 0) or MinLBFGS::__run (src/min_l-bfgs.h:114-217, lbfgs = 1, memory
- * m), each with LineSearchBackTrack::min (src/ls.h:711-769; c, rho,
- * min_alpha as in src/ls.cpp:456-461) and MinMDHandler::ls_prep
- * (src/min_cg.h:231-257: no step component larger than max_dx). The box stays
+ * m) with MinMDHandler::ls_prep (src/min_cg.h:231-257: no step component
+ * larger than max_dx) and one of the reference's line searches: ls = 0
+ * LineSearchBackTrack::min (src/ls.h:711-769; c, rho, min_alpha as in
+ * src/ls.cpp:456-461), ls = 1 LineSearchGoldenSection::min (src/ls.h:341-520)
+ * or ls = 2 LineSearchBrent::min (src/ls.h:525-706), both over
+ * LineSearch::bracket (src/ls.h:213-335) when ls_brack != 0, with tol,
+ * max_iter and zeps as in src/ls.cpp:95-101,262-268. The box stays
  * fixed (H_dof all False) and displacements are per atom (affine = False).
  * status follows the reference's err_msgs order (src/min.cpp:7-25): 1 not
  * downhill, 2 zero gradient, 3 minimum alpha, 4 energy tolerance, 5 maximum
- * iterations. Rows (step, energy), the start and every accepted step, go to
+ * iterations, 8 bracketing found no lower energy. Rows (step, energy), the start and every accepted step, go to
  * tally [max_tally_rows][2]; *tally_rows counts the rows written. */
 typedef struct {
     int lbfgs, m;
     double e_tol, max_dx;
     int max_nsteps;
     double ls_c, ls_rho, ls_min_alpha;
+    int ls, ls_max_iter, ls_brack;
+    double ls_tol, ls_zeps;
 } mapp_min_params;
 typedef struct {
     int steps, status, evals, force_calcs, rebuilds;
@@ -119,6 +125,22 @@ typedef struct {
 } mapp_min_report;
 mapp_status mapp_md_minimize(mapp_md* md, const mapp_min_params* params, mapp_min_report* report, double* tally,
                              int max_tally_rows, int* tally_rows);
+/* One line search over a caller's energy along a line, as the minimisers
+ * run it: LineSearch*::min (src/ls.h:213-769) entered with what
+ * MinMDHandler::ls_prep (src/min_cg.h:231-257) leaves, dfa = -f.h and max_a.
+ * energy(ctx, alpha) moves the caller's state to x0 + alpha h and returns its
+ * energy; reset(ctx) moves it back to x0. params->ls, ls_c, ls_rho,
+ * ls_min_alpha, ls_max_iter, ls_brack, ls_tol and ls_zeps choose and tune the
+ * search (the other fields are not read); *prev_val carries the back-tracking
+ * search's memory between calls (-1 after LineSearch::reset). On return
+ * *status is the reference's code (0 success, 1-3 line-search failures,
+ * 8 bracketing failure), *energy the accepted energy and *alpha the step. Runs
+ * on the host only. */
+typedef double (*mapp_line_energy_fn)(void* ctx, double alpha);
+typedef void (*mapp_line_reset_fn)(void* ctx);
+mapp_status mapp_line_search(const mapp_min_params* params, double dfa, double max_a, int init_flag,
+                             double* prev_val, mapp_line_energy_fn energy, mapp_line_reset_fn reset, void* ctx,
+                             double* energy_io, double* alpha, int* status);
 /* Kernel timing with HIP events on the system's stream (no counterpart in
  * the reference, which has no profiler). enable != 0 starts recording and
  * clears the totals; read returns the summed duration and launch count of a

@@ -8,8 +8,9 @@ The Python layer is a thin mirror of the reference's objects over the C ABI
 * :class:`AtomsMD`        -- md.atoms with its ForceField, NeighborMD and DynamicMD
   (force_calc / value / update of src/ff_md.cpp:125-179, src/dynamic_md.h:38-68)
 * :class:`NVT`            -- md.nvt(T, dt).run(sim, nsteps) (src/md_nvt.cpp:383-480)
-* :class:`MinCG`, :class:`MinLBFGS`, :class:`LineSearchBackTrack` -- md.min_cg, md.min_lbfgs
-  and md.ls_bt (src/min_cg.h, src/min_l-bfgs.h, src/ls.h)
+* :class:`MinCG`, :class:`MinLBFGS` with :class:`LineSearchBackTrack`, :class:`LineSearchGoldenSection`,
+  :class:`LineSearchBrent` -- md.min_cg, md.min_lbfgs, ls_bt, ls_golden, ls_brent (src/min_cg.h,
+  src/min_l-bfgs.h, src/ls.h)
 * :class:`AtomsDMD`       -- dmd.atoms with ForceFieldEAMDMD (src/ff_eam_dmd.cpp, src/ff_dmd.cpp)
 * :class:`GCMCBatch`      -- many independent cells under SGCMC exchange (src/sgcmc.cpp,
   ForceFieldEAM's exchange hooks at src/ff_eam.cpp:323-578)
@@ -23,7 +24,8 @@ import numpy as np
 
 from ._lib import MappError, check, f64, fp, i8p, i32p, i64p, ip, lib, ptr, u8p
 
-__all__ = ["ForceFieldEAM", "AtomsMD", "AtomsDMD", "NVT", "MinCG", "MinLBFGS", "LineSearchBackTrack", "GCMCBatch", "Surrogate", "MappError", "create_temp",
+__all__ = ["ForceFieldEAM", "AtomsMD", "AtomsDMD", "NVT", "MinCG", "MinLBFGS", "LineSearchBackTrack",
+           "LineSearchGoldenSection", "LineSearchBrent", "GCMCBatch", "Surrogate", "MappError", "create_temp",
            "gauss_hermite", "lib"]
 
 
@@ -230,11 +232,40 @@ class LineSearchBackTrack:
         self.c, self.rho, self.min_alpha = float(c), float(rho), float(min_alpha)
 
 
+class LineSearchGoldenSection:
+    """mapp4py.ls_golden: golden-section line search over the bracketing
+    routine (src/ls.cpp:95-101 defaults, src/ls.h:213-335,341-520)."""
+
+    _kind = 1
+
+    def __init__(self, tol=(2.0 * 2.220446049250313e-16) ** 0.5, max_iter=5, brack=True):
+        if not tol >= 0.0 or int(max_iter) < 0:
+            raise ValueError("line search needs tol >= 0 and max_iter >= 0")
+        self.tol, self.max_iter, self.brack = float(tol), int(max_iter), bool(brack)
+        self.zeps = 2.220446049250313e-16 * 1.0e-3
+
+
+class LineSearchBrent(LineSearchGoldenSection):
+    """mapp4py.ls_brent: Brent's line search over the bracketing routine
+    (src/ls.cpp:262-268 defaults, src/ls.h:525-706)."""
+
+    _kind = 2
+
+    def __init__(self, tol=(2.0 * 2.220446049250313e-16) ** 0.5, max_iter=5, zeps=2.220446049250313e-16 * 1.0e-3,
+                 brack=True):
+        super().__init__(tol, max_iter, brack)
+        if not zeps >= 0.0:
+            raise ValueError("zeps must be >= 0")
+        self.zeps = float(zeps)
+
+
 _MIN_MSGS = {1: "line search failed: not downhill direction",
              2: "line search failed: gradient is zero",
              3: "line search failed: minimum alpha reached",
              4: "minimization finished: energy tolerance reached",
-             5: "minimization finished: maximum iteration reached"}
+             5: "minimization finished: maximum iteration reached",
+             7: "bracketing failed: maximum alpha reached",
+             8: "bracketing failed: not downhill direction"}
 
 
 class MinCG:
@@ -253,8 +284,8 @@ class MinCG:
             raise NotImplementedError("affine displacements are not built")
         if ls is None:
             ls = LineSearchBackTrack()
-        if not isinstance(ls, LineSearchBackTrack):
-            raise NotImplementedError("only the back-tracking line search (ls_bt) is built")
+        if not isinstance(ls, (LineSearchBackTrack, LineSearchGoldenSection)):
+            raise TypeError("ls must be LineSearchBackTrack, LineSearchGoldenSection or LineSearchBrent")
         if not e_tol >= 0.0 or not max_dx > 0.0:
             raise ValueError("e_tol must be >= 0 and max_dx > 0")
         self.e_tol, self.max_dx, self.ls, self.m = float(e_tol), float(max_dx), ls, 0
@@ -264,8 +295,13 @@ class MinCG:
 
     def run(self, atoms: "AtomsMD", max_nsteps):
         from ._lib import MinParams, MinReport
-        p = MinParams(self._lbfgs, int(self.m), self.e_tol, self.max_dx, int(max_nsteps), self.ls.c, self.ls.rho,
-                      self.ls.min_alpha)
+        ls = self.ls
+        if isinstance(ls, LineSearchBackTrack):
+            p = MinParams(self._lbfgs, int(self.m), self.e_tol, self.max_dx, int(max_nsteps), ls.c, ls.rho,
+                          ls.min_alpha, 0, 5, 1, (2.0 * 2.220446049250313e-16) ** 0.5, 2.220446049250313e-19)
+        else:
+            p = MinParams(self._lbfgs, int(self.m), self.e_tol, self.max_dx, int(max_nsteps), 0.4, 0.5, 0.0,
+                          ls._kind, ls.max_iter, int(ls.brack), ls.tol, ls.zeps)
         r = MinReport()
         cap = int(max_nsteps) + 1
         tally = np.zeros((cap, 2))

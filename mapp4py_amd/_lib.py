@@ -52,6 +52,8 @@ SIGNATURES = {
     "mapp_md_list_stats": (C.c_int, [C.c_void_p, ip, i64p, ip]),
     "mapp_md_nvt_run": (C.c_int, [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, ip]),
     "mapp_md_minimize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, dp, C.c_int, ip]),
+    "mapp_line_search": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int, dp, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, dp, dp, ip]),
     "mapp_md_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "mapp_md_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, dp, i64p]),
     "mapp_md_set_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
@@ -153,7 +155,8 @@ def ptr(a, t=dp):
 class MinParams(C.Structure):
     _fields_ = [("lbfgs", C.c_int), ("m", C.c_int), ("e_tol", C.c_double), ("max_dx", C.c_double),
                 ("max_nsteps", C.c_int), ("ls_c", C.c_double), ("ls_rho", C.c_double),
-                ("ls_min_alpha", C.c_double)]
+                ("ls_min_alpha", C.c_double), ("ls", C.c_int), ("ls_max_iter", C.c_int), ("ls_brack", C.c_int),
+                ("ls_tol", C.c_double), ("ls_zeps", C.c_double)]
 
 
 class MinReport(C.Structure):

@@ -11,6 +11,7 @@
 #include "gcmc.hpp"
 #include "surrogate.hpp"
 #include "md.hpp"
+#include "linesearch.hpp"
 #include "rng.hpp"
 
 namespace mapp {
@@ -268,6 +269,11 @@ mapp_status mapp_md_minimize(mapp_md* md, const mapp_min_params* params, mapp_mi
         p.ls_c = params->ls_c;
         p.ls_rho = params->ls_rho;
         p.ls_min_alpha = params->ls_min_alpha;
+        p.ls = params->ls;
+        p.ls_max_iter = params->ls_max_iter;
+        p.ls_brack = params->ls_brack;
+        p.ls_tol = params->ls_tol;
+        p.ls_zeps = params->ls_zeps;
         std::vector<double> rows;
         const mapp::MinReport r = md->sys->minimize(p, tally ? &rows : nullptr);
         if (report) {
@@ -284,6 +290,40 @@ mapp_status mapp_md_minimize(mapp_md* md, const mapp_min_params* params, mapp_mi
         const int nrows = std::min((int)(rows.size() / 2), max_tally_rows);
         if (tally && nrows) std::memcpy(tally, rows.data(), sizeof(double) * 2 * nrows);
         if (tally_rows) *tally_rows = nrows;
+    });
+}
+
+mapp_status mapp_line_search(const mapp_min_params* params, double dfa, double max_a, int init_flag,
+                             double* prev_val, mapp_line_energy_fn energy, mapp_line_reset_fn reset, void* ctx,
+                             double* energy_io, double* alpha, int* status)
+{
+    return guard([&] {
+        NEED(params);
+        NEED(prev_val);
+        NEED(energy);
+        NEED(reset);
+        NEED(energy_io);
+        NEED(alpha);
+        NEED(status);
+        MAPP_REQUIRE(params->ls >= 0 && params->ls <= 2, mapp::ERR_ARG, "line search must be 0, 1 or 2");
+        mapp::LineSearchParams lp;
+        lp.kind = params->ls;
+        lp.c = params->ls_c;
+        lp.rho = params->ls_rho;
+        lp.min_alpha = params->ls_min_alpha;
+        lp.max_iter = params->ls_max_iter;
+        lp.brack = params->ls_brack;
+        lp.tol = params->ls_tol;
+        lp.zeps = params->ls_zeps;
+        auto f = [&](double t) { return energy(ctx, t); };
+        auto r = [&]() { reset(ctx); };
+        mapp::LineSearcher<decltype(f), decltype(r)> ls(lp, f, r);
+        ls.prev_val = *prev_val;
+        double e = *energy_io, a = 0.0;
+        *status = ls.min(e, a, dfa, max_a, init_flag);
+        *energy_io = e;
+        *alpha = a;
+        *prev_val = ls.prev_val;
     });
 }
 

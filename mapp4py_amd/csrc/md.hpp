@@ -73,20 +73,25 @@ struct MDReport {
     double e[7];   // energy, F_H Voigt (xx,xy,xz,yy,yz,zz)
 };
 
-// MinCG / MinLBFGS with LineSearchBackTrack over the atom positions (no box
-// degrees of freedom, affine = False)
+// MinCG / MinLBFGS with LineSearchBackTrack, LineSearchGoldenSection or
+// LineSearchBrent over the atom positions (no box degrees of freedom,
+// affine = False)
 struct MinParams {
     int lbfgs = 1;        // 0: conjugate gradients, 1: L-BFGS
     int m = 2;            // L-BFGS memory (pairs of s, y)
     double e_tol = 1e-8;
     double max_dx = 1.0;
     int max_nsteps = 1000;
-    double ls_c = 0.4, ls_rho = 0.5, ls_min_alpha = 0.0;
+    double ls_c = 0.4, ls_rho = 0.5, ls_min_alpha = 0.0;      // back-tracking (ls = 0)
+    int ls = 0;                                               // 0 back-track, 1 golden section, 2 Brent
+    int ls_max_iter = 5, ls_brack = 1;                         // golden section and Brent
+    double ls_tol = 2.1073424255447017e-08, ls_zeps = 2.220446049250313e-19;
 };
 
 // status: 0 line search succeeded (not returned), 1 not a downhill direction,
 // 2 zero gradient, 3 minimum step length reached, 4 energy tolerance
-// reached, 5 maximum number of steps (the reference's err_msgs order)
+// reached, 5 maximum number of steps, 8 bracketing found no lower energy
+// (the reference's err_msgs order)
 struct MinReport {
     int steps = 0, status = 0, evals = 0, force_calcs = 0, rebuilds = 0;
     double e0 = 0.0, e = 0.0, f_norm = 0.0, f_max = 0.0;

@@ -13,6 +13,7 @@
 
 #include "md.hpp"
 #include "kern.hpp"
+#include "linesearch.hpp"
 
 namespace mapp {
 
@@ -20,7 +21,6 @@ namespace {
 
 constexpr int kMinRows = 2048;  // grid-stride vector kernels: at most this many workgroups
 
-enum { LS_S = 0, LS_F_DOWNHILL, LS_F_GRAD0, LS_MIN_ALPHA, MIN_S_TOLERANCE, MIN_F_MAX_ITER };
 
 __device__ inline double4 fdbl(float4 f) { return make_double4((double)f.x, (double)f.y, (double)f.z, 0.0); }
 __device__ inline double dot3(double4 a, double4 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
@@ -234,6 +234,8 @@ MinReport MDSystem::minimize(const MinParams& P, std::vector<double>* tally)
     MAPP_REQUIRE(P.ls_c > 0.0 && P.ls_c < 1.0, ERR_ARG, "line search c must lie in (0, 1)");
     MAPP_REQUIRE(P.ls_rho > 0.0 && P.ls_rho < 1.0, ERR_ARG, "line search rho must lie in (0, 1)");
     MAPP_REQUIRE(P.ls_min_alpha >= 0.0, ERR_ARG, "line search min_alpha must be non-negative");
+    MAPP_REQUIRE(P.ls >= 0 && P.ls <= 2, ERR_ARG, "line search must be 0 (back-track), 1 (golden) or 2 (Brent)");
+    MAPP_REQUIRE(P.ls_max_iter >= 0 && P.ls_tol >= 0.0 && P.ls_zeps >= 0.0, ERR_ARG, "invalid line search tolerances");
     const bool lbfgs = P.lbfgs == 1;
     const int m = lbfgs ? P.m : 0;
     mn_x0_.reserve(n_);
@@ -333,7 +335,8 @@ MinReport MDSystem::minimize(const MinParams& P, std::vector<double>* tally)
         MAPP_HIP(hipMemcpyAsync(&flag, flags_.get() + 1, sizeof(int), hipMemcpyDeviceToHost, stream_));
         if (energy) MAPP_HIP(hipMemcpyAsync(&e, red_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
         MAPP_HIP(hipStreamSynchronize(stream_));
-        f_fresh = false;
+        // forces at the new positions come with every evaluated trial point
+        f_fresh = energy;
         if (energy) {
             ++rep.evals;
             ++rep.force_calcs;
@@ -366,11 +369,21 @@ MinReport MDSystem::minimize(const MinParams& P, std::vector<double>* tally)
         f_fresh = true;
         return r.e[0];
     };
-    // LineSearchBackTrack::min with MinMDHandler::ls_prep (src/min_cg.h:231-257):
-    // max_dx bounds the largest component of the step
-    double prev_val = -1.0;
+    LineSearchParams lp;
+    lp.kind = P.ls;
+    lp.c = P.ls_c;
+    lp.rho = P.ls_rho;
+    lp.min_alpha = P.ls_min_alpha;
+    lp.max_iter = P.ls_max_iter;
+    lp.brack = P.ls_brack;
+    lp.tol = P.ls_tol;
+    lp.zeps = P.ls_zeps;
+    auto f_trial = [&](double t) { return F(t, true); };
+    auto f_reset = [&]() { F(0.0, false); };
+    LineSearcher<decltype(f_trial), decltype(f_reset)> searcher(lp, f_trial, f_reset);
+    // MinMDHandler::ls_prep (src/min_cg.h:231-257): max_dx bounds the largest
+    // component of the step; then the chosen line search
     auto line_search = [&](double& nrgy, double& alpha, double f_h, const Sums& S, int init_flag) {
-        const double init_energy = nrgy;
         double dfa = -f_h, max_a;
         if (S.hh == 0.0) {
             max_a = 0.0;
@@ -381,35 +394,7 @@ MinReport MDSystem::minimize(const MinParams& P, std::vector<double>* tally)
         } else {
             max_a = std::fabs(P.max_dx / S.hmax);
         }
-        if (dfa == 0.0) return (int)LS_F_GRAD0;
-        if (max_a == 0.0) return (int)LS_MIN_ALPHA;
-        if (dfa >= 0.0) return (int)LS_F_DOWNHILL;
-        if (init_flag == 0) {
-            max_a = std::min(max_a, 1.0);
-        } else {
-            if (prev_val > 0.0 && dfa < 0.0) max_a = std::min(max_a, -prev_val / dfa);
-            prev_val = -dfa;
-        }
-        if (max_a <= P.ls_min_alpha) return (int)LS_MIN_ALPHA;
-        alpha = max_a;
-        while (true) {
-            const double ideal = nrgy + alpha * P.ls_c * dfa;
-            const double cur = F(alpha, true);
-            if (cur <= ideal) {
-                if (init_flag == 1) prev_val *= alpha;
-                nrgy = cur;
-                f_fresh = true;
-                return (int)LS_S;
-            }
-            alpha *= P.ls_rho;
-            // a nan step (a 0/0 curvature pair once the forces stop changing)
-            // never meets the reference's exit test; it ends the search here
-            if (alpha <= P.ls_min_alpha || std::isnan(alpha)) {
-                nrgy = init_energy;
-                F(0.0, false);
-                return (int)LS_MIN_ALPHA;
-            }
-        }
+        return searcher.min(nrgy, alpha, dfa, max_a, init_flag);
     };
 
     // NewDynamicMD::init (src/dynamic_md.h:145-163): a fresh list at the start

@@ -160,3 +160,66 @@ def test_eam_minimum_matches_scipy(which):
     L = np.diag(H)
     d = (o.x - x_ref + 0.5 * L) % L - 0.5 * L
     assert np.abs(d).max() < 1e-4
+
+
+class Line:
+    """A function of the step length alone, for the bracketing routine."""
+
+    def __init__(self, fn):
+        self.fn = fn
+        self.calls = []
+
+    def F(self, a):
+        self.calls.append(float(a))
+        return self.fn(float(a))
+
+
+@pytest.mark.parametrize("amin,max_a", [(0.3, 1.0), (2.5, 10.0), (0.9, 1.0), (1e-4, 1.0), (7.0, 3.0)])
+def test_bracket_encloses_the_minimum(amin, max_a):
+    fn = lambda a: (a - amin) ** 2  # noqa: E731
+    dfa = -2.0 * amin
+    st, a, b, c, fa, fb, fc = o_min.bracket(Line(fn), dfa, max_a, 0.0, fn(0.0))
+    assert st == o_min.B_S
+    assert fa == fn(a) and fb == fn(b) and fc == fn(c)
+    assert 0.0 <= a < b <= c <= max_a
+    assert fb <= fa
+    if c < max_a:  # a true bracket: the minimum lies inside
+        assert fb <= fc and a <= amin <= c
+    else:
+        assert amin >= b or fb <= fc
+
+
+def test_bracket_reports_no_lower_energy():
+    st = o_min.bracket(Line(lambda a: 1.0 + a), -1.0, 1.0, 0.0, 1.0)[0]
+    assert st == o_min.B_F_DOWNHILL
+
+
+@pytest.mark.parametrize("ls", ["golden", "brent"])
+def test_exact_line_minimum_on_a_quadratic(ls):
+    """With enough iterations both searches land on the exact minimiser of the
+    quadratic along h = f: alpha* = f.f / f.A.f."""
+    n = 12
+    A = spd(n, 8) * 2.0
+    b = np.random.default_rng(9).normal(size=n)
+    q = Quad(A, b, np.zeros((n // 3, 3)))
+    f = b.copy()
+    a_star = (f @ f) / (f @ A @ f)
+    search = o_min.GoldenSection(max_iter=200) if ls == "golden" else o_min.Brent(max_iter=200)
+    r = o_min.min_lbfgs(q, 1, m=2, e_tol=0.0, max_dx=10.0, ls=search)
+    x = q.x.ravel()
+    alpha = (x @ f) / (f @ f)
+    e_star = Quad(A, b, (a_star * f).reshape(-1, 3)).energy()
+    print(ls, alpha, a_star)
+    assert alpha == pytest.approx(a_star, rel=1e-6)
+    assert r["e"] == pytest.approx(e_star, rel=1e-12)
+
+
+@pytest.mark.parametrize("ls", ["golden", "brent"])
+def test_eam_minimum_with_golden_and_brent(ls):
+    p, H, x, elem = cu_system(8)
+    e_ref, _ = scipy_minimum(p, H, x, elem)
+    o = O.OMD(O.OEam.from_knots(p), H, x, elem, skin=0.5)
+    search = o_min.GoldenSection() if ls == "golden" else o_min.Brent()
+    r = o_min.min_lbfgs(o, 3000, m=5, e_tol=1e-13, ls=search)
+    print(ls, r["steps"], r["status"], r["e"], e_ref)
+    assert r["e"] == pytest.approx(e_ref, rel=1e-10)
