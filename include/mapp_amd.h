@@ -308,6 +308,12 @@ mapp_status mapp_dmd_smear(mapp_dmd* d, int ei, int ej, int npts, const double* 
  * failed, 5 e_tol reached, 0 iteration limit. */
 mapp_status mapp_dmd_min_cg(mapp_dmd* d, int max_iter, double e_tol, double max_dx, double max_dalpha,
                             double grad_tol, int* iters, int* status, double* out4);
+/* MinLBFGSDMD::__run over x and alpha with c fixed (src/min_l-bfgs_dmd.h:113-210,
+ * memory m; MinDMDHandler::ls_prep at src/min_cg_dmd.h:432-458) and the
+ * back-tracking line search with its defaults (src/ls.h:711-769). Stops,
+ * out4 and status as mapp_dmd_min_cg. */
+mapp_status mapp_dmd_min_lbfgs(mapp_dmd* d, int m, int max_iter, double e_tol, double max_dx, double max_dalpha,
+                               double grad_tol, int* iters, int* status, double* out4);
 mapp_status mapp_dmd_profile(mapp_dmd* d, int enable);
 mapp_status mapp_dmd_profile_read(mapp_dmd* d, const char* kernel, double* total_ms, int64_t* launches);
 /* XMath::quadrature_hg (src/xmath.cpp:440-589): n-point physicists'

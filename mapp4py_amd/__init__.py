@@ -497,6 +497,16 @@ class AtomsDMD:
         return {"iters": it.value, "status": st.value, "fe0": out[0], "fe": out[1], "grad_norm": out[2],
                 "grad_max": out[3]}
 
+    def min_lbfgs(self, m=2, max_iter=1000, e_tol=1e-8, max_dx=1.0, max_dalpha=0.1, grad_tol=0.0):
+        """dmd.min_lbfgs(m, e_tol, max_dx, max_dalpha).run(atoms, max_iter) over
+        x and alpha (src/min_l-bfgs_dmd.h:113-210); report as min_cg."""
+        it, st = C.c_int(), C.c_int()
+        out = np.zeros(4)
+        check(lib().mapp_dmd_min_lbfgs(self._h, int(m), int(max_iter), float(e_tol), float(max_dx),
+                                       float(max_dalpha), float(grad_tol), C.byref(it), C.byref(st), ptr(out)))
+        return {"iters": it.value, "status": st.value, "fe0": out[0], "fe": out[1], "grad_norm": out[2],
+                "grad_max": out[3]}
+
     def use_surrogate(self, s=None):
         """Evaluate with the networks of s inside their training box (None: quadrature only)."""
         self._surr = s

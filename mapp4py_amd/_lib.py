@@ -90,6 +90,7 @@ SIGNATURES = {
     "mapp_dmd_pairs": (C.c_int, [C.c_void_p, i32p, i32p, i8p]),
     "mapp_dmd_smear": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, dp, dp]),
     "mapp_dmd_min_cg": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, ip, ip, dp]),
+    "mapp_dmd_min_lbfgs": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, ip, ip, dp]),
     "mapp_dmd_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "mapp_dmd_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, dp, i64p]),
     "mapp_gauss_hermite": (C.c_int, [C.c_int, dp, dp]),

@@ -650,6 +650,23 @@ mapp_status mapp_dmd_min_cg(mapp_dmd* d, int max_iter, double e_tol, double max_
     });
 }
 
+mapp_status mapp_dmd_min_lbfgs(mapp_dmd* d, int m, int max_iter, double e_tol, double max_dx, double max_dalpha,
+                               double grad_tol, int* iters, int* status, double* out4)
+{
+    return guard([&] {
+        NEED(d);
+        const mapp::CGReport r = d->sys->lbfgs(m, max_iter, e_tol, max_dx, max_dalpha, grad_tol);
+        if (iters) *iters = r.iters;
+        if (status) *status = r.status;
+        if (out4) {
+            out4[0] = r.fe0;
+            out4[1] = r.fe;
+            out4[2] = r.grad_norm;
+            out4[3] = r.grad_max;
+        }
+    });
+}
+
 mapp_status mapp_dmd_profile(mapp_dmd* d, int enable)
 {
     return guard([&] {

@@ -14,6 +14,7 @@
 #include <cstring>
 
 #include "dmd.hpp"
+#include "linesearch.hpp"
 #include "cfg.hpp"
 #include "kern.hpp"
 #include "surrogate.hpp"
@@ -1318,6 +1319,55 @@ __global__ __launch_bounds__(kBlock) void k_cg_dir(int n, int nc, double ratio, 
     if (i < nc) ha[i] = ratio * ha[i] + fa[i];
 }
 
+// h = (h + a u) g over the position and alpha parts (u optional)
+__global__ __launch_bounds__(kBlock) void k_dlb_axpy(int n, int nc, double a, const double4* __restrict__ ux,
+                                                     const double* __restrict__ ua, double g,
+                                                     double4* __restrict__ hx, double* __restrict__ ha)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        double4 h = hx[i];
+        if (ux) {
+            const double4 u = ux[i];
+            h = make_double4(h.x + a * u.x, h.y + a * u.y, h.z + a * u.z, 0.0);
+        }
+        hx[i] = make_double4(h.x * g, h.y * g, h.z * g, 0.0);
+    }
+    if (i < nc) ha[i] = (ua ? ha[i] + a * ua[i] : ha[i]) * g;
+}
+
+// the newest L-BFGS pair: s = t h, y = f0 - f over both parts
+__global__ __launch_bounds__(kBlock) void k_dlb_pair(int n, int nc, double t, const double4* __restrict__ hx,
+                                                     const double* __restrict__ ha, const double4* __restrict__ f0x,
+                                                     const double* __restrict__ fa0, const double4* __restrict__ fx,
+                                                     const double* __restrict__ fa, double4* __restrict__ sx,
+                                                     double* __restrict__ sa, double4* __restrict__ yx,
+                                                     double* __restrict__ ya)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const double4 h = hx[i], g = f0x[i], f = fx[i];
+        sx[i] = make_double4(t * h.x, t * h.y, t * h.z, 0.0);
+        yx[i] = make_double4(g.x - f.x, g.y - f.y, g.z - f.z, 0.0);
+    }
+    if (i < nc) {
+        sa[i] = t * ha[i];
+        ya[i] = fa0[i] - fa[i];
+    }
+}
+
+// an L-BFGS pair follows its sites into a rebuilt slot order
+__global__ __launch_bounds__(kBlock) void k_dlb_follow(int n, int C, const int* __restrict__ perm,
+                                                       const double4* __restrict__ sx, const double* __restrict__ sa,
+                                                       double4* __restrict__ dx, double* __restrict__ da)
+{
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    const int o = perm[p];
+    dx[p] = sx[o];
+    for (int k = 0; k < C; ++k) da[p * C + k] = sa[o * C + k];
+}
+
 // grid-stride partial max |f_x|, |f_alpha|: one row per workgroup
 __global__ __launch_bounds__(kBlock) void k_gmax(int n, int nc, const double4* __restrict__ fx,
                                                  const double* __restrict__ fa, double* __restrict__ part)
@@ -2517,6 +2567,192 @@ CGReport DMDSystem::cg(int max_iter, double e_tol, double max_dx, double max_dal
         hipLaunchKernelGGL(k_cg_dir, dim3(nbe), dim3(kBlock), 0, stream_, n_, nc_, ratio, f_.get(), fa_.get(),
                            cg_h_.get(), cg_ha_.get());
         f0f0 = ff;
+    }
+    rep.iters = it;
+    rep.status = status;
+    rep.fe = e_curr;
+    rep.grad_norm = std::sqrt(dotv(f_.get(), fa_.get(), f_.get(), fa_.get()));
+    rep.grad_max = gmax();
+    rep.rebuilds = nrebuild_ - reb0;
+    return rep;
+}
+
+CGReport DMDSystem::lbfgs(int m, int max_iter, double e_tol, double max_dx, double max_dalpha, double grad_tol)
+{
+    MAPP_REQUIRE(m >= 0 && m <= 64, ERR_ARG, "L-BFGS memory must lie in [0, 64]");
+    ++ver_;
+    const int nb = grid_for(std::max(n_, nc_));
+    const int nbe = (std::max(n_, nc_) + kBlock - 1) / kBlock;
+    cg_x0_.reserve(n_); cg_h_.reserve(n_); cg_f0_.reserve(n_);
+    cg_a0_.reserve(nc_); cg_ha_.reserve(nc_); cg_fa0_.reserve(nc_);
+    // the stored pairs (s, y), position and alpha parts
+    std::vector<std::unique_ptr<DevBuf<double4>>> sx, yx;
+    std::vector<std::unique_ptr<DevBuf<double>>> sa, ya;
+    for (int i = 0; i < m; ++i) {
+        sx.push_back(std::make_unique<DevBuf<double4>>());
+        yx.push_back(std::make_unique<DevBuf<double4>>());
+        sa.push_back(std::make_unique<DevBuf<double>>());
+        ya.push_back(std::make_unique<DevBuf<double>>());
+        sx[i]->reserve(n_); yx[i]->reserve(n_); sa[i]->reserve(nc_); ya[i]->reserve(nc_);
+    }
+    double* part = part_.get();
+    auto dotv = [&](const double4* ax, const double* aa, const double4* bx, const double* ba) {
+        hipLaunchKernelGGL(k_dot2, dim3(nb), dim3(kBlock), 0, stream_, n_, nc_, ax, aa, bx, ba, part);
+        reduce_rows(part, nb, 1, red_.get() + 24);
+        double v;
+        MAPP_HIP(hipMemcpyAsync(&v, red_.get() + 24, sizeof(double), hipMemcpyDeviceToHost, stream_));
+        MAPP_HIP(hipStreamSynchronize(stream_));
+        return v;
+    };
+    const int gx = std::min(256, grid_for(std::max(n_, nc_)));
+    auto gmax = [&]() {
+        hipLaunchKernelGGL(k_gmax, dim3(gx), dim3(kBlock), 0, stream_, n_, nc_, f_.get(), fa_.get(), part);
+        hipLaunchKernelGGL(k_extrema_rows, dim3(1), dim3(kBlock), 0, stream_, gx, 1, 0, part, red_.get() + 28);
+        double v;
+        MAPP_HIP(hipMemcpyAsync(&v, red_.get() + 28, sizeof(double), hipMemcpyDeviceToHost, stream_));
+        MAPP_HIP(hipStreamSynchronize(stream_));
+        return v;
+    };
+    auto copy_state = [&](DevBuf<double4>& dx, DevBuf<double>& da, const double4* src_x, const double* src_a) {
+        MAPP_HIP(hipMemcpyAsync(dx.get(), src_x, sizeof(double4) * n_, hipMemcpyDeviceToDevice, stream_));
+        MAPP_HIP(hipMemcpyAsync(da.get(), src_a, sizeof(double) * nc_, hipMemcpyDeviceToDevice, stream_));
+    };
+    auto axpy = [&](double a, const DevBuf<double4>* ux, const DevBuf<double>* ua, double g) {
+        hipLaunchKernelGGL(k_dlb_axpy, dim3(nbe), dim3(kBlock), 0, stream_, n_, nc_, a, ux ? ux->get() : nullptr,
+                           ua ? ua->get() : nullptr, g, cg_h_.get(), cg_ha_.get());
+    };
+    auto take = [](auto& a, auto& b) {
+        std::swap(a.p, b.p);
+        std::swap(a.n, b.n);
+    };
+    // the list update of a trial point; after a rebuild every vector follows
+    // its site into the new slot order (the origin x0 with the site's wrap)
+    auto move_update = [&]() {
+        if (!update()) return;
+        DevBuf<double4> x0n, hn, f0n;
+        DevBuf<double> a0n, han, fa0n;
+        x0n.reserve(n_); hn.reserve(n_); f0n.reserve(n_);
+        a0n.reserve(nc_); han.reserve(nc_); fa0n.reserve(nc_);
+        const dim3 g((n_ + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_dcg_follow, g, dim3(kBlock), 0, stream_, n_, C_, vals_alt_.get(), x_.get(),
+                           x_alt_.get(), cg_x0_.get(), cg_h_.get(), cg_f0_.get(), cg_a0_.get(), cg_ha_.get(),
+                           cg_fa0_.get(), x0n.get(), hn.get(), f0n.get(), a0n.get(), han.get(), fa0n.get());
+        take(cg_x0_, x0n); take(cg_h_, hn); take(cg_f0_, f0n);
+        take(cg_a0_, a0n); take(cg_ha_, han); take(cg_fa0_, fa0n);
+        for (int i = 0; i < m; ++i) {
+            DevBuf<double4> tx;
+            DevBuf<double> ta;
+            tx.reserve(n_);
+            ta.reserve(nc_);
+            hipLaunchKernelGGL(k_dlb_follow, g, dim3(kBlock), 0, stream_, n_, C_, vals_alt_.get(), sx[i]->get(),
+                               sa[i]->get(), tx.get(), ta.get());
+            take(*sx[i], tx); take(*sa[i], ta);
+            hipLaunchKernelGGL(k_dlb_follow, g, dim3(kBlock), 0, stream_, n_, C_, vals_alt_.get(), yx[i]->get(),
+                               ya[i]->get(), tx.get(), ta.get());
+            take(*yx[i], tx); take(*ya[i], ta);
+            MAPP_HIP(hipStreamSynchronize(stream_));  // tx, ta are freed at the end of this scope
+        }
+        MAPP_HIP(hipStreamSynchronize(stream_));
+    };
+    CGReport rep{};
+    const int reb0 = nrebuild_;
+    // MinDMDHandler::F / F_reset (src/min_cg_dmd.h:462-475)
+    auto f_trial = [&](double t) {
+        hipLaunchKernelGGL(k_cg_move, dim3(nbe), dim3(kBlock), 0, stream_, n_, nc_, t, cg_x0_.get(), cg_h_.get(),
+                           cg_a0_.get(), cg_ha_.get(), c_.get(), x_.get(), al_.get());
+        move_update();
+        ++rep.evals;
+        return value(false, nullptr);
+    };
+    auto f_reset = [&]() {
+        hipLaunchKernelGGL(k_cg_move, dim3(nbe), dim3(kBlock), 0, stream_, n_, nc_, 0.0, cg_x0_.get(), cg_h_.get(),
+                           cg_a0_.get(), cg_ha_.get(), c_.get(), x_.get(), al_.get());
+        move_update();
+    };
+    LineSearchParams lp;  // LineSearchBackTrack's defaults (src/ls.cpp:456-461)
+    LineSearcher<decltype(f_trial), decltype(f_reset)> searcher(lp, f_trial, f_reset);
+
+    // MinLBFGSDMD::__run (src/min_l-bfgs_dmd.h:113-210)
+    DMDReport d = derivative(false, nullptr);
+    rep.evals = 1;
+    rep.fe0 = d.fe;
+    double e_curr = d.fe;
+    std::vector<double> rho(m, 0.0), al(m, 0.0);
+    std::vector<int> ord(m);
+    for (int i = 0; i < m; ++i) ord[i] = i;
+    int k = 0, status = 0, it = 0;
+    double gamma = 1.0;
+    for (; it < max_iter; ++it) {
+        if (grad_tol > 0.0 && gmax() < grad_tol) { status = 3; break; }
+        copy_state(cg_x0_, cg_a0_, x_.get(), al_.get());
+        copy_state(cg_f0_, cg_fa0_, f_.get(), fa_.get());
+        copy_state(cg_h_, cg_ha_, f_.get(), fa_.get());
+        for (int i = 0; i < k; ++i) {
+            const int b = ord[i];
+            al[i] = -rho[i] * dotv(sx[b]->get(), sa[b]->get(), cg_h_.get(), cg_ha_.get());
+            axpy(al[i], yx[b].get(), ya[b].get(), 1.0);
+        }
+        if (gamma != 1.0) axpy(0.0, nullptr, nullptr, gamma);
+        for (int i = k - 1; i > -1; --i) {
+            const int b = ord[i];
+            const double scl0 = -al[i] - rho[i] * dotv(yx[b]->get(), ya[b]->get(), cg_h_.get(), cg_ha_.get());
+            axpy(scl0, sx[b].get(), sa[b].get(), 1.0);
+        }
+        const double e_prev = e_curr;
+        double f_h = dotv(f_.get(), fa_.get(), cg_h_.get(), cg_ha_.get());
+        if (f_h < 0.0) {
+            copy_state(cg_h_, cg_ha_, f_.get(), fa_.get());
+            k = 0;
+            f_h = dotv(f_.get(), fa_.get(), f_.get(), fa_.get());
+        }
+        // MinDMDHandler::ls_prep (src/min_cg_dmd.h:432-458)
+        const double hh = dotv(cg_h_.get(), cg_ha_.get(), cg_h_.get(), cg_ha_.get());
+        double dfa = -f_h, max_a = 0.0;
+        if (hh == 0.0) {
+            dfa = 0.0;
+        } else if (dfa >= 0.0) {
+            dfa = 1.0;
+        } else {
+            hipLaunchKernelGGL(k_cg_limits, dim3(gx), dim3(kBlock), 0, stream_, n_, nc_, cg_h_.get(), cg_ha_.get(),
+                               al_.get(), c_.get(), part);
+            hipLaunchKernelGGL(k_extrema_rows, dim3(1), dim3(kBlock), 0, stream_, gx, 2, 1, part, red_.get() + 32);
+            double lim[3];
+            MAPP_HIP(hipMemcpyAsync(lim, red_.get() + 32, sizeof(double) * 3, hipMemcpyDeviceToHost, stream_));
+            MAPP_HIP(hipStreamSynchronize(stream_));
+            max_a = std::numeric_limits<double>::infinity();
+            if (lim[0] > 0.0) max_a = std::min(max_a, std::fabs(max_dx / lim[0]));
+            if (lim[1] > 0.0) max_a = std::min(max_a, std::fabs(max_dalpha / lim[1]));
+            max_a = std::min(max_a, lim[2]);
+        }
+        double alpha_m = 0.0;
+        const int ls = searcher.min(e_curr, alpha_m, dfa, max_a, 0);
+        if (ls != LS_S) {
+            status = ls == LS_F_GRAD0 ? 1 : ls == LS_MIN_ALPHA ? 4 : 2;
+            break;
+        }
+        d = derivative(false, nullptr);
+        ++rep.evals;
+        if (e_prev - e_curr < e_tol) { status = 5; ++it; break; }
+        if (m) {
+            if (k != m) ++k;
+            std::rotate(ord.begin(), ord.begin() + (k - 1), ord.begin() + k);
+            for (int i = m - 1; i > 0; --i) rho[i] = rho[i - 1];
+            const int b = ord[0];
+            hipLaunchKernelGGL(k_dlb_pair, dim3(nbe), dim3(kBlock), 0, stream_, n_, nc_, alpha_m, cg_h_.get(),
+                               cg_ha_.get(), cg_f0_.get(), cg_fa0_.get(), f_.get(), fa_.get(), sx[b]->get(),
+                               sa[b]->get(), yx[b]->get(), ya[b]->get());
+            const double inner0 = dotv(sx[b]->get(), sa[b]->get(), yx[b]->get(), ya[b]->get());
+            const double inner1 = dotv(yx[b]->get(), ya[b]->get(), yx[b]->get(), ya[b]->get());
+            gamma = inner0 / inner1;
+            rho[0] = 1.0 / inner0;
+        } else {
+            const double hf0 = dotv(cg_h_.get(), cg_ha_.get(), cg_f0_.get(), cg_fa0_.get());
+            const double hf = dotv(cg_h_.get(), cg_ha_.get(), f_.get(), fa_.get());
+            const double ff = dotv(f_.get(), fa_.get(), f_.get(), fa_.get());
+            const double f0f0 = dotv(cg_f0_.get(), cg_fa0_.get(), cg_f0_.get(), cg_fa0_.get());
+            const double ff0 = dotv(f_.get(), fa_.get(), cg_f0_.get(), cg_fa0_.get());
+            gamma = alpha_m * (hf0 - hf) / (ff + f0f0 - 2.0 * ff0);
+        }
     }
     rep.iters = it;
     rep.status = status;

@@ -307,3 +307,51 @@ def test_cg_through_list_rebuilds(M, cuh):
     ds = (sa[0] - sb[0]) @ np.linalg.inv(H)
     ds -= np.round(ds)
     assert np.abs(ds @ H).max() < 1e-6 and np.abs(sa[1] - sb[1]).max() < 1e-7
+
+
+def test_lbfgs_lowers_free_energy_below_the_cg(M, cuh):
+    # MinLBFGSDMD over x and alpha: its free energy and gradient are the
+    # oracle's at the state it stops in, and it gets lower than the CG does
+    # from the same start (both stop on the energy tolerance while widths
+    # approach zero, the limit that caps their steps)
+    x, H, elem, alpha, c = sites((3, 3, 3), 11)
+    g, _ = both(M, cuh, x, H, elem, alpha, c)
+    r = g.min_lbfgs(m=5, max_iter=300, e_tol=1e-14, grad_tol=1e-6)
+    f, fa, mu, fe, FH, S = g.derivative()
+    say(f"DMD L-BFGS: {r['iters']} iterations status {r['status']} fe {r['fe0']:.6f} -> {r['fe']:.10f} "
+        f"max|grad| {r['grad_max']:.2e}")
+    assert r["fe"] < r["fe0"]
+    assert abs(fe - r["fe"]) < 1e-9 * abs(fe)
+    xs, as_, cs = g.state()
+    o = O.ODMD(O.OEam.from_knots(cuh, dmd=True), H, xs, elem, as_, cs, nquad=3, masses=cuh.masses)
+    fo, fao, _, feo, _, _ = o.force()
+    assert abs(feo - r["fe"]) < 1e-9 * abs(feo)
+    assert max(np.abs(fo).max(), np.abs(fao).max()) == pytest.approx(r["grad_max"], rel=1e-6)
+    g2, _ = both(M, cuh, x, H, elem, alpha, c)
+    rc = g2.min_cg(max_iter=3000, e_tol=1e-14, grad_tol=1e-6)
+    say(f"  CG from the same start: {rc['iters']} iterations fe {rc['fe']:.10f} max|grad| {rc['grad_max']:.2e}")
+    assert r["fe"] < rc["fe"]
+
+
+def test_lbfgs_through_list_rebuilds(M, cuh):
+    # the stored pairs follow their sites through the rebuilds of a tiny skin
+    x, H, elem, alpha, c = sites((3, 3, 3), 31)
+    x = x + np.random.default_rng(32).normal(0, 0.12, x.shape)
+    out = []
+    for skin in (0.02, 1.0):
+        g = M.AtomsDMD(M.ForceFieldEAM.from_knots(cuh, dmd=True), H, x, elem, alpha, c, nquad=3, skin=skin,
+                       masses=cuh.masses)
+        g.profile(True)
+        r = g.min_lbfgs(m=3, max_iter=25, e_tol=0.0, max_dx=0.1, grad_tol=0.0)
+        rebuilds = g.profile_read("cell_sort")[1]
+        g.profile(False)
+        out.append((r, rebuilds, g.state()))
+    (ra, na, sa), (rb, nb, sb) = out
+    say(f"DMD L-BFGS 25 iterations: skin 0.02 {na} rebuilds fe {ra['fe']:.10f}; skin 1.0 {nb} rebuilds "
+        f"fe {rb['fe']:.10f}")
+    assert na > 3 * max(nb, 1)
+    assert ra["iters"] == rb["iters"] and abs(ra["fe"] - rb["fe"]) < 1e-8 * abs(rb["fe"])
+    assert ra["fe"] < ra["fe0"] - 1.0
+    ds = (sa[0] - sb[0]) @ np.linalg.inv(H)
+    ds -= np.round(ds)
+    assert np.abs(ds @ H).max() < 1e-6 and np.abs(sa[1] - sb[1]).max() < 1e-7
