@@ -2499,13 +2499,27 @@ CGReport DMDSystem::cg(int max_iter, double e_tol, double max_dx, double max_dal
     };
     CGReport rep{};
     const int reb0 = nrebuild_;
+    // MinDMDHandler::F / F_reset (src/min_cg_dmd.h:462-475)
+    auto f_trial = [&](double t) {
+        hipLaunchKernelGGL(k_cg_move, dim3(nbe), dim3(kBlock), 0, stream_, n_, nc_, t, cg_x0_.get(), cg_h_.get(),
+                           cg_a0_.get(), cg_ha_.get(), c_.get(), x_.get(), al_.get());
+        move_update();
+        ++rep.evals;
+        return value(false, nullptr);
+    };
+    auto f_reset = [&]() {
+        hipLaunchKernelGGL(k_cg_move, dim3(nbe), dim3(kBlock), 0, stream_, n_, nc_, 0.0, cg_x0_.get(), cg_h_.get(),
+                           cg_a0_.get(), cg_ha_.get(), c_.get(), x_.get(), al_.get());
+        move_update();
+    };
+    LineSearchParams lp;  // LineSearchBackTrack's defaults (src/ls.cpp:456-461)
+    LineSearcher<decltype(f_trial), decltype(f_reset)> searcher(lp, f_trial, f_reset);
     DMDReport d = derivative(false, nullptr);
     rep.evals = 1;
     rep.fe0 = d.fe;
     double e_curr = d.fe;
     copy_state(cg_h_, cg_ha_, f_.get(), fa_.get());
     double f0f0 = dotv(f_.get(), fa_.get(), f_.get(), fa_.get());
-    double prev_val = -1.0;
     int status = 0, it = 0;
     for (; it < max_iter; ++it) {
         if (f0f0 == 0.0) { status = 1; break; }
@@ -2531,35 +2545,14 @@ CGReport DMDSystem::cg(int max_iter, double e_tol, double max_dx, double max_dal
         max_a = std::min(max_a, lim[2]);
         const double dfa = -fh;
         if (dfa >= 0.0 || !(max_a > 0.0)) { status = 2; break; }
-        if (prev_val > 0.0) max_a = std::min(max_a, -prev_val / dfa);
-        prev_val = -dfa;
-        double t = max_a;
-        bool ok = false;
-        for (int bt = 0; bt < 60; ++bt) {
-            const double ideal = e_curr + t * 0.4 * dfa;
-            hipLaunchKernelGGL(k_cg_move, dim3(nbe), dim3(kBlock), 0, stream_, n_, nc_, t, cg_x0_.get(), cg_h_.get(),
-                               cg_a0_.get(), cg_ha_.get(), c_.get(), x_.get(), al_.get());
-            move_update();
-            const double e = value(false, nullptr);
-            ++rep.evals;
-            if (e <= ideal) {
-                prev_val *= t;
-                e_curr = e;
-                ok = true;
-                break;
-            }
-            t *= 0.5;
-        }
-        if (!ok) {
-            hipLaunchKernelGGL(k_cg_move, dim3(nbe), dim3(kBlock), 0, stream_, n_, nc_, 0.0, cg_x0_.get(),
-                               cg_h_.get(), cg_a0_.get(), cg_ha_.get(), c_.get(), x_.get(), al_.get());
-            move_update();
+        // LineSearchBackTrack::min with init_flag 1 (src/ls.h:711-769)
+        double t = 0.0;
+        if (searcher.min(e_curr, t, dfa, max_a, 1) != LS_S) {
             status = 4;
             break;
         }
         d = derivative(false, nullptr);
         ++rep.evals;
-        e_curr = d.fe;
         if (e_prev - e_curr < e_tol) { status = 5; ++it; break; }
         const double ff = dotv(f_.get(), fa_.get(), f_.get(), fa_.get());
         const double ff0 = dotv(f_.get(), fa_.get(), cg_f0_.get(), cg_fa0_.get());
