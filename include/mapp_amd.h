@@ -314,6 +314,14 @@ mapp_status mapp_dmd_min_cg(mapp_dmd* d, int max_iter, double e_tol, double max_
  * out4 and status as mapp_dmd_min_cg. */
 mapp_status mapp_dmd_min_lbfgs(mapp_dmd* d, int m, int max_iter, double e_tol, double max_dx, double max_dalpha,
                                double grad_tol, int* iters, int* status, double* out4);
+/* mapp_dmd_min_cg (lbfgs = 0) or mapp_dmd_min_lbfgs (lbfgs = 1, memory m) with the
+ * line search ls->ls and its parameters ls_c, ls_rho, ls_min_alpha, ls_max_iter,
+ * ls_brack, ls_tol, ls_zeps as mapp_md_minimize reads them: the reference's
+ * dmd.min_cg / dmd.min_lbfgs accept ls_bt, ls_golden or ls_brent
+ * (src/min_cg_dmd.cpp:172, src/min_l-bfgs_dmd.cpp:76). */
+mapp_status mapp_dmd_min_run(mapp_dmd* d, int lbfgs, int m, int max_iter, double e_tol, double max_dx,
+                             double max_dalpha, double grad_tol, const mapp_min_params* ls, int* iters, int* status,
+                             double* out4);
 mapp_status mapp_dmd_profile(mapp_dmd* d, int enable);
 mapp_status mapp_dmd_profile_read(mapp_dmd* d, const char* kernel, double* total_ms, int64_t* launches);
 /* XMath::quadrature_hg (src/xmath.cpp:440-589): n-point physicists'

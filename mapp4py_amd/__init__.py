@@ -489,7 +489,29 @@ class AtomsDMD:
         check(lib().mapp_dmd_smear(self._h, int(ei), int(ej), len(r), ptr(r), ptr(a), ptr(out)))
         return out
 
-    def min_cg(self, max_iter=1000, e_tol=1e-8, max_dx=1.0, max_dalpha=0.1, grad_tol=0.0):
+    def _min_run(self, lbfgs, m, max_iter, e_tol, max_dx, max_dalpha, grad_tol, ls):
+        from ._lib import MinParams
+        it, st = C.c_int(), C.c_int()
+        out = np.zeros(4)
+        if isinstance(ls, LineSearchBackTrack):
+            p = MinParams(lbfgs, m, e_tol, max_dx, max_iter, ls.c, ls.rho, ls.min_alpha, 0, 5, 1,
+                          (2.0 * 2.220446049250313e-16) ** 0.5, 2.220446049250313e-19)
+        elif isinstance(ls, LineSearchGoldenSection):
+            p = MinParams(lbfgs, m, e_tol, max_dx, max_iter, 0.4, 0.5, 0.0, ls._kind, ls.max_iter, int(ls.brack),
+                          ls.tol, ls.zeps)
+        else:
+            raise TypeError("ls must be LineSearchBackTrack, LineSearchGoldenSection or LineSearchBrent")
+        check(lib().mapp_dmd_min_run(self._h, int(lbfgs), int(m), int(max_iter), float(e_tol), float(max_dx),
+                                     float(max_dalpha), float(grad_tol), C.byref(p), C.byref(it), C.byref(st),
+                                     ptr(out)))
+        return {"iters": it.value, "status": st.value, "fe0": out[0], "fe": out[1], "grad_norm": out[2],
+                "grad_max": out[3]}
+
+    def min_cg(self, max_iter=1000, e_tol=1e-8, max_dx=1.0, max_dalpha=0.1, grad_tol=0.0, ls=None):
+        """dmd.min_cg(e_tol, max_dx, max_dalpha, ls).run(atoms, max_iter) over x and
+        alpha (src/min_cg_dmd.h:633-700); ls defaults to the back-tracking search."""
+        if ls is not None:
+            return self._min_run(0, 0, max_iter, e_tol, max_dx, max_dalpha, grad_tol, ls)
         it, st = C.c_int(), C.c_int()
         out = np.zeros(4)
         check(lib().mapp_dmd_min_cg(self._h, int(max_iter), float(e_tol), float(max_dx), float(max_dalpha),
@@ -497,9 +519,11 @@ class AtomsDMD:
         return {"iters": it.value, "status": st.value, "fe0": out[0], "fe": out[1], "grad_norm": out[2],
                 "grad_max": out[3]}
 
-    def min_lbfgs(self, m=2, max_iter=1000, e_tol=1e-8, max_dx=1.0, max_dalpha=0.1, grad_tol=0.0):
-        """dmd.min_lbfgs(m, e_tol, max_dx, max_dalpha).run(atoms, max_iter) over
+    def min_lbfgs(self, m=2, max_iter=1000, e_tol=1e-8, max_dx=1.0, max_dalpha=0.1, grad_tol=0.0, ls=None):
+        """dmd.min_lbfgs(m, e_tol, max_dx, max_dalpha, ls).run(atoms, max_iter) over
         x and alpha (src/min_l-bfgs_dmd.h:113-210); report as min_cg."""
+        if ls is not None:
+            return self._min_run(1, m, max_iter, e_tol, max_dx, max_dalpha, grad_tol, ls)
         it, st = C.c_int(), C.c_int()
         out = np.zeros(4)
         check(lib().mapp_dmd_min_lbfgs(self._h, int(m), int(max_iter), float(e_tol), float(max_dx),

@@ -52,6 +52,8 @@ SIGNATURES = {
     "mapp_md_list_stats": (C.c_int, [C.c_void_p, ip, i64p, ip]),
     "mapp_md_nvt_run": (C.c_int, [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, ip]),
     "mapp_md_minimize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, dp, C.c_int, ip]),
+    "mapp_dmd_min_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, C.c_void_p, ip, ip, dp]),
     "mapp_line_search": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int, dp, C.c_void_p, C.c_void_p,
                                    C.c_void_p, dp, dp, ip]),
     "mapp_md_profile": (C.c_int, [C.c_void_p, C.c_int]),

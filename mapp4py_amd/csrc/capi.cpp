@@ -667,6 +667,36 @@ mapp_status mapp_dmd_min_lbfgs(mapp_dmd* d, int m, int max_iter, double e_tol, d
     });
 }
 
+mapp_status mapp_dmd_min_run(mapp_dmd* d, int lbfgs, int m, int max_iter, double e_tol, double max_dx,
+                             double max_dalpha, double grad_tol, const mapp_min_params* ls, int* iters, int* status,
+                             double* out4)
+{
+    return guard([&] {
+        NEED(d);
+        NEED(ls);
+        MAPP_REQUIRE(ls->ls >= 0 && ls->ls <= 2, mapp::ERR_ARG, "line search must be 0, 1 or 2");
+        mapp::LineSearchParams lp;
+        lp.kind = ls->ls;
+        lp.c = ls->ls_c;
+        lp.rho = ls->ls_rho;
+        lp.min_alpha = ls->ls_min_alpha;
+        lp.max_iter = ls->ls_max_iter;
+        lp.brack = ls->ls_brack;
+        lp.tol = ls->ls_tol;
+        lp.zeps = ls->ls_zeps;
+        const mapp::CGReport r = lbfgs ? d->sys->lbfgs(m, max_iter, e_tol, max_dx, max_dalpha, grad_tol, &lp)
+                                       : d->sys->cg(max_iter, e_tol, max_dx, max_dalpha, grad_tol, &lp);
+        if (iters) *iters = r.iters;
+        if (status) *status = r.status;
+        if (out4) {
+            out4[0] = r.fe0;
+            out4[1] = r.fe;
+            out4[2] = r.grad_norm;
+            out4[3] = r.grad_max;
+        }
+    });
+}
+
 mapp_status mapp_dmd_profile(mapp_dmd* d, int enable)
 {
     return guard([&] {

@@ -2447,7 +2447,8 @@ void DMDSystem::smear(int ei, int ej, int npts, const double* r, const double* a
 // MinDMDHandler::ls_prep (max_dx, max_dalpha, alpha kept positive). Stops on
 // the energy decrement e_tol like the reference, or when the largest
 // gradient component drops below grad_tol (0 disables).
-CGReport DMDSystem::cg(int max_iter, double e_tol, double max_dx, double max_dalpha, double grad_tol)
+CGReport DMDSystem::cg(int max_iter, double e_tol, double max_dx, double max_dalpha, double grad_tol,
+                       const LineSearchParams* ls)
 {
     ++ver_;
     const int nb = grid_for(std::max(n_, nc_));
@@ -2512,7 +2513,8 @@ CGReport DMDSystem::cg(int max_iter, double e_tol, double max_dx, double max_dal
                            cg_a0_.get(), cg_ha_.get(), c_.get(), x_.get(), al_.get());
         move_update();
     };
-    LineSearchParams lp;  // LineSearchBackTrack's defaults (src/ls.cpp:456-461)
+    // LineSearchBackTrack's defaults (src/ls.cpp:456-461) unless the caller chose a search
+    const LineSearchParams lp = ls ? *ls : LineSearchParams{};
     LineSearcher<decltype(f_trial), decltype(f_reset)> searcher(lp, f_trial, f_reset);
     DMDReport d = derivative(false, nullptr);
     rep.evals = 1;
@@ -2570,7 +2572,8 @@ CGReport DMDSystem::cg(int max_iter, double e_tol, double max_dx, double max_dal
     return rep;
 }
 
-CGReport DMDSystem::lbfgs(int m, int max_iter, double e_tol, double max_dx, double max_dalpha, double grad_tol)
+CGReport DMDSystem::lbfgs(int m, int max_iter, double e_tol, double max_dx, double max_dalpha, double grad_tol,
+                          const LineSearchParams* ls)
 {
     MAPP_REQUIRE(m >= 0 && m <= 64, ERR_ARG, "L-BFGS memory must lie in [0, 64]");
     ++ver_;
@@ -2662,7 +2665,8 @@ CGReport DMDSystem::lbfgs(int m, int max_iter, double e_tol, double max_dx, doub
                            cg_a0_.get(), cg_ha_.get(), c_.get(), x_.get(), al_.get());
         move_update();
     };
-    LineSearchParams lp;  // LineSearchBackTrack's defaults (src/ls.cpp:456-461)
+    // LineSearchBackTrack's defaults (src/ls.cpp:456-461) unless the caller chose a search
+    const LineSearchParams lp = ls ? *ls : LineSearchParams{};
     LineSearcher<decltype(f_trial), decltype(f_reset)> searcher(lp, f_trial, f_reset);
 
     // MinLBFGSDMD::__run (src/min_l-bfgs_dmd.h:113-210)

@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 
+#include "linesearch.hpp"
 #include "md.hpp"
 #include "surrogate.hpp"
 
@@ -108,10 +109,13 @@ public:
     int64_t half_pairs_count();
     void half_pairs(int32_t* i, int32_t* j, int8_t* img);
     // MinCGDMD over x and alpha (c fixed)
-    CGReport cg(int max_iter, double e_tol, double max_dx, double max_dalpha, double grad_tol);
+    // ls: the line search and its parameters (nullptr: LineSearchBackTrack's defaults)
+    CGReport cg(int max_iter, double e_tol, double max_dx, double max_dalpha, double grad_tol,
+                const LineSearchParams* ls = nullptr);
     // MinLBFGSDMD::__run (src/min_l-bfgs_dmd.h:113-210) over x and alpha with
     // memory m and the back-tracking line search; report and stops as cg
-    CGReport lbfgs(int m, int max_iter, double e_tol, double max_dx, double max_dalpha, double grad_tol);
+    CGReport lbfgs(int m, int max_iter, double e_tol, double max_dx, double max_dalpha, double grad_tol,
+                   const LineSearchParams* ls = nullptr);
     // smeared pair functions at given (r, alpha_ij): out[k*9 + q], q = rho_phi[3], drho_phi_dr[3], drho_phi_dalpha[3]
     void smear(int ei, int ej, int npts, const double* r, const double* a, double* out);
     // the same for device arrays and a table index tt = ei * ne + ej (synchronous)

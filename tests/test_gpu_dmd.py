@@ -355,3 +355,24 @@ def test_lbfgs_through_list_rebuilds(M, cuh):
     ds = (sa[0] - sb[0]) @ np.linalg.inv(H)
     ds -= np.round(ds)
     assert np.abs(ds @ H).max() < 1e-6 and np.abs(sa[1] - sb[1]).max() < 1e-7
+
+
+@pytest.mark.parametrize("which,ls", [("cg", "golden"), ("cg", "brent"), ("lbfgs", "golden"), ("lbfgs", "brent")])
+def test_dmd_minimisers_with_golden_and_brent(M, cuh, which, ls):
+    # dmd.min_cg / dmd.min_lbfgs with ls_golden or ls_brent: the free energy
+    # falls, and at the state they stop in it and its gradient are the oracle's
+    x, H, elem, alpha, c = sites((3, 3, 3), 41)
+    g, _ = both(M, cuh, x, H, elem, alpha, c)
+    search = M.LineSearchGoldenSection() if ls == "golden" else M.LineSearchBrent()
+    if which == "cg":
+        r = g.min_cg(max_iter=30, e_tol=0.0, ls=search)
+    else:
+        r = g.min_lbfgs(m=3, max_iter=30, e_tol=0.0, ls=search)
+    say(f"DMD {which} with {ls}: {r['iters']} iterations status {r['status']} fe {r['fe0']:.6f} -> {r['fe']:.8f}")
+    assert r["fe"] < r["fe0"] - 1.0
+    f, fa, mu, fe, FH, S = g.derivative()
+    xs, as_, cs = g.state()
+    o = O.ODMD(O.OEam.from_knots(cuh, dmd=True), H, xs, elem, as_, cs, nquad=3, masses=cuh.masses)
+    fo, fao, _, feo, _, _ = o.force()
+    assert abs(feo - fe) < 1e-9 * abs(feo)
+    assert max(np.abs(fo - f).max(), np.abs(fao - fa).max()) < 1e-8 * max(np.abs(fo).max(), 1.0)
