@@ -700,44 +700,6 @@ __global__ __launch_bounds__(kBigBlock) void k_eam_force_lds(int n, const uint4*
     }
 }
 
-// value(): rho and pair energy in one pass, then F(rho)
-__global__ __launch_bounds__(kBlock) void k_eam_energy(int n, const double4* __restrict__ x,
-                                                       const int* __restrict__ nn, const uint32_t* __restrict__ list,
-                                                       int ldl, EamArgs P, BoxDev bd, double* __restrict__ part)
-{
-    const int blk = xcd_block();
-    const int i = blk * kBlock + threadIdx.x;
-    double acc[1] = {0.0};
-    if (i < n) {
-        const double4 xi = x[i];
-        const int ti = (int)xi.w;
-        const int cnt = nn[i];
-        double rho = 0.0, ep = 0.0;
-        for (int k = 0; k < cnt; ++k) {
-            const uint32_t e = list[(size_t)k * ldl + i];
-            const double4 xj = x[e & kIndexMask];
-            double dx, dy, dz;
-            pair_dx(xi, xj, e >> kIndexBits, bd.h, dx, dy, dz);
-            const double rsq = dx * dx + dy * dy + dz * dz;
-            const int tt = ti * P.ne + (int)xj.w;
-            if (rsq < P.cut_sq[tt]) {
-                const double r = root64(rsq);
-                float p;
-                const int m = knot(r, P.dr_inv, P.nr, p);
-                const float4 ca = P.tabA[(size_t)tt * P.nr + m];
-                const float4* c = P.tabB + ((size_t)tt * P.nr + m) * 3;
-                const float4 c0 = c[0], cp = c[1];
-                rho += (double)(((ca.w * p + ca.z) * p + ca.y) * p + ca.x);
-                ep += 0.5 * (double)((((cp.z * p + cp.y) * p + cp.x) * p + c0.w) / (float)r);
-            }
-        }
-        double F, dF;
-        embed(P, ti, rho, F, dF);
-        acc[0] = F + ep;
-    }
-    block_sum_store<1>(acc, part + (size_t)blk * kPartW);
-}
-
 // ---------------- NVT ----------------
 
 // nhc state layout (doubles)
@@ -1346,19 +1308,24 @@ MDReport MDSystem::force_calc()
 
 double MDSystem::value()
 {
-    const int nb = grid_for(n_);
-    const EamArgs P = ff_->args();
-    BoxDev bd;
-    std::memcpy(bd.h, box_.h, sizeof bd.h);
-    std::memcpy(bd.b, box_.b, sizeof bd.b);
-    prof_.begin(K_EAM_ENERGY, stream_);
-    hipLaunchKernelGGL(k_eam_energy, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), nn_.get(), list_.get(), ldl_,
-                       P, bd, part_.get());
-    prof_.end(stream_);
-    reduce_rows(part_.get(), nb, 1, red_.get());
+    // the energy of the two force passes (the EAM energy the reference's
+    // __energy_calc sums); their forces and densities go to spare buffers so
+    // that the last force_calc's stay readable
+    val_f_.reserve(n_);
+    val_rho_.reserve(n_);
+    std::swap(f_.p, val_f_.p);
+    std::swap(f_.n, val_f_.n);
+    std::swap(rho_.p, val_rho_.p);
+    std::swap(rho_.n, val_rho_.n);
+    force_passes(nullptr);
+    std::swap(f_.p, val_f_.p);
+    std::swap(f_.n, val_f_.n);
+    std::swap(rho_.p, val_rho_.p);
+    std::swap(rho_.n, val_rho_.n);
     double e = 0.0;
     MAPP_HIP(hipMemcpyAsync(&e, red_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
     MAPP_HIP(hipStreamSynchronize(stream_));
+    prof_.flush();
     return e;
 }
 

@@ -171,6 +171,8 @@ private:
     DevBuf<int> slot_of_;            // caller index -> slot
     DevBuf<float4> f_;               // forces (w unused)
     DevBuf<double> rho_;
+    DevBuf<float4> val_f_;           // forces and densities written by value()'s passes
+    DevBuf<double> val_rho_;
     DevBuf<uint4> rec_;              // 16-byte position records (coordinates on the grid, element)
     DevBuf<uint4> xr_;               // 16-byte pass-2 records (coordinates, F'(rho)), one element
     DevBuf<float> dF_;               // F'(rho) per slot, several elements
