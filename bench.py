@@ -141,9 +141,10 @@ def feni_run(steps=1000, warmup=20):
 # minimiser vector kernels, bytes per atom as laid out (double4 rows of 32 B,
 # float4 forces of 16 B): one L-BFGS direction with two stored pairs (begin
 # 176, four two-loop passes 144 + 112 + 144 + 112), the pair update 144, and
-# 128 per line-search trial point (x0, h, build positions read, x written)
+# 144 per line-search trial point (x0, h, build positions read, x and the
+# 16-byte record written)
 MIN_VEC_BYTES_PER_ATOM_STEP = 832
-MIN_VEC_BYTES_PER_ATOM_EVAL = 128
+MIN_VEC_BYTES_PER_ATOM_EVAL = 144
 
 
 def min_run(steps=30):
