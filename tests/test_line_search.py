@@ -121,3 +121,13 @@ def test_prep_failures_and_the_backtrack_memory():
         same_calls(rl[3], ro[3])
         pv_l, pv_o = rl[4], ro[4]
         assert pv_l == pytest.approx(pv_o, rel=1e-14)
+
+
+def test_rejects_an_unknown_search():
+    p = _lib.MinParams(1, 2, 1e-8, 1.0, 10, 0.4, 0.5, 0.0, 3, 5, 1, 1e-8, 1e-19)
+    pv, e, a, st = C.c_double(-1.0), C.c_double(0.0), C.c_double(), C.c_int()
+    ef = ENERGY(lambda ctx, x: x * x)
+    rf = RESET(lambda ctx: None)
+    with pytest.raises(_lib.MappError):
+        _lib.check(_lib.lib().mapp_line_search(C.byref(p), -1.0, 1.0, 0, C.byref(pv), ef, rf, None, C.byref(e),
+                                              C.byref(a), C.byref(st)))
