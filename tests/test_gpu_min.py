@@ -180,3 +180,21 @@ def test_golden_and_brent_follow_the_oracle(M, cu_pot, which, ls):
     # 1.3e-4 eV at one step, the trajectories then rejoin)
     assert de < E_ATOM * n + 1e-6 + 3e-4
     assert dx < 5e-4
+
+
+def test_invalid_parameters_are_rejected(M, cu_pot):
+    H, x, elem = cu_sites(3, 16, 0.05)
+    g = M.AtomsMD(M.ForceFieldEAM.from_knots(cu_pot), H, x, elem, skin=0.5)
+    from mapp4py_amd._lib import MinParams, MinReport
+    import ctypes as C
+    good = dict(lbfgs=1, m=2, e_tol=1e-8, max_dx=1.0, max_nsteps=5, ls_c=0.4, ls_rho=0.5, ls_min_alpha=0.0, ls=0,
+                ls_max_iter=5, ls_brack=1, ls_tol=1e-8, ls_zeps=1e-19)
+    for bad in (dict(m=65), dict(m=-1), dict(max_dx=0.0), dict(ls_c=1.0), dict(ls_rho=0.0), dict(ls=3),
+                dict(max_nsteps=-1), dict(lbfgs=2), dict(ls_max_iter=-1)):
+        p = MinParams(**{**good, **bad})
+        r = MinReport()
+        with pytest.raises(M.MappError):
+            M._lib.check(M.lib().mapp_md_minimize(g._h, C.byref(p), C.byref(r), None, 0, None))
+    # the system is untouched and still minimises
+    rep = M.MinLBFGS().run(g, 3)
+    assert rep["steps"] == 3 and rep["e"] < rep["e0"]
