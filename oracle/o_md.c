@@ -541,7 +541,30 @@ double o_md_energy(o_md* md)
 }
 
 /* ---------------- NVT (src/md_nvt.cpp, src/thermostat.cpp:47-143) ---------------- */
+/* Test hook: a nonzero value plants a known mistake in the thermostat so that
+ * the tests can show that their checks catch it. 1 swaps ddt and ddt2; 2 flips
+ * the sign of the coupling -xi_{j+1} xi_j in the forward sweep; 3 drives xi_2
+ * by N_f xi_1^2 in the reverse sweep as well (the textbook chain, where the
+ * reference's reverse sweep has xi_1^2). 0 is the reference's thermostat. */
+int o_nhc_mutation = 0;
+
 typedef struct { double ddt, ddt2, freq_sq; double eta[3]; int L, niters; } o_nhc;
+
+/* ThermostatNHC(dt, t_relax, L = 3, niters) (src/thermostat.cpp:19-36) */
+static void nhc_init(o_nhc* t, double dt, double t_relax, int niters)
+{
+    t->L = 3;
+    t->niters = niters;
+    t->ddt = dt / (double)niters;
+    t->ddt2 = 0.5 * dt / (double)niters;
+    if (o_nhc_mutation == 1) {
+        double tmp = t->ddt;
+        t->ddt = t->ddt2;
+        t->ddt2 = tmp;
+    }
+    t->freq_sq = 1.0 / (t_relax * t_relax);
+    t->eta[0] = t->eta[1] = t->eta[2] = 0.0;
+}
 
 static double nhc_f(double x)
 {
@@ -549,33 +572,57 @@ static double nhc_f(double x)
     return (exp(x) - 1.0) / x;
 }
 
-static double nhc_step(o_nhc* t, double T_r, int ndof)
+/* ThermostatNHC::operator() (src/thermostat.cpp:47-104), L > 1 branch.
+ * mid (may be NULL) receives the chain after the reverse sweep of the last
+ * iteration, the state at which the velocities are scaled. */
+static double nhc_step(o_nhc* t, double T_r, int ndof, double* mid)
 {
     double rescale = 1.0, prev, tmp;
     int L = t->L;
     double* e = t->eta;
+    double sgn = o_nhc_mutation == 2 ? -1.0 : 1.0;
+    double rev0 = o_nhc_mutation == 3 ? (double)ndof : 1.0;
     for (int it = 0; it < t->niters; it++) {
         e[L - 1] += t->ddt2 * (e[L - 2] * e[L - 2] - t->freq_sq);
         prev = -e[L - 1];
         for (int ich = L - 2; ich > 0; ich--) {
-            e[ich] += t->ddt2 * nhc_f(prev * t->ddt2) * (prev * e[ich] + e[ich - 1] * e[ich - 1] - t->freq_sq);
+            double drive = (ich == 1 ? rev0 : 1.0) * e[ich - 1] * e[ich - 1];
+            e[ich] += t->ddt2 * nhc_f(prev * t->ddt2) * (prev * e[ich] + drive - t->freq_sq);
             prev = -e[ich];
         }
         e[0] += t->ddt2 * nhc_f(prev * t->ddt2) * (prev * e[0] + t->freq_sq * (T_r - 1.0));
         prev = -e[0];
+        if (mid) memcpy(mid, e, sizeof(double) * 3);
         tmp = exp(t->ddt * prev);
         rescale *= tmp;
         T_r *= tmp * tmp;
         prev = (T_r - 1.0) * t->freq_sq;
-        e[0] += t->ddt2 * nhc_f(-e[1] * t->ddt2) * (-e[1] * e[0] + prev);
+        e[0] += t->ddt2 * nhc_f(-e[1] * t->ddt2) * (-sgn * e[1] * e[0] + prev);
         prev = ndof * e[0] * e[0] - t->freq_sq;
         for (int ich = 1; ich < L - 1; ich++) {
-            e[ich] += t->ddt2 * nhc_f(-e[ich + 1] * t->ddt2) * (-e[ich + 1] * e[ich] + prev);
+            e[ich] += t->ddt2 * nhc_f(-e[ich + 1] * t->ddt2) * (-sgn * e[ich + 1] * e[ich] + prev);
             prev = e[ich] * e[ich] - t->freq_sq;
         }
         e[L - 1] += t->ddt2 * prev;
     }
     return rescale;
+}
+
+double o_nhc_run(double T_r, int ndof, double dt, double t_relax, int niters, int ncalls, double* hist)
+{
+    o_nhc th;
+    nhc_init(&th, dt, t_relax, niters);
+    double scale = 1.0;
+    for (int k = 0; k < ncalls; k++) {
+        double r = nhc_step(&th, T_r, ndof, NULL);
+        scale *= r;
+        T_r *= r * r;
+        if (hist) {
+            hist[4 * k] = T_r;
+            memcpy(hist + 4 * k + 1, th.eta, sizeof(double) * 3);
+        }
+    }
+    return scale;
 }
 
 static void kin_tensor(const o_md* md, const double* v, const double* masses, double* mvv)
@@ -589,19 +636,24 @@ static void kin_tensor(const o_md* md, const double* v, const double* masses, do
     }
 }
 
-int o_md_nvt_run(o_md* md, double* v, const double* masses, double kB, double T,
-                 double dt, int nsteps, double* out7, double* T_part_out)
+/* the thermostat's integral of xi_j over one call spanning tcall (one
+ * iteration): the call scales the velocities by exp(-tcall xi_1) at its mid
+ * state, so it adds tcall * xi(mid) */
+static void eta_accumulate(double* eta_int, double tcall, const double* mid)
+{
+    for (int k = 0; k < 3; k++) eta_int[k] += tcall * mid[k];
+}
+
+int o_md_nvt_run_hist(o_md* md, double* v, const double* masses, double kB, double T,
+                      double dt, int nsteps, double* out7, double* T_part_out, double* hist)
 {
     int n = md->n;
     double dt2 = dt / 2.0;
     int ndof = n * 3;
     o_nhc th;
     /* ThermostatNHC(dt/2, 100 dt, 3, 1) (src/md_nvt.cpp:19) */
-    th.L = 3; th.niters = 1;
-    th.ddt = (dt / 2.0) / 1.0;
-    th.ddt2 = 0.5 * (dt / 2.0) / 1.0;
-    th.freq_sq = 1.0 / ((100.0 * dt) * (100.0 * dt));
-    th.eta[0] = th.eta[1] = th.eta[2] = 0.0;
+    nhc_init(&th, dt / 2.0, 100.0 * dt, 1);
+    double mid[3], eta_int[3] = {0.0, 0.0, 0.0};
     double* f = (double*)malloc(sizeof(double) * 3 * (n > 0 ? n : 1));
     double mvv[6];
     kin_tensor(md, v, masses, mvv);
@@ -610,7 +662,8 @@ int o_md_nvt_run(o_md* md, double* v, const double* masses, double kB, double T,
     int nrebuild = 0;
     double fac, fac_x_d = 1.0;
     for (int step = 0; step < nsteps; step++) {
-        fac_x_d *= fac = nhc_step(&th, T_part / T, ndof);
+        fac_x_d *= fac = nhc_step(&th, T_part / T, ndof, mid);
+        eta_accumulate(eta_int, dt2, mid);
         fac *= fac;
         for (int k = 0; k < 6; k++) mvv[k] *= fac;
         T_part *= fac;
@@ -637,15 +690,29 @@ int o_md_nvt_run(o_md* md, double* v, const double* masses, double kB, double T,
         }
         kin_tensor(md, v, masses, mvv);
         T_part = (mvv[0] + (mvv[3] + mvv[5])) / (ndof * kB);
-        fac_x_d = fac = nhc_step(&th, T_part / T, ndof);
+        fac_x_d = fac = nhc_step(&th, T_part / T, ndof, mid);
+        eta_accumulate(eta_int, dt2, mid);
         fac *= fac;
         for (int k = 0; k < 6; k++) mvv[k] *= fac;
         T_part *= fac;
+        if (hist) {
+            double* h = hist + 8 * step;
+            h[0] = T_part;
+            h[1] = out7[0];
+            memcpy(h + 2, th.eta, sizeof(double) * 3);
+            memcpy(h + 5, eta_int, sizeof(double) * 3);
+        }
     }
     for (int i = 0; i < 3 * n; i++) v[i] *= fac_x_d;
     if (T_part_out) *T_part_out = T_part;
     free(f);
     return nrebuild;
+}
+
+int o_md_nvt_run(o_md* md, double* v, const double* masses, double kB, double T,
+                 double dt, int nsteps, double* out7, double* T_part_out)
+{
+    return o_md_nvt_run_hist(md, v, masses, kB, T, dt, nsteps, out7, T_part_out, NULL);
 }
 
 /* AtomsMD::create_T (src/atoms_md.cpp:130-160), one rank (seed + 0) */

@@ -22,7 +22,9 @@
  *   DMD           closed-form Gaussian smearing, finite differences of the
  *                 free energy vs f_x, f_alpha, mu; alpha -> 0 gives MD
  *   Park-Miller   published check value (seed 1 -> 10000th = 1043618065)
- *   NVT / NHC     energy conservation of the extended system (NVE limit)
+ *   NVT / NHC     zero-force temperature history against an independent
+ *                 integration of the chain's equations of motion; the
+ *                 extended energy of that chain conserved with forces
  */
 #ifndef MAPP_ORACLE_H
 #define MAPP_ORACLE_H
@@ -93,6 +95,21 @@ double o_md_energy(o_md*);
  * neighbor-list rebuilds. out7 as o_md_force after the last step. */
 int o_md_nvt_run(o_md*, double* v, const double* masses, double kB, double T,
                  double dt, int nsteps, double* out7, double* T_part_out);
+/* o_md_nvt_run with a record per step after its second thermostat call:
+ * hist[nsteps][8] = T_part, potential energy, xi_1..xi_3 (the chain's
+ * eta_d), and the integrals of xi_1..xi_3 over time, each thermostat call
+ * adding its span dt/2 times the chain at its mid state. */
+int o_md_nvt_run_hist(o_md*, double* v, const double* masses, double kB, double T,
+                      double dt, int nsteps, double* out7, double* T_part_out, double* hist);
+/* ThermostatNHC(dt, t_relax, 3, niters) (src/thermostat.cpp:19-104) called
+ * ncalls times on a reduced temperature T_r that nothing else changes;
+ * hist[ncalls][4] (may be NULL) = T_r, xi_1..xi_3 after each call. Returns
+ * the product of the velocity rescales. */
+double o_nhc_run(double T_r, int ndof, double dt, double t_relax, int niters, int ncalls, double* hist);
+/* Test hook (0 = the reference's thermostat): plants a known mistake in the
+ * thermostat, 1 ddt and ddt2 swapped, 2 the sign of the forward sweep's chain
+ * coupling flipped, 3 N_f xi_1^2 in the reverse sweep too. */
+extern int o_nhc_mutation;
 /* AtomsMD::create_T (src/atoms_md.cpp:130-160) */
 void o_create_T(int natms, const uint8_t* elem, const double* masses, double kB,
                 double T, int seed, double* v);

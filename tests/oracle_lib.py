@@ -57,6 +57,9 @@ def L():
             "o_md_force": (None, [C.c_void_p, dp, dp, dp]),
             "o_md_energy": (C.c_double, [C.c_void_p]),
             "o_md_nvt_run": (C.c_int, [C.c_void_p, dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp]),
+            "o_md_nvt_run_hist": (C.c_int, [C.c_void_p, dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp,
+                                            dp, dp]),
+            "o_nhc_run": (C.c_double, [C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, dp]),
             "o_create_T": (None, [C.c_int, u8p, dp, C.c_double, C.c_double, C.c_int, dp]),
             "o_dmd_create": (C.c_void_p, [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, dp, u8p, dp, dp, C.c_double,
                                           C.c_double, C.c_double, C.c_double, dp]),
@@ -209,10 +212,45 @@ class OMD:
         nreb = L().o_md_nvt_run(self.h, P(v), P(masses), kB, T, dt, nsteps, P(out), C.byref(Tp))
         return v, out, Tp.value, nreb
 
+    def nvt_run_hist(self, v, masses, kB, T, dt, nsteps):
+        """nvt_run with the per-step record of o_md_nvt_run_hist: columns
+        T_part, potential energy, xi_1..3, integral of xi_1..3."""
+        v = f64(v).copy()
+        masses = f64(masses)
+        out = np.zeros(7)
+        Tp = C.c_double()
+        hist = np.zeros((nsteps, 8))
+        nreb = L().o_md_nvt_run_hist(self.h, P(v), P(masses), kB, T, dt, nsteps, P(out), C.byref(Tp), P(hist))
+        return v, out, hist, nreb
+
     def __del__(self):
         if getattr(self, "h", None):
             L().o_md_free(self.h)
             self.h = None
+
+
+def nhc_run(T_r, ndof, dt, t_relax, ncalls, niters=1):
+    """the oracle's thermostat alone on a reduced temperature: returns the
+    product of the rescales and hist[ncalls][4] = T_r, xi_1..3 per call"""
+    hist = np.zeros((ncalls, 4))
+    scale = L().o_nhc_run(T_r, ndof, dt, t_relax, niters, ncalls, P(hist))
+    return scale, hist
+
+
+class nhc_mutation:
+    """context manager planting one of the oracle's thermostat mistakes"""
+
+    def __init__(self, k):
+        self.k = k
+
+    def __enter__(self):
+        self.var = C.c_int.in_dll(L(), "o_nhc_mutation")
+        self.var.value = self.k
+        return self
+
+    def __exit__(self, *exc):
+        self.var.value = 0
+        return False
 
 
 def o_create_T(elem, masses, kB, T, seed):

@@ -145,22 +145,137 @@ def test_small_box_multiple_images(cu):
     np.testing.assert_allclose(f, fa, atol=2e-5)
 
 
-def test_nvt_conserves_extended_energy_trend(cu):
-    # NVE-like check at short times: a tiny-thermostat NVT run keeps PE + KE drift small
+# ---------------- the Nose-Hoover chain of the NVT driver ----------------
+# ThermostatNHC::operator() (src/thermostat.cpp:47-104) integrates a chain of
+# three thermostat velocities xi_1..3 (eta_d) acting on the reduced temperature
+# T_r = T_part / T. One call is a reverse sweep (xi_3, xi_2, xi_1), the velocity
+# scale exp(-ddt xi_1), and a forward sweep (xi_1, xi_2, xi_3), each chain
+# update the exact flow of y' = -a y + b over ddt2 with a, b frozen. The two
+# sweeps are not mirror images: the reverse sweep drives xi_2 by xi_1^2 - w^2
+# (src/thermostat.cpp:79), the forward sweep by N_f xi_1^2 - w^2 (:94). A call
+# is therefore consistent, at first order in ddt, with the chain
+#     T_r'  = -2 xi_1 T_r
+#     xi_1' = w^2 (T_r - 1) - xi_1 xi_2
+#     xi_2' = c xi_1^2 - w^2 - xi_2 xi_3,     c = (1 + N_f) / 2
+#     xi_3' = xi_2^2 - w^2
+# with w = 1 / t_relax. These equations are integrated here independently
+# (scipy DOP853) and the oracle is held to them. The same chain conserves
+#     H' = K + U + Q1 xi_1^2 / 2 + Q2 (xi_2^2 + xi_3^2) / 2
+#          + N_f kT eta_1 + (N_f kT / c) (eta_2 + eta_3)
+# with Q1 = N_f kT t_relax^2, Q2 = Q1 / c and eta_j the time integral of xi_j
+# (dH'/dt = 0 term by term, as for the textbook chain, where c = N_f). Each
+# test also plants a mistake in the oracle's thermostat and shows it fails:
+# ddt and ddt2 swapped, the forward coupling's sign flipped, and the textbook
+# drive N_f xi_1^2 in both sweeps.
+
+
+def chain_ode(n_f, w2, T_r0, ts, c):
+    from scipy.integrate import solve_ivp
+
+    def rhs(t, y):
+        tr, a, b, g = y
+        return [-2.0 * a * tr, w2 * (tr - 1.0) - a * b, c * a * a - w2 - b * g, b * b - w2]
+
+    sol = solve_ivp(rhs, (0.0, ts[-1]), [T_r0, 0.0, 0.0, 0.0], method="DOP853", rtol=1e-13, atol=1e-15,
+                    t_eval=ts)
+    assert sol.success
+    return sol.y
+
+
+def test_thermostat_converges_to_the_chain_equations():
+    # the thermostat alone on a reduced temperature, t_relax = 1, over six
+    # t_relax at shrinking steps: first order towards c = (1 + N_f) / 2
+    n_f, T_r0 = 24, 2.5
+
+    def err(k, c, mutation=0):
+        dt = 1.0 / k
+        with O.nhc_mutation(mutation):
+            _, h = O.nhc_run(T_r0, n_f, dt, 1.0, 6 * k)
+        y = chain_ode(n_f, 1.0, T_r0, dt * np.arange(1, 6 * k + 1), c)
+        return np.abs(h[:, 0] - y[0]).max()
+
+    c = (1 + n_f) / 2.0
+    e = [err(k, c) for k in (100, 200, 400)]
+    # measured 4.3e-3, 2.2e-3, 1.1e-3
+    assert e[2] < 2e-3
+    for a, b in zip(e, e[1:]):
+        assert 1.8 < a / b < 2.2, e
+    # the textbook drive in both sweeps makes the call symmetric: second order
+    # towards c = N_f, which is what fixes c for the reference's sweeps
+    e3 = [err(k, float(n_f), 3) for k in (100, 200)]
+    assert 3.6 < e3[0] / e3[1] < 4.4 and e3[1] < 1e-4, e3
+    for mutation in (1, 2, 3):
+        assert err(400, c, mutation) > 0.05, mutation
+
+
+def zero_force_cu():
+    # F = 0 and phi = 0 leave every force exactly zero; rho keeps the cut-off
+    p = pot.johnson_cu(nr=2000, nrho=2000)
+    q = pot.EamKnots(p.names, p.masses, p.nr, p.dr, p.nrho, p.drho, np.zeros_like(p.F), p.rho,
+                     np.zeros_like(p.r_phi), False)
+    return O.OEam.from_knots(q)
+
+
+def test_zero_force_nvt_temperature_follows_the_chain_equations():
+    # MDNVT::run (src/md_nvt.cpp:405-470) calls ThermostatNHC(dt/2, 100 dt, 3, 1)
+    # twice per step; without forces nothing else changes T_part
+    ff = zero_force_cu()
+    x, H = pot.fcc(3.615, (2, 2, 2))
+    n = len(x)
+    elem = np.zeros(n, np.uint8)
+    masses = [pot.MASS["Cu"]]
+    v = O.o_create_T(elem, masses, pot.KB_EV, 900.0, 3)
+    T0 = (masses[0] * (v ** 2).sum()) / (3 * n * pot.KB_EV)
+    dt, T, nsteps = pot.FS, 300.0, 800  # eight t_relax
+    ts = dt * np.arange(1, nsteps + 1)
+    y = chain_ode(3 * n, 1.0 / (100.0 * dt) ** 2, T0 / T, ts, (1 + 3 * n) / 2.0)
+
+    def run(mutation=0):
+        with O.nhc_mutation(mutation):
+            md = O.OMD(ff, H, x, elem, skin=0.5)
+            v1, out, h, _ = md.nvt_run_hist(v, masses, pot.KB_EV, T, dt, nsteps)
+        assert out[0] == 0.0
+        return h
+
+    h = run()
+    # measured 2.5e-3 at w dt = 0.01 (T_r from 2.8 to 0.9)
+    assert np.abs(h[:, 0] / T - y[0]).max() < 5e-3
+    # the chain itself, in units of w: measured 2.2e-2 (xi_2 at the end)
+    np.testing.assert_allclose(h[:, 2:5] * (100.0 * dt), y[1:].T * (100.0 * dt), atol=4e-2)
+    for mutation in (1, 2, 3):
+        assert np.abs(run(mutation)[:, 0] / T - y[0]).max() > 0.1, mutation
+
+
+def test_nvt_conserves_the_chains_extended_energy(cu):
+    # with forces, cooling from about 600 K to a 300 K target over 20 t_relax
     p, ff = cu
     x, H = pot.fcc(3.615, (3, 3, 3))
-    x = jitter(x, 0.02, 12)
-    elem = np.zeros(len(x), np.uint8)
+    x = jitter(x, 0.05, 12)
+    n = len(x)
+    n_f = 3 * n
+    elem = np.zeros(n, np.uint8)
     masses = [pot.MASS["Cu"]]
-    v = O.o_create_T(elem, masses, pot.KB_EV, 300.0, 7)
-    v -= v.mean(0)
-    md = O.OMD(ff, H, x, elem)
-    _, e0, _, _ = md.force()
-    ke0 = 0.5 * masses[0] * (v ** 2).sum()
-    T0 = 2 * ke0 / (3 * len(x) * pot.KB_EV)
-    v1, out, Tp, nreb = md.nvt_run(v, masses, pot.KB_EV, T0, 0.2 * pot.FS, 20)
-    ke1 = 0.5 * masses[0] * (v1 ** 2).sum()
-    # thermostat target = initial temperature: over 4 fs PE + KE moves by
-    # the thermostat's small work only (measured 3e-4 eV of 4.6 eV)
-    assert abs((out[0] + ke1) - (e0 + ke0)) < 1e-3 * abs(ke0)
-    assert Tp > 0
+    v = O.o_create_T(elem, masses, pot.KB_EV, 600.0, 7)
+    dt, T, nsteps = pot.FS, 300.0, 2000
+    kT, tau = pot.KB_EV * T, 100.0 * dt
+    c = (1 + n_f) / 2.0
+    Q1 = n_f * kT * tau ** 2
+
+    def run(mutation=0):
+        with O.nhc_mutation(mutation):
+            md = O.OMD(ff, H, x, elem, skin=0.5)
+            _, e0, _, _ = md.force()
+            _, _, h, nreb = md.nvt_run_hist(v, masses, pot.KB_EV, T, dt, nsteps)
+        k = 0.5 * n_f * pot.KB_EV * h[:, 0]
+        ext = (k + h[:, 1] + 0.5 * Q1 * h[:, 2] ** 2 + 0.5 * (Q1 / c) * (h[:, 3] ** 2 + h[:, 4] ** 2)
+               + n_f * kT * h[:, 5] + (n_f * kT / c) * (h[:, 6] + h[:, 7]))
+        h0 = 0.5 * masses[0] * (v ** 2).sum() + e0
+        return np.abs(ext - h0).max(), (k + h[:, 1])[-1] - h0, nreb
+
+    drift, work, nreb = run()
+    assert nreb > 10 and work < -3.0  # the chain removed several eV
+    # measured 3.6e-3 eV against 3.8 eV of thermostat work
+    assert drift < 1e-2, drift
+    for mutation in (1, 2, 3):
+        # measured 3.8, 196 and 0.52 eV
+        assert run(mutation)[0] > 0.2, mutation
