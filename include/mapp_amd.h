@@ -305,7 +305,9 @@ mapp_status mapp_dmd_smear(mapp_dmd* d, int ei, int ej, int npts, const double* 
  * below e_tol (the reference's criterion) or when max |gradient component|
  * < grad_tol (grad_tol <= 0 disables). out4 = fe0, fe, |grad|_2, |grad|_max;
  * status: 1 zero gradient, 2 not downhill, 3 grad_tol reached, 4 line search
- * failed, 5 e_tol reached, 0 iteration limit. */
+ * failed (minimum alpha reached, or for golden / Brent no bracket found),
+ * 5 e_tol reached, 0 iteration limit; the same codes for every minimiser and
+ * line search of mapp_dmd_min_lbfgs and mapp_dmd_min_run. */
 mapp_status mapp_dmd_min_cg(mapp_dmd* d, int max_iter, double e_tol, double max_dx, double max_dalpha,
                             double grad_tol, int* iters, int* status, double* out4);
 /* MinLBFGSDMD::__run over x and alpha with c fixed (src/min_l-bfgs_dmd.h:113-210,

@@ -293,6 +293,24 @@ mapp_status mapp_md_minimize(mapp_md* md, const mapp_min_params* params, mapp_mi
     });
 }
 
+// the line-search fields of a mapp_min_params, checked: every search ends
+// for parameters that pass (mapp::line_search_params_error)
+static mapp::LineSearchParams line_search_params(const mapp_min_params& q)
+{
+    mapp::LineSearchParams lp;
+    lp.kind = q.ls;
+    lp.c = q.ls_c;
+    lp.rho = q.ls_rho;
+    lp.min_alpha = q.ls_min_alpha;
+    lp.max_iter = q.ls_max_iter;
+    lp.brack = q.ls_brack;
+    lp.tol = q.ls_tol;
+    lp.zeps = q.ls_zeps;
+    const char* err = mapp::line_search_params_error(lp);
+    MAPP_REQUIRE(err == nullptr, mapp::ERR_ARG, err ? err : "");
+    return lp;
+}
+
 mapp_status mapp_line_search(const mapp_min_params* params, double dfa, double max_a, int init_flag,
                              double* prev_val, mapp_line_energy_fn energy, mapp_line_reset_fn reset, void* ctx,
                              double* energy_io, double* alpha, int* status)
@@ -305,16 +323,7 @@ mapp_status mapp_line_search(const mapp_min_params* params, double dfa, double m
         NEED(energy_io);
         NEED(alpha);
         NEED(status);
-        MAPP_REQUIRE(params->ls >= 0 && params->ls <= 2, mapp::ERR_ARG, "line search must be 0, 1 or 2");
-        mapp::LineSearchParams lp;
-        lp.kind = params->ls;
-        lp.c = params->ls_c;
-        lp.rho = params->ls_rho;
-        lp.min_alpha = params->ls_min_alpha;
-        lp.max_iter = params->ls_max_iter;
-        lp.brack = params->ls_brack;
-        lp.tol = params->ls_tol;
-        lp.zeps = params->ls_zeps;
+        const mapp::LineSearchParams lp = line_search_params(*params);
         auto f = [&](double t) { return energy(ctx, t); };
         auto r = [&]() { reset(ctx); };
         mapp::LineSearcher<decltype(f), decltype(r)> ls(lp, f, r);
@@ -672,18 +681,9 @@ mapp_status mapp_dmd_min_run(mapp_dmd* d, int lbfgs, int m, int max_iter, double
                              double* out4)
 {
     return guard([&] {
-        NEED(d);
         NEED(ls);
-        MAPP_REQUIRE(ls->ls >= 0 && ls->ls <= 2, mapp::ERR_ARG, "line search must be 0, 1 or 2");
-        mapp::LineSearchParams lp;
-        lp.kind = ls->ls;
-        lp.c = ls->ls_c;
-        lp.rho = ls->ls_rho;
-        lp.min_alpha = ls->ls_min_alpha;
-        lp.max_iter = ls->ls_max_iter;
-        lp.brack = ls->ls_brack;
-        lp.tol = ls->ls_tol;
-        lp.zeps = ls->ls_zeps;
+        const mapp::LineSearchParams lp = line_search_params(*ls);
+        NEED(d);
         const mapp::CGReport r = lbfgs ? d->sys->lbfgs(m, max_iter, e_tol, max_dx, max_dalpha, grad_tol, &lp)
                                        : d->sys->cg(max_iter, e_tol, max_dx, max_dalpha, grad_tol, &lp);
         if (iters) *iters = r.iters;

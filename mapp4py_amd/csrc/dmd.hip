@@ -2447,6 +2447,17 @@ void DMDSystem::smear(int ei, int ej, int npts, const double* r, const double* a
 // MinDMDHandler::ls_prep (max_dx, max_dalpha, alpha kept positive). Stops on
 // the energy decrement e_tol like the reference, or when the largest
 // gradient component drops below grad_tol (0 disables).
+// The DMD minimisers' status for a failed line search, the same for CG and
+// L-BFGS: 1 zero gradient (LS_F_GRAD0), 2 not downhill (LS_F_DOWNHILL), 4 any
+// other failure (minimum alpha reached, or a bracketing failure of the golden
+// or Brent search).
+static int dmd_min_status(int ls)
+{
+    if (ls == LS_F_GRAD0) return 1;
+    if (ls == LS_F_DOWNHILL) return 2;
+    return 4;
+}
+
 CGReport DMDSystem::cg(int max_iter, double e_tol, double max_dx, double max_dalpha, double grad_tol,
                        const LineSearchParams* ls)
 {
@@ -2549,8 +2560,9 @@ CGReport DMDSystem::cg(int max_iter, double e_tol, double max_dx, double max_dal
         if (dfa >= 0.0 || !(max_a > 0.0)) { status = 2; break; }
         // LineSearchBackTrack::min with init_flag 1 (src/ls.h:711-769)
         double t = 0.0;
-        if (searcher.min(e_curr, t, dfa, max_a, 1) != LS_S) {
-            status = 4;
+        const int ls = searcher.min(e_curr, t, dfa, max_a, 1);
+        if (ls != LS_S) {
+            status = dmd_min_status(ls);
             break;
         }
         d = derivative(false, nullptr);
@@ -2724,7 +2736,7 @@ CGReport DMDSystem::lbfgs(int m, int max_iter, double e_tol, double max_dx, doub
         double alpha_m = 0.0;
         const int ls = searcher.min(e_curr, alpha_m, dfa, max_a, 0);
         if (ls != LS_S) {
-            status = ls == LS_F_GRAD0 ? 1 : ls == LS_MIN_ALPHA ? 4 : 2;
+            status = dmd_min_status(ls);
             break;
         }
         d = derivative(false, nullptr);

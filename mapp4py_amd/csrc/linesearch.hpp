@@ -21,6 +21,22 @@ struct LineSearchParams {
     double tol = 2.1073424255447017e-08, zeps = 2.220446049250313e-19;
 };
 
+// The parameter ranges under which every search ends: 0 < c < 1 and
+// 0 < rho < 1 (the back-tracking step shrinks), min_alpha >= 0, max_iter,
+// tol and zeps >= 0, all finite. Returns nullptr when p is valid, else the
+// message of the first violated condition.
+inline const char* line_search_params_error(const LineSearchParams& p)
+{
+    if (p.kind < 0 || p.kind > 2) return "line search must be 0 (back-track), 1 (golden) or 2 (Brent)";
+    if (!(p.c > 0.0 && p.c < 1.0)) return "line search c must lie in (0, 1)";
+    if (!(p.rho > 0.0 && p.rho < 1.0)) return "line search rho must lie in (0, 1)";
+    if (!(p.min_alpha >= 0.0 && std::isfinite(p.min_alpha))) return "line search min_alpha must be finite and >= 0";
+    if (p.max_iter < 0) return "line search max_iter must be non-negative";
+    if (!(p.tol >= 0.0 && std::isfinite(p.tol)) || !(p.zeps >= 0.0 && std::isfinite(p.zeps)))
+        return "line search tol and zeps must be finite and >= 0";
+    return nullptr;
+}
+
 template <class Fn, class Reset>
 class LineSearcher {
 public:
@@ -51,9 +67,12 @@ private:
         const double init_energy = nrgy;
         if (init_flag == 0) {
             max_a = std::min(max_a, 1.0);
-        } else {
+        } else if (init_flag == 1) {
             if (prev_val > 0.0 && dfa < 0.0) max_a = std::min(max_a, -prev_val / dfa);
             prev_val = -dfa;
+        } else if (init_flag == 2) {
+            if (prev_val > 0.0 && dfa < 0.0) max_a = std::min(max_a, std::min(1.0, 2.02 * (prev_val + nrgy) / dfa));
+            prev_val = -nrgy;
         }
         if (max_a <= p_.min_alpha) return (int)LS_MIN_ALPHA;
         alpha = max_a;
@@ -264,6 +283,7 @@ private:
             nrgy = f2;
             alpha = x2;
         }
+        prev_val = -dfa * alpha;
         return (int)LS_S;
     }
     // LineSearchBrent::min (src/ls.h:525-706)
@@ -291,6 +311,7 @@ private:
                 if (u != x) f_(x);
                 nrgy = fx;
                 alpha = x;
+                prev_val = alpha;
                 return (int)LS_S;
             }
             if (std::fabs(e) > tol1) {
@@ -354,6 +375,7 @@ private:
         if (u != x) f_(x);
         nrgy = fx;
         alpha = x;
+        prev_val = alpha;
         return (int)LS_S;
     }
 

@@ -1217,9 +1217,11 @@ void MDSystem::build_list()
     throw Error(ERR_STATE, "neighbor list capacity did not converge");
 }
 
-void MDSystem::force_passes(const KickArgs* kick, bool records_fresh)
+void MDSystem::force_passes(const KickArgs* kick, bool records_fresh, float4* f_out, double* rho_out)
 {
     const int nb = grid_for(n_);
+    float4* fo = f_out ? f_out : f_.get();
+    double* ro = rho_out ? rho_out : rho_.get();
     const EamArgs P = ff_->args();
     prof_.begin(K_EAM_RHO, stream_);
     if (!records_fresh)
@@ -1237,7 +1239,7 @@ void MDSystem::force_passes(const KickArgs* kick, bool records_fresh)
         const int G = std::min(lds_grid_, nb);
 #define MAPP_RHO(B) hipLaunchKernelGGL((k_eam_rho_lds<B>), dim3(G), dim3(kBigBlock), rho_lds_bytes_, stream_, n_, \
                                        rec_.get(), nn_.get(), list_.get(), ldl_, maxn_, P, r16_, rho_m_lo_, rho_nwin_, \
-                                       rho_.get(), xr_.get(), part_.get())
+                                       ro, xr_.get(), part_.get())
         switch (batch_rho_) {
             case 1: MAPP_RHO(1); break;
             case 2: MAPP_RHO(2); break;
@@ -1250,7 +1252,7 @@ void MDSystem::force_passes(const KickArgs* kick, bool records_fresh)
         double* p2 = part_.get() + (size_t)G * kPartW;
 #define MAPP_FORCE(B) hipLaunchKernelGGL((k_eam_force_lds<B>), dim3(G), dim3(kBigBlock), f_lds_bytes_, stream_, n_, \
                                          xr_.get(), r16_, nn_.get(), list_.get(), ldl_, maxn_, P, f_m_lo_, f_nwin_, \
-                                         f_.get(), p2, K, v_.get(), p3)
+                                         fo, p2, K, v_.get(), p3)
         // 1024-thread workgroups leave 128 VGPRs per lane: batches above 2 would spill
         if (batch_force_ == 1)
             MAPP_FORCE(1);
@@ -1263,7 +1265,7 @@ void MDSystem::force_passes(const KickArgs* kick, bool records_fresh)
         return;
     }
 #define MAPP_RHO(B, ONE) hipLaunchKernelGGL((k_eam_rho<B, ONE>), dim3(nb), dim3(kBlock), 0, stream_, n_, rec_.get(), \
-                                            nn_.get(), list_.get(), ldl_, maxn_, P, r16_, rho_.get(), xr_.get(), \
+                                            nn_.get(), list_.get(), ldl_, maxn_, P, r16_, ro, xr_.get(), \
                                             dF_.get(), part_.get())
     switch (batch_rho_) {
         case 1: if (one) MAPP_RHO(1, true); else MAPP_RHO(1, false); break;
@@ -1277,7 +1279,7 @@ void MDSystem::force_passes(const KickArgs* kick, bool records_fresh)
     double* p2 = part_.get() + (size_t)nb * kPartW;
 #define MAPP_FORCE(B, ONE) hipLaunchKernelGGL((k_eam_force<B, ONE>), dim3(nb), dim3(kBlock), 0, stream_, n_, \
                                               rec_.get(), xr_.get(), dF_.get(), r16_, nn_.get(), list_.get(), ldl_, \
-                                              maxn_, P, f_.get(), p2, K, v_.get(), p3)
+                                              maxn_, P, fo, p2, K, v_.get(), p3)
     switch (batch_force_) {
         case 1: if (one) MAPP_FORCE(1, true); else MAPP_FORCE(1, false); break;
         case 2: if (one) MAPP_FORCE(2, true); else MAPP_FORCE(2, false); break;
@@ -1313,15 +1315,7 @@ double MDSystem::value()
     // that the last force_calc's stay readable
     val_f_.reserve(n_);
     val_rho_.reserve(n_);
-    std::swap(f_.p, val_f_.p);
-    std::swap(f_.n, val_f_.n);
-    std::swap(rho_.p, val_rho_.p);
-    std::swap(rho_.n, val_rho_.n);
-    force_passes(nullptr);
-    std::swap(f_.p, val_f_.p);
-    std::swap(f_.n, val_f_.n);
-    std::swap(rho_.p, val_rho_.p);
-    std::swap(rho_.n, val_rho_.n);
+    force_passes(nullptr, false, val_f_.get(), val_rho_.get());
     double e = 0.0;
     MAPP_HIP(hipMemcpyAsync(&e, red_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
     MAPP_HIP(hipStreamSynchronize(stream_));

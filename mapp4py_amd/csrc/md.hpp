@@ -150,7 +150,10 @@ private:
     void setup_grid();
     void build_list();
     // records_fresh: rec_ already holds the current positions (written by the skin check or a rebuild)
-    void force_passes(const KickArgs* kick, bool records_fresh = false);
+    // the two EAM passes; f_out / rho_out (nullptr: f_ and rho_) receive the
+    // forces and densities
+    void force_passes(const KickArgs* kick, bool records_fresh = false, float4* f_out = nullptr,
+                      double* rho_out = nullptr);
     MDReport reduce_report(int nparts);
     void reduce_rows(const double* part, int nrows, int ncols, double* out);
     bool decide();

@@ -231,11 +231,17 @@ MinReport MDSystem::minimize(const MinParams& P, std::vector<double>* tally)
     MAPP_REQUIRE(P.m >= 0 && P.m <= kMaxLbfgsM, ERR_ARG, "L-BFGS memory out of range");
     MAPP_REQUIRE(P.max_nsteps >= 0, ERR_ARG, "max_nsteps must be non-negative");
     MAPP_REQUIRE(P.max_dx > 0.0, ERR_ARG, "max_dx must be positive");
-    MAPP_REQUIRE(P.ls_c > 0.0 && P.ls_c < 1.0, ERR_ARG, "line search c must lie in (0, 1)");
-    MAPP_REQUIRE(P.ls_rho > 0.0 && P.ls_rho < 1.0, ERR_ARG, "line search rho must lie in (0, 1)");
-    MAPP_REQUIRE(P.ls_min_alpha >= 0.0, ERR_ARG, "line search min_alpha must be non-negative");
-    MAPP_REQUIRE(P.ls >= 0 && P.ls <= 2, ERR_ARG, "line search must be 0 (back-track), 1 (golden) or 2 (Brent)");
-    MAPP_REQUIRE(P.ls_max_iter >= 0 && P.ls_tol >= 0.0 && P.ls_zeps >= 0.0, ERR_ARG, "invalid line search tolerances");
+    LineSearchParams lp;
+    lp.kind = P.ls;
+    lp.c = P.ls_c;
+    lp.rho = P.ls_rho;
+    lp.min_alpha = P.ls_min_alpha;
+    lp.max_iter = P.ls_max_iter;
+    lp.brack = P.ls_brack;
+    lp.tol = P.ls_tol;
+    lp.zeps = P.ls_zeps;
+    const char* ls_err = line_search_params_error(lp);
+    MAPP_REQUIRE(ls_err == nullptr, ERR_ARG, ls_err ? ls_err : "");
     const bool lbfgs = P.lbfgs == 1;
     const int m = lbfgs ? P.m : 0;
     mn_x0_.reserve(n_);
@@ -369,15 +375,6 @@ MinReport MDSystem::minimize(const MinParams& P, std::vector<double>* tally)
         f_fresh = true;
         return r.e[0];
     };
-    LineSearchParams lp;
-    lp.kind = P.ls;
-    lp.c = P.ls_c;
-    lp.rho = P.ls_rho;
-    lp.min_alpha = P.ls_min_alpha;
-    lp.max_iter = P.ls_max_iter;
-    lp.brack = P.ls_brack;
-    lp.tol = P.ls_tol;
-    lp.zeps = P.ls_zeps;
     auto f_trial = [&](double t) { return F(t, true); };
     auto f_reset = [&]() { F(0.0, false); };
     LineSearcher<decltype(f_trial), decltype(f_reset)> searcher(lp, f_trial, f_reset);

@@ -197,6 +197,36 @@ def test_nvt_matches_oracle_short_run(M, cu_pot):
     assert abs(out[0] - outo[0]) < E_ATOM * len(x)
 
 
+def test_feni_nvt_matches_oracle(M, feni_pot):
+    # the two-element eam/fs path (global-table passes, the F' array, the
+    # fused second half-kick) through 250 NVT steps at 600 K in a sheared box,
+    # with on-device rebuilds: MDNVT::run (src/md_nvt.cpp:383-480) over
+    # ForceFieldEAM::__force_calc (src/ff_eam.cpp:99-239) with the FS density
+    # convention rho[e_j][e_i] (src/import_eam.h:836-851)
+    x, H = pot.bcc(2.86, (8, 8, 8))
+    shear = np.eye(3)
+    shear[1, 0], shear[2, 0], shear[2, 1] = 0.04, -0.03, 0.05
+    x, H = jitter(x @ shear, 0.05, 31), H @ shear
+    elem = (np.random.default_rng(32).random(len(x)) < 0.2).astype(np.uint8)
+    masses = [pot.MASS["Fe"], pot.MASS["Ni"]]
+    v = O.o_create_T(elem, masses, pot.KB_EV, 600.0, 5)
+    g, o = both(M, feni_pot, H, x, elem)
+    g.x_d = v
+    nvt = M.NVT(600.0, 1.0 * pot.FS)
+    out = nvt.run(g, 250, masses)
+    vo, outo, Tpo, nreb_o = o.nvt_run(v, masses, pot.KB_EV, 600.0, 1.0 * pot.FS, 250)
+    dx = np.abs(g.x - o.x).max()
+    dv = np.abs(g.x_d - vo).max()
+    say(f"Fe-Ni NVT 250 steps, {len(x)} atoms ({int(elem.sum())} Ni): max|dx|={dx:.2e} A  max|dv|={dv:.2e}  "
+        f"T {nvt.T_part:.6f} vs {Tpo:.6f}  rebuilds {nvt.rebuilds} vs {nreb_o}  dE {out[0] - outo[0]:.2e} eV")
+    assert nvt.rebuilds == nreb_o >= 2
+    assert dx < 1e-6 and dv < 1e-6
+    assert abs(nvt.T_part - Tpo) < 1e-6 * Tpo
+    assert abs(out[0] - outo[0]) < E_ATOM * len(x) + 1e-6
+    # forces at the GPU's final positions against a fresh oracle
+    check_forces(g, O.OMD(O.OEam.from_knots(feni_pot), H, g.x, elem), "Fe-Ni after 250 NVT steps")
+
+
 def test_nvt_long_run_forces_stay_exact(M, cu_pot):
     # after many steps and rebuilds the GPU list must still cover every pair
     # within r_c: compare with a fresh oracle at the GPU's positions

@@ -165,6 +165,21 @@ def test_golden_and_brent_follow_the_oracle(M, cu_pot, which, ls):
     mn = M.MinCG(e_tol=0.0, ls=gls) if which == "cg" else M.MinLBFGS(m=2, e_tol=0.0, ls=gls)
     rg = mn.run(g, 12)
     tg = mn.tally
+    # the oracle's trial energies of every step's search (its start energy first)
+    trials = []
+    search = ols.min
+
+    def recorded(hd, e, flag):
+        es = [e]
+        evaluate = hd.F
+        hd.F = lambda a: (lambda v: (es.append(v), v)[1])(evaluate(a))
+        try:
+            return search(hd, e, flag)
+        finally:
+            del hd.F
+            trials.append(es)
+
+    ols.min = recorded
     ro = (o_min.min_cg(o, 12, e_tol=0.0, ls=ols) if which == "cg"
           else o_min.min_lbfgs(o, 12, m=2, e_tol=0.0, ls=ols))
     de = np.abs(tg[:, 1] - ro["tally"][:, 1]).max() if len(tg) == len(ro["tally"]) else np.inf
@@ -174,11 +189,26 @@ def test_golden_and_brent_follow_the_oracle(M, cu_pot, which, ls):
     if len(tg) == len(ro["tally"]):
         say("  per step dE: " + " ".join(f"{v:.1e}" for v in tg[:, 1] - ro["tally"][:, 1]))
     assert rg["steps"] == ro["steps"] and rg["status"] == ro["status"]
-    # golden section and Brent compare trial energies that differ by less than
-    # the FP32 rounding of the GPU energies near the line minimum; a flipped
-    # comparison moves the accepted point inside the search's tolerance (seen
-    # 1.3e-4 eV at one step, the trajectories then rejoin)
-    assert de < E_ATOM * n + 1e-6 + 3e-4
+    # golden section and Brent compare trial energies that may differ by less
+    # than the FP32 rounding of the GPU energies near the line minimum; a
+    # flipped comparison moves the accepted point inside the search's
+    # tolerance (seen 1.3e-4 eV at one step, the trajectories then rejoin).
+    # Every step meets the per-step bar up to the first step whose oracle
+    # search compared two energies closer than that bar; only from there on
+    # may the accepted energies differ by more, and at no more than three
+    # steps (a search that accepted a different point every time would
+    # differ at every step).
+    bar = E_ATOM * n + 1e-6
+    d = np.abs(tg[:, 1] - ro["tally"][:, 1])
+    late = np.nonzero(d > bar)[0]
+    if len(late):
+        k = int(late[0])
+        es = np.array(trials[k - 1])
+        gaps = np.abs(es[:, None] - es[None, :])[~np.eye(len(es), dtype=bool)]
+        say(f"  steps over the bar: {late.tolist()}, closest trial energies of step {k}: {gaps.min():.1e} eV")
+        assert gaps.min() < bar, (k, d[k], gaps.min())
+        assert len(late) <= 3, late
+    assert de < bar + 3e-4
     assert dx < 5e-4
 
 

@@ -105,6 +105,8 @@ def test_library_line_search_follows_the_oracle(name, kind, kw):
     assert rl[1] == pytest.approx(ro[1], rel=1e-12, abs=1e-15)
     if rl[0] == o_min.LS_S:
         assert rl[2] == pytest.approx(ro[2], rel=1e-12)
+        # the memory each search leaves (src/ls.h:516,584,705,765)
+        assert rl[4] == pytest.approx(ro[4], rel=1e-12)
 
 
 def test_prep_failures_and_the_backtrack_memory():
@@ -121,13 +123,49 @@ def test_prep_failures_and_the_backtrack_memory():
         same_calls(rl[3], ro[3])
         pv_l, pv_o = rl[4], ro[4]
         assert pv_l == pytest.approx(pv_o, rel=1e-14)
+    # init_flag 2: the previous energy caps the next start
+    # (src/ls.h:745-750); prev_val keeps minus the energy
+    pv_l = pv_o = -1.0
+    for dfa, e0 in ((-0.6, 0.5), (-0.2, 0.3), (-0.05, 0.29)):
+        rl = lib_search(0, fn, dfa, 1.0, e0, init_flag=2, prev_val=pv_l)
+        ro = oracle_search(0, fn, dfa, 1.0, e0, init_flag=2, prev_val=pv_o)
+        assert rl[0] == ro[0]
+        same_calls(rl[3], ro[3])
+        pv_l, pv_o = rl[4], ro[4]
+        assert pv_l == pv_o == -e0
+    # a flag outside 0..2 leaves max_a and the memory as they are
+    rl = lib_search(0, fn, -0.6, 3.0, fn(0.0), init_flag=5, prev_val=0.7)
+    ro = oracle_search(0, fn, -0.6, 3.0, fn(0.0), init_flag=5, prev_val=0.7)
+    same_calls(rl[3], ro[3])
+    assert rl[4] == ro[4] == 0.7
 
 
-def test_rejects_an_unknown_search():
-    p = _lib.MinParams(1, 2, 1e-8, 1.0, 10, 0.4, 0.5, 0.0, 3, 5, 1, 1e-8, 1e-19)
+BAD_PARAMS = [  # (ls, c, rho, min_alpha, max_iter, tol, zeps), each with one value out of range
+    (3, 0.4, 0.5, 0.0, 5, 1e-8, 1e-19), (0, 1.0, 0.5, 0.0, 5, 1e-8, 1e-19), (0, 0.0, 0.5, 0.0, 5, 1e-8, 1e-19),
+    (0, 0.4, 1.0, 0.0, 5, 1e-8, 1e-19), (0, 0.4, math.inf, 0.0, 5, 1e-8, 1e-19), (0, 0.4, 0.5, -1.0, 5, 1e-8, 1e-19),
+    (1, 0.4, 0.5, 0.0, -1, 1e-8, 1e-19), (2, 0.4, 0.5, 0.0, 5, -1e-8, 1e-19), (2, 0.4, 0.5, 0.0, 5, 1e-8, math.nan),
+    (0, math.nan, 0.5, 0.0, 5, 1e-8, 1e-19)]
+
+
+@pytest.mark.parametrize("bad", BAD_PARAMS)
+def test_rejects_parameters_under_which_a_search_may_not_end(bad):
+    ls, c, rho, min_alpha, max_iter, tol, zeps = bad
+    p = _lib.MinParams(1, 2, 1e-8, 1.0, 10, c, rho, min_alpha, ls, max_iter, 1, tol, zeps)
     pv, e, a, st = C.c_double(-1.0), C.c_double(0.0), C.c_double(), C.c_int()
     ef = ENERGY(lambda ctx, x: x * x)
     rf = RESET(lambda ctx: None)
-    with pytest.raises(_lib.MappError):
+    with pytest.raises(_lib.MappError, match="line search"):
         _lib.check(_lib.lib().mapp_line_search(C.byref(p), -1.0, 1.0, 0, C.byref(pv), ef, rf, None, C.byref(e),
                                               C.byref(a), C.byref(st)))
+    # the DMD minimisers check the same fields before they touch the system
+    it, stat = C.c_int(), C.c_int()
+    out4 = (C.c_double * 4)()
+    for lbfgs in (0, 1):
+        with pytest.raises(_lib.MappError, match="line search"):
+            _lib.check(_lib.lib().mapp_dmd_min_run(None, lbfgs, 2, 10, 1e-8, 0.1, 0.01, 0.0, C.byref(p),
+                                                  C.byref(it), C.byref(stat), out4))
+    # valid parameters reach the handle check
+    good = _lib.MinParams(1, 2, 1e-8, 1.0, 10, 0.4, 0.5, 0.0, 0, 5, 1, 1e-8, 1e-19)
+    with pytest.raises(_lib.MappError, match="d must not be NULL"):
+        _lib.check(_lib.lib().mapp_dmd_min_run(None, 0, 2, 10, 1e-8, 0.1, 0.01, 0.0, C.byref(good), C.byref(it),
+                                              C.byref(stat), out4))
