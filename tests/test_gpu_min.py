@@ -170,9 +170,9 @@ def test_golden_and_brent_follow_the_oracle(M, cu_pot, which, ls):
     search = ols.min
 
     def recorded(hd, e, flag):
-        es = [e]
+        es = [(0.0, e)]
         evaluate = hd.F
-        hd.F = lambda a: (lambda v: (es.append(v), v)[1])(evaluate(a))
+        hd.F = lambda a: (lambda v: (es.append((float(a), v)), v)[1])(evaluate(a))
         try:
             return search(hd, e, flag)
         finally:
@@ -203,8 +203,9 @@ def test_golden_and_brent_follow_the_oracle(M, cu_pot, which, ls):
     late = np.nonzero(d > bar)[0]
     if len(late):
         k = int(late[0])
-        es = np.array(trials[k - 1])
-        gaps = np.abs(es[:, None] - es[None, :])[~np.eye(len(es), dtype=bool)]
+        a, es = np.array(trials[k - 1]).T
+        # energies at distinct steps (a search re-evaluates its accepted point)
+        gaps = np.abs(es[:, None] - es[None, :])[a[:, None] != a[None, :]]
         say(f"  steps over the bar: {late.tolist()}, closest trial energies of step {k}: {gaps.min():.1e} eV")
         assert gaps.min() < bar, (k, d[k], gaps.min())
         assert len(late) <= 3, late
