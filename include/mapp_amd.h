@@ -84,6 +84,15 @@ mapp_status mapp_md_force_calc(mapp_md* md, double* f, double* out7);
 /* ForceFieldMD::value (src/ff_md.cpp:148-155) with
  * ForceFieldEAM::__energy_calc (src/ff_eam.cpp:244-318). */
 mapp_status mapp_md_value(mapp_md* md, double* energy);
+/* mapp_md_value without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops, code that forbids hipMalloc). The same passes as mapp_md_value, the
+ * same energy; their forces and densities go to a static scratch array that
+ * the library holds on every GPU it runs on (24 MiB) instead of buffers that
+ * mapp_md_value allocates on its first call. At most 1,048,576 atoms
+ * (MAPP_ERR_CAPACITY beyond). Every call on one GPU uses that GPU's one
+ * array: calls that overlap on one GPU (two threads, two handles) race. */
+mapp_status mapp_md_value_noalloc(mapp_md* md, double* energy);
 /* per-atom electron density of the last force_calc (the "rho" vector) */
 mapp_status mapp_md_get_rho(mapp_md* md, double* rho);
 /* Half list as NeighborMD::create_list stores it (src/neighbor_md.cpp:82-152):

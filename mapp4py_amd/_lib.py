@@ -46,6 +46,7 @@ SIGNATURES = {
     "mapp_md_update": (C.c_int, [C.c_void_p, ip]),
     "mapp_md_force_calc": (C.c_int, [C.c_void_p, dp, dp]),
     "mapp_md_value": (C.c_int, [C.c_void_p, dp]),
+    "mapp_md_value_noalloc": (C.c_int, [C.c_void_p, dp]),
     "mapp_md_get_rho": (C.c_int, [C.c_void_p, dp]),
     "mapp_md_pairs_count": (C.c_int, [C.c_void_p, i64p]),
     "mapp_md_pairs": (C.c_int, [C.c_void_p, i32p, i32p, i8p]),

@@ -212,6 +212,15 @@ mapp_status mapp_md_value(mapp_md* md, double* energy)
     });
 }
 
+mapp_status mapp_md_value_noalloc(mapp_md* md, double* energy)
+{
+    return guard([&] {
+        NEED(md);
+        NEED(energy);
+        *energy = md->sys->value_noalloc();
+    });
+}
+
 mapp_status mapp_md_get_rho(mapp_md* md, double* rho)
 {
     return guard([&] { NEED(md); NEED(rho); md->sys->get_rho(rho); });

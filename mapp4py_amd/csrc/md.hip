@@ -11,8 +11,18 @@
 #include <cstring>
 #include <numeric>
 
+#include <mutex>
+
 #include "md.hpp"
 #include "kern.hpp"
+
+// Scratch rows of MDSystem::value_noalloc: the forces (float4, first
+// kNoAllocMaxAtoms * 16 bytes) and densities (double, after them) that the two
+// EAM passes write and nobody reads. It is a device symbol, so every GPU
+// holds its own copy, at its own address, from the library's load to the end
+// of the process; host code must use that address (scratch_rows below), never
+// this name.
+static __device__ __align__(256) unsigned char scratch[(size_t)mapp::kNoAllocMaxAtoms * (16 + 8)];
 
 namespace mapp {
 
@@ -1316,6 +1326,45 @@ double MDSystem::value()
     val_f_.reserve(n_);
     val_rho_.reserve(n_);
     force_passes(nullptr, false, val_f_.get(), val_rho_.get());
+    double e = 0.0;
+    MAPP_HIP(hipMemcpyAsync(&e, red_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    prof_.flush();
+    return e;
+}
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[kMaxScratchDevices];
+
+// this GPU's address of the scratch array, looked up once per device with the
+// device current
+unsigned char* scratch_rows()
+{
+    int dev = 0;
+    MAPP_HIP(hipGetDevice(&dev));
+    MAPP_REQUIRE(dev >= 0 && dev < kMaxScratchDevices, ERR_ARG, "device index out of range of the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        MAPP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        MAPP_REQUIRE(p != nullptr, ERR_HIP, "the scratch array has no device address");
+        g_scratch_dev[dev] = static_cast<unsigned char*>(p);
+    }
+    return g_scratch_dev[dev];
+}
+}  // namespace
+
+double MDSystem::value_noalloc()
+{
+    MAPP_REQUIRE(n_ <= kNoAllocMaxAtoms, ERR_CAPACITY,
+                 "value without allocation holds at most " + std::to_string(kNoAllocMaxAtoms) + " atoms, the system has " +
+                     std::to_string(n_));
+    unsigned char* rows = scratch_rows();
+    // the passes overwrite every row they use: nothing needs clearing
+    force_passes(nullptr, false, reinterpret_cast<float4*>(rows),
+                 reinterpret_cast<double*>(rows + (size_t)kNoAllocMaxAtoms * 16));
     double e = 0.0;
     MAPP_HIP(hipMemcpyAsync(&e, red_.get(), sizeof(double), hipMemcpyDeviceToHost, stream_));
     MAPP_HIP(hipStreamSynchronize(stream_));

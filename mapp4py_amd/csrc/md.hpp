@@ -98,6 +98,10 @@ struct MinReport {
 };
 constexpr int kMaxLbfgsM = 64;
 
+// largest system of MDSystem::value_noalloc: its scratch rows take 24 bytes
+// per atom (24 MiB per GPU)
+constexpr int kNoAllocMaxAtoms = 1 << 20;
+
 // AtomsMD + NeighborMD + DynamicMD + ForceFieldEAM state resident in HBM.
 // Atoms are kept in cell order ("slots"); id[slot] is the caller's index.
 class MDSystem {
@@ -119,6 +123,9 @@ public:
     // ForceFieldMD::force_calc -> energy + F_H
     MDReport force_calc();
     double value();
+    // value() with the passes' forces and densities in the library's static
+    // scratch rows (at most kNoAllocMaxAtoms atoms): no device allocation
+    double value_noalloc();
     int64_t half_pairs_count();
     void half_pairs(int32_t* i, int32_t* j, int8_t* img);
     void list_stats(int* maxn, int64_t* nfull);
