@@ -86,24 +86,58 @@ def gpu_run(args):
     return n, wall, kern, nvt, out, nfull
 
 
+def host_threads():
+    """threads for the all-cores CPU baseline: the process's CPU share (the
+    OpenMP setting where one is given, else the affinity mask), at most 16"""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    k = int(env) if env.isdigit() and int(env) > 0 else len(os.sched_getaffinity(0))
+    return max(1, min(16, k))
+
+
+def timed_median(run, reps=5):
+    """one untimed warm-up call, then the median of reps timed calls"""
+    run()
+    t = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run()
+        t.append(time.perf_counter() - t0)
+    return float(np.median(t)), t
+
+
 def cpu_baseline(seconds):
+    """The oracle's NVT (FP64 C restatement of MDNVT::run over the EAM force)
+    on config 1's 32,000 Cu atoms, timed on 1 thread and on all the process's
+    cores (OpenMP over atoms, the same per-pair arithmetic): one warm-up and
+    the median of 5 repetitions each, the step count sized so both modes
+    together take about `seconds`."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from mapp4py_amd import potentials as pot
 
     p, x, H, elem, masses, v = build_workload(20)  # config 1: 32,000 atoms
     ff = O.OEam.from_knots(p)
-    md = O.OMD(ff, H, x, elem, skin=0.5)
-    t0 = time.perf_counter()
-    md.nvt_run(v, masses, pot.KB_EV, 300.0, 1.0 * pot.FS, 2)
-    per = (time.perf_counter() - t0) / 3.0
-    steps = max(3, int(seconds / max(per, 1e-6)))
-    t0 = time.perf_counter()
-    md.nvt_run(v, masses, pot.KB_EV, 300.0, 1.0 * pot.FS, steps)
-    dt = time.perf_counter() - t0
-    return {"value": len(x) * steps / dt, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (FP64 C restatement), fcc Cu 20^3 cells = {len(x)} atoms, {steps} NVT steps "
-                      f"at 1 fs, same table, 1 thread, {dt:.1f} s"}
+    n = len(x)
+    threads = host_threads()
+    out = {}
+    for mode, k in (("one_thread", 1), ("all_cores", threads)):
+        with O.threads(k) as used:
+            md = O.OMD(ff, H, x, elem, skin=0.5)
+            t0 = time.perf_counter()
+            md.nvt_run(v, masses, pot.KB_EV, 300.0, 1.0 * pot.FS, 4)
+            per = (time.perf_counter() - t0) / 4.0
+            # 6 runs (warm-up + 5) of this mode within half the budget
+            steps = max(4, int(seconds / 2 / 6 / max(per, 1e-6)))
+            med, all_t = timed_median(lambda: md.nvt_run(v, masses, pot.KB_EV, 300.0, 1.0 * pot.FS, steps))
+        out[mode] = {"value": n * steps / med, "threads": used, "steps": steps, "median_s": med,
+                     "runs_s": [round(t, 4) for t in all_t]}
+    allc = out["all_cores"]
+    return {"value": allc["value"], "unit": "atom-steps/s", "cores": allc["threads"], "kind": "port",
+            "one_thread": out["one_thread"]["value"], "all_cores": allc, "one_thread_detail": out["one_thread"],
+            "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "timing": "median of 5 after 1 warm-up",
+            "sample": f"oracle (FP64 C restatement), fcc Cu 20^3 cells = {n} atoms, NVT at 1 fs, same table: "
+                      f"{out['one_thread']['steps']} steps on 1 thread and {allc['steps']} steps on "
+                      f"{allc['threads']} OpenMP threads per run"}
 
 
 def feni_run(steps=1000, warmup=20):

@@ -159,8 +159,9 @@ mapp_status mapp_md_profile(mapp_md* md, int enable);
 mapp_status mapp_md_profile_read(mapp_md* md, const char* kernel, double* total_ms, int64_t* launches);
 /* Launch-shape knobs (no counterpart in the reference): neighbors resolved
  * per batch in the density pass (1, 2, 4, 8) and the force pass (1, 2, 4, 8;
- * the LDS force pass uses at most 2). Any value gives the same results to
- * rounding; the tuner picks the fastest. */
+ * the LDS force pass uses at most 4 in the integrator's steps, which sum no
+ * energy or virial, and at most 2 otherwise). Any value gives the same
+ * results to rounding. */
 mapp_status mapp_md_set_batch(mapp_md* md, int rho_batch, int force_batch);
 /* Where the passes of a one-element table read their spline rows (no
  * counterpart in the reference): on != 0 keeps a window of rows in LDS and

@@ -98,21 +98,63 @@ __global__ __launch_bounds__(kBlock) void k_cell_start(int n, int ncells, const 
     cell_start[r] = lo;
 }
 
+// The stencil of every cell in increasing Morton rank of its 27 neighbor cells
+// (a 3x3x3 stencil of distinct cells): order[cell][q] is the stencil index
+// t = (o0+1) + 3 (o1+1) + 9 (o2+1) of the q-th cell to visit, 27 bytes in a
+// 32-byte row. It depends only on the grid, so it is made once per grid.
+__global__ __launch_bounds__(kBlock) void k_cell_order(int ncells, GridArgs g, const int* __restrict__ cell_rank,
+                                                       uint4* __restrict__ order)
+{
+    const int w = blockIdx.x * kBlock + threadIdx.x;
+    if (w >= ncells) return;
+    const int c0 = w % g.nc[0], c1 = (w / g.nc[0]) % g.nc[1], c2 = w / (g.nc[0] * g.nc[1]);
+    // key = rank * 32 + stencil index; 27 rounds of "smallest key above the last"
+    uint32_t key[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+        const int w0 = (c0 + t % 3 - 1 + g.nc[0]) % g.nc[0];
+        const int w1 = (c1 + t / 3 % 3 - 1 + g.nc[1]) % g.nc[1];
+        const int w2 = (c2 + t / 9 - 1 + g.nc[2]) % g.nc[2];
+        key[t] = ((uint32_t)cell_rank[w0 + g.nc[0] * (w1 + g.nc[1] * w2)] << 5) | (uint32_t)t;
+    }
+    uint32_t word[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+    uint32_t last = 0u;
+#pragma unroll
+    for (int round = 0; round < 27; ++round) {
+        uint32_t best = 0xffffffffu;
+#pragma unroll
+        for (int t = 0; t < 27; ++t) {
+            const uint32_t k = key[t];
+            best = ((round == 0 || k > last) && k < best) ? k : best;
+        }
+        last = best;
+        word[round >> 2] |= (best & 31u) << (8 * (round & 3));
+    }
+    order[2 * (size_t)w] = make_uint4(word[0], word[1], word[2], word[3]);
+    order[2 * (size_t)w + 1] = make_uint4(word[4], word[5], word[6], word[7]);
+}
+
 // Full neighbor list with the reference's FP64 predicate: x_i = s_i H and the
 // image of j at (s_j + n) H, rsq = dx0^2 + (dx1^2 + dx2^2) < (r_c + skin)^2.
-// With a 3x3x3 stencil of distinct cells (the usual case) the cells are
-// visited in increasing Morton rank, so every row lists j in increasing slot
-// order: the k-th entries of a wave's 64 spatially adjacent atoms then lie
-// close together, and the passes' neighbor gathers touch few cache lines.
-// Candidates are screened on the 16-byte records: one gather each, exact
-// integer separations. Only a pair within the quantisation band of the
-// cut-off (a few 1e-8 A wide) is decided by the FP64 predicate on s, so the
-// neighbor set is that of the predicate for every pair.
+// Layout: entries k .. k+3 of slot i form one 16-byte block at
+// list4[(k / 4) * ldl + i], so a wave's 64 atoms read or write one block
+// each as 1 KB of contiguous memory. Each lane gathers its entries in
+// registers and stores whole blocks: a block is written once, by one 16-byte
+// store, instead of four scattered 4-byte stores that leave lines partly
+// written in L2. With the per-cell stencil order (order != nullptr) the
+// cells are visited in increasing Morton rank, so every row lists j in
+// increasing slot order: the k-th entries of a wave's 64 spatially adjacent
+// atoms then lie close together, and the passes' neighbor gathers touch few
+// cache lines. Candidates are screened on the 16-byte records: one gather
+// each, exact integer separations. Only a pair within the quantisation band
+// of the cut-off (a few 1e-8 A wide) is decided by the FP64 predicate on s,
+// so the neighbor set is that of the predicate for every pair.
 __global__ __launch_bounds__(kBlock) void k_build_list(int n, const double4* __restrict__ s4,
                                                        const uint4* __restrict__ rec, Rec16Frame R, BoxDev bd,
                                                        GridArgs g, const int* __restrict__ cell_rank,
-                                                       const int* __restrict__ cell_start, int maxn, int ldl,
-                                                       uint32_t* __restrict__ list, int* __restrict__ nn,
+                                                       const int* __restrict__ cell_start,
+                                                       const uint4* __restrict__ order, int maxb, int ldl,
+                                                       uint4* __restrict__ list4, int* __restrict__ nn,
                                                        int* __restrict__ flags)
 {
     const int i = xcd_block() * kBlock + threadIdx.x;
@@ -124,6 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_build_list(int n, const double4* __r
     s_to_x(bd.h, si.x, si.y, si.z, xi0, xi1, xi2);
     const int c0 = cell_coord(si.x, g.nc[0]), c1 = cell_coord(si.y, g.nc[1]), c2 = cell_coord(si.z, g.nc[2]);
     int cnt = 0;
+    uint32_t p0 = 0u, p1 = 0u, p2 = 0u, p3 = 0u;  // the block being filled
     // candidates of the stencil cell at offset (o0, o1, o2)
     auto scan = [&](int o0, int o1, int o2) {
         const int u2 = c2 + o2, n2 = floordiv(u2, g.nc[2]), w2 = u2 - n2 * g.nc[2];
@@ -151,42 +194,39 @@ __global__ __launch_bounds__(kBlock) void k_build_list(int n, const double4* __r
                 in = rsq_exact(xi0, xi1, xi2, xj0, xj1, xj2) < g.cut_sk_sq[tt];
             }
             if (in) {
-                if (cnt < maxn) list[(size_t)cnt * ldl + i] = (uint32_t)j | code;
+                const uint32_t e = (uint32_t)j | code;
+                const int q = cnt & 3;
+                p0 = q == 0 ? e : p0;
+                p1 = q == 1 ? e : p1;
+                p2 = q == 2 ? e : p2;
+                p3 = q == 3 ? e : p3;
                 ++cnt;
+                if (q == 3 && (cnt >> 2) <= maxb) list4[(size_t)((cnt >> 2) - 1) * ldl + i] = make_uint4(p0, p1, p2, p3);
             }
         }
     };
-    const bool ordered = g.R[0] == 1 && g.R[1] == 1 && g.R[2] == 1 && g.nc[0] >= 3 && g.nc[1] >= 3 &&
-                         g.nc[2] >= 3 && (long long)g.nc[0] * g.nc[1] * g.nc[2] < (1ll << 27);
-    if (ordered) {
-        // key = rank * 32 + stencil index; 27 rounds of "smallest key above the last"
-        uint32_t key[27];
-#pragma unroll
-        for (int t = 0; t < 27; ++t) {
-            const int w0 = (c0 + t % 3 - 1 + g.nc[0]) % g.nc[0];
-            const int w1 = (c1 + t / 3 % 3 - 1 + g.nc[1]) % g.nc[1];
-            const int w2 = (c2 + t / 9 - 1 + g.nc[2]) % g.nc[2];
-            key[t] = ((uint32_t)cell_rank[w0 + g.nc[0] * (w1 + g.nc[1] * w2)] << 5) | (uint32_t)t;
-        }
-        uint32_t last = 0u;
-        for (int round = 0; round < 27; ++round) {
-            uint32_t best = 0xffffffffu;
-#pragma unroll
-            for (int t = 0; t < 27; ++t) {
-                const uint32_t k = key[t];
-                best = ((round == 0 || k > last) && k < best) ? k : best;
-            }
-            last = best;
-            const int t = (int)(best & 31u);
+    if (order) {
+        const size_t w = (size_t)c0 + (size_t)g.nc[0] * (c1 + (size_t)g.nc[1] * c2);
+        const uint4 oa = order[2 * w], ob = order[2 * w + 1];
+        uint32_t cur = oa.x;
+        for (int q = 0; q < 27; ++q) {
+            const int t = (int)(cur & 31u);
             scan(t % 3 - 1, t / 3 % 3 - 1, t / 9 - 1);
+            cur >>= 8;
+            if ((q & 3) == 3) {
+                const int nq = (q >> 2) + 1;
+                cur = nq == 1 ? oa.y : nq == 2 ? oa.z : nq == 3 ? oa.w : nq == 4 ? ob.x : nq == 5 ? ob.y : ob.z;
+            }
         }
     } else {
         for (int o2 = -g.R[2]; o2 <= g.R[2]; ++o2)
             for (int o1 = -g.R[1]; o1 <= g.R[1]; ++o1)
                 for (int o0 = -g.R[0]; o0 <= g.R[0]; ++o0) scan(o0, o1, o2);
     }
+    // the last, partly filled block (its entries past cnt are never read)
+    if ((cnt & 3) != 0 && (cnt >> 2) < maxb) list4[(size_t)(cnt >> 2) * ldl + i] = make_uint4(p0, p1, p2, p3);
     nn[i] = cnt;
-    if (cnt > maxn) atomicMax(&flags[0], cnt);
+    if (cnt > 4 * maxb) atomicMax(&flags[0], cnt);
 }
 
 // ---------------- EAM ----------------
@@ -243,13 +283,38 @@ __device__ inline void embed(const EamArgs& P, int t, double rho, double& F, dou
 // arithmetic is kept to what the result needs (FP32 splines and reciprocal
 // root, FP64 where a sum or the spline abscissa needs it).
 
-// list entry k of slot i; the load is unconditional (a row past maxn is never
-// read; the list always has maxn >= 16 rows) and entries past the row end
-// become the atom itself, zero image
-__device__ inline uint32_t list_entry(const uint32_t* __restrict__ list, int ldl, int maxn, int k, int i, int cnt)
+// entries k0 .. k0+G-1 of slot i, G = 1, 2 or a multiple of 4 and k0 a
+// multiple of G: the G/4 16-byte blocks, or the part of one block that holds
+// them. The loads are unconditional (a block past the capacity maxb is never
+// read: the last block is read again instead) and entries past the row end
+// become the atom itself, zero image.
+template <int G>
+__device__ inline void list_entries(const uint4* __restrict__ list4, int ldl, int maxb, int k0, int i, int cnt,
+                                    uint32_t (&e)[G])
 {
-    const uint32_t raw = list[(size_t)(k < maxn ? k : maxn - 1) * ldl + i];
-    return k < cnt ? raw : ((uint32_t)i | ((uint32_t)kZeroImage << kIndexBits));
+    const uint32_t self = (uint32_t)i | ((uint32_t)kZeroImage << kIndexBits);
+    if constexpr (G >= 4) {
+#pragma unroll
+        for (int q = 0; q < G / 4; ++q) {
+            const int kb = (k0 >> 2) + q;
+            const uint4 v = list4[(size_t)(kb < maxb ? kb : maxb - 1) * ldl + i];
+            const int k = k0 + 4 * q;
+            e[4 * q] = k < cnt ? v.x : self;
+            e[4 * q + 1] = k + 1 < cnt ? v.y : self;
+            e[4 * q + 2] = k + 2 < cnt ? v.z : self;
+            e[4 * q + 3] = k + 3 < cnt ? v.w : self;
+        }
+    } else {
+        const int kb = (k0 >> 2) < maxb ? (k0 >> 2) : maxb - 1;
+        const uint32_t* blk = (const uint32_t*)(list4 + (size_t)kb * ldl + i) + (k0 & 3);
+        if constexpr (G == 2) {
+            const uint2 v = *(const uint2*)blk;
+            e[0] = k0 < cnt ? v.x : self;
+            e[1] = k0 + 1 < cnt ? v.y : self;
+        } else {
+            e[0] = k0 < cnt ? blk[0] : self;
+        }
+    }
 }
 
 // whether any lane of the wave holds a shifted periodic image among B entries
@@ -319,7 +384,7 @@ __global__ __launch_bounds__(kBlock) void k_make_records(int n, const double4* _
 // values. Tab supplies the spline rows (global table or an LDS window).
 template <int B, bool ONE, class Tab>
 __device__ inline double rho_atom(int i, const uint4& ri, int cnt, const uint4* __restrict__ rec,
-                                  const uint32_t* __restrict__ list, int ldl, int maxn, const EamArgs& P,
+                                  const uint4* __restrict__ list4, int ldl, int maxb, const EamArgs& P,
                                   const Rec16Frame& R, const Tab& T)
 {
     const int ti = ONE ? 0 : (int)ri.w;
@@ -330,8 +395,7 @@ __device__ inline double rho_atom(int i, const uint4& ri, int cnt, const uint4* 
         int row[B];
         float p[B];
         bool in[B];
-#pragma unroll
-        for (int b = 0; b < B; ++b) e[b] = list_entry(list, ldl, maxn, k0 + b, i, cnt);
+        list_entries<B>(list4, ldl, maxb, k0, i, cnt, e);
 #pragma unroll
         for (int b = 0; b < B; ++b) rj[b] = rec[e[b] & kIndexMask];
         const bool any_img = wave_has_image(e);
@@ -390,22 +454,24 @@ struct RhoRowsLds {
     }
 };
 
-// pass 2 per atom: pair + embedding forces on the owner (no scatter), half
-// of each full-list pair's energy (ep) and virial (w, in grid units: u dx dy).
-// One element: neighbor records are the 16-byte pass-2 records (coordinates,
-// F'); otherwise the density records (coordinates, element) and F' from dF.
-template <int B, bool ONE, class Tab>
+// pass 2 per atom: pair + embedding forces on the owner (no scatter), and with
+// EV the sums of the full-list pairs' energy (ev[0], phi) and virial
+// (ev[1..6], in grid units: u dx dy) added to ev; without EV (the
+// integrator's steps that read neither) the FP64 energy and virial arithmetic
+// is left out. One element: neighbor records are the 16-byte pass-2 records
+// (coordinates, F'); otherwise the density records (coordinates, element) and
+// F' from dF.
+template <int B, bool ONE, bool EV, class Tab>
 __device__ inline void force_atom(int i, const uint4& ri, int ti, float dFi, int cnt, const uint4* __restrict__ rec,
                                   const uint4* __restrict__ xr, const float* __restrict__ dF,
-                                  const uint32_t* __restrict__ list, int ldl, int maxn, const EamArgs& P,
-                                  const Rec16Frame& R, const Tab& T, float& fx, float& fy, float& fz, double& ep,
-                                  double (&w)[6])
+                                  const uint4* __restrict__ list4, int ldl, int maxb, const EamArgs& P,
+                                  const Rec16Frame& R, const Tab& T, float& fx, float& fy, float& fz,
+                                  double (&ev)[7])
 {
     // the list entries of the next batch are loaded one batch ahead (this
     // pass's batches are short, so one memory round trip less per batch pays)
     uint32_t en[B];
-#pragma unroll
-    for (int b = 0; b < B; ++b) en[b] = list_entry(list, ldl, maxn, b, i, cnt);
+    list_entries<B>(list4, ldl, maxb, 0, i, cnt, en);
     for (int k0 = 0; k0 < cnt; k0 += B) {
         uint32_t e[B];
         uint4 rj[B];
@@ -425,8 +491,7 @@ __device__ inline void force_atom(int i, const uint4& ri, int ti, float dFi, int
                 dFj[b] = dF[j];
             }
         }
-#pragma unroll
-        for (int b = 0; b < B; ++b) en[b] = list_entry(list, ldl, maxn, k0 + B + b, i, cnt);
+        list_entries<B>(list4, ldl, maxb, k0 + B, i, cnt, en);
         const bool any_img = wave_has_image(e);
 #pragma unroll
         for (int b = 0; b < B; ++b) {
@@ -456,18 +521,20 @@ __device__ inline void force_atom(int i, const uint4& ri, int ti, float dFi, int
             // u = f_pair * q, so that the force is u times the grid separation
             float u = -(dphi + dFi * d_in + dFj[b] * d_out) * g[b].rinv;
             u = in[b] ? u : 0.f;
-            ep += (double)(in[b] ? phi : 0.f);
             fx += u * (float)g[b].dx;
             fy += u * (float)g[b].dy;
             fz += u * (float)g[b].dz;
-            const double ud = (double)u;
-            const double ux = ud * g[b].dx, uy = ud * g[b].dy, uz = ud * g[b].dz;
-            w[0] += ux * g[b].dx;
-            w[1] += ux * g[b].dy;
-            w[2] += ux * g[b].dz;
-            w[3] += uy * g[b].dy;
-            w[4] += uy * g[b].dz;
-            w[5] += uz * g[b].dz;
+            if (EV) {
+                ev[0] += (double)(in[b] ? phi : 0.f);
+                const double ud = (double)u;
+                const double ux = ud * g[b].dx, uy = ud * g[b].dy, uz = ud * g[b].dz;
+                ev[1] += ux * g[b].dx;
+                ev[2] += ux * g[b].dy;
+                ev[3] += ux * g[b].dz;
+                ev[4] += uy * g[b].dy;
+                ev[5] += uy * g[b].dz;
+                ev[6] += uz * g[b].dz;
+            }
         }
     }
 }
@@ -532,9 +599,10 @@ struct ForceRowsLds {
     }
 };
 
-// second velocity-Verlet half kick of the owner of f_i, and m v (x) v
+// second velocity-Verlet half kick of the owner of f_i, m v (x) v, and the sum
+// of f / m that the next step's centre-of-mass shift needs
 __device__ inline void kick_atom(int i, int ti, float fx, float fy, float fz, const KickArgs& K,
-                                 double4* __restrict__ v, double (&mvv)[6])
+                                 double4* __restrict__ v, double (&mvv)[6], double (&fm)[3])
 {
     double4 vi = v[i];
     const double m = K.mass[ti];
@@ -548,6 +616,9 @@ __device__ inline void kick_atom(int i, int ti, float fx, float fy, float fz, co
     mvv[3] += vi.y * vi.y * m;
     mvv[4] += vi.y * vi.z * m;
     mvv[5] += vi.z * vi.z * m;
+    fm[0] += (double)fx / m;
+    fm[1] += (double)fy / m;
+    fm[2] += (double)fz / m;
 }
 
 // pass 1 (one thread per atom, 256-thread workgroups): rho_i, F(rho_i) and
@@ -556,7 +627,7 @@ __device__ inline void kick_atom(int i, int ti, float fx, float fy, float fz, co
 // into dF.
 template <int B, bool ONE>
 __global__ __launch_bounds__(kBlock) void k_eam_rho(int n, const uint4* __restrict__ rec, const int* __restrict__ nn,
-                                                    const uint32_t* __restrict__ list, int ldl, int maxn, EamArgs P,
+                                                    const uint4* __restrict__ list4, int ldl, int maxb, EamArgs P,
                                                     Rec16Frame R, double* __restrict__ rho_out,
                                                     uint4* __restrict__ xr, float* __restrict__ dF_out,
                                                     double* __restrict__ part)
@@ -567,7 +638,7 @@ __global__ __launch_bounds__(kBlock) void k_eam_rho(int n, const uint4* __restri
     if (i < n) {
         const uint4 ri = rec[i];
         const int ti = ONE ? 0 : (int)ri.w;
-        const double rho = rho_atom<B, ONE>(i, ri, nn[i], rec, list, ldl, maxn, P, R, RhoRowsGlobal{P.tabA});
+        const double rho = rho_atom<B, ONE>(i, ri, nn[i], rec, list4, ldl, maxb, P, R, RhoRowsGlobal{P.tabA});
         double F, dF;
         embed(P, ti, rho, F, dF);
         rho_out[i] = rho;
@@ -581,38 +652,43 @@ __global__ __launch_bounds__(kBlock) void k_eam_rho(int n, const uint4* __restri
 }
 
 // pass 2 (one thread per atom, 256-thread workgroups) with the fused second
-// half kick when K.on
-template <int B, bool ONE>
+// half kick when K.on; energy and virial partials only with EV
+template <int B, bool ONE, bool EV>
 __global__ __launch_bounds__(kBlock) void k_eam_force(int n, const uint4* __restrict__ rec,
                                                       const uint4* __restrict__ xr, const float* __restrict__ dF,
                                                       Rec16Frame R, const int* __restrict__ nn,
-                                                      const uint32_t* __restrict__ list, int ldl, int maxn, EamArgs P,
+                                                      const uint4* __restrict__ list4, int ldl, int maxb, EamArgs P,
                                                       float4* __restrict__ f_out, double* __restrict__ part,
                                                       KickArgs K, double4* __restrict__ v,
-                                                      double* __restrict__ part_mvv)
+                                                      double* __restrict__ part_mvv, double* __restrict__ part_cv)
 {
     const int blk = xcd_block();
     const int i = blk * kBlock + threadIdx.x;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     double mvv[6] = {0, 0, 0, 0, 0, 0};
+    double fm[3] = {0, 0, 0};
     if (i < n) {
         const uint4 ri = ONE ? xr[i] : rec[i];
         const int ti = ONE ? 0 : (int)ri.w;
         const float dFi = ONE ? __uint_as_float(ri.w) : dF[i];
         float fx = 0.f, fy = 0.f, fz = 0.f;
-        double ep = 0.0, w[6] = {0, 0, 0, 0, 0, 0};
-        force_atom<B, ONE>(i, ri, ti, dFi, nn[i], rec, xr, dF, list, ldl, maxn, P, R,
-                           ForceRowsGlobal<ONE>{ONE ? P.tabC : P.tabB}, fx, fy, fz, ep, w);
+        force_atom<B, ONE, EV>(i, ri, ti, dFi, nn[i], rec, xr, dF, list4, ldl, maxb, P, R,
+                               ForceRowsGlobal<ONE>{ONE ? P.tabC : P.tabB}, fx, fy, fz, acc);
         f_out[i] = make_float4(fx, fy, fz, 0.f);
-        acc[0] = 0.5 * ep;
-        const double hq = 0.5 * R.q;  // f_pair dx_A dy_A / 2 = (q/2) u dx dy
-        for (int k = 0; k < 6; ++k) acc[1 + k] = hq * w[k];
-        if (K.on) kick_atom(i, ti, fx, fy, fz, K, v, mvv);
+        if (K.on) kick_atom(i, ti, fx, fy, fz, K, v, mvv, fm);
     }
-    block_sum_store<7>(acc, part + (size_t)blk * kPartW);
+    if (EV) {
+        // half of each full-list pair: energy phi / 2, virial (q/2) u dx dy
+        // (powers of two: scaling the sums is exact)
+        acc[0] *= 0.5;
+        for (int k = 1; k < 7; ++k) acc[k] *= 0.5 * R.q;
+        block_sum_store<7>(acc, part + (size_t)blk * kPartW);
+    }
     if (K.on) {
-        __syncthreads();
+        if (EV) __syncthreads();
         block_sum_store<6>(mvv, part_mvv + (size_t)blk * kPartW);
+        __syncthreads();
+        block_sum_store<3>(fm, part_cv + (size_t)blk * kPartW);
     }
 }
 
@@ -643,7 +719,7 @@ __device__ inline ChunkWalk chunk_walk(int n)
 template <int B>
 __global__ __launch_bounds__(kBigBlock) void k_eam_rho_lds(int n, const uint4* __restrict__ rec,
                                                            const int* __restrict__ nn,
-                                                           const uint32_t* __restrict__ list, int ldl, int maxn,
+                                                           const uint4* __restrict__ list4, int ldl, int maxb,
                                                            EamArgs P, Rec16Frame R, int m_lo, int nwin,
                                                            double* __restrict__ rho_out, uint4* __restrict__ xr,
                                                            double* __restrict__ part)
@@ -657,7 +733,7 @@ __global__ __launch_bounds__(kBigBlock) void k_eam_rho_lds(int n, const uint4* _
         const int i = w.c * kBigBlock + threadIdx.x;
         if (i >= n) continue;
         const uint4 ri = rec[i];
-        const double rho = rho_atom<B, true>(i, ri, nn[i], rec, list, ldl, maxn, P, R, T);
+        const double rho = rho_atom<B, true>(i, ri, nn[i], rec, list4, ldl, maxb, P, R, T);
         double F, dF;
         embed(P, 0, rho, F, dF);
         rho_out[i] = rho;
@@ -667,14 +743,15 @@ __global__ __launch_bounds__(kBigBlock) void k_eam_rho_lds(int n, const uint4* _
     block_sum_store<1, kBigBlock>(acc, part + (size_t)blockIdx.x * kPartW);
 }
 
-template <int B>
+template <int B, bool EV>
 __global__ __launch_bounds__(kBigBlock) void k_eam_force_lds(int n, const uint4* __restrict__ xr, Rec16Frame R,
                                                              const int* __restrict__ nn,
-                                                             const uint32_t* __restrict__ list, int ldl, int maxn,
+                                                             const uint4* __restrict__ list4, int ldl, int maxb,
                                                              EamArgs P, int m_lo, int nwin,
                                                              float4* __restrict__ f_out, double* __restrict__ part,
                                                              KickArgs K, double4* __restrict__ v,
-                                                             double* __restrict__ part_mvv)
+                                                             double* __restrict__ part_mvv,
+                                                             double* __restrict__ part_cv)
 {
     extern __shared__ float4 lds_rows[];
     float2* w2 = (float2*)(lds_rows + nwin);
@@ -689,66 +766,43 @@ __global__ __launch_bounds__(kBigBlock) void k_eam_force_lds(int n, const uint4*
     const ForceRowsLds T{P.tabC, lds_rows, w2, w1, m_lo};
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     double mvv[6] = {0, 0, 0, 0, 0, 0};
-    const double hq = 0.5 * R.q;  // f_pair dx_A dy_A / 2 = (q/2) u dx dy
+    double fm[3] = {0, 0, 0};
     for (ChunkWalk cw = chunk_walk(n); cw.c < cw.end; cw.c += cw.step) {
         const int i = cw.c * kBigBlock + threadIdx.x;
         if (i >= n) continue;
         const uint4 ri = xr[i];
         float fx = 0.f, fy = 0.f, fz = 0.f;
-        double ep = 0.0, w[6] = {0, 0, 0, 0, 0, 0};
-        force_atom<B, true>(i, ri, 0, __uint_as_float(ri.w), nn[i], nullptr, xr, nullptr, list, ldl, maxn, P, R, T,
-                            fx, fy, fz, ep, w);
+        force_atom<B, true, EV>(i, ri, 0, __uint_as_float(ri.w), nn[i], nullptr, xr, nullptr, list4, ldl, maxb, P,
+                                R, T, fx, fy, fz, acc);
         f_out[i] = make_float4(fx, fy, fz, 0.f);
-        acc[0] += 0.5 * ep;
-        for (int k = 0; k < 6; ++k) acc[1 + k] += hq * w[k];
-        if (K.on) kick_atom(i, 0, fx, fy, fz, K, v, mvv);
+        if (K.on) kick_atom(i, 0, fx, fy, fz, K, v, mvv, fm);
     }
-    block_sum_store<7, kBigBlock>(acc, part + (size_t)blockIdx.x * kPartW);
+    if (EV) {
+        // half of each full-list pair: energy phi / 2, virial (q/2) u dx dy
+        // (powers of two: scaling the sums is exact)
+        acc[0] *= 0.5;
+        for (int k = 1; k < 7; ++k) acc[k] *= 0.5 * R.q;
+        block_sum_store<7, kBigBlock>(acc, part + (size_t)blockIdx.x * kPartW);
+    }
     if (K.on) {
-        __syncthreads();
+        if (EV) __syncthreads();
         block_sum_store<6, kBigBlock>(mvv, part_mvv + (size_t)blockIdx.x * kPartW);
+        __syncthreads();
+        block_sum_store<3, kBigBlock>(fm, part_cv + (size_t)blockIdx.x * kPartW);
     }
 }
 
 // ---------------- NVT ----------------
 
 // nhc state layout (doubles)
-enum { NH_FAC = 0, NH_T = 1, NH_ETA = 2, NH_MVV = 5, NH_SIZE = 16 };
+// NH_SV: sum of v after the last kick, NH_SF: sum of f / m of the last forces
+enum { NH_FAC = 0, NH_T = 1, NH_ETA = 2, NH_MVV = 5, NH_COM = 11, NH_SV = 14, NH_SF = 17, NH_SIZE = 20 };
 
 struct NvtArgs {
     double dt, dt2, kB, T, ddt, ddt2, freq_sq;
     int ndof, natms;
     double mass[kMaxElems];
 };
-
-// v <- v*fac + f dt/2 / m ; x <- x + v dt ; partial sums of v dt
-__global__ __launch_bounds__(kBlock) void k_kick_drift(int n, double4* __restrict__ x, double4* __restrict__ v,
-                                                       const float4* __restrict__ f, NvtArgs A,
-                                                       const double* __restrict__ nh, double* __restrict__ part)
-{
-    const int blk = xcd_block();
-    const int i = blk * kBlock + threadIdx.x;
-    double acc[3] = {0.0, 0.0, 0.0};
-    if (i < n) {
-        const double fac = nh[NH_FAC];
-        double4 xi = x[i];
-        double4 vi = v[i];
-        const float4 fi = f[i];
-        const double m = A.mass[(int)xi.w];
-        vi.x = vi.x * fac + (double)fi.x * A.dt2 / m;
-        vi.y = vi.y * fac + (double)fi.y * A.dt2 / m;
-        vi.z = vi.z * fac + (double)fi.z * A.dt2 / m;
-        acc[0] = vi.x * A.dt;
-        acc[1] = vi.y * A.dt;
-        acc[2] = vi.z * A.dt;
-        xi.x += vi.x * A.dt;
-        xi.y += vi.y * A.dt;
-        xi.z += vi.z * A.dt;
-        x[i] = xi;
-        v[i] = vi;
-    }
-    block_sum_store<3>(acc, part + (size_t)blk * kPartW);
-}
 
 // remove the centre-of-mass drift of the step, then the skin check of
 // DynamicMD::decide (|x - x0|^2 > skin^2 / 4 anywhere -> rebuild); with rec
@@ -772,19 +826,65 @@ __global__ __launch_bounds__(kBlock) void k_shift_decide(int n, double4* __restr
     if (rsq_exact(a.x, a.y, a.z, xi.x, xi.y, xi.z) > skin_sq4) atomicOr(&flags[1], 1);
 }
 
-// second half kick and the kinetic tensor sum m v (x) v
+// The first half of a velocity-Verlet step in one pass (update_x_d__x,
+// src/md_nvt.cpp:70-102, then DynamicMD::decide): v <- v fac + f dt/2 / m,
+// x <- x + v dt, then x <- x - c, where c = (sum of v dt) / natms is the
+// centre-of-mass displacement of the step, which k_nhc computes before the
+// pass from the sums of v and f / m the previous force pass left; then the
+// skin check against x0 and the 16-byte position record. Partial sums of the
+// new v go to part_sv.
+__global__ __launch_bounds__(kBlock) void k_step(int n, double4* __restrict__ x, double4* __restrict__ v,
+                                                 const float4* __restrict__ f, NvtArgs A,
+                                                 const double* __restrict__ nh, const double4* __restrict__ x0,
+                                                 double skin_sq4, int* __restrict__ flags, Rec16Frame R,
+                                                 uint4* __restrict__ rec, double* __restrict__ part_sv)
+{
+    const int blk = xcd_block();
+    const int i = blk * kBlock + threadIdx.x;
+    double sv[3] = {0.0, 0.0, 0.0};
+    if (i < n) {
+        const double fac = nh[NH_FAC];
+        double4 xi = x[i];
+        double4 vi = v[i];
+        const float4 fi = f[i];
+        const double m = A.mass[(int)xi.w];
+        vi.x = vi.x * fac + (double)fi.x * A.dt2 / m;
+        vi.y = vi.y * fac + (double)fi.y * A.dt2 / m;
+        vi.z = vi.z * fac + (double)fi.z * A.dt2 / m;
+        xi.x += vi.x * A.dt;
+        xi.y += vi.y * A.dt;
+        xi.z += vi.z * A.dt;
+        xi.x -= nh[NH_COM];
+        xi.y -= nh[NH_COM + 1];
+        xi.z -= nh[NH_COM + 2];
+        x[i] = xi;
+        v[i] = vi;
+        rec[i] = make_uint4((unsigned)__double2ll_rn(xi.x * R.inv_q), (unsigned)__double2ll_rn(xi.y * R.inv_q),
+                            (unsigned)__double2ll_rn(xi.z * R.inv_q), (unsigned)(int)xi.w);
+        const double4 a = x0[i];
+        if (rsq_exact(a.x, a.y, a.z, xi.x, xi.y, xi.z) > skin_sq4) atomicOr(&flags[1], 1);
+        sv[0] = vi.x;
+        sv[1] = vi.y;
+        sv[2] = vi.z;
+    }
+    block_sum_store<3>(sv, part_sv + (size_t)blk * kPartW);
+}
+
+// second half kick (when kick), the kinetic tensor sum m v (x) v, and the sums
+// of v and f / m for the next step's centre-of-mass shift
 __global__ __launch_bounds__(kBlock) void k_kick2(int n, const double4* __restrict__ x, double4* __restrict__ v,
                                                   const float4* __restrict__ f, NvtArgs A, int kick,
-                                                  double* __restrict__ part)
+                                                  double* __restrict__ part, double* __restrict__ part_cv)
 {
     const int blk = xcd_block();
     const int i = blk * kBlock + threadIdx.x;
     double acc[6] = {0, 0, 0, 0, 0, 0};
+    double cv[6] = {0, 0, 0, 0, 0, 0};
     if (i < n) {
         double4 vi = v[i];
         const double m = A.mass[(int)x[i].w];
+        const float4 fi = f[i];
         if (kick) {
-            const float4 fi = f[i];
             vi.x += (double)fi.x * A.dt2 / m;
             vi.y += (double)fi.y * A.dt2 / m;
             vi.z += (double)fi.z * A.dt2 / m;
@@ -796,8 +896,16 @@ __global__ __launch_bounds__(kBlock) void k_kick2(int n, const double4* __restri
         acc[3] = vi.y * vi.y * m;
         acc[4] = vi.y * vi.z * m;
         acc[5] = vi.z * vi.z * m;
+        cv[0] = vi.x;
+        cv[1] = vi.y;
+        cv[2] = vi.z;
+        cv[3] = (double)fi.x / m;
+        cv[4] = (double)fi.y / m;
+        cv[5] = (double)fi.z / m;
     }
     block_sum_store<6>(acc, part + (size_t)blk * kPartW);
+    __syncthreads();
+    block_sum_store<6>(cv, part_cv + (size_t)blk * kPartW);
 }
 
 __device__ inline double nhc_phi(double x)
@@ -829,11 +937,26 @@ __device__ inline double nhc_half(double* e, const NvtArgs& A, double T_r)
     return rescale;
 }
 
-// mode 0: initial T from mvv, fac = 1; mode 1: thermostat at the start of a
-// step (fac *= s); mode 2: T from the new mvv, thermostat after the step (fac = s)
-__global__ void k_nhc(double* __restrict__ nh, const double* __restrict__ mvv, NvtArgs A, int mode)
+// mode 0: initial T from mvv, fac = 1, the sums of v and f / m from cv[0..5];
+// mode 1: thermostat at the start of a step (fac *= s), then the step's
+// centre-of-mass displacement c = dt (fac sum v + dt/2 sum f / m) / natms;
+// mode 2: T from the new mvv, the sums of v after the kick (cv[0..2], the
+// step's v, plus dt/2 cv[3..5], the new f / m), thermostat after the step
+// (fac = s)
+__global__ void k_nhc(double* __restrict__ nh, const double* __restrict__ mvv, const double* __restrict__ cv,
+                      NvtArgs A, int mode)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (mode == 0)
+        for (int k = 0; k < 3; ++k) {
+            nh[NH_SV + k] = cv[k];
+            nh[NH_SF + k] = cv[3 + k];
+        }
+    if (mode == 2)
+        for (int k = 0; k < 3; ++k) {
+            nh[NH_SV + k] = cv[k] + A.dt2 * cv[3 + k];
+            nh[NH_SF + k] = cv[3 + k];
+        }
     if (mode == 0 || mode == 2) {
         for (int k = 0; k < 6; ++k) nh[NH_MVV + k] = mvv[k];
         nh[NH_T] = (nh[NH_MVV] + (nh[NH_MVV + 3] + nh[NH_MVV + 5])) / (A.ndof * A.kB);
@@ -847,6 +970,9 @@ __global__ void k_nhc(double* __restrict__ nh, const double* __restrict__ mvv, N
     const double s2 = s * s;
     for (int k = 0; k < 6; ++k) nh[NH_MVV + k] *= s2;
     nh[NH_T] *= s2;
+    if (mode == 1)
+        for (int k = 0; k < 3; ++k)
+            nh[NH_COM + k] = (nh[NH_FAC] * nh[NH_SV + k] + A.dt2 * nh[NH_SF + k]) * A.dt / A.natms;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scale_v(int n, double4* __restrict__ v, const double* __restrict__ nh)
@@ -993,7 +1119,7 @@ MDSystem::MDSystem(std::shared_ptr<EamForceField> ff, int natms, const double* H
     keys_.reserve(n_); keys_alt_.reserve(n_); vals_.reserve(n_); vals_alt_.reserve(n_); s_tmp_.reserve(n_);
     nn_.reserve(n_);
     flags_.reserve(4);
-    part_.reserve((size_t)3 * grid_for(n_) * kPartW + 64);  // rho, force, integrator rows
+    part_.reserve((size_t)5 * grid_for(n_) * kPartW + 64);  // rho, force, integrator, two centre-of-mass rows
     red_.reserve(64);
     red2_.reserve(256 * kPartW);
     std::vector<double4> xh(n_);
@@ -1051,10 +1177,12 @@ void MDSystem::plan_lds()
     rho_lds_bytes_ = (size_t)rho_nwin_ * 16;
     f_lds_bytes_ = (size_t)f_nwin_ * 28;
     lds_grid_ = std::max(1, ncu / kXcd) * kXcd;
-    const void* fns[] = {(const void*)k_eam_rho_lds<1>, (const void*)k_eam_rho_lds<2>, (const void*)k_eam_rho_lds<4>,
-                         (const void*)k_eam_rho_lds<8>, (const void*)k_eam_force_lds<1>,
-                         (const void*)k_eam_force_lds<2>};
-    for (int k = 0; k < 6; ++k) {
+    const void* fns[] = {(const void*)k_eam_rho_lds<1>,         (const void*)k_eam_rho_lds<2>,
+                         (const void*)k_eam_rho_lds<4>,         (const void*)k_eam_rho_lds<8>,
+                         (const void*)k_eam_force_lds<1, true>, (const void*)k_eam_force_lds<2, true>,
+                         (const void*)k_eam_force_lds<1, false>, (const void*)k_eam_force_lds<2, false>,
+                         (const void*)k_eam_force_lds<4, false>};
+    for (int k = 0; k < 9; ++k) {
         const size_t bytes = k < 4 ? rho_lds_bytes_ : f_lds_bytes_;
         if (hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
             (void)hipGetLastError();
@@ -1134,10 +1262,23 @@ void MDSystem::setup_grid()
     cell_rank_.reserve(ncells);
     cell_start_.reserve(ncells + 1);
     MAPP_HIP(hipMemcpy(cell_rank_.get(), rank.data(), sizeof(int) * ncells, hipMemcpyHostToDevice));
-    // neighbors expected in the search sphere at this density, with margin
+    // with a 3x3x3 stencil of distinct cells, each cell's stencil in Morton-rank order
+    ordered_ = grid_.R[0] == 1 && grid_.R[1] == 1 && grid_.R[2] == 1 && grid_.nc[0] >= 3 && grid_.nc[1] >= 3 &&
+               grid_.nc[2] >= 3 && (long long)ncells < (1ll << 27);
+    if (ordered_) {
+        GridArgs g{};
+        for (int d = 0; d < 3; ++d) g.nc[d] = grid_.nc[d];
+        cell_order_.reserve((size_t)2 * ncells);
+        hipLaunchKernelGGL(k_cell_order, dim3((ncells + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, ncells, g,
+                           cell_rank_.get(), cell_order_.get());
+        MAPP_HIP(hipGetLastError());
+    }
+    // neighbors expected in the search sphere at this density, with margin;
+    // the capacity is a whole number of 4-entry blocks
     const double dens = n_ / box_.vol;
     const double sphere = 4.0 / 3.0 * M_PI * max_cut_ * max_cut_ * max_cut_;
     maxn_ = std::max(maxn_, (int)(1.25 * dens * sphere) + 16);
+    maxn_ = (maxn_ + 3) & ~3;
 }
 
 void MDSystem::rebuild()
@@ -1216,18 +1357,19 @@ void MDSystem::build_list()
         MAPP_HIP(hipMemsetAsync(flags_.get(), 0, sizeof(int), stream_));
         prof_.begin(K_BUILD_LIST, stream_);
         hipLaunchKernelGGL(k_build_list, dim3(nb), dim3(kBlock), 0, stream_, n_, s_.get(), rec_.get(), r16_, bd, g,
-                           cell_rank_.get(), cell_start_.get(), maxn_, ldl_, list_.get(), nn_.get(), flags_.get());
+                           cell_rank_.get(), cell_start_.get(), ordered_ ? cell_order_.get() : nullptr, maxn_ / 4,
+                           ldl_, list4(), nn_.get(), flags_.get());
         prof_.end(stream_);
         int over = 0;
         MAPP_HIP(hipMemcpyAsync(&over, flags_.get(), sizeof(int), hipMemcpyDeviceToHost, stream_));
         MAPP_HIP(hipStreamSynchronize(stream_));
         if (over <= maxn_) return;
-        maxn_ = over + over / 8 + 8;
+        maxn_ = (over + over / 8 + 8 + 3) & ~3;
     }
     throw Error(ERR_STATE, "neighbor list capacity did not converge");
 }
 
-void MDSystem::force_passes(const KickArgs* kick, bool records_fresh, float4* f_out, double* rho_out)
+void MDSystem::force_passes(const KickArgs* kick, bool records_fresh, float4* f_out, double* rho_out, bool ev)
 {
     const int nb = grid_for(n_);
     float4* fo = f_out ? f_out : f_.get();
@@ -1244,11 +1386,13 @@ void MDSystem::force_passes(const KickArgs* kick, bool records_fresh, float4* f_
         K.on = 1;
     }
     double* p3 = part_.get() + (size_t)2 * nb * kPartW;
+    double* p4 = part_.get() + (size_t)3 * nb * kPartW;
+    const int maxb = maxn_ / 4;
     if (lds) {
         // at most one workgroup per CU, and never more rows than the partial-sum buffer has per pass
         const int G = std::min(lds_grid_, nb);
 #define MAPP_RHO(B) hipLaunchKernelGGL((k_eam_rho_lds<B>), dim3(G), dim3(kBigBlock), rho_lds_bytes_, stream_, n_, \
-                                       rec_.get(), nn_.get(), list_.get(), ldl_, maxn_, P, r16_, rho_m_lo_, rho_nwin_, \
+                                       rec_.get(), nn_.get(), list4(), ldl_, maxb, P, r16_, rho_m_lo_, rho_nwin_, \
                                        ro, xr_.get(), part_.get())
         switch (batch_rho_) {
             case 1: MAPP_RHO(1); break;
@@ -1260,22 +1404,26 @@ void MDSystem::force_passes(const KickArgs* kick, bool records_fresh, float4* f_
         prof_.end(stream_);
         prof_.begin(K_EAM_FORCE, stream_);
         double* p2 = part_.get() + (size_t)G * kPartW;
-#define MAPP_FORCE(B) hipLaunchKernelGGL((k_eam_force_lds<B>), dim3(G), dim3(kBigBlock), f_lds_bytes_, stream_, n_, \
-                                         xr_.get(), r16_, nn_.get(), list_.get(), ldl_, maxn_, P, f_m_lo_, f_nwin_, \
-                                         fo, p2, K, v_.get(), p3)
-        // 1024-thread workgroups leave 128 VGPRs per lane: batches above 2 would spill
-        if (batch_force_ == 1)
-            MAPP_FORCE(1);
-        else
-            MAPP_FORCE(2);
+#define MAPP_FORCE(B, EV) hipLaunchKernelGGL((k_eam_force_lds<B, EV>), dim3(G), dim3(kBigBlock), f_lds_bytes_, \
+                                             stream_, n_, xr_.get(), r16_, nn_.get(), list4(), ldl_, maxb, P, f_m_lo_, \
+                                             f_nwin_, fo, p2, K, v_.get(), p3, p4)
+        // 1024-thread workgroups leave 128 VGPRs per lane: batches of 4 fit
+        // without the FP64 energy and virial sums only
+        if (batch_force_ == 1) {
+            if (ev) MAPP_FORCE(1, true); else MAPP_FORCE(1, false);
+        } else if (batch_force_ >= 4 && !ev) {
+            MAPP_FORCE(4, false);
+        } else {
+            if (ev) MAPP_FORCE(2, true); else MAPP_FORCE(2, false);
+        }
 #undef MAPP_FORCE
         prof_.end(stream_);
         mvv_rows_ = G;
-        reduce_rows(part_.get(), 2 * G, 7, red_.get());
+        if (ev) reduce_rows(part_.get(), 2 * G, 7, red_.get());
         return;
     }
 #define MAPP_RHO(B, ONE) hipLaunchKernelGGL((k_eam_rho<B, ONE>), dim3(nb), dim3(kBlock), 0, stream_, n_, rec_.get(), \
-                                            nn_.get(), list_.get(), ldl_, maxn_, P, r16_, ro, xr_.get(), \
+                                            nn_.get(), list4(), ldl_, maxb, P, r16_, ro, xr_.get(), \
                                             dF_.get(), part_.get())
     switch (batch_rho_) {
         case 1: if (one) MAPP_RHO(1, true); else MAPP_RHO(1, false); break;
@@ -1287,19 +1435,27 @@ void MDSystem::force_passes(const KickArgs* kick, bool records_fresh, float4* f_
     prof_.end(stream_);
     prof_.begin(K_EAM_FORCE, stream_);
     double* p2 = part_.get() + (size_t)nb * kPartW;
-#define MAPP_FORCE(B, ONE) hipLaunchKernelGGL((k_eam_force<B, ONE>), dim3(nb), dim3(kBlock), 0, stream_, n_, \
-                                              rec_.get(), xr_.get(), dF_.get(), r16_, nn_.get(), list_.get(), ldl_, \
-                                              maxn_, P, fo, p2, K, v_.get(), p3)
+#define MAPP_FORCE(B, ONE, EV) hipLaunchKernelGGL((k_eam_force<B, ONE, EV>), dim3(nb), dim3(kBlock), 0, stream_, n_, \
+                                                  rec_.get(), xr_.get(), dF_.get(), r16_, nn_.get(), list4(), ldl_, \
+                                                  maxb, P, fo, p2, K, v_.get(), p3, p4)
+#define MAPP_FORCE_EV(B, ONE) \
+    do {                      \
+        if (ev)               \
+            MAPP_FORCE(B, ONE, true); \
+        else                  \
+            MAPP_FORCE(B, ONE, false); \
+    } while (0)
     switch (batch_force_) {
-        case 1: if (one) MAPP_FORCE(1, true); else MAPP_FORCE(1, false); break;
-        case 2: if (one) MAPP_FORCE(2, true); else MAPP_FORCE(2, false); break;
-        case 8: if (one) MAPP_FORCE(8, true); else MAPP_FORCE(8, false); break;
-        default: if (one) MAPP_FORCE(4, true); else MAPP_FORCE(4, false); break;
+        case 1: if (one) MAPP_FORCE_EV(1, true); else MAPP_FORCE_EV(1, false); break;
+        case 2: if (one) MAPP_FORCE_EV(2, true); else MAPP_FORCE_EV(2, false); break;
+        case 8: if (one) MAPP_FORCE_EV(8, true); else MAPP_FORCE_EV(8, false); break;
+        default: if (one) MAPP_FORCE_EV(4, true); else MAPP_FORCE_EV(4, false); break;
     }
+#undef MAPP_FORCE_EV
 #undef MAPP_FORCE
     prof_.end(stream_);
     mvv_rows_ = nb;
-    reduce_rows(part_.get(), 2 * nb, 7, red_.get());
+    if (ev) reduce_rows(part_.get(), 2 * nb, 7, red_.get());
 }
 
 MDReport MDSystem::reduce_report(int)
@@ -1502,7 +1658,7 @@ int64_t MDSystem::half_pairs_count()
     int64_t c = 0;
     for (int s = 0; s < n_; ++s)
         for (int k = 0; k < h.nn[s]; ++k) {
-            const uint32_t e = h.list[(size_t)k * ldl_ + s];
+            const uint32_t e = h.list[((size_t)(k >> 2) * ldl_ + s) * 4 + (k & 3)];
             const uint32_t code = e >> kIndexBits;
             if (code != (uint32_t)kZeroImage || h.id[s] < h.id[e & kIndexMask]) ++c;
         }
@@ -1515,7 +1671,7 @@ void MDSystem::half_pairs(int32_t* pi, int32_t* pj, int8_t* img)
     int64_t c = 0;
     for (int s = 0; s < n_; ++s)
         for (int k = 0; k < h.nn[s]; ++k) {
-            const uint32_t e = h.list[(size_t)k * ldl_ + s];
+            const uint32_t e = h.list[((size_t)(k >> 2) * ldl_ + s) * 4 + (k & 3)];
             const uint32_t code = e >> kIndexBits;
             const int j = h.id[e & kIndexMask];
             if (code != (uint32_t)kZeroImage || h.id[s] < j) {
@@ -1548,15 +1704,19 @@ int MDSystem::nvt_run(const double* masses, double kB, double T, double dt, int 
         A.mass[a] = masses[a];
     }
     double* nh = red_.get() + 32;  // NH_SIZE doubles of thermostat state
-    double* com = red_.get() + 16;
+    double* cv = red_.get() + 24;  // sums of v and f / m for the centre-of-mass shift
     double* mvv = red_.get() + 8;
+    double* rows_mvv = part_.get() + (size_t)2 * nb * kPartW;
+    double* rows_cv = part_.get() + (size_t)3 * nb * kPartW;
+    double* rows_sv = part_.get() + (size_t)4 * nb * kPartW;
     MAPP_HIP(hipMemsetAsync(nh, 0, sizeof(double) * NH_SIZE, stream_));
     const int rebuilds0 = nrebuild_;
     force_passes(nullptr);
-    hipLaunchKernelGGL(k_kick2, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), v_.get(), f_.get(), A, 0,
-                       part_.get() + (size_t)2 * nb * kPartW);
-    reduce_rows(part_.get() + (size_t)2 * nb * kPartW, nb, 6, mvv);
-    hipLaunchKernelGGL(k_nhc, dim3(1), dim3(64), 0, stream_, nh, mvv, A, 0);
+    hipLaunchKernelGGL(k_kick2, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), v_.get(), f_.get(), A, 0, rows_mvv,
+                       rows_cv);
+    reduce_rows(rows_mvv, nb, 6, mvv);
+    reduce_rows(rows_cv, nb, 6, cv);
+    hipLaunchKernelGGL(k_nhc, dim3(1), dim3(64), 0, stream_, nh, mvv, cv, A, 0);
     const double skin_sq4 = 0.25 * skin_ * skin_;
     KickArgs kick{};
     kick.dt2 = A.dt2;
@@ -1564,21 +1724,21 @@ int MDSystem::nvt_run(const double* masses, double kB, double T, double dt, int 
     int flag = 0;
     for (int step = 0; step < nsteps; ++step) {
         prof_.begin(K_INTEGRATE, stream_);
-        hipLaunchKernelGGL(k_nhc, dim3(1), dim3(64), 0, stream_, nh, mvv, A, 1);
-        hipLaunchKernelGGL(k_kick_drift, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), v_.get(), f_.get(), A, nh,
-                           part_.get() + (size_t)2 * nb * kPartW);
-        reduce_rows(part_.get() + (size_t)2 * nb * kPartW, nb, 3, com);
+        hipLaunchKernelGGL(k_nhc, dim3(1), dim3(64), 0, stream_, nh, mvv, cv, A, 1);
         MAPP_HIP(hipMemsetAsync(flags_.get() + 1, 0, sizeof(int), stream_));
-        hipLaunchKernelGGL(k_shift_decide, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), x0_.get(), com,
-                           1.0 / n_, skin_sq4, flags_.get(), r16_, rec_.get());
+        hipLaunchKernelGGL(k_step, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), v_.get(), f_.get(), A, nh,
+                           x0_.get(), skin_sq4, flags_.get(), r16_, rec_.get(), rows_sv);
+        reduce_rows(rows_sv, nb, 3, cv);
         prof_.end(stream_);
         MAPP_HIP(hipMemcpyAsync(&flag, flags_.get() + 1, sizeof(int), hipMemcpyDeviceToHost, stream_));
         MAPP_HIP(hipStreamSynchronize(stream_));
         if (flag) rebuild();  // the permute rewrites the records
-        force_passes(&kick, true);
+        // energy and virial only where they are read: after the last step
+        force_passes(&kick, true, nullptr, nullptr, step == nsteps - 1);
         prof_.begin(K_INTEGRATE, stream_);
-        reduce_rows(part_.get() + (size_t)2 * nb * kPartW, mvv_rows_, 6, mvv);
-        hipLaunchKernelGGL(k_nhc, dim3(1), dim3(64), 0, stream_, nh, mvv, A, 2);
+        reduce_rows(rows_mvv, mvv_rows_, 6, mvv);
+        reduce_rows(rows_cv, mvv_rows_, 3, cv + 3);
+        hipLaunchKernelGGL(k_nhc, dim3(1), dim3(64), 0, stream_, nh, mvv, cv, A, 2);
         prof_.end(stream_);
     }
     hipLaunchKernelGGL(k_scale_v, dim3(nb), dim3(kBlock), 0, stream_, n_, v_.get(), nh);

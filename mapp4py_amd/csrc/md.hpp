@@ -160,7 +160,7 @@ private:
     // the two EAM passes; f_out / rho_out (nullptr: f_ and rho_) receive the
     // forces and densities
     void force_passes(const KickArgs* kick, bool records_fresh = false, float4* f_out = nullptr,
-                      double* rho_out = nullptr);
+                      double* rho_out = nullptr, bool ev = true);
     MDReport reduce_report(int nparts);
     void reduce_rows(const double* part, int nrows, int ncols, double* out);
     bool decide();
@@ -194,8 +194,11 @@ private:
     DevBuf<unsigned char> sort_tmp_;
     size_t sort_tmp_bytes_ = 0;
     DevBuf<int> cell_rank_, cell_start_;
+    DevBuf<uint4> cell_order_;       // per cell: the 27 stencil cells in Morton-rank order (32 bytes)
+    bool ordered_ = false;           // a 3x3x3 stencil of distinct cells: cell_order_ holds it
     DevBuf<int> nn_;
-    DevBuf<uint32_t> list_;
+    DevBuf<uint32_t> list_;          // 16-byte blocks: entries k..k+3 of slot i at block (k / 4) * ldl_ + i
+    uint4* list4() const { return reinterpret_cast<uint4*>(list_.p); }
     int maxn_ = 0;
     int ldl_ = 0;                    // leading dimension of the list (slots)
     DevBuf<int> flags_;              // [0] overflow max count, [1] rebuild needed

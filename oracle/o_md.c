@@ -8,6 +8,9 @@
  */
 #include "o_internal.h"
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -259,6 +262,8 @@ struct o_md {
     int64_t npairs, cap;
     double *rho, *dF, *drhoi, *drhoj;
     int64_t nscratch;
+    double* thr_rows; /* per-thread rho and f rows of the threaded force */
+    size_t thr_cap;
 };
 
 typedef struct { o_md* md; const double* xi; int ei; } list_ctx;
@@ -292,6 +297,131 @@ static void list_post(void* v, int i)
     c->md->row[i + 1] = c->md->npairs;
 }
 
+/* ---------------- threads (CPU baseline only) ----------------
+ * With o_threads > 1 (o_set_threads) the list build and the EAM force run on
+ * that many OpenMP threads: the same rows in the same order, the same
+ * arithmetic per pair, sums of per-thread rows in a different order. The
+ * checker itself always runs with one thread. */
+static int o_threads = 1;
+
+void o_set_threads(int n)
+{
+#ifdef _OPENMP
+    o_threads = n > 1 ? (n < 16 ? n : 16) : 1;
+#else
+    (void)n;
+    o_threads = 1;
+#endif
+}
+
+int o_get_threads(void) { return o_threads; }
+
+/* Cell<1>::DO's traversal for one local atom i (the sequential loop of
+ * o_cell_do): calls emit(ctx, j) for every j of the half list in list order */
+typedef struct {
+    int* head; int* next; int* cell; int rel[27];
+} o_cells;
+
+static void cells_build(o_cells* cs, const o_ghosts* g, const o_box* bx, double max_cut)
+{
+    const int M = 1;
+    double cut_s[3], csz[3];
+    int nc[3], den[3], ncells = 1;
+    for (int d = 0; d < 3; d++) {
+        cut_s[d] = max_cut * bx->depth_inv[d];
+        csz[d] = cut_s[d] / (double)M;
+        nc[d] = (int)((1.0 - 0.0) / csz[d]) + 1 + 2 * M;
+        den[d] = ncells;
+        ncells *= nc[d];
+    }
+    int cnt[3] = {-1, -1, -1};
+    for (int k = 0; k < 27; k++) {
+        cs->rel[k] = cnt[0] * den[0] + cnt[1] * den[1] + cnt[2] * den[2];
+        cnt[0]++;
+        for (int d = 0; d < 2; d++)
+            if (cnt[d] == M + 1) { cnt[d] = -M; cnt[d + 1]++; }
+    }
+    cs->head = (int*)malloc(sizeof(int) * ncells);
+    for (int c = 0; c < ncells; c++) cs->head[c] = -1;
+    cs->next = (int*)malloc(sizeof(int) * (g->nall > 0 ? g->nall : 1));
+    cs->cell = (int*)malloc(sizeof(int) * (g->nall > 0 ? g->nall : 1));
+    for (int i = g->nall - 1; i > g->nlcl - 1; i--) {
+        const double* sv = g->s + 3 * i;
+        int c = 0;
+        for (int d = 0; d < 3; d++) {
+            int k;
+            if (sv[d] < 0.0) {
+                k = (int)((sv[d] - (0.0 - cut_s[d])) / csz[d]);
+                if (k > M - 1) k = M - 1;
+            } else if (1.0 <= sv[d]) {
+                k = (int)((sv[d] - 0.0) / csz[d]);
+                if (k < nc[d] - 2 * M - 1) k = nc[d] - 2 * M - 1;
+                if (k > nc[d] - 1 - M) k = nc[d] - 1 - M;
+                k += M;
+            } else {
+                k = (int)((sv[d] - 0.0) / csz[d]);
+                if (k > nc[d] - 2 * M - 1) k = nc[d] - 2 * M - 1;
+                k += M;
+            }
+            c += den[d] * k;
+        }
+        cs->cell[i] = c;
+        cs->next[i] = cs->head[c];
+        cs->head[c] = i;
+    }
+    for (int i = g->nlcl - 1; i > -1; i--) {
+        const double* sv = g->s + 3 * i;
+        int c = 0;
+        for (int d = 0; d < 3; d++) {
+            int k = (int)((sv[d] - 0.0) / csz[d]);
+            if (k > nc[d] - 2 * M - 1) k = nc[d] - 2 * M - 1;
+            c += den[d] * (k + M);
+        }
+        cs->cell[i] = c;
+        cs->next[i] = cs->head[c];
+        cs->head[c] = i;
+    }
+}
+
+static void cells_free(o_cells* cs) { free(cs->head); free(cs->next); free(cs->cell); }
+
+/* the half list by threads: rows counted, offsets summed, rows filled, each
+ * row in the order of the sequential traversal */
+static void list_build_threaded(o_md* md)
+{
+    o_cells cs;
+    cells_build(&cs, &md->g, &md->bx, md->max_cut);
+    const int n = md->n, ne = md->ff->ne;
+    const double* x = md->g.x;
+    int64_t* cnt = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+#pragma omp parallel for schedule(dynamic, 64) num_threads(o_threads)
+    for (int i = 0; i < n; i++) {
+        int64_t c = 0;
+        const int ei = md->elem[i];
+        for (int k = 0; k < 27; k++)
+            for (int j = cs.head[cs.cell[i] + cs.rel[k]]; j != -1; j = cs.next[j])
+                if (i < j && o_rsq(x + 3 * i, x + 3 * j) < md->cut_sk_sq[ei * ne + md->elem[j]]) c++;
+        cnt[i + 1] = c;
+    }
+    md->row[0] = 0;
+    for (int i = 0; i < n; i++) md->row[i + 1] = md->row[i] + cnt[i + 1];
+    free(cnt);
+    md->npairs = md->row[n];
+    if (md->cap < md->npairs) {
+        md->cap = md->npairs + 1024;
+        md->col = (int32_t*)realloc(md->col, sizeof(int32_t) * md->cap);
+    }
+#pragma omp parallel for schedule(dynamic, 64) num_threads(o_threads)
+    for (int i = 0; i < n; i++) {
+        int64_t c = md->row[i];
+        const int ei = md->elem[i];
+        for (int k = 0; k < 27; k++)
+            for (int j = cs.head[cs.cell[i] + cs.rel[k]]; j != -1; j = cs.next[j])
+                if (i < j && o_rsq(x + 3 * i, x + 3 * j) < md->cut_sk_sq[ei * ne + md->elem[j]]) md->col[c++] = j;
+    }
+    cells_free(&cs);
+}
+
 /* rebuild: wrap locals (x2s_lcl), ghosts (Update::list), Cell<1>::DO with
  * s2x_all, NeighborMD::create_list (src/neighbor_md.cpp:82-152), store x0 */
 static void md_rebuild(o_md* md, const double* x_lcl)
@@ -302,10 +432,14 @@ static void md_rebuild(o_md* md, const double* x_lcl)
     free(s);
     md->elem = (uint8_t*)realloc(md->elem, md->g.nall > 0 ? md->g.nall : 1);
     for (int k = 0; k < md->g.nall; k++) md->elem[k] = md->elem_lcl[md->g.owner[k]];
-    md->row[0] = 0;
-    md->npairs = 0;
-    list_ctx c = {md, NULL, 0};
-    o_cell_do(&md->g, &md->bx, md->max_cut, list_pre, list_visit, list_post, &c);
+    if (o_threads > 1) {
+        list_build_threaded(md);
+    } else {
+        md->row[0] = 0;
+        md->npairs = 0;
+        list_ctx c = {md, NULL, 0};
+        o_cell_do(&md->g, &md->bx, md->max_cut, list_pre, list_visit, list_post, &c);
+    }
     memcpy(md->x0, md->g.x, sizeof(double) * 3 * md->n);
 }
 
@@ -345,6 +479,7 @@ void o_md_free(o_md* md)
     o_ghosts_free(&md->g);
     free(md->cut_sq); free(md->cut_sk_sq); free(md->elem); free(md->elem_lcl); free(md->x0);
     free(md->row); free(md->col); free(md->rho); free(md->dF); free(md->drhoi); free(md->drhoj);
+    free(md->thr_rows);
     free(md);
 }
 
@@ -485,11 +620,146 @@ static void eam_force(o_md* md, double* f, double* v7, double* rho_out)
     if (rho_out) memcpy(rho_out, rho, sizeof(double) * n);
 }
 
+/* eam_force on o_threads threads: the same per-pair arithmetic over the same
+ * half list; rho, f and the sums go to per-thread rows that are added at the
+ * end (so F(rho_i) follows the complete density of every atom) */
+static void eam_force_threaded(o_md* md, double* f, double* v7, double* rho_out)
+{
+    const o_eam* e = md->ff;
+    const int ne = e->ne, n = md->n, T = o_threads;
+    const double* x = md->g.x;
+    const uint8_t* el = md->elem;
+    const double dr_inv = 1.0 / e->dr, drho_inv = 1.0 / e->drho;
+    const double rho_max = (double)e->nrho * e->drho;
+    if (md->nscratch < md->npairs || !md->rho) {
+        free(md->drhoi); free(md->drhoj);
+        md->nscratch = md->npairs > 0 ? md->npairs : 1;
+        md->drhoi = (double*)malloc(sizeof(double) * md->nscratch);
+        md->drhoj = (double*)malloc(sizeof(double) * md->nscratch);
+    }
+    md->rho = (double*)realloc(md->rho, sizeof(double) * (n > 0 ? n : 1));
+    md->dF = (double*)realloc(md->dF, sizeof(double) * (md->g.nall > 0 ? md->g.nall : 1));
+    double* rho = md->rho;
+    double* dF = md->dF;
+    const size_t w = (size_t)(n > 0 ? n : 1);
+    if (md->thr_cap < (size_t)T * 4 * w) {
+        free(md->thr_rows);
+        md->thr_cap = (size_t)T * 4 * w;
+        md->thr_rows = (double*)malloc(sizeof(double) * md->thr_cap);
+    }
+    double* rho_t = md->thr_rows;
+    double* f_t = md->thr_rows + (size_t)T * w;
+    double v_t[16 * 8];
+    memset(v_t, 0, sizeof v_t);
+#pragma omp parallel num_threads(T)
+    {
+#ifdef _OPENMP
+        const int t = omp_get_thread_num();
+#else
+        const int t = 0;
+#endif
+        double* rt = rho_t + (size_t)t * w;
+        double* ft = f_t + (size_t)t * 3 * w;
+        double* vt = v_t + (size_t)t * 8;
+        memset(rt, 0, sizeof(double) * w);
+        memset(ft, 0, sizeof(double) * 3 * w);
+#pragma omp barrier
+#pragma omp for schedule(dynamic, 64)
+        for (int i = 0; i < n; i++) {
+            const int ei = el[i];
+            for (int64_t k = md->row[i]; k < md->row[i + 1]; k++) {
+                const int j = md->col[k];
+                const int ej = el[j];
+                const double dx0 = x[3 * i] - x[3 * j];
+                const double dx1 = x[3 * i + 1] - x[3 * j + 1];
+                const double dx2 = x[3 * i + 2] - x[3 * j + 2];
+                const double rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+                md->drhoi[k] = md->drhoj[k] = 0.0;
+                if (rsq >= md->cut_sq[ei * ne + ej]) continue;
+                const double r = sqrt(rsq);
+                const double r_inv = 1.0 / r;
+                size_t m;
+                double p;
+                o_spline_pm(r, dr_inv, e->nr, &m, &p);
+                const double* c = ORHO(e, ej, ei)[m];
+                const double rho_i = hval(c, p), drho_i = hder(c, p, dr_inv);
+                c = ORHO(e, ei, ej)[m];
+                const double rho_j = hval(c, p), drho_j = hder(c, p, dr_inv);
+                c = OPHI(e, ei, ej)[m];
+                const double z2p = hder(c, p, dr_inv), z2 = hval(c, p);
+                double phi = z2 * r_inv;
+                const double dphi = z2p * r_inv - phi * r_inv;
+                rt[i] += rho_i;
+                if (j < n) rt[j] += rho_j;
+                double fp = -dphi * r_inv;
+                ft[3 * i] += fp * dx0; ft[3 * i + 1] += fp * dx1; ft[3 * i + 2] += fp * dx2;
+                if (j < n) { ft[3 * j] -= fp * dx0; ft[3 * j + 1] -= fp * dx1; ft[3 * j + 2] -= fp * dx2; }
+                else { fp *= 0.5; phi *= 0.5; }
+                vt[0] += phi;
+                vt[1] -= fp * dx0 * dx0; vt[2] -= fp * dx0 * dx1; vt[3] -= fp * dx0 * dx2;
+                vt[4] -= fp * dx1 * dx1; vt[5] -= fp * dx1 * dx2; vt[6] -= fp * dx2 * dx2;
+                md->drhoi[k] = -drho_i * r_inv;
+                md->drhoj[k] = -drho_j * r_inv;
+            }
+        }
+        /* complete densities, then F(rho_i) and F' */
+#pragma omp for schedule(static)
+        for (int i = 0; i < n; i++) {
+            double sum = 0.0;
+            for (int q = 0; q < T; q++) sum += rho_t[(size_t)q * w + i];
+            rho[i] = sum;
+            size_t m;
+            double p;
+            o_spline_pm(sum, drho_inv, e->nrho, &m, &p);
+            const double* c = OF(e, el[i]) + m * e->nf;
+            const double t1 = hder(c, p, drho_inv);
+            double t0 = hval(c, p);
+            if (sum > rho_max) t0 += t1 * (sum - rho_max);
+            vt[7] += t0;
+            dF[i] = t1;
+        }
+        /* a ghost's source may be another ghost: take the local owner's F' */
+#pragma omp for schedule(static)
+        for (int k = n; k < md->g.nall; k++) dF[k] = dF[md->g.owner[k]];
+#pragma omp for schedule(dynamic, 64)
+        for (int i = 0; i < n; i++) {
+            for (int64_t k = md->row[i]; k < md->row[i + 1]; k++) {
+                if (md->drhoi[k] == 0.0 && md->drhoj[k] == 0.0) continue;
+                const int j = md->col[k];
+                double fp = dF[i] * md->drhoi[k] + dF[j] * md->drhoj[k];
+                const double dx0 = x[3 * i] - x[3 * j];
+                const double dx1 = x[3 * i + 1] - x[3 * j + 1];
+                const double dx2 = x[3 * i + 2] - x[3 * j + 2];
+                ft[3 * i] += dx0 * fp; ft[3 * i + 1] += dx1 * fp; ft[3 * i + 2] += dx2 * fp;
+                if (j < n) { ft[3 * j] -= dx0 * fp; ft[3 * j + 1] -= dx1 * fp; ft[3 * j + 2] -= dx2 * fp; }
+                else fp *= 0.5;
+                vt[1] -= fp * dx0 * dx0; vt[2] -= fp * dx0 * dx1; vt[3] -= fp * dx0 * dx2;
+                vt[4] -= fp * dx1 * dx1; vt[5] -= fp * dx1 * dx2; vt[6] -= fp * dx2 * dx2;
+            }
+        }
+#pragma omp for schedule(static)
+        for (int i = 0; i < 3 * n; i++) {
+            double sum = 0.0;
+            for (int q = 0; q < T; q++) sum += f_t[(size_t)q * 3 * w + i];
+            f[i] = sum;
+        }
+    }
+    for (int k = 0; k < 7; k++) v7[k] = 0.0;
+    for (int q = 0; q < T; q++) {
+        for (int k = 0; k < 7; k++) v7[k] += v_t[8 * q + k];
+        v7[0] += v_t[8 * q + 7];
+    }
+    if (rho_out) memcpy(rho_out, rho, sizeof(double) * n);
+}
+
 /* ForceFieldMD::force_calc (src/ff_md.cpp:125-144): out7[1..6] = -W (F_H) */
 void o_md_force(o_md* md, double* f, double* out7, double* rho_out)
 {
     double v7[7];
-    eam_force(md, f, v7, rho_out);
+    if (o_threads > 1)
+        eam_force_threaded(md, f, v7, rho_out);
+    else
+        eam_force(md, f, v7, rho_out);
     out7[0] = v7[0];
     for (int k = 1; k < 7; k++) out7[k] = -v7[k];
 }

@@ -110,6 +110,11 @@ double o_nhc_run(double T_r, int ndof, double dt, double t_relax, int niters, in
  * thermostat, 1 ddt and ddt2 swapped, 2 the sign of the forward sweep's chain
  * coupling flipped, 3 N_f xi_1^2 in the reverse sweep too. */
 extern int o_nhc_mutation;
+/* Threads of the MD list build and force (1, the default, for the checker;
+ * more only for the CPU baseline's all-cores timing). o_get_threads gives
+ * the count in effect (1 without OpenMP). */
+void o_set_threads(int n);
+int o_get_threads(void);
 /* AtomsMD::create_T (src/atoms_md.cpp:130-160) */
 void o_create_T(int natms, const uint8_t* elem, const double* masses, double kB,
                 double T, int seed, double* v);

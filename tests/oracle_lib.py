@@ -61,6 +61,8 @@ def L():
                                             dp, dp]),
             "o_nhc_run": (C.c_double, [C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, dp]),
             "o_create_T": (None, [C.c_int, u8p, dp, C.c_double, C.c_double, C.c_int, dp]),
+            "o_set_threads": (None, [C.c_int]),
+            "o_get_threads": (C.c_int, []),
             "o_dmd_create": (C.c_void_p, [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, dp, u8p, dp, dp, C.c_double,
                                           C.c_double, C.c_double, C.c_double, dp]),
             "o_dmd_free": (None, [C.c_void_p]),
@@ -235,6 +237,22 @@ def nhc_run(T_r, ndof, dt, t_relax, ncalls, niters=1):
     hist = np.zeros((ncalls, 4))
     scale = L().o_nhc_run(T_r, ndof, dt, t_relax, niters, ncalls, P(hist))
     return scale, hist
+
+
+class threads:
+    """context manager: the oracle's MD list build and force on k OpenMP
+    threads (the CPU baseline's all-cores mode); the checker uses one"""
+
+    def __init__(self, k):
+        self.k = k
+
+    def __enter__(self):
+        L().o_set_threads(self.k)
+        return L().o_get_threads()
+
+    def __exit__(self, *exc):
+        L().o_set_threads(1)
+        return False
 
 
 class nhc_mutation:

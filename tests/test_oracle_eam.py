@@ -279,3 +279,33 @@ def test_nvt_conserves_the_chains_extended_energy(cu):
     for mutation in (1, 2, 3):
         # measured 3.8, 196 and 0.52 eV
         assert run(mutation)[0] > 0.2, mutation
+
+
+def test_threaded_oracle_is_the_same_computation(feni):
+    # the CPU baseline's all-cores mode: identical half list (same rows, same
+    # order) and the same forces, energy and virial to rounding of the sums
+    p, ff = feni
+    x, H = pot.bcc(2.86, (6, 6, 6))
+    x = jitter(x, 0.08, 13)
+    elem = (np.random.default_rng(14).random(len(x)) < 0.3).astype(np.uint8)
+    one = O.OMD(ff, H, x, elem, skin=0.4)
+    f1, e1, W1, rho1 = one.force()
+    pairs1 = one.pairs()
+    with O.threads(4) as k:
+        assert k == 4
+        many = O.OMD(ff, H, x, elem, skin=0.4)
+        f4, e4, W4, rho4 = many.force()
+        pairs4 = many.pairs()
+        masses = [pot.MASS["Fe"], pot.MASS["Ni"]]
+        v = O.o_create_T(elem, masses, pot.KB_EV, 800.0, 3)
+        v4, out4, T4, nreb4 = many.nvt_run(v, masses, pot.KB_EV, 800.0, pot.FS, 60)
+    for a, b in zip(pairs1, pairs4):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_allclose(f4, f1, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(rho4, rho1, rtol=1e-14)
+    assert abs(e4 - e1) < 1e-11 * abs(e1)
+    np.testing.assert_allclose(W4, W1, rtol=0, atol=1e-10)
+    v1, out1, T1, nreb1 = one.nvt_run(v, masses, pot.KB_EV, 800.0, pot.FS, 60)
+    assert nreb1 == nreb4 >= 1
+    np.testing.assert_allclose(v4, v1, rtol=0, atol=1e-9)
+    assert abs(T4 - T1) < 1e-9 * T1
