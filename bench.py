@@ -83,7 +83,30 @@ def gpu_run(args):
         f"PE/atom {out[0] / n:.6f} eV, list {nfull / n:.1f}/atom (max {maxn})")
     for k, (ms, c) in kern.items():
         log(f"[bench]   {k:11s} {ms:9.2f} ms total, {c:5d} launches, {ms / max(c, 1):8.3f} ms avg")
-    return n, wall, kern, nvt, out, nfull
+    return n, wall, kern, nvt, out, nfull, md, p, H, elem
+
+
+def headline_error(md, p, H, elem, threads):
+    """Forces, energy and virial of the GPU at the end of the timed run against
+    the FP64 oracle (threaded) on the same 8,388,608 positions."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    t0 = time.perf_counter()
+    f, e, FH = md.force_calc()
+    x = md.x
+    n = len(x)
+    with O.threads(threads):
+        o = O.OMD(O.OEam.from_knots(p), H, x, elem, skin=0.5)
+        fo, eo, FHo, _ = o.force()
+        del o
+    df = np.abs(f - fo)
+    dt = time.perf_counter() - t0
+    return {"max_abs_df_eV_per_A": float(df.max()), "rms_df_eV_per_A": float(np.sqrt((df ** 2).mean())),
+            "bar_eV_per_A": 1e-5, "abs_dE_eV": float(abs(e - eo)), "abs_dE_per_atom_eV": float(abs(e - eo) / n),
+            "max_abs_dF_H_eV": float(np.abs(FH - FHo).max()), "max_abs_F_H_eV": float(np.abs(FHo).max()),
+            "natoms": n, "oracle_threads": threads, "seconds": dt,
+            "what": "GPU force_calc at the final state of the timed run vs the FP64 oracle on the same positions"}
 
 
 def host_threads():
@@ -290,10 +313,32 @@ def dmd_run(evals=10):
     for _ in range(evals):
         d.derivative()
     dt_host = time.perf_counter() - t0
-    f0, fa0, _, fe0, _, _ = d.derivative()
+    f0, fa0, mu0, fe0, FH0, _ = d.derivative()
     half_pairs = int(d.half_pairs()[0].shape[0])
     n = len(x)
     ms = dt / evals * 1e3
+    # FP32 table rows (FP64 moments and sums): the same evaluations, and their
+    # error against the FP64 rows (which equal the oracle to 1e-9)
+    d.set_fp32(True)
+    d.evaluate()
+    d.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(evals):
+        d.evaluate()
+    dt32 = time.perf_counter() - t0
+    k1f, k2f = d.profile_read("dmd_pass1"), d.profile_read("dmd_pass2")
+    d.profile(False)
+    f32r, fa32, mu32, fe32, FH32, _ = d.derivative()
+    d.set_fp32(False)
+    fp32 = {"value": n * evals / dt32, "ms_per_eval": dt32 / evals * 1e3, "pass1_ms": k1f[0] / max(k1f[1], 1),
+            "pass2_ms": k2f[0] / max(k2f[1], 1), "speed_vs_fp64_rows": dt / dt32,
+            "max_abs_df_x_eV_per_A": float(np.abs(f32r - f0).max()),
+            "max_abs_df_alpha_eV_per_A": float(np.abs(fa32 - fa0).max()),
+            "max_abs_dmu_eV": float(np.abs(mu32 - mu0).max()),
+            "free_energy_rel_err": abs(fe32 - fe0) / abs(fe0),
+            "virial_rel_err": float(np.abs(FH32 - FH0).max() / np.abs(FH0).max()),
+            "dtype": "f32 table rows, f64 moments and sums",
+            "errors_against": "the FP64 rows on the GPU (equal to the FP64 oracle to 1e-9 relative)"}
     # the learned surrogate (config 4): trained on the GPU from this rule, then the same evaluation
     s = M.Surrogate(2, box=(2.2, 6.1, 0.05, 0.30), seed=1)
     t0 = time.perf_counter()
@@ -379,6 +424,7 @@ def dmd_run(evals=10):
                         "alpha log-uniform [0.05,0.30] A, c_H ~ Beta(2,5)",
             "metric": "DMD site-evals/s (free energy + gradient)", "value": n * evals / dt, "unit": "site-evals/s",
             "ms_per_eval": ms, "pass1_ms": k1[0] / max(k1[1], 1), "pass2_ms": k2[0] / max(k2[1], 1),
+            "fp32_rows": fp32,
             "host_copy_inclusive": {"value": n * evals / dt_host, "ms_per_eval": dt_host / evals * 1e3,
                                     "note": "f, f_alpha, mu gathered to caller order and copied to host each time"},
             "dtype": "f64", "half_pairs": half_pairs,
@@ -665,12 +711,13 @@ def gcmc_cpu_baseline(seconds):
                       f"attempts only, {dt:.1f} s"}
 
 
-def load_traffic():
-    path = os.path.join(ROOT, "profiles", "eam_force_traffic.json")
+def load_profiled(name):
+    """a profiled constant committed under profiles/ (JSON), or None"""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        return json.load(fh).get("bytes_per_launch")
+        return json.load(fh)
 
 
 def main():
@@ -700,20 +747,38 @@ def main():
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
-    n, wall, kern, nvt, out, nfull = gpu_run(args)
+    n, wall, kern, nvt, out, nfull, md, p, H, elem = gpu_run(args)
     value = n * args.steps / wall
     ms_f, c_f = kern["eam_force"]
     avg_s = ms_f / max(c_f, 1) / 1e3
     achieved = PASS2_BYTES_PER_ATOM * n / avg_s if avg_s > 0 else 0.0
-    traffic = load_traffic()
-    roof = {"kernel": "k_eam_force_lds (EAM pass 2 + second half-kick, spline rows in LDS)", "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+    traffic = load_profiled("eam_force_traffic.json")
+    issue = load_profiled("eam_force_issue.json")
+    roof = {"kernel": "k_eam_force_lds (EAM pass 2 + second half-kick, spline rows in LDS)", "bound": "hbm",
+            "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK,
+            "traffic": traffic.get("bytes_per_launch") if traffic else None,
+            "traffic_source": (f"profiled constant: rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel, "
+                               f"{traffic.get('measured', '?')} at commit {traffic.get('commit', '?')} "
+                               f"(profiles/eam_force_traffic.json)") if traffic else None,
+            "limiter": "VALU issue and gather latency, not HBM (the list and neighbor records are L2-served)",
+            "valu_busy": issue.get("valu_busy") if issue else None,
+            "valu_busy_source": (f"profiled constant: SQ_ACTIVE_INST_VALU / SQ_BUSY_CU_CYCLES, "
+                                 f"{issue.get('measured', '?')} at commit {issue.get('commit', '?')} "
+                                 f"(profiles/eam_force_issue.json)") if issue else None,
             "avg_launch_ms": avg_s * 1e3, "launches": c_f,
             "algorithmic_bytes_per_launch": PASS2_BYTES_PER_ATOM * n,
             "list_inclusive_bytes_per_launch": PASS2_BYTES_PER_ATOM * n + 4 * nfull,
             "whole_step": {"algorithmic_bytes_per_atom_step": STEP_BYTES_PER_ATOM,
                            "achieved_GBps": value * STEP_BYTES_PER_ATOM / 1e9,
                            "frac": value * STEP_BYTES_PER_ATOM / HBM_PEAK}}
+    fp64 = None
+    if not args.no_cpu:
+        log("[bench] FP64 oracle check of the final state")
+        fp64 = headline_error(md, p, H, elem, host_threads())
+        log(f"[bench] max|df| {fp64['max_abs_df_eV_per_A']:.2e} eV/A, |dE|/N {fp64['abs_dE_per_atom_eV']:.1e} eV "
+            f"({fp64['seconds']:.1f} s)")
+    del md
     cpu = None
     if not args.no_cpu:
         log(f"[bench] cpu baseline (~{args.cpu_seconds:.0f} s of oracle work)")
@@ -759,6 +824,7 @@ def main():
         "config": {"workload": f"EAM NVT, fcc Cu {args.cells}^3 cells ({n} atoms), Johnson setfl Nr=Nrho=10000, "
                                f"r_c=4.95 A, skin 0.5 A, dt 1 fs, 300 K", "natoms": n, "rebuilds": nvt.rebuilds,
                    "pe_per_atom_eV": out[0] / n, "T_K": nvt.T_part},
+        "error_vs_fp64": fp64,
         "roofline": roof, "cpu_baseline": cpu, "feni": feni, "minimize": mini, "dmd": dmd, "gcmc": gcmc,
     }
     print(json.dumps(line), flush=True)

@@ -73,6 +73,14 @@ mapp_status mapp_md_set_x(mapp_md* md, const double* x);
 mapp_status mapp_md_get_x(mapp_md* md, double* x);
 mapp_status mapp_md_set_v(mapp_md* md, const double* v);
 mapp_status mapp_md_get_v(mapp_md* md, double* v);
+/* A new box, as the reference's callers change atoms->H (src/atoms.cpp:187-194:
+ * H and B = H^-1) when they strain the system or integrate the box (md_nst,
+ * the minimisers' H degrees of freedom), with the positions x in caller order
+ * (NULL keeps the Cartesian positions). Rebuilds the cell grid, the image
+ * shifts and the neighbor list; the next force_calc is that of the new box.
+ * A box narrower than half the cut-off + skin leaves the system unchanged
+ * (MAPP_ERR_CAPACITY). */
+mapp_status mapp_md_set_box(mapp_md* md, const double* H, const double* x);
 /* DynamicMD::update<true> (src/dynamic_md.h:38-68, decide at
  * src/dynamic_md.cpp:90-105): rebuilds when an atom moved more than skin/2. */
 mapp_status mapp_md_update(mapp_md* md, int* rebuilt);
@@ -192,6 +200,18 @@ mapp_status mapp_dmd_create(mapp_eam* ff, int nsites, int c_dim, int nquad, cons
 void mapp_dmd_destroy(mapp_dmd* d);
 mapp_status mapp_dmd_set_state(mapp_dmd* d, const double* x, const double* alpha, const double* c);
 mapp_status mapp_dmd_get_state(mapp_dmd* d, double* x, double* alpha, double* c);
+/* A new box for the DMD system (atoms->H and B, src/atoms.cpp:187-194), with
+ * the positions x in caller order (NULL keeps the Cartesian positions; alpha
+ * and c are kept). Rebuilds the grid and the component lists. A box narrower
+ * than half the list cut-off leaves the system unchanged (MAPP_ERR_CAPACITY). */
+mapp_status mapp_dmd_set_box(mapp_dmd* d, const double* H, const double* x);
+/* ForceFieldDMD::force_calc_static (src/ff_dmd.cpp:429-445) with
+ * ForceFieldEAMDMD::__force_calc_static (src/ff_eam_dmd.cpp:1284-1340): the
+ * free energy, F_H (Voigt) and entropy sum (out8 as mapp_dmd_derivative) of
+ * the state the last mapp_dmd_c_d call evaluated, without new passes (as the
+ * reference re-sums what calc_mu left); forces are not recomputed.
+ * MAPP_ERR_STATE if x, alpha or c changed since that call. */
+mapp_status mapp_dmd_force_calc_static(mapp_dmd* d, double* out8);
 /* DynamicDMD::update (decide at src/dynamic_dmd.cpp:103-126) */
 mapp_status mapp_dmd_update(mapp_dmd* d, int* rebuilt);
 /* ForceFieldDMD::new_derivative<with_c> (src/ff_dmd.cpp:285-366) over
@@ -208,6 +228,15 @@ mapp_status mapp_dmd_derivative(mapp_dmd* d, int with_c, const double* ext_mu, d
  * of the quadrature passes, 1 (one pair at a time), 2 or 4 (default). Any
  * value gives the same results to rounding. */
 mapp_status mapp_dmd_set_batch(mapp_dmd* d, int batch);
+/* Precision of the quadrature passes behind mapp_dmd_derivative,
+ * mapp_dmd_value and the DMD minimisers (ForceFieldEAMDMD::__force_calc /
+ * __energy_calc, src/ff_eam_dmd.cpp:126-360,835-953; no such switch in the
+ * reference, which is FP64 throughout): on != 0 evaluates the pair splines
+ * from FP32 copies of the table rows, the quadrature moments and every sum
+ * staying FP64 (batches 2 and 4; batch 1 stays FP64); 0 (the default) reads
+ * the FP64 rows. The concentration-rate path (mapp_dmd_c_d, mapp_dmd_J,
+ * mapp_dmd_newton, mapp_dmd_bdf_run) always reads the FP64 rows. */
+mapp_status mapp_dmd_set_fp32(mapp_dmd* d, int on);
 /* ForceFieldDMD::c_d_calc (src/ff_dmd.cpp:449-456) with
  * ForceFieldEAMDMD::__c_d_calc (src/ff_eam_dmd.cpp:1359-1473): the
  * concentration rates dc/dt of every component at the current state, over the
@@ -412,6 +441,26 @@ mapp_status mapp_gcmc_set_waves(mapp_gcmc* g, const int32_t* waves);
 /* on != 0: the attempt launch is captured once in a HIP graph and replayed */
 mapp_status mapp_gcmc_use_graph(mapp_gcmc* g, int on);
 mapp_status mapp_gcmc_counts(mapp_gcmc* g, int32_t* natms, int32_t* ngas);
+/* The exchange hooks of one trial, for a caller that keeps its own attempt
+ * loop and random stream (SGCMC::attmpt, src/sgcmc.cpp:255-327, or PGCMC):
+ * mapp_gcmc_xchng_energy is ForceFieldEAM::pre_xchng_energy + xchng_energy
+ * (src/ff_eam.cpp:323-446) on cell `cell`: mode 1 an insertion of the gas
+ * element at fractional s[3] (in [0, 1); the reference's s_buff), mode 2 the
+ * deletion of the igas-th gas atom in atom order (the reference's igas);
+ * *dU is the exchange energy in FP64. The trial is kept until the next
+ * call. mapp_gcmc_xchng_commit applies it as accepted:
+ * ForceFieldEAM::post_xchng_energy (src/ff_eam.cpp:451-483) with
+ * SGCMC::ins_succ / del_succ (src/sgcmc.cpp:331-444) -- densities and
+ * embedding energies of the touched atoms, the new atom appended with
+ * velocity v[3] (the reference's gaussian() * sigma) or the last atom moved
+ * into the deleted one's place. A rejected trial is simply not committed.
+ * init_xchng (src/ff_eam.cpp:509-578) runs when the cells are created.
+ * MAPP_ERR_STATE when there is no trial to commit (none made, already
+ * committed, or mapp_gcmc_attempts ran since); MAPP_ERR_CAPACITY when an
+ * insertion finds the cell full; MAPP_ERR_ARG when igas is not below the gas
+ * count. */
+mapp_status mapp_gcmc_xchng_energy(mapp_gcmc* g, int cell, int mode, const double* s, int igas, double* dU);
+mapp_status mapp_gcmc_xchng_commit(mapp_gcmc* g, const double* v);
 /* one cell's atoms in the reference's order (appended on insertion, last
  * atom moved into the hole on deletion): Cartesian x [n][3], elem, rho, F
  * (the exchange bookkeeping of ForceFieldEAM), v, and its stream's seed */

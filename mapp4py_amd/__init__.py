@@ -137,6 +137,14 @@ class AtomsMD:
         check(lib().mapp_md_update(self._h, C.byref(r)))
         return bool(r.value)
 
+    def set_box(self, H, x=None):
+        """a new box H (lower triangular, rows = box vectors) and optionally
+        new positions; the grid and the neighbor list are rebuilt"""
+        Hn = f64(H).reshape(3, 3)
+        xv = None if x is None else f64(x).reshape(self.natms, 3)
+        check(lib().mapp_md_set_box(self._h, ptr(Hn), ptr(xv) if xv is not None else None))
+        self.H = Hn
+
     def force_calc(self, want_f=True):
         """Returns (f [N][3], energy, F_H Voigt[6])."""
         f = np.zeros((self.natms, 3)) if want_f else None
@@ -468,6 +476,26 @@ class AtomsDMD:
         """List entries per batch of the quadrature passes (1, 2 or 4)."""
         check(lib().mapp_dmd_set_batch(self._h, int(b)))
 
+    def set_box(self, H, x=None):
+        """a new box H and optionally new positions (alpha and c kept); the
+        grid and the component lists are rebuilt"""
+        Hn = f64(H).reshape(3, 3)
+        xv = None if x is None else f64(x).reshape(self.nsites, 3)
+        check(lib().mapp_dmd_set_box(self._h, ptr(Hn), ptr(xv) if xv is not None else None))
+        self.H = Hn
+
+    def force_calc_static(self):
+        """ForceFieldDMD::force_calc_static: (fe, F_H[6], S) of the state of the
+        last c_d call, without new passes"""
+        out = np.zeros(8)
+        check(lib().mapp_dmd_force_calc_static(self._h, ptr(out)))
+        return out[0], out[1:7].copy(), out[7]
+
+    def set_fp32(self, on=True):
+        """FP32 table rows (FP64 moments and sums) in the free energy and its
+        gradient; the concentration-rate path stays FP64"""
+        check(lib().mapp_dmd_set_fp32(self._h, int(bool(on))))
+
     def value(self, with_c=False, ext_mu=None):
         e = C.c_double()
         em = None if ext_mu is None else f64(ext_mu)
@@ -652,6 +680,25 @@ class GCMCBatch:
 
     def use_graph(self, on=True):
         check(lib().mapp_gcmc_use_graph(self._h, int(on)))
+
+    def xchng_energy(self, cell, s=None, igas=None):
+        """ForceFieldEAM::pre_xchng_energy + xchng_energy of one trial in one
+        cell: an insertion of the gas element at fractional s, or (igas given)
+        the deletion of the igas-th gas atom. Returns the exchange energy; the
+        trial is kept for xchng_commit."""
+        du = C.c_double()
+        if igas is None:
+            sv = f64(s).reshape(3)
+            check(lib().mapp_gcmc_xchng_energy(self._h, int(cell), 1, ptr(sv), 0, C.byref(du)))
+        else:
+            check(lib().mapp_gcmc_xchng_energy(self._h, int(cell), 2, None, int(igas), C.byref(du)))
+        return du.value
+
+    def xchng_commit(self, v=None):
+        """the kept trial accepted: post_xchng_energy after ins_succ / del_succ
+        (v: the inserted atom's velocity)"""
+        vv = None if v is None else f64(v).reshape(3)
+        check(lib().mapp_gcmc_xchng_commit(self._h, ptr(vv) if vv is not None else None))
 
     def counts(self):
         n = np.zeros(self.ncells, np.int32)

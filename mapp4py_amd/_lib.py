@@ -43,6 +43,7 @@ SIGNATURES = {
     "mapp_md_get_x": (C.c_int, [C.c_void_p, dp]),
     "mapp_md_set_v": (C.c_int, [C.c_void_p, dp]),
     "mapp_md_get_v": (C.c_int, [C.c_void_p, dp]),
+    "mapp_md_set_box": (C.c_int, [C.c_void_p, dp, dp]),
     "mapp_md_update": (C.c_int, [C.c_void_p, ip]),
     "mapp_md_force_calc": (C.c_int, [C.c_void_p, dp, dp]),
     "mapp_md_value": (C.c_int, [C.c_void_p, dp]),
@@ -62,6 +63,9 @@ SIGNATURES = {
     "mapp_md_set_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mapp_md_set_lds": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "mapp_md_lds_window": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "mapp_dmd_set_box": (C.c_int, [C.c_void_p, dp, dp]),
+    "mapp_dmd_force_calc_static": (C.c_int, [C.c_void_p, dp]),
+    "mapp_dmd_set_fp32": (C.c_int, [C.c_void_p, C.c_int]),
     "mapp_dmd_set_batch": (C.c_int, [C.c_void_p, C.c_int]),
     "mapp_dmd_c_d": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                C.POINTER(C.c_double)]),
@@ -115,6 +119,8 @@ SIGNATURES = {
     "mapp_gcmc_tune": (C.c_int, [C.c_void_p, C.c_int, i32p, dp]),
     "mapp_gcmc_set_waves": (C.c_int, [C.c_void_p, i32p]),
     "mapp_gcmc_use_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "mapp_gcmc_xchng_energy": (C.c_int, [C.c_void_p, C.c_int, C.c_int, dp, C.c_int, dp]),
+    "mapp_gcmc_xchng_commit": (C.c_int, [C.c_void_p, dp]),
     "mapp_gcmc_counts": (C.c_int, [C.c_void_p, i32p, i32p]),
     "mapp_gcmc_get_cell": (C.c_int, [C.c_void_p, C.c_int, dp, u8p, dp, dp, dp, i32p]),
     "mapp_gcmc_profile": (C.c_int, [C.c_void_p, C.c_int]),

@@ -184,6 +184,15 @@ mapp_status mapp_md_get_v(mapp_md* md, double* v)
     return guard([&] { NEED(md); NEED(v); md->sys->get_v(v); });
 }
 
+mapp_status mapp_md_set_box(mapp_md* md, const double* H, const double* x)
+{
+    return guard([&] {
+        NEED(md);
+        NEED(H);
+        md->sys->set_box(H, x);
+    });
+}
+
 mapp_status mapp_md_update(mapp_md* md, int* rebuilt)
 {
     return guard([&] {
@@ -441,6 +450,32 @@ mapp_status mapp_dmd_derivative(mapp_dmd* d, int with_c, const double* ext_mu, d
         }
         if (f || f_alpha || mu) d->sys->get_forces(f, f_alpha, mu);
     });
+}
+
+mapp_status mapp_dmd_set_box(mapp_dmd* d, const double* H, const double* x)
+{
+    return guard([&] {
+        NEED(d);
+        NEED(H);
+        d->sys->set_box(H, x);
+    });
+}
+
+mapp_status mapp_dmd_force_calc_static(mapp_dmd* d, double* out8)
+{
+    return guard([&] {
+        NEED(d);
+        NEED(out8);
+        const mapp::DMDReport r = d->sys->force_calc_static();
+        out8[0] = r.fe;
+        for (int k = 0; k < 6; ++k) out8[1 + k] = r.FH[k];
+        out8[7] = r.s;
+    });
+}
+
+mapp_status mapp_dmd_set_fp32(mapp_dmd* d, int on)
+{
+    return guard([&] { NEED(d); d->sys->set_f32(on != 0); });
 }
 
 mapp_status mapp_dmd_set_batch(mapp_dmd* d, int batch)
@@ -814,6 +849,23 @@ mapp_status mapp_gcmc_set_waves(mapp_gcmc* g, const int32_t* waves)
 mapp_status mapp_gcmc_use_graph(mapp_gcmc* g, int on)
 {
     return guard([&] { NEED(g); g->sys->use_graph(on != 0); });
+}
+
+mapp_status mapp_gcmc_xchng_energy(mapp_gcmc* g, int cell, int mode, const double* s, int igas, double* dU)
+{
+    return guard([&] {
+        NEED(g);
+        NEED(dU);
+        *dU = g->sys->xchng_energy(cell, mode, s, igas);
+    });
+}
+
+mapp_status mapp_gcmc_xchng_commit(mapp_gcmc* g, const double* v)
+{
+    return guard([&] {
+        NEED(g);
+        g->sys->xchng_commit(v);
+    });
 }
 
 mapp_status mapp_gcmc_counts(mapp_gcmc* g, int32_t* natms, int32_t* ngas)

@@ -10,6 +10,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 
@@ -207,6 +208,25 @@ __device__ inline PairGeo pair_geo(const double4& xi, const double4& xj, uint32_
 }
 
 __device__ inline double hv(const double4& c, double p) { return ((c.w * p + c.z) * p + c.y) * p + c.x; }
+// an FP32 row: the cubic in FP32 at the FP32-rounded offset, returned as FP64
+// for the FP64 moment sums
+__device__ inline double hv(const float4& c, double p)
+{
+    const float q = (float)p;
+    return (double)(((c.w * q + c.z) * q + c.y) * q + c.x);
+}
+
+// the quadrature passes' table rows: FP64 (parity mode) or FP32
+template <bool F32>
+using DRow = typename std::conditional<F32, float4, double4>::type;
+template <bool F32>
+__device__ inline const DRow<F32>* drows(const DmdArgs& A)
+{
+    if constexpr (F32)
+        return A.tabf;
+    else
+        return A.tab;
+}
 
 // Smeared r*f over the odd extension, N-point Gauss-Hermite
 // (src/ff_eam_dmd.cpp:195-253): rp = g, dr = (1/r) dg/dr, da = (1/a) dg/da.
@@ -439,7 +459,7 @@ __device__ inline int dknot(const DmdArgs& A, double u, double& p)
 
 // pass 1: rho of every component (smeared r*rho_ba) and the pair energy
 // (smeared r*phi); only table functions 0 and 2 are read
-template <int B>
+template <int B, bool F32>
 __global__ __launch_bounds__(kBlock) void k_dmd_pass1q(int n, const double4* __restrict__ x, const double* __restrict__ al,
                                                        const double* __restrict__ cc, const int* __restrict__ el,
                                                        const int* __restrict__ nn, const uint32_t* __restrict__ list,
@@ -488,9 +508,10 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass1q(int n, const double4* __r
                 double H0[B], H2[B];
 #pragma unroll
                 for (int b = 0; b < B; ++b) H0[b] = H2[b] = 0.0;
+                const DRow<F32>* T = drows<F32>(A);
                 for (int l = 0; l < A.N; ++l) {
                     const double xl = A.xi[l], wl = A.wi[l];
-                    double4 c0[B], c2[B];
+                    DRow<F32> c0[B], c2[B];
                     double p[B], s[B];
 #pragma unroll
                     for (int b = 0; b < B; ++b) {
@@ -498,8 +519,8 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass1q(int n, const double4* __r
                         const int m = dknot(A, u, p[b]);
                         s[b] = u < 0.0 ? -wl : wl;
                         const size_t row = live[b] ? ((size_t)tt[b] * A.nr + m) * 3 : 0;
-                        c0[b] = A.tab[row];
-                        c2[b] = A.tab[row + 2];
+                        c0[b] = T[row];
+                        c2[b] = T[row + 2];
                     }
 #pragma unroll
                     for (int b = 0; b < B; ++b) {
@@ -533,7 +554,7 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass1q(int n, const double4* __r
 // leaves in f and f_alpha (src/ff_eam_dmd.cpp:1189-1278): every pair weighted
 // by c_j and the component's fcoef (src/ff_eam_dmd.cpp:995-1046) instead of
 // c_i c_j, and d/dalpha with -3 kT / alpha, defined also where c_i = 0.
-template <int B, bool UNIT>
+template <int B, bool UNIT, bool F32>
 __global__ __launch_bounds__(kBlock) void k_dmd_pass2q(int n, const double4* __restrict__ x, const double* __restrict__ al,
                                                        const double* __restrict__ cc, const int* __restrict__ el,
                                                        const int* __restrict__ nn, const uint32_t* __restrict__ list,
@@ -606,9 +627,10 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass2q(int n, const double4* __r
                 for (int b = 0; b < B; ++b)
 #pragma unroll
                     for (int k = 0; k < 3; ++k) H[b][k][0] = H[b][k][1] = H[b][k][2] = 0.0;
+                const DRow<F32>* T = drows<F32>(A);
                 for (int l = 0; l < A.N; ++l) {
                     const double xl = A.xi[l], wl = A.wi[l];
-                    double4 c[B][3];
+                    DRow<F32> c[B][3];
                     double p[B], s[B];
 #pragma unroll
                     for (int b = 0; b < B; ++b) {
@@ -617,7 +639,7 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass2q(int n, const double4* __r
                         s[b] = u < 0.0 ? -wl : wl;
                         const size_t row = live[b] ? ((size_t)tt[b] * A.nr + m) * 3 : 0;
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) c[b][k] = A.tab[row + k];
+                        for (int k = 0; k < 3; ++k) c[b][k] = T[row + k];
                     }
 #pragma unroll
                     for (int b = 0; b < B; ++b)
@@ -1607,6 +1629,12 @@ DMDSystem::DMDSystem(std::shared_ptr<EamForceField> ff, int nsites, int c_dim, i
                 }
         tab_.reserve(T4.size());
         MAPP_HIP(hipMemcpy(tab_.get(), T4.data(), T4.size() * sizeof(double4), hipMemcpyHostToDevice));
+        std::vector<float4> Tf(T4.size());
+        for (size_t k = 0; k < T4.size(); ++k)
+            Tf[k] = make_float4((float)T4[k].x, (float)T4[k].y, (float)T4[k].z, (float)T4[k].w);
+        tabf_.reserve(Tf.size());
+        MAPP_HIP(hipMemcpy(tabf_.get(), Tf.data(), Tf.size() * sizeof(float4), hipMemcpyHostToDevice));
+        A_.tabf = tabf_.get();
         F5_.reserve(t.F.size());
         MAPP_HIP(hipMemcpy(F5_.get(), t.F.data(), t.F.size() * sizeof(double), hipMemcpyHostToDevice));
         A_.tab = tab_.get();
@@ -1851,7 +1879,7 @@ void DMDSystem::stage_surrogate()
     prof_.end(stream_);
 }
 
-void DMDSystem::pass1(bool energy_only)
+void DMDSystem::pass1(bool energy_only, bool f32)
 {
     const int nb = grid_for(n_);
     BoxDev bd;
@@ -1871,19 +1899,22 @@ void DMDSystem::pass1(bool energy_only)
         hipLaunchKernelGGL(k_dmd_pass1<false>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
                            el_.get(), nn_.get(), list_.get(), ldl_, A_, bd, S, energy_only ? 1 : 0, E_.get(),
                            dE_.get(), part_.get());
-    else if (batch_ == 2)
-        hipLaunchKernelGGL(k_dmd_pass1q<2>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
-                           el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, energy_only ? 1 : 0, E_.get(),
-                           dE_.get(), part_.get());
-    else
-        hipLaunchKernelGGL(k_dmd_pass1q<4>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
-                           el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, energy_only ? 1 : 0, E_.get(),
-                           dE_.get(), part_.get());
+    else {
+#define MAPP_P1(B, F) hipLaunchKernelGGL((k_dmd_pass1q<B, F>), dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), \
+                                         al_.get(), c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, \
+                                         energy_only ? 1 : 0, E_.get(), dE_.get(), part_.get())
+        if (batch_ == 2) {
+            if (f32) MAPP_P1(2, true); else MAPP_P1(2, false);
+        } else {
+            if (f32) MAPP_P1(4, true); else MAPP_P1(4, false);
+        }
+#undef MAPP_P1
+    }
     prof_.end(stream_);
     reduce_rows(part_.get(), nb, 2, red_.get());
 }
 
-void DMDSystem::pass2(bool unit)
+void DMDSystem::pass2(bool unit, bool f32)
 {
     const int nb = grid_for(n_);
     BoxDev bd;
@@ -1901,7 +1932,7 @@ void DMDSystem::pass2(bool unit)
         gu_.reserve(n_);
         gau_.reserve(nc_);
         j_reserve();
-        hipLaunchKernelGGL((k_dmd_pass2q<4, true>), dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(),
+        hipLaunchKernelGGL((k_dmd_pass2q<4, true, false>), dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(),
                            c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, E_.get(), dE_.get(),
                            f_.get(), fa_.get(), mu_.get(), p2, gu_.get(), gau_.get(), jstate());
         gu_ver_ = ver_;
@@ -1910,17 +1941,23 @@ void DMDSystem::pass2(bool unit)
         hipLaunchKernelGGL(k_dmd_pass2<false>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
                            el_.get(), nn_.get(), list_.get(), ldl_, A_, bd, S, E_.get(), dE_.get(), f_.get(),
                            fa_.get(), mu_.get(), p2);
-    else if (batch_ == 2)
-        hipLaunchKernelGGL((k_dmd_pass2q<2, false>), dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(),
-                           c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, E_.get(), dE_.get(),
-                           f_.get(), fa_.get(), mu_.get(), p2, nullptr, nullptr, JState{});
-    else
-        hipLaunchKernelGGL((k_dmd_pass2q<4, false>), dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(),
-                           c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, E_.get(), dE_.get(),
-                           f_.get(), fa_.get(), mu_.get(), p2, nullptr, nullptr, JState{});
+    else {
+#define MAPP_P2(B, F) hipLaunchKernelGGL((k_dmd_pass2q<B, false, F>), dim3(nb), dim3(kBlock), 0, stream_, n_, \
+                                         x_.get(), al_.get(), c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, maxn_, \
+                                         A_, bd, E_.get(), dE_.get(), f_.get(), fa_.get(), mu_.get(), p2, nullptr, \
+                                         nullptr, JState{})
+        if (batch_ == 2) {
+            if (f32) MAPP_P2(2, true); else MAPP_P2(2, false);
+        } else {
+            if (f32) MAPP_P2(4, true); else MAPP_P2(4, false);
+        }
+#undef MAPP_P2
+    }
     prof_.end(stream_);
     reduce_rows(p2, nb, 6, red_.get() + 8);
 }
+
+void DMDSystem::set_f32(bool on) { f32_ = on; }
 
 void DMDSystem::set_batch(int b)
 {
@@ -1944,6 +1981,10 @@ double DMDSystem::c_d(const double* r_crd, const double* zeta, double* c_d_host)
     // calc_mu: rho, E, dE, the raw chemical potentials and the gradient per unit c
     pass1(false);
     pass2(!surr_);
+    // the free energy, entropy and virial sums of these passes, for force_calc_static
+    static_.reserve(14);
+    MAPP_HIP(hipMemcpyAsync(static_.get(), red_.get(), sizeof(double) * 14, hipMemcpyDeviceToDevice, stream_));
+    static_ver_ = ver_;
     // what J reads later at this state: the parameters, mu and dE (the
     // slot values and F'' follow on the first J call)
     jmu_.reserve(nc_);
@@ -2223,8 +2264,8 @@ bool DMDSystem::newton(const double* r_crd, const double* zeta, double beta, con
 
 DMDReport DMDSystem::derivative(bool with_c, const double* ext_mu)
 {
-    pass1(false);
-    pass2();
+    pass1(false, f32_);
+    pass2(false, f32_);
     if (with_c) {
         MAPP_REQUIRE(ext_mu != nullptr, ERR_ARG, "ext_mu is required with c degrees of freedom");
         DevBuf<double> em;
@@ -2250,7 +2291,7 @@ DMDReport DMDSystem::derivative(bool with_c, const double* ext_mu)
 
 double DMDSystem::value(bool with_c, const double* ext_mu)
 {
-    pass1(true);
+    pass1(true, f32_);
     double h[2];
     MAPP_HIP(hipMemcpyAsync(h, red_.get(), sizeof(double) * 2, hipMemcpyDeviceToHost, stream_));
     MAPP_HIP(hipStreamSynchronize(stream_));
@@ -2283,6 +2324,48 @@ void DMDSystem::set_state(const double* x, const double* alpha, const double* c)
     hipLaunchKernelGGL(k_dscatter, dim3((n_ + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, n_, C_, tx.get(),
                        ta.get(), tc.get(), slot_of_.get(), x_.get(), al_.get(), c_.get());
     MAPP_HIP(hipStreamSynchronize(stream_));
+}
+
+// A new box and, unless x is null, new positions in caller order (alpha and c
+// kept): the grid, then a rebuild. A box the grid cannot hold leaves the
+// system as it was.
+void DMDSystem::set_box(const double* H, const double* x)
+{
+    for (int i = 0; i < 3; ++i)
+        MAPP_REQUIRE(H[i * 3 + i] > 0.0, ERR_ARG, "H must be lower triangular with a positive diagonal");
+    MAPP_REQUIRE(H[1] == 0.0 && H[2] == 0.0 && H[5] == 0.0, ERR_ARG, "H must be lower triangular");
+    const Box old = box_;
+    box_ = make_box(H);
+    try {
+        setup_grid();
+    } catch (...) {
+        box_ = old;
+        setup_grid();
+        throw;
+    }
+    if (x) {
+        std::vector<double> xo((size_t)3 * n_), a(nc_), c(nc_);
+        get_state(xo.data(), a.data(), c.data());
+        set_state(x, a.data(), c.data());
+    }
+    rebuild();
+}
+
+// ForceFieldDMD::force_calc_static (src/ff_dmd.cpp:429-445,
+// src/ff_eam_dmd.cpp:1284-1340): the free energy, virial and entropy of the
+// state the last c_d left, from the sums its passes already formed
+DMDReport DMDSystem::force_calc_static()
+{
+    MAPP_REQUIRE(static_ver_ >= 0 && static_ver_ == ver_, ERR_STATE,
+                 "force_calc_static reads the state of the last c_d call; x, alpha or c changed since");
+    double h[14];
+    MAPP_HIP(hipMemcpyAsync(h, static_.get(), sizeof(double) * 14, hipMemcpyDeviceToHost, stream_));
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    DMDReport r{};
+    r.s = h[1];
+    r.fe = h[0] + A_.kBT * h[1];
+    for (int k = 0; k < 6; ++k) r.FH[k] = h[8 + k];
+    return r;
 }
 
 void DMDSystem::export_cfg(const char* path, int nelems, const double* masses, const std::vector<std::string>& names)

@@ -22,6 +22,7 @@ struct DmdArgs {
     double cut[kMaxElems * kMaxElems];  // r_c (no skin)
     double mu0[kMaxElems];
     const double4* tab;  // [ne*ne][nr][3]: r*phi[a][b], r*rho[a][b], r*rho[b][a] (FP64)
+    const float4* tabf;  // the same rows rounded to FP32 (the FP32-row passes)
     const double* F5;    // [ne][nrho][5], quartic embedding
 };
 
@@ -68,6 +69,15 @@ public:
               double hP, const double* masses);
     // list entries per batch of the quadrature passes (1 = one pair at a time)
     void set_batch(int b);
+    // a new box and, unless x is null, new positions; rebuilds the grid and lists
+    void set_box(const double* H, const double* x);
+    // free energy, F_H and entropy of the state of the last c_d (no new passes)
+    DMDReport force_calc_static();
+    // free energy and gradient (derivative, value, the minimisers) from FP32
+    // table rows with FP64 moments and sums; the concentration-rate path
+    // (c_d, J, the implicit and BDF steps) always reads the FP64 rows
+    void set_f32(bool on);
+    bool f32() const { return f32_; }
     ~DMDSystem();
 
     int nsites() const { return n_; }
@@ -132,8 +142,8 @@ public:
 private:
     void setup_grid();
     void build_list();
-    void pass1(bool energy_only);
-    void pass2(bool unit = false);
+    void pass1(bool energy_only, bool f32 = false);
+    void pass2(bool unit = false, bool f32 = false);
     bool decide();
     void reduce_rows(const double* part, int nrows, int ncols, double* out);
     void J_prepare();
@@ -163,6 +173,8 @@ private:
     DevBuf<double4> gu_;   // gradient per unit c of the last c_d, per site
     DevBuf<double> gau_;   // and per component (alpha)
     int64_t gu_ver_ = -1;
+    int64_t static_ver_ = -1;        // state of the sums in static_ (the last c_d)
+    DevBuf<double> static_;          // free energy, entropy and virial sums of the last c_d
     DevBuf<double> jmu_, jdE_, jddE_;
     DevBuf<uint32_t> jidx_, jidx2_;   // list entries / second list, slot-major
     DevBuf<double> jr2_, jr1_, jt0_, jf0_, jf1_, jf2_, jS_;
@@ -181,9 +193,11 @@ private:
     DevBuf<uint32_t> list_;
     int maxn_ = 0, ldl_ = 0;
     int batch_ = 4;                  // quadrature pass batch: 1 (per-pair loop), 2 or 4 list entries
+    bool f32_ = false;               // FP32 table rows in derivative / value
     DevBuf<int> flags_;
     DevBuf<double> part_, red_, red2_;
     DevBuf<double4> tab_;
+    DevBuf<float4> tabf_;
     DevBuf<double> F5_;
     // CG work vectors (x part as double4 slots, alpha part per component)
     DevBuf<double4> cg_x0_, cg_h_, cg_f0_;

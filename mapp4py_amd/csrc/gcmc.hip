@@ -485,6 +485,180 @@ __global__ __launch_bounds__(256) void k_gcmc_init(GcmcArgs A, const GcmcCell* _
     }
 }
 
+// ---- single-trial hooks (one cell, one trial, driven by the caller) ----
+// k_gcmc_xtrial: the exchange energy of one trial in cell c -- an insertion
+// at s (mode MODE_INS) or the deletion of the igas-th gas atom in index order
+// (MODE_DEL) -- computed as run_cell computes it with four waves (each thread
+// takes atoms t, t + 256, ..., the same sums in the same order). rho_x / F_x
+// of every atom of the cell take the trial's values (untouched atoms keep
+// theirs). out: du, trial density, deletion index, status (0 ok, 2 no such
+// gas atom, 3 too many images), trial position x (image 0), s.
+__global__ __launch_bounds__(256) void k_gcmc_xtrial(GcmcArgs A, const GcmcCell* __restrict__ cells, int c, int mode,
+                                                     double s0, double s1, double s2, int igas, Arrays R,
+                                                     double* __restrict__ out)
+{
+    __shared__ double trial[3 * kGcmcMaxTrials];
+    __shared__ double red[12];
+    __shared__ double sb[3];
+    __shared__ int sh_del;
+    const GcmcCell hd = cells[c];
+    const int off = hd.off, n = hd.n, gas = A.gas, ne = A.ne, nr = A.nr;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t == 0) {
+        int del = -1;
+        if (mode == MODE_DEL)
+            for (int j = 0, seen = 0; j < n; ++j)
+                if (R.el[off + j] == gas) {
+                    if (seen == igas) {
+                        del = j;
+                        break;
+                    }
+                    ++seen;
+                }
+        sh_del = del;
+        if (mode == MODE_DEL && del >= 0) {
+            const double4 s4 = R.s[off + del];
+            sb[0] = s4.x;
+            sb[1] = s4.y;
+            sb[2] = s4.z;
+        } else {
+            sb[0] = s0;
+            sb[1] = s1;
+            sb[2] = s2;
+        }
+    }
+    __syncthreads();
+    const int del_idx = sh_del;
+    const double sv[3] = {sb[0], sb[1], sb[2]};
+    const Images im = count_images(sv, hd.lo, hd.hi);
+    const int ntrial = im.total();
+    // uniform exits: every thread reads the same shared values
+    if ((mode == MODE_DEL && del_idx < 0) || ntrial > kGcmcMaxTrials) {
+        if (t == 0) {
+            out[0] = 0.0;
+            out[3] = mode == MODE_DEL && del_idx < 0 ? 2.0 : 3.0;
+        }
+        return;
+    }
+    if (t < ntrial) image_x(hd.h, sv, im, t, trial + 3 * t);
+    __syncthreads();
+    const double c0 = mode == MODE_DEL ? -1.0 : 1.0;
+    double en = 0.0, en0 = 0.0, rhoi = 0.0;
+    for (int j = t; j < n; j += 256) {
+        const double4 xj = R.xe[off + j];
+        const int ej = (int)xj.w;
+        const double cut = A.cut_sq[gas * ne + ej];
+        const double4* tp = A.phi + (size_t)(gas * ne + ej) * nr;
+        const double4* tr_ij = A.rho + (size_t)(gas * ne + ej) * nr;
+        const double4* tr_ji = A.rho + (size_t)(ej * ne + gas) * nr;
+        bool hit = false;
+        double rx = 0.0;
+        for (int k = 0; k < ntrial; ++k) {
+            const double rsq = rsq_of(trial[3 * k], trial[3 * k + 1], trial[3 * k + 2], xj.x, xj.y, xj.z);
+            if (rsq < cut && !(k == 0 && j == del_idx)) {
+                const double r = sqrt(rsq);
+                int m;
+                double p;
+                spline_at(r, A.dr_inv, nr, m, p);
+                en += hval(tp[m], p) / r;
+                if (!hit) {
+                    rx = R.rho[off + j];
+                    hit = true;
+                }
+                rx += c0 * hval(tr_ij[m], p);
+                rhoi += hval(tr_ji[m], p);
+            }
+        }
+        if (hit) {
+            const double fx = f_embed(A, ej, rx, true);
+            en0 += fx - R.F[off + j];
+            R.rho_x[off + j] = rx;
+            R.F_x[off + j] = fx;
+        } else {
+            R.rho_x[off + j] = R.rho[off + j];
+            R.F_x[off + j] = R.F[off + j];
+        }
+    }
+    // the trial point's own images, half of each pair (wave 0, as run_cell)
+    if (w == 0 && lane >= 1 && lane < ntrial) {
+        const double rsq = rsq_of(trial[0], trial[1], trial[2], trial[3 * lane], trial[3 * lane + 1], trial[3 * lane + 2]);
+        if (rsq < A.cut_sq[gas * ne + gas]) {
+            const double r = sqrt(rsq);
+            int m;
+            double p;
+            spline_at(r, A.dr_inv, nr, m, p);
+            en += 0.5 * hval(A.phi[(size_t)(gas * ne + gas) * nr + m], p) / r;
+            rhoi += hval(A.rho[(size_t)(gas * ne + gas) * nr + m], p);
+        }
+    }
+    en = wave_sum(en);
+    en0 = wave_sum(en0);
+    rhoi = wave_sum(rhoi);
+    if (lane == 0) {
+        red[3 * w] = en;
+        red[3 * w + 1] = en0;
+        red[3 * w + 2] = rhoi;
+    }
+    __syncthreads();
+    if (t == 0) {
+        en = en0 = rhoi = 0.0;
+        for (int q = 0; q < 4; ++q) {
+            en += red[3 * q];
+            en0 += red[3 * q + 1];
+            rhoi += red[3 * q + 2];
+        }
+        out[0] = en + en0 * c0 + f_embed(A, gas, rhoi, true);
+        out[1] = rhoi;
+        out[2] = (double)del_idx;
+        out[3] = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            out[4 + d] = trial[d];
+            out[7 + d] = sv[d];
+        }
+    }
+}
+
+// k_gcmc_xcommit: an accepted trial of cell c (post_xchng_energy, then
+// ins_succ / del_succ): every atom takes its trial rho / F; an insertion
+// appends the gas atom at x (fractional s, velocity v) with the trial
+// density; a deletion moves the last atom into the hole.
+__global__ __launch_bounds__(256) void k_gcmc_xcommit(GcmcArgs A, GcmcCell* __restrict__ cells, int c, int mode,
+                                                      int del_idx, double rhoi, double x0, double x1, double x2,
+                                                      double s0, double s1, double s2, double v0, double v1,
+                                                      double v2, Arrays R)
+{
+    const int off = cells[c].off, n = cells[c].n, gas = A.gas;
+    for (int j = threadIdx.x; j < n; j += 256) {
+        R.rho[off + j] = R.rho_x[off + j];
+        R.F[off + j] = R.F_x[off + j];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    if (mode == MODE_INS) {
+        const int k = off + n;
+        R.xe[k] = make_double4(x0, x1, x2, (double)gas);
+        R.s[k] = make_double4(s0, s1, s2, 0.0);
+        R.v[k] = make_double4(v0, v1, v2, 0.0);
+        R.rho[k] = rhoi;
+        R.F[k] = f_embed(A, gas, rhoi, true);
+        R.el[k] = (uint8_t)gas;
+        cells[c].n = n + 1;
+        cells[c].ngas += 1;
+    } else {
+        if (del_idx != n - 1) {
+            const int to = off + del_idx, from = off + n - 1;
+            R.xe[to] = R.xe[from];
+            R.s[to] = R.s[from];
+            R.v[to] = R.v[from];
+            R.rho[to] = R.rho[from];
+            R.F[to] = R.F[from];
+            R.el[to] = R.el[from];
+        }
+        cells[c].n = n - 1;
+        cells[c].ngas -= 1;
+    }
+}
+
 Arrays arrays_of(GcmcState& st)
 {
     return Arrays{st.xe.get(), st.s.get(), st.v.get(), st.rho.get(), st.F.get(), st.rho_x.get(), st.F_x.get(),
@@ -736,6 +910,7 @@ void GcmcBatch::attempts(int nattempts, int8_t* decisions, double* dU)
 {
     MAPP_REQUIRE(nattempts >= 0, ERR_ARG, "nattempts must be non-negative");
     if (nattempts == 0) return;
+    ++ver_;
     if (queues_dirty_) build_queues();
     const size_t m = (size_t)nc_ * nattempts;
     dec_.reserve(m);
@@ -831,6 +1006,60 @@ GcmcTuneReport GcmcBatch::tune(int nattempts)
     for (int k = 0; k < kGcmcClasses; ++k) rep.waves[k] = k < best_t ? 1 : 4;
     set_waves(rep.waves);
     return rep;
+}
+
+// ForceFieldEAM::pre_xchng_energy + xchng_energy (src/ff_eam.cpp:323-446) for
+// one trial of one cell, as SGCMC::attmpt (src/sgcmc.cpp:255-327) calls them:
+// mode 1 inserts the gas element at fractional s, mode 2 deletes the igas-th
+// gas atom. The trial is kept for xchng_commit.
+double GcmcBatch::xchng_energy(int cell, int mode, const double* s, int igas)
+{
+    MAPP_REQUIRE(cell >= 0 && cell < nc_, ERR_ARG, "cell index out of range");
+    MAPP_REQUIRE(mode == 1 || mode == 2, ERR_ARG, "mode must be 1 (insertion) or 2 (deletion)");
+    if (mode == 1) {
+        MAPP_REQUIRE(s != nullptr, ERR_ARG, "an insertion needs its fractional position s");
+        for (int d = 0; d < 3; ++d)
+            MAPP_REQUIRE(s[d] >= 0.0 && s[d] < 1.0, ERR_ARG, "s must lie in [0, 1) in every dimension");
+    } else {
+        MAPP_REQUIRE(igas >= 0, ERR_ARG, "igas must be non-negative");
+    }
+    pending_.cell = -1;
+    xout_.reserve(10);
+    const double s0 = mode == 1 ? s[0] : 0.0, s1 = mode == 1 ? s[1] : 0.0, s2 = mode == 1 ? s[2] : 0.0;
+    k_gcmc_xtrial<<<1, 256, 0, stream_>>>(A_, st_.cells.get(), cell, mode == 1 ? MODE_INS : MODE_DEL, s0, s1, s2,
+                                          igas, arrays_of(st_), xout_.get());
+    double o[10];
+    MAPP_HIP(hipMemcpyAsync(o, xout_.get(), sizeof o, hipMemcpyDeviceToHost, stream_));
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    MAPP_REQUIRE(o[3] != 2.0, ERR_ARG, "igas is not below the cell's gas count");
+    MAPP_REQUIRE(o[3] == 0.0, ERR_CAPACITY, "the trial point has more periodic images than the kernel holds");
+    pending_ = PendingTrial{cell, mode, (int)o[2], o[1], {o[4], o[5], o[6]}, {o[7], o[8], o[9]}, ver_};
+    return o[0];
+}
+
+// ForceFieldEAM::post_xchng_energy (src/ff_eam.cpp:451-483) after
+// SGCMC::ins_succ / del_succ (src/sgcmc.cpp:331-444) for the trial of the last
+// xchng_energy call, which the caller accepted; v is the inserted atom's
+// velocity (the reference draws it as gaussian() * sigma per component).
+void GcmcBatch::xchng_commit(const double* v)
+{
+    MAPP_REQUIRE(pending_.cell >= 0 && pending_.ver == ver_, ERR_STATE,
+                 "no trial to commit: call xchng_energy first (the cells changed since, or it was committed)");
+    const int c = pending_.cell;
+    GcmcCell hc;
+    MAPP_HIP(hipMemcpy(&hc, st_.cells.get() + c, sizeof hc, hipMemcpyDeviceToHost));
+    if (pending_.mode == 1) {
+        MAPP_REQUIRE(v != nullptr, ERR_ARG, "an insertion needs the new atom's velocity");
+        MAPP_REQUIRE(hc.n < hc.cap, ERR_CAPACITY, "the cell is full: the insertion cannot be committed");
+    }
+    const double v0 = v ? v[0] : 0.0, v1 = v ? v[1] : 0.0, v2 = v ? v[2] : 0.0;
+    k_gcmc_xcommit<<<1, 256, 0, stream_>>>(A_, st_.cells.get(), c, pending_.mode == 1 ? MODE_INS : MODE_DEL,
+                                           pending_.del_idx, pending_.rhoi, pending_.x[0], pending_.x[1],
+                                           pending_.x[2], pending_.s[0], pending_.s[1], pending_.s[2], v0, v1, v2,
+                                           arrays_of(st_));
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    pending_.cell = -1;
+    ++ver_;
 }
 
 void GcmcBatch::counts(int* natms, int* ngas)

@@ -68,6 +68,12 @@ public:
     void use_graph(bool on);
     void counts(int* natms, int* ngas);
     void get_cell(int c, double* x, uint8_t* elem, double* rho, double* F, double* v, int* seed);
+    // one trial of one cell, driven by the caller (pre_xchng_energy +
+    // xchng_energy): mode 1 inserts at fractional s, mode 2 deletes the
+    // igas-th gas atom; returns the exchange energy and keeps the trial
+    double xchng_energy(int cell, int mode, const double* s, int igas);
+    // the kept trial accepted (post_xchng_energy after ins_succ / del_succ)
+    void xchng_commit(const double* v);
     Profiler& profiler() { return prof_; }
 
 private:
@@ -98,6 +104,14 @@ private:
     hipGraphExec_t graph_ = nullptr;
     int graph_k_ = -1;
     Profiler prof_;
+    struct PendingTrial {
+        int cell = -1, mode = 0, del_idx = -1;
+        double rhoi = 0.0, x[3] = {0, 0, 0}, s[3] = {0, 0, 0};
+        int64_t ver = -1;
+    };
+    PendingTrial pending_;
+    int64_t ver_ = 0;  // bumped by every change of the cells
+    DevBuf<double> xout_;
 };
 
 }  // namespace mapp

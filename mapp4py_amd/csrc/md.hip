@@ -1136,6 +1136,27 @@ MDSystem::MDSystem(std::shared_ptr<EamForceField> ff, int natms, const double* H
     rebuild();
 }
 
+// A new box (and, when x is given, new positions in caller order): the box
+// algebra, the cell grid, the record frame and image shifts, then a rebuild.
+// A box the grid cannot hold leaves the system as it was.
+void MDSystem::set_box(const double* H, const double* x)
+{
+    for (int i = 0; i < 3; ++i)
+        MAPP_REQUIRE(H[i * 3 + i] > 0.0, ERR_ARG, "H must be lower triangular with a positive diagonal");
+    MAPP_REQUIRE(H[1] == 0.0 && H[2] == 0.0 && H[5] == 0.0, ERR_ARG, "H must be lower triangular");
+    const Box old = box_;
+    box_ = make_box(H);
+    try {
+        setup_grid();
+    } catch (...) {
+        box_ = old;
+        setup_grid();
+        throw;
+    }
+    if (x) set_x(x);
+    rebuild();
+}
+
 MDSystem::~MDSystem()
 {
     if (stream_) {

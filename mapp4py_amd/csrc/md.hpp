@@ -111,6 +111,9 @@ public:
     ~MDSystem();
 
     int natms() const { return n_; }
+    // a new box H (lower triangular, rows = box vectors) and, unless x is
+    // null, new positions; rebuilds the grid and the list
+    void set_box(const double* H, const double* x);
     void set_x(const double* x_host);          // caller order
     void get_x(double* x_host);
     void set_v(const double* v_host);

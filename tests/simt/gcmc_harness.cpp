@@ -39,6 +39,26 @@ void emu_gcmc_get_cell(void* g, int c, double* x, uint8_t* e, double* rho, doubl
 {
     ((GcmcBatch*)g)->get_cell(c, x, e, rho, F, v, seed);
 }
+int emu_gcmc_xchng_energy(void* g, int cell, int mode, const double* s, int igas, double* du)
+{
+    try {
+        *du = ((GcmcBatch*)g)->xchng_energy(cell, mode, s, igas);
+        return 0;
+    } catch (const mapp::Error& e) {
+        fprintf(stderr, "emu xchng_energy: %s\n", e.what());
+        return e.code;
+    }
+}
+int emu_gcmc_xchng_commit(void* g, const double* v)
+{
+    try {
+        ((GcmcBatch*)g)->xchng_commit(v);
+        return 0;
+    } catch (const mapp::Error& e) {
+        fprintf(stderr, "emu xchng_commit: %s\n", e.what());
+        return e.code;
+    }
+}
 void emu_gcmc_tune(void* g, int k, int* w)
 {
     auto r = ((GcmcBatch*)g)->tune(k);

@@ -59,6 +59,65 @@ def close(a, b, tag):
     return err
 
 
+def test_box_change_and_static_force_calc_match_oracle(M, cuh):
+    # a strained box through mapp_dmd_set_box gives the oracle's derivative
+    # in that box; force_calc_static then re-sums the free energy, virial and
+    # entropy of the state c_d left (src/ff_dmd.cpp:429-445), and refuses a
+    # changed state
+    x, H, elem, alpha, c = sites((3, 3, 3), 7)
+    g, _ = both(M, cuh, x, H, elem, alpha, c)
+    strain = np.eye(3) + np.array([[0.015, 0, 0], [0.04, -0.01, 0], [-0.03, 0.02, 0.012]])
+    H1, x1 = H @ strain, x @ strain
+    g.set_box(H1, x1)
+    o = O.ODMD(O.OEam.from_knots(cuh, dmd=True), H1, x1, elem, alpha, c, nquad=3, masses=cuh.masses)
+    assert triples(*g.half_pairs()) == triples(*o.pairs())
+    fo, fao, muo, feo, FHo, So = o.force()
+    f, fa, mu, fe, FH, S = g.derivative()
+    errs = [close(f, fo, "f_x"), close(fa, fao, "f_alpha"), close([fe], [feo], "fe"), close(FH, FHo, "F_H")]
+    say(f"DMD strained box: rel errors f {errs[0]:.1e} f_a {errs[1]:.1e} fe {errs[2]:.1e} F_H {errs[3]:.1e}")
+    g.c_d([3.5, 3.5], [1.7, 1.7])
+    fe_s, FH_s, S_s = g.force_calc_static()
+    close([fe_s], [feo], "static fe")
+    close(FH_s, FHo, "static F_H")
+    close([S_s], [So], "static S")
+    assert abs(fe_s - fe) < 1e-12 * abs(fe) and np.abs(FH_s - FH).max() < 1e-12 * np.abs(FH).max()
+    g.set_state(*g.state())
+    with pytest.raises(M.MappError):
+        g.force_calc_static()
+    with pytest.raises(M.MappError):
+        g.set_box(np.diag([2.0, 30.0, 30.0]))
+    f2 = g.derivative()[0]
+    close(f2, fo, "f_x after a rejected box")
+
+
+def test_fp32_rows_stay_within_the_stated_bars(M, cuh):
+    # FP32 copies of the table rows, FP64 moments and sums: the free energy and
+    # gradient against the FP64 oracle. Bars: forces on x and alpha 1e-4 eV/A
+    # (the DMD force bar), mu 1e-4 eV, free energy 1e-8 relative, virial 1e-6
+    # of its largest component. The rate path does not read the FP32 rows.
+    x, H, elem, alpha, c = sites((4, 4, 4), 5, (0.3, -0.2, 0.4))
+    g, o = both(M, cuh, x, H, elem, alpha, c)
+    fo, fao, muo, feo, FHo, So = o.force()
+    r_crd, zeta = [3.5, 3.5], [1.7, 1.7]
+    cd64 = g.c_d(r_crd, zeta)[0]
+    g.set_fp32(True)
+    for batch in (2, 4):
+        g.set_batch(batch)
+        f, fa, mu, fe, FH, S = g.derivative()
+        df, dfa, dmu = np.abs(f - fo).max(), np.abs(fa - fao).max(), np.abs(mu - muo).max()
+        dfe = abs(fe - feo) / abs(feo)
+        dFH = np.abs(FH - FHo).max() / np.abs(FHo).max()
+        say(f"DMD FP32 rows batch {batch}: max|df| {df:.1e} eV/A, max|df_a| {dfa:.1e}, max|dmu| {dmu:.1e} eV, "
+            f"fe rel {dfe:.1e}, F_H rel {dFH:.1e}, S rel {abs(S - So) / abs(So):.1e}")
+        assert df < 1e-4 and dfa < 1e-4 and dmu < 1e-4
+        assert dfe < 1e-8 and dFH < 1e-6
+        assert abs(S - So) < 1e-12 * abs(So)
+        assert abs(g.value() - o.energy()) < 1e-8 * abs(feo)
+    cd32 = g.c_d(r_crd, zeta)[0]
+    np.testing.assert_array_equal(cd32, cd64)
+    g.set_fp32(False)
+
+
 @pytest.mark.parametrize("cells,tilt,seed", [((3, 3, 3), (0, 0, 0), 1), ((3, 4, 3), (0.6, -0.4, 0.5), 2),
                                              ((2, 2, 2), (0, 0, 0), 3)])
 def test_dmd_lists_and_derivative_match_oracle(M, cuh, cells, tilt, seed):

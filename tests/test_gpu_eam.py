@@ -197,6 +197,30 @@ def test_nvt_matches_oracle_short_run(M, cu_pot):
     assert abs(out[0] - outo[0]) < E_ATOM * len(x)
 
 
+def test_box_change_gives_the_oracle_of_the_new_box(M, cu_pot):
+    # what md_nst or a strain step does to atoms->H (src/atoms.cpp:187-194):
+    # the handle takes the new box and positions, and its forces, energy and
+    # virial are those of a fresh oracle in that box; then a box change that
+    # keeps the Cartesian positions; then one the grid cannot hold, which
+    # leaves the system as it was
+    x, H = pot.fcc(3.615, (5, 5, 5))
+    x = jitter(x, 0.05, 41)
+    elem = np.zeros(len(x), np.uint8)
+    g = M.AtomsMD(M.ForceFieldEAM.from_knots(cu_pot), H, x, elem)
+    check_forces(g, O.OMD(O.OEam.from_knots(cu_pot), H, x, elem), "before the box change")
+    strain = np.eye(3) + np.array([[0.02, 0, 0], [0.05, -0.015, 0], [-0.04, 0.03, 0.01]])
+    H1, x1 = H @ strain, x @ strain
+    g.set_box(H1, x1)
+    check_forces(g, O.OMD(O.OEam.from_knots(cu_pot), H1, x1, elem), "strained triclinic box, new positions")
+    H2 = H1.copy()
+    H2[2, 2] *= 1.01
+    g.set_box(H2)
+    check_forces(g, O.OMD(O.OEam.from_knots(cu_pot), H2, g.x, elem), "box change keeping the positions")
+    with pytest.raises(M.MappError):
+        g.set_box(np.diag([2.0, 40.0, 40.0]))
+    check_forces(g, O.OMD(O.OEam.from_knots(cu_pot), H2, g.x, elem), "after a rejected box")
+
+
 def test_feni_nvt_matches_oracle(M, feni_pot):
     # the two-element eam/fs path (global-table passes, the F' array, the
     # fused second half-kick) through 250 NVT steps at 600 K in a sheared box,

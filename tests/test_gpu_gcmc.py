@@ -148,6 +148,67 @@ def test_graph_replays_match_eager(M, ff):
         assert np.array_equal(a.counts()[0], b.counts()[0]), rep
 
 
+def test_single_trial_hooks_follow_the_oracle_chain(M, ff, off):
+    # SGCMC::attmpt (src/sgcmc.cpp:255-327) kept on the caller's side -- its
+    # random stream, draw order and acceptance -- with the exchange energy and
+    # the commit of every trial through the single-trial hooks: the decisions
+    # and energies of the oracle's own chain, trial by trial, and its final
+    # atoms, densities and embedding energies
+    from test_oracle_gcmc import PM, xexp
+
+    cells = batch_cells(3, seed=17, nmax=900)
+    seeds = np.arange(len(cells), dtype=np.int32) * 13 + 401
+    gb = M.GCMCBatch(ff, cells, GAS, -2.0, T, MASSES, seeds, cap=[len(c[2]) + 96 for c in cells])
+    k = 120
+    for c, (x, H, e) in enumerate(cells):
+        o = O.OGCMC(off, H, x, e, GAS, -2.0, T, MASSES, seed=int(seeds[c]))
+        info = o.info()
+        zv, beta, sigma = info["z_fac"] * info["vol"], info["beta"], info["sigma"]
+        d_o, u_o = o.attempts(k)
+        rng = PM(int(seeds[c]))
+        ngas = int((e == GAS).sum())
+        dec, du = np.zeros(k, np.int8), np.zeros(k)
+        for a in range(k):
+            if rng.uniform() < 0.5:
+                sb = [rng.uniform() for _ in range(3)]
+                du[a] = gb.xchng_energy(c, s=sb)
+                acc = rng.uniform() < zv / ((ngas + 1.0) * xexp(beta * du[a]))
+                if acc:
+                    gb.xchng_commit([rng.gaussian() * sigma for _ in range(3)])
+                    ngas += 1
+                dec[a] = 1 if acc else -1
+            else:
+                if ngas == 0:
+                    continue
+                du[a] = gb.xchng_energy(c, igas=int(ngas * rng.uniform()))
+                acc = rng.uniform() < ngas * xexp(beta * du[a]) / zv
+                if acc:
+                    gb.xchng_commit()
+                    ngas -= 1
+                dec[a] = 2 if acc else -2
+        np.testing.assert_array_equal(dec, d_o)
+        err = np.abs(du - u_o).max()
+        assert err < DU_TOL * max(1.0, np.abs(u_o).max()), (c, err)
+        st, cg = o.state(), gb.cell(c)
+        np.testing.assert_array_equal(cg["elem"], st["elem"])
+        np.testing.assert_array_equal(cg["x"], st["x"])
+        np.testing.assert_allclose(cg["rho"], st["rho"], rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(cg["F"], st["F"], rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(cg["v"], st["v"], rtol=0, atol=0)
+        assert rng.seed == o.rng_seed
+        say(f"hooks, cell {c} ({len(e)} atoms): decisions {dict(zip(*np.unique(dec, return_counts=True)))}, "
+            f"max |dU - oracle| {err:.1e} eV")
+    # a trial commits once, and not after a batch launch
+    gb.xchng_energy(0, s=[0.5, 0.5, 0.5])
+    gb.xchng_commit([0.0, 0.0, 0.0])
+    with pytest.raises(M.MappError, match="no trial to commit"):
+        gb.xchng_commit([0.0, 0.0, 0.0])
+    gb.xchng_energy(0, s=[0.25, 0.5, 0.5])
+    gb.attempts(1)
+    with pytest.raises(M.MappError, match="no trial to commit"):
+        gb.xchng_commit([0.0, 0.0, 0.0])
+
+
 def test_full_cell_stops_with_a_capacity_error(M, ff):
     x, H = pot.fcc(3.52, (3, 3, 3))
     cells = [(x, H, np.zeros(len(x), np.uint8))]

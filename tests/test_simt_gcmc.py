@@ -56,6 +56,8 @@ def emu():
     L.emu_gcmc_use_graph.argtypes = [C.c_void_p, C.c_int]
     L.emu_gcmc_counts.argtypes = [C.c_void_p, ip, ip]
     L.emu_gcmc_get_cell.argtypes = [C.c_void_p, C.c_int, dp, u8p, dp, dp, dp, ip]
+    L.emu_gcmc_xchng_energy.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, C.c_int, dp]
+    L.emu_gcmc_xchng_commit.argtypes = [C.c_void_p, dp]
     return L
 
 
@@ -119,3 +121,76 @@ def test_emulated_kernel_reproduces_the_oracle_chains(emu, mu, waves, graph, ks)
         np.testing.assert_array_equal(states[c]["x"], st["x"])
         np.testing.assert_allclose(states[c]["rho"], st["rho"], rtol=1e-10, atol=1e-12)
         assert states[c]["seed"] == g.rng_seed and states[c]["ngas"] == g.ngas
+
+
+def test_emulated_single_trial_hooks_follow_the_oracle_chain(emu):
+    # the hook kernels (one trial's exchange energy, its commit) driven by a
+    # caller-side SGCMC::attmpt with the reference's draw order: the oracle's
+    # decisions, energies and final cells
+    from test_oracle_gcmc import PM, xexp
+
+    cells = cells_for_test()
+    mu, k = -2.0, 40
+    seeds = np.arange(len(cells), dtype=np.int32) * 13 + 401
+    natms = np.array([len(e) for _, _, e in cells], np.int32)
+    x = np.ascontiguousarray(np.concatenate([np.asarray(c[0], float).reshape(-1, 3) for c in cells]))
+    el = np.ascontiguousarray(np.concatenate([np.asarray(c[2], np.uint8) for c in cells]))
+    H = np.ascontiguousarray(np.stack([np.asarray(c[1], float) for c in cells]))
+    cap = natms + 96
+    g = emu.emu_gcmc_create(pot.NIALH_JEA.encode(), len(cells), P(natms, ip), P(H), P(x), P(el, u8p), P(cap, ip),
+                            GAS, mu, 300.0, pot.KB_EV, pot.HP, P(np.array(MASSES)), P(seeds, ip))
+    assert g
+    off = O.OEam.read(pot.NIALH_JEA)
+    try:
+        for c, (xc, Hc, e) in enumerate(cells):
+            o = O.OGCMC(off, Hc, xc, e, GAS, mu, 300.0, MASSES, seed=int(seeds[c]))
+            info = o.info()
+            zv, beta, sigma = info["z_fac"] * info["vol"], info["beta"], info["sigma"]
+            d_o, u_o = o.attempts(k)
+            rng = PM(int(seeds[c]))
+            ngas = int((e == GAS).sum())
+            dec, du = np.zeros(k, np.int8), np.zeros(k)
+            d = C.c_double()
+            for a in range(k):
+                if rng.uniform() < 0.5:
+                    sb = np.array([rng.uniform() for _ in range(3)])
+                    assert emu.emu_gcmc_xchng_energy(g, c, 1, P(sb), 0, C.byref(d)) == 0
+                    du[a] = d.value
+                    acc = rng.uniform() < zv / ((ngas + 1.0) * xexp(beta * du[a]))
+                    if acc:
+                        v = np.array([rng.gaussian() * sigma for _ in range(3)])
+                        assert emu.emu_gcmc_xchng_commit(g, P(v)) == 0
+                        ngas += 1
+                    dec[a] = 1 if acc else -1
+                else:
+                    if ngas == 0:
+                        continue
+                    assert emu.emu_gcmc_xchng_energy(g, c, 2, None, int(ngas * rng.uniform()), C.byref(d)) == 0
+                    du[a] = d.value
+                    acc = rng.uniform() < ngas * xexp(beta * du[a]) / zv
+                    if acc:
+                        assert emu.emu_gcmc_xchng_commit(g, None) == 0
+                        ngas -= 1
+                    dec[a] = 2 if acc else -2
+            np.testing.assert_array_equal(dec, d_o)
+            assert np.abs(du - u_o).max() < 1e-9 * max(1.0, np.abs(u_o).max())
+            n = np.zeros(len(cells), np.int32)
+            ng = np.zeros(len(cells), np.int32)
+            emu.emu_gcmc_counts(g, P(n, ip), P(ng, ip))
+            xs, es, rho, F, v = (np.zeros((n[c], 3)), np.zeros(n[c], np.uint8), np.zeros(n[c]), np.zeros(n[c]),
+                                 np.zeros((n[c], 3)))
+            seed = C.c_int()
+            emu.emu_gcmc_get_cell(g, c, P(xs), P(es, u8p), P(rho), P(F), P(v), C.byref(seed))
+            st = o.state()
+            np.testing.assert_array_equal(es, st["elem"])
+            np.testing.assert_array_equal(xs, st["x"])
+            np.testing.assert_allclose(rho, st["rho"], rtol=1e-10, atol=1e-12)
+            np.testing.assert_allclose(F, st["F"], rtol=1e-10, atol=1e-12)
+            assert ng[c] == o.ngas and rng.seed == o.rng_seed
+        # a trial commits once; a second commit is refused (MAPP_ERR_STATE)
+        sb, v, d = np.array([0.5, 0.5, 0.5]), np.zeros(3), C.c_double()
+        assert emu.emu_gcmc_xchng_energy(g, 0, 1, P(sb), 0, C.byref(d)) == 0
+        assert emu.emu_gcmc_xchng_commit(g, P(v)) == 0
+        assert emu.emu_gcmc_xchng_commit(g, P(v)) == 5
+    finally:
+        emu.emu_gcmc_free(g)
