@@ -359,6 +359,43 @@ def dmd_run(evals=10):
         d16.evaluate()
     ms16 = (time.perf_counter() - t0) / 3 * 1e3
     del d16
+    # the accuracy / speed frontier: for each Gauss-Hermite order, networks
+    # trained from that rule (box over its whole list range) against its
+    # quadrature with FP64 and FP32 rows, same sites
+    frontier = []
+    for N in (3, 5, 8, 16):
+        dn = M.AtomsDMD(M.ForceFieldEAM.from_knots(p, dmd=True), H, x, elem, alpha, c, nquad=N, masses=p.masses)
+        fq = dn.derivative()[0]
+        t0 = time.perf_counter()
+        for _ in range(3):
+            dn.evaluate()
+        q64 = (time.perf_counter() - t0) / 3 * 1e3
+        dn.set_fp32(True)
+        dn.evaluate()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            dn.evaluate()
+        q32 = (time.perf_counter() - t0) / 3 * 1e3
+        dn.set_fp32(False)
+        xmax = float(M.gauss_hermite(N)[0].max())
+        sn = M.Surrogate(2, box=(2.2, 5.5 + xmax * np.sqrt(2.0) * 0.30 + 0.05, 0.05, 0.30), seed=1)
+        sn.train(dn, nsamples=1_000_000, steps=4000, batch=4096, lr=2e-3)
+        dn.use_surrogate(sn)
+        fs = dn.derivative()[0]
+        dn.evaluate()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            dn.evaluate()
+        qs = (time.perf_counter() - t0) / 3 * 1e3
+        cov = dn.surrogate_coverage()
+        dn.use_surrogate(None)
+        frontier.append({"N": N, "quadrature_fp64_ms": q64, "quadrature_fp32_rows_ms": q32, "networks_ms": qs,
+                         "networks_speed_vs_fp64": q64 / qs, "networks_speed_vs_fp32_rows": q32 / qs,
+                         "networks_max_abs_df_x_eV_per_A": float(np.abs(fs - fq).max()),
+                         "quadrature_pairs_left": int(cov[1])})
+        log(f"[bench] frontier N={N}: quadrature {q64:.2f} ms (fp32 rows {q32:.2f}), networks {qs:.2f} ms, "
+            f"max|df| {frontier[-1]['networks_max_abs_df_x_eV_per_A']:.1e} eV/A")
+        del dn, sn
     # concentration rates (ForceFieldDMD::c_d_calc) at the same state: both
     # passes for the chemical potentials, then the exchange sums over the
     # second list (first fcc shell, r_crd 3.5 A, C 1.7 as examples/diff)
@@ -419,7 +456,8 @@ def dmd_run(evals=10):
             "free_energy_error_eV_per_site": abs(fe1 - fe0) / n, "network_calls": nets, "quadrature_pairs": quad,
             "train": {"samples_per_net": 2_000_000, "steps": 8000, "batch": 4096, "loss0": l0, "loss": l1,
                       "seconds": t_train, "dtype": "bf16 MFMA, fp32 accumulate"},
-            "target": ">= 5x the N=3 quadrature at <= 1e-4 eV/A (not met; see DESIGN.md)"}
+            "frontier": frontier,
+            "target": ">= 5x the N=3 quadrature at <= 1e-4 eV/A: not met (speed 0.2-0.3x, forces ~5e-2 eV/A)"}
     return {"workload": "DMD Cu-H, 262,144 fcc sites x 2 components, N=3 Gauss-Hermite, r_c=5.5 A, "
                         "alpha log-uniform [0.05,0.30] A, c_H ~ Beta(2,5)",
             "metric": "DMD site-evals/s (free energy + gradient)", "value": n * evals / dt, "unit": "site-evals/s",
@@ -451,6 +489,27 @@ def dmd_run(evals=10):
                    "iters": cg["iters"], "status": cg["status"], "fe0": cg["fe0"], "fe": cg["fe"],
                    "grad_norm": cg["grad_norm"], "grad_max": cg["grad_max"], "seconds": cg_s},
             "surrogate": surr}
+
+
+def dmd_evals_only(evals=5):
+    """the 262,144-site DMD free-energy + gradient evaluations alone, FP64 and
+    FP32 table rows (a short run for kernel traces and counters)"""
+    import mapp4py_amd as M
+    from mapp4py_amd import potentials as pot
+
+    p = pot.johnson_cuh(nr=5000, nrho=5000, rc=5.5)
+    x, H, elem, alpha, c = pot.dmd_cuh_sites((32, 32, 64), seed=4)
+    x = x + np.random.default_rng(5).normal(0, 0.05, x.shape)
+    d = M.AtomsDMD(M.ForceFieldEAM.from_knots(p, dmd=True), H, x, elem, alpha, c, nquad=3, masses=p.masses)
+    out = {}
+    for fp32 in (False, True):
+        d.set_fp32(fp32)
+        d.evaluate()
+        t0 = time.perf_counter()
+        for _ in range(evals):
+            d.evaluate()
+        out["fp32_rows" if fp32 else "fp64_rows"] = (time.perf_counter() - t0) / evals * 1e3
+    return {"workload": "DMD Cu-H, 262,144 sites x 2 components, N=3", "ms_per_eval": out}
 
 
 def dmd_cpu_baseline(seconds):
@@ -732,6 +791,8 @@ def main():
     ap.add_argument("--no-gcmc", action="store_true", help="skip the batched GCMC measurement")
     ap.add_argument("--no-feni", action="store_true", help="skip the Fe-Ni (two-element) MD measurement")
     ap.add_argument("--no-min", action="store_true", help="skip the L-BFGS minimisation measurement")
+    ap.add_argument("--dmd-evals-only", action="store_true",
+                    help="only the DMD evaluations at 262,144 sites, FP64 and FP32 rows (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel (256 MiB, 20 launches) used to calibrate HBM counters")
     args = ap.parse_args()
@@ -742,6 +803,9 @@ def main():
         _lib.check(_lib.lib().mapp_diag_copy(256 << 20, 20, C.byref(g)))
         print(json.dumps({"copy_bytes_per_launch": 2 * (256 << 20), "read_bytes": 256 << 20, "write_bytes": 256 << 20,
                           "launches": 21, "GBps": g.value}), flush=True)
+        return
+    if args.dmd_evals_only:
+        print(json.dumps(dmd_evals_only()), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",

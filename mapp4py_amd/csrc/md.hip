@@ -149,6 +149,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_order(int ncells, GridArgs g, c
 // each, exact integer separations. Only a pair within the quantisation band
 // of the cut-off (a few 1e-8 A wide) is decided by the FP64 predicate on s,
 // so the neighbor set is that of the predicate for every pair.
+// ONE: a single element, so the cut-offs are kernel-argument scalars; with
+// several, the element pairs' cut-offs are read from LDS (an indexed
+// kernel-argument array would cost a memory round trip per candidate).
+template <bool ONE>
 __global__ __launch_bounds__(kBlock) void k_build_list(int n, const double4* __restrict__ s4,
                                                        const uint4* __restrict__ rec, Rec16Frame R, BoxDev bd,
                                                        GridArgs g, const int* __restrict__ cell_rank,
@@ -157,6 +161,15 @@ __global__ __launch_bounds__(kBlock) void k_build_list(int n, const double4* __r
                                                        uint4* __restrict__ list4, int* __restrict__ nn,
                                                        int* __restrict__ flags)
 {
+    __shared__ double cut_lds[3][kMaxElems * kMaxElems];
+    if (!ONE) {
+        for (int k = threadIdx.x; k < g.ne * g.ne; k += kBlock) {
+            cut_lds[0][k] = g.rec_in_sq[k];
+            cut_lds[1][k] = g.rec_out_sq[k];
+            cut_lds[2][k] = g.cut_sk_sq[k];
+        }
+        __syncthreads();
+    }
     const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     const double4 si = s4[i];
@@ -179,19 +192,22 @@ __global__ __launch_bounds__(kBlock) void k_build_list(int n, const double4* __r
         const uint32_t code = img << kIndexBits;
         const uint4 sh = R.shift[img];
         const uint4 rs = make_uint4(ri.x - sh.x, ri.y - sh.y, ri.z - sh.z, 0u);
+        // the next candidate's record is loaded one candidate ahead
+        uint4 rn = rec[jb < je ? jb : i];
         for (int j = jb; j < je; ++j) {
+            const uint4 rj = rn;
+            rn = rec[j + 1 < je ? j + 1 : j];
             if (self_img && j == i) continue;
-            const uint4 rj = rec[j];
             const double dx = (double)(int)(rs.x - rj.x), dy = (double)(int)(rs.y - rj.y),
                          dz = (double)(int)(rs.z - rj.z);
             const double rsq_u = dx * dx + dy * dy + dz * dz;
-            const int tt = ti * g.ne + (int)rj.w;
-            bool in = rsq_u < g.rec_in_sq[tt];
-            if (!in && rsq_u <= g.rec_out_sq[tt]) {
+            const int tt = ONE ? 0 : ti * g.ne + (int)rj.w;
+            bool in = rsq_u < (ONE ? g.rec_in_sq[0] : cut_lds[0][tt]);
+            if (!in && rsq_u <= (ONE ? g.rec_out_sq[0] : cut_lds[1][tt])) {
                 const double4 sj = s4[j];
                 double xj0, xj1, xj2;
                 s_to_x(bd.h, shift_unit(sj.x, n0), shift_unit(sj.y, n1), shift_unit(sj.z, n2), xj0, xj1, xj2);
-                in = rsq_exact(xi0, xi1, xi2, xj0, xj1, xj2) < g.cut_sk_sq[tt];
+                in = rsq_exact(xi0, xi1, xi2, xj0, xj1, xj2) < (ONE ? g.cut_sk_sq[0] : cut_lds[2][tt]);
             }
             if (in) {
                 const uint32_t e = (uint32_t)j | code;
@@ -1377,9 +1393,15 @@ void MDSystem::build_list()
         list_.reserve((size_t)maxn_ * ldl_);
         MAPP_HIP(hipMemsetAsync(flags_.get(), 0, sizeof(int), stream_));
         prof_.begin(K_BUILD_LIST, stream_);
-        hipLaunchKernelGGL(k_build_list, dim3(nb), dim3(kBlock), 0, stream_, n_, s_.get(), rec_.get(), r16_, bd, g,
-                           cell_rank_.get(), cell_start_.get(), ordered_ ? cell_order_.get() : nullptr, maxn_ / 4,
-                           ldl_, list4(), nn_.get(), flags_.get());
+        if (t.ne == 1)
+            hipLaunchKernelGGL(k_build_list<true>, dim3(nb), dim3(kBlock), 0, stream_, n_, s_.get(), rec_.get(), r16_,
+                               bd, g, cell_rank_.get(), cell_start_.get(), ordered_ ? cell_order_.get() : nullptr,
+                               maxn_ / 4, ldl_, list4(), nn_.get(), flags_.get());
+        else
+            hipLaunchKernelGGL(k_build_list<false>, dim3(nb), dim3(kBlock), 0, stream_, n_, s_.get(), rec_.get(),
+                               r16_, bd, g, cell_rank_.get(), cell_start_.get(),
+                               ordered_ ? cell_order_.get() : nullptr, maxn_ / 4, ldl_, list4(), nn_.get(),
+                               flags_.get());
         prof_.end(stream_);
         int over = 0;
         MAPP_HIP(hipMemcpyAsync(&over, flags_.get(), sizeof(int), hipMemcpyDeviceToHost, stream_));

@@ -113,7 +113,8 @@ def test_networks_against_the_quadrature(M, trained):
             rel = err / (np.abs(want).max(0) + 1e-12)
             say(f"net ({a},{b}) max abs error per output {np.array2string(err, precision=2)} "
                 f"(relative to the output range {np.array2string(rel, precision=2)})")
-            assert np.all(rel < 0.25)
+            # measured at most 5% of an output's range after 3000 steps
+            assert np.all(rel < 0.08)
 
 
 def test_dmd_evaluation_through_the_networks(M, trained):
@@ -131,7 +132,11 @@ def test_dmd_evaluation_through_the_networks(M, trained):
     say(f"surrogate vs quadrature: max|df_x| {df:.2e} eV/A, max|df_alpha| {dfa:.2e} eV/A, "
         f"|dFE|/N {abs(fe1 - fe0) / len(f0):.2e} eV, network calls {nets}, quadrature slots {quad}")
     assert nets > 0
-    assert df < 0.5 and abs(fe1 - fe0) / len(f0) < 0.05
+    # What this network achieves, not the target: forces 5.7e-2 eV/A and the
+    # free energy 5.7e-3 eV per site from the quadrature (measured), about 570
+    # times the 1e-4 eV/A the surrogate was meant to reach; the quadrature
+    # stays the path that meets it
+    assert df < 0.1 and dfa < 0.2 and abs(fe1 - fe0) / len(f0) < 0.01
 
 
 def _time(fn, k=5):
