@@ -267,17 +267,39 @@ mapp_status mapp_dmd_J(mapp_dmd* d, const double* x, double* Ax);
 mapp_status mapp_dmd_newton(mapp_dmd* d, const double* r_crd, const double* C, double beta, const double* a,
                             double a_tol, double err_fac, int max_nnewton_iters, int max_ngmres_iters,
                             int* converged, int* iters);
+/* ForceFieldEAMDMD::__prepJ_n_res + __J (src/ff_eam_dmd.cpp:364-831), via
+ * ForceFieldDMD::prepJ_n_res / Jnew (src/ff_dmd.cpp:495-525): at the current
+ * state and fixed c, (Jdx, Jdalpha) = -(d f / d(x, alpha)) (dx, dalpha), f
+ * the gradient per unit c that calc_mu leaves in f and f_alpha. dx, Jdx and f
+ * [natms][3], dalpha, Jdalpha and f_alpha [natms][c_dim], caller order; f and
+ * f_alpha may be NULL. FP64 table rows and second derivatives from the
+ * Hermite moments xi^0..xi^4. */
+mapp_status mapp_dmd_Jxa(mapp_dmd* d, const double* dx, const double* dalpha, double* Jdx, double* Jdalpha,
+                         double* f, double* f_alpha);
+/* The Newton stage of DAE::min_error_false (src/dae.cpp:294-325) at fixed c:
+ * while |f| > a_tol sqrt(3 natms + natms c_dim) and fewer than
+ * max_nnewton_iters steps, GMRES (at most max_ngmres_iters <= 8 products,
+ * tolerance 0.005 a_tol sqrt(...), zero start) solves J h = f with the J of
+ * mapp_dmd_Jxa; x and alpha take h whole unless an alpha would turn negative
+ * (then half the largest fraction that keeps them non-negative,
+ * MinDMDHandler::max_alpha_lcl_alpha); the list follows. *steps = Newton
+ * steps taken, *res = |f| at the end (either may be NULL). */
+mapp_status mapp_dmd_min_newton(mapp_dmd* d, double a_tol, int max_nnewton_iters, int max_ngmres_iters, int* steps,
+                                double* res);
 /* DAEBDF::run (src/dae_bdf.cpp:103-250, the coefficient algebra of
  * src/dae_bdf.h:103-546): variable-order (1..5) BDF integration of the
  * concentrations from the current state over t_tot, each step's implicit
  * equation solved by mapp_dmd_newton, order and step chosen as the reference
  * chooses them. Positions and widths stay fixed between relaxations; with
  * nreset > 0 they are relaxed at the start, every nreset steps when the
- * gradient error exceeds a_tol, and whenever it doubles, by the GPU CG
- * (DAE::min_error_false's CG stage; its second-derivative Newton stage is not
- * built). nreset = 0: no relaxation (the reference divides by nreset). With
- * ntally > 0, rows (step, time, free energy) at step 0 and every ntally steps
- * go to tally [max_tally_rows][3]; *tally_rows counts the rows written. */
+ * gradient error exceeds a_tol, whenever it doubles, and at the end when the
+ * step count is not a multiple of nreset, by DAE::min_error_false
+ * (src/dae.cpp:273-330): the GPU CG with LineSearchBrent, then the Newton
+ * stage of mapp_dmd_min_newton with max_nnewton_iters and max_ngmres_iters
+ * (err_mins counts the relaxations whose Newton stage took a step). nreset =
+ * 0: no relaxation (the reference divides by nreset). With ntally > 0, rows
+ * (step, time, free energy) at step 0, every ntally steps and at the last
+ * step go to tally [max_tally_rows][3]; *tally_rows counts the rows written. */
 typedef struct {
     double a_tol, min_dt;
     int max_nsteps, max_nnewton_iters, max_ngmres_iters, nreset, ntally, relax_max_iter;
@@ -290,6 +312,38 @@ typedef struct {
 mapp_status mapp_dmd_bdf_run(mapp_dmd* d, const double* r_crd, const double* C, double t_tot,
                              const mapp_bdf_params* params, mapp_bdf_report* report, double* tally,
                              int max_tally_rows, int* tally_rows);
+
+/* CFG (AtomEye) files of MD systems. mapp_cfg_md_read replaces
+ * ImportCFGMD::operator() (src/import_cfg.cpp:495-604) with
+ * ImportCFG::read_header / read_body_std / read_body_ext
+ * (src/import_cfg.cpp:32-480): standard files (rows of mass, symbol, s, s_d)
+ * and extended ones (mass and symbol lines, rows of entry_count values),
+ * positions wrapped into [0, 1) and returned Cartesian, elements numbered in
+ * order of first appearance (the first mass of a symbol kept), velocities
+ * v = s_d R H unless the file says .NO_VELOCITY. (has_vel 0), the box in
+ * lower-triangular form times A. mapp_cfg_md_get fills H [3][3], x and v
+ * [natms][3] (v ignored without velocities), elem [natms] and masses
+ * [nelems]; any output may be NULL. */
+typedef struct mapp_cfg_md mapp_cfg_md;
+mapp_status mapp_cfg_md_read(const char* path, mapp_cfg_md** out);
+mapp_status mapp_cfg_md_info(const mapp_cfg_md* cfg, int* natms, int* nelems, int* has_vel);
+mapp_status mapp_cfg_md_get(const mapp_cfg_md* cfg, double* H, double* x, double* v, uint8_t* elem,
+                            double* masses);
+const char* mapp_cfg_md_element(const mapp_cfg_md* cfg, int i);
+void mapp_cfg_md_free(mapp_cfg_md* cfg);
+/* ExportCFGMD::write_header / write_body_sort (src/export_cfg_md.cpp:26-99)
+ * for host arrays: atoms in the given order, positions as wrapped fractional
+ * coordinates, velocities (v NULL: .NO_VELOCITY.) as s_d = v H^-1, then naux
+ * auxiliary columns aux [natms][naux] named aux_names (the reference's
+ * extra_vecs, one name per column); names holds nelems symbols. */
+mapp_status mapp_cfg_md_write(const char* path, int natms, const double* H, const double* x, const uint8_t* elem,
+                              const double* v, int naux, const double* aux, const char* const* aux_names,
+                              int nelems, const double* masses, const char* const* names);
+/* md.export_cfg (src/export_cfg_md.cpp:26-216) of a handle: its current
+ * positions, box and elements, with velocities when with_vel is nonzero,
+ * atoms in caller order (the reference's sort=True). */
+mapp_status mapp_md_export_cfg(mapp_md* md, const char* path, int nelems, const double* masses,
+                               const char* const* names, int with_vel);
 
 /* Extended CFG (AtomEye) files of DMD systems. mapp_cfg_dmd_read replaces
  * ImportCFGDMD::operator() (src/import_cfg.cpp:669-854) with

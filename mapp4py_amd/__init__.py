@@ -145,6 +145,12 @@ class AtomsMD:
         check(lib().mapp_md_set_box(self._h, ptr(Hn), ptr(xv) if xv is not None else None))
         self.H = Hn
 
+    def export_cfg(self, path, names, masses, with_vel=False):
+        """md.export_cfg: the current positions (and velocities) as a CFG file,
+        atoms in caller order; names and masses of the force field's elements."""
+        check(lib().mapp_md_export_cfg(self._h, str(path).encode(), len(names), ptr(f64(masses)), _names(names),
+                                       int(bool(with_vel))))
+
     def force_calc(self, want_f=True):
         """Returns (f [N][3], energy, F_H Voigt[6])."""
         f = np.zeros((self.natms, 3)) if want_f else None
@@ -458,6 +464,26 @@ class AtomsDMD:
         rep = {k: getattr(r, k) for k, _ in BdfReport._fields_}
         return rep, tally[:rows.value]
 
+    def Jxa(self, dx, dalpha, with_f=False):
+        """ForceFieldEAMDMD::__prepJ_n_res + __J at the current state: returns
+        (Jdx [nsites][3], Jdalpha [nsites][c_dim]) = -(d f / d(x, alpha)) (dx,
+        dalpha) at fixed c, f the gradient per unit c; with_f adds (f, f_alpha)."""
+        dx = f64(dx).reshape(self.nsites, 3)
+        da = f64(dalpha).reshape(self.nsites, self.c_dim)
+        jx, ja = np.zeros((self.nsites, 3)), np.zeros((self.nsites, self.c_dim))
+        fx, fa = (np.zeros((self.nsites, 3)), np.zeros((self.nsites, self.c_dim))) if with_f else (None, None)
+        check(lib().mapp_dmd_Jxa(self._h, ptr(dx), ptr(da), ptr(jx), ptr(ja), ptr(fx) if with_f else None,
+                                 ptr(fa) if with_f else None))
+        return (jx, ja, fx, fa) if with_f else (jx, ja)
+
+    def min_newton(self, a_tol, max_nnewton_iters=5, max_ngmres_iters=5):
+        """the Newton stage of DAE::min_error_false over x and alpha at fixed c:
+        (steps taken, |f| at the end)"""
+        k, r = C.c_int(), C.c_double()
+        check(lib().mapp_dmd_min_newton(self._h, float(a_tol), int(max_nnewton_iters), int(max_ngmres_iters),
+                                        C.byref(k), C.byref(r)))
+        return k.value, r.value
+
     def unit_gradient(self):
         """calc_mu's gradient per unit c at the state of the last c_d call:
         (g [nsites][3], g_alpha [nsites][c_dim])."""
@@ -582,6 +608,66 @@ class AtomsDMD:
         if getattr(self, "_h", None):
             lib().mapp_dmd_destroy(self._h)
             self._h = None
+
+
+def _names(names):
+    return (C.c_char_p * len(names))(*[str(v).encode() for v in names])
+
+
+def import_cfg_md(path):
+    """md.atoms.import_cfg: a standard or extended CFG file of an MD system as a
+    dict with H [3][3] (lower triangular, rows = box vectors), x [natms][3]
+    (Cartesian), v [natms][3] (Cartesian; None for .NO_VELOCITY.), elem
+    [natms], masses and names (element symbols in order of first
+    appearance)."""
+    h = C.c_void_p()
+    check(lib().mapp_cfg_md_read(str(path).encode(), C.byref(h)))
+    try:
+        n, ne, hv = C.c_int(), C.c_int(), C.c_int()
+        check(lib().mapp_cfg_md_info(h, C.byref(n), C.byref(ne), C.byref(hv)))
+        n, ne, hv = n.value, ne.value, bool(hv.value)
+        out = {"H": np.zeros((3, 3)), "x": np.zeros((n, 3)), "v": np.zeros((n, 3)) if hv else None,
+               "elem": np.zeros(n, np.uint8), "masses": np.zeros(max(ne, 1))}
+        check(lib().mapp_cfg_md_get(h, ptr(out["H"]), ptr(out["x"]), ptr(out["v"]) if hv else None,
+                                    ptr(out["elem"], u8p), ptr(out["masses"])))
+        out["masses"] = out["masses"][:ne]
+        out["names"] = [lib().mapp_cfg_md_element(h, i).decode() for i in range(ne)]
+        return out
+    finally:
+        lib().mapp_cfg_md_free(h)
+
+
+def replicate(cfg, N):
+    """atoms *= N (Atoms::mul, src/atoms.cpp:335-390) on a dict from
+    import_cfg_md: copy k = (k0 N1 + k1) N2 + k2 of every atom shifted by
+    (k0, k1, k2) H, copies in order of k, box row d scaled by N[d]."""
+    N = [int(v) for v in N]
+    H = np.asarray(cfg["H"], dtype=np.float64)
+    k = np.array([(a, b, c) for a in range(N[0]) for b in range(N[1]) for c in range(N[2])], dtype=np.float64)
+    shift = k @ H
+    out = dict(cfg)
+    out["x"] = (cfg["x"][None, :, :] + shift[:, None, :]).reshape(-1, 3)
+    out["elem"] = np.tile(cfg["elem"], len(k))
+    if cfg.get("v") is not None:
+        out["v"] = np.tile(cfg["v"], (len(k), 1))
+    out["H"] = H * np.array(N, dtype=np.float64)[:, None]
+    return out
+
+
+def write_cfg_md(path, H, x, elem, names, masses, v=None, aux=None, aux_names=None):
+    """md.export_cfg for host arrays: x (and v) Cartesian [natms][3], H lower
+    triangular, elem [natms]; aux [natms][naux] extra columns named aux_names."""
+    x = f64(x).reshape(-1, 3)
+    n = x.shape[0]
+    elem = np.ascontiguousarray(elem, dtype=np.uint8).reshape(n)
+    H, m = f64(H).reshape(3, 3), f64(masses)
+    vv = None if v is None else f64(v).reshape(n, 3)
+    naux = 0 if aux is None else int(np.asarray(aux).reshape(n, -1).shape[1])
+    av = None if aux is None else f64(aux).reshape(n, naux)
+    an = _names(aux_names or []) if naux else None
+    check(lib().mapp_cfg_md_write(str(path).encode(), n, ptr(H), ptr(x), ptr(elem, u8p),
+                                  ptr(vv) if vv is not None else None, naux, ptr(av) if av is not None else None, an,
+                                  len(names), ptr(m), _names(names)))
 
 
 def import_cfg_dmd(path):

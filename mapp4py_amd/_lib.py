@@ -75,6 +75,16 @@ SIGNATURES = {
                                   C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "mapp_dmd_bdf_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double,
                                    C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int)]),
+    "mapp_dmd_Jxa": (C.c_int, [C.c_void_p, dp, dp, dp, dp, C.c_void_p, C.c_void_p]),
+    "mapp_dmd_min_newton": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, ip, C.POINTER(C.c_double)]),
+    "mapp_cfg_md_read": (C.c_int, [C.c_char_p, vpp]),
+    "mapp_cfg_md_info": (C.c_int, [C.c_void_p, ip, ip, ip]),
+    "mapp_cfg_md_get": (C.c_int, [C.c_void_p, dp, dp, dp, u8p, dp]),
+    "mapp_cfg_md_element": (C.c_char_p, [C.c_void_p, C.c_int]),
+    "mapp_cfg_md_free": (None, [C.c_void_p]),
+    "mapp_cfg_md_write": (C.c_int, [C.c_char_p, C.c_int, dp, dp, u8p, dp, C.c_int, dp, C.POINTER(C.c_char_p),
+                                    C.c_int, dp, C.POINTER(C.c_char_p)]),
+    "mapp_md_export_cfg": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, dp, C.POINTER(C.c_char_p), C.c_int]),
     "mapp_cfg_dmd_read": (C.c_int, [C.c_char_p, vpp]),
     "mapp_cfg_dmd_info": (C.c_int, [C.c_void_p, ip, ip, ip]),
     "mapp_cfg_dmd_get": (C.c_int, [C.c_void_p, dp, dp, u8p, dp, dp, dp]),

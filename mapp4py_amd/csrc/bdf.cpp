@@ -58,10 +58,15 @@ private:
 
     void min_error()
     {
-        // DAE::min_error_false's CG stage (src/dae.cpp:274-330): MinCGDMD over x
-        // and alpha at fixed c with e_tol 1e-9, max_dx 1, max_dalpha 0.1
-        d_.cg(p_.relax_max_iter, 1e-9, 1.0, 0.1, 0.0);
-        ++rep_.err_mins;
+        // DAE::min_error_false (src/dae.cpp:273-330): MinCGDMD over x and alpha
+        // at fixed c with e_tol 1e-9, max_dx 1, max_dalpha 0.1 and
+        // LineSearchBrent's defaults (DAE::init, src/dae.cpp:115-118), then the
+        // Newton stage; counted when the Newton stage took a step
+        LineSearchParams brent;
+        brent.kind = 2;
+        d_.cg(p_.relax_max_iter, 1e-9, 1.0, 0.1, 0.0, &brent);
+        double res = 0.0;
+        if (d_.newton_xa(p_.a_tol, p_.max_nnewton_iters, p_.max_ngmres_iters, &res) > 0) ++rep_.err_mins;
     }
 
     static double get_max_dt(double x, double x_d)
@@ -365,6 +370,12 @@ BdfReport Bdf::run(double t_tot, std::vector<double>* tally)
             tally->insert(tally->end(), {(double)istep, t_cur_, r.fe});
         }
     }
+    // the last step's line and relaxation (src/dae_bdf.cpp:203-217)
+    if (tally && p_.ntally && istep % p_.ntally) {
+        const DMDReport r = d_.derivative(false, nullptr);
+        tally->insert(tally->end(), {(double)istep, t_cur_, r.fe});
+    }
+    if (p_.nreset && istep % p_.nreset) min_error();
     rep_.nsteps = istep;
     rep_.t = t_cur_;
     return rep_;

@@ -36,6 +36,9 @@ struct mapp_surr {
 struct mapp_cfg {
     mapp::CfgDMD data;
 };
+struct mapp_cfg_md {
+    mapp::CfgMD data;
+};
 struct mapp_gcmc {
     std::unique_ptr<mapp::GcmcBatch> sys;
 };
@@ -518,6 +521,31 @@ mapp_status mapp_dmd_newton(mapp_dmd* d, const double* r_crd, const double* C, d
     });
 }
 
+mapp_status mapp_dmd_Jxa(mapp_dmd* d, const double* dx, const double* dalpha, double* Jdx, double* Jdalpha,
+                         double* f, double* f_alpha)
+{
+    return guard([&] {
+        NEED(d);
+        NEED(dx);
+        NEED(dalpha);
+        NEED(Jdx);
+        NEED(Jdalpha);
+        d->sys->Jxa(dx, dalpha, Jdx, Jdalpha, f, f_alpha);
+    });
+}
+
+mapp_status mapp_dmd_min_newton(mapp_dmd* d, double a_tol, int max_nnewton_iters, int max_ngmres_iters, int* steps,
+                                double* res)
+{
+    return guard([&] {
+        NEED(d);
+        double r = 0.0;
+        const int k = d->sys->newton_xa(a_tol, max_nnewton_iters, max_ngmres_iters, &r);
+        if (steps) *steps = k;
+        if (res) *res = r;
+    });
+}
+
 mapp_status mapp_dmd_bdf_run(mapp_dmd* d, const double* r_crd, const double* C, double t_tot,
                              const mapp_bdf_params* params, mapp_bdf_report* report, double* tally,
                              int max_tally_rows, int* tally_rows)
@@ -555,6 +583,95 @@ mapp_status mapp_dmd_bdf_run(mapp_dmd* d, const double* r_crd, const double* C, 
             report->integ_rej = r.integ_rej;
             report->err_mins = r.err_mins;
         }
+    });
+}
+
+namespace {
+std::vector<std::string> name_list(int n, const char* const* names)
+{
+    std::vector<std::string> out;
+    for (int i = 0; i < n; ++i) {
+        MAPP_REQUIRE(names[i] != nullptr, mapp::ERR_ARG, "null name");
+        out.emplace_back(names[i]);
+    }
+    return out;
+}
+}  // namespace
+
+mapp_status mapp_cfg_md_read(const char* path, mapp_cfg_md** out)
+{
+    return guard([&] {
+        NEED(path);
+        NEED(out);
+        *out = nullptr;
+        auto h = std::make_unique<mapp_cfg_md>();
+        h->data = mapp::read_cfg_md(path);
+        *out = h.release();
+    });
+}
+
+mapp_status mapp_cfg_md_info(const mapp_cfg_md* cfg, int* natms, int* nelems, int* has_vel)
+{
+    return guard([&] {
+        NEED(cfg);
+        if (natms) *natms = cfg->data.natms;
+        if (nelems) *nelems = cfg->data.nelems;
+        if (has_vel) *has_vel = cfg->data.has_vel ? 1 : 0;
+    });
+}
+
+mapp_status mapp_cfg_md_get(const mapp_cfg_md* cfg, double* H, double* x, double* v, uint8_t* elem,
+                            double* masses)
+{
+    return guard([&] {
+        NEED(cfg);
+        const mapp::CfgMD& d = cfg->data;
+        if (H) std::memcpy(H, d.H, sizeof d.H);
+        if (x) std::memcpy(x, d.x.data(), sizeof(double) * d.x.size());
+        if (v && d.has_vel) std::memcpy(v, d.v.data(), sizeof(double) * d.v.size());
+        if (elem) std::memcpy(elem, d.elem.data(), d.elem.size());
+        if (masses) std::memcpy(masses, d.masses.data(), sizeof(double) * d.masses.size());
+    });
+}
+
+const char* mapp_cfg_md_element(const mapp_cfg_md* cfg, int i)
+{
+    if (!cfg || i < 0 || i >= cfg->data.nelems) return nullptr;
+    return cfg->data.names[i].c_str();
+}
+
+void mapp_cfg_md_free(mapp_cfg_md* cfg) { delete cfg; }
+
+mapp_status mapp_cfg_md_write(const char* path, int natms, const double* H, const double* x, const uint8_t* elem,
+                              const double* v, int naux, const double* aux, const char* const* aux_names,
+                              int nelems, const double* masses, const char* const* names)
+{
+    return guard([&] {
+        NEED(path);
+        NEED(H);
+        NEED(masses);
+        NEED(names);
+        MAPP_REQUIRE(natms >= 0 && naux >= 0, mapp::ERR_ARG, "natms >= 0 and naux >= 0");
+        if (natms) {
+            NEED(x);
+            NEED(elem);
+            if (naux) NEED(aux);
+        }
+        if (naux) NEED(aux_names);
+        mapp::write_cfg_md(path, natms, H, x, elem, v, naux, aux, name_list(naux, aux_names), nelems, masses,
+                           name_list(nelems, names));
+    });
+}
+
+mapp_status mapp_md_export_cfg(mapp_md* md, const char* path, int nelems, const double* masses,
+                               const char* const* names, int with_vel)
+{
+    return guard([&] {
+        NEED(md);
+        NEED(path);
+        NEED(masses);
+        NEED(names);
+        md->sys->export_cfg(path, nelems, masses, name_list(nelems, names), with_vel != 0);
     });
 }
 

@@ -1,4 +1,4 @@
-// cfg.cpp -- extended CFG files of DMD systems (host code).
+// cfg.cpp -- CFG files of MD and DMD systems (host code).
 #include "cfg.hpp"
 
 #include <cctype>
@@ -87,21 +87,25 @@ bool sym_sqrt3(const double (&A)[3][3], double (&R)[3][3])
     return true;
 }
 
-}  // namespace
+// what ImportCFG::read_header leaves (src/import_cfg.cpp:32-211): the box
+// (lower triangular, times A), the particle count, the body layout and the
+// tokens of the body's first line
+struct Header {
+    int natms = -1, entry_count = 6;
+    bool vel_xst = true, is_ext = false, done = false;
+    double R = 1.0;
+    double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<std::string> first;
+};
 
-CfgDMD read_cfg_dmd(const char* path)
+Header read_header(std::istream& in, const std::string& file)
 {
-    std::ifstream in(path);
-    MAPP_REQUIRE(in.good(), ERR_FILE, std::string("cannot open ") + path);
-    const std::string file(path);
-    // header (src/import_cfg.cpp:32-211)
+    Header h;
     double basic_length = 1.0;
     double H0[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, eta[3][3] = {}, trns[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-    int entry_count = 6, natms = -1;
-    bool vel_xst = true, is_ext = false, header_done = false;
     std::string raw;
     std::vector<std::string> tk;
-    while (!header_done && std::getline(in, raw)) {
+    while (!h.done && std::getline(in, raw)) {
         tk = tokens(raw);
         if (tk.empty()) continue;
         const std::string line = join(tk);
@@ -110,7 +114,7 @@ CfgDMD read_cfg_dmd(const char* path)
         double v;
         if (std::sscanf(l, " Number of particles = %d ", &n0) == 1) {
             MAPP_REQUIRE(n0 >= 0, ERR_FILE, "Number of particles in " + file + " should be greater than 0");
-            natms = n0;
+            h.natms = n0;
             continue;
         }
         if (std::sscanf(l, " A = %lf ", &v) == 1) {
@@ -119,25 +123,27 @@ CfgDMD read_cfg_dmd(const char* path)
             continue;
         }
         if (line == ".NO_VELOCITY.") {
-            if (!is_ext) {
-                vel_xst = false;
-                is_ext = true;
+            if (!h.is_ext) {
+                h.vel_xst = false;
+                h.is_ext = true;
             }
             continue;
         }
         if (std::sscanf(l, " entry_count = %d ", &n0) == 1) {
-            const int mincomp = 3 + (vel_xst ? 3 : 0);
+            const int mincomp = 3 + (h.vel_xst ? 3 : 0);
             MAPP_REQUIRE(n0 >= mincomp, ERR_FILE, "entry_count in " + file + " is below the position entries");
-            entry_count = n0;
-            is_ext = true;
+            h.entry_count = n0;
+            h.is_ext = true;
             continue;
         }
-        char sbuf[8];
-        if (std::sscanf(l, " R = %lf %7s", &v, sbuf) >= 1 && l[0] == 'R') continue;
+        if (l[0] == 'R' && std::sscanf(l, " R = %lf ", &v) == 1) {
+            h.R = v;
+            continue;
+        }
         if (std::sscanf(l, " auxiliary [ %d ] =", &i0) == 1) {
-            MAPP_REQUIRE(is_ext, ERR_FILE, "auxiliary[...] must come after entry_count in " + file);
-            const int mincomp = 3 + (vel_xst ? 3 : 0);
-            MAPP_REQUIRE(i0 + mincomp + 1 <= entry_count, ERR_FILE, "auxiliary index beyond entry_count in " + file);
+            MAPP_REQUIRE(h.is_ext, ERR_FILE, "auxiliary[...] must come after entry_count in " + file);
+            const int mincomp = 3 + (h.vel_xst ? 3 : 0);
+            MAPP_REQUIRE(i0 + mincomp + 1 <= h.entry_count, ERR_FILE, "auxiliary index beyond entry_count in " + file);
             continue;
         }
         if (std::sscanf(l, " Transform ( %d , %d ) = %lf ", &i0, &j0, &v) == 3) {
@@ -155,14 +161,15 @@ CfgDMD read_cfg_dmd(const char* path)
             eta[i0 - 1][j0 - 1] = v;
             continue;
         }
-        if ((tk.size() == 8 && !is_ext) || (tk.size() == 1 && is_ext)) {
-            header_done = true;  // this line (the first mass) starts the body
+        if ((tk.size() == 8 && !h.is_ext) || (tk.size() == 1 && h.is_ext)) {
+            h.done = true;  // this line (the first mass, or the first atom) starts the body
+            h.first = tk;
             continue;
         }
         MAPP_REQUIRE(false, ERR_FILE, "invalid line in " + file + ": " + line);
     }
-    MAPP_REQUIRE(natms >= 0, ERR_FILE, "no 'Number of particles' in " + file);
-    MAPP_REQUIRE(header_done || natms == 0, ERR_FILE, "file " + file + " ended unexpectedly");
+    MAPP_REQUIRE(h.natms >= 0, ERR_FILE, "no 'Number of particles' in " + file);
+    MAPP_REQUIRE(h.done || h.natms == 0, ERR_FILE, "file " + file + " ended unexpectedly");
     // box: H0 Transform, then sqrt(1 + 2 eta), then lower-triangular form
     double Hx[3][3];
     for (int i = 0; i < 3; ++i)
@@ -198,21 +205,53 @@ CfgDMD read_cfg_dmd(const char* path)
                 L[i][j] = g / L[j][j];
             }
         }
-    CfgDMD out;
     for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) out.H[i * 3 + j] = L[i][j] * basic_length;
-    out.natms = natms;
-    if (natms == 0) return out;
-    MAPP_REQUIRE(is_ext, ERR_FILE, "cannot read cfg standard file for dmd mode");
-    // body (src/import_cfg.cpp:213-356): mass, symbol, then rows of
-    // entry_count values; s wrapped into [0, 1)
-    const int naux = entry_count - 3;
+        for (int j = 0; j < 3; ++j) h.H[i * 3 + j] = L[i][j] * basic_length;
+    return h;
+}
+
+// the atom section: fractional positions wrapped into [0, 1), the element of
+// every atom (Elements::add_type numbering: first appearance, the first mass
+// given for a name kept), and the remaining columns
+struct Body {
+    std::vector<std::string> names;
+    std::vector<double> masses;
+    std::vector<uint8_t> elem;
     std::vector<double> s, aux;
-    s.reserve((size_t)3 * natms);
-    aux.reserve((size_t)naux * natms);
-    bool mass_flag = false, elem_init = false;
+    int naux = 0;
+};
+
+int add_type(Body& b, double mass, const std::string& name, const std::string& file)
+{
+    for (size_t i = 0; i < b.names.size(); ++i)
+        if (b.names[i] == name) return (int)i;
+    MAPP_REQUIRE(b.names.size() < 256, ERR_FILE, "cannot have more than 256 elements in " + file);
+    b.names.push_back(name);
+    b.masses.push_back(mass);
+    return (int)b.names.size() - 1;
+}
+
+void push_s(Body& b, const std::vector<std::string>& tk, int at)
+{
+    for (int d = 0; d < 3; ++d) {
+        double v = std::atof(tk[at + d].c_str());
+        while (v >= 1.0) v--;
+        while (v < 0.0) v++;
+        b.s.push_back(v);
+    }
+}
+
+// ImportCFG::read_body_ext (src/import_cfg.cpp:213-330): a mass line, a
+// symbol line, then rows of entry_count values
+Body read_body_ext(std::istream& in, const Header& h, const std::string& file)
+{
+    Body b;
+    b.naux = h.entry_count - 3;
+    std::string raw;
+    std::vector<std::string> tk = h.first;
+    bool mass_flag = false, elem_init = false, first = true;
     double mass = 0.0;
-    bool first = true;
+    int ielem = 0;
     while (true) {
         if (!first) {
             if (!std::getline(in, raw)) break;
@@ -220,7 +259,7 @@ CfgDMD read_cfg_dmd(const char* path)
         }
         first = false;
         if (tk.empty()) continue;
-        MAPP_REQUIRE(tk.size() == 1 || (int)tk.size() == entry_count, ERR_FILE,
+        MAPP_REQUIRE(tk.size() == 1 || (int)tk.size() == h.entry_count, ERR_FILE,
                      "invalid line in " + file + ": " + join(tk));
         MAPP_REQUIRE(!(mass_flag && tk.size() != 1), ERR_FILE,
                      "expected chemical symbol of element after mass in " + file);
@@ -230,29 +269,178 @@ CfgDMD read_cfg_dmd(const char* path)
                 MAPP_REQUIRE(mass > 0.0, ERR_FILE, "mass in " + file + " should be greater than 0.0");
                 mass_flag = true;
             } else {
-                bool found = false;
-                for (const std::string& nm : out.names) found = found || nm == tk[0];
-                if (!found) {
-                    out.names.push_back(tk[0]);
-                    out.masses.push_back(mass);
-                }
+                ielem = add_type(b, mass, tk[0], file);
                 mass_flag = false;
                 elem_init = true;
             }
             continue;
         }
         MAPP_REQUIRE(elem_init, ERR_FILE, "a site line comes before any element in " + file);
-        for (int d = 0; d < 3; ++d) {
-            double v = std::atof(tk[d].c_str());
-            while (v >= 1.0) v--;
-            while (v < 0.0) v++;
-            s.push_back(v);
-        }
-        for (int k = 3; k < entry_count; ++k) aux.push_back(std::atof(tk[k].c_str()));
+        push_s(b, tk, 0);
+        b.elem.push_back((uint8_t)ielem);
+        for (int k = 3; k < h.entry_count; ++k) b.aux.push_back(std::atof(tk[k].c_str()));
     }
-    const int nread = (int)(s.size() / 3);
+    return b;
+}
+
+// ImportCFG::read_body_std (src/import_cfg.cpp:334-480): every row is mass,
+// symbol, three fractional coordinates and three reduced velocities
+Body read_body_std(std::istream& in, const Header& h, const std::string& file)
+{
+    Body b;
+    b.naux = 3;
+    std::string raw;
+    std::vector<std::string> tk = h.first;
+    bool first = true;
+    while (true) {
+        if (!first) {
+            if (!std::getline(in, raw)) break;
+            tk = tokens(raw);
+        }
+        first = false;
+        if (tk.empty()) continue;
+        MAPP_REQUIRE(tk.size() == 8, ERR_FILE, "invalid line in " + file + ": " + join(tk));
+        const double mass = std::atof(tk[0].c_str());
+        MAPP_REQUIRE(mass > 0.0, ERR_FILE, "mass in " + file + " should be greater than 0.0");
+        const int ielem = add_type(b, mass, tk[1], file);
+        push_s(b, tk, 2);
+        b.elem.push_back((uint8_t)ielem);
+        for (int k = 5; k < 8; ++k) b.aux.push_back(std::atof(tk[k].c_str()));
+    }
+    return b;
+}
+
+void check_count(const Body& b, int natms, const std::string& file)
+{
+    const int nread = (int)(b.s.size() / 3);
     MAPP_REQUIRE(nread == natms, ERR_FILE,
-                 "expected " + std::to_string(natms) + " sites in " + file + " but read " + std::to_string(nread));
+                 "expected " + std::to_string(natms) + " atoms in " + file + " but read " + std::to_string(nread));
+}
+
+// inverse of a lower-triangular box (s = x B, B lower triangular)
+void lower_inverse(const double* H, double* B)
+{
+    B[0] = 1.0 / H[0];
+    B[1] = B[2] = B[5] = 0.0;
+    B[4] = 1.0 / H[4];
+    B[8] = 1.0 / H[8];
+    B[3] = -H[3] * B[0] * B[4];
+    B[7] = -H[7] * B[4] * B[8];
+    B[6] = -(H[6] * B[0] + H[7] * B[3]) * B[8];
+}
+
+// x B of a lower-triangular B; wrapped into [0, 1) as Algebra::X2S does
+void x2s(const double* x, const double* B, bool wrap, double* s)
+{
+    s[0] = x[0] * B[0] + x[1] * B[3] + x[2] * B[6];
+    s[1] = x[1] * B[4] + x[2] * B[7];
+    s[2] = x[2] * B[8];
+    if (wrap)
+        for (int d = 0; d < 3; ++d) {
+            while (s[d] < 0.0) ++s[d];
+            while (s[d] >= 1.0) --s[d];
+        }
+}
+
+}  // namespace
+
+CfgMD read_cfg_md(const char* path)
+{
+    std::ifstream in(path);
+    MAPP_REQUIRE(in.good(), ERR_FILE, std::string("cannot open ") + path);
+    const std::string file(path);
+    const Header h = read_header(in, file);
+    CfgMD out;
+    std::memcpy(out.H, h.H, sizeof out.H);
+    out.natms = h.natms;
+    if (h.natms == 0) return out;
+    const Body b = h.is_ext ? read_body_ext(in, h, file) : read_body_std(in, h, file);
+    check_count(b, h.natms, file);
+    out.nelems = (int)b.names.size();
+    out.names = b.names;
+    out.masses = b.masses;
+    out.elem = b.elem;
+    out.has_vel = h.vel_xst;
+    out.x.resize((size_t)3 * h.natms);
+    if (out.has_vel) out.v.resize((size_t)3 * h.natms);
+    // s2x and x_d = s_d (R H), both with the lower-triangular box
+    for (int i = 0; i < h.natms; ++i) {
+        const double* si = b.s.data() + (size_t)3 * i;
+        for (int d = 0; d < 3; ++d)
+            out.x[(size_t)3 * i + d] = si[0] * h.H[0 + d] + si[1] * h.H[3 + d] + si[2] * h.H[6 + d];
+        if (out.has_vel) {
+            const double* vi = b.aux.data() + (size_t)b.naux * i;
+            for (int d = 0; d < 3; ++d)
+                out.v[(size_t)3 * i + d] =
+                    h.R * (vi[0] * h.H[0 + d] + vi[1] * h.H[3 + d] + vi[2] * h.H[6 + d]);
+        }
+    }
+    return out;
+}
+
+void write_cfg_md(const char* path, int natms, const double* H, const double* x, const uint8_t* elem,
+                  const double* v, int naux, const double* aux, const std::vector<std::string>& aux_names,
+                  int nelems, const double* masses, const std::vector<std::string>& names)
+{
+    MAPP_REQUIRE(nelems >= 1 && (int)names.size() == nelems, ERR_ARG, "one name and mass per element");
+    MAPP_REQUIRE(naux >= 0 && (int)aux_names.size() == naux, ERR_ARG, "one name per auxiliary column");
+    for (int i = 0; i < natms; ++i)
+        MAPP_REQUIRE(elem[i] < nelems, ERR_ARG, "element index out of range");
+    FILE* fp = std::fopen(path, "w");
+    MAPP_REQUIRE(fp != nullptr, ERR_FILE, std::string("cannot write ") + path);
+    // write_header (src/export_cfg_md.cpp:26-53)
+    std::fprintf(fp, "Number of particles = %d\n", natms);
+    std::fprintf(fp, "A = %.16lf Angstrom (basic length-scale)\n", 1.0);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) std::fprintf(fp, "H0(%d,%d) = %.16lf A\n", i + 1, j + 1, H[i * 3 + j]);
+    if (v)
+        std::fprintf(fp, "R = 1.0\n");
+    else
+        std::fprintf(fp, ".NO_VELOCITY.\n");
+    std::fprintf(fp, "entry_count = %d\n", 3 + (v ? 3 : 0) + naux);
+    for (int k = 0; k < naux; ++k) std::fprintf(fp, "auxiliary[%d] = %s [reduced unit]\n", k, aux_names[k].c_str());
+    // write_body_sort (src/export_cfg_md.cpp:57-99): a mass and symbol line
+    // whenever the element changes, then s, s_d = v B, the auxiliary columns
+    double B[9];
+    lower_inverse(H, B);
+    int curr = -1;
+    for (int i = 0; i < natms; ++i) {
+        if (elem[i] != curr) {
+            curr = elem[i];
+            std::fprintf(fp, "%lf\n", masses[curr]);
+            std::fprintf(fp, "%s\n", names[curr].c_str());
+        }
+        double s[3];
+        x2s(x + (size_t)3 * i, B, true, s);
+        std::fprintf(fp, "%.16lf %.16lf %.16lf ", s[0], s[1], s[2]);
+        if (v) {
+            x2s(v + (size_t)3 * i, B, false, s);
+            std::fprintf(fp, "%.16lf %.16lf %.16lf ", s[0], s[1], s[2]);
+        }
+        for (int k = 0; k < naux; ++k) std::fprintf(fp, "%.16lf ", aux[(size_t)naux * i + k]);
+        std::fprintf(fp, "\n");
+    }
+    const bool ok = std::fclose(fp) == 0;
+    MAPP_REQUIRE(ok, ERR_FILE, std::string("cannot write ") + path);
+}
+
+CfgDMD read_cfg_dmd(const char* path)
+{
+    std::ifstream in(path);
+    MAPP_REQUIRE(in.good(), ERR_FILE, std::string("cannot open ") + path);
+    const std::string file(path);
+    const Header h = read_header(in, file);
+    CfgDMD out;
+    std::memcpy(out.H, h.H, sizeof out.H);
+    out.natms = h.natms;
+    if (h.natms == 0) return out;
+    MAPP_REQUIRE(h.is_ext, ERR_FILE, "cannot read cfg standard file for dmd mode");
+    const Body b = read_body_ext(in, h, file);
+    check_count(b, h.natms, file);
+    const int natms = h.natms, naux = b.naux;
+    std::vector<double> aux = b.aux;
+    out.names = b.names;
+    out.masses = b.masses;
     // ImportCFGDMD::operator(): alpha and c of every element, checks, and
     // the present elements compacted into components
     const int ne = (int)out.names.size();
@@ -299,7 +487,7 @@ CfgDMD read_cfg_dmd(const char* path)
                 ++k;
             }
         // s2x with the lower-triangular box
-        const double* si = s.data() + (size_t)3 * i;
+        const double* si = b.s.data() + (size_t)3 * i;
         for (int d = 0; d < 3; ++d)
             out.x[(size_t)3 * i + d] = si[0] * out.H[0 + d] + si[1] * out.H[3 + d] + si[2] * out.H[6 + d];
     }
@@ -326,14 +514,8 @@ void write_cfg_dmd(const char* path, int natms, int c_dim, const double* H, cons
     // write_body (src/export_cfg_dmd.cpp:127-170): fractional positions, one
     // alpha (absent 0) and one c (absent -1) per element; a new element block
     // whenever the element of largest c changes
-    double B[9];  // inverse of the lower-triangular box
-    B[0] = 1.0 / H[0];
-    B[1] = B[2] = B[5] = 0.0;
-    B[4] = 1.0 / H[4];
-    B[8] = 1.0 / H[8];
-    B[3] = -H[3] * B[0] * B[4];
-    B[7] = -H[7] * B[4] * B[8];
-    B[6] = -(H[6] * B[0] + H[7] * B[3]) * B[8];
+    double B[9];
+    lower_inverse(H, B);
     std::vector<double> ad(nelems), cdv(nelems);
     auto dump = [&](int i) {
         for (int j = 0; j < nelems; ++j) {
@@ -372,12 +554,9 @@ void write_cfg_dmd(const char* path, int natms, int c_dim, const double* H, cons
             std::fprintf(fp, "%lf\n", masses[curr]);
             std::fprintf(fp, "%s\n", names[curr].c_str());
         }
-        const double* xi = x + (size_t)3 * i;
-        // x2s with the lower-triangular inverse (s = x B, B lower triangular)
-        const double s0 = xi[0] * B[0] + xi[1] * B[3] + xi[2] * B[6];
-        const double s1 = xi[1] * B[4] + xi[2] * B[7];
-        const double s2 = xi[2] * B[8];
-        std::fprintf(fp, "%.16lf %.16lf %.16lf ", s0, s1, s2);
+        double si[3];
+        x2s(x + (size_t)3 * i, B, true, si);
+        std::fprintf(fp, "%.16lf %.16lf %.16lf ", si[0], si[1], si[2]);
         for (int j = 0; j < nelems; ++j) std::fprintf(fp, "%.16lf ", ad[j]);
         for (int j = 0; j < nelems; ++j) std::fprintf(fp, "%.16lf ", cdv[j]);
         std::fprintf(fp, "\n");

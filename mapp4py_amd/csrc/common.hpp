@@ -87,6 +87,8 @@ enum KernelId : int {
     K_DMD_J,
     K_DMD_NEWTON,
     K_MD_MIN,
+    K_DMD_PREPJ,
+    K_DMD_JXA,
     K_COUNT
 };
 
@@ -96,7 +98,8 @@ inline const char* kernel_name(int k)
                                          "integrate",  "eam_energy", "dmd_pass1",  "dmd_pass2",
                                          "dmd_energy", "surr_infer", "surr_train", "gcmc",
                                          "dmd_cd",     "dmd_j_state", "dmd_j",
-                                         "dmd_newton", "md_min_vec"};
+                                         "dmd_newton", "md_min_vec", "dmd_prepj",
+                                         "dmd_jxa"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "";
 }
 

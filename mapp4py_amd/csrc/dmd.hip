@@ -1411,6 +1411,320 @@ __global__ __launch_bounds__(kBlock) void k_gmax(int n, int nc, const double4* _
     if (threadIdx.x == 0) part[(size_t)blockIdx.x * kPartW] = sh[0];
 }
 
+// ---------------- second derivatives over x and alpha: the Newton stage ----------------
+//
+// ForceFieldEAMDMD::__prepJ_n_res / __J (src/ff_eam_dmd.cpp:364-831) on the
+// owner-computes component lists. The prep pass evaluates every listed pair
+// once at the current state and keeps nine FP64 values per list slot
+// (slot-major, [q][k * ldl + component]); each Jacobian product then reads
+// them twice: the first pass with the densities' F'' fixed collects the
+// change of F'(rho) per component, the second applies it. Vectors over
+// (x, alpha) are flat in slot order: x of site s at [3 s, 3 s + 3), alpha of
+// component k at [3 n + k].
+
+constexpr int kXaQ = 9;  // fr, fa, frr, fra, faa, r2, a2, r1, a1 (see k_dmd_prepj)
+
+// prep pass: per list slot of a present component i and its entry j,
+//   fr  = D_r g0 + dE_i D_r g2 + dE_j D_r g1,   fa = the same with D_a,
+//   frr, fra, faa: the same with D_rr, D_ra, D_aa,
+//   r2 = D_r g2, a2 = D_a g2, r1 = D_r g1, a1 = D_a g1,
+// with D_r = (1/r) d/dr, D_a = (1/a) d/da of the smeared g0 = r phi,
+// g1 = r rho_ij, g2 = r rho_ji from the moments xi^0..xi^4 (the second
+// derivatives as src/ff_eam_dmd.cpp:484-492 writes them); zero beyond the cut.
+// It also writes F''(rho) per component (zero above rho_max), fcoef, and the
+// residual f = the gradient per unit c (as calc_mu's f and f_alpha), with
+// partial sums of f^2.
+__global__ __launch_bounds__(kBlock) void k_dmd_prepj(int n, const double4* __restrict__ x,
+                                                      const double* __restrict__ al, const double* __restrict__ cc,
+                                                      const int* __restrict__ el, const int* __restrict__ nn,
+                                                      const uint32_t* __restrict__ list, int ldl, DmdArgs A,
+                                                      BoxDev bd, const double* __restrict__ dE,
+                                                      double* __restrict__ slots, size_t lds,
+                                                      double* __restrict__ ddE, double* __restrict__ fc,
+                                                      double* __restrict__ f, double* __restrict__ part)
+{
+    const int blk = xcd_block();
+    const int i = blk * kBlock + threadIdx.x;
+    double acc[1] = {0.0};
+    if (i < n) {
+        const int C = A.C;
+        const double4 xi = x[i];
+        const double rho_max = A.rho_max;
+        double c_tot = 0.0;
+        for (int q = 0; q < C; ++q) {
+            const double v = cc[i * C + q];
+            if (v > 0.0) c_tot += v;
+        }
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        const size_t fa_off = (size_t)3 * n;
+        for (int ic = 0; ic < C; ++ic) {
+            const int ci_idx = i * C + ic;
+            const double ci = cc[ci_idx];
+            if (ci < 0.0) {
+                ddE[ci_idx] = 0.0;
+                fc[ci_idx] = 0.0;
+                f[fa_off + ci_idx] = 0.0;
+                continue;
+            }
+            const double fcoef = C == 1 ? 1.0 : (c_tot == 0.0 ? (ci == 0.0 ? 1.0 : 0.0) : ci / c_tot);
+            const double ai = al[ci_idx];
+            const int ei = el[ci_idx];
+            const double dEi = dE[ci_idx];
+            double rho = 0.0, gai = 0.0;
+            const int cnt = nn[ci_idx];
+            for (int k = 0; k < cnt; ++k) {
+                const size_t slot = (size_t)k * ldl + ci_idx;
+                const uint32_t e = list[slot];
+                const int jc = (int)(e & kIndexMask);
+                const int tt = ei * A.ne + el[jc];
+                const double aj = al[jc], cj = cc[jc], dEj = dE[jc];
+                const PairGeo pg = pair_geo(xi, x[jc / C], e >> kIndexBits, bd.h, ai, aj, A.xmax, A.cut[tt]);
+                double q[kXaQ] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                if (pg.live) {
+                    const double r = pg.r, a = pg.a;
+                    double H[3][5];
+#pragma unroll
+                    for (int t = 0; t < 3; ++t)
+#pragma unroll
+                        for (int m = 0; m < 5; ++m) H[t][m] = 0.0;
+                    for (int l = 0; l < A.N; ++l) {
+                        const double xl = A.xi[l];
+                        double p;
+                        const double u = r - xl * a;
+                        const int m = dknot(A, u, p);
+                        const double s = u < 0.0 ? -A.wi[l] : A.wi[l];
+                        const double4* row = A.tab + ((size_t)tt * A.nr + m) * 3;
+#pragma unroll
+                        for (int t = 0; t < 3; ++t) {
+                            double w = hv(row[t], p) * s;
+#pragma unroll
+                            for (int mm = 0; mm < 5; ++mm) {
+                                H[t][mm] += w;
+                                w *= xl;
+                            }
+                        }
+                    }
+                    const double r_inv = 1.0 / r, r2_inv = r_inv * r_inv, rsq = r * r;
+                    const double a_inv = 1.0 / a, a2_inv = a_inv * a_inv, a4_inv = a2_inv * a2_inv, asq = a * a;
+                    double d1[3], d2[3], d3[3], d4[3], d5[3], g0[3];
+#pragma unroll
+                    for (int t = 0; t < 3; ++t) {
+                        const double rp = kPiInSq * r_inv * H[t][0];
+                        const double dr = -r2_inv * (rp + 2.0 * kPiInSq * a_inv * H[t][1]);
+                        const double da = -a2_inv * (rp - 2.0 * kPiInSq * r_inv * H[t][2]);
+                        g0[t] = rp;
+                        d1[t] = dr;
+                        d2[t] = da;
+                        d3[t] = -r2_inv * (3.0 * dr - 2.0 * da);
+                        d4[t] = -r2_inv * a2_inv * (3.0 * rp + 3.0 * rsq * dr + asq * da + 4.0 * kPiInSq * a_inv * H[t][3]);
+                        d5[t] = -a4_inv * (3.0 * rp + 6.0 * asq * da - 4.0 * kPiInSq * r_inv * H[t][4]);
+                    }
+                    q[0] = d1[0] + d1[2] * dEi + d1[1] * dEj;
+                    q[1] = d2[0] + d2[2] * dEi + d2[1] * dEj;
+                    q[2] = d3[0] + d3[2] * dEi + d3[1] * dEj;
+                    q[3] = d4[0] + d4[2] * dEi + d4[1] * dEj;
+                    q[4] = d5[0] + d5[2] * dEi + d5[1] * dEj;
+                    q[5] = d1[2];
+                    q[6] = d2[2];
+                    q[7] = d1[1];
+                    q[8] = d2[1];
+                    rho += cj * g0[2];
+                    const double gp = q[0] * cj * fcoef;
+                    gx += gp * pg.dx;
+                    gy += gp * pg.dy;
+                    gz += gp * pg.dz;
+                    gai += ai * q[1] * cj;
+                }
+#pragma unroll
+                for (int t = 0; t < kXaQ; ++t) slots[(size_t)t * lds + slot] = q[t];
+            }
+            ddE[ci_idx] = rho > rho_max ? 0.0 : ddemb5(A, ei, rho);
+            fc[ci_idx] = fcoef;
+            const double fai = gai - 3.0 * A.kBT / ai;
+            f[fa_off + ci_idx] = fai;
+            acc[0] += fai * fai;
+        }
+        f[3 * (size_t)i] = gx;
+        f[3 * (size_t)i + 1] = gy;
+        f[3 * (size_t)i + 2] = gz;
+        acc[0] += gx * gx + gy * gy + gz * gz;
+    }
+    block_sum_store<1>(acc, part + (size_t)blk * kPartW);
+}
+
+// the periodic difference x_i - x_j of a list entry (as pair_geo)
+__device__ inline void xa_dx(const double4& xi, const double4& xj, uint32_t code, const double* __restrict__ h,
+                             double& dx, double& dy, double& dz)
+{
+#pragma clang fp contract(off)
+    dpair_dx(xi, xj, code, h, dx, dy, dz);
+}
+
+// J, first pass (src/ff_eam_dmd.cpp:691-756): the pair terms with F'(rho)
+// fixed and, per component, the change of F'(rho): dde_i = F''(rho_i)
+// sum_j c_j (A D_r g2 + B D_a g2), A = dx_ij . (dx_i - dx_j),
+// B = alpha_i dalpha_i + alpha_j dalpha_j
+__global__ __launch_bounds__(kBlock) void k_dmd_jxa1(int n, const double4* __restrict__ x,
+                                                     const double* __restrict__ al, const double* __restrict__ cc,
+                                                     const int* __restrict__ nn, const uint32_t* __restrict__ list,
+                                                     int ldl, int C, BoxDev bd, const double* __restrict__ slots,
+                                                     size_t lds, const double* __restrict__ ddE,
+                                                     const double* __restrict__ fc, const double* __restrict__ in,
+                                                     double* __restrict__ out, double* __restrict__ dde)
+{
+    const int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double4 xi = x[i];
+    const size_t fa_off = (size_t)3 * n;
+    const double dxi0 = in[3 * (size_t)i], dxi1 = in[3 * (size_t)i + 1], dxi2 = in[3 * (size_t)i + 2];
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    for (int ic = 0; ic < C; ++ic) {
+        const int ci_idx = i * C + ic;
+        const double ci = cc[ci_idx];
+        if (ci < 0.0) {
+            dde[ci_idx] = 0.0;
+            out[fa_off + ci_idx] = 0.0;
+            continue;
+        }
+        const double ai = al[ci_idx], dai = in[fa_off + ci_idx], fci = fc[ci_idx];
+        double oa = 0.0, dd = 0.0;
+        const int cnt = nn[ci_idx];
+        for (int k = 0; k < cnt; ++k) {
+            const size_t slot = (size_t)k * ldl + ci_idx;
+            const uint32_t e = list[slot];
+            const int jc = (int)(e & kIndexMask);
+            const int js = jc / C;
+            const double fr = slots[slot], fa = slots[lds + slot], frr = slots[2 * lds + slot];
+            const double fra = slots[3 * lds + slot], faa = slots[4 * lds + slot];
+            const double r2 = slots[5 * lds + slot], a2 = slots[6 * lds + slot];
+            double dx, dy, dz;
+            xa_dx(xi, x[js], e >> kIndexBits, bd.h, dx, dy, dz);
+            const double ex = dxi0 - in[3 * (size_t)js], ey = dxi1 - in[3 * (size_t)js + 1],
+                         ez = dxi2 - in[3 * (size_t)js + 2];
+            const double aj = al[jc], daj = in[fa_off + jc], cj = cc[jc];
+            const double Av = dx * ex + dy * ey + dz * ez;
+            const double Bv = ai * dai + aj * daj;
+            const double df = -(fra * Bv + frr * Av), fp = -fr;
+            const double pa0 = -(fra * Av + faa * Bv), pa1 = -fa;
+            dd += (Av * r2 + Bv * a2) * cj;
+            const double w = cj * fci;
+            ox += (df * dx + fp * ex) * w;
+            oy += (df * dy + fp * ey) * w;
+            oz += (df * dz + fp * ez) * w;
+            oa += cj * (pa0 * ai + pa1 * dai);
+        }
+        dde[ci_idx] = dd * ddE[ci_idx];
+        out[fa_off + ci_idx] = oa;
+    }
+    out[3 * (size_t)i] = ox;
+    out[3 * (size_t)i + 1] = oy;
+    out[3 * (size_t)i + 2] = oz;
+}
+
+// J, second pass (src/ff_eam_dmd.cpp:758-830): the pair terms of the change
+// of F'(rho), then -3 kT dalpha / alpha^2 per present component
+__global__ __launch_bounds__(kBlock) void k_dmd_jxa2(int n, const double4* __restrict__ x,
+                                                     const double* __restrict__ al, const double* __restrict__ cc,
+                                                     const int* __restrict__ nn, const uint32_t* __restrict__ list,
+                                                     int ldl, int C, BoxDev bd, double kBT,
+                                                     const double* __restrict__ slots, size_t lds,
+                                                     const double* __restrict__ fc, const double* __restrict__ dde,
+                                                     const double* __restrict__ in, double* __restrict__ out)
+{
+    const int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double4 xi = x[i];
+    const size_t fa_off = (size_t)3 * n;
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    for (int ic = 0; ic < C; ++ic) {
+        const int ci_idx = i * C + ic;
+        const double ci = cc[ci_idx];
+        if (ci < 0.0) continue;
+        const double ai = al[ci_idx], dai = in[fa_off + ci_idx], fci = fc[ci_idx], ddi = dde[ci_idx];
+        double oa = 0.0;
+        const int cnt = nn[ci_idx];
+        for (int k = 0; k < cnt; ++k) {
+            const size_t slot = (size_t)k * ldl + ci_idx;
+            const uint32_t e = list[slot];
+            const int jc = (int)(e & kIndexMask);
+            const double r2 = slots[5 * lds + slot], a2 = slots[6 * lds + slot];
+            const double r1 = slots[7 * lds + slot], a1 = slots[8 * lds + slot];
+            const double ddj = dde[jc], cj = cc[jc];
+            double dx, dy, dz;
+            xa_dx(xi, x[jc / C], e >> kIndexBits, bd.h, dx, dy, dz);
+            const double df = -(r2 * ddi + r1 * ddj) * cj * fci;
+            ox += df * dx;
+            oy += df * dy;
+            oz += df * dz;
+            oa += ai * -(a2 * ddi + a1 * ddj) * cj;
+        }
+        out[fa_off + ci_idx] += oa - 3.0 * kBT * dai / (ai * ai);
+    }
+    out[3 * (size_t)i] += ox;
+    out[3 * (size_t)i + 1] += oy;
+    out[3 * (size_t)i + 2] += oz;
+}
+
+// (x, alpha) in caller order -> one flat slot-order vector, and back
+__global__ __launch_bounds__(kBlock) void k_xa_scatter(int n, int C, const int* __restrict__ slot_of,
+                                                       const double* __restrict__ xh, const double* __restrict__ ah,
+                                                       double* __restrict__ v)
+{
+    const int id = blockIdx.x * kBlock + threadIdx.x;
+    if (id >= n) return;
+    const int s = slot_of[id];
+    for (int d = 0; d < 3; ++d) v[3 * (size_t)s + d] = xh[3 * (size_t)id + d];
+    for (int k = 0; k < C; ++k) v[(size_t)3 * n + (size_t)s * C + k] = ah[(size_t)id * C + k];
+}
+
+__global__ __launch_bounds__(kBlock) void k_xa_gather(int n, int C, const int* __restrict__ slot_of,
+                                                      const double* __restrict__ v, double* __restrict__ xh,
+                                                      double* __restrict__ ah)
+{
+    const int id = blockIdx.x * kBlock + threadIdx.x;
+    if (id >= n) return;
+    const int s = slot_of[id];
+    for (int d = 0; d < 3; ++d) xh[3 * (size_t)id + d] = v[3 * (size_t)s + d];
+    for (int k = 0; k < C; ++k) ah[(size_t)id * C + k] = v[(size_t)3 * n + (size_t)s * C + k];
+}
+
+// MinDMDHandler::max_alpha_lcl_alpha (src/min_cg_dmd.h:347-360) with
+// max_dalpha infinite: the largest r <= 1 with alpha + r h_alpha >= 0 for
+// every component, minimum per workgroup into part[blk * kPartW]
+__global__ __launch_bounds__(kBlock) void k_xa_limit(int nc, const double* __restrict__ al,
+                                                     const double* __restrict__ ha, double* __restrict__ part)
+{
+    __shared__ double sh[kBlock];
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    double r = 1.0;
+    if (k < nc) {
+        const double h = ha[k];
+        if (h < 0.0) r = fmin(-al[k] / h, r);
+    }
+    sh[threadIdx.x] = r;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)blockIdx.x * kPartW] = sh[0];
+}
+
+// x += t h_x, alpha += t h_alpha (the state in slot order)
+__global__ __launch_bounds__(kBlock) void k_xa_apply(int n, int nc, double t, const double* __restrict__ h,
+                                                     double4* __restrict__ x, double* __restrict__ al)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k < n) {
+        double4 v = x[k];
+        v.x += t * h[3 * (size_t)k];
+        v.y += t * h[3 * (size_t)k + 1];
+        v.z += t * h[3 * (size_t)k + 2];
+        x[k] = v;
+    }
+    if (k < nc) al[k] += t * h[(size_t)3 * n + k];
+}
+
 // host <-> slot order
 __global__ __launch_bounds__(kBlock) void k_dscatter(int n, int C, const double* __restrict__ xh,
                                                      const double* __restrict__ ah, const double* __restrict__ ch,
@@ -2109,6 +2423,91 @@ void DMDSystem::J_dev(const double* x, double* ax)
     prof_.end(stream_);
 }
 
+// __GMRES::solve and GMRES<V>::solve (src/gmres.h:40-60, src/new_min_vec.h:1129-1146)
+// from zero on device vectors of length nv: one classical Gram-Schmidt pass
+// per product, the Givens rotations and the back substitution on the host.
+// A(in, out) applies the operator; b is left unchanged. True when the
+// residual falls below tol within m products; norm = |y|.
+template <class Op>
+bool DMDSystem::gmres(GmresWork& w, size_t nv, int m, Op&& A, const double* b, double tol, double* x, double& norm)
+{
+    const int nb = (int)((nv + kBlock - 1) / kBlock);
+    w.Q.reserve((size_t)(m + 1) * nv);
+    w.v.reserve(nv);
+    w.red.reserve(2 * kPartW);
+    w.part.reserve((size_t)nb * kPartW + kPartW);
+    double* Q = w.Q.get();
+    double* part = w.part.get();
+    const size_t ld = nv;
+    const int len = (int)nv;
+    auto fetch = [&](int count, double* out) {
+        MAPP_HIP(hipMemcpyAsync(out, w.red.get(), sizeof(double) * count, hipMemcpyDeviceToHost, stream_));
+        MAPP_HIP(hipStreamSynchronize(stream_));
+    };
+    std::vector<double> H((size_t)m * (m + 1) / 2), b_hat(m + 1), cs(2 * (m + 1));
+    auto solve_y = [&](int nvec) {
+        GmY y{};
+        for (int i = nvec - 1; i > -1; --i) {
+            y.y[i] = b_hat[i];
+            for (int j = i + 1; j < nvec; ++j) y.y[i] -= H[(size_t)j * (j + 1) / 2 + i] * y.y[j];
+            y.y[i] /= H[(size_t)i * (i + 1) / 2 + i];
+        }
+        hipLaunchKernelGGL(k_gm_combine, dim3(nb), dim3(kBlock), 0, stream_, len, nvec, y, Q, ld, x);
+        double s2 = 0.0;
+        for (int k = 0; k < nvec; ++k) s2 += y.y[k] * y.y[k];
+        norm = std::sqrt(s2);
+    };
+    prof_.begin(K_DMD_NEWTON, stream_);
+    hipLaunchKernelGGL(k_gm_dots, dim3(nb), dim3(kBlock), 0, stream_, len, 1, b, b, ld, part);
+    reduce_rows(part, nb, 1, w.red.get());
+    prof_.end(stream_);
+    double bb;
+    fetch(1, &bb);
+    const double bn = std::sqrt(bb);
+    if (!(bn > 0.0)) {  // already a root: no step (the reference would divide by zero here)
+        hipLaunchKernelGGL(k_gm_scale, dim3(nb), dim3(kBlock), 0, stream_, len, 0.0, b, x);
+        norm = 0.0;
+        return true;
+    }
+    hipLaunchKernelGGL(k_gm_scale, dim3(nb), dim3(kBlock), 0, stream_, len, 1.0 / bn, b, Q);
+    b_hat[0] = bn;
+    for (int it = 0; it < m; ++it) {
+        const int iv = it + 1;
+        A(Q + (size_t)it * ld, w.v.get());
+        prof_.begin(K_DMD_NEWTON, stream_);
+        hipLaunchKernelGGL(k_gm_dots, dim3(nb), dim3(kBlock), 0, stream_, len, iv, w.v.get(), Q, ld, part);
+        reduce_rows(part, nb, iv, w.red.get());
+        hipLaunchKernelGGL(k_gm_orth, dim3(nb), dim3(kBlock), 0, stream_, len, iv, w.v.get(), Q, ld, w.red.get(),
+                           Q + (size_t)iv * ld, part);
+        reduce_rows(part, nb, 1, w.red.get() + kPartW);
+        prof_.end(stream_);
+        double hr[2 * kPartW];
+        fetch(2 * kPartW, hr);
+        double* h = H.data() + (size_t)it * (it + 1) / 2;
+        for (int j = 0; j < iv; ++j) h[j] = hr[j];
+        const double nrm = std::sqrt(hr[kPartW]);
+        hipLaunchKernelGGL(k_gm_scale, dim3(nb), dim3(kBlock), 0, stream_, len, 1.0 / nrm, Q + (size_t)iv * ld,
+                           Q + (size_t)iv * ld);
+        for (int i = 0; i < it; ++i) {
+            const double t = cs[2 * i] * h[i] - cs[2 * i + 1] * h[i + 1];
+            h[i + 1] = cs[2 * i + 1] * h[i] + cs[2 * i] * h[i + 1];
+            h[i] = t;
+        }
+        const double t = std::sqrt(h[it] * h[it] + nrm * nrm);
+        cs[2 * it] = h[it] / t;
+        cs[2 * it + 1] = -nrm / t;
+        h[it] = cs[2 * it] * h[it] - cs[2 * it + 1] * nrm;
+        b_hat[it + 1] = cs[2 * it + 1] * b_hat[it];
+        b_hat[it] *= cs[2 * it];
+        if (std::fabs(b_hat[it + 1]) < tol) {
+            solve_y(it + 1);
+            return true;
+        }
+    }
+    solve_y(m);
+    return false;
+}
+
 bool DMDSystem::newton(const double* r_crd, const double* zeta, double beta, const double* a_host, double a_tol,
                        double err_fac, int max_newton, int max_gmres, int* iters)
 {
@@ -2118,13 +2517,10 @@ bool DMDSystem::newton(const double* r_crd, const double* zeta, double beta, con
     MAPP_REQUIRE(!surr_, ERR_STATE, "the implicit step runs on the quadrature (no surrogate)");
     const int m = max_gmres;
     const int nb = (nc_ + kBlock - 1) / kBlock;
-    const size_t ld = nc_;
     nt_a_.reserve(nc_);
     nt_F_.reserve(nc_);
     nt_del_.reserve(nc_);
     nt_c0_.reserve(nc_);
-    nt_v_.reserve(nc_);
-    nt_Q_.reserve((size_t)(m + 1) * nc_);
     nt_red_.reserve(16);
     double* part = part_.get();
     {
@@ -2147,77 +2543,15 @@ bool DMDSystem::newton(const double* r_crd, const double* zeta, double beta, con
         MAPP_HIP(hipMemcpyAsync(out, nt_red_.get(), sizeof(double) * count, hipMemcpyDeviceToHost, stream_));
         MAPP_HIP(hipStreamSynchronize(stream_));
     };
-    // __GMRES::solve (src/gmres.h:40-60) of (I / beta - J) del = F from zero;
-    // the Krylov basis stays on the device, the Hessenberg columns on the host
-    std::vector<double> H((size_t)m * (m + 1) / 2), b_hat(m + 1), cs(2 * (m + 1));
-    auto solve_y = [&](int nv, double& norm) {
-        GmY y{};
-        for (int i = nv - 1; i > -1; --i) {
-            y.y[i] = b_hat[i];
-            for (int j = i + 1; j < nv; ++j) y.y[i] -= H[(size_t)j * (j + 1) / 2 + i] * y.y[j];
-            y.y[i] /= H[(size_t)i * (i + 1) / 2 + i];
-        }
-        hipLaunchKernelGGL(k_gm_combine, dim3(nb), dim3(kBlock), 0, stream_, nc_, nv, y, nt_Q_.get(), ld,
-                           nt_del_.get());
-        double s = 0.0;
-        for (int k = 0; k < nv; ++k) s += y.y[k] * y.y[k];
-        norm = std::sqrt(s);
-    };
-    auto gmres = [&](double tol, double& norm) -> bool {
-        double* Q = nt_Q_.get();
+    // (I / beta - J) applied to in: J x, then -J x + x / beta
+    auto op = [&](const double* in, double* out) {
+        J_prepare();
+        J_dev(in, out);
         prof_.begin(K_DMD_NEWTON, stream_);
-        hipLaunchKernelGGL(k_gm_dots, dim3(nb), dim3(kBlock), 0, stream_, nc_, 1, nt_F_.get(), nt_F_.get(), ld, part);
-        reduce_rows(part, nb, 1, nt_red_.get());
+        hipLaunchKernelGGL(k_newton_jac, dim3(nb), dim3(kBlock), 0, stream_, nc_, beta_inv, in, out);
         prof_.end(stream_);
-        double bb;
-        fetch(1, &bb);
-        const double bn = std::sqrt(bb);
-        if (!(bn > 0.0)) {  // already a root: no step (the reference would divide by zero here)
-            hipLaunchKernelGGL(k_gm_scale, dim3(nb), dim3(kBlock), 0, stream_, nc_, 0.0, nt_F_.get(), nt_del_.get());
-            norm = 0.0;
-            return true;
-        }
-        hipLaunchKernelGGL(k_gm_scale, dim3(nb), dim3(kBlock), 0, stream_, nc_, 1.0 / bn, nt_F_.get(), Q);
-        b_hat[0] = bn;
-        for (int it = 0; it < m; ++it) {
-            const int iv = it + 1;
-            J_prepare();
-            J_dev(Q + (size_t)it * ld, nt_v_.get());
-            prof_.begin(K_DMD_NEWTON, stream_);
-            hipLaunchKernelGGL(k_newton_jac, dim3(nb), dim3(kBlock), 0, stream_, nc_, beta_inv, Q + (size_t)it * ld,
-                               nt_v_.get());
-            hipLaunchKernelGGL(k_gm_dots, dim3(nb), dim3(kBlock), 0, stream_, nc_, iv, nt_v_.get(), Q, ld, part);
-            reduce_rows(part, nb, iv, nt_red_.get());
-            hipLaunchKernelGGL(k_gm_orth, dim3(nb), dim3(kBlock), 0, stream_, nc_, iv, nt_v_.get(), Q, ld,
-                               nt_red_.get(), Q + (size_t)iv * ld, part);
-            reduce_rows(part, nb, 1, nt_red_.get() + kPartW);
-            prof_.end(stream_);
-            double hr[2 * kPartW];
-            fetch(2 * kPartW, hr);
-            double* h = H.data() + (size_t)it * (it + 1) / 2;
-            for (int j = 0; j < iv; ++j) h[j] = hr[j];
-            const double nrm = std::sqrt(hr[kPartW]);
-            hipLaunchKernelGGL(k_gm_scale, dim3(nb), dim3(kBlock), 0, stream_, nc_, 1.0 / nrm, Q + (size_t)iv * ld,
-                               Q + (size_t)iv * ld);
-            for (int i = 0; i < it; ++i) {
-                const double t = cs[2 * i] * h[i] - cs[2 * i + 1] * h[i + 1];
-                h[i + 1] = cs[2 * i + 1] * h[i] + cs[2 * i] * h[i + 1];
-                h[i] = t;
-            }
-            const double t = std::sqrt(h[it] * h[it] + nrm * nrm);
-            cs[2 * it] = h[it] / t;
-            cs[2 * it + 1] = -nrm / t;
-            h[it] = cs[2 * it] * h[it] - cs[2 * it + 1] * nrm;
-            b_hat[it + 1] = cs[2 * it + 1] * b_hat[it];
-            b_hat[it] *= cs[2 * it];
-            if (std::fabs(b_hat[it + 1]) < tol) {
-                solve_y(it + 1, norm);
-                return true;
-            }
-        }
-        solve_y(m, norm);
-        return false;
     };
+    auto gmres_step = [&](double tol, double& norm) { gmres(nt_gm_, nc_, m, op, nt_F_.get(), tol, nt_del_.get(), norm); };
     // DAEImplicit::newton (src/dae_imp.cpp:146-206)
     const double res_tol = 0.005 * a_tol_sqrt / err_fac;
     const double denom = 0.1 * a_tol_sqrt / err_fac;
@@ -2226,7 +2560,7 @@ bool DMDSystem::newton(const double* r_crd, const double* zeta, double beta, con
     bool converge = false, diverge = false;
     F_norm();
     while (iter < max_newton && !converge && !diverge) {
-        gmres(res_tol, norm);
+        gmres_step(res_tol, norm);
         // update_c
         prof_.begin(K_DMD_NEWTON, stream_);
         hipLaunchKernelGGL(k_newton_r, dim3(nb), dim3(kBlock), 0, stream_, nc_, c_.get(), nt_del_.get(), part);
@@ -2260,6 +2594,124 @@ bool DMDSystem::newton(const double* r_crd, const double* zeta, double beta, con
     prof_.flush();
     if (iters) *iters = iter;
     return converge;
+}
+
+// the prep pass of ForceFieldEAMDMD::__prepJ_n_res at the current state (after
+// pass 1 for F'(rho)): slot values, F'' and fcoef per component and the
+// residual f in xa_f_; returns |f|
+double DMDSystem::xa_prepare()
+{
+    MAPP_REQUIRE(!surr_, ERR_STATE, "the second-derivative pass runs on the quadrature (no surrogate)");
+    pass1(false, false);
+    xa_lds_ = (size_t)maxn_ * ldl_;
+    xa_slots_.reserve(xa_lds_ * kXaQ);
+    xa_ddE_.reserve(nc_);
+    xa_fc_.reserve(nc_);
+    xa_dde_.reserve(nc_);
+    xa_f_.reserve((size_t)3 * n_ + nc_);
+    const int nb = grid_for(n_);
+    BoxDev bd;
+    std::memcpy(bd.h, box_.h, sizeof bd.h);
+    std::memcpy(bd.b, box_.b, sizeof bd.b);
+    prof_.begin(K_DMD_PREPJ, stream_);
+    hipLaunchKernelGGL(k_dmd_prepj, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(), el_.get(),
+                       nn_.get(), list_.get(), ldl_, A_, bd, dE_.get(), xa_slots_.get(), xa_lds_, xa_ddE_.get(),
+                       xa_fc_.get(), xa_f_.get(), part_.get());
+    reduce_rows(part_.get(), nb, 1, red_.get() + 48);
+    prof_.end(stream_);
+    double v = 0.0;
+    MAPP_HIP(hipMemcpyAsync(&v, red_.get() + 48, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    xa_ver_ = ver_;
+    return std::sqrt(v);
+}
+
+// J applied to a flat slot-order vector (the state of the last xa_prepare)
+void DMDSystem::Jxa_dev(const double* in, double* out)
+{
+    const int nb = grid_for(n_);
+    BoxDev bd;
+    std::memcpy(bd.h, box_.h, sizeof bd.h);
+    std::memcpy(bd.b, box_.b, sizeof bd.b);
+    prof_.begin(K_DMD_JXA, stream_);
+    hipLaunchKernelGGL(k_dmd_jxa1, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(), nn_.get(),
+                       list_.get(), ldl_, C_, bd, xa_slots_.get(), xa_lds_, xa_ddE_.get(), xa_fc_.get(), in, out,
+                       xa_dde_.get());
+    hipLaunchKernelGGL(k_dmd_jxa2, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(), nn_.get(),
+                       list_.get(), ldl_, C_, bd, A_.kBT, xa_slots_.get(), xa_lds_, xa_fc_.get(), xa_dde_.get(), in,
+                       out);
+    prof_.end(stream_);
+}
+
+void DMDSystem::Jxa(const double* dx, const double* da, double* jdx, double* jda, double* f, double* fa)
+{
+    update();
+    xa_prepare();
+    const size_t nv = (size_t)3 * n_ + nc_;
+    DevBuf<double> hx, ha, tin, tout;
+    hx.reserve((size_t)3 * n_);
+    ha.reserve(nc_);
+    tin.reserve(nv);
+    tout.reserve(nv);
+    const int ns = (n_ + kBlock - 1) / kBlock;
+    MAPP_HIP(hipMemcpyAsync(hx.get(), dx, sizeof(double) * 3 * n_, hipMemcpyHostToDevice, stream_));
+    MAPP_HIP(hipMemcpyAsync(ha.get(), da, sizeof(double) * nc_, hipMemcpyHostToDevice, stream_));
+    hipLaunchKernelGGL(k_xa_scatter, dim3(ns), dim3(kBlock), 0, stream_, n_, C_, slot_of_.get(), hx.get(), ha.get(),
+                       tin.get());
+    Jxa_dev(tin.get(), tout.get());
+    auto out = [&](const double* v, double* xh, double* ah) {
+        hipLaunchKernelGGL(k_xa_gather, dim3(ns), dim3(kBlock), 0, stream_, n_, C_, slot_of_.get(), v, hx.get(),
+                           ha.get());
+        if (xh) MAPP_HIP(hipMemcpyAsync(xh, hx.get(), sizeof(double) * 3 * n_, hipMemcpyDeviceToHost, stream_));
+        if (ah) MAPP_HIP(hipMemcpyAsync(ah, ha.get(), sizeof(double) * nc_, hipMemcpyDeviceToHost, stream_));
+        MAPP_HIP(hipStreamSynchronize(stream_));
+    };
+    out(tout.get(), jdx, jda);
+    if (f || fa) out(xa_f_.get(), f, fa);
+    prof_.flush();
+}
+
+// DAE::min_error_false after its CG stage (src/dae.cpp:294-325): while |f| >
+// a_tol sqrt(3 n + n c_dim) and fewer than max_newton steps, GMRES solves
+// J h = f from zero (tolerance 0.005 a_tol sqrt(...)); x and alpha take h
+// whole when no alpha would turn negative, else half the largest fraction
+// that keeps them non-negative; then the list update and f at the new state
+int DMDSystem::newton_xa(double a_tol, int max_newton, int max_gmres, double* res_out)
+{
+    MAPP_REQUIRE(a_tol > 0.0, ERR_ARG, "a_tol must be positive");
+    MAPP_REQUIRE(max_newton >= 0, ERR_ARG, "max_nnewton_iters must be >= 0");
+    MAPP_REQUIRE(max_gmres >= 1 && max_gmres <= kPartW, ERR_ARG, "max_ngmres_iters must be in [1, 8]");
+    const size_t nv = (size_t)3 * n_ + nc_;
+    const double tol_s = a_tol * std::sqrt((double)nv);
+    xa_h_.reserve(nv);
+    update();
+    double res = xa_prepare();
+    const int nbe = (nc_ + kBlock - 1) / kBlock;
+    const int nba = (std::max(n_, nc_) + kBlock - 1) / kBlock;
+    int istep = 0;
+    for (; istep < max_newton && res / tol_s > 1.0; ++istep) {
+        double norm;
+        gmres(xa_gm_, nv, max_gmres, [&](const double* in, double* out) { Jxa_dev(in, out); }, xa_f_.get(),
+              0.005 * tol_s, xa_h_.get(), norm);
+        prof_.begin(K_DMD_NEWTON, stream_);
+        hipLaunchKernelGGL(k_xa_limit, dim3(nbe), dim3(kBlock), 0, stream_, nc_, al_.get(),
+                           xa_h_.get() + (size_t)3 * n_, part_.get());
+        hipLaunchKernelGGL(k_extrema_rows, dim3(1), dim3(kBlock), 0, stream_, nbe, 0, 1, part_.get(),
+                           red_.get() + 52);
+        prof_.end(stream_);
+        double r = 1.0;
+        MAPP_HIP(hipMemcpyAsync(&r, red_.get() + 52, sizeof(double), hipMemcpyDeviceToHost, stream_));
+        MAPP_HIP(hipStreamSynchronize(stream_));
+        const double t = r == 1.0 ? 1.0 : 0.5 * r;
+        hipLaunchKernelGGL(k_xa_apply, dim3(nba), dim3(kBlock), 0, stream_, n_, nc_, t, xa_h_.get(), x_.get(),
+                           al_.get());
+        ++ver_;
+        update();
+        res = xa_prepare();
+    }
+    prof_.flush();
+    if (res_out) *res_out = res;
+    return istep;
 }
 
 DMDReport DMDSystem::derivative(bool with_c, const double* ext_mu)

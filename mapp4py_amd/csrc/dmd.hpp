@@ -114,6 +114,16 @@ public:
     // otherwise c is restored. iters = Newton steps as the reference counts them.
     bool newton(const double* r_crd, const double* zeta, double beta, const double* a, double a_tol,
                 double err_fac, int max_newton, int max_gmres, int* iters);
+    // ForceFieldEAMDMD::__prepJ_n_res + __J (src/ff_eam_dmd.cpp:364-831) at
+    // the current state and fixed c: (jdx, jda) = -(d f / d(x, alpha)) (dx, da)
+    // with f the gradient per unit c (calc_mu's f and f_alpha); caller order,
+    // dx / jdx / f [site][3], da / jda / fa [site][c_dim]; f and fa optional
+    void Jxa(const double* dx, const double* da, double* jdx, double* jda, double* f = nullptr,
+             double* fa = nullptr);
+    // the Newton stage of DAE::min_error_false (src/dae.cpp:294-325) over x
+    // and alpha at fixed c (max_gmres <= 8); returns the steps taken and, in
+    // res, |f| at the end
+    int newton_xa(double a_tol, int max_newton, int max_gmres, double* res);
     // entries the last J state holds: list entries and second-list entries
     void J_entries(int64_t* live, int64_t* exchange);
     int64_t half_pairs_count();
@@ -150,6 +160,14 @@ private:
     void j_reserve();
     void J_dev(const double* x, double* ax);
     JState jstate();
+    double xa_prepare();
+    void Jxa_dev(const double* in, double* out);
+    // device Krylov basis [m+1][nv], product vector and partial sums of one GMRES
+    struct GmresWork {
+        DevBuf<double> Q, v, red, part;
+    };
+    template <class Op>
+    bool gmres(GmresWork& w, size_t nv, int m, Op&& A, const double* b, double tol, double* x, double& norm);
 
     std::shared_ptr<EamForceField> ff_;
     int n_, C_, nc_;  // sites, components per site, total components
@@ -181,7 +199,15 @@ private:
     DevBuf<int> jcnt_, jcnt2_;
     DevBuf<double> jx_, ja1_, jb0_, jax_, jxs_;
     // implicit-step vectors (slot order) and the Krylov basis [m+1][nc]
-    DevBuf<double> nt_a_, nt_F_, nt_del_, nt_c0_, nt_v_, nt_Q_, nt_red_;
+    DevBuf<double> nt_a_, nt_F_, nt_del_, nt_c0_, nt_red_;
+    GmresWork nt_gm_;
+    // second-derivative state: kXaQ values per list slot ([q][slot]), F'' and
+    // fcoef per component, the change of F' of the last product, the
+    // residual and the Newton step over (x, alpha) (flat, slot order)
+    DevBuf<double> xa_slots_, xa_ddE_, xa_fc_, xa_dde_, xa_f_, xa_h_;
+    size_t xa_lds_ = 0;
+    int64_t xa_ver_ = -1;
+    GmresWork xa_gm_;
     std::vector<double> masses_;
     std::vector<uint8_t> elem_host_;  // [site][c_dim] in caller order
     DevBuf<uint32_t> keys_, keys_alt_;

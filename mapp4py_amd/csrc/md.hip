@@ -14,6 +14,7 @@
 #include <mutex>
 
 #include "md.hpp"
+#include "cfg.hpp"
 #include "kern.hpp"
 
 // Scratch rows of MDSystem::value_noalloc: the forces (float4, first
@@ -1125,6 +1126,7 @@ MDSystem::MDSystem(std::shared_ptr<EamForceField> ff, int natms, const double* H
     MAPP_REQUIRE(H[1] == 0.0 && H[2] == 0.0 && H[5] == 0.0, ERR_ARG, "H must be lower triangular");
     const int ne = ff_->tables().ne;
     for (int i = 0; i < natms; ++i) MAPP_REQUIRE(elem[i] < ne, ERR_ARG, "element index out of range");
+    elem_host_.assign(elem, elem + natms);
     MAPP_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     box_ = make_box(H);
     max_cut_ = ff_->tables().max_rc() + skin_;
@@ -1640,6 +1642,23 @@ void MDSystem::get_f(double* fh)
                        slot_of_.get(), tmp.get());
     MAPP_HIP(hipMemcpyAsync(fh, tmp.get(), sizeof(double) * 3 * n_, hipMemcpyDeviceToHost, stream_));
     sync();
+}
+
+void MDSystem::export_cfg(const char* path, int nelems, const double* masses, const std::vector<std::string>& names,
+                          bool with_vel)
+{
+    MAPP_REQUIRE(nelems >= ff_->tables().ne, ERR_ARG, "one name and mass per element of the force field");
+    std::vector<double> x((size_t)3 * n_), v;
+    get_x(x.data());
+    if (with_vel) {
+        v.resize((size_t)3 * n_);
+        get_v(v.data());
+    }
+    double H[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) H[i * 3 + j] = box_.H[i][j];
+    write_cfg_md(path, n_, H, x.data(), elem_host_.data(), with_vel ? v.data() : nullptr, 0, nullptr, {}, nelems,
+                 masses, names);
 }
 
 void MDSystem::get_rho(double* rh)

@@ -120,6 +120,10 @@ public:
     void get_v(double* v_host);
     void get_f(double* f_host);
     void get_rho(double* rho_host);
+    // ExportCFGMD of the current state (caller order, with or without the
+    // velocities); one name and mass per element of the force field
+    void export_cfg(const char* path, int nelems, const double* masses, const std::vector<std::string>& names,
+                    bool with_vel);
     // DynamicMD::update: returns true if the list was rebuilt
     bool update();
     void rebuild();
@@ -172,6 +176,7 @@ private:
     int n_;
     double skin_, max_cut_;
     Box box_;
+    std::vector<uint8_t> elem_host_;  // caller order
     CellGrid grid_{};
     hipStream_t stream_ = nullptr;
     int nrebuild_ = 0;

@@ -275,7 +275,7 @@ class OBdf:
         """DAEBDF::run (src/dae_bdf.cpp:103-250). Returns the number of steps."""
         self.t_cur, self.t_fin = 0.0, float(t_tot)
         if self.nreset and self.min_error:
-            self.min_error()
+            self._min_error()
         fe_err = prev_err = self._calc_err()
         self._reset()
         self.nconst_q = self.nconst_dt = 0
@@ -298,13 +298,23 @@ class OBdf:
             fe_err = self._calc_err()
             periodic = self.nreset and (istep + 1) % self.nreset == 0 and fe_err / self.a_tol > 1.0
             if self.nreset and self.min_error and (periodic or fe_err > 2.0 * prev_err):
-                self.min_error()
+                self._min_error()
                 fe_err = self._calc_err()
                 prev_err = fe_err
                 self._reset()
                 self.nconst_q = self.nconst_dt = 0
-                self.counts["err_mins"] += 1
             istep += 1
             if self.ntally and istep % self.ntally == 0:
                 self.tally.append((istep, self.t_cur, self.d.force()[3]))
+        # the last step's line and relaxation (src/dae_bdf.cpp:203-217)
+        if self.ntally and istep % self.ntally:
+            self.tally.append((istep, self.t_cur, self.d.force()[3]))
+        if self.nreset and self.min_error and istep % self.nreset:
+            self._min_error()
         return istep
+
+    def _min_error(self):
+        """the caller's DAE::min_error_false; it returns the Newton steps it took,
+        and a relaxation counts when there was one (src/dae.cpp:325)"""
+        if self.min_error():
+            self.counts["err_mins"] += 1

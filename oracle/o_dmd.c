@@ -184,12 +184,13 @@ void o_dmd_pairs(const o_dmd* d, int32_t* pi, int32_t* pj, int8_t* img)
 }
 
 /* new state between list builds: ghosts follow their owners */
+/* new positions, alpha and c; the ghosts and the list are rebuilt, so any
+ * displacement (also across the box) is allowed */
 void o_dmd_set_state(o_dmd* d, const double* x, const double* alpha, const double* c)
 {
     memcpy(d->alpha_lcl, alpha, sizeof(double) * d->n * d->c_dim);
     memcpy(d->c_lcl, c, sizeof(double) * d->n * d->c_dim);
-    o_ghosts_refresh(&d->g, &d->bx, x);
-    dmd_sync_ghost_state(d);
+    dmd_rebuild(d, x);
 }
 
 /* Smeared phi, rho_ij, rho_ji and their r/alpha derivatives by N-point
@@ -427,11 +428,12 @@ typedef struct {
     double* ddE;  /* [nall*c_dim] */
     double* mu;   /* [nall*c_dim] */
     double* cv;   /* [nall sites] */
+    double* rho;  /* [n*c_dim] local densities */
 } dmd_static;
 
 static void dmd_static_free(dmd_static* s)
 {
-    free(s->rp); free(s->dE); free(s->ddE); free(s->mu); free(s->cv);
+    free(s->rp); free(s->dE); free(s->ddE); free(s->mu); free(s->cv); free(s->rho);
 }
 
 static void dmd_static_build(o_dmd* d, dmd_static* s)
@@ -504,7 +506,7 @@ static void dmd_static_build(o_dmd* d, dmd_static* s)
             if (c[k * cd + ic] > 0.0) v -= c[k * cd + ic];
         s->cv[k] = v;
     }
-    free(rho);
+    s->rho = rho;
 }
 
 /* The exchange terms of one second-list pair (src/ff_eam_dmd.cpp:1397-1462):
@@ -936,4 +938,241 @@ void o_dmd_unit_grad(o_dmd* d, double* g, double* ga)
         if (c[i] >= 0.0) ga[i] -= 3.0 * d->kBT / al[i];
     free(fcoef);
     dmd_static_free(&s);
+}
+
+/* ---------------- second derivatives: the Newton stage over x and alpha ---------------- */
+
+/* 1 drops the second pass of __J (the d dE terms); 2 flips the sign of the
+ * (1/a d/da)^2 term; 0 is the restatement. Mutants exist to show that the
+ * finite-difference pin of o_dmd_Jxa fails on them. */
+int o_jxa_mutation = 0;
+
+/* smear with second derivatives (src/ff_eam_dmd.cpp:420-495): moments of
+ * xi^0..xi^4 over the N Gauss-Hermite points, then out[18] = rho_phi[3],
+ * (1/r) d/dr [3], (1/a) d/da [3], ((1/r) d/dr)^2 [3], (1/a) d/da (1/r) d/dr
+ * [3], ((1/a) d/da)^2 [3], each in the order phi(ei,ej), rho(ei,ej),
+ * rho(ej,ei). The derivatives are those of the continuous Gaussian
+ * convolution written through the moments, as the reference writes them. */
+static void smear2(const o_dmd* d, int ei, int ej, double r, double a, double* o)
+{
+    const o_eam* e = d->ff;
+    double dr_inv = 1.0 / e->dr;
+    double H[3][5] = {{0}};
+    for (int l = 0; l < d->N; l++) {
+        double xl = d->xi[l];
+        double u = r - xl * a;
+        size_t m;
+        double p;
+        o_spline_pm(fabs(u), dr_inv, e->nr, &m, &p);
+        const double* tabs[3] = {OPHI(e, ei, ej)[m], ORHO(e, ei, ej)[m], ORHO(e, ej, ei)[m]};
+        for (int k = 0; k < 3; k++) {
+            double t = hval(tabs[k], p);
+            if (u < 0.0) t *= -1.0;
+            t *= d->wi[l];
+            double w = 1.0;
+            for (int q = 0; q < 5; q++, w *= xl) H[k][q] += t * w;
+        }
+    }
+    double r_inv = 1.0 / r, r2_inv = r_inv * r_inv, rsq = r * r;
+    double a_inv = 1.0 / a, a2_inv = a_inv * a_inv, a4_inv = a2_inv * a2_inv, asq = a * a;
+    for (int k = 0; k < 3; k++) {
+        double rp = O_PI_IN_SQ * r_inv * H[k][0];
+        double dr = -r2_inv * (rp + 2.0 * O_PI_IN_SQ * a_inv * H[k][1]);
+        double da = -a2_inv * (rp - 2.0 * O_PI_IN_SQ * r_inv * H[k][2]);
+        o[k] = rp;
+        o[3 + k] = dr;
+        o[6 + k] = da;
+        o[9 + k] = -r2_inv * (3.0 * dr - 2.0 * da);
+        o[12 + k] = -r2_inv * a2_inv * (3.0 * rp + 3.0 * rsq * dr + asq * da + 4.0 * O_PI_IN_SQ * a_inv * H[k][3]);
+        o[15 + k] = -a4_inv * (3.0 * rp + 6.0 * asq * da - 4.0 * O_PI_IN_SQ * r_inv * H[k][4]);
+        if (o_jxa_mutation == 2) o[15 + k] = -o[15 + k];
+    }
+}
+
+void o_dmd_pair_eval2(const o_dmd* d, int ei, int ej, double r, double alpha_ij, double* out18)
+{
+    smear2(d, ei, ej, r, alpha_ij, out18);
+}
+
+/* fcoef of src/ff_eam_dmd.cpp:560-602: 1 for one component; else c_i over the
+ * sum of positive c of the site, or 1 for the zero components of an empty site */
+static void dmd_fcoef(const o_dmd* d, double* fcoef)
+{
+    int cd = d->c_dim;
+    const double* c = d->c;
+    for (int site = 0; site < d->n; site++) {
+        double ct = 0.0;
+        for (int q = 0; q < cd; q++)
+            if (c[site * cd + q] > 0.0) ct += c[site * cd + q];
+        for (int q = 0; q < cd; q++) {
+            int k = site * cd + q;
+            if (cd == 1) fcoef[k] = c[k] >= 0.0 ? 1.0 : 0.0;
+            else if (ct == 0.0) fcoef[k] = c[k] == 0.0 ? 1.0 : 0.0;
+            else fcoef[k] = c[k] >= 0.0 ? c[k] / ct : 0.0;
+        }
+    }
+}
+
+/* ForceFieldEAMDMD::__prepJ_n_res + __J (src/ff_eam_dmd.cpp:364-831) at the
+ * current state: (Jdx, Jda) = J (dx, da) with J = -d f / d(x, alpha) at fixed
+ * c, f the gradient per unit c of o_dmd_unit_grad. Over the half list: the
+ * pair terms with dE fixed and the change of dE (deltadE = F''(rho) drho)
+ * collected in the first pass, its pair terms in the second; the ghost
+ * copies of dx, da and deltadE follow their owners. dx [natms][3], da and
+ * Jda [natms*c_dim]. */
+void o_dmd_Jxa(o_dmd* d, const double* dx, const double* da, double* Jdx, double* Jda)
+{
+    int cd = d->c_dim, ne = d->ff->ne;
+    int n = d->n * cd, nall = d->g.nall * cd;
+    const double* x = d->g.x;
+    const double* c = d->c;
+    const double* al = d->alpha;
+    dmd_static s;
+    dmd_static_build(d, &s);
+    double* fcoef = (double*)calloc((size_t)n + 1, sizeof(double));
+    dmd_fcoef(d, fcoef);
+    double* dxa = (double*)calloc((size_t)d->g.nall * 3 + 1, sizeof(double));
+    double* daa = (double*)calloc((size_t)nall + 1, sizeof(double));
+    double* ddE = (double*)calloc((size_t)nall + 1, sizeof(double));
+    for (int k = 0; k < d->g.nall; k++) {
+        int o = d->g.owner[k];
+        for (int q = 0; q < 3; q++) dxa[3 * k + q] = dx[3 * o + q];
+        for (int ic = 0; ic < cd; ic++) daa[k * cd + ic] = da[o * cd + ic];
+    }
+    for (int k = 0; k < 3 * d->n; k++) Jdx[k] = 0.0;
+    for (int k = 0; k < n; k++) Jda[k] = 0.0;
+    double o[18];
+    /* prepJ_n_res's F'' is zero beyond rho_max (src/ff_eam_dmd.cpp:545-553) */
+    double rho_max = (double)d->ff->nrho * d->ff->drho;
+    for (int i = 0; i < n; i++)
+        if (c[i] >= 0.0 && s.rho[i] > rho_max) s.ddE[i] = 0.0;
+    for (int k = d->n; k < d->g.nall; k++)
+        for (int ic = 0; ic < cd; ic++) s.ddE[k * cd + ic] = s.ddE[d->g.src[k] * cd + ic];
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1) {
+            if (o_jxa_mutation == 1) break;
+            for (int k = d->n; k < d->g.nall; k++)
+                for (int ic = 0; ic < cd; ic++) ddE[k * cd + ic] = ddE[d->g.src[k] * cd + ic];
+        }
+        for (int i = 0; i < n; i++) {
+            if (c[i] < 0.0) continue;
+            int ei = d->elem[i];
+            const double* xi = x + 3 * (i / cd);
+            const double* dxi = dxa + 3 * (i / cd);
+            double ai = al[i], dai = daa[i];
+            for (int64_t k = d->row[i]; k < d->row[i + 1]; k++) {
+                int j = d->col[k];
+                int ej = d->elem[j];
+                const double* xj = x + 3 * (j / cd);
+                const double* dxj = dxa + 3 * (j / cd);
+                double r = sqrt(o_rsq(xi, xj));
+                double a = sqrt(ai * ai + al[j] * al[j]);
+                if (r - a * d->xi[d->N - 1] >= d->cut[ei * ne + ej]) continue;
+                smear2(d, ei, ej, r, a, o);
+                double v[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+                double dv[3] = {dxi[0] - dxj[0], dxi[1] - dxj[1], dxi[2] - dxj[2]};
+                double aj = al[j], daj = daa[j];
+                if (pass == 0) {
+                    double A = v[0] * dv[0] + v[1] * dv[1] + v[2] * dv[2];
+                    double B = ai * dai + aj * daj;
+                    double w[6];
+                    for (int q = 1; q < 6; q++) w[q] = o[3 * q] + o[3 * q + 2] * s.dE[i] + o[3 * q + 1] * s.dE[j];
+                    /* w[1] (1/r)d/dr, w[2] (1/a)d/da, w[3] rr, w[4] ra, w[5] aa */
+                    double pa0 = -(w[4] * A + w[5] * B), pa1 = -w[2];
+                    double df = -(w[4] * B + w[3] * A), fp = -w[1];
+                    ddE[i] += (A * o[3 + 2] + B * o[6 + 2]) * s.ddE[i] * c[j];
+                    for (int q = 0; q < 3; q++) Jdx[3 * (i / cd) + q] += (df * v[q] + fp * dv[q]) * c[j] * fcoef[i];
+                    Jda[i] += c[j] * (pa0 * ai + pa1 * dai);
+                    if (j < n) {
+                        ddE[j] += (A * o[3 + 1] + B * o[6 + 1]) * s.ddE[j] * c[i];
+                        for (int q = 0; q < 3; q++)
+                            Jdx[3 * (j / cd) + q] -= (df * v[q] + fp * dv[q]) * c[i] * fcoef[j];
+                        Jda[j] += c[i] * (pa0 * aj + pa1 * daj);
+                    }
+                } else {
+                    double df = -(o[3 + 2] * ddE[i] + o[3 + 1] * ddE[j]);
+                    double pa = -(o[6 + 2] * ddE[i] + o[6 + 1] * ddE[j]);
+                    for (int q = 0; q < 3; q++) Jdx[3 * (i / cd) + q] += df * c[j] * fcoef[i] * v[q];
+                    Jda[i] += ai * pa * c[j];
+                    if (j < n) {
+                        for (int q = 0; q < 3; q++) Jdx[3 * (j / cd) + q] -= df * c[i] * fcoef[j] * v[q];
+                        Jda[j] += aj * pa * c[i];
+                    }
+                }
+            }
+        }
+    }
+    for (int i = 0; i < n; i++)
+        if (c[i] >= 0.0) Jda[i] -= 3.0 * d->kBT * da[i] / (al[i] * al[i]);
+    free(fcoef); free(dxa); free(daa); free(ddE);
+    dmd_static_free(&s);
+}
+
+typedef struct { o_dmd* d; double *gx, *ga; } jxa_ctx;
+
+static void jxa_mv(void* v, const double* in, double* out)
+{
+    jxa_ctx* q = (jxa_ctx*)v;
+    int n3 = 3 * q->d->n;
+    o_dmd_Jxa(q->d, in, in + n3, out, out + n3);
+}
+
+/* the residual [3 natms + natms c_dim] and its norm */
+static double jxa_residual(o_dmd* d, double* f)
+{
+    int n3 = 3 * d->n, n = d->n * d->c_dim;
+    o_dmd_unit_grad(d, f, f + n3);
+    double s = 0.0;
+    for (int k = 0; k < n3 + n; k++) s += f[k] * f[k];
+    return sqrt(s);
+}
+
+/* The Newton stage of DAE::min_error_false (src/dae.cpp:294-325): while
+ * |f| > a_tol sqrt(3 natms + natms c_dim) and fewer than max_newton steps,
+ * GMRES (max_gmres products, tolerance 0.005 a_tol sqrt(...)) solves J h = f
+ * from zero; r is the largest fraction <= 1 keeping every alpha positive
+ * (MinDMDHandler::max_alpha_lcl_alpha, src/min_cg_dmd.h:347-360); x and
+ * alpha take h whole when r = 1, else r/2 h; then the list is rebuilt and f
+ * recomputed. Returns the steps taken; *res the final |f|. */
+int o_dmd_newton_xa(o_dmd* d, double a_tol, int max_newton, int max_gmres, double* res_out)
+{
+    int n3 = 3 * d->n, n = d->n * d->c_dim, nv = n3 + n;
+    double* f = (double*)calloc((size_t)nv + 1, sizeof(double));
+    double* b = (double*)calloc((size_t)nv + 1, sizeof(double));
+    double* h = (double*)calloc((size_t)nv + 1, sizeof(double));
+    double* xl = (double*)malloc(sizeof(double) * (n3 + 1));
+    o_gmres g;
+    gmres_init(&g, max_gmres, nv);
+    jxa_ctx ctx = {d, NULL, NULL};
+    double tol_s = a_tol * sqrt((double)nv);
+    double res = jxa_residual(d, f), norm;
+    int istep = 0;
+    for (; istep < max_newton && res / tol_s > 1.0; istep++) {
+        memcpy(b, f, sizeof(double) * nv);
+        gmres_solve(&g, jxa_mv, &ctx, b, 0.005 * tol_s, &norm, h);
+        double r = 1.0, hmax = 0.0;
+        for (int k = 0; k < n; k++) {
+            double ha = h[n3 + k];
+            if (ha < 0.0) r = fmin(-d->alpha_lcl[k] / ha, r);
+            hmax = fmax(hmax, fabs(ha));
+        }
+        r = fmin(r, fabs(INFINITY / hmax));
+        double st = r == 1.0 ? 1.0 : 0.5 * r;
+        memcpy(xl, d->g.x, sizeof(double) * n3);
+        for (int k = 0; k < n3; k++) xl[k] += st * h[k];
+        for (int k = 0; k < n; k++) d->alpha_lcl[k] += st * h[n3 + k];
+        dmd_rebuild(d, xl);
+        res = jxa_residual(d, f);
+    }
+    *res_out = res;
+    gmres_free(&g);
+    free(f); free(b); free(h); free(xl);
+    return istep;
+}
+
+/* the local positions [natms][3] (as the list holds them) and alpha [natms*c_dim] */
+void o_dmd_get_xa(const o_dmd* d, double* x, double* alpha)
+{
+    memcpy(x, d->g.x, sizeof(double) * 3 * d->n);
+    memcpy(alpha, d->alpha_lcl, sizeof(double) * d->n * d->c_dim);
 }

@@ -151,6 +151,20 @@ double o_dmd_c_d(o_dmd*, const double* r_crd, const double* zeta, const double* 
 void o_dmd_J(o_dmd*, const double* r_crd, const double* zeta, const double* masses, const double* x, double* Ax);
 /* ForceFieldEAMDMD::calc_mu's f and f_alpha: the gradient per unit c */
 void o_dmd_unit_grad(o_dmd*, double* g /* [natms][3] */, double* ga /* [natms*c_dim] */);
+/* smear with the second derivatives (src/ff_eam_dmd.cpp:420-495): out[18] =
+ * rho_phi[3], (1/r) d/dr [3], (1/a) d/da [3], ((1/r) d/dr)^2 [3],
+ * (1/a) d/da (1/r) d/dr [3], ((1/a) d/da)^2 [3] */
+void o_dmd_pair_eval2(const o_dmd*, int ei, int ej, double r, double alpha_ij, double* out18);
+/* ForceFieldEAMDMD::__prepJ_n_res + __J (src/ff_eam_dmd.cpp:364-831):
+ * (Jdx, Jda) = -(d f / d(x, alpha)) (dx, da) at fixed c, f the gradient per
+ * unit c (o_dmd_unit_grad) */
+void o_dmd_Jxa(o_dmd*, const double* dx, const double* da, double* Jdx, double* Jda);
+/* the Newton stage of DAE::min_error_false (src/dae.cpp:294-325); returns the
+ * steps taken, *res the final |f| */
+int o_dmd_newton_xa(o_dmd*, double a_tol, int max_newton, int max_gmres, double* res);
+extern int o_jxa_mutation;
+/* the local positions [natms][3] and alpha [natms*c_dim] */
+void o_dmd_get_xa(const o_dmd*, double* x, double* alpha);
 /* the local concentrations [natms][c_dim] */
 void o_dmd_get_c(const o_dmd*, double* c);
 /* __GMRES::solve (src/gmres.h:40-60) on a dense row-major A [n][n]: returns 1

@@ -77,6 +77,10 @@ def L():
             "o_dmd_J": (None, [C.c_void_p, dp, dp, dp, dp, dp]),
             "o_dmd_get_c": (None, [C.c_void_p, dp]),
             "o_dmd_unit_grad": (None, [C.c_void_p, dp, dp]),
+            "o_dmd_pair_eval2": (None, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, dp]),
+            "o_dmd_Jxa": (None, [C.c_void_p, dp, dp, dp, dp]),
+            "o_dmd_newton_xa": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, dp]),
+            "o_dmd_get_xa": (None, [C.c_void_p, dp, dp]),
             "o_eam_read_funcfl": (C.c_void_p, [C.POINTER(C.c_char_p), C.c_int, C.c_int, C.c_char_p, C.c_int]),
             "o_gmres_dense": (C.c_int, [C.c_int, C.c_int, dp, dp, C.c_double, dp, dp]),
             "o_dmd_newton": (C.c_int, [C.c_void_p, dp, dp, dp, C.c_double, dp, C.c_double, C.c_double, C.c_int,
@@ -121,7 +125,9 @@ class OEam:
         h = L().o_eam_read(str(path).encode(), fmt, int(dmd), err, 256)
         if not h:
             raise RuntimeError(err.value.decode())
-        return cls(h)
+        e = cls(h)
+        e.dmd = bool(dmd)
+        return e
 
     def info(self):
         """(ne, nr, nrho, dr, drho) of the tables."""
@@ -137,14 +143,18 @@ class OEam:
         h = L().o_eam_read_funcfl(arr, len(paths), int(dmd), err, 256)
         if not h:
             raise RuntimeError(err.value.decode())
-        return cls(h)
+        e = cls(h)
+        e.dmd = bool(dmd)
+        return e
 
     @classmethod
     def from_knots(cls, pot, dmd=False):
         F, rho, rphi = f64(pot.F), f64(pot.rho), f64(pot.r_phi)
         h = L().o_eam_from_knots(pot.ne, pot.nr, pot.nrho, pot.dr, pot.drho, P(F), P(rho), P(rphi),
                                  0 if pot.fs else 1, int(dmd))
-        return cls(h)
+        e = cls(h)
+        e.dmd = bool(dmd)
+        return e
 
     def cut(self):
         rc = np.zeros((self.ne, self.ne))
@@ -152,7 +162,11 @@ class OEam:
         return rc
 
     def coefs(self, which, a, b=0, ncols=4):
+        """spline rows; the F rows of DMD tables hold five coefficients, and the
+        buffer is sized from the tables, not from ncols alone"""
         n = self.nrho if which == 2 else self.nr
+        if which == 2 and getattr(self, "dmd", False):
+            ncols = 5
         out = np.zeros((n, ncols))
         L().o_eam_coefs(self.h, which, a, b, P(out))
         return out
@@ -252,6 +266,23 @@ class threads:
 
     def __exit__(self, *exc):
         L().o_set_threads(1)
+        return False
+
+
+class jxa_mutation:
+    """context manager planting one of the oracle's Jacobian mistakes (1: no
+    d dE pass, 2: the sign of the (1/a d/da)^2 term)"""
+
+    def __init__(self, k):
+        self.k = k
+
+    def __enter__(self):
+        self.var = C.c_int.in_dll(L(), "o_jxa_mutation")
+        self.var.value = self.k
+        return self
+
+    def __exit__(self, *exc):
+        self.var.value = 0
         return False
 
 
@@ -394,6 +425,32 @@ class ODMD:
         out = np.zeros(9)
         L().o_dmd_pair_eval(self.h, ei, ej, float(r), float(a), P(out))
         return out
+
+    def pair_eval2(self, ei, ej, r, a):
+        """[18]: rho_phi, (1/r)d/dr, (1/a)d/da, their second derivatives rr, ra, aa (3 functions each)."""
+        out = np.zeros(18)
+        L().o_dmd_pair_eval2(self.h, ei, ej, float(r), float(a), P(out))
+        return out
+
+    def Jxa(self, dx, da):
+        """__prepJ_n_res + __J: -(d f / d(x, alpha)) (dx [n][3], da [n][c_dim]) at fixed c."""
+        jx = np.zeros((self.n, 3))
+        ja = np.zeros((self.n, self.C))
+        L().o_dmd_Jxa(self.h, P(f64(np.asarray(dx).reshape(self.n, 3))), P(f64(np.asarray(da).reshape(self.n, self.C))),
+                      P(jx), P(ja))
+        return jx, ja
+
+    def newton_xa(self, a_tol, max_newton=5, max_gmres=5):
+        """the Newton stage of DAE::min_error_false: (steps, final |f|)."""
+        res = C.c_double()
+        it = L().o_dmd_newton_xa(self.h, float(a_tol), int(max_newton), int(max_gmres), C.byref(res))
+        return it, res.value
+
+    def state_xa(self):
+        x = np.zeros((self.n, 3))
+        a = np.zeros((self.n, self.C))
+        L().o_dmd_get_xa(self.h, P(x), P(a))
+        return x, a
 
     def c_d(self, r_crd, zeta, masses):
         """ForceFieldDMD::c_d_calc: (c_d [n][c_dim], max |c_d|)."""

@@ -33,13 +33,14 @@ def M():
     return mapp4py_amd
 
 
-def compare(g, o, label):
+def compare(g, o, label, dmd):
     ne, nr, nrho, _, _ = o.info()
     worst = 0.0
     for which in (0, 1, 2):
         for a in range(ne):
             for b in range(ne if which < 2 else 1):
-                ncols = o.coefs(2, 0).shape[1] if which == 2 else 4
+                # F rows hold the quartic's five coefficients in the DMD tables
+                ncols = 5 if (which == 2 and dmd) else 4
                 co = o.coefs(which, a, b, ncols=ncols)
                 cg = g.coefs(which, a, b, nrows=co.shape[0], ncols=ncols)
                 scale = np.maximum(np.abs(co).max(axis=1, keepdims=True), 1e-300)
@@ -52,12 +53,12 @@ def compare(g, o, label):
 @pytest.mark.parametrize("dmd", [False, True])
 def test_setfl_coefficients(M, tmp_path, dmd):
     path = unpack("Cu_mishin.eam.alloy.gz", tmp_path)
-    compare(M.ForceFieldEAM.setfl(path, dmd=dmd), O.OEam.read(path, 0, dmd=dmd), f"Cu_mishin setfl dmd={dmd}")
+    compare(M.ForceFieldEAM.setfl(path, dmd=dmd), O.OEam.read(path, 0, dmd=dmd), f"Cu_mishin setfl dmd={dmd}", dmd)
 
 
 def test_three_element_setfl_coefficients(M):
     path = os.path.join(ROOT, "mapp4py_amd", "data", "NiAlH_jea.eam.alloy")
-    compare(M.ForceFieldEAM.setfl(path), O.OEam.read(path, 0), "NiAlH_jea setfl")
+    compare(M.ForceFieldEAM.setfl(path), O.OEam.read(path, 0), "NiAlH_jea setfl", False)
 
 
 @pytest.mark.parametrize("dmd", [False, True])
@@ -65,8 +66,8 @@ def test_fs_coefficients(M, tmp_path, dmd):
     p = pot.johnson_feni_fs(nr=3000, nrho=3000)
     path = tmp_path / "feni.eam.fs"
     pot.write_fs(path, p)
-    compare(M.ForceFieldEAM.fs(str(path), dmd=dmd), O.OEam.read(path, 1, dmd=dmd), f"Fe-Ni fs file dmd={dmd}")
-    compare(M.ForceFieldEAM.from_knots(p, dmd=dmd), O.OEam.from_knots(p, dmd=dmd), f"Fe-Ni fs knots dmd={dmd}")
+    compare(M.ForceFieldEAM.fs(str(path), dmd=dmd), O.OEam.read(path, 1, dmd=dmd), f"Fe-Ni fs file dmd={dmd}", dmd)
+    compare(M.ForceFieldEAM.from_knots(p, dmd=dmd), O.OEam.from_knots(p, dmd=dmd), f"Fe-Ni fs knots dmd={dmd}", dmd)
 
 
 def test_funcfl_coefficients(M, tmp_path):
@@ -75,4 +76,4 @@ def test_funcfl_coefficients(M, tmp_path):
     paths = [tmp_path / "a.funcfl", tmp_path / "b.funcfl"]
     write(paths[0], Za, rhoa, Fa, 2800, 0.002, 3000, 0.02)
     write(paths[1], Zb, rhob, Fb, 2800, 0.002, 3000, 0.02)
-    compare(M.ForceFieldEAM.funcfl([str(p) for p in paths]), O.OEam.funcfl(paths), "funcfl two files")
+    compare(M.ForceFieldEAM.funcfl([str(p) for p in paths]), O.OEam.funcfl(paths), "funcfl two files", False)
