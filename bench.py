@@ -163,6 +163,120 @@ def cpu_baseline(seconds):
                       f"{allc['threads']} OpenMP threads per run"}
 
 
+def nst_run(md, n, steps=100, warmup=10):
+    """MDNST on the headline's 8,388,608 Cu atoms after their NVT run: all six
+    stress components driven to zero at 300 K, dt 1 fs, box relaxation times
+    1000 dt; the box changes every step and the list is rebuilt on the skin
+    flag only."""
+    import mapp4py_amd as M
+    from mapp4py_amd import potentials as pot
+
+    masses = [pot.MASS["Cu"]]
+    nst = M.NST(np.zeros((3, 3)), 300.0, 1.0 * pot.FS, kB=pot.KB_EV)
+    H0 = md.H.copy()
+    nst.run(md, warmup, masses)
+    md.profile(True)
+    t0 = time.perf_counter()
+    nst.run(md, steps, masses)
+    wall = time.perf_counter() - t0
+    kern = {k: md.profile_read(k) for k in ("eam_rho", "eam_force", "build_list", "integrate")}
+    md.profile(False)
+    h = nst.hist
+    vol = h[:, 2] * h[:, 4] * h[:, 7]
+    return {"workload": f"NST (MDNST), the headline's {n} Cu atoms after their NVT run, S = 0 on all six "
+                        f"components, 300 K, dt 1 fs, tau 1000 dt",
+            "metric": "NST atom-steps/s", "value": n * steps / wall, "unit": "atom-steps/s", "steps": steps,
+            "warmup": warmup, "ms_per_step": wall / steps * 1e3, "rebuilds": nst.rebuilds, "T_K": nst.T_part,
+            "box_change_A": float(np.abs(md.H - H0).max()), "relative_volume_change": float(vol[-1] / vol[0] - 1.0),
+            "stress_last_xx_yx_zx_yy_zy_zz": [float(t) for t in h[-1, 8:]],
+            "kernel_ms": {k: ms / max(c, 1) for k, (ms, c) in kern.items()},
+            "kernel_launches": {k: c for k, (ms, c) in kern.items()}, "dtype": "f32+f64"}
+
+
+def nst_cpu_baseline(seconds):
+    """the oracle's MDNST on 32,000 Cu atoms (fcc 20^3 cells), one thread"""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from mapp4py_amd import potentials as pot
+
+    p, x, H, elem, masses, v = build_workload(20)
+    S = np.zeros((3, 3))
+    o = O.OMD(O.OEam.from_knots(p), H, x, elem, skin=0.5)
+    t0 = time.perf_counter()
+    o.nst_run(v.copy(), masses, pot.KB_EV, 300.0, 1.0 * pot.FS, S, 3)
+    per = (time.perf_counter() - t0) / 3.0
+    steps = max(3, int(seconds / max(per, 1e-6)))
+    o = O.OMD(O.OEam.from_knots(p), H, x, elem, skin=0.5)
+    t0 = time.perf_counter()
+    o.nst_run(v.copy(), masses, pot.KB_EV, 300.0, 1.0 * pot.FS, S, steps)
+    dt = time.perf_counter() - t0
+    return {"value": len(x) * steps / dt, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle MDNST (FP64 C restatement), fcc Cu 20^3 cells = {len(x)} atoms, S = 0, {steps} steps, "
+                      f"1 thread, {dt:.1f} s"}
+
+
+def muvt_system():
+    """fcc Ni 14^3 cells (10,976 atoms, 0.05 A jitter) with H on 200 random
+    octahedral sites, NiAlH_jea EAM"""
+    from mapp4py_amd import potentials as pot
+
+    rng = np.random.default_rng(41)
+    a = 3.52
+    x, H = pot.fcc(a, (14, 14, 14))
+    x = x + rng.normal(0.0, 0.05, x.shape)
+    corners = np.argwhere(np.ones((14, 14, 14), bool)).astype(float) * a
+    octa = corners[rng.choice(len(corners), 200, replace=False)] + np.array([a / 2, 0.0, 0.0])
+    elem = np.concatenate([np.zeros(len(x), np.uint8), np.full(len(octa), 2, np.uint8)])
+    return np.concatenate([x, octa]), H, elem
+
+
+def muvt_run(steps=1000, nevery=100, nattempts=1000):
+    """MDMuVT (src/md_muvt.cpp): NVT of Ni-H at 300 K with nattempts exchange
+    attempts of H at mu = -2.3 eV every nevery steps"""
+    import mapp4py_amd as M
+    from mapp4py_amd import potentials as pot
+
+    x, H, elem = muvt_system()
+    masses = [pot.MASS["Ni"], pot.MASS["Al"], pot.MASS["H"]]
+    g = M.AtomsMD(M.ForceFieldEAM.setfl(pot.NIALH_JEA), H, x, elem, skin=0.5)
+    g.x_d = M.create_temp(elem, masses, 300.0, 43)
+    run = M.MuVT(-2.3, 300.0, pot.FS, 2, 2411895, nevery=nevery, nattempts=nattempts)
+    run.run(g, nevery, masses)  # warm-up: one exchange
+    g.profile(True)
+    t0 = time.perf_counter()
+    run.run(g, steps, masses)
+    wall = time.perf_counter() - t0
+    kern = {k: g.profile_read(k) for k in ("eam_rho", "eam_force", "build_list", "integrate")}
+    g.profile(False)
+    nx = steps // nevery
+    return {"workload": f"MDMuVT, fcc Ni 14^3 cells + 200 H (10,976 + 200 atoms), NiAlH_jea, mu = -2.3 eV, 300 K, "
+                        f"dt 1 fs, {nattempts} attempts every {nevery} steps",
+            "metric": "muVT atom-steps/s", "value": len(x) * steps / wall, "unit": "atom-steps/s", "steps": steps,
+            "ms_per_step": wall / steps * 1e3, "exchanges": nx, "attempts_per_s": nx * nattempts / wall,
+            "counts_ins_del_acc_ins_del_tried": [int(c) for c in run.counts], "natms_end": g.natms,
+            "gas_end": int(run.hist[-1, 1]), "T_K": run.T_part, "builds": run.builds,
+            "kernel_ms": {k: ms / max(c, 1) for k, (ms, c) in kern.items()}, "dtype": "f32+f64 (exchange f64)"}
+
+
+def muvt_cpu_baseline():
+    """the oracle's MDMuVT on the same system: 200 steps with two exchanges"""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from mapp4py_amd import potentials as pot
+
+    x, H, elem = muvt_system()
+    masses = [pot.MASS["Ni"], pot.MASS["Al"], pot.MASS["H"]]
+    v = O.o_create_T(elem, masses, pot.KB_EV, 300.0, 43)
+    o = O.OMD(O.OEam.read(pot.NIALH_JEA), H, x, elem, skin=0.5)
+    hp = 4.13566766225 * 0.1 * (1.60217656535 / 1.66053904020) ** 0.5
+    t0 = time.perf_counter()
+    o.muvt_run(v, masses, pot.KB_EV, hp, 300.0, pot.FS, -2.3, 2, 2411895, 100, 1000, 200)
+    dt = time.perf_counter() - t0
+    return {"value": len(x) * 200 / dt, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle MDMuVT on the same {len(x)} atoms, 200 steps with 2 exchanges of 1000 attempts, "
+                      f"1 thread, {dt:.1f} s"}
+
+
 def feni_run(steps=1000, warmup=20):
     """Binary Fe-Ni eam/fs alloy: bcc 64^3 x 2 = 1,048,576 atoms, Ni on a random
     20% of sites, Nr = Nrho = 5000, r_c = 5.3 A, NVT at 600 K and 1 fs with
@@ -285,6 +399,12 @@ def min_cpu_baseline(seconds):
 J_BYTES_PER_ENTRY = 32
 J_BYTES_PER_EXCHANGE = 28
 J_BYTES_PER_COMPONENT = 120
+# algorithmic bytes of one J(x, alpha) product of the second-derivative pass:
+# per list entry the index and seven slot values (first pass) and the index
+# and four slot values (second); per degree of freedom (3 per site + 1 per
+# component) the input read by both passes and the output written twice
+JXA_BYTES_PER_ENTRY = 60 + 36
+JXA_BYTES_PER_DOF = 32
 DMD_BYTES_PER_SITE = 132  # x 16, alpha 8, c 16 (read twice), f 12, f_alpha 8, mu 16, E/dE write + read 32
 
 
@@ -420,6 +540,20 @@ def dmd_run(evals=10):
     kj = d.profile_read("dmd_j")
     d.profile(False)
     j_entries, j_exch = d.J_entries()
+    # second-derivative products J(x, alpha) (__prepJ_n_res + __J) at that
+    # state: every call runs pass 1 and the slot pass, then one product
+    rng = np.random.default_rng(8)
+    dxa, daa = rng.normal(0.0, 0.01, (n, 3)), rng.normal(0.0, 1e-3, c.shape)
+    d.Jxa(dxa, daa)
+    d.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(evals):
+        d.Jxa(dxa, daa)
+    dt_xa = time.perf_counter() - t0
+    kprep, kxa = d.profile_read("dmd_prepj"), d.profile_read("dmd_jxa")
+    d.profile(False)
+    xa_ms = kxa[0] / max(kxa[1], 1)
+    xa_bytes = JXA_BYTES_PER_ENTRY * j_entries + JXA_BYTES_PER_DOF * (3 * n + c.size)
     # one implicit backward-Euler step of the concentrations (DAEImplicit::newton,
     # GMRES over J): c - c_n = dt c_d(c) from c_n, dt = 30 (time units of
     # the table: sqrt(amu/eV) A), then c restored for the CG leg
@@ -475,6 +609,13 @@ def dmd_run(evals=10):
                   "slot_state_ms": kjs[0] / max(kjs[1], 1),
                   "list_entries": j_entries, "exchange_entries": j_exch, "algorithmic_bytes": j_bytes,
                   "hbm_fraction": j_bytes / (kj[0] / kj[1] / 1e3) / HBM_PEAK},
+            "Jxa": {"metric": "second-derivative products J(x, alpha) site-evals/s, device kernels",
+                    "value": n / (xa_ms / 1e3), "product_ms": xa_ms,
+                    "slot_pass_ms": kprep[0] / max(kprep[1], 1),
+                    "host_copy_inclusive_ms": dt_xa / evals * 1e3,
+                    "list_entries": j_entries, "algorithmic_bytes": xa_bytes,
+                    "hbm_fraction": xa_bytes / (xa_ms / 1e3) / HBM_PEAK,
+                    "note": "each host call also runs pass 1 and the slot pass (prepJ_n_res) for its state"},
             "implicit_step": {"solver": "DAEImplicit::newton, GMRES(5) over __J, a_tol 1e-8, backward Euler dt 30",
                               "converged": nt_ok, "newton_iters": nt_it, "ms": dt_nt * 1e3, "max_abs_dc": nt_dc,
                               "site_steps_per_s": n / dt_nt,
@@ -607,15 +748,15 @@ def dmd_newton_cpu_baseline(seconds):
 def example_replay(cpu_seconds):
     """The reference's diffusion example up to its first printed line:
     examples/diff/configs/Cu-DMD.cfg (16,384 sites, one vacancy) through the
-    library's CFG importer, x and alpha relaxed at fixed c (CG on the GPU, then
-    Newton-Krylov on the GPU gradient), FE and S against step 0 of
-    examples/diff/output_cu."""
+    library's CFG importer, x and alpha relaxed at fixed c as DAE::min_error_false
+    does (CG on the GPU, then the library's Newton stage: first with the
+    example's a_tol 1e-7, 10 Newton steps and GMRES(4), then to a tight
+    tolerance), FE and S against step 0 of examples/diff/output_cu."""
     import gzip
     import shutil
     import tempfile
 
     import mapp4py_amd as M
-    from scipy.optimize import newton_krylov
 
     import math
 
@@ -648,27 +789,23 @@ def example_replay(cpu_seconds):
     t0 = time.perf_counter()
     r = g.min_cg(max_iter=2000, e_tol=0.0, grad_tol=1e-7)
     t_cg = time.perf_counter() - t0
-    xs, as_, cs = g.state()
-
-    def grad(v):
-        g.set_state(v[:3 * n].reshape(n, 3), v[3 * n:].reshape(n, 1), cs)
-        g.update()
-        f, fa, _, _, _, _ = g.derivative()
-        return -np.concatenate([f.ravel(), fa.ravel()])
-
     t0 = time.perf_counter()
-    v = newton_krylov(grad, np.concatenate([xs.ravel(), as_.ravel()]), f_tol=1e-9, maxiter=30, inner_maxiter=20,
-                      rdiff=1e-7)
-    t_nk = time.perf_counter() - t0
-    g.set_state(v[:3 * n].reshape(n, 3), v[3 * n:].reshape(n, 1), cs)
-    g.update()
+    k_ex, res_ex = g.min_newton(1e-7, 10, 4)
+    t_nx = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    k_t, res_t = g.min_newton(1e-11, 20, 8)
+    t_nt = time.perf_counter() - t0
     f, fa, _, fe, FH, _ = g.derivative()
     S = -np.asarray(FH) / np.linalg.det(d["H"])
     out = {"workload": "examples/diff Cu-DMD.cfg, 16,384 sites, Cu_mishin setfl, N=5, 300 K",
            "import_ms": t_imp * 1e3, "import_sites_per_s": n / t_imp,
            "gpu_evals_per_s": 1.0 / gpu_eval, "gpu_ms_per_eval": gpu_eval * 1e3,
            "cg": {"iters": r["iters"], "seconds": t_cg, "fe": r["fe"], "grad_max": r["grad_max"]},
-           "newton_krylov_seconds": t_nk, "fe": fe, "S_diag": [S[0], S[3], S[5]],
+           "newton_example_settings": {"a_tol": 1e-7, "max_nnewton_iters": 10, "max_ngmres_iters": 4,
+                                       "steps": k_ex, "residual": res_ex, "seconds": t_nx},
+           "newton_tight": {"a_tol": 1e-11, "max_nnewton_iters": 20, "max_ngmres_iters": 8, "steps": k_t,
+                            "residual": res_t, "seconds": t_nt},
+           "fe": fe, "S_diag": [S[0], S[3], S[5]],
            "max_grad": float(max(np.abs(f).max(), np.abs(fa).max())),
            "reference_output_step0": {"FE": gd["FE"], "S_diag": gd["S_diag"][0]},
            "abs_err_fe": abs(fe - gd["FE"]), "abs_err_S": float(np.abs(S[[0, 3, 5]] - gd["S_diag"][0]).max())}
@@ -791,6 +928,8 @@ def main():
     ap.add_argument("--no-gcmc", action="store_true", help="skip the batched GCMC measurement")
     ap.add_argument("--no-feni", action="store_true", help="skip the Fe-Ni (two-element) MD measurement")
     ap.add_argument("--no-min", action="store_true", help="skip the L-BFGS minimisation measurement")
+    ap.add_argument("--no-nst", action="store_true", help="skip the NST measurement")
+    ap.add_argument("--no-muvt", action="store_true", help="skip the muVT measurement")
     ap.add_argument("--dmd-evals-only", action="store_true",
                     help="only the DMD evaluations at 262,144 sites, FP64 and FP32 rows (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -842,7 +981,15 @@ def main():
         fp64 = headline_error(md, p, H, elem, host_threads())
         log(f"[bench] max|df| {fp64['max_abs_df_eV_per_A']:.2e} eV/A, |dE|/N {fp64['abs_dE_per_atom_eV']:.1e} eV "
             f"({fp64['seconds']:.1f} s)")
+    nst = None
+    if not args.no_nst:
+        log("[bench] NST")
+        nst = nst_run(md, n)
+        log(f"[bench] NST {nst['value']:.3e} atom-steps/s ({nst['ms_per_step']:.3f} ms/step, "
+            f"{nst['rebuilds']} rebuilds, box moved {nst['box_change_A']:.2e} A)")
     del md
+    if nst is not None and not args.no_cpu:
+        nst["cpu_baseline"] = nst_cpu_baseline(args.cpu_seconds / 5)
     cpu = None
     if not args.no_cpu:
         log(f"[bench] cpu baseline (~{args.cpu_seconds:.0f} s of oracle work)")
@@ -856,6 +1003,14 @@ def main():
             mini["cpu_baseline"] = min_cpu_baseline(args.cpu_seconds / 3)
         log(f"[bench] L-BFGS {mini['value']:.3e} atom-steps/s ({mini['ms_per_step']:.3f} ms/step, "
             f"{mini['energy_evals']} energy evals, {mini['rebuilds']} rebuilds)")
+    muvt = None
+    if not args.no_muvt:
+        log("[bench] muVT")
+        muvt = muvt_run()
+        if not args.no_cpu:
+            muvt["cpu_baseline"] = muvt_cpu_baseline()
+        log(f"[bench] muVT {muvt['value']:.3e} atom-steps/s, {muvt['attempts_per_s']:.3e} attempts/s, "
+            f"counts {muvt['counts_ins_del_acc_ins_del_tried']}")
     if not args.no_feni:
         log("[bench] Fe-Ni")
         feni = feni_run()
@@ -889,7 +1044,7 @@ def main():
                                f"r_c=4.95 A, skin 0.5 A, dt 1 fs, 300 K", "natoms": n, "rebuilds": nvt.rebuilds,
                    "pe_per_atom_eV": out[0] / n, "T_K": nvt.T_part},
         "error_vs_fp64": fp64,
-        "roofline": roof, "cpu_baseline": cpu, "feni": feni, "minimize": mini, "dmd": dmd, "gcmc": gcmc,
+        "roofline": roof, "cpu_baseline": cpu, "feni": feni, "nst": nst, "muvt": muvt, "minimize": mini, "dmd": dmd, "gcmc": gcmc,
     }
     print(json.dumps(line), flush=True)
 

@@ -114,6 +114,47 @@ mapp_status mapp_md_list_stats(mapp_md* md, int* max_neighbors, int64_t* full_en
  * each step, neighbor rebuild on the skin criterion. masses[ne]. */
 mapp_status mapp_md_nvt_run(mapp_md* md, const double* masses, double kB, double T, double dt, int nsteps,
                             double* out7, double* T_part, int* rebuilds);
+/* MDNST::run (src/md_nst.cpp:376-460, pre_init and update_V_H at :291-351,
+ * the exponential updates of update_x_d__x__x_d at :80-153 with MDMath::calc
+ * at :843-1002): velocity Verlet under NHC(dt/2, 100 dt) with a lower-
+ * triangular box velocity driven toward the target stress S [3][3]
+ * (row-major, lower triangle read, NaN = a free component; at least one set)
+ * under NHC(dt/2, 1000 dt), tau [3][3] the box relaxation times (NULL:
+ * 1000 dt each). The reference box B_ref and volume are re-taken every nreset
+ * steps (0: never). The box changes every step; the list is rebuilt only when
+ * an atom has moved half the skin. masses per element; out7 = energy and F_H
+ * of the last force call, T_part, H [3][3] the final box (any may be NULL),
+ * hist [nsteps][14] (T_part, pe, H00, H10, H11, H20, H21, H22, then the
+ * stress -F_H / vol as xx, yx, zx, yy, zy, zz after each step; may be NULL).
+ * The handle keeps the new box. */
+mapp_status mapp_md_nst_run(mapp_md* md, const double* masses, double kB, double T, double dt, const double* S,
+                            const double* tau, int nreset, int nsteps, double* out7, double* T_part, double* H,
+                            double* hist, int* rebuilds);
+/* MDMuVT::run (src/md_muvt.cpp:122-280, the run without fixed degrees of
+ * freedom): MDNVT's velocity Verlet under NHC(dt/2, 100 dt), where after every
+ * nevery-th first half step the exchange gcmc.xchng(false, nattempts) takes
+ * the place of the skin check: nattempts insertion/deletion attempts of
+ * element gas_elem at chemical potential mu (eV), temperature T, with the
+ * Planck constant hP in the table's units; the chain's Random(seed) stream
+ * runs across the exchanges of one call. At one process PGCMC, which MDMuVT
+ * uses, draws and decides as SGCMC (src/pgcmc.cpp:562-735,1163-1266,
+ * src/sgcmc.cpp:251-444). Each exchange wraps the atoms into the box, leaves
+ * them in the reference's order (an inserted atom appended, the last atom
+ * moved into a deleted one's place; velocities travel with them, an inserted
+ * atom's drawn by the chain) and rebuilds the list; ndof_part = 3 natms. The
+ * system may hold at most 16,384 atoms. masses per element; out7 = energy and
+ * F_H of the last force call; counts[4] = accepted insertions, accepted
+ * deletions, insertion trials, deletion trials; hist [nsteps][2] = T_part and
+ * gas atoms after each step (any may be NULL); builds = list builds
+ * (rebuilds and exchanges). The handle keeps the new atoms: read their count
+ * with mapp_md_natms, then positions, velocities and elements. */
+mapp_status mapp_md_muvt_run(mapp_md* md, const double* masses, double kB, double hP, double T, double dt, double mu,
+                             int gas_elem, int seed, int nevery, int nattempts, int nsteps, double* out7,
+                             double* T_part, int* counts, double* hist, int* builds);
+/* the handle's atom count (it changes under mapp_md_muvt_run) */
+mapp_status mapp_md_natms(mapp_md* md, int* natms);
+/* element index per atom, caller order (AtomsMD's elem vector) */
+mapp_status mapp_md_get_elem(mapp_md* md, uint8_t* elem);
 /* Energy minimisation of the positions: MinCG::__run (src/min_cg.h:453-533,
  * lbfgs = 0) or MinLBFGS::__run (src/min_l-bfgs.h:114-217, lbfgs = 1, memory
  * m) with MinMDHandler::ls_prep (src/min_cg.h:231-257: no step component

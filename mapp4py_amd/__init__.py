@@ -24,7 +24,7 @@ import numpy as np
 
 from ._lib import MappError, check, f64, fp, i8p, i32p, i64p, ip, lib, ptr, u8p
 
-__all__ = ["ForceFieldEAM", "AtomsMD", "AtomsDMD", "NVT", "MinCG", "MinLBFGS", "LineSearchBackTrack",
+__all__ = ["ForceFieldEAM", "AtomsMD", "AtomsDMD", "NVT", "NST", "MuVT", "MinCG", "MinLBFGS", "LineSearchBackTrack",
            "LineSearchGoldenSection", "LineSearchBrent", "GCMCBatch", "Surrogate", "MappError", "create_temp",
            "gauss_hermite", "lib"]
 
@@ -109,6 +109,15 @@ class AtomsMD:
     @property
     def vol(self):
         return float(self.H[0, 0] * self.H[1, 1] * self.H[2, 2])
+
+    def _refresh_atoms(self):
+        """the atom count and elements after a run that exchanges atoms"""
+        n = C.c_int()
+        check(lib().mapp_md_natms(self._h, C.byref(n)))
+        self.natms = n.value
+        el = np.zeros(self.natms, np.uint8)
+        check(lib().mapp_md_get_elem(self._h, el.ctypes.data_as(C.c_void_p)))
+        self.elem = el
 
     @property
     def x(self):
@@ -231,6 +240,81 @@ class NVT:
                                     C.byref(Tp), C.byref(nr)))
         self.T_part = Tp.value
         self.rebuilds = nr.value
+        atoms.pe = out[0]
+        atoms.F_H = out[1:].copy()
+        return out
+
+
+class NST:
+    """md.nst(S, T, dt): NVT with a box driven toward the target stress S
+    (3x3, NaN for free components; src/md_nst.cpp). tau: box relaxation times
+    (3x3, default 1000 dt); nreset: steps between re-takes of the reference
+    box (0: never)."""
+
+    def __init__(self, S, T, dt, kB=8.617330350e-5, tau=None, nreset=0):
+        self.S = np.array(S, dtype=np.float64).reshape(3, 3)
+        self.T, self.dt, self.kB = float(T), float(dt), float(kB)
+        self.tau = None if tau is None else np.array(tau, dtype=np.float64).reshape(3, 3)
+        self.nreset = int(nreset)
+        self.T_part = None
+        self.rebuilds = 0
+        self.hist = None
+
+    def run(self, atoms: AtomsMD, nsteps, masses):
+        masses = f64(masses)
+        out = np.zeros(7)
+        H = np.zeros((3, 3))
+        hist = np.zeros((max(int(nsteps), 1), 14))
+        Tp = C.c_double()
+        nr = C.c_int()
+        tau = f64(self.tau) if self.tau is not None else None
+        check(lib().mapp_md_nst_run(atoms._h, ptr(masses), self.kB, self.T, self.dt, ptr(f64(self.S)),
+                                    ptr(tau) if tau is not None else None, self.nreset, int(nsteps), ptr(out),
+                                    C.byref(Tp), ptr(H), ptr(hist), C.byref(nr)))
+        self.T_part = Tp.value
+        self.rebuilds = nr.value
+        self.hist = hist[:int(nsteps)]
+        atoms.H = H
+        atoms.pe = out[0]
+        atoms.F_H = out[1:].copy()
+        return out
+
+
+class MuVT:
+    """md.muvt(mu, T, dt, gas_elem, seed): NVT whose gas element is exchanged
+    with a reservoir at chemical potential mu every nevery steps (nattempts
+    insertion/deletion attempts each; src/md_muvt.cpp). gas is the element's
+    index in the force field; hP the Planck constant in the table's units
+    (eV, A, amu: 4.13566766225e-1 * sqrt(1.60217656535 / 1.66053904020))."""
+
+    HP = 4.13566766225 * 0.1 * (1.60217656535 / 1.66053904020) ** 0.5
+
+    def __init__(self, mu, T, dt, gas, seed, nevery=1000, nattempts=1000, kB=8.617330350e-5, hP=None):
+        self.mu, self.T, self.dt = float(mu), float(T), float(dt)
+        self.gas, self.seed = int(gas), int(seed)
+        self.nevery, self.nattempts = int(nevery), int(nattempts)
+        self.kB = float(kB)
+        self.hP = float(hP) if hP is not None else self.HP
+        self.T_part = None
+        self.counts = None
+        self.hist = None
+        self.builds = 0
+
+    def run(self, atoms: AtomsMD, nsteps, masses):
+        masses = f64(masses)
+        out = np.zeros(7)
+        Tp = C.c_double()
+        counts = np.zeros(4, np.int32)
+        hist = np.zeros((max(int(nsteps), 1), 2))
+        nb = C.c_int()
+        check(lib().mapp_md_muvt_run(atoms._h, ptr(masses), self.kB, self.hP, self.T, self.dt, self.mu, self.gas,
+                                     self.seed, self.nevery, self.nattempts, int(nsteps), ptr(out), C.byref(Tp),
+                                     counts.ctypes.data_as(C.POINTER(C.c_int)), ptr(hist), C.byref(nb)))
+        self.T_part = Tp.value
+        self.counts = counts
+        self.hist = hist[:int(nsteps)]
+        self.builds = nb.value
+        atoms._refresh_atoms()
         atoms.pe = out[0]
         atoms.F_H = out[1:].copy()
         return out

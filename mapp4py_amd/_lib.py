@@ -53,6 +53,12 @@ SIGNATURES = {
     "mapp_md_pairs": (C.c_int, [C.c_void_p, i32p, i32p, i8p]),
     "mapp_md_list_stats": (C.c_int, [C.c_void_p, ip, i64p, ip]),
     "mapp_md_nvt_run": (C.c_int, [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, ip]),
+    "mapp_md_nst_run": (C.c_int, [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, dp, C.c_void_p, C.c_int,
+                                  C.c_int, dp, dp, dp, C.c_void_p, ip]),
+    "mapp_md_muvt_run": (C.c_int, [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, ip, dp, ip]),
+    "mapp_md_natms": (C.c_int, [C.c_void_p, ip]),
+    "mapp_md_get_elem": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mapp_md_minimize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, dp, C.c_int, ip]),
     "mapp_dmd_min_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                    C.c_double, C.c_void_p, ip, ip, dp]),

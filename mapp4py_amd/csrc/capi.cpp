@@ -274,6 +274,54 @@ mapp_status mapp_md_nvt_run(mapp_md* md, const double* masses, double kB, double
     });
 }
 
+mapp_status mapp_md_nst_run(mapp_md* md, const double* masses, double kB, double T, double dt, const double* S,
+                            const double* tau, int nreset, int nsteps, double* out7, double* T_part, double* H,
+                            double* hist, int* rebuilds)
+{
+    return guard([&] {
+        NEED(md);
+        NEED(masses);
+        NEED(S);
+        mapp::MDReport r{};
+        const int nr = md->sys->nst_run(masses, kB, T, dt, S, tau, nreset, nsteps, &r, T_part, H, hist);
+        if (out7) std::memcpy(out7, r.e, sizeof r.e);
+        if (rebuilds) *rebuilds = nr;
+    });
+}
+
+mapp_status mapp_md_muvt_run(mapp_md* md, const double* masses, double kB, double hP, double T, double dt, double mu,
+                             int gas_elem, int seed, int nevery, int nattempts, int nsteps, double* out7,
+                             double* T_part, int* counts, double* hist, int* builds)
+{
+    return guard([&] {
+        NEED(md);
+        NEED(masses);
+        mapp::MDReport r{};
+        const int nb = md->sys->muvt_run(masses, kB, hP, T, dt, mu, gas_elem, seed, nevery, nattempts, nsteps, &r,
+                                         T_part, counts, hist);
+        if (out7) std::memcpy(out7, r.e, sizeof r.e);
+        if (builds) *builds = nb;
+    });
+}
+
+mapp_status mapp_md_natms(mapp_md* md, int* natms)
+{
+    return guard([&] {
+        NEED(md);
+        NEED(natms);
+        *natms = md->sys->natms();
+    });
+}
+
+mapp_status mapp_md_get_elem(mapp_md* md, uint8_t* elem)
+{
+    return guard([&] {
+        NEED(md);
+        NEED(elem);
+        md->sys->get_elem(elem);
+    });
+}
+
 mapp_status mapp_md_minimize(mapp_md* md, const mapp_min_params* params, mapp_min_report* report, double* tally,
                              int max_tally_rows, int* tally_rows)
 {

@@ -455,34 +455,47 @@ __global__ void k_gcmc_reset(int* queue)
 // init_xchng: rho_i over every periodic image of every atom within the
 // cutoff, then F(rho_i) without the extension past rho_max. One workgroup
 // per cell.
+__device__ inline void init_atom(const GcmcArgs& A, const GcmcCell& hd, const Arrays& R, int i)
+{
+    const int off = hd.off, n = hd.n, ne = A.ne, nr = A.nr;
+    const double4 si4 = R.s[off + i];
+    const double si[3] = {si4.x, si4.y, si4.z};
+    const int ei = (int)R.xe[off + i].w;
+    const Images im = count_images(si, hd.lo2, hd.hi2);
+    double rho = 0.0;
+    for (int k = 0; k < im.total(); ++k) {
+        double xi[3];
+        image_x(hd.h, si, im, k, xi);
+        for (int j = 0; j < n; ++j) {
+            if (k == 0 && j == i) continue;
+            const double4 xj = R.xe[off + j];
+            const int ej = (int)xj.w;
+            const double rsq = rsq_of(xi[0], xi[1], xi[2], xj.x, xj.y, xj.z);
+            if (rsq >= A.cut_sq[ei * ne + ej]) continue;
+            int m;
+            double p;
+            spline_at(sqrt(rsq), A.dr_inv, nr, m, p);
+            rho += hval(A.rho[(size_t)(ej * ne + ei) * nr + m], p);
+        }
+    }
+    R.rho[off + i] = rho;
+    R.F[off + i] = f_embed(A, ei, rho, false);
+}
+
 __global__ __launch_bounds__(256) void k_gcmc_init(GcmcArgs A, const GcmcCell* __restrict__ cells, Arrays R)
 {
     const GcmcCell hd = cells[blockIdx.x];
-    const int off = hd.off, n = hd.n, ne = A.ne, nr = A.nr;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const double4 si4 = R.s[off + i];
-        const double si[3] = {si4.x, si4.y, si4.z};
-        const int ei = (int)R.xe[off + i].w;
-        const Images im = count_images(si, hd.lo2, hd.hi2);
-        double rho = 0.0;
-        for (int k = 0; k < im.total(); ++k) {
-            double xi[3];
-            image_x(hd.h, si, im, k, xi);
-            for (int j = 0; j < n; ++j) {
-                if (k == 0 && j == i) continue;
-                const double4 xj = R.xe[off + j];
-                const int ej = (int)xj.w;
-                const double rsq = rsq_of(xi[0], xi[1], xi[2], xj.x, xj.y, xj.z);
-                if (rsq >= A.cut_sq[ei * ne + ej]) continue;
-                int m;
-                double p;
-                spline_at(sqrt(rsq), A.dr_inv, nr, m, p);
-                rho += hval(A.rho[(size_t)(ej * ne + ei) * nr + m], p);
-            }
-        }
-        R.rho[off + i] = rho;
-        R.F[off + i] = f_embed(A, ei, rho, false);
-    }
+    for (int i = threadIdx.x; i < hd.n; i += blockDim.x) init_atom(A, hd, R, i);
+}
+
+// the same for one cell c over many workgroups, one thread per atom (a large
+// cell: the grand-canonical exchange of an MD system)
+__global__ __launch_bounds__(256) void k_gcmc_init_cell(GcmcArgs A, const GcmcCell* __restrict__ cells, int c,
+                                                        Arrays R)
+{
+    const GcmcCell hd = cells[c];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < hd.n) init_atom(A, hd, R, i);
 }
 
 // ---- single-trial hooks (one cell, one trial, driven by the caller) ----
@@ -813,7 +826,13 @@ GcmcBatch::GcmcBatch(std::shared_ptr<EamForceField> ff, int ncells, const int* n
     }
     MAPP_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     queue_.reserve(2);
-    k_gcmc_init<<<ncells, 256, 0, stream_>>>(A_, st_.cells.get(), arrays_of(st_));
+    if (ncells == 1) {
+        if (host_cells_[0].n)
+            k_gcmc_init_cell<<<(host_cells_[0].n + 255) / 256, 256, 0, stream_>>>(A_, st_.cells.get(), 0,
+                                                                                 arrays_of(st_));
+    } else {
+        k_gcmc_init<<<ncells, 256, 0, stream_>>>(A_, st_.cells.get(), arrays_of(st_));
+    }
     MAPP_HIP(hipGetLastError());
     int dev = 0, ncu = 0, per = 0;
     MAPP_HIP(hipGetDevice(&dev));
@@ -1070,6 +1089,58 @@ void GcmcBatch::counts(int* natms, int* ngas)
         if (natms) natms[c] = hc[c].n;
         if (ngas) ngas[c] = hc[c].ngas;
     }
+}
+
+void GcmcBatch::load_cell(int c, int n, const double* x, const uint8_t* elem, const double* v)
+{
+    MAPP_REQUIRE(c >= 0 && c < nc_, ERR_ARG, "cell index out of range");
+    GcmcCell hc;
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    MAPP_HIP(hipMemcpy(&hc, st_.cells.get() + c, sizeof(GcmcCell), hipMemcpyDeviceToHost));
+    MAPP_REQUIRE(n >= 0 && n <= hc.cap, ERR_CAPACITY,
+                 "cell " + std::to_string(c) + " holds at most " + std::to_string(hc.cap) + " atoms");
+    const int ne = A_.ne;
+    double b6[6];
+    {
+        double H9[9] = {hc.h[0], 0.0, 0.0, hc.h[1], hc.h[3], 0.0, hc.h[2], hc.h[4], hc.h[5]};
+        const Box bx = make_box(H9);
+        for (int k = 0; k < 6; ++k) b6[k] = bx.b[k];
+    }
+    std::vector<double4> xe(std::max(n, 1)), sv(std::max(n, 1)), vv(std::max(n, 1));
+    std::vector<uint8_t> el(std::max(n, 1));
+    int ngas = 0;
+    for (int i = 0; i < n; ++i) {
+        const uint8_t e = elem[i];
+        MAPP_REQUIRE(e < ne, ERR_ARG, "element index out of range");
+        double s0, s1, s2, x0, x1, x2;
+        x_to_s(b6, x[3 * (size_t)i], x[3 * (size_t)i + 1], x[3 * (size_t)i + 2], s0, s1, s2);
+        s_to_x(hc.h, s0, s1, s2, x0, x1, x2);
+        xe[i] = make_double4(x0, x1, x2, (double)e);
+        sv[i] = make_double4(s0, s1, s2, 0.0);
+        vv[i] = v ? make_double4(v[3 * (size_t)i], v[3 * (size_t)i + 1], v[3 * (size_t)i + 2], 0.0)
+                  : make_double4(0.0, 0.0, 0.0, 0.0);
+        el[i] = e;
+        if (e == A_.gas) ++ngas;
+    }
+    hc.n = n;
+    hc.ngas = ngas;
+    hc.status = 0;
+    if (n) {
+        MAPP_HIP(hipMemcpy(st_.xe.get() + hc.off, xe.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+        MAPP_HIP(hipMemcpy(st_.s.get() + hc.off, sv.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+        MAPP_HIP(hipMemcpy(st_.v.get() + hc.off, vv.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+        MAPP_HIP(hipMemcpy(st_.el.get() + hc.off, el.data(), n, hipMemcpyHostToDevice));
+    }
+    MAPP_HIP(hipMemcpy(st_.cells.get() + c, &hc, sizeof(GcmcCell), hipMemcpyHostToDevice));
+    host_cells_[c].n = n;
+    host_cells_[c].ngas = ngas;
+    if (n) {
+        k_gcmc_init_cell<<<(n + 255) / 256, 256, 0, stream_>>>(A_, st_.cells.get(), c, arrays_of(st_));
+        MAPP_HIP(hipGetLastError());
+    }
+    MAPP_HIP(hipStreamSynchronize(stream_));
+    ++ver_;
+    pending_.cell = -1;
 }
 
 void GcmcBatch::get_cell(int c, double* x, uint8_t* elem, double* rho, double* F, double* v, int* seed)

@@ -68,6 +68,10 @@ public:
     void use_graph(bool on);
     void counts(int* natms, int* ngas);
     void get_cell(int c, double* x, uint8_t* elem, double* rho, double* F, double* v, int* seed);
+    // cell c's atoms replaced by n atoms (x Cartesian, wrapped into the box
+    // as x2s_lcl does; v may be null: zero), their densities and F recomputed
+    // (init_xchng); the cell's random stream continues
+    void load_cell(int c, int n, const double* x, const uint8_t* elem, const double* v);
     // one trial of one cell, driven by the caller (pre_xchng_energy +
     // xchng_energy): mode 1 inserts at fractional s, mode 2 deletes the
     // igas-th gas atom; returns the exchange energy and keeps the trial

@@ -1004,6 +1004,22 @@ __global__ __launch_bounds__(kBlock) void k_scale_v(int n, double4* __restrict__
     v[i] = vi;
 }
 
+// partial sums of the velocities (the centre-of-mass sums after an exchange
+// has changed the atoms)
+__global__ __launch_bounds__(kBlock) void k_sum_v(int n, const double4* __restrict__ v, double* __restrict__ part)
+{
+    const int blk = xcd_block();
+    const int i = blk * kBlock + threadIdx.x;
+    double sv[3] = {0.0, 0.0, 0.0};
+    if (i < n) {
+        const double4 vi = v[i];
+        sv[0] = vi.x;
+        sv[1] = vi.y;
+        sv[2] = vi.z;
+    }
+    block_sum_store<3>(sv, part + (size_t)blk * kPartW);
+}
+
 // ---------------- host <-> slot order ----------------
 
 __global__ __launch_bounds__(kBlock) void k_scatter_x(int n, const double* __restrict__ xh, const int* __restrict__ slot_of,
@@ -1175,6 +1191,46 @@ void MDSystem::set_box(const double* H, const double* x)
     rebuild();
 }
 
+// A new set of atoms in the same box (the end of a grand-canonical exchange,
+// DynamicMD::fin_xchng): positions, elements and velocities in the new caller
+// order, then the cell sort and the list as at construction.
+void MDSystem::reset_atoms(int natms, const double* x, const uint8_t* elem, const double* v)
+{
+    MAPP_REQUIRE(natms >= 1, ERR_ARG, "at least one atom is required");
+    MAPP_REQUIRE((uint32_t)natms <= kIndexMask, ERR_CAPACITY, "at most 33554431 atoms per system");
+    const int ne = ff_->tables().ne;
+    for (int i = 0; i < natms; ++i) MAPP_REQUIRE(elem[i] < ne, ERR_ARG, "element index out of range");
+    sync();
+    n_ = natms;
+    elem_host_.assign(elem, elem + natms);
+    x_.reserve(n_); x_alt_.reserve(n_); s_.reserve(n_); x0_.reserve(n_);
+    v_.reserve(n_); v_alt_.reserve(n_); id_.reserve(n_); id_alt_.reserve(n_); slot_of_.reserve(n_);
+    f_.reserve(n_); rho_.reserve(n_); dF_.reserve(n_); xr_.reserve(n_); rec_.reserve(n_);
+    keys_.reserve(n_); keys_alt_.reserve(n_); vals_.reserve(n_); vals_alt_.reserve(n_); s_tmp_.reserve(n_);
+    nn_.reserve(n_);
+    part_.reserve((size_t)5 * grid_for(n_) * kPartW + 64);
+    std::vector<double4> xh(n_), vh(n_);
+    std::vector<int> idh(n_);
+    for (int i = 0; i < n_; ++i) {
+        xh[i] = make_double4(x[3 * (size_t)i], x[3 * (size_t)i + 1], x[3 * (size_t)i + 2], (double)elem[i]);
+        vh[i] = v ? make_double4(v[3 * (size_t)i], v[3 * (size_t)i + 1], v[3 * (size_t)i + 2], 0.0)
+                  : make_double4(0.0, 0.0, 0.0, 0.0);
+        idh[i] = i;
+    }
+    MAPP_HIP(hipMemcpyAsync(x_.get(), xh.data(), sizeof(double4) * n_, hipMemcpyHostToDevice, stream_));
+    MAPP_HIP(hipMemcpyAsync(v_.get(), vh.data(), sizeof(double4) * n_, hipMemcpyHostToDevice, stream_));
+    MAPP_HIP(hipMemcpyAsync(id_.get(), idh.data(), sizeof(int) * n_, hipMemcpyHostToDevice, stream_));
+    MAPP_HIP(hipMemsetAsync(f_.get(), 0, sizeof(float4) * n_, stream_));
+    setup_grid();
+    rebuild();
+    sync();
+}
+
+void MDSystem::get_elem(uint8_t* elem) const
+{
+    std::memcpy(elem, elem_host_.data(), elem_host_.size());
+}
+
 MDSystem::~MDSystem()
 {
     if (stream_) {
@@ -1268,21 +1324,8 @@ void MDSystem::setup_grid()
     // 16-byte position records: separations up to 4 (cut + skin) stay below 2^31 q
     r16_.q = std::ldexp(1.0, (int)std::ceil(std::log2(4.0 * max_cut_ / std::ldexp(1.0, 31))));
     r16_.inv_q = 1.0 / r16_.q;
-    for (int k = 0; k < 3; ++k)
-        for (int d = 0; d < 3; ++d)
-            r16_.cH[3 * k + d] = (unsigned)(unsigned long long)std::llrint(box_.H[k][d] * r16_.inv_q);
     // grid shift of every image code (n0+2) + 5 (n1+2) + 25 (n2+2)
-    {
-        std::vector<uint4> sh(128, make_uint4(0u, 0u, 0u, 0u));  // 7-bit codes; 125..127 unused
-        for (int c = 0; c < 125; ++c) {
-            const unsigned n0 = (unsigned)(c % 5 - 2), n1 = (unsigned)(c / 5 % 5 - 2), n2 = (unsigned)(c / 25 - 2);
-            sh[c] = make_uint4(n0 * r16_.cH[0] + n1 * r16_.cH[3] + n2 * r16_.cH[6],
-                               n1 * r16_.cH[4] + n2 * r16_.cH[7], n2 * r16_.cH[8], 0u);
-        }
-        shift_.reserve(sh.size());
-        MAPP_HIP(hipMemcpy(shift_.get(), sh.data(), sizeof(uint4) * sh.size(), hipMemcpyHostToDevice));
-        r16_.shift = shift_.get();
-    }
+    refresh_frame();
     // q is a power of two: the scaled cut-offs and knot spacing are exact
     {
         const EamArgs P = ff_->args();
@@ -1745,10 +1788,11 @@ void MDSystem::half_pairs(int32_t* pi, int32_t* pj, int8_t* img)
         }
 }
 
-int MDSystem::nvt_run(const double* masses, double kB, double T, double dt, int nsteps, MDReport* rep, double* T_part)
+int MDSystem::nvt_run(const double* masses, double kB, double T, double dt, int nsteps, MDReport* rep, double* T_part,
+                      const ExchangeHook* hook, double* hist_T)
 {
     MAPP_REQUIRE(nsteps >= 0 && dt > 0.0 && T > 0.0 && kB > 0.0, ERR_ARG, "invalid NVT parameters");
-    const int nb = grid_for(n_);
+    int nb = grid_for(n_);
     const EamTables& t = ff_->tables();
     NvtArgs A{};
     A.dt = dt;
@@ -1771,6 +1815,7 @@ int MDSystem::nvt_run(const double* masses, double kB, double T, double dt, int 
     double* rows_mvv = part_.get() + (size_t)2 * nb * kPartW;
     double* rows_cv = part_.get() + (size_t)3 * nb * kPartW;
     double* rows_sv = part_.get() + (size_t)4 * nb * kPartW;
+    const int nevery = hook && hook->run ? hook->nevery : 0;
     MAPP_HIP(hipMemsetAsync(nh, 0, sizeof(double) * NH_SIZE, stream_));
     const int rebuilds0 = nrebuild_;
     force_passes(nullptr);
@@ -1794,7 +1839,22 @@ int MDSystem::nvt_run(const double* masses, double kB, double T, double dt, int 
         prof_.end(stream_);
         MAPP_HIP(hipMemcpyAsync(&flag, flags_.get() + 1, sizeof(int), hipMemcpyDeviceToHost, stream_));
         MAPP_HIP(hipStreamSynchronize(stream_));
-        if (flag) rebuild();  // the permute rewrites the records
+        if (nevery > 0 && (step + 1) % nevery == 0) {
+            // the exchange takes the place of the skin check (MDMuVT::run,
+            // src/md_muvt.cpp:172-195); it ends with a rebuild, and the atom
+            // count, ndof_part = 3 natms and the sums of v follow it
+            hook->run();
+            nb = grid_for(n_);
+            rows_mvv = part_.get() + (size_t)2 * nb * kPartW;
+            rows_cv = part_.get() + (size_t)3 * nb * kPartW;
+            rows_sv = part_.get() + (size_t)4 * nb * kPartW;
+            A.ndof = 3 * n_;
+            A.natms = n_;
+            hipLaunchKernelGGL(k_sum_v, dim3(nb), dim3(kBlock), 0, stream_, n_, v_.get(), rows_sv);
+            reduce_rows(rows_sv, nb, 3, cv);
+        } else if (flag) {
+            rebuild();  // the permute rewrites the records
+        }
         // energy and virial only where they are read: after the last step
         force_passes(&kick, true, nullptr, nullptr, step == nsteps - 1);
         prof_.begin(K_INTEGRATE, stream_);
@@ -1802,6 +1862,7 @@ int MDSystem::nvt_run(const double* masses, double kB, double T, double dt, int 
         reduce_rows(rows_cv, mvv_rows_, 3, cv + 3);
         hipLaunchKernelGGL(k_nhc, dim3(1), dim3(64), 0, stream_, nh, mvv, cv, A, 2);
         prof_.end(stream_);
+        if (hist_T) MAPP_HIP(hipMemcpyAsync(hist_T + step, nh + NH_T, sizeof(double), hipMemcpyDeviceToHost, stream_));
     }
     hipLaunchKernelGGL(k_scale_v, dim3(nb), dim3(kBlock), 0, stream_, n_, v_.get(), nh);
     double nhh[NH_SIZE];

@@ -1,5 +1,6 @@
 // md.hpp -- device-resident MD system: atoms, neighbor list, tabulated EAM.
 #pragma once
+#include <functional>
 #include <memory>
 #include <vector>
 
@@ -98,6 +99,14 @@ struct MinReport {
 };
 constexpr int kMaxLbfgsM = 64;
 
+// A grand-canonical exchange run by nvt_run after every nevery-th first half
+// step in place of the skin check (MDMuVT::run); it may replace the atoms
+// through reset_atoms.
+struct ExchangeHook {
+    int nevery = 0;
+    std::function<void()> run;
+};
+
 // largest system of MDSystem::value_noalloc: its scratch rows take 24 bytes
 // per atom (24 MiB per GPU)
 constexpr int kNoAllocMaxAtoms = 1 << 20;
@@ -114,6 +123,10 @@ public:
     // a new box H (lower triangular, rows = box vectors) and, unless x is
     // null, new positions; rebuilds the grid and the list
     void set_box(const double* H, const double* x);
+    // the atoms replaced by natms new ones in the same box (caller order; v
+    // may be null: zero), then the cell sort and the list
+    void reset_atoms(int natms, const double* x, const uint8_t* elem, const double* v);
+    void get_elem(uint8_t* elem) const;        // caller order
     void set_x(const double* x_host);          // caller order
     void get_x(double* x_host);
     void set_v(const double* v_host);
@@ -136,8 +149,27 @@ public:
     int64_t half_pairs_count();
     void half_pairs(int32_t* i, int32_t* j, int8_t* img);
     void list_stats(int* maxn, int64_t* nfull);
-    // MDNVT::run; masses per element; returns rebuild count
-    int nvt_run(const double* masses, double kB, double T, double dt, int nsteps, MDReport* rep, double* T_part);
+    // MDNVT::run; masses per element; returns rebuild count. hook: an
+    // exchange after every hook->nevery-th first half step; hist_T: T_part
+    // after each step
+    int nvt_run(const double* masses, double kB, double T, double dt, int nsteps, MDReport* rep, double* T_part,
+                const ExchangeHook* hook = nullptr, double* hist_T = nullptr);
+    // MDMuVT::run (src/md_muvt.cpp:122-280): NVT with nattempts exchange
+    // attempts of element gas at chemical potential mu after every nevery-th
+    // first half step, the chain's random stream seeded once per run; counts:
+    // accepted insertions, accepted deletions, insertion trials, deletion
+    // trials; hist [nsteps][2]: T_part and gas atoms after each step; returns
+    // the list builds (rebuilds and exchanges)
+    int muvt_run(const double* masses, double kB, double hP, double T, double dt, double mu, int gas, int seed,
+                 int nevery, int nattempts, int nsteps, MDReport* rep, double* T_part, int* counts, double* hist);
+    // MDNST::run (src/md_nst.cpp:376-460): S9 the target stress (row-major,
+    // lower triangle read, NaN = free), tau9 the box relaxation times (nullptr:
+    // 1000 dt), B_ref and the reference volume re-taken every nreset steps (0:
+    // never); H9 the final box, hist [nsteps][14] (T_part, pe, H00, H10, H11,
+    // H20, H21, H22, S_pe xx yx zx yy zy zz) when not null; returns the list
+    // rebuilds
+    int nst_run(const double* masses, double kB, double T, double dt, const double* S9, const double* tau9,
+                int nreset, int nsteps, MDReport* rep, double* T_part, double* H9, double* hist);
     // MinCG::run / MinLBFGS::run (src/min_cg.h:453-533, src/min_l-bfgs.h:114-217);
     // tally rows (step, energy): the start and every accepted step
     MinReport minimize(const MinParams& p, std::vector<double>* tally);
@@ -162,6 +194,7 @@ public:
 
 private:
     void setup_grid();
+    void refresh_frame();  // image shifts of the records for the current box
     void build_list();
     // records_fresh: rec_ already holds the current positions (written by the skin check or a rebuild)
     // the two EAM passes; f_out / rho_out (nullptr: f_ and rho_) receive the

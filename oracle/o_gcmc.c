@@ -92,6 +92,8 @@ static void cell_coord(const o_gcmc* g, const double* s, int* cc)
     }
 }
 
+static void gcmc_load(o_gcmc* g, int natms, const double* x, const uint8_t* elem, const double* v);
+
 o_gcmc* o_gcmc_create(const o_eam* ff, int natms, const double* H, const double* x, const uint8_t* elem,
                       int gas_elem, double mu, double T, double kB, double hP, const double* masses, int seed,
                       int cap)
@@ -166,7 +168,26 @@ o_gcmc* o_gcmc_create(const o_eam* ff, int natms, const double* H, const double*
     g->cell = (int*)malloc(sizeof(int) * cd);
     g->next = (int*)malloc(sizeof(int) * cd);
     g->head = (int*)malloc(sizeof(int) * g->ncells);
+    gcmc_load(g, natms, x, elem, NULL);
+    return g;
+}
+
+/* GCMC::xchng's set-up over the current atoms (src/sgcmc.cpp xchng,
+ * src/pgcmc.cpp:204-282): x2s of the locals with the wrap, the densities and
+ * F of init_xchng, the cell lists and the gas count; v travels with the atoms
+ * (NULL: zero). The random stream is not touched. */
+static void gcmc_load(o_gcmc* g, int natms, const double* x, const uint8_t* elem, const double* v)
+{
+    const o_eam* ff = g->ff;
+    int ne = ff->ne;
+    int gas_elem = g->gas;
+    double H[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) H[3 * i + j] = g->bx.H[i][j];
+    g->n = natms;
     memcpy(g->elem, elem, natms);
+    if (v) memcpy(g->v, v, sizeof(double) * 3 * natms);
+    else memset(g->v, 0, sizeof(double) * 3 * natms);
 
     /* init_xchng runs over the half list of a fresh rebuild: reuse the MD
      * oracle for the ghosts and the list, then accumulate rho as written
@@ -212,7 +233,13 @@ o_gcmc* o_gcmc_create(const o_eam* ff, int natms, const double* H, const double*
     g->ngas = 0;
     for (int i = 0; i < natms; i++)
         if (g->elem[i] == gas_elem) g->ngas++;
-    return g;
+}
+
+int o_gcmc_reload(o_gcmc* g, int natms, const double* x, const uint8_t* elem, const double* v)
+{
+    if (natms < 0 || natms > g->cap) return -1;
+    gcmc_load(g, natms, x, elem, v);
+    return 0;
 }
 
 void o_gcmc_free(o_gcmc* g)

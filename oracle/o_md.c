@@ -985,6 +985,124 @@ int o_md_nvt_run(o_md* md, double* v, const double* masses, double kB, double T,
     return o_md_nvt_run_hist(md, v, masses, kB, T, dt, nsteps, out7, T_part_out, NULL);
 }
 
+/* MDMuVT::run (src/md_muvt.cpp:122-280), the branch without fixed degrees of
+ * freedom: MDNVT's step (src/md_nvt.cpp:70-128) where, after every nevery-th
+ * first half step, gcmc.xchng(false, nattempts) takes the place of the skin
+ * check (src/md_muvt.cpp:172-195). At one process PGCMC draws, decides and
+ * orders the atoms exactly as SGCMC does (one concurrent attempt per round,
+ * the same Random(seed) stream, s drawn in [0, 1), the same acceptance
+ * expressions and add/del order: src/pgcmc.cpp:562-735,1163-1266 against
+ * src/sgcmc.cpp:251-444), so the exchange runs the oracle's SGCMC chain,
+ * created once per run (gcmc.init) and reloaded with the current atoms at
+ * each exchange (x2s with the wrap, init_xchng). fin_xchng rebuilds the list
+ * from the wrapped positions; ndof_part = 3 natms follows the count. The
+ * exchange's update of mvv and T_part (src/md_muvt.cpp:192-194) is
+ * overwritten by update_x_d before anything reads it, so it is not kept.
+ * *pmd is replaced by the system after each exchange; v holds cap atoms.
+ * counts: accepted insertions, accepted deletions, insertion trials, deletion
+ * trials. hist [nsteps][3]: T_part, potential energy, gas atoms. Returns the
+ * list builds (rebuilds and exchanges), or -1 when an insertion would exceed
+ * cap. */
+int o_md_muvt_run(o_md** pmd, double* v, int cap, const double* masses, double kB, double hP, double T, double dt,
+                  double mu, int gas, int seed, int nevery, int nattempts, int nsteps, double* out7,
+                  double* T_part_out, int* natms_out, int* counts, double* hist)
+{
+    o_md* md = *pmd;
+    const o_eam* ff = md->ff;
+    int n = md->n;
+    double dt2 = dt / 2.0;
+    int ndof = n * 3;
+    double skin = md->skin;
+    double H9[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) H9[3 * i + j] = md->bx.H[i][j];
+    o_nhc th;
+    nhc_init(&th, dt / 2.0, 100.0 * dt, 1);
+    size_t cd = (size_t)(cap > n ? cap : n);
+    double* f = (double*)malloc(sizeof(double) * 3 * cd);
+    double* xs = (double*)malloc(sizeof(double) * 3 * cd);
+    double* ss = (double*)malloc(sizeof(double) * 3 * cd);
+    double* rs = (double*)malloc(sizeof(double) * cd);
+    double* Fs = (double*)malloc(sizeof(double) * cd);
+    uint8_t* es = (uint8_t*)malloc(cd);
+    int* dec = (int*)malloc(sizeof(int) * (nattempts > 0 ? nattempts : 1));
+    double* du = (double*)malloc(sizeof(double) * (nattempts > 0 ? nattempts : 1));
+    for (int k = 0; k < 4; k++) counts[k] = 0;
+    /* PGCMC gcmc(...); gcmc.init() at the start of run */
+    o_gcmc* g = o_gcmc_create(ff, n, H9, md->g.x, md->elem_lcl, gas, mu, T, kB, hP, masses, seed, (int)cd);
+    double mvv[6];
+    kin_tensor(md, v, masses, mvv);
+    double T_part = (mvv[0] + (mvv[3] + mvv[5])) / (ndof * kB);
+    o_md_force(md, f, out7, NULL);
+    int nbuild = 0, status = 0;
+    double fac, fac_x_d = 1.0;
+    for (int step = 0; step < nsteps; step++) {
+        fac_x_d *= fac = nhc_step(&th, T_part / T, ndof, NULL);
+        fac *= fac;
+        for (int k = 0; k < 6; k++) mvv[k] *= fac;
+        T_part *= fac;
+        double dx_l[3] = {0.0, 0.0, 0.0};
+        double* x = md->g.x;
+        for (int i = 0; i < n; i++) {
+            double m = masses[md->elem_lcl[i]];
+            for (int d = 0; d < 3; d++) {
+                v[3 * i + d] = v[3 * i + d] * fac_x_d + f[3 * i + d] * dt2 / m;
+                dx_l[d] += v[3 * i + d] * dt;
+                x[3 * i + d] += v[3 * i + d] * dt;
+            }
+        }
+        for (int d = 0; d < 3; d++) dx_l[d] /= (double)n;
+        for (int i = 0; i < n; i++)
+            for (int d = 0; d < 3; d++) x[3 * i + d] -= dx_l[d];
+        if (nevery <= 0 || (step + 1) % nevery) {
+            nbuild += o_md_update(md);
+        } else {
+            o_gcmc_reload(g, n, md->g.x, md->elem_lcl, v);
+            if (o_gcmc_attempts(g, nattempts, dec, du) != 0) {
+                status = -1;
+                break;
+            }
+            for (int k = 0; k < nattempts; k++) {
+                if (dec[k] == 1) counts[0]++;
+                if (dec[k] == 2) counts[1]++;
+                if (dec[k] == 1 || dec[k] == -1) counts[2]++; else counts[3]++;
+            }
+            n = o_gcmc_natms(g);
+            o_gcmc_state(g, xs, ss, es, rs, Fs, v);
+            ndof = 3 * n;
+            o_md_free(md);
+            md = o_md_create(ff, n, H9, xs, es, skin);
+            *pmd = md;
+            nbuild++;
+        }
+        o_md_force(md, f, out7, NULL);
+        for (int i = 0; i < n; i++) {
+            double m = masses[md->elem_lcl[i]];
+            for (int d = 0; d < 3; d++) v[3 * i + d] += f[3 * i + d] * dt2 / m;
+        }
+        kin_tensor(md, v, masses, mvv);
+        T_part = (mvv[0] + (mvv[3] + mvv[5])) / (ndof * kB);
+        fac_x_d = fac = nhc_step(&th, T_part / T, ndof, NULL);
+        fac *= fac;
+        for (int k = 0; k < 6; k++) mvv[k] *= fac;
+        T_part *= fac;
+        if (hist) {
+            int ng = 0;
+            for (int i = 0; i < n; i++) ng += md->elem_lcl[i] == gas;
+            hist[3 * step] = T_part;
+            hist[3 * step + 1] = out7[0];
+            hist[3 * step + 2] = (double)ng;
+        }
+    }
+    if (status == 0)
+        for (int i = 0; i < 3 * n; i++) v[i] *= fac_x_d;
+    if (T_part_out) *T_part_out = T_part;
+    if (natms_out) *natms_out = n;
+    o_gcmc_free(g);
+    free(f); free(xs); free(ss); free(rs); free(Fs); free(es); free(dec); free(du);
+    return status == 0 ? nbuild : -1;
+}
+
 /* AtomsMD::create_T (src/atoms_md.cpp:130-160), one rank (seed + 0) */
 void o_create_T(int natms, const uint8_t* elem, const double* masses, double kB,
                 double T, int seed, double* v)
@@ -1005,4 +1123,325 @@ void o_md_view(const o_md* md, const double** x, const double** s, const uint8_t
     *elem = md->elem;
     *row = md->row;
     *col = md->col;
+}
+
+/* ---------------- NST (src/md_nst.cpp) ---------------- */
+
+/* MDMath::f, df, ddf (src/md_nst.cpp:843-871): phi(x) = (e^x - 1) / x and its
+ * first two derivatives, by their Taylor series below |x| = 1/64 */
+static double mdm_f(double x)
+{
+    if (fabs(x) < 0.015625) return 1.0 + x / 2.0 + x * x / 6.0 + x * x * x / 24.0;
+    return (exp(x) - 1.0) / x;
+}
+static double mdm_df(double x)
+{
+    if (fabs(x) < 0.015625) return 0.5 + x / 3.0 + x * x / 8.0 + x * x * x / 30.0;
+    double e = exp(x);
+    return (e - (e - 1.0) / x) / x;
+}
+static double mdm_ddf(double x)
+{
+    if (fabs(x) < 0.015625) return 1.0 / 3.0 + x / 4.0 + x * x / 10.0 + x * x * x / 36.0;
+    double e = exp(x);
+    return (e - 2.0 * ((e - (e - 1.0) / x) / x)) / x;
+}
+
+static const double MDM_SQRT_EPS = 1.4901161193847656e-08; /* sqrt(DBL_EPSILON) */
+
+/* first and second divided differences of exp and of phi, switching to the
+ * derivative at the midpoint when the nodes are closer than the rounding
+ * allows (src/md_nst.cpp:890-901, 960-986) */
+static double dd_exp(double x, double y, double fx, double fy)
+{
+    if (fabs(x - y) < MDM_SQRT_EPS) return exp(0.5 * (x + y));
+    return (fx - fy) / (x - y);
+}
+static double dd_phi(double x, double y, double fx, double fy)
+{
+    double m = 0.5 * (x + y);
+    if (fabs(x - y) < sqrt(2.220446049250313e-16 * mdm_f(m) / mdm_ddf(m))) return mdm_df(m);
+    return (fx - fy) / (x - y);
+}
+static double dd2_phi(double x, double y, double z, double f_yz, double f_zx, double f_xy)
+{
+    double m = (x + y + z) / 3.0, d2 = mdm_ddf(m);
+    double lim = sqrt(2.220446049250313e-16 * mdm_f(m) / d2);
+    if (fabs(x - z) > lim) return (f_xy - f_yz) / (x - z);
+    if (fabs(y - x) > lim) return (f_yz - f_zx) / (y - x);
+    if (fabs(z - y) > lim) return (f_zx - f_xy) / (z - y);
+    return d2;
+}
+static double dd2_exp(double x, double y, double z, double g_yz, double g_zx, double g_xy)
+{
+    if (fabs(x - z) > MDM_SQRT_EPS) return (g_xy - g_yz) / (x - z);
+    if (fabs(y - x) > MDM_SQRT_EPS) return (g_yz - g_zx) / (y - x);
+    if (fabs(z - y) > MDM_SQRT_EPS) return (g_zx - g_xy) / (z - y);
+    return exp((x + y + z) / 3.0);
+}
+
+/* MDMath::calc (src/md_nst.cpp:887-1002) for a lower-triangular A (row-major
+ * [9]): E = exp(Z0), F0 = dt2 phi(Z0), F1 = dt phi(Z1) with Z0 = -dt2 (A + a I)
+ * and Z1 = dt A, each from the diagonal values and the divided differences of
+ * exp and phi (the functions of a triangular matrix, Opitz) */
+void o_nst_calc(double a, double dt, double dt2, const double* A, double* E, double* F0, double* F1)
+{
+    double z0[3], z1[3], f0[3], f1[3], e0[3];
+    for (int i = 0; i < 3; i++) {
+        z0[i] = -dt2 * (A[4 * i] + a);
+        z1[i] = dt * A[4 * i];
+        f0[i] = mdm_f(z0[i]);
+        f1[i] = mdm_f(z1[i]);
+        e0[i] = exp(z0[i]);
+    }
+    for (int k = 0; k < 9; k++) E[k] = F0[k] = F1[k] = 0.0;
+    /* pairs (1,0), (2,0), (2,1) */
+    const int pi[3] = {1, 2, 2}, pj[3] = {0, 0, 1};
+    double d0[3] = {0, 0, 0}, d1[3] = {0, 0, 0}, de[3] = {0, 0, 0};
+    for (int p = 0; p < 3; p++) {
+        int i = pi[p], j = pj[p];
+        if (A[3 * i + j] == 0.0) continue;
+        d0[p] = dd_phi(z0[i], z0[j], f0[i], f0[j]);
+        d1[p] = dd_phi(z1[i], z1[j], f1[i], f1[j]);
+        de[p] = dd_exp(z0[i], z0[j], e0[i], e0[j]);
+    }
+    for (int i = 0; i < 3; i++) {
+        F0[4 * i] = f0[i] * dt2;
+        F1[4 * i] = f1[i] * dt;
+        E[4 * i] = e0[i];
+    }
+    for (int p = 0; p < 3; p++) {
+        int i = pi[p], j = pj[p];
+        double aij = A[3 * i + j];
+        F0[3 * i + j] = -aij * dt2 * dt2 * d0[p];
+        F1[3 * i + j] = aij * dt * dt * d1[p];
+        E[3 * i + j] = -aij * dt2 * de[p];
+    }
+    double a10 = A[3], a21 = A[7];
+    if (a10 == 0.0 || a21 == 0.0) return;
+    /* second order term of entry (2,0): z21 z10 [x, y, z] */
+    F0[6] += dt2 * dt2 * dt2 * a21 * a10 * dd2_phi(z0[0], z0[1], z0[2], d0[2], d0[1], d0[0]);
+    F1[6] += dt * dt * dt * a21 * a10 * dd2_phi(z1[0], z1[1], z1[2], d1[2], d1[1], d1[0]);
+    E[6] += dt2 * dt2 * a21 * a10 * dd2_exp(z0[0], z0[1], z0[2], de[2], de[1], de[0]);
+}
+
+/* row vector times lower-triangular matrix: out_j = sum_{i >= j} v_i M_ij */
+static void v_mlt(const double* v, const double* M, double* out)
+{
+    double o0 = v[0] * M[0] + v[1] * M[3] + v[2] * M[6];
+    double o1 = v[1] * M[4] + v[2] * M[7];
+    double o2 = v[2] * M[8];
+    out[0] = o0; out[1] = o1; out[2] = o2;
+}
+
+static int lt_voigt(int i, int j) { return i + j * 3 - j * (j + 1) / 2; }
+
+/* 1 flips the sign of the stress term of the box kick; 0 is the restatement.
+ * The mutant exists to show that the physical checks of the NST fail on it. */
+int o_nst_mutation = 0;
+
+typedef struct {
+    double T, kB, dt, dt2, ndof;
+    int sdof[9];
+    double S_dev[9], prefac[9], tau[9], VH[9], B_ref[9];
+    double s_hyd, vol_ref, T_baro;
+    int ndof_baro;
+} nst_state;
+
+/* MDNST::update_V_H (src/md_nst.cpp:320-351): the lower triangle of
+ * L^T S_dev L with L = B_ref H and S_dev as pre_init leaves it (its lower
+ * triangle; the upper stays zero), then the box-velocity kicks */
+static void nst_update_VH(nst_state* s, const o_box* bx, const double* S_pe, const double* mvv, double T_part)
+{
+    double L[9] = {0}, SL[9] = {0}, M2[9] = {0};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j <= i; j++) {
+            double v = 0.0;
+            for (int k = j; k <= i; k++) v += s->B_ref[3 * i + k] * bx->H[k][j];
+            L[3 * i + j] = v;
+        }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double v = 0.0;
+            for (int k = 0; k < 3; k++) v += s->S_dev[3 * i + k] * L[3 * k + j];
+            SL[3 * i + j] = v;
+        }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j <= i; j++) {
+            double v = 0.0;
+            for (int k = 0; k < 3; k++) v += L[3 * k + i] * SL[3 * k + j];
+            M2[3 * i + j] = v;
+        }
+    for (int i = 0; i < 3; i++)
+        if (s->sdof[4 * i]) s->VH[4 * i] += (s->kB * T_part + bx->vol * s->s_hyd) * s->prefac[4 * i] * s->dt2;
+    s->T_baro = 0.0;
+    for (int j = 0; j < 3; j++)
+        for (int i = j; i < 3; i++) {
+            int k = 3 * i + j;
+            if (!s->sdof[k]) continue;
+            const double sg = o_nst_mutation == 1 ? -1.0 : 1.0;
+            s->VH[k] += (s->vol_ref * M2[k] - sg * bx->vol * S_pe[k] + mvv[lt_voigt(i, j)]) * s->prefac[k] * s->dt2;
+            s->T_baro += s->VH[k] * s->VH[k] * s->tau[k] * s->tau[k];
+        }
+    s->T_baro *= s->ndof * s->T / s->ndof_baro;
+}
+
+/* S_pe (lower triangle, row-major [9]) from ForceFieldMD::force_calc's F_H:
+ * S_pe = -F_H / vol (src/ff_md.cpp:134-136) */
+static void nst_spe(const double* out7, double vol, double* S_pe)
+{
+    for (int k = 0; k < 9; k++) S_pe[k] = 0.0;
+    for (int j = 0; j < 3; j++)
+        for (int i = j; i < 3; i++) S_pe[3 * i + j] = -out7[1 + lt_voigt(i, j)] / vol;
+}
+
+/* MDNST::run (src/md_nst.cpp:376-460) with update_x_d__x__x_d
+ * (src/md_nst.cpp:80-153) and pre_init (src/md_nst.cpp:291-319) on one rank:
+ * S9 the target stress (row-major, lower triangle read, NaN = free
+ * component), tau9 the relaxation times (NULL: 1000 dt each), thermostats
+ * NHC(dt/2, 100 dt) on the particles and NHC(dt/2, 1000 dt) on the box, both
+ * L = 3 and one iteration; every nreset steps (0: never) the reference box
+ * B_ref and volume follow the current ones. H9 gets the final box; hist, when
+ * given, [nsteps][14]: T_part, pe, H00, H10, H11, H20, H21, H22 and S_pe
+ * (xx, yx, zx, yy, zy, zz) after each step. Returns the number of list
+ * rebuilds, or -1 when the box left [vol0 / 4, 4 vol0] and the run stopped. */
+int o_md_nst_run(o_md* md, double* v, const double* masses, double kB, double T, double dt, const double* S9,
+                 const double* tau9, int nreset, int nsteps, double* out7, double* T_part_out, double* H9,
+                 double* hist)
+{
+    int n = md->n;
+    nst_state s;
+    memset(&s, 0, sizeof s);
+    s.T = T; s.kB = kB; s.dt = dt; s.dt2 = dt / 2.0; s.ndof = 3.0 * n;
+    o_nhc tp, tb;
+    nhc_init(&tp, dt / 2.0, 100.0 * dt, 1);
+    nhc_init(&tb, dt / 2.0, 1000.0 * dt, 1);
+    double mid[3];
+    double mvv[6];
+    kin_tensor(md, v, masses, mvv);
+    double T_part = (mvv[0] + (mvv[3] + mvv[5])) / (s.ndof * kB);
+    /* pre_init */
+    double S[9] = {0};
+    for (int j = 0; j < 3; j++)
+        for (int i = j; i < 3; i++) {
+            int k = 3 * i + j;
+            s.tau[k] = tau9 ? tau9[k] : 1000.0 * dt;
+            if (isnan(S9[k])) {
+                s.sdof[k] = 0;
+                S[k] = 0.0;
+            } else {
+                s.sdof[k] = 1;
+                S[k] = S9[k];
+                s.prefac[k] = 1.0 / (s.tau[k] * s.tau[k] * s.ndof * kB * T);
+                s.ndof_baro++;
+            }
+            s.S_dev[k] = S[k];
+        }
+    s.T_baro = 0.0; /* V_H starts at zero */
+    s.s_hyd = (S[0] + S[4] + S[8]) / 3.0;
+    for (int i = 0; i < 3; i++) s.S_dev[4 * i] = S[4 * i] - s.s_hyd;
+    /* run */
+    double* f = (double*)malloc(sizeof(double) * 3 * (n > 0 ? n : 1));
+    double S_pe[9];
+    o_md_force(md, f, out7, NULL);
+    nst_spe(out7, md->bx.vol, S_pe);
+    for (int k = 0; k < 9; k++) s.B_ref[k] = md->bx.B[k / 3][k % 3];
+    s.vol_ref = md->bx.vol;
+    int nrebuild = 0;
+    double fac, fac_x_d = 1.0;
+    const double vol0 = md->bx.vol;
+    for (int step = 0; step < nsteps; step++) {
+        /* a box that has left [vol0 / 4, 4 vol0] (or is not finite) ends the
+         * run: -1 is returned */
+        if (!(md->bx.vol > 0.25 * vol0 && md->bx.vol < 4.0 * vol0)) {
+            nrebuild = -1;
+            break;
+        }
+        fac = nhc_step(&tb, s.T_baro / T, s.ndof_baro, mid);
+        for (int k = 0; k < 9; k++) s.VH[k] *= fac;
+        s.T_baro *= fac * fac;
+        fac_x_d *= fac = nhc_step(&tp, T_part / T, (int)s.ndof, mid);
+        fac *= fac;
+        for (int k = 0; k < 6; k++) mvv[k] *= fac;
+        T_part *= fac;
+        nst_update_VH(&s, &md->bx, S_pe, mvv, T_part);
+        /* update_x_d__x__x_d */
+        {
+            double E[9], F0[9], F1[9];
+            o_nst_calc((s.VH[0] + s.VH[4] + s.VH[8]) / s.ndof, dt, s.dt2, s.VH, E, F0, F1);
+            double* x = md->g.x;
+            double dx[3] = {0.0, 0.0, 0.0};
+            for (int i = 0; i < n; i++) {
+                double m_inv = 1.0 / masses[md->elem_lcl[i]];
+                double a[3], b[3], v0[3];
+                v_mlt(v + 3 * i, E, a);
+                v_mlt(f + 3 * i, F0, b);
+                for (int d = 0; d < 3; d++) v[3 * i + d] = fac_x_d * a[d] + m_inv * b[d];
+                v_mlt(x + 3 * i, s.VH, v0);
+                for (int d = 0; d < 3; d++) v0[d] += v[3 * i + d];
+                v_mlt(v0, F1, v0);
+                for (int d = 0; d < 3; d++) {
+                    x[3 * i + d] += v0[d];
+                    dx[d] += v0[d];
+                }
+            }
+            for (int d = 0; d < 3; d++) dx[d] /= (double)n;
+            for (int i = 0; i < n; i++)
+                for (int d = 0; d < 3; d++) x[3 * i + d] -= dx[d];
+            /* H <- H (I + F1 V_H) */
+            double M[9] = {0}, Hn[9] = {0};
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j <= i; j++) {
+                    double t = 0.0;
+                    for (int k = j; k <= i; k++) t += F1[3 * i + k] * s.VH[3 * k + j];
+                    M[3 * i + j] = t + (i == j ? 1.0 : 0.0);
+                }
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j <= i; j++) {
+                    double t = 0.0;
+                    for (int k = j; k <= i; k++) t += md->bx.H[i][k] * M[3 * k + j];
+                    Hn[3 * i + j] = t;
+                }
+            o_box_set(&md->bx, Hn);
+            nrebuild += o_md_update(md);
+            o_md_force(md, f, out7, NULL);
+            nst_spe(out7, md->bx.vol, S_pe);
+            for (int i = 0; i < n; i++) {
+                double m_inv = 1.0 / masses[md->elem_lcl[i]];
+                double a[3], b[3];
+                v_mlt(v + 3 * i, E, a);
+                v_mlt(f + 3 * i, F0, b);
+                for (int d = 0; d < 3; d++) v[3 * i + d] = a[d] + m_inv * b[d];
+            }
+            kin_tensor(md, v, masses, mvv);
+            T_part = (mvv[0] + (mvv[3] + mvv[5])) / (s.ndof * kB);
+        }
+        nst_update_VH(&s, &md->bx, S_pe, mvv, T_part);
+        fac_x_d = fac = nhc_step(&tp, T_part / T, (int)s.ndof, mid);
+        fac *= fac;
+        for (int k = 0; k < 6; k++) mvv[k] *= fac;
+        T_part *= fac;
+        fac = nhc_step(&tb, s.T_baro / T, s.ndof_baro, mid);
+        for (int k = 0; k < 9; k++) s.VH[k] *= fac;
+        s.T_baro *= fac * fac;
+        if (nreset && (step + 1) % nreset == 0) {
+            for (int k = 0; k < 9; k++) s.B_ref[k] = md->bx.B[k / 3][k % 3];
+            s.vol_ref = md->bx.vol;
+        }
+        if (hist) {
+            double* h = hist + 14 * step;
+            h[0] = T_part;
+            h[1] = out7[0];
+            h[2] = md->bx.H[0][0]; h[3] = md->bx.H[1][0]; h[4] = md->bx.H[1][1];
+            h[5] = md->bx.H[2][0]; h[6] = md->bx.H[2][1]; h[7] = md->bx.H[2][2];
+            for (int j = 0; j < 3; j++)
+                for (int i = j; i < 3; i++) h[8 + lt_voigt(i, j)] = S_pe[3 * i + j];
+        }
+    }
+    for (int i = 0; i < 3 * n; i++) v[i] *= fac_x_d;
+    if (T_part_out) *T_part_out = T_part;
+    if (H9)
+        for (int k = 0; k < 9; k++) H9[k] = md->bx.H[k / 3][k % 3];
+    free(f);
+    return nrebuild;
 }

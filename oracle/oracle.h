@@ -115,6 +115,14 @@ extern int o_nhc_mutation;
  * the count in effect (1 without OpenMP). */
 void o_set_threads(int n);
 int o_get_threads(void);
+/* MDMath::calc (src/md_nst.cpp:887-1002): E = exp(-dt2 (A + a I)),
+ * F0 = dt2 phi(-dt2 (A + a I)), F1 = dt phi(dt A), phi(z) = (e^z - 1) / z,
+ * for a lower-triangular A; all row-major [9] */
+void o_nst_calc(double a, double dt, double dt2, const double* A, double* E, double* F0, double* F1);
+/* MDNST::run (src/md_nst.cpp:376-460): see o_md.c; returns the rebuilds */
+extern int o_nst_mutation;
+int o_md_nst_run(o_md*, double* v, const double* masses, double kB, double T, double dt, const double* S9,
+                 const double* tau9, int nreset, int nsteps, double* out7, double* T_part, double* H9, double* hist);
 /* AtomsMD::create_T (src/atoms_md.cpp:130-160) */
 void o_create_T(int natms, const uint8_t* elem, const double* masses, double kB,
                 double T, int seed, double* v);
@@ -188,6 +196,10 @@ void o_gcmc_free(o_gcmc*);
  * drawn with no gas atom (no further draws); dU[k] = delta_u (0 when 0).
  * Returns 0, or -1 when an insertion would exceed cap. */
 int o_gcmc_attempts(o_gcmc*, int nattempts, int* decisions, double* dU);
+/* the cell's atoms replaced by natms atoms (x Cartesian, wrapped on entry;
+ * v NULL: zero), densities and F recomputed, the random stream kept (the
+ * set-up of each GCMC::xchng); -1 if natms exceeds cap */
+int o_gcmc_reload(o_gcmc*, int natms, const double* x, const uint8_t* elem, const double* v);
 int o_gcmc_natms(const o_gcmc*);
 int o_gcmc_ngas(const o_gcmc*);
 int o_gcmc_rng_seed(const o_gcmc*);
@@ -196,6 +208,16 @@ int o_gcmc_rng_seed(const o_gcmc*);
 void o_gcmc_info(const o_gcmc*, double* out8, int* iout4);
 void o_gcmc_state(const o_gcmc*, double* x, double* s, uint8_t* elem, double* rho,
                   double* F, double* v);
+
+/* MDMuVT::run (src/md_muvt.cpp:122-280): NVT with an SGCMC exchange of the
+ * gas element after every nevery-th first half step. *md is replaced when the
+ * atom count changes; v holds cap atoms; counts[4]: accepted insertions,
+ * accepted deletions, insertion trials, deletion trials; hist [nsteps][3]:
+ * T_part, potential energy, gas atoms (may be NULL). Returns the list builds,
+ * -1 when an insertion would exceed cap. */
+int o_md_muvt_run(o_md** md, double* v, int cap, const double* masses, double kB, double hP, double T, double dt,
+                  double mu, int gas, int seed, int nevery, int nattempts, int nsteps, double* out7,
+                  double* T_part, int* natms, int* counts, double* hist);
 
 #ifdef __cplusplus
 }

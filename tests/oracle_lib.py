@@ -57,8 +57,15 @@ def L():
             "o_md_force": (None, [C.c_void_p, dp, dp, dp]),
             "o_md_energy": (C.c_double, [C.c_void_p]),
             "o_md_nvt_run": (C.c_int, [C.c_void_p, dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp]),
+            "o_nst_calc": (None, [C.c_double, C.c_double, C.c_double, dp, dp, dp, dp]),
+            "o_md_nst_run": (C.c_int, [C.c_void_p, dp, dp, C.c_double, C.c_double, C.c_double, dp, C.c_void_p, C.c_int,
+                                       C.c_int, dp, dp, dp, C.c_void_p]),
             "o_md_nvt_run_hist": (C.c_int, [C.c_void_p, dp, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp,
                                             dp, dp]),
+            "o_md_muvt_run": (C.c_int, [C.c_void_p, dp, C.c_int, dp, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp,
+                                        C.c_void_p, C.c_void_p, dp]),
+            "o_md_view": (None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
             "o_nhc_run": (C.c_double, [C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, dp]),
             "o_create_T": (None, [C.c_int, u8p, dp, C.c_double, C.c_double, C.c_int, dp]),
             "o_set_threads": (None, [C.c_int]),
@@ -228,6 +235,41 @@ class OMD:
         nreb = L().o_md_nvt_run(self.h, P(v), P(masses), kB, T, dt, nsteps, P(out), C.byref(Tp))
         return v, out, Tp.value, nreb
 
+    def nst_run(self, v, masses, kB, T, dt, S, nsteps, tau=None, nreset=0):
+        """MDNST::run: (v, H, T_part, out7, hist [nsteps][14], rebuilds); v updated in place."""
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros(7)
+        H = np.zeros((3, 3))
+        hist = np.zeros((max(int(nsteps), 1), 14))
+        Tp = C.c_double()
+        tau = None if tau is None else f64(np.asarray(tau).reshape(3, 3))
+        nr = L().o_md_nst_run(self.h, P(v), P(f64(masses)), kB, T, dt, P(f64(np.asarray(S).reshape(3, 3))),
+                              P(tau) if tau is not None else None, int(nreset), int(nsteps), P(out), C.byref(Tp),
+                              P(H), P(hist))
+        return v, H, Tp.value, out, hist[:int(nsteps)], nr
+
+    def muvt_run(self, v, masses, kB, hP, T, dt, mu, gas, seed, nevery, nattempts, nsteps, cap=None):
+        """MDMuVT::run: (v [natms][3], T_part, out7, hist [nsteps][3], counts[4], builds). The system
+        is replaced by the one after the last exchange (self.n and self.elem follow)."""
+        cap = int(cap) if cap is not None else self.n + 4096
+        vv = np.zeros((cap, 3))
+        vv[:self.n] = f64(v).reshape(self.n, 3)
+        out = np.zeros(7)
+        Tp = C.c_double()
+        na = C.c_int()
+        counts = np.zeros(4, np.int32)
+        hist = np.zeros((max(int(nsteps), 1), 3))
+        hp = C.c_void_p(self.h)
+        nb = L().o_md_muvt_run(C.byref(hp), P(vv), cap, P(f64(masses)), kB, hP, T, dt, mu, int(gas), int(seed),
+                               int(nevery), int(nattempts), int(nsteps), P(out), C.byref(Tp), C.byref(na),
+                               counts.ctypes.data_as(C.c_void_p), P(hist))
+        self.h = hp.value
+        self.n = na.value
+        d0, d1, el, d3, d4 = (C.c_void_p() for _ in range(5))
+        L().o_md_view(self.h, C.byref(d0), C.byref(d1), C.byref(el), C.byref(d3), C.byref(d4))
+        self.elem = np.ctypeslib.as_array(C.cast(el, C.POINTER(C.c_uint8)), shape=(self.n,)).copy()
+        return vv[:self.n].copy(), Tp.value, out, hist[:int(nsteps)], counts, nb
+
     def nvt_run_hist(self, v, masses, kB, T, dt, nsteps):
         """nvt_run with the per-step record of o_md_nvt_run_hist: columns
         T_part, potential energy, xi_1..3, integral of xi_1..3."""
@@ -243,6 +285,14 @@ class OMD:
         if getattr(self, "h", None):
             L().o_md_free(self.h)
             self.h = None
+
+
+def nst_calc(a, dt, dt2, A):
+    """MDMath::calc: (E, F0, F1) for a lower-triangular A [3][3]."""
+    A = f64(np.asarray(A).reshape(3, 3))
+    E, F0, F1 = np.zeros((3, 3)), np.zeros((3, 3)), np.zeros((3, 3))
+    L().o_nst_calc(float(a), float(dt), float(dt2), P(A), P(E), P(F0), P(F1))
+    return E, F0, F1
 
 
 def nhc_run(T_r, ndof, dt, t_relax, ncalls, niters=1):
@@ -278,6 +328,23 @@ class jxa_mutation:
 
     def __enter__(self):
         self.var = C.c_int.in_dll(L(), "o_jxa_mutation")
+        self.var.value = self.k
+        return self
+
+    def __exit__(self, *exc):
+        self.var.value = 0
+        return False
+
+
+class nst_mutation:
+    """context manager planting the oracle's NST mistake (1: the stress term of
+    the box kick with the wrong sign)"""
+
+    def __init__(self, k):
+        self.k = k
+
+    def __enter__(self):
+        self.var = C.c_int.in_dll(L(), "o_nst_mutation")
         self.var.value = self.k
         return self
 

@@ -66,6 +66,35 @@ def test_relaxed_example_matches_the_reference_output(example):
         assert abs(S[k]) < 1e-12
 
 
+def test_library_newton_stage_reaches_the_reference_output(example, tmp_path):
+    # DAE::min_error_false as the example runs it: CG, then the Newton stage
+    # with the example's a_tol 1e-7, 10 Newton steps and GMRES(4)
+    # (examples/diff/in_Cu_diff.py), then the same stage to a tight tolerance
+    d, _, _ = example
+    gd = golden()
+    pot = unpack("Cu_mishin.eam.alloy.gz", tmp_path)
+    g = M.AtomsDMD(M.ForceFieldEAM.setfl(pot, ["Cu"], dmd=True), d["H"], d["x"], d["elem"], d["alpha"], d["c"],
+                   nquad=5, T=300.0, kB=KB_EXAMPLE, hP=HP_EXAMPLE, masses=d["masses"])
+    n = len(d["c"])
+    r = g.min_cg(max_iter=2000, e_tol=0.0, grad_tol=1e-7)
+    _, res0 = g.min_newton(1e-7, 0, 4)
+    k1, res1 = g.min_newton(1e-7, 10, 4)
+    k2, res2 = g.min_newton(1e-11, 20, 8)
+    f, fa, _, fe, FH, _ = g.derivative()
+    S = -np.asarray(FH) / np.linalg.det(d["H"])
+    say(f"CG {r['iters']} iterations, max|grad| {r['grad_max']:.1e}, |f| {res0:.2e}; Newton (1e-7, 10, 4): {k1} "
+        f"steps |f| {res1:.2e}; Newton (1e-11, 20, 8): {k2} steps |f| {res2:.2e}; FE {fe:.6f}, S diag {S[0]:.8e}")
+    # GMRES(4) steps shrink the residual; within 10 of them the example's
+    # stage either meets its tolerance or ends unconverged, as the reference's
+    assert res1 < 0.5 * res0
+    assert res1 <= 1e-7 * math.sqrt(4 * n) or k1 == 10
+    assert res2 <= 1e-11 * math.sqrt(4 * n)
+    assert max(np.abs(f).max(), np.abs(fa).max()) < 1e-8
+    assert abs(fe - gd["FE"]) < 0.005
+    for k in (0, 3, 5):
+        assert abs(S[k] - gd["S_diag"][0]) < 5e-9
+
+
 def test_export_then_import_returns_the_state(example):
     d, g, tmp = example
     p = tmp / "dump.cfg"
