@@ -884,6 +884,64 @@ def gcmc_run(launches=10, attempts=64):
             "dtype": "f64"}
 
 
+def replica_run(nsteps=30):
+    """Batched replica minimisation: the GCMC workload's 4096 periodic Ni-H
+    cells (108-4000 atoms, log-uniform; Ni jittered by 0.1 A), each minimised
+    by MinCG with LineSearchBackTrack for nsteps steps in one persistent
+    launch, launch shape from the tuner, replayed from a HIP graph."""
+    import mapp4py_amd as M
+    from mapp4py_amd import potentials as pot
+
+    cells = [(H, x, e) for x, H, e in pot.gcmc_nih_cells(4096, seed=6, jitter=0.1)]
+    natms = np.array([len(x) for _, x, _ in cells])
+    ff = M.ForceFieldEAM.setfl(pot.NIALH_JEA)
+    rb = M.ReplicaBatch(ff, cells, skin=0.5)
+    waves, tune_ms = rb.tune(nsteps, e_tol=0.0, max_dx=1.0)
+    rb.use_graph(True)
+    rb2 = M.ReplicaBatch(ff, cells, skin=0.5)  # a fresh copy of the batch for the timed run
+    rb2.set_waves(waves)
+    rb2.use_graph(True)
+    rb2.profile(True)
+    t0 = time.perf_counter()
+    res = rb2.min_cg(nsteps, e_tol=0.0, max_dx=1.0)
+    dt = time.perf_counter() - t0
+    ms_k, nk = rb2.profile_read()
+    rb2.profile(False)
+    steps = res["steps"]
+    return {"workload": f"4096 periodic Ni-H cells, 108-4000 atoms log-uniform (4.34 M atoms), NiAlH_jea EAM, "
+                        f"MinCG + LineSearchBackTrack, {nsteps} steps each",
+            "metric": "replica minimisation atom-steps/s", "value": float((natms * steps).sum()) / dt,
+            "unit": "atom-steps/s", "seconds": dt, "kernel_ms": ms_k, "replicas_per_s": len(cells) / dt,
+            "steps_total": int(steps.sum()), "evals_total": int(res["evals"].sum()),
+            "status_counts": {int(k): int(v) for k, v in zip(*np.unique(res["status"], return_counts=True))},
+            "mean_energy_drop_eV_per_atom": float(((res["e0"] - res["e"]) / natms).mean()),
+            "waves_per_class": [int(w) for w in waves], "tuner_ms": [float(t) for t in tune_ms], "dtype": "f64"}
+
+
+def replica_cpu_baseline(seconds):
+    """the oracle's MinCG (back-tracking) over the oracle EAM, one cell after
+    another, on the first cells of the same batch for about `seconds`"""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import o_min
+    import oracle_lib as O
+    from mapp4py_amd import potentials as pot
+
+    cells = pot.gcmc_nih_cells(4096, seed=6, jitter=0.1)
+    ff = O.OEam.read(pot.NIALH_JEA)
+    work, k, t0 = 0, 0, time.perf_counter()
+    while k < len(cells) and (k < 2 or time.perf_counter() - t0 < seconds):
+        x, H, e = cells[k]
+        o = O.OMD(ff, H, x, e, skin=0.5)
+        r = o_min.min_cg(o, 30, e_tol=0.0, max_dx=1.0)
+        work += len(x) * r["steps"]
+        k += 1
+    dt = time.perf_counter() - t0
+    return {"value": work / dt, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle MinCG over the FP64 C restatement, the first {k} cells of the same batch, 30 steps "
+                      f"each, 1 thread, {dt:.1f} s"}
+
+
 def gcmc_cpu_baseline(seconds):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
@@ -930,6 +988,7 @@ def main():
     ap.add_argument("--no-min", action="store_true", help="skip the L-BFGS minimisation measurement")
     ap.add_argument("--no-nst", action="store_true", help="skip the NST measurement")
     ap.add_argument("--no-muvt", action="store_true", help="skip the muVT measurement")
+    ap.add_argument("--no-replica", action="store_true", help="skip the batched replica minimisation")
     ap.add_argument("--dmd-evals-only", action="store_true",
                     help="only the DMD evaluations at 262,144 sites, FP64 and FP32 rows (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -1029,6 +1088,14 @@ def main():
             dmd["implicit_step"]["cpu_baseline"] = dmd_newton_cpu_baseline(args.cpu_seconds / 5)
             dmd["bdf"]["cpu_baseline"] = dmd_bdf_cpu_baseline()
         log(f"[bench] DMD {dmd['value']:.3e} site-evals/s")
+    replica = None
+    if not args.no_replica:
+        log("[bench] replica minimisation")
+        replica = replica_run()
+        if not args.no_cpu:
+            replica["cpu_baseline"] = replica_cpu_baseline(args.cpu_seconds / 3)
+        log(f"[bench] replicas {replica['value']:.3e} atom-steps/s ({replica['seconds']:.3f} s, "
+            f"status {replica['status_counts']}, waves {replica['waves_per_class']})")
     if not args.no_gcmc:
         log("[bench] GCMC")
         gcmc = gcmc_run()
@@ -1044,7 +1111,7 @@ def main():
                                f"r_c=4.95 A, skin 0.5 A, dt 1 fs, 300 K", "natoms": n, "rebuilds": nvt.rebuilds,
                    "pe_per_atom_eV": out[0] / n, "T_K": nvt.T_part},
         "error_vs_fp64": fp64,
-        "roofline": roof, "cpu_baseline": cpu, "feni": feni, "nst": nst, "muvt": muvt, "minimize": mini, "dmd": dmd, "gcmc": gcmc,
+        "roofline": roof, "cpu_baseline": cpu, "feni": feni, "nst": nst, "muvt": muvt, "minimize": mini, "dmd": dmd, "gcmc": gcmc, "replica": replica,
     }
     print(json.dumps(line), flush=True)
 

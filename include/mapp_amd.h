@@ -564,6 +564,43 @@ mapp_status mapp_gcmc_get_cell(mapp_gcmc* g, int cell, double* x, uint8_t* elem,
 mapp_status mapp_gcmc_profile(mapp_gcmc* g, int enable);
 mapp_status mapp_gcmc_profile_read(mapp_gcmc* g, const char* kernel, double* total_ms, int64_t* launches);
 
+/* ---------------- batched replica minimisation ---------------- */
+/* Many small periodic cells ("replicas", each its own box H [9], atoms and
+ * elements, concatenated in replica order), each minimised over its atom
+ * positions by MinCG::__run (src/min_cg.h:453-533) with MinMDHandler's
+ * ls_prep / F / F_reset (src/min_cg.h:231-278) and LineSearchBackTrack::min
+ * (src/ls.h:711-769, init_flag 1), on the EAM energy and forces of
+ * ForceFieldEAM (src/ff_eam.cpp:99-318), with the list rebuilt when an atom
+ * has moved half the skin. The reference runs one system per MinCG; a batch
+ * of replicas (the minimisations a replica driver issues one after another)
+ * runs here in one persistent launch, one wave or one workgroup per replica.
+ * FP64 throughout. Every box must be at least r_c + skin deep along each box
+ * vector. */
+typedef struct mapp_replica mapp_replica;
+mapp_status mapp_replica_create(mapp_eam* ff, int nreplicas, const int32_t* natms, const double* H, const double* x,
+                                const uint8_t* elem, double skin, mapp_replica** out);
+void mapp_replica_destroy(mapp_replica* r);
+/* every replica minimised from its current positions; per replica (any
+ * may be NULL): steps taken, the reference's status (0..5 as MinCG's err;
+ * 101 a list row over capacity, 102 an atom beyond two box lengths, both
+ * reported as errors too), energies at the start and end, energy evaluations */
+mapp_status mapp_replica_min_cg(mapp_replica* r, int max_nsteps, double e_tol, double max_dx, double ls_c,
+                                double ls_rho, double ls_min_alpha, int32_t* steps, int32_t* status, double* e0,
+                                double* e, int32_t* evals);
+/* launch-shape tuner (no counterpart in the reference): one wave per replica
+ * below a size-class threshold, a 256-thread workgroup from there up; every
+ * threshold timed on a copy of the batch, the fastest kept. waves [5] per
+ * class (atoms < 256, < 512, < 1024, < 2048, more), ms [6] per threshold */
+mapp_status mapp_replica_tune(mapp_replica* r, int max_nsteps, double e_tol, double max_dx, double ls_c, double ls_rho,
+                              double ls_min_alpha, int32_t* waves, double* ms);
+mapp_status mapp_replica_set_waves(mapp_replica* r, const int32_t* waves);
+/* capture the launch in a hipGraph and replay it on later calls with the same parameters */
+mapp_status mapp_replica_use_graph(mapp_replica* r, int on);
+/* positions of one replica after the last minimisation (not wrapped back into the box) */
+mapp_status mapp_replica_get_x(mapp_replica* r, int replica, double* x);
+mapp_status mapp_replica_profile(mapp_replica* r, int enable);
+mapp_status mapp_replica_profile_read(mapp_replica* r, double* total_ms, int64_t* launches);
+
 /* Diagnostic (no counterpart in the reference): copies `bytes` between two
  * device buffers `reps` times with a plain streaming kernel and reports the
  * achieved bandwidth (read + write bytes per second / 1e9). Used to calibrate

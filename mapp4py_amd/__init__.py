@@ -25,7 +25,7 @@ import numpy as np
 from ._lib import MappError, check, f64, fp, i8p, i32p, i64p, ip, lib, ptr, u8p
 
 __all__ = ["ForceFieldEAM", "AtomsMD", "AtomsDMD", "NVT", "NST", "MuVT", "MinCG", "MinLBFGS", "LineSearchBackTrack",
-           "LineSearchGoldenSection", "LineSearchBrent", "GCMCBatch", "Surrogate", "MappError", "create_temp",
+           "LineSearchGoldenSection", "LineSearchBrent", "GCMCBatch", "ReplicaBatch", "Surrogate", "MappError", "create_temp",
            "gauss_hermite", "lib"]
 
 
@@ -793,6 +793,80 @@ def gauss_hermite(n):
     w = np.zeros(n)
     check(lib().mapp_gauss_hermite(int(n), ptr(x), ptr(w)))
     return x, w
+
+
+class ReplicaBatch:
+    """Many small periodic cells ("replicas") minimised together: each runs
+    md.min_cg with LineSearchBackTrack over its atom positions
+    (src/min_cg.h:453-533, src/ls.h:711-769) inside one persistent launch.
+    cells: a list of (H, x, elem) per replica."""
+
+    def __init__(self, ff: ForceFieldEAM, cells, skin=0.5):
+        self.ff = ff
+        natms = np.array([len(f64(x).reshape(-1, 3)) for _, x, _ in cells], np.int32)
+        H = np.ascontiguousarray(np.concatenate([f64(h).reshape(1, 9) for h, _, _ in cells]))
+        x = np.ascontiguousarray(np.concatenate([f64(xx).reshape(-1, 3) for _, xx, _ in cells]))
+        elem = np.ascontiguousarray(np.concatenate([np.asarray(e, np.uint8).ravel() for _, _, e in cells]))
+        self.natms = natms
+        self.H = H.reshape(-1, 3, 3)
+        h = C.c_void_p()
+        check(lib().mapp_replica_create(ff._h, len(natms), natms.ctypes.data_as(i32p), ptr(H), ptr(x),
+                                        ptr(elem, u8p), float(skin), C.byref(h)))
+        self._h = h
+
+    def __len__(self):
+        return len(self.natms)
+
+    @staticmethod
+    def _ls(ls):
+        ls = ls or LineSearchBackTrack()
+        if not isinstance(ls, LineSearchBackTrack):
+            raise ValueError("replica minimisation runs the back-tracking line search")
+        return ls
+
+    def min_cg(self, max_nsteps=1000, e_tol=1e-8, max_dx=1.0, ls=None):
+        """per replica: steps, status (MinCG's err), e0, e, evals"""
+        ls = self._ls(ls)
+        n = len(self.natms)
+        steps, status, evals = (np.zeros(n, np.int32) for _ in range(3))
+        e0, e = np.zeros(n), np.zeros(n)
+        check(lib().mapp_replica_min_cg(self._h, int(max_nsteps), float(e_tol), float(max_dx), ls.c, ls.rho,
+                                        ls.min_alpha, steps.ctypes.data_as(i32p), status.ctypes.data_as(i32p),
+                                        ptr(e0), ptr(e), evals.ctypes.data_as(i32p)))
+        return {"steps": steps, "status": status, "e0": e0, "e": e, "evals": evals}
+
+    def tune(self, max_nsteps=1000, e_tol=1e-8, max_dx=1.0, ls=None):
+        ls = self._ls(ls)
+        w = np.zeros(5, np.int32)
+        ms = np.zeros(6)
+        check(lib().mapp_replica_tune(self._h, int(max_nsteps), float(e_tol), float(max_dx), ls.c, ls.rho,
+                                      ls.min_alpha, w.ctypes.data_as(i32p), ptr(ms)))
+        return w, ms
+
+    def set_waves(self, waves):
+        w = np.ascontiguousarray(waves, dtype=np.int32)
+        check(lib().mapp_replica_set_waves(self._h, w.ctypes.data_as(i32p)))
+
+    def use_graph(self, on=True):
+        check(lib().mapp_replica_use_graph(self._h, int(bool(on))))
+
+    def x(self, r):
+        out = np.zeros((int(self.natms[r]), 3))
+        check(lib().mapp_replica_get_x(self._h, int(r), ptr(out)))
+        return out
+
+    def profile(self, enable=True):
+        check(lib().mapp_replica_profile(self._h, int(enable)))
+
+    def profile_read(self):
+        ms, n = C.c_double(), C.c_int64()
+        check(lib().mapp_replica_profile_read(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mapp_replica_destroy(self._h)
+            self._h = None
 
 
 class GCMCBatch:

@@ -131,6 +131,17 @@ SIGNATURES = {
     "mapp_gcmc_create": (C.c_int, [C.c_void_p, C.c_int, i32p, dp, dp, u8p, i32p, C.c_int, C.c_double, C.c_double,
                                    C.c_double, C.c_double, dp, i32p, vpp]),
     "mapp_gcmc_destroy": (None, [C.c_void_p]),
+    "mapp_replica_create": (C.c_int, [C.c_void_p, C.c_int, i32p, dp, dp, u8p, C.c_double, vpp]),
+    "mapp_replica_destroy": (None, [C.c_void_p]),
+    "mapp_replica_min_cg": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, i32p, i32p, dp, dp, i32p]),
+    "mapp_replica_tune": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, i32p, dp]),
+    "mapp_replica_set_waves": (C.c_int, [C.c_void_p, i32p]),
+    "mapp_replica_use_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "mapp_replica_get_x": (C.c_int, [C.c_void_p, C.c_int, dp]),
+    "mapp_replica_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "mapp_replica_profile_read": (C.c_int, [C.c_void_p, dp, C.POINTER(C.c_int64)]),
     "mapp_gcmc_attempts": (C.c_int, [C.c_void_p, C.c_int, i8p, dp]),
     "mapp_gcmc_tune": (C.c_int, [C.c_void_p, C.c_int, i32p, dp]),
     "mapp_gcmc_set_waves": (C.c_int, [C.c_void_p, i32p]),

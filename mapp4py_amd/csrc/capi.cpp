@@ -9,6 +9,7 @@
 #include "cfg.hpp"
 #include "dmd.hpp"
 #include "gcmc.hpp"
+#include "replica.hpp"
 #include "surrogate.hpp"
 #include "md.hpp"
 #include "linesearch.hpp"
@@ -41,6 +42,10 @@ struct mapp_cfg_md {
 };
 struct mapp_gcmc {
     std::unique_ptr<mapp::GcmcBatch> sys;
+};
+
+struct mapp_replica {
+    std::unique_ptr<mapp::ReplicaBatch> sys;
 };
 
 namespace {
@@ -1146,6 +1151,92 @@ mapp_status mapp_dmd_use_surrogate(mapp_dmd* d, mapp_surr* s)
 mapp_status mapp_dmd_surrogate_coverage(mapp_dmd* d, int64_t* net_evals, int64_t* quadrature_slots)
 {
     return guard([&] { NEED(d); d->sys->surrogate_coverage(net_evals, quadrature_slots); });
+}
+
+mapp_status mapp_replica_create(mapp_eam* ff, int nreplicas, const int32_t* natms, const double* H, const double* x,
+                                const uint8_t* elem, double skin, mapp_replica** out)
+{
+    return guard([&] {
+        NEED(ff);
+        NEED(natms);
+        NEED(H);
+        NEED(x);
+        NEED(elem);
+        NEED(out);
+        *out = nullptr;
+        auto r = std::make_unique<mapp_replica>();
+        r->sys = std::make_unique<mapp::ReplicaBatch>(ff->ff, nreplicas, natms, H, x, elem, skin);
+        *out = r.release();
+    });
+}
+
+void mapp_replica_destroy(mapp_replica* r) { delete r; }
+
+mapp_status mapp_replica_min_cg(mapp_replica* r, int max_nsteps, double e_tol, double max_dx, double ls_c,
+                                double ls_rho, double ls_min_alpha, int32_t* steps, int32_t* status, double* e0,
+                                double* e, int32_t* evals)
+{
+    return guard([&] {
+        NEED(r);
+        r->sys->min_cg(max_nsteps, e_tol, max_dx, ls_c, ls_rho, ls_min_alpha, steps, status, e0, e, evals);
+    });
+}
+
+mapp_status mapp_replica_tune(mapp_replica* r, int max_nsteps, double e_tol, double max_dx, double ls_c, double ls_rho,
+                              double ls_min_alpha, int32_t* waves, double* ms)
+{
+    return guard([&] {
+        NEED(r);
+        const mapp::RepTuneReport t = r->sys->tune(max_nsteps, e_tol, max_dx, ls_c, ls_rho, ls_min_alpha);
+        for (int k = 0; k < 5; ++k)
+            if (waves) waves[k] = t.waves[k];
+        for (int k = 0; k <= 5; ++k)
+            if (ms) ms[k] = t.ms[k];
+    });
+}
+
+mapp_status mapp_replica_set_waves(mapp_replica* r, const int32_t* waves)
+{
+    return guard([&] {
+        NEED(r);
+        NEED(waves);
+        r->sys->set_waves(waves);
+    });
+}
+
+mapp_status mapp_replica_use_graph(mapp_replica* r, int on)
+{
+    return guard([&] { NEED(r); r->sys->use_graph(on != 0); });
+}
+
+mapp_status mapp_replica_get_x(mapp_replica* r, int replica, double* x)
+{
+    return guard([&] {
+        NEED(r);
+        NEED(x);
+        r->sys->get(replica, x);
+    });
+}
+
+mapp_status mapp_replica_profile(mapp_replica* r, int enable)
+{
+    return guard([&] {
+        NEED(r);
+        auto& p = r->sys->profiler();
+        p.reset();
+        p.on = enable != 0;
+    });
+}
+
+mapp_status mapp_replica_profile_read(mapp_replica* r, double* total_ms, int64_t* launches)
+{
+    return guard([&] {
+        NEED(r);
+        auto& p = r->sys->profiler();
+        p.flush();
+        if (total_ms) *total_ms = p.total_ms[mapp::K_REPLICA];
+        if (launches) *launches = p.launches[mapp::K_REPLICA];
+    });
 }
 
 }  // extern "C"

@@ -89,6 +89,7 @@ enum KernelId : int {
     K_MD_MIN,
     K_DMD_PREPJ,
     K_DMD_JXA,
+    K_REPLICA,
     K_COUNT
 };
 
@@ -99,7 +100,7 @@ inline const char* kernel_name(int k)
                                          "dmd_energy", "surr_infer", "surr_train", "gcmc",
                                          "dmd_cd",     "dmd_j_state", "dmd_j",
                                          "dmd_newton", "md_min_vec", "dmd_prepj",
-                                         "dmd_jxa"};
+                                         "dmd_jxa",    "replica"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "";
 }
 
