@@ -269,15 +269,24 @@ mapp_status mapp_dmd_derivative(mapp_dmd* d, int with_c, const double* ext_mu, d
  * of the quadrature passes, 1 (one pair at a time), 2 or 4 (default). Any
  * value gives the same results to rounding. */
 mapp_status mapp_dmd_set_batch(mapp_dmd* d, int batch);
-/* Precision of the quadrature passes behind mapp_dmd_derivative,
+/* Table evaluation of the quadrature passes behind mapp_dmd_derivative,
  * mapp_dmd_value and the DMD minimisers (ForceFieldEAMDMD::__force_calc /
  * __energy_calc, src/ff_eam_dmd.cpp:126-360,835-953; no such switch in the
- * reference, which is FP64 throughout): on != 0 evaluates the pair splines
- * from FP32 copies of the table rows, the quadrature moments and every sum
- * staying FP64 (batches 2 and 4; batch 1 stays FP64); 0 (the default) reads
- * the FP64 rows. The concentration-rate path (mapp_dmd_c_d, mapp_dmd_J,
+ * reference, which is FP64 throughout). mode 0 (the default) reads the FP64
+ * coefficient rows. mode 1 evaluates each pair spline in FP32 from (value,
+ * slope) pairs of the knots of the table's distinct functions, the top
+ * knots of which every workgroup holds in LDS; where a table function is not
+ * the Hermite cubic of its knots, mode 1 reads the FP32 rows. mode 2 reads
+ * FP32 copies of the coefficient rows. The FP32 modes keep the quadrature
+ * moments and every sum in FP64 and apply to batches 2 and 4 (batch 1 stays
+ * FP64). The concentration-rate path (mapp_dmd_c_d, mapp_dmd_J,
  * mapp_dmd_newton, mapp_dmd_bdf_run) always reads the FP64 rows. */
-mapp_status mapp_dmd_set_fp32(mapp_dmd* d, int on);
+mapp_status mapp_dmd_set_fp32(mapp_dmd* d, int mode);
+
+/* The first knot of table mode 1's LDS window (the knots below it are read
+ * from device memory), or -1 where mode 1 reads the FP32 rows. No
+ * counterpart in the reference: it reports this library's table layout. */
+mapp_status mapp_dmd_knot_window(mapp_dmd* d, int* first_knot);
 /* ForceFieldDMD::c_d_calc (src/ff_dmd.cpp:449-456) with
  * ForceFieldEAMDMD::__c_d_calc (src/ff_eam_dmd.cpp:1359-1473): the
  * concentration rates dc/dt of every component at the current state, over the

@@ -601,10 +601,18 @@ class AtomsDMD:
         check(lib().mapp_dmd_force_calc_static(self._h, ptr(out)))
         return out[0], out[1:7].copy(), out[7]
 
-    def set_fp32(self, on=True):
-        """FP32 table rows (FP64 moments and sums) in the free energy and its
-        gradient; the concentration-rate path stays FP64"""
-        check(lib().mapp_dmd_set_fp32(self._h, int(bool(on))))
+    def set_fp32(self, on=True, rows=False):
+        """FP32 pair splines (FP64 moments and sums) in the free energy and its
+        gradient: from the knots held in LDS, or with rows=True from FP32
+        copies of the coefficient rows; the concentration-rate path stays FP64"""
+        check(lib().mapp_dmd_set_fp32(self._h, 0 if not on else (2 if rows else 1)))
+
+    def knot_window(self):
+        """first knot of the LDS window of the FP32 knots, or -1 where they
+        are not in LDS"""
+        k = C.c_int()
+        check(lib().mapp_dmd_knot_window(self._h, C.byref(k)))
+        return k.value
 
     def value(self, with_c=False, ext_mu=None):
         e = C.c_double()

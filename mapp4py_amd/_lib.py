@@ -72,6 +72,7 @@ SIGNATURES = {
     "mapp_dmd_set_box": (C.c_int, [C.c_void_p, dp, dp]),
     "mapp_dmd_force_calc_static": (C.c_int, [C.c_void_p, dp]),
     "mapp_dmd_set_fp32": (C.c_int, [C.c_void_p, C.c_int]),
+    "mapp_dmd_knot_window": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "mapp_dmd_set_batch": (C.c_int, [C.c_void_p, C.c_int]),
     "mapp_dmd_c_d": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                C.POINTER(C.c_double)]),

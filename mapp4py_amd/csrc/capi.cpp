@@ -529,9 +529,18 @@ mapp_status mapp_dmd_force_calc_static(mapp_dmd* d, double* out8)
     });
 }
 
-mapp_status mapp_dmd_set_fp32(mapp_dmd* d, int on)
+mapp_status mapp_dmd_set_fp32(mapp_dmd* d, int mode)
 {
-    return guard([&] { NEED(d); d->sys->set_f32(on != 0); });
+    return guard([&] { NEED(d); d->sys->set_f32(mode); });
+}
+
+mapp_status mapp_dmd_knot_window(mapp_dmd* d, int* first_knot)
+{
+    return guard([&] {
+        NEED(d);
+        NEED(first_knot);
+        *first_knot = d->sys->knots_in_lds() ? d->sys->knot_window_first() : -1;
+    });
 }
 
 mapp_status mapp_dmd_set_batch(mapp_dmd* d, int batch)

@@ -435,7 +435,7 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass2(int n, const double4* __re
 // point the table rows of every pair of the batch. A wave keeps many
 // independent loads in flight instead of one dependent chain per pair (the
 // passes are bound by load latency, not FP64 arithmetic). Entries past the
-// end of a row, and pairs beyond the cut, read table row 0 and are selected
+// end of a row, and pairs beyond the cut, read a valid row and are selected
 // away, so a batch has no branches. The arithmetic per pair is that of
 // smear_quad, point by point in the same order.
 
@@ -457,8 +457,405 @@ __device__ inline int dknot(const DmdArgs& A, double u, double& p)
     return m;
 }
 
-// pass 1: rho of every component (smeared r*rho_ba) and the pair energy
-// (smeared r*phi); only table functions 0 and 2 are read
+// Table sources of the batched passes. fetch<B, K>(ks, tt, m, live, c) gives,
+// for each pair b of a batch, interval m[b] of function ks[j] (0 r*phi,
+// 1 r*rho_{i->j}, 2 r*rho_{j->i}) of element pair tt[b]; a pair that is not
+// live reads a valid row that the caller selects away. basis(p) holds what
+// the offset p in [0, 1] contributes to every function of the interval, and
+// eval(c, basis) is the interval's cubic there. Acc1 and Acc2 are the types
+// in which passes 1 and 2 sum the Hermite points of one pair into its
+// moments, and recip is the reciprocal of a pair's distance and width.
+
+// coefficient rows [ne*ne][nr][3] in global memory, FP64 or FP32
+template <bool F32>
+struct RowsGlobal {
+    using Row = DRow<F32>;
+    const Row* T;
+    int nr;
+    template <int B, int K>
+    __device__ void fetch(const int (&ks)[K], const int (&tt)[B], const int (&m)[B], const bool (&live)[B],
+                          Row (&c)[B][K]) const
+    {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const size_t row = live[b] ? ((size_t)tt[b] * nr + m[b]) * 3 : 0;
+#pragma unroll
+            for (int j = 0; j < K; ++j) c[b][j] = T[row + ks[j]];
+        }
+    }
+    using Acc1 = double;
+    using Acc2 = double;
+    using Basis = double;
+    __device__ static Basis basis(double p) { return p; }
+    __device__ static double eval(const Row& c, double p) { return hv(c, p); }
+    __device__ static double recip(double v) { return 1.0 / v; }
+};
+
+// FP32 knots. Every table function is the Hermite cubic of its interval's
+// two knot values and slopes per unit offset (c2 = 3 dy - 2 s0 - s1,
+// c3 = s0 + s1 - 2 dy, as the spline builder makes them), so one (value,
+// slope) pair per knot holds a function in 8 bytes per knot, and an interval
+// is two adjacent pairs. Only the distinct functions are kept: fn maps
+// (element pair, function) to one of them. The top nwin knots of every
+// distinct function sit in LDS; intervals below m_lo read the global copy.
+// A random 16-byte row from L2 costs the vector cache one tag lookup per
+// lane; from LDS a few bank cycles. The cubic is evaluated in the Hermite
+// basis, y0 + (y1 - y0) h1 + s0 h2 + s1 h3, with h1..h3 of the offset shared
+// by the three functions. Pass 2 sums one pair's Hermite points in FP32;
+// pass 1, whose pair terms make the free energy, and both passes' sums over
+// pairs stay FP64.
+struct KnotsLds {
+    using Row = float4;  // y0, s0, y1, s1
+    const float2* glob;  // [nf][nr]
+    const float2* win;   // [nf][nwin], LDS
+    const int* fn;       // [ne*ne*3], LDS
+    int nr, nwin, m_lo;
+    template <int B, int K>
+    __device__ void fetch(const int (&ks)[K], const int (&tt)[B], const int (&m)[B], const bool (&live)[B],
+                          float4 (&c)[B][K]) const
+    {
+        bool low[B], any = false;
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            low[b] = live[b] && m[b] < m_lo;
+            any |= low[b];
+            // m <= nr - 2, so r + 1 <= nwin - 1; pairs not in the window read row 0 (nwin >= 2)
+            const int r = live[b] && !low[b] ? m[b] - m_lo : 0;
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                const float2* q = win + fn[tt[b] * 3 + ks[j]] * nwin + r;
+                const float2 a = q[0], d = q[1];
+                c[b][j] = make_float4(a.x, a.y, d.x, d.y);
+            }
+        }
+        if (__ballot(any) != 0ull) {
+#pragma unroll
+            for (int b = 0; b < B; ++b)
+                if (low[b])
+#pragma unroll
+                    for (int j = 0; j < K; ++j) {
+                        const float2* q = glob + (size_t)fn[tt[b] * 3 + ks[j]] * nr + m[b];
+                        const float2 a = q[0], d = q[1];
+                        c[b][j] = make_float4(a.x, a.y, d.x, d.y);
+                    }
+        }
+    }
+    using Acc1 = double;
+    using Acc2 = float;
+    struct Basis {
+        float h1, h2, h3;
+    };
+    __device__ static Basis basis(double p)
+    {
+        const float q = (float)p, q1 = 1.0f - q, q2 = q * q;
+        return Basis{q2 * (3.0f - 2.0f * q), q * q1 * q1, -q2 * q1};
+    }
+    __device__ static float eval(const float4& c, const Basis& h)
+    {
+        return c.x + (c.z - c.x) * h.h1 + c.y * h.h2 + c.w * h.h3;
+    }
+    // the hardware reciprocal and one Newton step (squares its relative
+    // error), in place of the correctly rounded division
+    __device__ static double recip(double v)
+    {
+        const double y = __builtin_amdgcn_rcp(v);
+        return fma(fma(-v, y, 1.0), y, y);
+    }
+};
+
+// Neighbor data of the batched passes: width, concentration and element of
+// a listed component (pass 1), and its F'(rho) (pass 2). Every item is a
+// gather from a random address, so the LDS passes read them packed: (alpha,
+// c) in one 16-byte record, and (F', element) in another that pass 1 writes
+// for pass 2.
+struct NbPlain {
+    const double* al;
+    const double* cc;
+    const int* el;
+    const double* dE;
+    __device__ void p1(int jc, double& a, double& c, int& e) const
+    {
+        a = al[jc];
+        c = cc[jc];
+        e = el[jc];
+    }
+    __device__ void p2(int jc, double& a, double& c, int& e, double& d) const
+    {
+        p1(jc, a, c, e);
+        d = dE[jc];
+    }
+    // pass 1's output for pass 2 beside dE: none
+    __device__ void put(int, double, int) const {}
+};
+
+struct NbPacked {
+    const double2* ac;  // alpha, c
+    const int* el;
+    double2* ed;        // F'(rho), element (pass 1 writes it)
+    __device__ void p1(int jc, double& a, double& c, int& e) const
+    {
+        const double2 v = ac[jc];
+        a = v.x;
+        c = v.y;
+        e = el[jc];
+    }
+    __device__ void p2(int jc, double& a, double& c, int& e, double& d) const
+    {
+        const double2 v = ac[jc], w = ed[jc];
+        a = v.x;
+        c = v.y;
+        d = w.x;
+        e = (int)w.y;
+    }
+    __device__ void put(int ci, double d, int e) const { ed[ci] = make_double2(d, (double)e); }
+};
+
+// pass 1 of site i: rho of every component (smeared r*rho_ba) and the pair
+// energy (smeared r*phi); only table functions 0 and 2 are read
+template <int B, class Src, class Nb>
+__device__ inline void pass1_site(int i, const double4* __restrict__ x, const double* __restrict__ al,
+                                  const double* __restrict__ cc, const int* __restrict__ el,
+                                  const int* __restrict__ nn, const uint32_t* __restrict__ list, int ldl, int maxn,
+                                  const DmdArgs& A, const BoxDev& bd, int energy_only, const Src& T, const Nb& nb,
+                                  double* __restrict__ E, double* __restrict__ dE, double (&acc)[2])
+{
+    const int ks[2] = {0, 2};
+    const int C = A.C;
+    const double4 xi = x[i];
+    double cv = 1.0;
+    for (int ic = 0; ic < C; ++ic) {
+        const int ci_idx = i * C + ic;
+        const double ci = cc[ci_idx];
+        if (ci > 0.0) cv -= ci;
+        if (ci < 0.0) continue;
+        const double ai = al[ci_idx];
+        const int ei = el[ci_idx];
+        double rho = 0.0, ep = 0.0;
+        const int cnt = nn[ci_idx];
+        for (int k0 = 0; k0 < cnt; k0 += B) {
+            uint32_t e[B];
+            double4 xj[B];
+            double aj[B], cj[B];
+            int tt[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) e[b] = dlist_entry(list, ldl, maxn, k0 + b, ci_idx, cnt);
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int jc = (int)(e[b] & kIndexMask);
+                int ej;
+                xj[b] = x[jc / C];
+                nb.p1(jc, aj[b], cj[b], ej);
+                tt[b] = ei * A.ne + ej;
+            }
+            PairGeo pg[B];
+            bool live[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                pg[b] = pair_geo(xi, xj[b], e[b] >> kIndexBits, bd.h, ai, aj[b], A.xmax, A.cut[tt[b]]);
+                live[b] = pg[b].live && k0 + b < cnt;
+            }
+            using Acc = typename Src::Acc1;
+            Acc H0[B], H2[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) H0[b] = H2[b] = 0;
+            for (int l = 0; l < A.N; ++l) {
+                const double xl = A.xi[l], wl = A.wi[l];
+                int m[B];
+                double p[B], s[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const double u = pg[b].r - xl * pg[b].a;
+                    m[b] = dknot(A, u, p[b]);
+                    s[b] = u < 0.0 ? -wl : wl;
+                }
+                typename Src::Row c[B][2];
+                T.template fetch<B, 2>(ks, tt, m, live, c);
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const typename Src::Basis h = Src::basis(p[b]);
+                    const Acc sb = (Acc)s[b];
+                    H0[b] += Src::eval(c[b][0], h) * sb;
+                    H2[b] += Src::eval(c[b][1], h) * sb;
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const double rinv = Src::recip(pg[b].r);
+                const double rp0 = kPiInSq * rinv * (double)H0[b], rp2 = kPiInSq * rinv * (double)H2[b];
+                rho += live[b] ? cj[b] * rp2 : 0.0;
+                ep += live[b] ? 0.5 * ci * cj[b] * rp0 : 0.0;
+            }
+        }
+        double F, dF;
+        femb5(A, ei, rho, F, dF);
+        E[ci_idx] = F;
+        dE[ci_idx] = dF;
+        nb.put(ci_idx, dF, ei);
+        if (!energy_only || ci != 0.0) acc[0] += ci * (F + A.mu0[ei] - 3.0 * A.kBT * log(ai));
+        acc[0] += ep;
+        acc[1] += xlogx(ci);
+    }
+    acc[1] += xlogx(cv);
+}
+
+// pass 2 of site i: forces on x and alpha, chemical-potential sums, virial.
+// With UNIT it also writes the gradient per unit c that
+// ForceFieldEAMDMD::calc_mu leaves in f and f_alpha
+// (src/ff_eam_dmd.cpp:1189-1278): every pair weighted by c_j and the
+// component's fcoef (src/ff_eam_dmd.cpp:995-1046) instead of c_i c_j, and
+// d/dalpha with -3 kT / alpha, defined also where c_i = 0.
+template <int B, bool UNIT, class Src, class Nb>
+__device__ inline void pass2_site(int i, const double4* __restrict__ x, const double* __restrict__ al,
+                                  const double* __restrict__ cc, const int* __restrict__ el,
+                                  const int* __restrict__ nn, const uint32_t* __restrict__ list, int ldl, int maxn,
+                                  const DmdArgs& A, const BoxDev& bd, const Src& T, const Nb& nb,
+                                  const double* __restrict__ E,
+                                  const double* __restrict__ dE, double4* __restrict__ f, double* __restrict__ fa,
+                                  double* __restrict__ mu, double4* __restrict__ gu, double* __restrict__ gau,
+                                  const JState& J, double (&acc)[6])
+{
+    const int ks[3] = {0, 1, 2};
+    const int C = A.C;
+    const double4 xi = x[i];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    double c_tot = 0.0;
+    if (UNIT)
+        for (int q = 0; q < C; ++q) {
+            const double v = cc[i * C + q];
+            if (v > 0.0) c_tot += v;
+        }
+    for (int ic = 0; ic < C; ++ic) {
+        const int ci_idx = i * C + ic;
+        const double ci = cc[ci_idx];
+        if (ci < 0.0) {
+            fa[ci_idx] = 0.0;
+            mu[ci_idx] = 0.0;
+            if (UNIT) {
+                gau[ci_idx] = 0.0;
+                J.cnt[ci_idx] = 0;
+                J.ddE[ci_idx] = 0.0;
+            }
+            continue;
+        }
+        const double ai = al[ci_idx];
+        const int ei = el[ci_idx];
+        const double dEi = dE[ci_idx];
+        // fcoef: 1 with one component; else c_i / sum of positive c, or
+        // 1 for the zero components of an empty site
+        const double fcoef = C == 1 ? 1.0 : (c_tot == 0.0 ? (ci == 0.0 ? 1.0 : 0.0) : ci / c_tot);
+        double fai = 0.0, mui = 0.0, gai = 0.0, rho = 0.0;
+        const int cnt = nn[ci_idx];
+        for (int k0 = 0; k0 < cnt; k0 += B) {
+            uint32_t e[B];
+            double4 xj[B];
+            double aj[B], cj[B], dEj[B];
+            int tt[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) e[b] = dlist_entry(list, ldl, maxn, k0 + b, ci_idx, cnt);
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int jc = (int)(e[b] & kIndexMask);
+                int ej;
+                xj[b] = x[jc / C];
+                nb.p2(jc, aj[b], cj[b], ej, dEj[b]);
+                tt[b] = ei * A.ne + ej;
+            }
+            PairGeo pg[B];
+            bool live[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                pg[b] = pair_geo(xi, xj[b], e[b] >> kIndexBits, bd.h, ai, aj[b], A.xmax, A.cut[tt[b]]);
+                live[b] = pg[b].live && k0 + b < cnt;
+            }
+            using Acc = typename Src::Acc2;
+            Acc H[B][3][3];
+#pragma unroll
+            for (int b = 0; b < B; ++b)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) H[b][k][0] = H[b][k][1] = H[b][k][2] = 0;
+            for (int l = 0; l < A.N; ++l) {
+                const double xl = A.xi[l], wl = A.wi[l];
+                const Acc xa = (Acc)xl, xa2 = (Acc)(xl * xl);
+                int m[B];
+                double p[B], s[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const double u = pg[b].r - xl * pg[b].a;
+                    m[b] = dknot(A, u, p[b]);
+                    s[b] = u < 0.0 ? -wl : wl;
+                }
+                typename Src::Row c[B][3];
+                T.template fetch<B, 3>(ks, tt, m, live, c);
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const typename Src::Basis h = Src::basis(p[b]);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const Acc t = Src::eval(c[b][k], h) * (Acc)s[b];
+                        H[b][k][0] += t;
+                        H[b][k][1] += t * xa;
+                        H[b][k][2] += t * xa2;
+                    }
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const double rinv = Src::recip(pg[b].r), ainv = Src::recip(pg[b].a);
+                double rp[3], drr[3], dal[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    rp[k] = kPiInSq * rinv * (double)H[b][k][0];
+                    drr[k] = -(rinv * rinv) * (rp[k] + 2.0 * kPiInSq * ainv * (double)H[b][k][1]);
+                    dal[k] = -(ainv * ainv) * (rp[k] - 2.0 * kPiInSq * rinv * (double)H[b][k][2]);
+                }
+                const double fpair = -(drr[2] * dEi + drr[1] * dEj[b] + drr[0]) * ci * cj[b];
+                const double apair = -(dal[2] * dEi + dal[1] * dEj[b] + dal[0]) * ci * cj[b];
+                const double du = cj[b] * (rp[0] + rp[1] * dEj[b]);
+                const double fp = live[b] ? fpair : 0.0;
+                mui += live[b] ? du : 0.0;
+                fai += live[b] ? ai * apair : 0.0;
+                const double dx = pg[b].dx, dy = pg[b].dy, dz = pg[b].dz;
+                fx += fp * dx;
+                fy += fp * dy;
+                fz += fp * dz;
+                if (UNIT && k0 + b < cnt) {  // the Jacobian's slot values (as k_dmd_jrp)
+                    const size_t slot = (size_t)(k0 + b) * ldl + ci_idx;
+                    J.idx[slot] = e[b] & kIndexMask;
+                    J.r2[slot] = live[b] ? rp[2] : 0.0;
+                    J.r1[slot] = live[b] ? rp[1] : 0.0;
+                    J.t0[slot] = live[b] ? rp[0] + dEi * rp[2] + dEj[b] * rp[1] : 0.0;
+                    rho += live[b] ? cj[b] * rp[2] : 0.0;
+                }
+                if (UNIT) {
+                    const double gp = live[b] ? (drr[2] * dEi + drr[1] * dEj[b] + drr[0]) * cj[b] * fcoef : 0.0;
+                    gx += gp * dx;
+                    gy += gp * dy;
+                    gz += gp * dz;
+                    gai += live[b] ? ai * (dal[2] * dEi + dal[1] * dEj[b] + dal[0]) * cj[b] : 0.0;
+                }
+                const double hf = 0.5 * fp;
+                acc[0] += hf * dx * dx;
+                acc[1] += hf * dx * dy;
+                acc[2] += hf * dx * dz;
+                acc[3] += hf * dy * dy;
+                acc[4] += hf * dy * dz;
+                acc[5] += hf * dz * dz;
+            }
+        }
+        fa[ci_idx] = fai + 3.0 * A.kBT * ci / ai;
+        mu[ci_idx] = E[ci_idx] + mui;
+        if (UNIT) {
+            gau[ci_idx] = gai - 3.0 * A.kBT / ai;
+            J.cnt[ci_idx] = cnt;
+            J.ddE[ci_idx] = ddemb5(A, ei, rho);
+        }
+    }
+    f[i] = make_double4(fx, fy, fz, 0.0);
+    if (UNIT) gu[i] = make_double4(gx, gy, gz, 0.0);
+}
+
+// the batched passes over the global rows, one site per thread
 template <int B, bool F32>
 __global__ __launch_bounds__(kBlock) void k_dmd_pass1q(int n, const double4* __restrict__ x, const double* __restrict__ al,
                                                        const double* __restrict__ cc, const int* __restrict__ el,
@@ -470,90 +867,12 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass1q(int n, const double4* __r
     const int blk = xcd_block();
     const int i = blk * kBlock + threadIdx.x;
     double acc[2] = {0.0, 0.0};  // free energy without the kT S term, S
-    if (i < n) {
-        const int C = A.C;
-        const double4 xi = x[i];
-        double cv = 1.0;
-        for (int ic = 0; ic < C; ++ic) {
-            const int ci_idx = i * C + ic;
-            const double ci = cc[ci_idx];
-            if (ci > 0.0) cv -= ci;
-            if (ci < 0.0) continue;
-            const double ai = al[ci_idx];
-            const int ei = el[ci_idx];
-            double rho = 0.0, ep = 0.0;
-            const int cnt = nn[ci_idx];
-            for (int k0 = 0; k0 < cnt; k0 += B) {
-                uint32_t e[B];
-                double4 xj[B];
-                double aj[B], cj[B];
-                int tt[B];
-#pragma unroll
-                for (int b = 0; b < B; ++b) e[b] = dlist_entry(list, ldl, maxn, k0 + b, ci_idx, cnt);
-#pragma unroll
-                for (int b = 0; b < B; ++b) {
-                    const int jc = (int)(e[b] & kIndexMask);
-                    xj[b] = x[jc / C];
-                    aj[b] = al[jc];
-                    cj[b] = cc[jc];
-                    tt[b] = ei * A.ne + el[jc];
-                }
-                PairGeo pg[B];
-                bool live[B];
-#pragma unroll
-                for (int b = 0; b < B; ++b) {
-                    pg[b] = pair_geo(xi, xj[b], e[b] >> kIndexBits, bd.h, ai, aj[b], A.xmax, A.cut[tt[b]]);
-                    live[b] = pg[b].live && k0 + b < cnt;
-                }
-                double H0[B], H2[B];
-#pragma unroll
-                for (int b = 0; b < B; ++b) H0[b] = H2[b] = 0.0;
-                const DRow<F32>* T = drows<F32>(A);
-                for (int l = 0; l < A.N; ++l) {
-                    const double xl = A.xi[l], wl = A.wi[l];
-                    DRow<F32> c0[B], c2[B];
-                    double p[B], s[B];
-#pragma unroll
-                    for (int b = 0; b < B; ++b) {
-                        const double u = pg[b].r - xl * pg[b].a;
-                        const int m = dknot(A, u, p[b]);
-                        s[b] = u < 0.0 ? -wl : wl;
-                        const size_t row = live[b] ? ((size_t)tt[b] * A.nr + m) * 3 : 0;
-                        c0[b] = T[row];
-                        c2[b] = T[row + 2];
-                    }
-#pragma unroll
-                    for (int b = 0; b < B; ++b) {
-                        H0[b] += hv(c0[b], p[b]) * s[b];
-                        H2[b] += hv(c2[b], p[b]) * s[b];
-                    }
-                }
-#pragma unroll
-                for (int b = 0; b < B; ++b) {
-                    const double rinv = 1.0 / pg[b].r;
-                    const double rp0 = kPiInSq * rinv * H0[b], rp2 = kPiInSq * rinv * H2[b];
-                    rho += live[b] ? cj[b] * rp2 : 0.0;
-                    ep += live[b] ? 0.5 * ci * cj[b] * rp0 : 0.0;
-                }
-            }
-            double F, dF;
-            femb5(A, ei, rho, F, dF);
-            E[ci_idx] = F;
-            dE[ci_idx] = dF;
-            if (!energy_only || ci != 0.0) acc[0] += ci * (F + A.mu0[ei] - 3.0 * A.kBT * log(ai));
-            acc[0] += ep;
-            acc[1] += xlogx(ci);
-        }
-        acc[1] += xlogx(cv);
-    }
+    if (i < n)
+        pass1_site<B>(i, x, al, cc, el, nn, list, ldl, maxn, A, bd, energy_only, RowsGlobal<F32>{drows<F32>(A), A.nr},
+                      NbPlain{al, cc, el, dE}, E, dE, acc);
     block_sum_store<2>(acc, part + (size_t)blk * kPartW);
 }
 
-// pass 2: forces on x and alpha, chemical-potential sums, virial. With UNIT
-// it also writes the gradient per unit c that ForceFieldEAMDMD::calc_mu
-// leaves in f and f_alpha (src/ff_eam_dmd.cpp:1189-1278): every pair weighted
-// by c_j and the component's fcoef (src/ff_eam_dmd.cpp:995-1046) instead of
-// c_i c_j, and d/dalpha with -3 kT / alpha, defined also where c_i = 0.
 template <int B, bool UNIT, bool F32>
 __global__ __launch_bounds__(kBlock) void k_dmd_pass2q(int n, const double4* __restrict__ x, const double* __restrict__ al,
                                                        const double* __restrict__ cc, const int* __restrict__ el,
@@ -567,147 +886,93 @@ __global__ __launch_bounds__(kBlock) void k_dmd_pass2q(int n, const double4* __r
     const int blk = xcd_block();
     const int i = blk * kBlock + threadIdx.x;
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    if (i < n) {
-        const int C = A.C;
-        const double4 xi = x[i];
-        double fx = 0.0, fy = 0.0, fz = 0.0;
-        double gx = 0.0, gy = 0.0, gz = 0.0;
-        double c_tot = 0.0;
-        if (UNIT)
-            for (int q = 0; q < C; ++q) {
-                const double v = cc[i * C + q];
-                if (v > 0.0) c_tot += v;
-            }
-        for (int ic = 0; ic < C; ++ic) {
-            const int ci_idx = i * C + ic;
-            const double ci = cc[ci_idx];
-            if (ci < 0.0) {
-                fa[ci_idx] = 0.0;
-                mu[ci_idx] = 0.0;
-                if (UNIT) {
-                    gau[ci_idx] = 0.0;
-                    J.cnt[ci_idx] = 0;
-                    J.ddE[ci_idx] = 0.0;
-                }
-                continue;
-            }
-            const double ai = al[ci_idx];
-            const int ei = el[ci_idx];
-            const double dEi = dE[ci_idx];
-            // fcoef: 1 with one component; else c_i / sum of positive c, or
-            // 1 for the zero components of an empty site
-            const double fcoef = C == 1 ? 1.0 : (c_tot == 0.0 ? (ci == 0.0 ? 1.0 : 0.0) : ci / c_tot);
-            double fai = 0.0, mui = 0.0, gai = 0.0, rho = 0.0;
-            const int cnt = nn[ci_idx];
-            for (int k0 = 0; k0 < cnt; k0 += B) {
-                uint32_t e[B];
-                double4 xj[B];
-                double aj[B], cj[B], dEj[B];
-                int tt[B];
-#pragma unroll
-                for (int b = 0; b < B; ++b) e[b] = dlist_entry(list, ldl, maxn, k0 + b, ci_idx, cnt);
-#pragma unroll
-                for (int b = 0; b < B; ++b) {
-                    const int jc = (int)(e[b] & kIndexMask);
-                    xj[b] = x[jc / C];
-                    aj[b] = al[jc];
-                    cj[b] = cc[jc];
-                    dEj[b] = dE[jc];
-                    tt[b] = ei * A.ne + el[jc];
-                }
-                PairGeo pg[B];
-                bool live[B];
-#pragma unroll
-                for (int b = 0; b < B; ++b) {
-                    pg[b] = pair_geo(xi, xj[b], e[b] >> kIndexBits, bd.h, ai, aj[b], A.xmax, A.cut[tt[b]]);
-                    live[b] = pg[b].live && k0 + b < cnt;
-                }
-                double H[B][3][3];
-#pragma unroll
-                for (int b = 0; b < B; ++b)
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) H[b][k][0] = H[b][k][1] = H[b][k][2] = 0.0;
-                const DRow<F32>* T = drows<F32>(A);
-                for (int l = 0; l < A.N; ++l) {
-                    const double xl = A.xi[l], wl = A.wi[l];
-                    DRow<F32> c[B][3];
-                    double p[B], s[B];
-#pragma unroll
-                    for (int b = 0; b < B; ++b) {
-                        const double u = pg[b].r - xl * pg[b].a;
-                        const int m = dknot(A, u, p[b]);
-                        s[b] = u < 0.0 ? -wl : wl;
-                        const size_t row = live[b] ? ((size_t)tt[b] * A.nr + m) * 3 : 0;
-#pragma unroll
-                        for (int k = 0; k < 3; ++k) c[b][k] = T[row + k];
-                    }
-#pragma unroll
-                    for (int b = 0; b < B; ++b)
-#pragma unroll
-                        for (int k = 0; k < 3; ++k) {
-                            const double t = hv(c[b][k], p[b]) * s[b];
-                            H[b][k][0] += t;
-                            H[b][k][1] += t * xl;
-                            H[b][k][2] += t * (xl * xl);
-                        }
-                }
-#pragma unroll
-                for (int b = 0; b < B; ++b) {
-                    const double r = pg[b].r, a = pg[b].a;
-                    const double rinv = 1.0 / r, ainv = 1.0 / a;
-                    double rp[3], drr[3], dal[3];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        rp[k] = kPiInSq * rinv * H[b][k][0];
-                        drr[k] = -(rinv * rinv) * (rp[k] + 2.0 * kPiInSq * ainv * H[b][k][1]);
-                        dal[k] = -(ainv * ainv) * (rp[k] - 2.0 * kPiInSq * rinv * H[b][k][2]);
-                    }
-                    const double fpair = -(drr[2] * dEi + drr[1] * dEj[b] + drr[0]) * ci * cj[b];
-                    const double apair = -(dal[2] * dEi + dal[1] * dEj[b] + dal[0]) * ci * cj[b];
-                    const double du = cj[b] * (rp[0] + rp[1] * dEj[b]);
-                    const double fp = live[b] ? fpair : 0.0;
-                    mui += live[b] ? du : 0.0;
-                    fai += live[b] ? ai * apair : 0.0;
-                    const double dx = pg[b].dx, dy = pg[b].dy, dz = pg[b].dz;
-                    fx += fp * dx;
-                    fy += fp * dy;
-                    fz += fp * dz;
-                    if (UNIT && k0 + b < cnt) {  // the Jacobian's slot values (as k_dmd_jrp)
-                        const size_t slot = (size_t)(k0 + b) * ldl + ci_idx;
-                        J.idx[slot] = e[b] & kIndexMask;
-                        J.r2[slot] = live[b] ? rp[2] : 0.0;
-                        J.r1[slot] = live[b] ? rp[1] : 0.0;
-                        J.t0[slot] = live[b] ? rp[0] + dEi * rp[2] + dEj[b] * rp[1] : 0.0;
-                        rho += live[b] ? cj[b] * rp[2] : 0.0;
-                    }
-                    if (UNIT) {
-                        const double gp = live[b] ? (drr[2] * dEi + drr[1] * dEj[b] + drr[0]) * cj[b] * fcoef : 0.0;
-                        gx += gp * dx;
-                        gy += gp * dy;
-                        gz += gp * dz;
-                        gai += live[b] ? ai * (dal[2] * dEi + dal[1] * dEj[b] + dal[0]) * cj[b] : 0.0;
-                    }
-                    const double hf = 0.5 * fp;
-                    acc[0] += hf * dx * dx;
-                    acc[1] += hf * dx * dy;
-                    acc[2] += hf * dx * dz;
-                    acc[3] += hf * dy * dy;
-                    acc[4] += hf * dy * dz;
-                    acc[5] += hf * dz * dz;
-                }
-            }
-            fa[ci_idx] = fai + 3.0 * A.kBT * ci / ai;
-            mu[ci_idx] = E[ci_idx] + mui;
-            if (UNIT) {
-                gau[ci_idx] = gai - 3.0 * A.kBT / ai;
-                J.cnt[ci_idx] = cnt;
-                J.ddE[ci_idx] = ddemb5(A, ei, rho);
-            }
-        }
-        f[i] = make_double4(fx, fy, fz, 0.0);
-        if (UNIT) gu[i] = make_double4(gx, gy, gz, 0.0);
-    }
+    if (i < n)
+        pass2_site<B, UNIT>(i, x, al, cc, el, nn, list, ldl, maxn, A, bd, RowsGlobal<F32>{drows<F32>(A), A.nr},
+                            NbPlain{al, cc, el, dE}, E, dE, f, fa, mu, gu, gau, J, acc);
     block_sum_store<6>(acc, part + (size_t)blk * kPartW);
+}
+
+// Persistent passes with the FP32 knots in LDS: one workgroup of NT threads
+// per CU loads the knot window once and walks chunks of NT sites. The chunks
+// form 8 contiguous ranges, one per XCD, and the workgroups of an XCD stride
+// through its range side by side (neighbor halos shared in that XCD's L2).
+template <int NT>
+struct SiteWalk {
+    int c, end, step;
+    __device__ explicit SiteWalk(int n)
+    {
+        const int nchunks = (n + NT - 1) / NT;
+        const int per = (nchunks + kXcd - 1) / kXcd;
+        const int xcd = (int)blockIdx.x % kXcd;
+        c = xcd * per + (int)blockIdx.x / kXcd;
+        end = min((xcd + 1) * per, nchunks);
+        step = (int)gridDim.x / kXcd;
+    }
+};
+
+template <int NT>
+__device__ inline KnotsLds load_knots(const KnotArgs& K, float2* win, int* fn)
+{
+    for (int r = threadIdx.x; r < K.nf * K.nwin; r += NT) {
+        const int q = r / K.nwin;
+        win[r] = K.glob[(size_t)q * K.nr + K.m_lo + (r - q * K.nwin)];
+    }
+    for (int t = threadIdx.x; t < K.nfn; t += NT) fn[t] = K.fn[t];
+    __syncthreads();
+    return KnotsLds{K.glob, win, fn, K.nr, K.nwin, K.m_lo};
+}
+
+// (alpha, c) of every component in one record for the LDS passes
+__global__ __launch_bounds__(kBlock) void k_pack_ac(int nc, const double* __restrict__ al,
+                                                    const double* __restrict__ cc, double2* __restrict__ ac)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k < nc) ac[k] = make_double2(al[k], cc[k]);
+}
+
+template <int B, int NT>
+__global__ __launch_bounds__(NT) void k_dmd_pass1k(int n, const double4* __restrict__ x, const double* __restrict__ al,
+                                                   const double* __restrict__ cc, const int* __restrict__ el,
+                                                   const int* __restrict__ nn, const uint32_t* __restrict__ list,
+                                                   int ldl, int maxn, DmdArgs A, BoxDev bd, KnotArgs K,
+                                                   const double2* __restrict__ ac, double2* __restrict__ ed,
+                                                   int energy_only, double* __restrict__ E, double* __restrict__ dE,
+                                                   double* __restrict__ part)
+{
+    extern __shared__ float2 knot_win[];
+    __shared__ int knot_fn[kMaxElems * kMaxElems * 3];
+    const KnotsLds T = load_knots<NT>(K, knot_win, knot_fn);
+    const NbPacked nb{ac, el, ed};
+    double acc[2] = {0.0, 0.0};
+    for (SiteWalk<NT> w(n); w.c < w.end; w.c += w.step) {
+        const int i = w.c * NT + (int)threadIdx.x;
+        if (i < n) pass1_site<B>(i, x, al, cc, el, nn, list, ldl, maxn, A, bd, energy_only, T, nb, E, dE, acc);
+    }
+    block_sum_store<2, NT>(acc, part + (size_t)blockIdx.x * kPartW);
+}
+
+template <int B, int NT>
+__global__ __launch_bounds__(NT) void k_dmd_pass2k(int n, const double4* __restrict__ x, const double* __restrict__ al,
+                                                   const double* __restrict__ cc, const int* __restrict__ el,
+                                                   const int* __restrict__ nn, const uint32_t* __restrict__ list,
+                                                   int ldl, int maxn, DmdArgs A, BoxDev bd, KnotArgs K,
+                                                   const double2* __restrict__ ac, double2* __restrict__ ed,
+                                                   const double* __restrict__ E, const double* __restrict__ dE,
+                                                   double4* __restrict__ f, double* __restrict__ fa,
+                                                   double* __restrict__ mu, double* __restrict__ part)
+{
+    extern __shared__ float2 knot_win[];
+    __shared__ int knot_fn[kMaxElems * kMaxElems * 3];
+    const KnotsLds T = load_knots<NT>(K, knot_win, knot_fn);
+    const NbPacked nb{ac, el, ed};
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (SiteWalk<NT> w(n); w.c < w.end; w.c += w.step) {
+        const int i = w.c * NT + (int)threadIdx.x;
+        if (i < n)
+            pass2_site<B, false>(i, x, al, cc, el, nn, list, ldl, maxn, A, bd, T, nb, E, dE, f, fa, mu, nullptr,
+                                 nullptr, JState{}, acc);
+    }
+    block_sum_store<6, NT>(acc, part + (size_t)blockIdx.x * kPartW);
 }
 
 // ---------------- concentration rates ----------------
@@ -1954,6 +2219,7 @@ DMDSystem::DMDSystem(std::shared_ptr<EamForceField> ff, int nsites, int c_dim, i
         A_.tab = tab_.get();
         A_.F5 = F5_.get();
     }
+    plan_knots();
     double max_alpha = 0.0;
     for (int k = 0; k < nsites * c_dim; ++k)
         if (c[k] != -1.0) max_alpha = std::max(max_alpha, alpha[k]);
@@ -2193,12 +2459,43 @@ void DMDSystem::stage_surrogate()
     prof_.end(stream_);
 }
 
-void DMDSystem::pass1(bool energy_only, bool f32)
+// Workgroup sizes of the LDS passes: all waves of the one workgroup per CU
+// fit the register file without spills (pass 1 of two entries fits 128
+// VGPRs, of four 256). Pass 2 takes two entries in 512 threads: 768 would
+// fit its registers too, but a persistent walk is as long as its busiest
+// workgroup, and 262,144 sites over 256 x 768 threads leave a third of the
+// workgroups a second chunk (measured 1.49 against 1.13 ms); one entry in
+// 1024 threads measured 1.42 ms, four entries spill.
+constexpr int knot_nt1(int B) { return B == 4 ? 512 : 1024; }
+constexpr int kKnotNT2 = 512;
+
+void DMDSystem::pass1(bool energy_only, int f32)
 {
-    const int nb = grid_for(n_);
     BoxDev bd;
     std::memcpy(bd.h, box_.h, sizeof bd.h);
     std::memcpy(bd.b, box_.b, sizeof bd.b);
+    if (f32 == 1 && knots_ok_ && !surr_ && batch_ != 1) {
+        const int nb = std::min(knot_grid_, grid_for(n_));
+        ac_.reserve(nc_);
+        ed_.reserve(nc_);
+        prof_.begin(energy_only ? K_DMD_ENERGY : K_DMD_PASS1, stream_);
+        hipLaunchKernelGGL(k_pack_ac, dim3((nc_ + kBlock - 1) / kBlock), dim3(kBlock), 0, stream_, nc_, al_.get(),
+                           c_.get(), ac_.get());
+        if (batch_ == 4)
+            hipLaunchKernelGGL((k_dmd_pass1k<4, knot_nt1(4)>), dim3(nb), dim3(knot_nt1(4)), knot_lds_, stream_, n_,
+                               x_.get(), al_.get(), c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd,
+                               kn_, ac_.get(), ed_.get(), energy_only ? 1 : 0, E_.get(), dE_.get(), part_.get());
+        else
+            hipLaunchKernelGGL((k_dmd_pass1k<2, knot_nt1(2)>), dim3(nb), dim3(knot_nt1(2)), knot_lds_, stream_, n_,
+                               x_.get(), al_.get(), c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd,
+                               kn_, ac_.get(), ed_.get(), energy_only ? 1 : 0, E_.get(), dE_.get(), part_.get());
+        prof_.end(stream_);
+        reduce_rows(part_.get(), nb, 2, red_.get());
+        ed_ver_ = ver_;
+        return;
+    }
+    ed_ver_ = -1;
+    const int nb = grid_for(n_);
     SurrogateArgs S{};
     if (surr_) {
         stage_surrogate();
@@ -2228,15 +2525,27 @@ void DMDSystem::pass1(bool energy_only, bool f32)
     reduce_rows(part_.get(), nb, 2, red_.get());
 }
 
-void DMDSystem::pass2(bool unit, bool f32)
+void DMDSystem::pass2(bool unit, int f32)
 {
     const int nb = grid_for(n_);
     BoxDev bd;
     std::memcpy(bd.h, box_.h, sizeof bd.h);
     std::memcpy(bd.b, box_.b, sizeof bd.b);
+    double* p2 = part_.get() + (size_t)nb * kPartW;
+    if (f32 == 1 && knots_ok_ && !surr_ && !unit && batch_ != 1) {
+        // the packed records of pass 1 of this state (pass 2 follows pass 1)
+        MAPP_REQUIRE(ed_ver_ == ver_, ERR_STATE, "the FP32-knot pass 2 needs pass 1 of the same state");
+        const int nbk = std::min(knot_grid_, nb);
+        prof_.begin(K_DMD_PASS2, stream_);
+        hipLaunchKernelGGL((k_dmd_pass2k<2, kKnotNT2>), dim3(nbk), dim3(kKnotNT2), knot_lds_, stream_, n_, x_.get(),
+                           al_.get(), c_.get(), el_.get(), nn_.get(), list_.get(), ldl_, maxn_, A_, bd, kn_, ac_.get(),
+                           ed_.get(), E_.get(), dE_.get(), f_.get(), fa_.get(), mu_.get(), p2);
+        prof_.end(stream_);
+        reduce_rows(p2, nbk, 6, red_.get() + 8);
+        return;
+    }
     SurrogateArgs S{};
     if (surr_) S = SurrogateArgs{s_y_.get(), s_off_.get(), n_, surr_->box()};
-    double* p2 = part_.get() + (size_t)nb * kPartW;
     prof_.begin(K_DMD_PASS2, stream_);
     if (surr_)
         hipLaunchKernelGGL(k_dmd_pass2<true>, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), al_.get(), c_.get(),
@@ -2271,7 +2580,83 @@ void DMDSystem::pass2(bool unit, bool f32)
     reduce_rows(p2, nb, 6, red_.get() + 8);
 }
 
-void DMDSystem::set_f32(bool on) { f32_ = on; }
+void DMDSystem::set_f32(int mode)
+{
+    MAPP_REQUIRE(mode >= 0 && mode <= 2, ERR_ARG, "table mode must be 0 (FP64 rows), 1 (FP32 knots) or 2 (FP32 rows)");
+    f32_ = mode;
+}
+
+// FP32 knots for the LDS passes. Each table function's interval cubic must
+// be the Hermite cubic of its two knots' values and slopes (what the spline
+// builder makes); a table that is not stays on the FP32 rows. Equal functions
+// (the density of one element seen from every pair of an eam/alloy table)
+// are kept once, and the window is the top knots that fit the workgroup LDS
+// beside the passes' static arrays.
+void DMDSystem::plan_knots()
+{
+    knots_ok_ = false;
+    const EamTables& t = ff_->tables();
+    const int ne = t.ne, nr = t.nr;
+    if (nr < 3) return;
+    std::vector<std::vector<double>> uniq;
+    std::vector<int> fn((size_t)ne * ne * 3);
+    for (int a = 0; a < ne; ++a)
+        for (int b = 0; b < ne; ++b)
+            for (int k = 0; k < 3; ++k) {
+                const double* c = k == 0 ? t.rphi_at(a, b) : (k == 1 ? t.rho_at(a, b) : t.rho_at(b, a));
+                std::vector<double> ys((size_t)2 * nr);
+                for (int m = 0; m < nr; ++m) {
+                    ys[2 * (size_t)m] = c[4 * (size_t)m];
+                    ys[2 * (size_t)m + 1] = c[4 * (size_t)m + 1];
+                }
+                for (int m = 0; m + 1 < nr; ++m) {
+                    const double y0 = ys[2 * m], s0 = ys[2 * m + 1], y1 = ys[2 * m + 2], s1 = ys[2 * m + 3];
+                    const double dy = y1 - y0;
+                    const double c2 = 3.0 * dy - 2.0 * s0 - s1, c3 = s0 + s1 - 2.0 * dy;
+                    const double scale =
+                        std::max({std::fabs(y0), std::fabs(y1), std::fabs(s0), std::fabs(s1), 1e-300});
+                    if (std::fabs(c[4 * (size_t)m + 2] - c2) > 1e-12 * scale ||
+                        std::fabs(c[4 * (size_t)m + 3] - c3) > 1e-12 * scale)
+                        return;
+                }
+                int id = -1;
+                for (size_t q = 0; q < uniq.size() && id < 0; ++q)
+                    if (uniq[q] == ys) id = (int)q;
+                if (id < 0) {
+                    id = (int)uniq.size();
+                    uniq.push_back(std::move(ys));
+                }
+                fn[((size_t)a * ne + b) * 3 + k] = id;
+            }
+    const int nf = (int)uniq.size();
+    int dev = 0, lds_max = 0, ncu = 0;
+    MAPP_HIP(hipGetDevice(&dev));
+    MAPP_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    MAPP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    // room for the passes' static LDS: the slot map and the block reduction
+    const long budget = (long)lds_max - 4096;
+    const int nwin = (int)std::min<long>(nr, budget / (8L * nf));
+    if (nwin < 2 || ncu < 1) return;
+    std::vector<float2> g((size_t)nf * nr);
+    for (int q = 0; q < nf; ++q)
+        for (int m = 0; m < nr; ++m)
+            g[(size_t)q * nr + m] = make_float2((float)uniq[q][2 * (size_t)m], (float)uniq[q][2 * (size_t)m + 1]);
+    knots_.reserve(g.size());
+    MAPP_HIP(hipMemcpy(knots_.get(), g.data(), g.size() * sizeof(float2), hipMemcpyHostToDevice));
+    knot_fn_.reserve(fn.size());
+    MAPP_HIP(hipMemcpy(knot_fn_.get(), fn.data(), fn.size() * sizeof(int), hipMemcpyHostToDevice));
+    kn_ = KnotArgs{knots_.get(), knot_fn_.get(), nf, nr, nwin, nr - nwin, (int)fn.size()};
+    knot_lds_ = (size_t)nf * nwin * sizeof(float2);
+    knot_grid_ = std::max(1, ncu / kXcd) * kXcd;
+    const void* fns[] = {(const void*)k_dmd_pass1k<2, knot_nt1(2)>, (const void*)k_dmd_pass1k<4, knot_nt1(4)>,
+                         (const void*)k_dmd_pass2k<2, kKnotNT2>};
+    for (const void* f : fns)
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)knot_lds_) != hipSuccess) {
+            (void)hipGetLastError();
+            return;
+        }
+    knots_ok_ = true;
+}
 
 void DMDSystem::set_batch(int b)
 {

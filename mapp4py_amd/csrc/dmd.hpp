@@ -26,6 +26,16 @@ struct DmdArgs {
     const double* F5;    // [ne][nrho][5], quartic embedding
 };
 
+// FP32 knots of the LDS passes: the distinct table functions as (value,
+// slope per unit offset) pairs [nf][nr], the distinct function of each
+// (element pair, function) slot [nfn = ne*ne*3], and the knot window
+// m_lo .. nr-1 (nwin knots per function) the workgroups hold in LDS
+struct KnotArgs {
+    const float2* glob;
+    const int* fn;
+    int nf, nr, nwin, m_lo, nfn;
+};
+
 
 // physicists' Gauss-Hermite rule (weight e^{-t^2}), n <= 16
 void gauss_hermite(int n, double* x, double* w);
@@ -73,11 +83,16 @@ public:
     void set_box(const double* H, const double* x);
     // free energy, F_H and entropy of the state of the last c_d (no new passes)
     DMDReport force_calc_static();
-    // free energy and gradient (derivative, value, the minimisers) from FP32
-    // table rows with FP64 moments and sums; the concentration-rate path
-    // (c_d, J, the implicit and BDF steps) always reads the FP64 rows
-    void set_f32(bool on);
-    bool f32() const { return f32_; }
+    // table evaluation of the free energy and gradient (derivative, value,
+    // the minimisers): 0 FP64 rows; 1 FP32 knots with the window in LDS (the
+    // FP32 rows where the table is not a Hermite spline of its knots); 2 FP32
+    // rows. FP32 modes keep FP64 moments and sums. The concentration-rate
+    // path (c_d, J, the implicit and BDF steps) always reads the FP64 rows.
+    void set_f32(int mode);
+    int f32() const { return f32_; }
+    // whether mode 1 holds the knots in LDS, and the window's first knot
+    bool knots_in_lds() const { return knots_ok_; }
+    int knot_window_first() const { return kn_.m_lo; }
     ~DMDSystem();
 
     int nsites() const { return n_; }
@@ -152,8 +167,9 @@ public:
 private:
     void setup_grid();
     void build_list();
-    void pass1(bool energy_only, bool f32 = false);
-    void pass2(bool unit = false, bool f32 = false);
+    void pass1(bool energy_only, int f32 = 0);
+    void pass2(bool unit = false, int f32 = 0);
+    void plan_knots();
     bool decide();
     void reduce_rows(const double* part, int nrows, int ncols, double* out);
     void J_prepare();
@@ -219,7 +235,18 @@ private:
     DevBuf<uint32_t> list_;
     int maxn_ = 0, ldl_ = 0;
     int batch_ = 4;                  // quadrature pass batch: 1 (per-pair loop), 2 or 4 list entries
-    bool f32_ = false;               // FP32 table rows in derivative / value
+    int f32_ = 0;                    // table mode of derivative / value (set_f32)
+    // FP32 knots and their LDS window (plan_knots)
+    DevBuf<float2> knots_;
+    DevBuf<int> knot_fn_;
+    KnotArgs kn_{};
+    bool knots_ok_ = false;
+    size_t knot_lds_ = 0;
+    int knot_grid_ = 0;
+    // their passes' packed neighbor records: (alpha, c), and (F', element)
+    // from pass 1 of state ed_ver_
+    DevBuf<double2> ac_, ed_;
+    int64_t ed_ver_ = -1;
     DevBuf<int> flags_;
     DevBuf<double> part_, red_, red2_;
     DevBuf<double4> tab_;

@@ -90,24 +90,36 @@ def test_box_change_and_static_force_calc_match_oracle(M, cuh):
     close(f2, fo, "f_x after a rejected box")
 
 
-def test_fp32_rows_stay_within_the_stated_bars(M, cuh):
-    # FP32 copies of the table rows, FP64 moments and sums: the free energy and
-    # gradient against the FP64 oracle. Bars: forces on x and alpha 1e-4 eV/A
-    # (the DMD force bar), mu 1e-4 eV, free energy 1e-8 relative, virial 1e-6
-    # of its largest component. The rate path does not read the FP32 rows.
+@pytest.mark.parametrize("nr,rows", [(5000, True), (5000, False), (20000, False)])
+def test_fp32_tables_stay_within_the_stated_bars(M, nr, rows):
+    # FP32 pair splines, FP64 moments and sums: the free energy and gradient
+    # against the FP64 oracle, from FP32 copies of the coefficient rows
+    # (rows) or from FP32 knot values and slopes with the top knots in LDS.
+    # At nr = 20000 the LDS window starts near 4.4 A, so most Hermite points
+    # read the knots below it from device memory. Bars: forces on x and
+    # alpha 1e-4 eV/A (the DMD force bar), mu 1e-4 eV, free energy 1e-8
+    # relative, virial 1e-6 of its largest component. The rate path does not
+    # read the FP32 tables.
+    p = pot.johnson_cuh(nr=nr, nrho=5000, rc=5.5)
     x, H, elem, alpha, c = sites((4, 4, 4), 5, (0.3, -0.2, 0.4))
-    g, o = both(M, cuh, x, H, elem, alpha, c)
+    g, o = both(M, p, x, H, elem, alpha, c)
     fo, fao, muo, feo, FHo, So = o.force()
     r_crd, zeta = [3.5, 3.5], [1.7, 1.7]
     cd64 = g.c_d(r_crd, zeta)[0]
-    g.set_fp32(True)
+    g.set_fp32(True, rows=rows)
+    first = g.knot_window()
+    say(f"DMD FP32 {'rows' if rows else 'knots'}, nr {nr}: LDS window from knot {first} "
+        f"(r = {first * p.dr:.2f} A)")
+    assert first >= 0
+    if nr == 20000:
+        assert first * p.dr > 3.0
     for batch in (2, 4):
         g.set_batch(batch)
         f, fa, mu, fe, FH, S = g.derivative()
         df, dfa, dmu = np.abs(f - fo).max(), np.abs(fa - fao).max(), np.abs(mu - muo).max()
         dfe = abs(fe - feo) / abs(feo)
         dFH = np.abs(FH - FHo).max() / np.abs(FHo).max()
-        say(f"DMD FP32 rows batch {batch}: max|df| {df:.1e} eV/A, max|df_a| {dfa:.1e}, max|dmu| {dmu:.1e} eV, "
+        say(f"  batch {batch}: max|df| {df:.1e} eV/A, max|df_a| {dfa:.1e}, max|dmu| {dmu:.1e} eV, "
             f"fe rel {dfe:.1e}, F_H rel {dFH:.1e}, S rel {abs(S - So) / abs(So):.1e}")
         assert df < 1e-4 and dfa < 1e-4 and dmu < 1e-4
         assert dfe < 1e-8 and dFH < 1e-6
@@ -116,6 +128,8 @@ def test_fp32_rows_stay_within_the_stated_bars(M, cuh):
     cd32 = g.c_d(r_crd, zeta)[0]
     np.testing.assert_array_equal(cd32, cd64)
     g.set_fp32(False)
+    f64 = g.derivative()[0]
+    close(f64, fo, "f_x back on the FP64 rows")
 
 
 @pytest.mark.parametrize("cells,tilt,seed", [((3, 3, 3), (0, 0, 0), 1), ((3, 4, 3), (0.6, -0.4, 0.5), 2),
