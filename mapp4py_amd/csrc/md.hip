@@ -1333,6 +1333,24 @@ void MDSystem::setup_grid()
         r16_.knot_u = r16_.q * P.dr_inv;
         r16_.inv_qf = (float)r16_.inv_q;
     }
+    // the cell ranks and stencil order depend on the cell counts alone: a box
+    // change that keeps them (the NST rebuilds) reuses them
+    const int topo[6] = {grid_.nc[0], grid_.nc[1], grid_.nc[2], grid_.R[0], grid_.R[1], grid_.R[2]};
+    if (!std::equal(topo, topo + 6, topo_)) {
+        std::copy(topo, topo + 6, topo_);
+        cell_topology();
+    }
+    // neighbors expected in the search sphere at this density, with margin;
+    // the capacity is a whole number of 4-entry blocks
+    const double dens = n_ / box_.vol;
+    const double sphere = 4.0 / 3.0 * M_PI * max_cut_ * max_cut_ * max_cut_;
+    maxn_ = std::max(maxn_, (int)(1.25 * dens * sphere) + 16);
+    maxn_ = (maxn_ + 3) & ~3;
+}
+
+void MDSystem::cell_topology()
+{
+    const int ncells = grid_.ncells;
     // Morton rank of every cell so that cell-sorted atoms are spatially compact
     std::vector<std::pair<uint32_t, int>> code(ncells);
     for (int c2 = 0, c = 0; c2 < grid_.nc[2]; ++c2)
@@ -1355,12 +1373,6 @@ void MDSystem::setup_grid()
                            cell_rank_.get(), cell_order_.get());
         MAPP_HIP(hipGetLastError());
     }
-    // neighbors expected in the search sphere at this density, with margin;
-    // the capacity is a whole number of 4-entry blocks
-    const double dens = n_ / box_.vol;
-    const double sphere = 4.0 / 3.0 * M_PI * max_cut_ * max_cut_ * max_cut_;
-    maxn_ = std::max(maxn_, (int)(1.25 * dens * sphere) + 16);
-    maxn_ = (maxn_ + 3) & ~3;
 }
 
 void MDSystem::rebuild()

@@ -194,6 +194,7 @@ public:
 
 private:
     void setup_grid();
+    void cell_topology();  // cell ranks and stencil order of grid_'s cell counts
     void refresh_frame();  // image shifts of the records for the current box
     void build_list();
     // records_fresh: rec_ already holds the current positions (written by the skin check or a rebuild)
@@ -237,6 +238,7 @@ private:
     DevBuf<int> cell_rank_, cell_start_;
     DevBuf<uint4> cell_order_;       // per cell: the 27 stencil cells in Morton-rank order (32 bytes)
     bool ordered_ = false;           // a 3x3x3 stencil of distinct cells: cell_order_ holds it
+    int topo_[6] = {0, 0, 0, 0, 0, 0};  // cells and stencil reach per direction of cell_rank_ / cell_order_
     DevBuf<int> nn_;
     DevBuf<uint32_t> list_;          // 16-byte blocks: entries k..k+3 of slot i at block (k / 4) * ldl_ + i
     uint4* list4() const { return reinterpret_cast<uint4*>(list_.p); }

@@ -82,11 +82,13 @@ __global__ __launch_bounds__(kBlock) void k_nst_drift(int n, double4* __restrict
 // the centre-of-mass correction x <- x - c and DynamicMD::decide's skin
 // check against the positions of the last build
 __global__ __launch_bounds__(kBlock) void k_nst_shift(int n, double4* __restrict__ x, const double4* __restrict__ x0,
-                                                      double c0, double c1, double c2, double skin_sq4,
+                                                      const double* __restrict__ com, double skin_sq4,
                                                       int* __restrict__ flags)
 {
     const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
+    // the centre-of-mass displacement: the drift's sums over n, read on the device
+    const double c0 = com[0] / (double)n, c1 = com[1] / (double)n, c2 = com[2] / (double)n;
     double4 xq = x[i];
     xq.x -= c0;
     xq.y -= c1;
@@ -94,6 +96,18 @@ __global__ __launch_bounds__(kBlock) void k_nst_shift(int n, double4* __restrict
     x[i] = xq;
     const double4 a = x0[i];
     if (rsq_exact(a.x, a.y, a.z, xq.x, xq.y, xq.z) > skin_sq4) atomicOr(&flags[1], 1);
+}
+
+// the 128-entry image-shift table of the record grid (refresh_frame)
+__global__ __launch_bounds__(128) void k_shift_table(Rec16Frame R, uint4* __restrict__ out)
+{
+    const int c = threadIdx.x;
+    uint4 sh = make_uint4(0u, 0u, 0u, 0u);
+    if (c < 125) {
+        const unsigned n0 = (unsigned)(c % 5 - 2), n1 = (unsigned)(c / 5 % 5 - 2), n2 = (unsigned)(c / 25 - 2);
+        sh = make_uint4(n0 * R.cH[0] + n1 * R.cH[3] + n2 * R.cH[6], n1 * R.cH[4] + n2 * R.cH[7], n2 * R.cH[8], 0u);
+    }
+    out[c] = sh;
 }
 
 // second half (src/md_nst.cpp:133-152): v <- v E + f F0 / m when kick, and
@@ -256,21 +270,17 @@ inline int lt_voigt(int i, int j) { return i + j * 3 - j * (j + 1) / 2; }
 }  // namespace
 
 // Image shifts of the 16-byte records for the current box (the record grid q
-// depends on the cut-off only)
+// depends on the cut-off only), made on the device in stream order so that a
+// box change costs no host synchronisation: code c = (n0+2) + 5(n1+2) +
+// 25(n2+2) shifts by n0 h0 + n1 h1 + n2 h2 on the grid (modulo 2^32); codes
+// 125..127 are unused
 void MDSystem::refresh_frame()
 {
     for (int k = 0; k < 3; ++k)
         for (int d = 0; d < 3; ++d)
             r16_.cH[3 * k + d] = (unsigned)(unsigned long long)std::llrint(box_.H[k][d] * r16_.inv_q);
-    std::vector<uint4> sh(128, make_uint4(0u, 0u, 0u, 0u));  // 7-bit codes; 125..127 unused
-    for (int c = 0; c < 125; ++c) {
-        const unsigned n0 = (unsigned)(c % 5 - 2), n1 = (unsigned)(c / 5 % 5 - 2), n2 = (unsigned)(c / 25 - 2);
-        sh[c] = make_uint4(n0 * r16_.cH[0] + n1 * r16_.cH[3] + n2 * r16_.cH[6], n1 * r16_.cH[4] + n2 * r16_.cH[7],
-                           n2 * r16_.cH[8], 0u);
-    }
-    shift_.reserve(sh.size());
-    MAPP_HIP(hipStreamSynchronize(stream_));
-    MAPP_HIP(hipMemcpy(shift_.get(), sh.data(), sizeof(uint4) * sh.size(), hipMemcpyHostToDevice));
+    shift_.reserve(128);
+    hipLaunchKernelGGL(k_shift_table, dim3(1), dim3(128), 0, stream_, r16_, shift_.get());
     r16_.shift = shift_.get();
 }
 
@@ -387,12 +397,11 @@ int MDSystem::nst_run(const double* masses, double kB, double T, double dt, cons
         hipLaunchKernelGGL(k_nst_drift, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), v_.get(), f_.get(), K,
                            fac_x_d, rows);
         prof_.end(stream_);
-        double du[3];
-        reduce(3, du);
+        reduce_rows(rows, nb, 3, red);
         MAPP_HIP(hipMemsetAsync(flags_.get() + 1, 0, sizeof(int), stream_));
         prof_.begin(K_INTEGRATE, stream_);
-        hipLaunchKernelGGL(k_nst_shift, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), x0_.get(), du[0] / n_,
-                           du[1] / n_, du[2] / n_, skin_sq4, flags_.get());
+        hipLaunchKernelGGL(k_nst_shift, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), x0_.get(), red, skin_sq4,
+                           flags_.get());
         prof_.end(stream_);
         // H <- H (I + F1 V_H)
         double M[3][3] = {}, Hn[9] = {};
@@ -414,13 +423,20 @@ int MDSystem::nst_run(const double* masses, double kB, double T, double dt, cons
         } else {
             refresh_frame();
         }
-        r = force_calc();
-        stress();
+        // the force call's energy and virial and the kick's kinetic tensor
+        // come back in one synchronisation (the kick uses the box velocity of
+        // the first half-step, not the new stress)
+        force_passes(nullptr);
         prof_.begin(K_INTEGRATE, stream_);
         hipLaunchKernelGGL(k_nst_kick, dim3(nb), dim3(kBlock), 0, stream_, n_, x_.get(), v_.get(), f_.get(), K, 1,
                            rows);
         prof_.end(stream_);
-        reduce(6, mvv);
+        reduce_rows(rows, nb, 6, red);
+        MAPP_HIP(hipMemcpyAsync(r.e, red_.get(), sizeof(double) * 7, hipMemcpyDeviceToHost, stream_));
+        MAPP_HIP(hipMemcpyAsync(mvv, red, sizeof(double) * 6, hipMemcpyDeviceToHost, stream_));
+        MAPP_HIP(hipStreamSynchronize(stream_));
+        prof_.flush();
+        stress();
         T_part = (mvv[0] + (mvv[3] + mvv[5])) / (ndof * kB);
         update_VH();
         fac_x_d = fac = tp(T_part / T, (int)ndof);
