@@ -488,14 +488,48 @@ __global__ __launch_bounds__(256) void k_gcmc_init(GcmcArgs A, const GcmcCell* _
     for (int i = threadIdx.x; i < hd.n; i += blockDim.x) init_atom(A, hd, R, i);
 }
 
-// the same for one cell c over many workgroups, one thread per atom (a large
-// cell: the grand-canonical exchange of an MD system)
+// init_atom with the 64 lanes of a wave splitting the other atoms (lane l
+// takes j = l, l + 64, ... for every image; the lanes' sums meet in a fixed
+// butterfly, so the result is deterministic)
+__device__ inline void init_atom_wave(const GcmcArgs& A, const GcmcCell& hd, const Arrays& R, int i, int lane)
+{
+    const int off = hd.off, n = hd.n, ne = A.ne, nr = A.nr;
+    const double4 si4 = R.s[off + i];
+    const double si[3] = {si4.x, si4.y, si4.z};
+    const int ei = (int)R.xe[off + i].w;
+    const Images im = count_images(si, hd.lo2, hd.hi2);
+    double rho = 0.0;
+    for (int k = 0; k < im.total(); ++k) {
+        double xi[3];
+        image_x(hd.h, si, im, k, xi);
+        for (int j = lane; j < n; j += 64) {
+            if (k == 0 && j == i) continue;
+            const double4 xj = R.xe[off + j];
+            const int ej = (int)xj.w;
+            const double rsq = rsq_of(xi[0], xi[1], xi[2], xj.x, xj.y, xj.z);
+            if (rsq >= A.cut_sq[ei * ne + ej]) continue;
+            int m;
+            double p;
+            spline_at(sqrt(rsq), A.dr_inv, nr, m, p);
+            rho += hval(A.rho[(size_t)(ej * ne + ei) * nr + m], p);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) rho += __shfl_xor(rho, o, 64);
+    if (lane == 0) {
+        R.rho[off + i] = rho;
+        R.F[off + i] = f_embed(A, ei, rho, false);
+    }
+}
+
+// the same for one cell c over many workgroups, one wave per atom (a large
+// cell: the grand-canonical exchange of an MD system, up to 16,384 atoms,
+// where one thread per atom would leave most CUs idle)
 __global__ __launch_bounds__(256) void k_gcmc_init_cell(GcmcArgs A, const GcmcCell* __restrict__ cells, int c,
                                                         Arrays R)
 {
     const GcmcCell hd = cells[c];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < hd.n) init_atom(A, hd, R, i);
+    const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);  // uniform over the wave
+    if (i < hd.n) init_atom_wave(A, hd, R, i, (int)(threadIdx.x & 63));
 }
 
 // ---- single-trial hooks (one cell, one trial, driven by the caller) ----
@@ -828,8 +862,8 @@ GcmcBatch::GcmcBatch(std::shared_ptr<EamForceField> ff, int ncells, const int* n
     queue_.reserve(2);
     if (ncells == 1) {
         if (host_cells_[0].n)
-            k_gcmc_init_cell<<<(host_cells_[0].n + 255) / 256, 256, 0, stream_>>>(A_, st_.cells.get(), 0,
-                                                                                 arrays_of(st_));
+            k_gcmc_init_cell<<<(host_cells_[0].n + 3) / 4, 256, 0, stream_>>>(A_, st_.cells.get(), 0,
+                                                                             arrays_of(st_));
     } else {
         k_gcmc_init<<<ncells, 256, 0, stream_>>>(A_, st_.cells.get(), arrays_of(st_));
     }
@@ -1135,7 +1169,7 @@ void GcmcBatch::load_cell(int c, int n, const double* x, const uint8_t* elem, co
     host_cells_[c].n = n;
     host_cells_[c].ngas = ngas;
     if (n) {
-        k_gcmc_init_cell<<<(n + 255) / 256, 256, 0, stream_>>>(A_, st_.cells.get(), c, arrays_of(st_));
+        k_gcmc_init_cell<<<(n + 3) / 4, 256, 0, stream_>>>(A_, st_.cells.get(), c, arrays_of(st_));
         MAPP_HIP(hipGetLastError());
     }
     MAPP_HIP(hipStreamSynchronize(stream_));

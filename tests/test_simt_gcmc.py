@@ -123,6 +123,22 @@ def test_emulated_kernel_reproduces_the_oracle_chains(emu, mu, waves, graph, ks)
         assert states[c]["seed"] == g.rng_seed and states[c]["ngas"] == g.ngas
 
 
+def test_emulated_one_cell_batch_reproduces_the_oracle_chain(emu):
+    # a batch of one cell takes its set-up densities from one wave per atom
+    # (the path of the MD exchange), the lanes splitting the other atoms
+    cells = cells_for_test()
+    x, H, e = max(cells, key=lambda c: len(c[2]))
+    seed, mu, k = 4242, -2.3, 30
+    dec, du, states = run(emu, [(x, H, e)], mu, np.array([seed], np.int32), [k], 4, False)
+    g = O.OGCMC(O.OEam.read(pot.NIALH_JEA), H, x, e, GAS, mu, 300.0, MASSES, seed=seed)
+    np.testing.assert_array_equal(dec[0], g.attempts(k)[0])
+    st = g.state()
+    np.testing.assert_array_equal(states[0]["elem"], st["elem"])
+    np.testing.assert_array_equal(states[0]["x"], st["x"])
+    np.testing.assert_allclose(states[0]["rho"], st["rho"], rtol=1e-10, atol=1e-12)
+    assert states[0]["seed"] == g.rng_seed and states[0]["ngas"] == g.ngas
+
+
 def test_emulated_single_trial_hooks_follow_the_oracle_chain(emu):
     # the hook kernels (one trial's exchange energy, its commit) driven by a
     # caller-side SGCMC::attmpt with the reference's draw order: the oracle's
