@@ -236,21 +236,30 @@ __device__ void run_cell(const GcmcArgs& A, GcmcCell* __restrict__ cells, int c,
             }
             mode = MODE_DEL;
             const int igas = (int)(ngas * rng.uniform());
-            // igas-th gas atom in index order: ballot over 64 atoms at a time
+            // igas-th gas atom in index order: ballots over 64 atoms, the
+            // element loads of four of them in flight at a time
             int seen = 0;
-            for (int base = 0; base < n; base += 64) {
-                const int j = base + lane;
-                const int jj = j < n ? j : n - 1;
-                const bool g = (j < n) & (R.el[off + jj] == gas);
-                const unsigned long long m = __ballot(g);
-                const int cnt = __popcll(m);
-                if (seen + cnt > igas) {
-                    unsigned long long mm = m;
-                    for (int k = 0; k < igas - seen; ++k) mm &= mm - 1;
-                    del_idx = base + __ffsll((long long)mm) - 1;
-                    break;
+            for (int base = 0; base < n && del_idx < 0; base += 256) {
+                uint8_t e[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int j = base + 64 * q + lane;
+                    e[q] = R.el[off + (j < n ? j : n - 1)];
                 }
-                seen += cnt;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const unsigned long long m = __ballot((base + 64 * q + lane < n) & (e[q] == gas));
+                    const int cnt = __popcll(m);
+                    if (del_idx < 0) {
+                        if (seen + cnt > igas) {
+                            unsigned long long mm = m;
+                            for (int k = 0; k < igas - seen; ++k) mm &= mm - 1;
+                            del_idx = base + 64 * q + __ffsll((long long)mm) - 1;
+                        } else {
+                            seen += cnt;
+                        }
+                    }
+                }
             }
             if (del_idx < 0) {  // gas count and element array disagree: stop this chain
                 status = 2;
