@@ -205,6 +205,54 @@ struct Arrays {
 // that one lane would store are stored by all lanes alike. No lane ever
 // leaves a loop that its wave stays in, so every cross-lane operation runs
 // with the whole wave.
+// gas atoms among el[lo, hi): ballots over 64 atoms, the loads of four of
+// them in flight at a time (the count is the same in every lane)
+__device__ inline int count_gas(const uint8_t* __restrict__ el, int lo, int hi, int gas, int lane)
+{
+    int cnt = 0;
+    for (int base = lo; base < hi; base += 256) {
+        uint8_t e[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = base + 64 * q + lane;
+            e[q] = el[j < hi ? j : hi - 1];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cnt += __popcll(__ballot((base + 64 * q + lane < hi) & (e[q] == gas)));
+    }
+    return cnt;
+}
+
+// index of the k-th gas atom (from 0) among el[lo, hi), or -1: the same
+// ballots, stopping at the chunk that holds it
+__device__ inline int nth_gas(const uint8_t* __restrict__ el, int lo, int hi, int gas, int k, int lane)
+{
+    int seen = 0, idx = -1;
+    for (int base = lo; base < hi && idx < 0; base += 256) {
+        uint8_t e[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = base + 64 * q + lane;
+            e[q] = el[j < hi ? j : hi - 1];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const unsigned long long m = __ballot((base + 64 * q + lane < hi) & (e[q] == gas));
+            const int cnt = __popcll(m);
+            if (idx < 0) {
+                if (seen + cnt > k) {
+                    unsigned long long mm = m;
+                    for (int t = 0; t < k - seen; ++t) mm &= mm - 1;
+                    idx = base + 64 * q + __ffsll((long long)mm) - 1;
+                } else {
+                    seen += cnt;
+                }
+            }
+        }
+    }
+    return idx;
+}
+
 template <int W>
 __device__ void run_cell(const GcmcArgs& A, GcmcCell* __restrict__ cells, int c, int nattempts, int8_t* dec,
                          double* dU, const Arrays& R, double* trial, double* red, int w, int lane)
@@ -236,30 +284,28 @@ __device__ void run_cell(const GcmcArgs& A, GcmcCell* __restrict__ cells, int c,
             }
             mode = MODE_DEL;
             const int igas = (int)(ngas * rng.uniform());
-            // igas-th gas atom in index order: ballots over 64 atoms, the
-            // element loads of four of them in flight at a time
-            int seen = 0;
-            for (int base = 0; base < n && del_idx < 0; base += 256) {
-                uint8_t e[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int j = base + 64 * q + lane;
-                    e[q] = R.el[off + (j < n ? j : n - 1)];
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const unsigned long long m = __ballot((base + 64 * q + lane < n) & (e[q] == gas));
-                    const int cnt = __popcll(m);
-                    if (del_idx < 0) {
-                        if (seen + cnt > igas) {
-                            unsigned long long mm = m;
-                            for (int k = 0; k < igas - seen; ++k) mm &= mm - 1;
-                            del_idx = base + 64 * q + __ffsll((long long)mm) - 1;
-                        } else {
-                            seen += cnt;
-                        }
-                    }
-                }
+            // igas-th gas atom in index order
+            if constexpr (W < 16) {
+                // every wave finds it alike (cells of at most 4,000 atoms in
+                // a batch: three workgroup barriers cost more than the scan)
+                del_idx = nth_gas(R.el + off, 0, n, gas, igas, lane);
+            } else {
+                // one large cell: each wave counts the gas atoms of its slice
+                // of 64-atom chunks; the wave whose slice holds the igas-th
+                // finds it
+                int* ish = reinterpret_cast<int*>(red);
+                const int slice = ((n + W - 1) / W + 63) & ~63;
+                const int lo = min(n, w * slice), hi = min(n, lo + slice);
+                const int cnt = count_gas(R.el + off, lo, hi, gas, lane);
+                if (w == 0) ish[W] = -1;
+                ish[w] = cnt;  // every lane of the wave stores the same count
+                __syncthreads();
+                int before = 0;
+                for (int q = 0; q < w; ++q) before += ish[q];
+                __syncthreads();
+                if (before <= igas && igas < before + cnt) ish[W] = nth_gas(R.el + off, lo, hi, gas, igas - before, lane);
+                __syncthreads();
+                del_idx = __builtin_amdgcn_readfirstlane(ish[W]);
             }
             if (del_idx < 0) {  // gas count and element array disagree: stop this chain
                 status = 2;
@@ -454,6 +500,20 @@ __global__ __launch_bounds__(256) void k_gcmc(GcmcArgs A, GcmcCell* cells, const
         if (t >= n1) break;
         run_cell<1>(A, cells, q1[t], nattempts, dec, dU, R, trial[w], nullptr, 0, lane);
     }
+}
+
+// A batch of one cell (the grand-canonical exchange of an MD system, up to
+// 16,384 atoms) over one workgroup of 16 waves: each attempt's sweep over the
+// atoms takes a quarter of the steps of four waves.
+constexpr int kGcmcOneWaves = 16;
+__global__ __launch_bounds__(64 * kGcmcOneWaves) void k_gcmc_one(GcmcArgs A, GcmcCell* cells, int nattempts,
+                                                                 int8_t* dec, double* dU, Arrays R)
+{
+    __shared__ double trial[3 * kGcmcMaxTrials];
+    __shared__ double red[4 * kGcmcOneWaves];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    run_cell<kGcmcOneWaves>(A, cells, 0, nattempts, dec, dU, R, trial, red, w, lane);
 }
 
 __global__ void k_gcmc_reset(int* queue)
@@ -937,6 +997,11 @@ void GcmcBatch::build_queues()
 void GcmcBatch::launch(GcmcState& st, int nattempts, int8_t* dec, double* du, const int* q4, int n4, const int* q1,
                        int n1)
 {
+    if (nc_ == 1) {
+        k_gcmc_one<<<1, 64 * kGcmcOneWaves, 0, stream_>>>(A_, st.cells.get(), nattempts, dec, du, arrays_of(st));
+        MAPP_HIP(hipGetLastError());
+        return;
+    }
     k_gcmc_reset<<<1, 64, 0, stream_>>>(queue_.get());
     const int blocks = std::max(1, std::min(grid_, std::max(n4, (n1 + 3) / 4)));
     k_gcmc<<<blocks, 256, 0, stream_>>>(A_, st.cells.get(), q4, n4, q1, n1, queue_.get(), nattempts, dec, du,
