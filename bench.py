@@ -278,14 +278,14 @@ def muvt_cpu_baseline():
 
 
 def feni_run(steps=1000, warmup=20):
-    """Binary Fe-Ni eam/fs alloy: bcc 64^3 x 2 = 1,048,576 atoms, Ni on a random
+    """Binary Fe-Ni eam/fs alloy: bcc 128 x 64 x 64 cells x 2 = 1,048,576 atoms, Ni on a random
     20% of sites, Nr = Nrho = 5000, r_c = 5.3 A, NVT at 600 K and 1 fs with
     on-device rebuilds (the multi-element passes: global spline tables)."""
     import mapp4py_amd as M
     from mapp4py_amd import potentials as pot
 
     p = pot.johnson_feni_fs(nr=5000, nrho=5000)
-    x, H = pot.bcc(2.86, (64, 64, 64))
+    x, H = pot.bcc(2.86, (128, 64, 64))
     x = x + np.random.default_rng(31).normal(0.0, 0.05, x.shape)
     elem = (np.random.default_rng(32).random(len(x)) < 0.2).astype(np.uint8)
     masses = [pot.MASS["Fe"], pot.MASS["Ni"]]
@@ -301,7 +301,7 @@ def feni_run(steps=1000, warmup=20):
     md.profile(False)
     n = len(x)
     maxn, nfull, _ = md.list_stats()
-    return {"workload": "EAM/FS Fe-Ni NVT, bcc 64^3 x 2 = 1,048,576 atoms (20% Ni), Nr=Nrho=5000, r_c=5.3 A, "
+    return {"workload": f"EAM/FS Fe-Ni NVT, bcc 128 x 64 x 64 cells x 2 = {n:,} atoms (20% Ni), Nr=Nrho=5000, r_c=5.3 A, "
                         "skin 0.5 A, dt 1 fs, 600 K",
             "metric": "EAM atom-steps/s", "value": n * steps / wall, "unit": "atom-steps/s", "steps": steps,
             "ms_per_step": wall / steps * 1e3, "rebuilds": nvt.rebuilds, "T_K": nvt.T_part,
