@@ -2604,21 +2604,8 @@ void DMDSystem::plan_knots()
         for (int b = 0; b < ne; ++b)
             for (int k = 0; k < 3; ++k) {
                 const double* c = k == 0 ? t.rphi_at(a, b) : (k == 1 ? t.rho_at(a, b) : t.rho_at(b, a));
-                std::vector<double> ys((size_t)2 * nr);
-                for (int m = 0; m < nr; ++m) {
-                    ys[2 * (size_t)m] = c[4 * (size_t)m];
-                    ys[2 * (size_t)m + 1] = c[4 * (size_t)m + 1];
-                }
-                for (int m = 0; m + 1 < nr; ++m) {
-                    const double y0 = ys[2 * m], s0 = ys[2 * m + 1], y1 = ys[2 * m + 2], s1 = ys[2 * m + 3];
-                    const double dy = y1 - y0;
-                    const double c2 = 3.0 * dy - 2.0 * s0 - s1, c3 = s0 + s1 - 2.0 * dy;
-                    const double scale =
-                        std::max({std::fabs(y0), std::fabs(y1), std::fabs(s0), std::fabs(s1), 1e-300});
-                    if (std::fabs(c[4 * (size_t)m + 2] - c2) > 1e-12 * scale ||
-                        std::fabs(c[4 * (size_t)m + 3] - c3) > 1e-12 * scale)
-                        return;
-                }
+                std::vector<double> ys;
+                if (!hermite_knots(c, nr, ys)) return;
                 int id = -1;
                 for (size_t q = 0; q < uniq.size() && id < 0; ++q)
                     if (uniq[q] == ys) id = (int)q;

@@ -7,6 +7,7 @@
 #include "tables.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cctype>
 #include <cstdlib>
 #include <cstring>
@@ -41,6 +42,25 @@ void spline_coefficients(int n, double* s, int stride)
         for (int k = 0; k < n - 1; ++k) c(k, 4) = (c(k + 1, 2) - c(k, 2)) / 6.0 - 0.5 * c(k, 3);
         c(n - 1, 4) = 0.0;
     }
+}
+
+bool hermite_knots(const double* c, int nr, std::vector<double>& ys)
+{
+    ys.assign((size_t)2 * nr, 0.0);
+    for (int m = 0; m < nr; ++m) {
+        ys[2 * (size_t)m] = c[4 * (size_t)m];
+        ys[2 * (size_t)m + 1] = c[4 * (size_t)m + 1];
+    }
+    for (int m = 0; m + 1 < nr; ++m) {
+        const double y0 = ys[2 * (size_t)m], s0 = ys[2 * (size_t)m + 1];
+        const double y1 = ys[2 * (size_t)m + 2], s1 = ys[2 * (size_t)m + 3];
+        const double dy = y1 - y0;
+        const double c2 = 3.0 * dy - 2.0 * s0 - s1, c3 = s0 + s1 - 2.0 * dy;
+        const double scale = std::max({std::fabs(y0), std::fabs(y1), std::fabs(s0), std::fabs(s1), 1e-300});
+        if (std::fabs(c[4 * (size_t)m + 2] - c2) > 1e-12 * scale || std::fabs(c[4 * (size_t)m + 3] - c3) > 1e-12 * scale)
+            return false;
+    }
+    return true;
 }
 
 namespace {

@@ -33,6 +33,13 @@ struct EamTables {
 // adds the quartic term used by the DMD embedding function).
 void spline_coefficients(int n, double* s, int stride);
 
+// The (value, slope per unit offset) pairs of the knots of one spline whose
+// coefficient rows are c[4m .. 4m+3], into ys[2m], ys[2m+1], when every
+// interval is the Hermite cubic of its two knots (c2 = 3 dy - 2 s0 - s1,
+// c3 = s0 + s1 - 2 dy, as spline_coefficients makes them; checked to 1e-12
+// of the knots' scale); false otherwise.
+bool hermite_knots(const double* c, int nr, std::vector<double>& ys);
+
 // fmt: 0 = setfl (eam/alloy), 1 = fs (eam/fs). elems selects and orders the
 // file's elements (empty: all, in file order).
 EamTables read_dynamo(const std::string& path, int fmt, const std::vector<std::string>& elems, bool dmd);
