@@ -132,6 +132,26 @@ def test_fp32_tables_stay_within_the_stated_bars(M, nr, rows):
     close(f64, fo, "f_x back on the FP64 rows")
 
 
+@pytest.mark.parametrize("lbfgs", [False, True])
+def test_minimisers_run_on_the_fp32_knots(M, cuh, lbfgs):
+    # the DMD minimisers with the FP32 knots in LDS (alternating energy and
+    # gradient passes over the packed records each pass 1 writes) descend as
+    # they do on the FP64 rows: the same start, and a descent within 1e-4 of
+    # the FP64 one after 25 iterations (seen 1e-6 to 2e-6)
+    x, H, elem, alpha, c = sites((3, 3, 3), 9, (0.2, 0.0, -0.1))
+    res = {}
+    for fp32 in (False, True):
+        g = M.AtomsDMD(M.ForceFieldEAM.from_knots(cuh, dmd=True), H, x, elem, alpha, c, nquad=3,
+                       masses=cuh.masses)
+        g.set_fp32(fp32)
+        res[fp32] = g.min_lbfgs(max_iter=25, e_tol=0.0) if lbfgs else g.min_cg(max_iter=25, e_tol=0.0)
+    r32, r64 = res[True], res[False]
+    d32, d64 = r32["fe0"] - r32["fe"], r64["fe0"] - r64["fe"]
+    say(f"DMD {'L-BFGS' if lbfgs else 'CG'} 25 iterations: descent FP32 knots {d32:.6e}, FP64 rows {d64:.6e} eV")
+    assert abs(r32["fe0"] - r64["fe0"]) < 1e-8 * abs(r64["fe0"])
+    assert d64 > 0.0 and abs(d32 - d64) < 1e-4 * d64
+
+
 @pytest.mark.parametrize("cells,tilt,seed", [((3, 3, 3), (0, 0, 0), 1), ((3, 4, 3), (0.6, -0.4, 0.5), 2),
                                              ((2, 2, 2), (0, 0, 0), 3)])
 def test_dmd_lists_and_derivative_match_oracle(M, cuh, cells, tilt, seed):
